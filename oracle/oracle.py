@@ -1,0 +1,331 @@
+"""ctypes bindings of the CPU FP64 oracle (oracle/liboracle.so).
+
+Test infrastructure: imported only by tests/, the smoke check and the CPU
+baseline leg of bench.py. The product path never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+POISSON, ELASTICITY, MATERN = 0, 1, 2
+
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_bp = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def _dpn():
+    return C.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    i, d, v, ll = C.c_int, C.c_double, C.c_void_p, C.c_longlong
+    sig = {
+        "or_quad_rule": (i, [i, i, _dp, _dp]),
+        "or_element_matrices": (i, [i, i, i, i, _dp, i, _ip, _dp, _dp, d, _dp]),
+        "or_body_force": (i, [i, i, i, _dp, i, _ip, i, v, v, d, _dp]),
+        "or_triplet_new": (v, [i]),
+        "or_triplet_free": (None, [v]),
+        "or_triplet_insert_elements": (i, [v, i, i, _ip, _dp]),
+        "or_triplet_apply_constraints": (i, [v, v, i, _ip, v, d, d]),
+        "or_triplet_nnz": (ll, [v]),
+        "or_triplet_csr": (i, [v, _ip, _ip, _dp]),
+        "or_lu_factor": (v, [i, _ip, _ip, _dp, C.POINTER(C.c_int)]),
+        "or_lu_free": (None, [v]),
+        "or_lu_solve": (i, [v, i, i, _dp, _dp]),
+        "or_chol_factor": (v, [i, _ip, _ip, _dp, v, C.POINTER(C.c_int)]),
+        "or_chol_free": (None, [v]),
+        "or_chol_solve": (i, [v, i, _dp, _dp]),
+        "or_spmv": (i, [i, _ip, _ip, _dp, i, _dp, _dp]),
+        "or_philox4x32_10": (None, [_up, _up, _up]),
+        "or_mcmc_normals": (i, [C.c_uint64, i, i, _dp, _dp]),
+        "or_se_kernel": (d, [i, _dp, _dp, d, d]),
+        "or_covforcing_project": (i, [i, i, _dp, _dp, d, d, d, i, _dp, _dp]),
+        "or_covforcing_apply": (i, [i, i, _dp, _dp, d, d, i, _dp, _dp]),
+        "or_mismatch_cov": (i, [i, i, _dp, d, d, d, _dp]),
+        "or_loglik": (i, [i, i, _dp, _dp, _dp, _dp, d, C.POINTER(C.c_double)]),
+        "or_post_proj": (i, [i, i, _dp, _dp, _dp, _dp, d, _dp, _dp]),
+        "or_post_woodbury": (i, [i, i, i, _dp, _dp, _dp, _dp, _dp, d, _dp, v]),
+        "or_gp": (i, [i, i, _dp, _dp, d, d, d, i, _dp, C.POINTER(C.c_double), _dp, _dp]),
+        "or_mcmc_koh": (i, [i, i, i, _dp, _dp, _dp, _dp, d, _dp, _dp, i, C.c_uint64, i, i,
+                            _dp, _dp, _dp, C.POINTER(C.c_double)]),
+        "or_mcmc_gauss": (i, [d, d, d, d, i, C.c_uint64, i, _dp, C.POINTER(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ----------------------------------------------------------------- FE half
+def quad_rule(kind, nq1d):
+    w = np.zeros(27)
+    xi = np.zeros(81)
+    n = lib().or_quad_rule(kind, nq1d, w, xi)
+    return w[:n], xi[: 3 * n].reshape(n, 3)
+
+
+def element_matrices(mesh, physics, p1, p2=None, nq1d=2, factor=1.0):
+    dofpn = 3 if physics == ELASTICITY else 1
+    ned = mesh.nen * dofpn
+    Ke = np.zeros((mesh.nel, ned, ned))
+    p1 = _c(np.broadcast_to(p1, (mesh.nel,)))
+    p2 = _c(np.broadcast_to(0.0 if p2 is None else p2, (mesh.nel,)))
+    rc = lib().or_element_matrices(physics, mesh.kind, nq1d, mesh.nnodes, _c(mesh.coords),
+                                   mesh.nel, _c(mesh.conn, np.int32), p1, p2, factor, Ke)
+    if rc != 0:
+        raise RuntimeError(f"or_element_matrices: {rc}")
+    return Ke
+
+
+def body_force(mesh, nq1d=2, dofpn=1, fconst=None, fnodal=None, factor=1.0):
+    F = np.zeros(mesh.nnodes * dofpn)
+    fc = None if fconst is None else _c(np.broadcast_to(fconst, (dofpn,)))
+    fn = None if fnodal is None else _c(fnodal)
+    rc = lib().or_body_force(mesh.kind, nq1d, mesh.nnodes, _c(mesh.coords), mesh.nel,
+                             _c(mesh.conn, np.int32), dofpn, _ptr(fc), _ptr(fn), factor, F)
+    if rc != 0:
+        raise RuntimeError(f"or_body_force: {rc}")
+    return F
+
+
+class TripletSystem:
+    """Triplet-backed assembly with the reference's sort-merge and row-zero
+    Dirichlet semantics (corlib/Triplet.hpp, corlib/SystemSolveEigenSparse.hpp)."""
+
+    def __init__(self, ndof):
+        self.n = ndof
+        self.h = lib().or_triplet_new(ndof)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_triplet_free(self.h)
+            self.h = None
+
+    def insert_elements(self, edofs, Ke):
+        edofs = _c(edofs, np.int32)
+        Ke = _c(Ke)
+        lib().or_triplet_insert_elements(self.h, edofs.shape[0], edofs.shape[1], edofs, Ke)
+
+    def apply_constraints(self, F, dofs, values=None, factor=1.0, scalar=1.0):
+        dofs = _c(dofs, np.int32)
+        vals = None if values is None else _c(values)
+        lib().or_triplet_apply_constraints(self.h, _ptr(F), len(dofs), dofs, _ptr(vals), factor, scalar)
+
+    def csr(self):
+        nnz = lib().or_triplet_nnz(self.h)
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(nnz, np.int32)
+        va = np.zeros(nnz)
+        rc = lib().or_triplet_csr(self.h, rp, ci, va)
+        if rc != 0:
+            raise RuntimeError("or_triplet_csr: row count mismatch")
+        return rp, ci, va
+
+
+def assemble(mesh, physics, p1, p2=None, nq1d=2, factor=1.0, bc_dofs=None, bc_vals=None, F=None):
+    """Full reference-form assembly: element matrices -> Triplet -> constraints -> CSR."""
+    dofpn = 3 if physics == ELASTICITY else 1
+    Ke = element_matrices(mesh, physics, p1, p2, nq1d, factor)
+    ts = TripletSystem(mesh.nnodes * dofpn)
+    ts.insert_elements(mesh.element_dofs(dofpn), Ke)
+    if F is None:
+        F = np.zeros(mesh.nnodes * dofpn)
+    else:
+        F = np.array(F, dtype=np.float64)
+    if bc_dofs is not None and len(bc_dofs):
+        ts.apply_constraints(F, bc_dofs, bc_vals)
+    rp, ci, va = ts.csr()
+    return rp, ci, va, F
+
+
+class LU:
+    """Direct solve with the reference semantics (SparseLU stand-in)."""
+
+    def __init__(self, rp, ci, va):
+        st = C.c_int(0)
+        self.n = len(rp) - 1
+        self.h = lib().or_lu_factor(self.n, _c(rp, np.int32), _c(ci, np.int32), _c(va), C.byref(st))
+        if not self.h:
+            raise RuntimeError(f"or_lu_factor: {st.value}")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_lu_free(self.h)
+            self.h = None
+
+    def solve(self, B, trans=False):
+        B = _c(B)
+        squeeze = B.ndim == 1
+        B2 = B.reshape(self.n, -1)
+        X = np.zeros_like(B2)
+        lib().or_lu_solve(self.h, int(trans), B2.shape[1], _c(B2), X)
+        return X.ravel() if squeeze else X
+
+
+class Chol:
+    """Banded Cholesky of the symmetric-elimination operator (CPU baseline)."""
+
+    def __init__(self, rp, ci, va, constrained_mask):
+        st = C.c_int(0)
+        self.n = len(rp) - 1
+        m = _c(constrained_mask, np.uint8)
+        self.h = lib().or_chol_factor(self.n, _c(rp, np.int32), _c(ci, np.int32), _c(va),
+                                      m.ctypes.data_as(C.c_void_p), C.byref(st))
+        if not self.h:
+            raise RuntimeError(f"or_chol_factor: {st.value}")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_chol_free(self.h)
+            self.h = None
+
+    def solve(self, B):
+        B = _c(B)
+        squeeze = B.ndim == 1
+        B2 = B.reshape(self.n, -1)
+        X = np.zeros_like(B2)
+        lib().or_chol_solve(self.h, B2.shape[1], _c(B2), X)
+        return X.ravel() if squeeze else X
+
+
+def spmv(rp, ci, va, X):
+    X = _c(X)
+    squeeze = X.ndim == 1
+    n = len(rp) - 1
+    X2 = X.reshape(n, -1)
+    Y = np.zeros_like(X2)
+    lib().or_spmv(n, _c(rp, np.int32), _c(ci, np.int32), _c(va), X2.shape[1], _c(X2), Y)
+    return Y.ravel() if squeeze else Y
+
+
+# --------------------------------------------------------------- statistics
+def philox(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().or_philox4x32_10(_c(ctr, np.uint32), _c(key, np.uint32), out)
+    return out
+
+
+def mcmc_draws(seed, chain, step):
+    z = np.zeros(3)
+    u = np.zeros(1)
+    lib().or_mcmc_normals(seed, chain, step, z, u)
+    return z, u[0]
+
+
+def covforcing_project(coords, covvec, sigma, ell, R, cut_eps=0.0):
+    coords = _c(coords)
+    n, dim = coords.shape
+    R = _c(R)
+    ns = R.shape[0]
+    Cu = np.zeros((ns, ns))
+    lib().or_covforcing_project(n, dim, coords, _c(covvec), sigma, ell, cut_eps, ns, R, Cu)
+    return Cu
+
+
+def covforcing_apply(coords, covvec, sigma, ell, W):
+    coords = _c(coords)
+    n, dim = coords.shape
+    W = _c(W).reshape(n, -1)
+    V = np.zeros_like(W)
+    lib().or_covforcing_apply(n, dim, coords, _c(covvec), sigma, ell, W.shape[1], W, V)
+    return V
+
+
+def mismatch_cov(xs, sigma, ell, eps):
+    xs = _c(np.atleast_2d(xs))
+    if xs.shape[0] == 1 and xs.shape[1] > 1:
+        xs = xs.T.copy()
+    ns, dim = xs.shape
+    Ks = np.zeros((ns, ns))
+    lib().or_mismatch_cov(ns, dim, xs, sigma, ell, eps, Ks)
+    return Ks
+
+
+def loglik(Cu, mean, Y, Ks, rho):
+    Y = _c(Y)
+    no, ns = Y.shape
+    out = C.c_double(0.0)
+    rc = lib().or_loglik(ns, no, _c(Cu), _c(mean), Y, _c(Ks), rho, C.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"or_loglik: {rc}")
+    return out.value
+
+
+def post_proj(Cu, mean, Y, Ks, rho):
+    Y = _c(Y)
+    no, ns = Y.shape
+    m = np.zeros(ns)
+    c = np.zeros((ns, ns))
+    rc = lib().or_post_proj(ns, no, _c(Cu), _c(mean), Y, _c(Ks), rho, m, c)
+    if rc != 0:
+        raise RuntimeError(f"or_post_proj: {rc}")
+    return m, c
+
+
+def post_woodbury(CuFull, ubar, P, Ks, Y, rho, want_cov=True):
+    Y = _c(Y)
+    no, ns = Y.shape
+    n = CuFull.shape[0]
+    m = np.zeros(n)
+    c = np.zeros((n, n)) if want_cov else None
+    rc = lib().or_post_woodbury(n, ns, no, _c(CuFull), _c(ubar), _c(P), _c(Ks), Y, rho, m, _ptr(c))
+    if rc != 0:
+        raise RuntimeError(f"or_post_woodbury: {rc}")
+    return m, c
+
+
+def gp(X, y, sigma, ell, eps, Xp):
+    X = _c(X).reshape(len(y), -1)
+    n, dim = X.shape
+    Xp = _c(Xp).reshape(-1, dim)
+    ll = C.c_double(0.0)
+    mean = np.zeros(Xp.shape[0])
+    var = np.zeros(Xp.shape[0])
+    rc = lib().or_gp(n, dim, X, _c(y), sigma, ell, eps, Xp.shape[0], Xp, C.byref(ll), mean, var)
+    if rc != 0:
+        raise RuntimeError(f"or_gp: {rc}")
+    return ll.value, mean, var
+
+
+def mcmc_koh(Cu, mean, Y, xs, eps, theta0, propsd, nsteps, seed=0, chain=0, prior_mode=0,
+             prior_mu=(0, 0, 0), prior_var=(1, 1, 1)):
+    Y = _c(Y)
+    no, ns = Y.shape
+    xs = _c(xs).reshape(ns, -1)
+    samples = np.zeros((nsteps, 4))
+    acc = C.c_double(0.0)
+    lib().or_mcmc_koh(ns, no, xs.shape[1], _c(Cu), _c(mean), Y, xs, eps, _c(theta0), _c(propsd),
+                      nsteps, seed, chain, prior_mode, _c(prior_mu), _c(prior_var), samples,
+                      C.byref(acc))
+    return samples, acc.value
+
+
+def mcmc_gauss(mu, sd, x0, propsd, nsteps, seed=0, chain=0):
+    samples = np.zeros((nsteps, 2))
+    acc = C.c_double(0.0)
+    lib().or_mcmc_gauss(mu, sd, x0, propsd, nsteps, seed, chain, samples, C.byref(acc))
+    return samples, acc.value

@@ -1,0 +1,299 @@
+"""Pins the CPU FP64 oracle against the reference's golden vectors, published
+known answers and closed forms, so that it can serve as the parity checker of
+the HIP path. Each test names what it pins."""
+import os
+
+import numpy as np
+import pytest
+
+from statfem_amd import mesh as M
+
+from .conftest import GOLDEN
+
+
+# ------------------------------------------------------------------ RNG
+def test_philox_known_answers(orc):
+    # Random123 kat_vectors for philox4x32-10 (Salmon et al., SC'11)
+    assert [hex(v) for v in orc.philox([0, 0, 0, 0], [0, 0])] == \
+        ["0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
+    assert [hex(v) for v in orc.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)] == \
+        ["0x408f276d", "0x41c83b0e", "0xa20bc7c6", "0x6d5451fd"]
+    assert [hex(v) for v in orc.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                                       [0xA4093822, 0x299F31D0])] == \
+        ["0xd16cfe09", "0x94fdcceb", "0x5001e420", "0x24126ea1"]
+
+
+def test_mcmc_draws_are_standard_normal(orc):
+    z = np.array([orc.mcmc_draws(7, c, t)[0] for c in range(4) for t in range(2500)]).ravel()
+    u = np.array([orc.mcmc_draws(7, c, t)[1] for c in range(4) for t in range(2500)])
+    assert abs(z.mean()) < 0.03 and abs(z.std() - 1) < 0.03
+    assert abs(u.mean() - 0.5) < 0.02 and u.min() > 0 and u.max() < 1
+
+
+# ------------------------------------------------------------ quadrature
+def test_quadrature_constants_and_order(orc):
+    w, xi = orc.quad_rule(M.QUAD4, 2)
+    # corlib/GaussLegendre.hpp:64-66 constants; TensorQuadrature: last axis outermost
+    assert np.allclose(w, 0.25, atol=0) and xi[0, 0] == 0.788675134594813
+    assert xi[1, 0] == 0.211324865405187 and xi[1, 1] == 0.788675134594813
+    w3, _ = orc.quad_rule(M.LINE2, 3)
+    assert w3[0] == 0.277777777777777 and w3[1] == 0.444444444444444
+    wt, xt = orc.quad_rule(M.TRI3, 1)
+    assert wt[0] == 0.5 and xt[0, 0] == 0.333333333333333
+    wh, _ = orc.quad_rule(M.HEX8, 2)
+    assert len(wh) == 8 and abs(wh.sum() - 1.0) < 1e-15
+
+
+# ------------------------------------------------------- closed forms
+def test_q1_uniform_grid_nine_point_stencil(orc):
+    m = M.grid2d(6)
+    rp, ci, va, _ = orc.assemble(m, orc.POISSON, 1.0)
+    i = 3 * 7 + 3   # interior node
+    row = dict(zip(ci[rp[i]:rp[i + 1]], va[rp[i]:rp[i + 1]]))
+    assert len(row) == 9
+    assert abs(row[i] - 8.0 / 3.0) < 1e-14
+    for j, v in row.items():
+        if j != i:
+            assert abs(v + 1.0 / 3.0) < 1e-14
+    # row sums vanish before constraints; symmetry
+    A = np.zeros((m.nnodes, m.nnodes))
+    for r in range(m.nnodes):
+        A[r, ci[rp[r]:rp[r + 1]]] = va[rp[r]:rp[r + 1]]
+    assert np.abs(A.sum(1)).max() < 1e-13
+    assert np.abs(A - A.T).max() == 0.0
+
+
+def test_p1_right_triangles_five_point_stencil_with_explicit_zeros(orc):
+    m = M.grid2d(5, kind=M.TRI3)
+    rp, ci, va, _ = orc.assemble(m, orc.POISSON, 2.0)
+    i = 2 * 6 + 2
+    row = dict(zip(ci[rp[i]:rp[i + 1]], va[rp[i]:rp[i + 1]]))
+    # pattern holds the two diagonal couplings along the split direction as explicit zeros
+    assert sorted(row) == sorted([i - 7, i - 6, i - 1, i, i + 1, i + 6, i + 7])
+    assert abs(row[i] - 8.0) < 1e-13
+    for j in (i - 6, i - 1, i + 1, i + 6):
+        assert abs(row[j] + 2.0) < 1e-13
+    assert abs(row[i - 7]) < 1e-15 and abs(row[i + 7]) < 1e-15
+
+
+def test_shape_integrals_sum_to_measure(orc):
+    for mesh, nq in ((M.grid2d(7), 2), (M.grid2d(7, kind=M.TRI3), 1), (M.grid3d(3), 2), (M.line1d(9), 3)):
+        F = orc.body_force(mesh, nq1d=nq, fconst=1.0)
+        assert abs(F.sum() - 1.0) < 1e-13
+
+
+def test_hex8_elasticity_rigid_body_modes_and_symmetry(orc):
+    m = M.grid3d(2)
+    rp, ci, va, _ = orc.assemble(m, orc.ELASTICITY, 1.0, 0.3)
+    n = 3 * m.nnodes
+    A = np.zeros((n, n))
+    for r in range(n):
+        A[r, ci[rp[r]:rp[r + 1]]] = va[rp[r]:rp[r + 1]]
+    assert np.abs(A - A.T).max() < 1e-15
+    x = m.coords
+    modes = []
+    for c in range(3):
+        u = np.zeros((m.nnodes, 3)); u[:, c] = 1; modes.append(u.ravel())
+    for a, b in ((0, 1), (1, 2), (0, 2)):
+        u = np.zeros((m.nnodes, 3)); u[:, a] = -x[:, b]; u[:, b] = x[:, a]; modes.append(u.ravel())
+    for u in modes:
+        assert np.abs(A @ u).max() < 1e-13
+    # a pure stretch is not in the kernel, and each DOF row spans 27 nodes x 3
+    u = np.zeros((m.nnodes, 3)); u[:, 0] = x[:, 0]
+    assert np.abs(A @ u.ravel()).max() > 1e-3
+    centre = (1 * 3 + 1) * 3 + 1
+    assert rp[3 * centre + 1] - rp[3 * centre] == 81
+
+
+def test_matern_row_sums_are_kappa2_lumped_mass(orc):
+    m = M.grid2d(5)
+    kap = 3.0
+    rp, ci, va, _ = orc.assemble(m, orc.MATERN, kap, 1.0)
+    ml = orc.body_force(m, fconst=1.0)
+    rows = np.add.reduceat(va, rp[:-1])
+    assert np.abs(rows - kap * kap * ml).max() < 1e-13
+
+
+# ------------------------------------------------------------ pattern
+def test_pattern_matches_brute_force_union(orc):
+    for mesh, dofpn in ((M.grid2d(4, 3, kind=M.TRI3), 1), (M.grid3d(2, 1, 2), 3), (M.line1d(5), 1)):
+        phys = orc.ELASTICITY if dofpn == 3 else orc.POISSON
+        rp, ci, _, _ = orc.assemble(mesh, phys, 1.0, 0.3)
+        ed = mesh.element_dofs(dofpn)
+        n = mesh.nnodes * dofpn
+        S = [set() for _ in range(n)]
+        for e in ed:
+            for a in e:
+                S[a].update(e.tolist())
+        exp_rp = np.concatenate([[0], np.cumsum([len(s) for s in S])])
+        assert np.array_equal(rp, exp_rp)
+        assert np.array_equal(ci, np.concatenate([sorted(s) for s in S]))
+
+
+def test_row_zero_dirichlet_keeps_pattern(orc):
+    m = M.grid2d(4)
+    bc = M.dirichlet_all_boundary(m)
+    rp0, ci0, _, _ = orc.assemble(m, orc.POISSON, 1.0)
+    F = np.ones(m.nnodes)
+    rp, ci, va, F2 = orc.assemble(m, orc.POISSON, 1.0, bc_dofs=bc, bc_vals=np.full(len(bc), 0.5), F=F)
+    assert np.array_equal(rp, rp0) and np.array_equal(ci, ci0)
+    for c in bc:
+        seg = slice(rp[c], rp[c + 1])
+        assert np.all(va[seg][ci[seg] != c] == 0.0) and va[seg][ci[seg] == c][0] == 1.0
+        assert F2[c] == 0.5
+
+
+# --------------------------------------------------------------- solves
+def test_direct_solves_match_dense(orc):
+    m = M.grid2d(6, 5, kind=M.TRI3)
+    bc = M.dirichlet_all_boundary(m)
+    rng = np.random.default_rng(0)
+    kap = np.exp(rng.normal(size=m.nel) * 0.3)
+    rp, ci, va, _ = orc.assemble(m, orc.POISSON, kap, bc_dofs=bc)
+    n = m.nnodes
+    A = np.zeros((n, n))
+    for r in range(n):
+        A[r, ci[rp[r]:rp[r + 1]]] = va[rp[r]:rp[r + 1]]
+    B = rng.normal(size=(n, 3))
+    lu = orc.LU(rp, ci, va)
+    assert np.abs(lu.solve(B) - np.linalg.solve(A, B)).max() < 1e-12
+    assert np.abs(lu.solve(B, trans=True) - np.linalg.solve(A.T, B)).max() < 1e-12
+    mask = np.zeros(n, np.uint8); mask[bc] = 1
+    ch = orc.Chol(rp, ci, va, mask)
+    Bh = B.copy(); Bh[bc] = 0
+    assert np.abs(ch.solve(Bh) - np.linalg.solve(A, Bh)).max() < 1e-12
+
+
+# --------------------------------------------------------- GP golden
+def _gp_fixture():
+    g = os.path.join(GOLDEN, "gp")
+    hp = dict(l.split() for l in open(os.path.join(g, "gpHps.dat")) if l.strip())
+    X = np.loadtxt(os.path.join(g, "gpCoordinates.dat"), skiprows=1)
+    y = np.array([float(hp[f"gaussianProcess.w{i}"]) for i in range(6)])
+    Xp = np.loadtxt(os.path.join(g, "predictionCoord.dat"))
+    mean = np.loadtxt(os.path.join(g, "watchDog.0.mk.predictionMean.dat"))
+    var = np.loadtxt(os.path.join(g, "watchDog.0.mk.predictionVar.dat"))
+    return hp, X, y, Xp, mean, var
+
+
+def test_gp_golden(orc):
+    hp, X, y, Xp, mean, var = _gp_fixture()
+    ll, mu, v = orc.gp(X, y, float(hp["gaussianProcess.sigma"]), float(hp["gaussianProcess.length"]),
+                       float(hp["gaussianProcess.eps"]), Xp)
+    assert abs(ll - (-4.837171259)) < 1e-9          # watchDog.0.mk:19
+    assert np.abs(mu - mean).max() < 1e-8            # watchDog.0.mk:18 TOL
+    assert np.abs(v - var).max() < 1e-8
+
+
+# ------------------------------------------- 1D statFEM end-to-end golden
+def laplace_rhs_infer_problem(orc):
+    """del2/apps/reference/laplaceRHSinfer: LINE2 bar, 3-point rule, f = pi^2/5,
+    u(0) = u(1) = 0, forcing (0.3, 0.25), reg 1e-3 (inference.hpp:136)."""
+    g = os.path.join(GOLDEN, "laplaceRHSinfer")
+    m = M.read_smf(os.path.join(g, "bar.smf"))
+    bc = M.fuzzy_boundary(m.coords, [0])
+    F = orc.body_force(m, nq1d=3, fconst=np.pi ** 2 / 5)
+    rp, ci, va, F = orc.assemble(m, orc.POISSON, 1.0, nq1d=3, bc_dofs=bc,
+                                 bc_vals=np.zeros(len(bc)), F=F)
+    covvec = orc.body_force(m, nq1d=3, fconst=1.0)
+    covvec[bc] = 0.0
+    trip = np.loadtxt(os.path.join(g, "phiMat.dat"))
+    ns = int(trip[:, 0].max()) + 1
+    P = np.zeros((ns, m.nnodes))
+    P[trip[:, 0].astype(int), trip[:, 1].astype(int)] = trip[:, 2]
+    Y = np.loadtxt(os.path.join(g, "sensorValues.dat"), skiprows=1).T.copy()
+    xs = np.loadtxt(os.path.join(g, "sensorCoordinates.dat"))[:, None]
+    ref = np.loadtxt(os.path.join(g, "watchDog.0.mk.inferredFEcondData.dat"))
+    return dict(mesh=m, bc=bc, rp=rp, ci=ci, va=va, F=F, covvec=covvec, P=P, Y=Y, xs=xs, ref=ref,
+                sf=0.3, lf=0.25, reg=1e-3, eps=0.005,
+                hp=(0.804412, 0.0191724, 0.0740285))
+
+
+def test_laplace_rhs_infer_woodbury_golden(orc):
+    p = laplace_rhs_infer_problem(orc)
+    m = p["mesh"]
+    lu = orc.LU(p["rp"], p["ci"], p["va"])
+    Rfull = lu.solve(np.eye(m.nnodes), trans=True).T
+    CuFull = orc.covforcing_project(m.coords, p["covvec"], p["sf"], p["lf"], Rfull) \
+        + (p["sf"] * p["reg"]) ** 2 * np.eye(m.nnodes)
+    ubar = Rfull @ p["F"]
+    rho, sd, ld = p["hp"]
+    Ks = orc.mismatch_cov(p["xs"], sd, ld, p["eps"])
+    mean, cov = orc.post_woodbury(CuFull, ubar, p["P"], Ks, p["Y"], rho)
+    # golden printed at 6 significant digits (inference.hpp:176)
+    assert np.abs(mean - p["ref"]).max() < 1e-6
+    # projected posterior of the information form == projection of the Woodbury form
+    R = p["P"] @ Rfull
+    Cu = orc.covforcing_project(m.coords, p["covvec"], p["sf"], p["lf"], R) \
+        + (p["sf"] * p["reg"]) ** 2 * p["P"] @ p["P"].T
+    mp, cp = orc.post_proj(Cu, p["P"] @ ubar, p["Y"], Ks, rho)
+    assert np.abs(mp - p["P"] @ mean).max() < 1e-9 * np.abs(mp).max()
+    assert np.abs(cp - p["P"] @ cov @ p["P"].T).max() < 1e-8 * np.abs(cp).max()
+
+
+def test_loglik_equals_dense_gaussian_logpdf(orc):
+    rng = np.random.default_rng(3)
+    ns, no = 7, 4
+    A = rng.normal(size=(ns, ns)); Cu = A @ A.T / ns
+    xs = rng.uniform(size=(ns, 2))
+    Ks = orc.mismatch_cov(xs, 0.2, 0.3, 0.05)
+    mean = rng.normal(size=ns)
+    Y = rng.normal(size=(no, ns))
+    rho = 1.3
+    l = orc.loglik(Cu, mean, Y, Ks, rho)
+    C = Ks + rho ** 2 * Cu
+    sign, logdet = np.linalg.slogdet(C)
+    ref = sum(-0.5 * ns * np.log(2 * np.pi) - 0.5 * logdet
+              - 0.5 * (y - rho * mean) @ np.linalg.solve(C, y - rho * mean) for y in Y)
+    assert abs(l - ref) < 1e-11 * abs(ref)
+
+
+def test_covforcing_reference_form_equals_dense_triple_product(orc):
+    rng = np.random.default_rng(4)
+    m = M.grid2d(5)
+    cv = orc.body_force(m, fconst=1.0)
+    R = rng.normal(size=(3, m.nnodes))
+    Cu = orc.covforcing_project(m.coords, cv, 0.1, 0.2, R)
+    d = m.coords[:, None, :] - m.coords[None, :, :]
+    K = 0.01 * np.exp(-(d ** 2).sum(-1) / (2 * 0.04))
+    Cf = cv[:, None] * K * cv[None, :]
+    assert np.abs(Cu - R @ Cf @ R.T).max() < 1e-14
+    W = rng.normal(size=(m.nnodes, 2))
+    assert np.abs(orc.covforcing_apply(m.coords, cv, 0.1, 0.2, W) - Cf @ W).max() < 1e-15
+
+
+# --------------------------------------------------------------- MCMC
+def test_mcmc_gaussian_statistics(orc):
+    g = os.path.join(GOLDEN, "mcmcGaussian")
+    inp = dict(l.split()[:2] for l in open(os.path.join(g, "input.dat"))
+               if l.strip() and not l.startswith("#"))
+    s, acc = orc.mcmc_gauss(float(inp["mean"]), float(inp["stdDev"]), float(inp["initialProp"]),
+                            float(inp["stdDevProp"]), int(inp["numSamples"]), seed=int(inp["seed"]))
+    ref = dict(l.split() for l in open(os.path.join(g, "chain_summary.txt")) if not l.startswith("#"))
+    # the recorded chain used Boost's mt19937; only its statistics can be matched
+    assert abs(acc - float(ref["acceptance_reported"])) < 0.01
+    assert abs(s[:, 0].mean() - 1.0) < 0.03 and abs(s[:, 0].std() - 0.5) < 0.03
+    lp = np.log(1 / (np.sqrt(2 * np.pi) * 0.5) * np.exp(-(s[:, 0] - 1) ** 2 / 0.5))
+    assert np.abs(s[:, 1] - lp).max() < 1e-12
+
+
+def test_mcmc_koh_detailed_balance_on_known_target(orc):
+    """With Cu = 0, rho enters nowhere except the Jacobian term; the chain must
+    then recover the analytic posterior of a 1-sensor GP noise model."""
+    ns, no = 1, 200
+    rng = np.random.default_rng(5)
+    sig_true = 0.3
+    Y = rng.normal(scale=sig_true, size=(no, ns))
+    xs = np.zeros((1, 1))
+    Cu = np.zeros((1, 1))
+    mean = np.zeros(1)
+    s, acc = orc.mcmc_koh(Cu, mean, Y, xs, 0.0, [0.0, np.log(0.25), 0.0], [0.0, 0.1, 0.0], 20000,
+                          prior_mode=2, prior_mu=[0, np.log(0.3), 0], prior_var=[1, 100.0, 1])
+    th = s[4000:, 1]
+    # Exact posterior of log sigma under the (nearly flat) prior, by quadrature
+    grid = np.linspace(np.log(0.2), np.log(0.45), 4001)
+    ss = (Y ** 2).sum()
+    logp = -no * grid - 0.5 * ss / np.exp(2 * grid) - 0.5 * (grid - np.log(0.3)) ** 2 / 100.0
+    w = np.exp(logp - logp.max()); w /= w.sum()
+    assert abs(th.mean() - (w * grid).sum()) < 0.01
+    assert 0.2 < acc < 0.9
