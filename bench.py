@@ -1,0 +1,184 @@
+"""Benchmark: assembled+solved DOF-RHS/s on one MI355X.
+
+Workload (one step): Poisson on the unit square, 512 x 512 bilinear Q1 cells
+(263,169 DOFs), element diffusivity exp(GP(ell=0.1, sigma=0.5)), homogeneous
+Dirichlet boundary, 256 sensors. A step is
+  GPU assembly (matrix FP64 + FP32 shadow, body force) + row-zero constraints
+  + R^T = A^-T P^T           (256-RHS Jacobi-PCG, FP32 inner / FP64 refinement)
+  + mean = R F, V = C_f R^T  (separable SE forcing covariance, FP64 MFMA)
+  + C_u = R C_f R^T          (Gram, FP64 MFMA)
+  + C_u P^T = A^-1 V (+reg)  (second 256-RHS solve) and ubar = A^-1 F.
+All inputs are resident in HBM before the timed region. value =
+263,169 * 256 / (seconds per step).
+
+Prints one JSON line (rank 0). --gpus N > 1 is not supported: exits with a
+message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def gpu_run(args):
+    from statfem_amd import configs, hip
+
+    n, ns = args.n, args.ns
+    cfg = configs.c2_inputs(seed=0, n=n, ns=ns)
+    m = cfg["mesh"]
+    log(f"mesh {m.nnodes} nodes, {m.nel} elements, {ns} sensors")
+    ctx = hip.Context(0)
+    s = hip.SystemSolveHip(ctx, m)
+    s.set_grid()
+    kappa = ctx.to_device(cfg["kappa"])
+    fg = hip.ForwardGaussRHSHip(s, cfg["cols"], cfg["vals"], reg_eps=cfg["reg"])
+    opts = hip.default_opts(tol=args.tol)
+    bc = cfg["bc"]
+
+    def step():
+        s.assemble(hip.POISSON, kappa, nq1d=2)
+        s.assemble_rhs(nq1d=2, fconst=1.0)
+        s.applyConstraints(bc)
+        return fg.updateSolution(cfg["sigma_f"], cfg["ell_f"], want_field=True, opts=opts)
+
+    for w in range(args.warmup):
+        _, _, st = step()
+        log(f"warmup {w}: {st['inner_iters']} inner iterations, {st['outer']} refinements, "
+            f"max rel. residual {st['max_relres']:.2e}, {st['ms']:.1f} ms")
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    stats = []
+    for k in range(args.steps):
+        _, _, st = step()
+        stats.append(st)
+        log(f"step {k}: {st['inner_iters']} inner iterations, max rel. residual {st['max_relres']:.2e}")
+    dev_ms = ctx.timer_stop()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ms_step = 1000.0 * wall / args.steps
+    value = m.nnodes * ns / (wall / args.steps)
+
+    # assembly alone (kernel + its status check), averaged
+    ctx.timer_start()
+    for _ in range(10):
+        s.assemble(hip.POISSON, kappa, nq1d=2)
+    asm_ms = ctx.timer_stop() / 10
+    s.applyConstraints(bc)
+    # per-launch times of the inner PCG kernels (HIP events, library stream)
+    kt = s.profile_pcg(ns, reps=args.kreps)
+    N, nnz = s.ndof, s.nnz
+    kp = (ns + 3) // 4 * 4
+    algo = {
+        "spmv_dot": 8.0 * nnz + 4.0 * (N + 1) + 8.0 * N * kp,     # FP32 CSR + p read + q write
+        "update": 24.0 * N * kp + 4.0 * N,                        # x,r,p,q read; x,r write; D^-1
+        "pupdate": 12.0 * N * kp + 4.0 * N,                       # r,p read; p write; D^-1
+    }
+    per = {k: {"ms": kt[k], "GBs": algo[k] / (kt[k] * 1e-3) / 1e9, "bytes": algo[k]} for k in kt}
+    dom = max(kt, key=lambda k: kt[k])
+    asm_bytes = m.nnodes * 2 * 8 + m.nel * 4 * 4 + m.nel * 8 + nnz * 8 + nnz * 4
+    res = {
+        "ms_per_step": ms_step, "value": value, "dev_ms_per_step": dev_ms / args.steps,
+        "inner_iters": stats[-1]["inner_iters"], "outer": stats[-1]["outer"],
+        "max_relres": max(st["max_relres"] for st in stats),
+        "kernel": dom, "kernels": per,
+        "assembly": {"ms": asm_ms, "GBs": asm_bytes / (asm_ms * 1e-3) / 1e9, "bytes": asm_bytes},
+        "N": N, "nnz": nnz,
+    }
+    return res, per[dom]
+
+
+def cpu_baseline(args, budget_s):
+    """The oracle (reference algorithm restated in C++, FP64, 1 thread):
+    Triplet sort-merge assembly, banded Cholesky of the eliminated operator,
+    then transposed + forward solve pairs for a sample of right-hand sides,
+    extrapolated to all 256. The dense C_f product is not included (it would
+    be 6.9e10 kernel evaluations per RHS set in the reference form)."""
+    from oracle import oracle as orc
+    from statfem_amd import configs, mesh as M
+    cfg = configs.c2_inputs(seed=0, n=args.cpu_n, ns=args.ns)
+    m = cfg["mesh"]
+    t0 = time.perf_counter()
+    F = orc.body_force(m, nq1d=2, fconst=1.0)
+    rp, ci, va, F = orc.assemble(m, orc.POISSON, cfg["kappa"], nq1d=2, bc_dofs=cfg["bc"], F=F)
+    t_asm = time.perf_counter() - t0
+    mask = np.zeros(m.nnodes, np.uint8)
+    mask[cfg["bc"]] = 1
+    t0 = time.perf_counter()
+    ch = orc.Chol(rp, ci, va, mask)
+    t_fac = time.perf_counter() - t0
+    P = M.dense_projection(m.nnodes, cfg["cols"], cfg["vals"]).T.copy()
+    P[cfg["bc"]] = 0.0
+    k = max(1, min(args.ns, args.cpu_rhs))
+    t0 = time.perf_counter()
+    X = ch.solve(P[:, :k])
+    ch.solve(X)
+    t_solve_k = time.perf_counter() - t0
+    t_total = t_asm + t_fac + t_solve_k * args.ns / k
+    N = m.nnodes
+    return {
+        "value": N * args.ns / t_total, "unit": "DOF-RHS/s", "cores": 1, "kind": "port",
+        "sample": (f"{args.cpu_n}x{args.cpu_n} Q1 ({N} DOFs): assembly {t_asm:.2f}s + banded Cholesky "
+                   f"{t_fac:.2f}s + {k} of {args.ns} solve pairs {t_solve_k:.2f}s, extrapolated to "
+                   f"{args.ns} RHS; C_f product excluded"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--ns", type=int, default=256)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--kreps", type=int, default=20)
+    ap.add_argument("--cpu-n", type=int, default=512)
+    ap.add_argument("--cpu-rhs", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see the sharding notes in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    res, dom = gpu_run(args)
+    cpu = None if args.no_cpu else cpu_baseline(args, 30.0)
+    line = {
+        "metric": "assembled+solved DOF-RHS/s",
+        "value": res["value"],
+        "unit": "DOF-RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"Poisson {args.n}x{args.n} Q1, {res['N']} DOFs, exp(GP) diffusivity, "
+                               f"{args.ns} sensors/RHS: assembly + A^-T P^T + C_f + Gram + A^-1 C_f R^T",
+                   "dofs": res["N"], "rhs": args.ns, "nnz": res["nnz"], "tol": args.tol,
+                   "inner_iters": res["inner_iters"], "max_relres": res["max_relres"],
+                   "kernels": res["kernels"], "assembly": res["assembly"]},
+        "roofline": {"kernel": res["kernel"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

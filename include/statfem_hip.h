@@ -1,0 +1,226 @@
+/* statfem_hip.h — C ABI of the MI355X statistical-FEM hot path.
+ *
+ * Plain C types only (pointers, sizes, doubles); no torch or C++ types cross
+ * this boundary. Every entry point returns an int status (SFH_OK = 0) and never
+ * aborts or throws; sfh_last_error() gives the message of the last failure on
+ * the calling thread. Arrays named *_dev are device pointers on the context's
+ * GPU; all others are host pointers.
+ *
+ * Multi-RHS blocks are row-major N x k ("DOF-major, right-hand sides
+ * contiguous"): entry (dof i, rhs c) is at i*k + c.
+ *
+ * Each declaration cites the reference interface it replaces (paths relative to
+ * the statFEM tree).
+ */
+#ifndef STATFEM_HIP_H
+#define STATFEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum {
+    SFH_OK = 0,
+    SFH_EARG = -1,        /* bad argument / size */
+    SFH_ENEGJAC = -2,     /* "Negative Jacobian." corlib/ElementBasic.ipp:199 */
+    SFH_ESINGULAR = -3,   /* singular system, corlib/SystemSolveEigenSparse.hpp:303-305 */
+    SFH_ENOTSPD = -4,     /* LLT failure, statX/infer/LikelihoodKOH.ipp:93 */
+    SFH_EQUADFORM = -5,   /* non-negative quadratic form, statX/infer/LikelihoodKOH.ipp:113 */
+    SFH_ENOCONV = -6,     /* iterative solve hit its iteration cap */
+    SFH_EDEVICE = -7,     /* HIP runtime error */
+    SFH_ENOMEM = -8,      /* device allocation failed */
+    SFH_ESTATE = -9       /* call out of order (e.g. solve before assembly) */
+};
+
+/* element kinds: corlib/Shapefun.ipp LINE2 :39-55, TRI3 :181-197, QUAD4 :460-476, HEX8 :926-975 */
+enum { SFH_LINE2 = 1, SFH_TRI3 = 2, SFH_QUAD4 = 3, SFH_HEX8 = 4 };
+/* element integrands */
+enum {
+    SFH_POISSON = 0,     /* del2/fem/ElementPotential.ipp:56-74, p1 = conductivity */
+    SFH_ELASTICITY = 1,  /* solid/fem/ElementStatic.ipp:284-321 at F=I, p1 = E, p2 = nu */
+    SFH_MATERN = 2       /* del2/fem/ElementMatern.ipp:57-82, p1 = kappa, p2 = diffusivity */
+};
+
+typedef struct sfh_ctx sfh_ctx;
+typedef struct sfh_system sfh_system;
+typedef struct sfh_lik sfh_lik;
+
+const char* sfh_status_string(int status);
+const char* sfh_last_error(void);
+const char* sfh_version(void);
+
+/* ----------------------------------------------------------------- context */
+/* Opaque device + stream. No reference counterpart (the reference is CPU only). */
+int sfh_ctx_create(int device, sfh_ctx** out);
+int sfh_ctx_destroy(sfh_ctx* ctx);
+int sfh_ctx_sync(sfh_ctx* ctx);
+/* Device memory helpers for callers without a GPU array library. */
+int sfh_malloc(sfh_ctx* ctx, size_t bytes, void** dptr);
+int sfh_free(sfh_ctx* ctx, void* dptr);
+int sfh_memcpy_h2d(sfh_ctx* ctx, void* dst_dev, const void* src, size_t bytes);
+int sfh_memcpy_d2h(sfh_ctx* ctx, void* dst, const void* src_dev, size_t bytes);
+int sfh_memcpy_d2d(sfh_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes);
+int sfh_memset(sfh_ctx* ctx, void* dst_dev, int value, size_t bytes);
+/* Elapsed device time between two points on the context stream, in ms. */
+int sfh_timer_start(sfh_ctx* ctx);
+int sfh_timer_stop(sfh_ctx* ctx, double* ms);
+
+/* ------------------------------------------------------------ FE system */
+/* Replaces SystemSolveEigenSparse(numDofs) (corlib/SystemSolveEigenSparse.hpp:54-60)
+ * together with Mesh<ELEMENT>(istream&) (corlib/Mesh.ipp:62-102) and the DOF
+ * numbering of NodePotential::numberDOFs (del2/fem/NodePotential.hpp:65) /
+ * NodeStatic::numberDOFs (solid/fem/NodeStatic.ipp:15-20): uploads the mesh and
+ * builds, on the GPU, the CSR pattern that Triplet::finishAssembly produces
+ * (corlib/Triplet.hpp:126-170,264-304) plus the element-to-CSR scatter map.
+ * dofpn = 1 (scalar) or 3 (interleaved vector DOFs). */
+int sfh_system_create(sfh_ctx* ctx, int kind, int dofpn, int nnodes, const double* coords,
+                      int nel, const int* conn, sfh_system** out);
+int sfh_system_destroy(sfh_system* sys);
+/* numNonZeros (corlib/SystemSolveEigenSparse.hpp:121) and size */
+int sfh_system_info(sfh_system* sys, int* ndof, long long* nnz);
+/* Triplet::setCompressedRowArrays pattern (corlib/Triplet.hpp:264-304), host copies */
+int sfh_system_pattern(sfh_system* sys, int* rowptr, int* colind);
+/* Declares the mesh a tensor grid (x fastest) with the given axis node
+ * coordinates, enabling the separable forcing-covariance product. Verified
+ * against the node coordinates. */
+int sfh_system_set_grid(sfh_system* sys, int nx, int ny, int nz, const double* gx,
+                        const double* gy, const double* gz);
+
+/* Whole-mesh MatrixComputeAndAssembleFun + mesh.iterateOverElements(maf)
+ * (corlib/MatrixAssembler.hpp:42-100,198-232 over ComputeElementMatrix,
+ * corlib/ComputeElementMatrix.hpp:62-77). Overwrites the matrix values.
+ * p1_dev/p2_dev: per-element parameters (nel), or NULL to use s1/s2 for all. */
+int sfh_assemble_matrix(sfh_system* sys, int physics, int nq1d, const double* p1_dev,
+                        const double* p2_dev, double s1, double s2, double factor);
+/* BodyForce + Integrator + VectorAssembler into the RHS (corlib/BodyForce.hpp:66-71,
+ * corlib/Integrator.hpp:62-74, corlib/VectorAssembler.hpp:101-114, integrand
+ * del2/fem/ElementPotential.ipp:146-175). f(x) = fconst[c] + sum_K fnodal[K*dofpn+c] phi_K(x).
+ * accumulate = 0 overwrites F. */
+int sfh_assemble_rhs(sfh_system* sys, int nq1d, const double* fconst, const double* fnodal_dev,
+                     double factor, int accumulate);
+/* Unit lumped mass int phi_i (ElementMatern::lumpedMassIntegrand,
+ * del2/fem/ElementMatern.ipp:96-119; CovarianceForcing's covVec,
+ * del2/apps/reference/laplaceRHSinfer/inference.hpp:103-114) into out_dev (ndof). */
+int sfh_lumped_mass(sfh_system* sys, int nq1d, double* out_dev);
+/* SystemSolveEigenSparse::applyConstraints (corlib/SystemSolveEigenSparse.hpp:197-215):
+ * row-zero, diagonal = scalar, F[c] = factor*value*scalar. values may be NULL (zeros). */
+int sfh_apply_constraints(sfh_system* sys, int nc, const int* dofs, const double* values,
+                          double factor, double scalar);
+/* clearMatrix / clearRhs (corlib/SystemSolveEigenSparse.hpp:104-107) */
+int sfh_clear_matrix(sfh_system* sys);
+int sfh_clear_rhs(sfh_system* sys);
+/* Host copies of the assembled values (row-zero form) and of F; device views. */
+int sfh_matrix_values(sfh_system* sys, double* vals);
+int sfh_rhs(sfh_system* sys, double* F);
+int sfh_system_device_ptrs(sfh_system* sys, const int** rowptr_dev, const int** colind_dev,
+                           const double** vals_dev, const double** F_dev);
+
+/* ------------------------------------------------------------------ solve */
+typedef struct {
+    double tol;          /* FP64 relative residual target per RHS (default 1e-10) */
+    double inner_tol;    /* FP32 inner PCG relative reduction (default 1e-5) */
+    int max_outer;       /* refinement steps (default 8) */
+    int max_inner;       /* inner PCG iteration cap per refinement step (default 20000) */
+    int check_every;     /* inner iterations between host convergence checks (default 32) */
+    int precision;       /* 0: FP32 inner + FP64 refinement; 1: FP64 PCG throughout */
+} sfh_solve_opts;
+
+typedef struct {
+    int outer;            /* refinement steps taken */
+    int inner_iters;      /* total inner PCG iterations */
+    double max_relres;    /* max over RHS of ||b - A x|| / ||b|| in FP64 */
+    double ms;            /* device time */
+    int converged;
+} sfh_solve_stats;
+
+void sfh_solve_opts_default(sfh_solve_opts* o);
+/* Multi-RHS solveSystem / solveSystemTranspose
+ * (corlib/SystemSolveEigenSparse.hpp:318-360 and :364-404): X = A^-1 B (trans = 0)
+ * or A^-T B (trans = 1) for the row-zero operator, k right-hand sides at once.
+ * The batched form of ForwardGaussRHS's loop of transposed solves
+ * (statX/infer/ForwardGaussRHS.hpp:105-112). B_dev, X_dev: N x k. */
+int sfh_solve(sfh_system* sys, int trans, int k, const double* B_dev, double* X_dev,
+              const sfh_solve_opts* opts, sfh_solve_stats* stats);
+/* solveSystem() on the assembled F (corlib/SystemSolveEigenSparse.hpp:273-311). */
+int sfh_solve_rhs(sfh_system* sys, double* X_dev, const sfh_solve_opts* opts,
+                  sfh_solve_stats* stats);
+
+/* Instrumentation, no reference counterpart: average device time per launch
+ * (ms) of the inner-PCG kernels (SpMV+dot, update+dots, direction update) on
+ * the workspace of the last k-RHS FP32 solve, timed with HIP events on the
+ * context stream. */
+int sfh_profile_pcg(sfh_system* sys, int k, int reps, double* ms_spmv, double* ms_update,
+                    double* ms_pupdate);
+
+/* ------------------------------------------------------------- statistics */
+/* CovarianceForcing::giveProjectedCovar building block (statX/infer/CovarianceForcing.ipp:37-109):
+ * V = C_f W with C_f(i,j) = covvec_i sigma^2 exp(-|x_i-x_j|^2/(2 ell^2)) covvec_j
+ * over the system's node coordinates (scalar DOFs). mode 0: separable if a grid
+ * is set, else dense; 1: dense matrix-free; 2: separable. W_dev, V_dev: N x k. */
+int sfh_covforcing_apply(sfh_system* sys, const double* covvec_dev, double sigma, double ell,
+                         int k, const double* W_dev, double* V_dev, int mode);
+/* Gram product C = X^T V (N x m and N x n -> m x n), FP64 MFMA: the dense
+ * R C_f R^T step (CovarianceForcing.ipp:106). */
+int sfh_gram(sfh_ctx* ctx, int N, int m, int n, const double* X_dev, int ldx, const double* V_dev,
+             int ldv, double* C_dev);
+
+/* ForwardGaussRHS::updateSolution (statX/infer/ForwardGaussRHS.hpp:95-139) for a
+ * sparse projection P given row-wise (ns rows of pn (col, val) pairs):
+ *   R^T = A^-T P^T, mean = R F, Cu = R C_f R^T + (sigma_f reg_eps)^2 P P^T.
+ * C_f's weights are the lumped mass zeroed on constrained DOFs
+ * (CovarianceForcing.hpp:101-114). When field_dev is non-NULL it also returns the
+ * field-by-sensor prior covariance C_u P^T = A^-1 C_f R^T + (sigma_f reg_eps)^2 P^T
+ * (N x ns), and ubar_dev (N) = A^-1 F, the inputs of the Woodbury posterior.
+ * Outputs mean (ns) and Cu (ns x ns) are host arrays. */
+int sfh_forward_gauss(sfh_system* sys, int ns, int pn, const int* pcols, const double* pvals,
+                      double sigma_f, double ell_f, double reg_eps, const sfh_solve_opts* opts,
+                      double* mean, double* Cu, double* field_dev, double* ubar_dev,
+                      sfh_solve_stats* stats);
+
+/* LikelihoodKOH + MismatchCovariance state (statX/infer/LikelihoodKOH.hpp:55-75,
+ * statX/infer/MismatchCovariance.hpp:62-99). xs: ns x dim sensor coordinates,
+ * Cu: ns x ns, mean: ns, Y: no x ns (snapshot-major = column-major ns x no),
+ * eps = mismatch.sensorEps. */
+int sfh_lik_create(sfh_ctx* ctx, int ns, int no, int dim, const double* xs, const double* Cu,
+                   const double* mean, const double* Y, double eps, sfh_lik** out);
+int sfh_lik_destroy(sfh_lik* lik);
+/* giveLogLikelihood (statX/infer/LikelihoodKOH.ipp:74-120) for M hyperparameter
+ * sets theta = (rho, mismatch.sigma, mismatch.length), M x 3 host; out, status: M. */
+int sfh_lik_loglik_batch(sfh_lik* lik, int M, const double* theta, double* out, int* status);
+/* MismatchCovariance::giveCovariance (MismatchCovariance.hpp:76-99), ns x ns host */
+int sfh_lik_mismatch(sfh_lik* lik, double sigma_d, double ell_d, double* Ks);
+/* giveProjFEconditionedData (statX/infer/LikelihoodKOH.ipp:310-361) in the
+ * equivalent Cholesky form; mean: ns, cov: ns x ns host. */
+int sfh_lik_post_proj(sfh_lik* lik, double rho, double sigma_d, double ell_d, double* mean,
+                      double* cov);
+/* giveFEconditionedDataWoodbury mean (statX/infer/LikelihoodKOH.ipp:441-506) on
+ * the whole field from field_dev = C_u P^T (N x ns) and ubar_dev (N); the
+ * posterior variance at the sensors comes from sfh_lik_post_proj. */
+int sfh_lik_post_field(sfh_lik* lik, double rho, double sigma_d, double ell_d, int N,
+                       const double* field_dev, const double* ubar_dev, double* mean_dev);
+
+/* MCMC<3>::sample (statX/infer/MCMC.hpp:123-158) with the target of
+ * hyperParamLearning (del2/apps/reference/laplaceRHSinfer/hyperParamLearning.hpp:172-190),
+ * for `chains` independent chains in lockstep. theta0: chains x 3 (log space),
+ * propsd: 3, prior_mode 0 = flat in x (+sum theta), 1 = reference enrichWithPrior,
+ * 2 = log-normal prior N(theta; mu, var). samples (host, may be NULL):
+ * chains x steps x 4 = (theta, log target); acc: chains. Streams: Philox4x32-10,
+ * counter {step, chain, purpose, 0}, key {seed}. */
+int sfh_mcmc(sfh_lik* lik, int chains, int steps, const double* theta0, const double* propsd,
+             uint64_t seed, int prior_mode, const double* prior_mu, const double* prior_var,
+             double* samples, double* acc);
+
+/* GaussianProcess<KERNEL,N> fit / giveLogLikelihood / predictMean / predictVariance
+ * (statX/infer/GaussianProcess.hpp:83-242) on the GPU. Host arrays. */
+int sfh_gp(sfh_ctx* ctx, int n, int dim, const double* X, const double* y, double sigma,
+           double ell, double eps, int np, const double* Xp, double* logl, double* mean,
+           double* var);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STATFEM_HIP_H */

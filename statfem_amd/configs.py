@@ -1,0 +1,77 @@
+"""Synthetic inputs of the benchmark configurations (seeded, deterministic).
+
+The reference ships no samplers for these fields; the choices here are the
+documented stand-ins: Gaussian-process samples on tensor grids are drawn
+through the Kronecker factorisation K = K_y (x) K_x with eigen-factors
+Q sqrt(max(Lambda, 0)) of each 1D squared-exponential matrix.
+"""
+import numpy as np
+
+from . import mesh as M
+
+
+def se_1d(x, ell):
+    d = x[:, None] - x[None, :]
+    return np.exp(-d * d / (2.0 * ell * ell))
+
+
+def _sqrt_factor(K):
+    lam, Q = np.linalg.eigh(K)
+    return Q * np.sqrt(np.clip(lam, 0.0, None))[None, :]
+
+
+def gp_sample_grid(axes, sigma, ell, rng):
+    """Sample of a zero-mean SE GP on the tensor grid of `axes` (x fastest)."""
+    L = [_sqrt_factor(se_1d(a, ell)) for a in axes]
+    shape = [len(a) for a in axes][::-1]          # (..., ny, nx)
+    Z = rng.standard_normal(shape)
+    if len(axes) == 1:
+        g = L[0] @ Z
+    elif len(axes) == 2:
+        g = L[1] @ Z @ L[0].T
+    else:
+        g = np.einsum("ia,jb,kc,abc->ijk", L[2], L[1], L[0], Z)
+    return sigma * g.ravel()
+
+
+def gp_sample_points(xs, sigma, ell, rng, nugget=1e-12):
+    d = xs[:, None, :] - xs[None, :, :]
+    K = sigma * sigma * np.exp(-(d ** 2).sum(-1) / (2 * ell * ell))
+    L = np.linalg.cholesky(K + nugget * sigma * sigma * np.eye(len(xs)))
+    return L @ rng.standard_normal(len(xs))
+
+
+def c1_inputs(seed=0, n=64, ns=32, no=10):
+    """Plain statFEM Poisson: unit square, n x n grid split into P1 triangles,
+    homogeneous Dirichlet, f = 1 + GP(0.1, 0.2) on nodes, 32 interior sensors,
+    discrepancy GP(0.05, 0.3), noise sigma_e = 1e-3, rho = 1, `no` snapshots."""
+    rng = np.random.default_rng(seed)
+    m = M.grid2d(n, kind=M.TRI3)
+    bc = M.dirichlet_all_boundary(m)
+    g = gp_sample_grid(m.grid, 0.1, 0.2, rng)
+    xs = rng.uniform(0.0, 1.0, size=(ns, 2)) * (1 - 2.0 / n) + 1.0 / n
+    cols, vals = M.sensor_rows(m, xs)
+    d = np.stack([gp_sample_points(xs, 0.05, 0.3, rng) for _ in range(no)])
+    e = rng.standard_normal((no, ns)) * 1e-3
+    return dict(mesh=m, bc=bc, f_true=1.0 + g, xs=xs, cols=cols, vals=vals, d=d, e=e, rho=1.0,
+                sigma_f=0.1, ell_f=0.2, sigma_d=0.05, ell_d=0.3, sigma_e=1e-3, nq1d=1, reg=1e-3)
+
+
+def kappa_field(n, seed, ell=0.1, sigma=0.5):
+    """Element diffusivity exp(g), g ~ GP(SE ell, sigma) on the n x n centroids."""
+    rng = np.random.default_rng(seed)
+    c = (np.arange(n) + 0.5) / n
+    return np.exp(gp_sample_grid((c, c), sigma, ell, rng))
+
+
+def c2_inputs(seed=0, n=512, ns=256):
+    """Poisson with many right-hand sides: n x n Q1, kappa = exp(GP(0.1, 0.5)) per
+    element, forcing covariance SE(0.1, 0.05), `ns` uniform sensors."""
+    m = M.grid2d(n, kind=M.QUAD4)
+    bc = M.dirichlet_all_boundary(m)
+    kappa = kappa_field(n, seed)
+    rng = np.random.default_rng(seed + 1)
+    xs = rng.uniform(0.0, 1.0, size=(ns, 2))
+    cols, vals = M.sensor_rows(m, xs)
+    return dict(mesh=m, bc=bc, kappa=kappa, xs=xs, cols=cols, vals=vals, sigma_f=0.1, ell_f=0.05,
+                nq1d=2, reg=1e-3)

@@ -1,0 +1,631 @@
+// extern "C" entry points of libstatfem_hip (declared in include/statfem_hip.h).
+// No exception or abort crosses this boundary: every call returns a status.
+#include <cstring>
+
+#include "common.hpp"
+
+namespace sfh {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int DevBuf::ensure(size_t b) {
+    if (b <= bytes && p) return SFH_OK;
+    release();
+    if (b == 0) b = 8;
+    if (hipMalloc(&p, b) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        bytes = 0;
+        set_error("device allocation of %zu bytes failed", b);
+        return SFH_ENOMEM;
+    }
+    bytes = b;
+    return SFH_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+namespace {
+
+inline unsigned nb(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__global__ void k_scatter_pt(int ns, int pn, const int* cols, const double* vals, int ld, double* Pt) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    for (int q = 0; q < pn; ++q) Pt[(size_t)cols[s * pn + q] * ld + s] += vals[s * pn + q];
+}
+
+__global__ void k_mask_zero(int n, const unsigned char* m, double* v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && m[i]) v[i] = 0.0;
+}
+
+__global__ void k_axpy(long long n, double a, const double* x, double* y) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
+// S = Cu + K_s / (rho^2 no) and B = [Cu | ybar/rho - mean] (ns x (ns+1))
+__global__ void k_post_build(int ns, int no, int dim, const double* xs, const double* Cu, const double* mean,
+                             const double* Y, double eps, double rho, double sd, double ld, double* S, double* B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ns * ns) return;
+    const int i = t / ns, j = t % ns;
+    double sq = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        const double dd = xs[(size_t)i * dim + d] - xs[(size_t)j * dim + d];
+        sq += dd * dd;
+    }
+    double k = sd * sd * exp(-sq / (2. * ld * ld));
+    if (i == j) k += eps * eps;
+    S[t] = Cu[t] + k / (rho * rho * no);
+    B[(size_t)i * (ns + 1) + j] = Cu[t];
+    if (j == 0) {
+        double ys = 0.0;
+        for (int q = 0; q < no; ++q) ys += Y[(size_t)q * ns + i];
+        B[(size_t)i * (ns + 1) + ns] = ys / no / rho - mean[i];
+    }
+}
+
+// cov = Cu - T[:, :ns], mean = mean + T[:, ns]
+__global__ void k_post_finish(int ns, const double* Cu, const double* mean, const double* T, double* cov,
+                              double* mpost) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ns * ns) return;
+    const int i = t / ns, j = t % ns;
+    cov[t] = Cu[t] - T[(size_t)i * (ns + 1) + j];
+    if (j == 0) mpost[i] = mean[i] + T[(size_t)i * (ns + 1) + ns];
+}
+
+}  // namespace
+}  // namespace sfh
+
+using namespace sfh;
+
+extern "C" {
+
+const char* sfh_status_string(int s) {
+    switch (s) {
+        case SFH_OK: return "ok";
+        case SFH_EARG: return "bad argument";
+        case SFH_ENEGJAC: return "negative Jacobian";
+        case SFH_ESINGULAR: return "singular system";
+        case SFH_ENOTSPD: return "covariance not positive definite";
+        case SFH_EQUADFORM: return "non-negative quadratic form";
+        case SFH_ENOCONV: return "solver did not converge";
+        case SFH_EDEVICE: return "device error";
+        case SFH_ENOMEM: return "out of device memory";
+        case SFH_ESTATE: return "call out of order";
+    }
+    return "unknown status";
+}
+
+const char* sfh_last_error(void) { return g_err; }
+const char* sfh_version(void) { return "statfem_hip 0.1 gfx950"; }
+
+int sfh_ctx_create(int device, sfh_ctx** out) {
+    if (!out) { set_error("null out"); return SFH_EARG; }
+    *out = nullptr;
+    int n = 0;
+    SFH_CHECK_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) { set_error("device %d of %d", device, n); return SFH_EARG; }
+    SFH_CHECK_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    SFH_CHECK_HIP(hipGetDeviceProperties(&prop, device));
+    sfh_ctx* c = new sfh_ctx;
+    c->device = device;
+    c->num_cu = prop.multiProcessorCount;
+    c->lds_per_block = prop.sharedMemPerBlock;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->tev0) != hipSuccess || hipEventCreate(&c->tev1) != hipSuccess) {
+        set_error("stream/event creation failed");
+        delete c;
+        return SFH_EDEVICE;
+    }
+    *out = c;
+    return SFH_OK;
+}
+
+int sfh_ctx_destroy(sfh_ctx* c) {
+    if (!c) return SFH_OK;
+    (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipEventDestroy(c->tev0);
+    (void)hipEventDestroy(c->tev1);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SFH_OK;
+}
+
+int sfh_ctx_sync(sfh_ctx* c) {
+    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SFH_OK;
+}
+
+int sfh_malloc(sfh_ctx* c, size_t bytes, void** d) {
+    if (!c || !d) { set_error("null argument"); return SFH_EARG; }
+    if (hipMalloc(d, bytes ? bytes : 8) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("device allocation of %zu bytes failed", bytes);
+        return SFH_ENOMEM;
+    }
+    c->allocs.push_back(*d);
+    return SFH_OK;
+}
+
+int sfh_free(sfh_ctx* c, void* d) {
+    if (!c || !d) return SFH_OK;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), d);
+    if (it == c->allocs.end()) { set_error("pointer not allocated by sfh_malloc"); return SFH_EARG; }
+    c->allocs.erase(it);
+    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    SFH_CHECK_HIP(hipFree(d));
+    return SFH_OK;
+}
+
+int sfh_memcpy_h2d(sfh_ctx* c, void* dst, const void* src, size_t bytes) {
+    SFH_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SFH_OK;
+}
+
+int sfh_memcpy_d2h(sfh_ctx* c, void* dst, const void* src, size_t bytes) {
+    SFH_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SFH_OK;
+}
+
+int sfh_memcpy_d2d(sfh_ctx* c, void* dst, const void* src, size_t bytes) {
+    SFH_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return SFH_OK;
+}
+
+int sfh_memset(sfh_ctx* c, void* dst, int v, size_t bytes) {
+    SFH_CHECK_HIP(hipMemsetAsync(dst, v, bytes, c->stream));
+    return SFH_OK;
+}
+
+int sfh_timer_start(sfh_ctx* c) {
+    SFH_CHECK_HIP(hipEventRecord(c->tev0, c->stream));
+    return SFH_OK;
+}
+
+int sfh_timer_stop(sfh_ctx* c, double* ms) {
+    SFH_CHECK_HIP(hipEventRecord(c->tev1, c->stream));
+    SFH_CHECK_HIP(hipEventSynchronize(c->tev1));
+    float f = 0.f;
+    SFH_CHECK_HIP(hipEventElapsedTime(&f, c->tev0, c->tev1));
+    if (ms) *ms = f;
+    return SFH_OK;
+}
+
+// ------------------------------------------------------------- system
+int sfh_system_create(sfh_ctx* c, int kind, int dofpn, int nnodes, const double* coords, int nel,
+                      const int* conn, sfh_system** out) {
+    if (!c || !out || !coords || !conn) { set_error("null argument"); return SFH_EARG; }
+    *out = nullptr;
+    const int d = kind_dim(kind), nen = kind_nen(kind);
+    if (!d) { set_error("unknown element kind %d", kind); return SFH_EARG; }
+    if (dofpn != 1 && dofpn != 3) { set_error("dofpn must be 1 or 3"); return SFH_EARG; }
+    if (nnodes <= 0 || nel <= 0) { set_error("empty mesh"); return SFH_EARG; }
+    if (nel >= (1 << 28)) { set_error("too many elements"); return SFH_EARG; }
+    for (long long i = 0; i < (long long)nel * nen; ++i)
+        if (conn[i] < 0 || conn[i] >= nnodes) { set_error("connectivity entry %lld out of range", i); return SFH_EARG; }
+    SFH_CHECK_HIP(hipSetDevice(c->device));
+    sfh_system* s = new sfh_system;
+    s->ctx = c;
+    s->kind = kind;
+    s->dofpn = dofpn;
+    s->nnodes = nnodes;
+    s->nel = nel;
+    s->nen = nen;
+    s->dim = d;
+    int rc = s->coords.ensure(sizeof(double) * nnodes * d);
+    if (rc == SFH_OK) rc = s->conn.ensure(sizeof(int) * (size_t)nel * nen);
+    if (rc == SFH_OK &&
+        (hipMemcpyAsync(s->coords.p, coords, sizeof(double) * nnodes * d, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+         hipMemcpyAsync(s->conn.p, conn, sizeof(int) * (size_t)nel * nen, hipMemcpyHostToDevice, c->stream) != hipSuccess)) {
+        set_error("upload failed");
+        rc = SFH_EDEVICE;
+    }
+    if (rc == SFH_OK) rc = build_pattern(s);
+    if (rc != SFH_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return SFH_OK;
+}
+
+int sfh_system_destroy(sfh_system* s) {
+    if (!s) return SFH_OK;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+    return SFH_OK;
+}
+
+int sfh_system_info(sfh_system* s, int* ndof, long long* nnz) {
+    if (!s) { set_error("null system"); return SFH_EARG; }
+    if (ndof) *ndof = s->ndof;
+    if (nnz) *nnz = s->nnz;
+    return SFH_OK;
+}
+
+int sfh_system_pattern(sfh_system* s, int* rowptr, int* colind) {
+    if (!s) { set_error("null system"); return SFH_EARG; }
+    if (rowptr)
+        SFH_CHECK_HIP(hipMemcpyAsync(rowptr, s->rowptr.p, sizeof(int) * (s->ndof + 1), hipMemcpyDeviceToHost, s->ctx->stream));
+    if (colind)
+        SFH_CHECK_HIP(hipMemcpyAsync(colind, s->colind.p, sizeof(int) * (size_t)s->nnz, hipMemcpyDeviceToHost, s->ctx->stream));
+    SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    return SFH_OK;
+}
+
+int sfh_system_set_grid(sfh_system* s, int nx, int ny, int nz, const double* gx, const double* gy, const double* gz) {
+    if (!s) { set_error("null system"); return SFH_EARG; }
+    const int d = s->dim;
+    const int n[3] = {nx, d > 1 ? ny : 1, d > 2 ? nz : 1};
+    const double* g[3] = {gx, gy, gz};
+    if ((long long)n[0] * n[1] * n[2] != s->nnodes) { set_error("grid size does not match node count"); return SFH_EARG; }
+    std::vector<double> X((size_t)s->nnodes * d);
+    SFH_CHECK_HIP(hipMemcpyAsync(X.data(), s->coords.p, X.size() * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+    SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const size_t id = (size_t)i + (size_t)n[0] * (j + (size_t)n[1] * k);
+                const int idx[3] = {i, j, k};
+                for (int a = 0; a < d; ++a)
+                    if (std::fabs(X[id * d + a] - g[a][idx[a]]) > 1e-12) {
+                        set_error("node %zu is not at grid point (%d,%d,%d)", id, i, j, k);
+                        return SFH_EARG;
+                    }
+            }
+    for (int a = 0; a < d; ++a) {
+        SFH_TRY(s->gaxis[a].ensure(sizeof(double) * n[a]));
+        SFH_CHECK_HIP(hipMemcpyAsync(s->gaxis[a].p, g[a], sizeof(double) * n[a], hipMemcpyHostToDevice, s->ctx->stream));
+        s->gn[a] = n[a];
+    }
+    SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    return SFH_OK;
+}
+
+int sfh_assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, const double* p2, double s1,
+                        double s2, double factor) {
+    if (!s) { set_error("null system"); return SFH_EARG; }
+    SFH_TRY(assemble_matrix(s, physics, nq1d, p1, p2, s1, s2, factor));
+    s->nq1d = nq1d;
+    return SFH_OK;
+}
+
+int sfh_assemble_rhs(sfh_system* s, int nq1d, const double* fconst, const double* fnodal, double factor, int accumulate) {
+    if (!s) { set_error("null system"); return SFH_EARG; }
+    return assemble_rhs(s, nq1d, fconst, fnodal, factor, accumulate, s->F.as<double>());
+}
+
+int sfh_lumped_mass(sfh_system* s, int nq1d, double* out) {
+    if (!s || !out) { set_error("null argument"); return SFH_EARG; }
+    const double one[3] = {1.0, 1.0, 1.0};
+    return assemble_rhs(s, nq1d, one, nullptr, 1.0, 0, out);
+}
+
+int sfh_apply_constraints(sfh_system* s, int nc, const int* dofs, const double* values, double factor, double scalar) {
+    if (!s) { set_error("null system"); return SFH_EARG; }
+    if (!s->assembled) { set_error("constraints before assembly"); return SFH_ESTATE; }
+    if (nc < 0 || (nc > 0 && !dofs)) { set_error("bad constraint list"); return SFH_EARG; }
+    SFH_TRY(apply_constraints(s, nc, dofs, values, factor, scalar));
+    s->nc += nc;
+    return SFH_OK;
+}
+
+int sfh_clear_matrix(sfh_system* s) {
+    SFH_CHECK_HIP(hipMemsetAsync(s->vals.p, 0, sizeof(double) * (size_t)s->nnz, s->ctx->stream));
+    SFH_CHECK_HIP(hipMemsetAsync(s->vals32.p, 0, sizeof(float) * (size_t)s->nnz, s->ctx->stream));
+    SFH_CHECK_HIP(hipMemsetAsync(s->cmask.p, 0, s->ndof, s->ctx->stream));
+    s->nc = 0;
+    s->dirty = true;
+    return SFH_OK;
+}
+
+int sfh_clear_rhs(sfh_system* s) {
+    SFH_CHECK_HIP(hipMemsetAsync(s->F.p, 0, sizeof(double) * s->ndof, s->ctx->stream));
+    return SFH_OK;
+}
+
+int sfh_matrix_values(sfh_system* s, double* vals) {
+    SFH_CHECK_HIP(hipMemcpyAsync(vals, s->vals.p, sizeof(double) * (size_t)s->nnz, hipMemcpyDeviceToHost, s->ctx->stream));
+    SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    return SFH_OK;
+}
+
+int sfh_rhs(sfh_system* s, double* F) {
+    SFH_CHECK_HIP(hipMemcpyAsync(F, s->F.p, sizeof(double) * s->ndof, hipMemcpyDeviceToHost, s->ctx->stream));
+    SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    return SFH_OK;
+}
+
+int sfh_system_device_ptrs(sfh_system* s, const int** rp, const int** ci, const double** va, const double** F) {
+    if (rp) *rp = s->rowptr.as<int>();
+    if (ci) *ci = s->colind.as<int>();
+    if (va) *va = s->vals.as<double>();
+    if (F) *F = s->F.as<double>();
+    return SFH_OK;
+}
+
+int sfh_solve(sfh_system* s, int trans, int k, const double* B, double* X, const sfh_solve_opts* o, sfh_solve_stats* st) {
+    if (!s) { set_error("null system"); return SFH_EARG; }
+    return solve(s, trans, k, B, k, X, k, o, st);
+}
+
+int sfh_solve_rhs(sfh_system* s, double* X, const sfh_solve_opts* o, sfh_solve_stats* st) {
+    if (!s) { set_error("null system"); return SFH_EARG; }
+    return solve(s, 0, 1, s->F.as<double>(), 1, X, 1, o, st);
+}
+
+int sfh_profile_pcg(sfh_system* s, int k, int reps, double* ms_spmv, double* ms_update, double* ms_pupdate) {
+    if (!s || k <= 0 || reps <= 0) { set_error("bad argument"); return SFH_EARG; }
+    double ms[3] = {0, 0, 0};
+    SFH_TRY(profile_pcg(s, k, reps, ms));
+    if (ms_spmv) *ms_spmv = ms[0];
+    if (ms_update) *ms_update = ms[1];
+    if (ms_pupdate) *ms_pupdate = ms[2];
+    return SFH_OK;
+}
+
+// ------------------------------------------------------------ statistics
+int sfh_covforcing_apply(sfh_system* s, const double* cv, double sigma, double ell, int k, const double* W, double* V,
+                         int mode) {
+    if (!s || !cv || !W || !V || k <= 0) { set_error("bad argument"); return SFH_EARG; }
+    return covforcing_apply(s, cv, sigma, ell, k, W, V, mode);
+}
+
+int sfh_gram(sfh_ctx* c, int N, int m, int n, const double* X, int ldx, const double* V, int ldv, double* C) {
+    if (!c || N <= 0 || m <= 0 || n <= 0) { set_error("bad argument"); return SFH_EARG; }
+    DevBuf work;
+    SFH_TRY(gram(c, N, m, n, X, ldx, V, ldv, C, work));
+    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SFH_OK;
+}
+
+int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const double* pvals, double sigma_f,
+                      double ell_f, double reg_eps, const sfh_solve_opts* opts, double* mean, double* Cu,
+                      double* field, double* ubar, sfh_solve_stats* stats) {
+    if (!s || ns <= 0 || pn <= 0 || !pcols || !pvals) { set_error("bad argument"); return SFH_EARG; }
+    if (s->dofpn != 1) { set_error("forward model needs scalar DOFs"); return SFH_EARG; }
+    if (!s->assembled) { set_error("forward model before assembly"); return SFH_ESTATE; }
+    for (int i = 0; i < ns * pn; ++i)
+        if (pcols[i] < 0 || pcols[i] >= s->ndof) { set_error("projection column out of range"); return SFH_EARG; }
+    hipStream_t st = s->ctx->stream;
+    const int N = s->ndof;
+    const size_t nk = (size_t)N * ns;
+    hipEvent_t e0, e1;
+    SFH_CHECK_HIP(hipEventCreate(&e0));
+    SFH_CHECK_HIP(hipEventCreate(&e1));
+    SFH_CHECK_HIP(hipEventRecord(e0, st));
+    SFH_TRY(s->fg_pt.ensure(nk * 8));
+    SFH_TRY(s->fg_rt.ensure(nk * 8));
+    SFH_TRY(s->fg_v.ensure(nk * 8));
+    SFH_TRY(s->fg_cv.ensure((size_t)N * 8));
+    SFH_TRY(s->fg_cu.ensure((size_t)ns * ns * 8 + (size_t)ns * 8));
+    SFH_TRY(s->fg_cols.ensure(sizeof(int) * ns * pn));
+    SFH_TRY(s->fg_vals.ensure(sizeof(double) * ns * pn));
+    double* Pt = s->fg_pt.as<double>();
+    double* Rt = s->fg_rt.as<double>();
+    double* V = s->fg_v.as<double>();
+    double* cv = s->fg_cv.as<double>();
+    double* dCu = s->fg_cu.as<double>();
+    double* dmean = dCu + (size_t)ns * ns;
+    // covVec = int phi, zero on constrained DOFs (CovarianceForcing.hpp:101-114)
+    const double one = 1.0;
+    SFH_TRY(assemble_rhs(s, s->nq1d, &one, nullptr, 1.0, 0, cv));
+    k_mask_zero<<<nb(N, 256), 256, 0, st>>>(N, s->cmask.as<unsigned char>(), cv);
+    // P^T (N x ns)
+    SFH_CHECK_HIP(hipMemsetAsync(Pt, 0, nk * 8, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(s->fg_cols.p, pcols, sizeof(int) * ns * pn, hipMemcpyHostToDevice, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(s->fg_vals.p, pvals, sizeof(double) * ns * pn, hipMemcpyHostToDevice, st));
+    k_scatter_pt<<<nb(ns, 64), 64, 0, st>>>(ns, pn, s->fg_cols.as<int>(), s->fg_vals.as<double>(), ns, Pt);
+    SFH_LAUNCH_CHECK();
+    sfh_solve_stats s1 = {}, s2 = {}, s3 = {};
+    // R^T = A^-T P^T (ForwardGaussRHS.hpp:105-112)
+    SFH_TRY(solve(s, 1, ns, Pt, ns, Rt, ns, opts, &s1));
+    // mean = R F (:124)
+    SFH_TRY(gram(s->ctx, N, ns, 1, Rt, ns, s->F.as<double>(), 1, dmean, s->fg_gram));
+    // V = C_f R^T, Cu = R C_f R^T (CovarianceForcing.ipp:37-109)
+    SFH_TRY(covforcing_apply(s, cv, sigma_f, ell_f, ns, Rt, V, 0));
+    SFH_TRY(gram(s->ctx, N, ns, ns, Rt, ns, V, ns, dCu, s->fg_gram));
+    const double reg = sigma_f * reg_eps;
+    if (field) {
+        // C_u P^T = A^-1 C_f R^T + reg^2 P^T
+        SFH_TRY(solve(s, 0, ns, V, ns, field, ns, opts, &s2));
+        k_axpy<<<nb((long long)nk, 256), 256, 0, st>>>((long long)nk, reg * reg, Pt, field);
+        SFH_LAUNCH_CHECK();
+    }
+    if (ubar) SFH_TRY(solve(s, 0, 1, s->F.as<double>(), 1, ubar, 1, opts, &s3));
+    SFH_CHECK_HIP(hipEventRecord(e1, st));
+    std::vector<double> hCu((size_t)ns * ns), hm(ns);
+    SFH_CHECK_HIP(hipMemcpyAsync(hCu.data(), dCu, hCu.size() * 8, hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(hm.data(), dmean, ns * 8, hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    float ms = 0.f;
+    SFH_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // + (sigma_f reg_eps)^2 P P^T (ForwardGaussRHS.hpp:131-136), from the sparse rows
+    for (int a = 0; a < ns; ++a)
+        for (int b = 0; b < ns; ++b) {
+            double acc = 0.0;
+            for (int qa = 0; qa < pn; ++qa)
+                for (int qb = 0; qb < pn; ++qb)
+                    if (pcols[a * pn + qa] == pcols[b * pn + qb]) acc += pvals[a * pn + qa] * pvals[b * pn + qb];
+            hCu[(size_t)a * ns + b] += reg * reg * acc;
+        }
+    if (Cu) std::memcpy(Cu, hCu.data(), hCu.size() * 8);
+    if (mean) std::memcpy(mean, hm.data(), ns * 8);
+    if (stats) {
+        stats->outer = std::max(s1.outer, std::max(s2.outer, s3.outer));
+        stats->inner_iters = s1.inner_iters + s2.inner_iters + s3.inner_iters;
+        stats->max_relres = std::max(s1.max_relres, std::max(s2.max_relres, s3.max_relres));
+        stats->ms = ms;
+        stats->converged = 1;
+    }
+    return SFH_OK;
+}
+
+int sfh_lik_create(sfh_ctx* c, int ns, int no, int dim, const double* xs, const double* Cu, const double* mean,
+                   const double* Y, double eps, sfh_lik** out) {
+    if (!c || !out || ns <= 0 || no <= 0 || dim <= 0 || !xs || !Cu || !mean || !Y) { set_error("bad argument"); return SFH_EARG; }
+    *out = nullptr;
+    sfh_lik* L = new sfh_lik;
+    L->ctx = c;
+    L->ns = ns;
+    L->no = no;
+    L->dim = dim;
+    L->eps = eps;
+    L->xs_h.assign(xs, xs + (size_t)ns * dim);
+    L->Cu_h.assign(Cu, Cu + (size_t)ns * ns);
+    L->mean_h.assign(mean, mean + ns);
+    L->Y_h.assign(Y, Y + (size_t)no * ns);
+    int rc = L->xs.ensure(8 * (size_t)ns * dim);
+    if (!rc) rc = L->Cu.ensure(8 * (size_t)ns * ns);
+    if (!rc) rc = L->mean.ensure(8 * (size_t)ns);
+    if (!rc) rc = L->Y.ensure(8 * (size_t)no * ns);
+    if (rc) { delete L; return rc; }
+    hipStream_t st = c->stream;
+    if (hipMemcpyAsync(L->xs.p, xs, 8 * (size_t)ns * dim, hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(L->Cu.p, Cu, 8 * (size_t)ns * ns, hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(L->mean.p, mean, 8 * (size_t)ns, hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(L->Y.p, Y, 8 * (size_t)no * ns, hipMemcpyHostToDevice, st) || hipStreamSynchronize(st)) {
+        set_error("upload failed");
+        delete L;
+        return SFH_EDEVICE;
+    }
+    *out = L;
+    return SFH_OK;
+}
+
+int sfh_lik_destroy(sfh_lik* L) {
+    if (!L) return SFH_OK;
+    (void)hipStreamSynchronize(L->ctx->stream);
+    delete L;
+    return SFH_OK;
+}
+
+int sfh_lik_loglik_batch(sfh_lik* L, int M, const double* theta, double* out, int* status) {
+    if (!L || M <= 0 || !theta || !out) { set_error("bad argument"); return SFH_EARG; }
+    hipStream_t st = L->ctx->stream;
+    SFH_TRY(L->theta.ensure(sizeof(double) * 3 * M));
+    SFH_TRY(L->out.ensure(sizeof(double) * M));
+    SFH_TRY(L->status.ensure(sizeof(int) * M));
+    SFH_CHECK_HIP(hipMemcpyAsync(L->theta.p, theta, sizeof(double) * 3 * M, hipMemcpyHostToDevice, st));
+    SFH_TRY(loglik_batch(L, M, L->theta.as<double>(), L->out.as<double>(), L->status.as<int>()));
+    std::vector<int> sth(M);
+    SFH_CHECK_HIP(hipMemcpyAsync(out, L->out.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(sth.data(), L->status.p, sizeof(int) * M, hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (status) std::memcpy(status, sth.data(), sizeof(int) * M);
+    return SFH_OK;
+}
+
+int sfh_lik_mismatch(sfh_lik* L, double sd, double ld, double* Ks) {
+    if (!L || !Ks) { set_error("bad argument"); return SFH_EARG; }
+    const int ns = L->ns, d = L->dim;
+    for (int k = 0; k < ns; ++k)
+        for (int l = 0; l < ns; ++l) {
+            double sq = 0.0;
+            for (int a = 0; a < d; ++a) {
+                const double dd = L->xs_h[(size_t)k * d + a] - L->xs_h[(size_t)l * d + a];
+                sq += dd * dd;
+            }
+            Ks[(size_t)k * ns + l] = sd * sd * std::exp(-sq / (2. * ld * ld)) + (k == l ? L->eps * L->eps : 0.0);
+        }
+    return SFH_OK;
+}
+
+static int post_solve(sfh_lik* L, double rho, double sd, double ld, DevBuf& S, DevBuf& B) {
+    const int ns = L->ns;
+    hipStream_t st = L->ctx->stream;
+    if (!(rho != 0.0)) { set_error("rho must be non-zero"); return SFH_EARG; }
+    SFH_TRY(S.ensure(8 * (size_t)ns * ns));
+    SFH_TRY(B.ensure(8 * (size_t)ns * (ns + 1)));
+    k_post_build<<<nb((long long)ns * ns, 256), 256, 0, st>>>(ns, L->no, L->dim, L->xs.as<double>(), L->Cu.as<double>(),
+                                                             L->mean.as<double>(), L->Y.as<double>(), L->eps, rho, sd,
+                                                             ld, S.as<double>(), B.as<double>());
+    SFH_LAUNCH_CHECK();
+    DevBuf dst;
+    SFH_TRY(dst.ensure(sizeof(int)));
+    SFH_TRY(dense_spd_solve(L->ctx, ns, S.as<double>(), ns + 1, B.as<double>(), dst.as<int>()));
+    int h = 0;
+    SFH_CHECK_HIP(hipMemcpyAsync(&h, dst.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (h != SFH_OK) { set_error("C_u + K_s/(rho^2 n_o) not SPD"); return h; }
+    return SFH_OK;
+}
+
+int sfh_lik_post_proj(sfh_lik* L, double rho, double sd, double ld, double* mean, double* cov) {
+    if (!L || !mean || !cov) { set_error("bad argument"); return SFH_EARG; }
+    const int ns = L->ns;
+    hipStream_t st = L->ctx->stream;
+    DevBuf S, B, T, C, m;
+    SFH_TRY(post_solve(L, rho, sd, ld, S, B));
+    SFH_TRY(T.ensure(8 * (size_t)ns * (ns + 1)));
+    SFH_TRY(C.ensure(8 * (size_t)ns * ns));
+    SFH_TRY(m.ensure(8 * (size_t)ns));
+    // T = Cu [S^-1 Cu | S^-1 (ybar/rho - mean)]
+    SFH_TRY(dgemm_shared_a(L->ctx, ns, ns + 1, ns, L->Cu.as<double>(), ns, B.as<double>(), ns + 1, 0, T.as<double>(),
+                           ns + 1, 0, 1));
+    k_post_finish<<<nb((long long)ns * ns, 256), 256, 0, st>>>(ns, L->Cu.as<double>(), L->mean.as<double>(),
+                                                              T.as<double>(), C.as<double>(), m.as<double>());
+    SFH_LAUNCH_CHECK();
+    SFH_CHECK_HIP(hipMemcpyAsync(cov, C.p, 8 * (size_t)ns * ns, hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(mean, m.p, 8 * (size_t)ns, hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    return SFH_OK;
+}
+
+int sfh_lik_post_field(sfh_lik* L, double rho, double sd, double ld, int N, const double* field, const double* ubar,
+                       double* mean) {
+    if (!L || N <= 0 || !field || !ubar || !mean) { set_error("bad argument"); return SFH_EARG; }
+    const int ns = L->ns;
+    hipStream_t st = L->ctx->stream;
+    DevBuf S, B;
+    SFH_TRY(post_solve(L, rho, sd, ld, S, B));
+    // v = column ns of B; mean = ubar + field v  (Woodbury mean, LikelihoodKOH.ipp:491-503)
+    SFH_TRY(dgemm_shared_a(L->ctx, N, 1, ns, field, ns, B.as<double>() + ns, ns + 1, 0, mean, 1, 0, 1));
+    k_axpy<<<nb(N, 256), 256, 0, st>>>(N, 1.0, ubar, mean);
+    SFH_LAUNCH_CHECK();
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    return SFH_OK;
+}
+
+int sfh_mcmc(sfh_lik* L, int chains, int steps, const double* theta0, const double* propsd, uint64_t seed,
+             int prior_mode, const double* prior_mu, const double* prior_var, double* samples, double* acc) {
+    if (!L || chains <= 0 || steps <= 0 || !theta0 || !propsd) { set_error("bad argument"); return SFH_EARG; }
+    if (prior_mode < 0 || prior_mode > 2) { set_error("prior_mode must be 0, 1 or 2"); return SFH_EARG; }
+    if (prior_mode && (!prior_mu || !prior_var)) { set_error("prior parameters missing"); return SFH_EARG; }
+    return mcmc_run(L, chains, steps, theta0, propsd, seed, prior_mode, prior_mu, prior_var, samples, acc);
+}
+
+int sfh_gp(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell, double eps, int np,
+           const double* Xp, double* logl, double* mean, double* var) {
+    if (!c || n <= 0 || dim <= 0 || !X || !y || !logl || (np > 0 && (!Xp || !mean || !var))) {
+        set_error("bad argument");
+        return SFH_EARG;
+    }
+    return gp_run(c, n, dim, X, y, sigma, ell, eps, np, Xp, logl, mean, var);
+}
+
+}  // extern "C"

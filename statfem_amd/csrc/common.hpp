@@ -1,0 +1,149 @@
+// Internal declarations shared by the HIP translation units of libstatfem_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdint>
+#include <cmath>
+#include <cstdio>
+#include <type_traits>
+#include <string>
+#include <vector>
+
+#include "../../include/statfem_hip.h"
+
+namespace sfh {
+
+void set_error(const char* fmt, ...);
+
+#define SFH_CHECK_HIP(expr)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ::sfh::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr,                  \
+                             hipGetErrorString(e_));                                     \
+            return SFH_EDEVICE;                                                          \
+        }                                                                                \
+    } while (0)
+
+#define SFH_TRY(expr)                                                                    \
+    do {                                                                                 \
+        int s_ = (expr);                                                                 \
+        if (s_ != SFH_OK) return s_;                                                     \
+    } while (0)
+
+#define SFH_LAUNCH_CHECK() SFH_CHECK_HIP(hipGetLastError())
+
+// Element traits (corlib/Shapefun.ipp node orders, [0,1]^d reference cells).
+__host__ __device__ inline int kind_dim(int kind) {
+    return kind == SFH_LINE2 ? 1 : (kind == SFH_HEX8 ? 3 : (kind == SFH_TRI3 || kind == SFH_QUAD4 ? 2 : 0));
+}
+__host__ __device__ inline int kind_nen(int kind) {
+    return kind == SFH_LINE2 ? 2 : kind == SFH_TRI3 ? 3 : kind == SFH_QUAD4 ? 4 : kind == SFH_HEX8 ? 8 : 0;
+}
+inline int num_qp(int kind, int nq1d) {
+    if (kind == SFH_TRI3) return 1;
+    int d = kind_dim(kind), n = 1;
+    for (int i = 0; i < d; ++i) n *= nq1d;
+    return n;
+}
+
+// Device buffer owned by a handle; grows on demand, never shrinks.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    int ensure(size_t b);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Multi-RHS mixed-precision PCG workspace (per system, sized for N x kp).
+struct SolveWork {
+    int kp = 0;
+    DevBuf b64, x64, r64;            // FP64 refinement state
+    DevBuf x32, r32, p32, q32;       // FP32 inner PCG (or FP64 when precision == 1)
+    DevBuf part0, part1;             // per-block partial dot products
+    DevBuf scal;                     // per-RHS scalars
+    DevBuf nact;                     // active counters
+};
+
+}  // namespace sfh
+
+struct sfh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;    // internal (solve timing)
+    hipEvent_t tev0 = nullptr, tev1 = nullptr;  // public timer
+    int num_cu = 0;
+    size_t lds_per_block = 0;
+    std::vector<void*> allocs;
+};
+
+struct sfh_system {
+    sfh_ctx* ctx = nullptr;
+    int kind = 0, dofpn = 1, nnodes = 0, nel = 0, nen = 0, dim = 0, ndof = 0;
+    long long nnz = 0;
+    long long nnz_node = 0;
+    int maxrow = 0;              // longest DOF row
+    sfh::DevBuf coords, conn;
+    sfh::DevBuf n2e_ptr, n2e;    // node -> (element<<3 | local node) adjacency, element order
+    sfh::DevBuf emap;            // per adjacency slot x nen: position of the element node in the node row
+    sfh::DevBuf rowptr_node, colind_node;
+    sfh::DevBuf rowptr, colind, diagpos;
+    sfh::DevBuf vals, vals32, F, dinv32, dinv64;
+    sfh::DevBuf cmask;           // 1 on constrained DOFs
+    sfh::DevBuf cdofs;           // constrained DOF list
+    int nc = 0;
+    bool assembled = false;
+    bool dirty = true;           // preconditioner needs rebuilding
+    // tensor grid (separable covariance)
+    int gn[3] = {0, 0, 0};
+    sfh::DevBuf gaxis[3];
+    int nq1d = 2;                // quadrature of the last matrix assembly
+    sfh::SolveWork work;
+    sfh::DevBuf scratch0, scratch1, scratch2, scratch3;
+    sfh::DevBuf fg_pt, fg_rt, fg_v, fg_cv, fg_cu, fg_gram, fg_cols, fg_vals;
+};
+
+struct sfh_lik {
+    sfh_ctx* ctx = nullptr;
+    int ns = 0, no = 0, dim = 0;
+    double eps = 0.0;
+    sfh::DevBuf xs, Cu, mean, Y;
+    sfh::DevBuf work, theta, out, status, ybar;
+    sfh::DevBuf mc_theta, mc_y, mc_acc, mc_samples, mc_prop, mc_yprop, mc_stat;
+    std::vector<double> Cu_h, mean_h, Y_h, xs_h;
+};
+
+// ---- entry points between translation units (host side)
+namespace sfh {
+int build_pattern(sfh_system* s);
+int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, const double* p2,
+                    double s1, double s2, double factor);
+int assemble_rhs(sfh_system* s, int nq1d, const double* fconst, const double* fnodal,
+                 double factor, int accumulate, double* out);
+int apply_constraints(sfh_system* s, int nc, const int* dofs, const double* values, double factor,
+                      double scalar);
+int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, int ldx,
+          const sfh_solve_opts* o, sfh_solve_stats* st);
+int covforcing_apply(sfh_system* s, const double* covvec, double sigma, double ell, int k,
+                     const double* W, double* V, int mode);
+int gram(sfh_ctx* c, int N, int m, int n, const double* X, int ldx, const double* V, int ldv,
+         double* C, DevBuf& work);
+int dgemm_shared_a(sfh_ctx* c, int M, int N, int K, const double* A, int lda, const double* B,
+                   int ldb, long long strideB, double* C, int ldc, long long strideC, int batch);
+int loglik_batch(sfh_lik* L, int M, const double* theta_dev, double* out_dev, int* status_dev);
+int dense_spd_solve(sfh_ctx* c, int n, double* A_dev, int m, double* B_dev, int* status_dev);
+int mcmc_run(sfh_lik* L, int C, int steps, const double* theta0, const double* propsd, uint64_t seed,
+             int mode, const double* pm, const double* pv, double* samples, double* acc);
+int profile_pcg(sfh_system* s, int k, int reps, double* ms);
+int gp_run(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell,
+           double eps, int np, const double* Xp, double* logl, double* mean, double* var);
+}  // namespace sfh

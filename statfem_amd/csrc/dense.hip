@@ -1,0 +1,694 @@
+// Dense / statistics kernels:
+//  * FP64 MFMA GEMM (v_mfma_f64_16x16x4f64): shared-A batched form for the
+//    separable forcing-covariance product and split-K transposed form for the
+//    sensor-space Gram product R C_f R^T (statX/infer/CovarianceForcing.ipp:106);
+//  * matrix-free dense C_f W for unstructured node sets
+//    (statX/infer/CovarianceForcing.ipp:37-100, ExpKernel.hpp:74-92);
+//  * one-work-group-per-matrix Cholesky, log-determinant and triangular solves:
+//    the batched KOH log-likelihood (statX/infer/LikelihoodKOH.ipp:74-120), the
+//    conditioned moments and the dense GP (statX/infer/GaussianProcess.hpp:117-242);
+//  * random-walk Metropolis in lockstep across chains (statX/infer/MCMC.hpp:123-158)
+//    with Philox4x32-10 streams.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace sfh {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int TM = 64, TN = 64, TK = 16;
+
+// C[z] (=|+=) op(A) B[z]. op(A) = A (M x K, lda) or, with TRANSA, X^T for X K x M (lda).
+// blockIdx.z selects a batch (B, C strides) and/or a K chunk [z*kstep, z*kstep + kz).
+template <bool TRANSA>
+__global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
+                                               const double* __restrict__ B, int ldb, long long strideB,
+                                               double* __restrict__ C, int ldc, long long strideC,
+                                               int kz, int kstep) {
+    __shared__ double As[TM][TK + 1];
+    __shared__ double Bs[TK][TN + 2];
+    const int z = blockIdx.z;
+    const double* Bz = B + (size_t)z * strideB;
+    double* Cz = C + (size_t)z * strideC;
+    const int kbeg = z * kstep;
+    const int kend = min(K, kbeg + kz);
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    v4d acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        // A tile TM x TK
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = tid + q * 256;
+            int r, c;
+            if (TRANSA) { c = idx / TM; r = idx % TM; }   // coalesced along M
+            else { r = idx / TK; c = idx % TK; }
+            const int gm = m0 + r, gk = k0 + c;
+            double v = 0.0;
+            if (gm < M && gk < kend) v = TRANSA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+            As[r][c] = v;
+        }
+        // B tile TK x TN
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = tid + q * 256;
+            const int r = idx / TN, c = idx % TN;
+            const int gk = k0 + r, gn = n0 + c;
+            Bs[r][c] = (gk < kend && gn < N) ? Bz[(size_t)gk * ldb + gn] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TK / 4; ++kk) {
+            const int kr = kk * 4 + (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const double a = As[wm * 32 + i * 16 + (lane & 15)][kr];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const double b = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // v_mfma_f64_16x16x4f64 result: register r of lane l holds D[l/16 + 4r][l%16]
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
+                const int gn = n0 + wn * 32 + j * 16 + (lane & 15);
+                if (gm < M && gn < N) Cz[(size_t)gm * ldc + gn] = acc[i][j][r];
+            }
+}
+
+// C = sum over nz partial products (deterministic split-K reduction)
+__global__ void k_sum_partials(long long mn, int nz, const double* __restrict__ part, double* __restrict__ C) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= mn) return;
+    double s = 0.0;
+    for (int z = 0; z < nz; ++z) s += part[(size_t)z * mn + t];
+    C[t] = s;
+}
+
+// ----------------------------------------------------- forcing covariance
+__device__ inline double se_kernel(int dim, const double* x, const double* y, double sigma, double ell) {
+    double sq = 0.0;
+    for (int i = 0; i < dim; ++i) {
+        const double d = x[i] - y[i];
+        sq += d * d;
+    }
+    return sigma * sigma * exp(-sq / (2. * ell * ell));
+}
+
+// V = C_f W, matrix-free: 16 rows x 64 columns per block, kernel tiles in LDS
+__global__ __launch_bounds__(256) void k_cf_dense(int n, int dim, int k, const double* __restrict__ X,
+                                                  const double* __restrict__ cv, double sigma, double ell,
+                                                  const double* __restrict__ W, double* __restrict__ V) {
+    __shared__ double Kt[16][65];
+    __shared__ double Wt[64][65];
+    const int tid = threadIdx.x, ty = tid / 16, tx = tid % 16;
+    const int i0 = blockIdx.x * 16, c0 = blockIdx.y * 64;
+    double acc[4] = {0, 0, 0, 0};
+    const double inv2l2 = 1.0 / (2. * ell * ell);
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        for (int q = 0; q < 4; ++q) {
+            const int idx = tid + q * 256;
+            const int r = idx / 64, c = idx % 64;
+            const int gi = i0 + r, gj = j0 + c;
+            double v = 0.0;
+            if (gi < n && gj < n) {
+                double sq = 0.0;
+                for (int d = 0; d < dim; ++d) {
+                    const double dd = X[(size_t)gi * dim + d] - X[(size_t)gj * dim + d];
+                    sq += dd * dd;
+                }
+                v = sigma * sigma * exp(-sq * inv2l2);
+            }
+            Kt[r][c] = v;
+        }
+        for (int q = 0; q < 16; ++q) {
+            const int idx = tid + q * 256;
+            const int r = idx / 64, c = idx % 64;
+            const int gj = j0 + r, gc = c0 + c;
+            Wt[r][c] = (gj < n && gc < k) ? cv[gj] * W[(size_t)gj * k + gc] : 0.0;
+        }
+        __syncthreads();
+        for (int jj = 0; jj < 64; ++jj) {
+            const double kv = Kt[ty][jj];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] += kv * Wt[jj][tx * 4 + q];
+        }
+        __syncthreads();
+    }
+    const int gi = i0 + ty;
+    if (gi < n)
+        for (int q = 0; q < 4; ++q) {
+            const int gc = c0 + tx * 4 + q;
+            if (gc < k) V[(size_t)gi * k + gc] = cv[gi] * acc[q];
+        }
+}
+
+// 1D factor of the separable SE kernel on a grid axis: K1[a][b] = exp(-(g_a-g_b)^2 / (2 ell^2))
+__global__ void k_axis_kernel(int n, const double* g, double ell, double* K1) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * n) return;
+    const int a = t / n, b = t % n;
+    const double d = g[a] - g[b];
+    K1[t] = exp(-(d * d) / (2. * ell * ell));
+}
+
+// out[i][c] = s * w[i] * in[i][c]
+__global__ void k_row_scale(long long total, int k, double s, const double* __restrict__ w,
+                            const double* __restrict__ in, double* __restrict__ out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    out[t] = s * w[t / k] * in[t];
+}
+
+// ------------------------------------------------------ dense SPD kernels
+// Right-looking Cholesky of the lower triangle of a (n x n, row stride lda)
+// by one work-group; col is an n-double LDS scratch. Returns false when a
+// pivot is not positive (Eigen::LLT failure).
+__device__ bool chol_block(int n, double* a, int lda, double* col, int* sflag) {
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const int lane = tid & 63, wv = tid >> 6, nw = bs >> 6;
+    if (tid == 0) *sflag = 0;
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        if (tid == 0) {
+            const double d = a[(size_t)j * lda + j];
+            if (!(d > 0.0)) {
+                *sflag = 1;
+            } else {
+                const double L = sqrt(d);
+                a[(size_t)j * lda + j] = L;
+                col[j] = L;
+            }
+        }
+        __syncthreads();
+        if (*sflag) return false;
+        const double L = col[j];
+        for (int i = j + 1 + tid; i < n; i += bs) {
+            const double v = a[(size_t)i * lda + j] / L;
+            a[(size_t)i * lda + j] = v;
+            col[i] = v;
+        }
+        __syncthreads();
+        for (int i = j + 1 + wv; i < n; i += nw) {
+            const double cij = col[i];
+            double* ai = a + (size_t)i * lda;
+            for (int k = j + 1 + lane; k <= i; k += 64) ai[k] -= cij * col[k];
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// forward substitution L Y = D for m columns, D row-major n x m (in place)
+__device__ void forward_block(int n, const double* a, int lda, double* D, int m) {
+    const int tid = threadIdx.x, bs = blockDim.x;
+    for (int j = 0; j < n; ++j) {
+        const double L = a[(size_t)j * lda + j];
+        for (int t = tid; t < m; t += bs) D[(size_t)j * m + t] /= L;
+        __syncthreads();
+        const int rows = n - j - 1;
+        for (int q = tid; q < rows * m; q += bs) {
+            const int i = j + 1 + q / m, t = q % m;
+            D[(size_t)i * m + t] -= a[(size_t)i * lda + j] * D[(size_t)j * m + t];
+        }
+        __syncthreads();
+    }
+}
+
+// backward substitution L^T X = Y
+__device__ void backward_block(int n, const double* a, int lda, double* D, int m) {
+    const int tid = threadIdx.x, bs = blockDim.x;
+    for (int j = n - 1; j >= 0; --j) {
+        const double L = a[(size_t)j * lda + j];
+        for (int t = tid; t < m; t += bs) D[(size_t)j * m + t] /= L;
+        __syncthreads();
+        for (int q = tid; q < j * m; q += bs) {
+            const int i = q / m, t = q % m;
+            D[(size_t)i * m + t] -= a[(size_t)j * lda + i] * D[(size_t)j * m + t];
+        }
+        __syncthreads();
+    }
+}
+
+__device__ double block_sum(double v, double* red) {
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// log-likelihood per hyperparameter set (one work-group per set):
+// C = K_d(sigma_d, ell_d) + eps^2 I + rho^2 Cu; LLT; logL = no(-ns/2 log 2pi - sum log L_ii)
+// - 1/2 sum_k |L^-1 (y_k - rho mean)|^2   (statX/infer/LikelihoodKOH.ipp:74-120)
+__global__ __launch_bounds__(256) void k_loglik(int ns, int no, int dim, const double* __restrict__ xs,
+                                                const double* __restrict__ Cu, const double* __restrict__ mean,
+                                                const double* __restrict__ Y, double eps,
+                                                const double* __restrict__ theta, double* __restrict__ work,
+                                                double* __restrict__ out, int* __restrict__ status) {
+    extern __shared__ double sh[];
+    double* col = sh;               // ns
+    double* D = sh + ns;            // ns * no
+    double* red = D + (size_t)ns * no;   // blockDim
+    __shared__ int sflag;
+    const int b = blockIdx.x, tid = threadIdx.x, bs = blockDim.x;
+    const double rho = theta[3 * b], sd = theta[3 * b + 1], ld = theta[3 * b + 2];
+    double* a = work + (size_t)b * ns * ns;
+    for (int idx = tid; idx < ns * ns; idx += bs) {
+        const int i = idx / ns, k = idx % ns;
+        if (k > i) continue;
+        double v = se_kernel(dim, xs + (size_t)i * dim, xs + (size_t)k * dim, sd, ld);
+        if (i == k) v += eps * eps;
+        a[idx] = v + rho * rho * Cu[idx];
+    }
+    for (int idx = tid; idx < ns * no; idx += bs) {
+        const int i = idx / no, k = idx % no;
+        D[idx] = Y[(size_t)k * ns + i] - rho * mean[i];
+    }
+    __syncthreads();
+    if (!chol_block(ns, a, ns, col, &sflag)) {
+        if (tid == 0) { out[b] = -INFINITY; status[b] = SFH_ENOTSPD; }
+        return;
+    }
+    double lsum = 0.0;
+    for (int i = tid; i < ns; i += bs) lsum += log(a[(size_t)i * ns + i]);
+    const double logdet = block_sum(lsum, red);
+    forward_block(ns, a, ns, D, no);
+    int bad = 0;
+    double tmp3 = 0.0;
+    for (int k = 0; k < no; ++k) {
+        double q = 0.0;
+        for (int i = tid; i < ns; i += bs) q += D[(size_t)i * no + k] * D[(size_t)i * no + k];
+        q = block_sum(q, red);
+        const double c = -0.5 * q;
+        if (!(c < 0.0)) bad = 1;
+        tmp3 += c;
+    }
+    if (tid == 0) {
+        const double tmp1 = -0.5 * ns * log(2. * M_PI);
+        out[b] = no * (tmp1 - logdet) + tmp3;
+        status[b] = bad ? SFH_EQUADFORM : SFH_OK;
+    }
+}
+
+// in-place SPD solve A X = B (A n x n lower used, B n x m), one work-group
+__global__ __launch_bounds__(256) void k_spd_solve(int n, double* A, int m, double* B, int* status) {
+    extern __shared__ double col[];
+    __shared__ int sflag;
+    if (!chol_block(n, A, n, col, &sflag)) {
+        if (threadIdx.x == 0) *status = SFH_ENOTSPD;
+        return;
+    }
+    forward_block(n, A, n, B, m);
+    backward_block(n, A, n, B, m);
+    if (threadIdx.x == 0) *status = SFH_OK;
+}
+
+// ------------------------------------------------------------ MCMC (Philox)
+__device__ inline void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                                uint32_t* out) {
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0; c1 = lo1; c2 = hi0 ^ c3 ^ k1; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__device__ inline double to_unit(uint32_t hi, uint32_t lo) {
+    const uint64_t w = ((uint64_t)hi << 32) | lo;
+    return ((double)(w >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// three N(0,1) increments and one U(0,1) per (chain, step): counter {step, chain, purpose, 0}
+__device__ inline void mcmc_draws(uint64_t seed, int chain, int step, double* z, double* u) {
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    uint32_t a[4], b[4], c[4];
+    philox10((uint32_t)step, (uint32_t)chain, 0u, 0u, k0, k1, a);
+    philox10((uint32_t)step, (uint32_t)chain, 1u, 0u, k0, k1, b);
+    philox10((uint32_t)step, (uint32_t)chain, 2u, 0u, k0, k1, c);
+    const double two_pi = 6.283185307179586;
+    const double u1 = to_unit(a[0], a[1]), u2 = to_unit(a[2], a[3]);
+    const double r1 = sqrt(-2.0 * log(u1));
+    z[0] = r1 * cos(two_pi * u2);
+    z[1] = r1 * sin(two_pi * u2);
+    const double u3 = to_unit(b[0], b[1]), u4 = to_unit(b[2], b[3]);
+    z[2] = sqrt(-2.0 * log(u3)) * cos(two_pi * u4);
+    *u = to_unit(c[0], c[1]);
+}
+
+// log target of hyperParamLearning.hpp:172-190 in log space
+__device__ inline double target(double ll, int st, const double* th, int mode, const double* pm, const double* pv) {
+    if (st != SFH_OK) return -INFINITY;
+    double l = ll;
+    if (mode == 0 || mode == 1) l += th[0] + th[1] + th[2];
+    if (mode == 1) {
+        double q = 0.0, det = 1.0;
+        for (int i = 0; i < 3; ++i) {
+            const double x = exp(th[i]);
+            q += (x - pm[i]) * (x - pm[i]) / pv[i];
+            det *= pv[i];
+        }
+        l += log(1. / (sqrt(pow(2. * M_PI, 3.0) * det)) * exp(-0.5 * q));
+    } else if (mode == 2) {
+        for (int i = 0; i < 3; ++i)
+            l += -0.5 * (th[i] - pm[i]) * (th[i] - pm[i]) / pv[i] - 0.5 * log(2. * M_PI * pv[i]);
+    }
+    return l;
+}
+
+__global__ void k_mc_natural(int C, const double* __restrict__ th, double* __restrict__ nat) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 3 * C) nat[t] = exp(th[t]);
+}
+
+__global__ void k_mc_propose(int C, int step, uint64_t seed, const double* __restrict__ cur,
+                             double p0, double p1, double p2, double* __restrict__ prop,
+                             double* __restrict__ nat) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    double z[3], u;
+    mcmc_draws(seed, c, step, z, &u);
+    const double sd[3] = {p0, p1, p2};
+    for (int i = 0; i < 3; ++i) {
+        const double v = cur[3 * c + i] + sd[i] * z[i];
+        prop[3 * c + i] = v;
+        nat[3 * c + i] = exp(v);
+    }
+}
+
+__global__ void k_mc_init(int C, const double* __restrict__ cur, const double* __restrict__ ll,
+                          const int* __restrict__ st, int mode, double m0, double m1, double m2,
+                          double v0, double v1, double v2, double* __restrict__ ycur) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const double pm[3] = {m0, m1, m2}, pv[3] = {v0, v1, v2};
+    ycur[c] = target(ll[c], st[c], cur + 3 * c, mode, pm, pv);
+}
+
+__global__ void k_mc_accept(int C, int step, int steps, uint64_t seed, double* __restrict__ cur,
+                            double* __restrict__ ycur, const double* __restrict__ prop,
+                            const double* __restrict__ ll, const int* __restrict__ st, int mode, double m0,
+                            double m1, double m2, double v0, double v1, double v2, int* __restrict__ nacc,
+                            double* __restrict__ samples) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    double z[3], u;
+    mcmc_draws(seed, c, step, z, &u);
+    const double pm[3] = {m0, m1, m2}, pv[3] = {v0, v1, v2};
+    const double yp = target(ll[c], st[c], prop + 3 * c, mode, pm, pv);
+    const double ratio = fmin(0.0, yp - ycur[c]);
+    if (log(u) <= ratio) {
+        for (int i = 0; i < 3; ++i) cur[3 * c + i] = prop[3 * c + i];
+        ycur[c] = yp;
+        nacc[c] += 1;
+    }
+    if (samples) {
+        double* s = samples + ((size_t)c * steps + step) * 4;
+        s[0] = cur[3 * c]; s[1] = cur[3 * c + 1]; s[2] = cur[3 * c + 2]; s[3] = ycur[c];
+    }
+}
+
+// ------------------------------------------------------------------- GP
+__global__ __launch_bounds__(256) void k_gp_fit(int n, int dim, const double* __restrict__ X,
+                                                const double* __restrict__ y, double sigma, double ell,
+                                                double eps, double* __restrict__ L, double* __restrict__ alpha,
+                                                double* __restrict__ logl, int* __restrict__ status) {
+    extern __shared__ double sh[];
+    double* col = sh;
+    double* red = sh + n;
+    __shared__ int sflag;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    for (int idx = tid; idx < n * n; idx += bs) {
+        const int i = idx / n, j = idx % n;
+        double v = se_kernel(dim, X + (size_t)i * dim, X + (size_t)j * dim, sigma, ell);
+        if (i == j) v += eps * eps;
+        L[idx] = v;
+    }
+    for (int i = tid; i < n; i += bs) alpha[i] = y[i];
+    __syncthreads();
+    if (!chol_block(n, L, n, col, &sflag)) {
+        if (tid == 0) *status = SFH_ENOTSPD;
+        return;
+    }
+    forward_block(n, L, n, alpha, 1);
+    double q = 0.0, ls = 0.0;
+    for (int i = tid; i < n; i += bs) {
+        q += alpha[i] * alpha[i];
+        ls += log(L[(size_t)i * n + i]);
+    }
+    q = block_sum(q, red);
+    ls = block_sum(ls, red);
+    backward_block(n, L, n, alpha, 1);
+    if (tid == 0) {
+        *logl = -0.5 * q - ls - 0.5 * n * log(2.0 * M_PI);
+        *status = SFH_OK;
+    }
+}
+
+// mean k*^T alpha, variance k** - |L^-1 k*|^2, one thread per prediction point
+__global__ void k_gp_predict(int n, int dim, int np, const double* __restrict__ X, const double* __restrict__ Xp,
+                             double sigma, double ell, const double* __restrict__ L,
+                             const double* __restrict__ alpha, double* __restrict__ work,
+                             double* __restrict__ mean, double* __restrict__ var) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const double* xp = Xp + (size_t)p * dim;
+    double* v = work + (size_t)p * n;
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double ks = se_kernel(dim, X + (size_t)i * dim, xp, sigma, ell);
+        v[i] = ks;
+        m += ks * alpha[i];
+    }
+    double q = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double s = v[i];
+        for (int k = 0; k < i; ++k) s -= L[(size_t)i * n + k] * v[k];
+        s /= L[(size_t)i * n + i];
+        v[i] = s;
+        q += s * s;
+    }
+    mean[p] = m;
+    var[p] = se_kernel(dim, xp, xp, sigma, ell) - q;
+}
+
+inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- host
+int dgemm_shared_a(sfh_ctx* c, int M, int N, int K, const double* A, int lda, const double* B, int ldb,
+                   long long strideB, double* C, int ldc, long long strideC, int batch) {
+    if (batch > 65535) { set_error("batch too large"); return SFH_EARG; }
+    dim3 g((N + TN - 1) / TN, (M + TM - 1) / TM, batch);
+    k_dgemm<false><<<g, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, strideB, C, ldc, strideC, K, 0);
+    SFH_LAUNCH_CHECK();
+    return SFH_OK;
+}
+
+int gram(sfh_ctx* c, int N, int m, int n, const double* X, int ldx, const double* V, int ldv, double* C,
+         DevBuf& work) {
+    const int tiles = ((m + TM - 1) / TM) * ((n + TN - 1) / TN);
+    int nz = std::max(1, std::min(256, (4 * c->num_cu + tiles - 1) / tiles));
+    int kz = (N + nz - 1) / nz;
+    kz = (kz + TK - 1) / TK * TK;
+    nz = (N + kz - 1) / kz;
+    const long long mn = (long long)m * n;
+    SFH_TRY(work.ensure((size_t)nz * mn * 8));
+    dim3 g((n + TN - 1) / TN, (m + TM - 1) / TM, nz);
+    k_dgemm<true><<<g, 256, 0, c->stream>>>(m, n, N, X, ldx, V, ldv, 0, work.as<double>(), n, mn, kz, kz);
+    SFH_LAUNCH_CHECK();
+    k_sum_partials<<<nblocks(mn, 256), 256, 0, c->stream>>>(mn, nz, work.as<double>(), C);
+    SFH_LAUNCH_CHECK();
+    return SFH_OK;
+}
+
+int covforcing_apply(sfh_system* s, const double* cv, double sigma, double ell, int k, const double* W,
+                     double* V, int mode) {
+    hipStream_t st = s->ctx->stream;
+    if (s->dofpn != 1) { set_error("forcing covariance needs scalar DOFs"); return SFH_EARG; }
+    const bool have_grid = s->gn[0] > 0;
+    if (mode == 2 && !have_grid) { set_error("separable mode needs sfh_system_set_grid"); return SFH_EARG; }
+    const int n = s->ndof;
+    if (mode == 1 || (mode == 0 && !have_grid)) {
+        dim3 g((n + 15) / 16, (k + 63) / 64);
+        k_cf_dense<<<g, 256, 0, st>>>(n, s->dim, k, s->coords.as<double>(), cv, sigma, ell, W, V);
+        SFH_LAUNCH_CHECK();
+        return SFH_OK;
+    }
+    // separable: V = sigma^2 D (Kz (x) Ky (x) Kx) D W, axes x fastest
+    const int d = s->dim;
+    const int nx = s->gn[0], ny = d > 1 ? s->gn[1] : 1, nz = d > 2 ? s->gn[2] : 1;
+    const size_t total = (size_t)n * k;
+    SFH_TRY(s->scratch0.ensure(total * 8));
+    SFH_TRY(s->scratch1.ensure(total * 8));
+    size_t kb = (size_t)nx * nx + (size_t)ny * ny + (size_t)nz * nz;
+    SFH_TRY(s->scratch2.ensure(kb * 8));
+    double* Kx = s->scratch2.as<double>();
+    double* Ky = Kx + (size_t)nx * nx;
+    double* Kz = Ky + (size_t)ny * ny;
+    const int gn[3] = {nx, ny, nz};
+    double* Ks[3] = {Kx, Ky, Kz};
+    for (int a = 0; a < d; ++a) {
+        k_axis_kernel<<<nblocks((long long)gn[a] * gn[a], 256), 256, 0, st>>>(gn[a], s->gaxis[a].as<double>(), ell, Ks[a]);
+        SFH_LAUNCH_CHECK();
+    }
+    double* t0 = s->scratch0.as<double>();
+    double* t1 = s->scratch1.as<double>();
+    k_row_scale<<<nblocks((long long)total, 256), 256, 0, st>>>((long long)total, k, 1.0, cv, W, t0);
+    SFH_LAUNCH_CHECK();
+    // x axis: batch over (iy, iz): Kx (nx x nx) * t0[b] (nx x k)
+    {
+        int batch = ny * nz;
+        for (int b0 = 0; b0 < batch; b0 += 65535) {
+            const int bb = std::min(65535, batch - b0);
+            SFH_TRY(dgemm_shared_a(s->ctx, nx, k, nx, Kx, nx, t0 + (size_t)b0 * nx * k, k, (long long)nx * k,
+                                   t1 + (size_t)b0 * nx * k, k, (long long)nx * k, bb));
+        }
+        std::swap(t0, t1);
+    }
+    if (d > 1) {   // y axis: batch over iz: Ky (ny x ny) * t0[iz] (ny x nx k)
+        const long long st2 = (long long)ny * nx * k;
+        SFH_TRY(dgemm_shared_a(s->ctx, ny, nx * k, ny, Ky, ny, t0, nx * k, st2, t1, nx * k, st2, nz));
+        std::swap(t0, t1);
+    }
+    if (d > 2) {   // z axis: Kz (nz x nz) * t0 (nz x ny nx k)
+        SFH_TRY(dgemm_shared_a(s->ctx, nz, ny * nx * k, nz, Kz, nz, t0, ny * nx * k, 0, t1, ny * nx * k, 0, 1));
+        std::swap(t0, t1);
+    }
+    k_row_scale<<<nblocks((long long)total, 256), 256, 0, st>>>((long long)total, k, sigma * sigma, cv, t0, V);
+    SFH_LAUNCH_CHECK();
+    return SFH_OK;
+}
+
+int loglik_batch(sfh_lik* L, int M, const double* theta, double* out, int* status) {
+    hipStream_t st = L->ctx->stream;
+    const int ns = L->ns, no = L->no;
+    SFH_TRY(L->work.ensure((size_t)M * ns * ns * 8));
+    const size_t lds = sizeof(double) * ((size_t)ns + (size_t)ns * no + 256);
+    if (lds > L->ctx->lds_per_block) { set_error("ns * no too large for LDS"); return SFH_EARG; }
+    k_loglik<<<M, 256, lds, st>>>(ns, no, L->dim, L->xs.as<double>(), L->Cu.as<double>(), L->mean.as<double>(),
+                                 L->Y.as<double>(), L->eps, theta, L->work.as<double>(), out, status);
+    SFH_LAUNCH_CHECK();
+    return SFH_OK;
+}
+
+int dense_spd_solve(sfh_ctx* c, int n, double* A, int m, double* B, int* status) {
+    const size_t lds = sizeof(double) * n;
+    k_spd_solve<<<1, 256, lds, c->stream>>>(n, A, m, B, status);
+    SFH_LAUNCH_CHECK();
+    return SFH_OK;
+}
+
+int mcmc_run(sfh_lik* L, int C, int steps, const double* theta0, const double* propsd, uint64_t seed, int mode,
+             const double* pm, const double* pv, double* samples, double* acc) {
+    hipStream_t st = L->ctx->stream;
+    SFH_TRY(L->mc_theta.ensure(sizeof(double) * 3 * C));
+    SFH_TRY(L->mc_prop.ensure(sizeof(double) * 3 * C));
+    SFH_TRY(L->theta.ensure(sizeof(double) * 3 * C));
+    SFH_TRY(L->mc_y.ensure(sizeof(double) * C));
+    SFH_TRY(L->out.ensure(sizeof(double) * C));
+    SFH_TRY(L->status.ensure(sizeof(int) * C));
+    SFH_TRY(L->mc_acc.ensure(sizeof(int) * C));
+    double* dsamp = nullptr;
+    if (samples) {
+        SFH_TRY(L->mc_samples.ensure(sizeof(double) * 4 * (size_t)C * steps));
+        dsamp = L->mc_samples.as<double>();
+    }
+    const double m0 = pm ? pm[0] : 0, m1 = pm ? pm[1] : 0, m2 = pm ? pm[2] : 0;
+    const double v0 = pv ? pv[0] : 1, v1 = pv ? pv[1] : 1, v2 = pv ? pv[2] : 1;
+    SFH_CHECK_HIP(hipMemcpyAsync(L->mc_theta.p, theta0, sizeof(double) * 3 * C, hipMemcpyHostToDevice, st));
+    SFH_CHECK_HIP(hipMemsetAsync(L->mc_acc.p, 0, sizeof(int) * C, st));
+    const unsigned g = nblocks(C, 64);
+    k_mc_natural<<<nblocks(3 * C, 64), 64, 0, st>>>(C, L->mc_theta.as<double>(), L->theta.as<double>());
+    SFH_LAUNCH_CHECK();
+    SFH_TRY(loglik_batch(L, C, L->theta.as<double>(), L->out.as<double>(), L->status.as<int>()));
+    k_mc_init<<<g, 64, 0, st>>>(C, L->mc_theta.as<double>(), L->out.as<double>(), L->status.as<int>(), mode, m0, m1,
+                                m2, v0, v1, v2, L->mc_y.as<double>());
+    SFH_LAUNCH_CHECK();
+    for (int t = 0; t < steps; ++t) {
+        k_mc_propose<<<g, 64, 0, st>>>(C, t, seed, L->mc_theta.as<double>(), propsd[0], propsd[1], propsd[2],
+                                       L->mc_prop.as<double>(), L->theta.as<double>());
+        SFH_TRY(loglik_batch(L, C, L->theta.as<double>(), L->out.as<double>(), L->status.as<int>()));
+        k_mc_accept<<<g, 64, 0, st>>>(C, t, steps, seed, L->mc_theta.as<double>(), L->mc_y.as<double>(),
+                                      L->mc_prop.as<double>(), L->out.as<double>(), L->status.as<int>(), mode, m0,
+                                      m1, m2, v0, v1, v2, L->mc_acc.as<int>(), dsamp);
+        if ((t & 255) == 255) SFH_LAUNCH_CHECK();
+    }
+    SFH_LAUNCH_CHECK();
+    std::vector<int> na(C);
+    SFH_CHECK_HIP(hipMemcpyAsync(na.data(), L->mc_acc.p, sizeof(int) * C, hipMemcpyDeviceToHost, st));
+    if (samples)
+        SFH_CHECK_HIP(hipMemcpyAsync(samples, dsamp, sizeof(double) * 4 * (size_t)C * steps,
+                                     hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (acc)
+        for (int c = 0; c < C; ++c) acc[c] = na[c] / (double)steps;
+    return SFH_OK;
+}
+
+int gp_run(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell, double eps,
+           int np, const double* Xp, double* logl, double* mean, double* var) {
+    hipStream_t st = c->stream;
+    DevBuf dX, dy, dL, da, dXp, dw, dm, dv, dl, ds;
+    SFH_TRY(dX.ensure(sizeof(double) * n * dim));
+    SFH_TRY(dy.ensure(sizeof(double) * n));
+    SFH_TRY(dL.ensure(sizeof(double) * n * n));
+    SFH_TRY(da.ensure(sizeof(double) * n));
+    SFH_TRY(dl.ensure(sizeof(double)));
+    SFH_TRY(ds.ensure(sizeof(int)));
+    SFH_CHECK_HIP(hipMemcpyAsync(dX.p, X, sizeof(double) * n * dim, hipMemcpyHostToDevice, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(dy.p, y, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    const size_t lds = sizeof(double) * (n + 256);
+    if (lds > c->lds_per_block) { set_error("GP too large"); return SFH_EARG; }
+    k_gp_fit<<<1, 256, lds, st>>>(n, dim, dX.as<double>(), dy.as<double>(), sigma, ell, eps, dL.as<double>(),
+                                  da.as<double>(), dl.as<double>(), ds.as<int>());
+    SFH_LAUNCH_CHECK();
+    int hst = 0;
+    SFH_CHECK_HIP(hipMemcpyAsync(&hst, ds.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(logl, dl.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (hst != SFH_OK) { set_error("GP covariance not SPD"); return hst; }
+    if (np > 0) {
+        SFH_TRY(dXp.ensure(sizeof(double) * np * dim));
+        SFH_TRY(dw.ensure(sizeof(double) * (size_t)np * n));
+        SFH_TRY(dm.ensure(sizeof(double) * np));
+        SFH_TRY(dv.ensure(sizeof(double) * np));
+        SFH_CHECK_HIP(hipMemcpyAsync(dXp.p, Xp, sizeof(double) * np * dim, hipMemcpyHostToDevice, st));
+        k_gp_predict<<<nblocks(np, 64), 64, 0, st>>>(n, dim, np, dX.as<double>(), dXp.as<double>(), sigma, ell,
+                                                     dL.as<double>(), da.as<double>(), dw.as<double>(),
+                                                     dm.as<double>(), dv.as<double>());
+        SFH_LAUNCH_CHECK();
+        SFH_CHECK_HIP(hipMemcpyAsync(mean, dm.p, sizeof(double) * np, hipMemcpyDeviceToHost, st));
+        SFH_CHECK_HIP(hipMemcpyAsync(var, dv.p, sizeof(double) * np, hipMemcpyDeviceToHost, st));
+        SFH_CHECK_HIP(hipStreamSynchronize(st));
+    }
+    return SFH_OK;
+}
+
+}  // namespace sfh

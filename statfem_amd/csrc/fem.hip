@@ -1,0 +1,695 @@
+// GPU pattern build, element-matrix assembly, body-force assembly and row-zero
+// Dirichlet constraints.
+//
+// Assembly is node-row-owner ("gather") and atomic-free: one thread owns one
+// DOF row, walks the elements adjacent to its node in ascending element order
+// (the node->element adjacency is sorted at setup), integrates that element's
+// row block and accumulates it into a per-thread row buffer in LDS through the
+// precomputed scatter map (element-local column -> CSR position). Rows are
+// written once, in FP64 and as the FP32 shadow used by the inner PCG.
+//
+// Reference semantics: corlib/MatrixAssembler.hpp:198-232 (compute, *factor,
+// insert), corlib/Triplet.hpp:126-170 (pattern = union of element couplings,
+// ascending columns, explicit zeros kept), corlib/ComputeElementMatrix.hpp:62-77
+// (quadrature loop), corlib/ElementBasic.ipp:177-205 (geometry).
+#include "common.hpp"
+
+namespace sfh {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxAdj = 64;      // max elements around a node
+constexpr int kMaxCand = 64;     // max distinct nodes coupled to a node
+
+// ------------------------------------------------------------ quadrature
+// corlib/GaussLegendre.hpp:58-80 (constants as stored there)
+__device__ inline void gl_point(int n, int i, double& w, double& x) {
+    if (n == 1) {
+        w = 1.0; x = 0.5;
+    } else if (n == 2) {
+        w = 0.5; x = (i == 0) ? 0.788675134594813 : 0.211324865405187;
+    } else {
+        w = (i == 1) ? 0.444444444444444 : 0.277777777777777;
+        x = (i == 0) ? 0.887298334620741 : ((i == 1) ? 0.5 : 0.112701665379259);
+    }
+}
+
+// Tensor ordering of corlib/TensorQuadrature.hpp:100-140 (x innermost);
+// 1-point triangle rule of corlib/SimplexQuadrature.hpp:162-171.
+template <int KIND>
+__device__ inline void quad_point(int q, int n, double xi[3], double& w) {
+    if (KIND == SFH_TRI3) {
+        xi[0] = 0.333333333333333; xi[1] = 0.333333333333333; xi[2] = 0.0; w = 0.5;
+        return;
+    }
+    double w0, w1, w2;
+    if (KIND == SFH_LINE2) {
+        gl_point(n, q, w0, xi[0]); xi[1] = xi[2] = 0.0; w = w0;
+    } else if (KIND == SFH_QUAD4) {
+        gl_point(n, q % n, w0, xi[0]); gl_point(n, q / n, w1, xi[1]); xi[2] = 0.0;
+        w = w0; w *= w1;
+    } else {
+        gl_point(n, q % n, w0, xi[0]); gl_point(n, (q / n) % n, w1, xi[1]);
+        gl_point(n, q / (n * n), w2, xi[2]);
+        w = w0; w *= w1; w *= w2;
+    }
+}
+
+template <int KIND> struct ET;
+template <> struct ET<SFH_LINE2> { static constexpr int nen = 2, dim = 1; };
+template <> struct ET<SFH_TRI3>  { static constexpr int nen = 3, dim = 2; };
+template <> struct ET<SFH_QUAD4> { static constexpr int nen = 4, dim = 2; };
+template <> struct ET<SFH_HEX8>  { static constexpr int nen = 8, dim = 3; };
+
+// shape functions and reference gradients, corlib/Shapefun.ipp
+template <int KIND>
+__device__ inline void shape(const double xi[3], double* phi, double (*dphi)[ET<KIND>::nen]) {
+    const double a = xi[0], b = xi[1], c = xi[2];
+    if (KIND == SFH_LINE2) {
+        phi[0] = 1. - a; phi[1] = a;
+        dphi[0][0] = -1.; dphi[0][1] = 1.;
+    } else if (KIND == SFH_TRI3) {
+        phi[0] = 1. - (a + b); phi[1] = a; phi[2] = b;
+        dphi[0][0] = -1.; dphi[0][1] = 1.; dphi[0][2] = 0.;
+        dphi[1][0] = -1.; dphi[1][1] = 0.; dphi[1][2] = 1.;
+    } else if (KIND == SFH_QUAD4) {
+        phi[0] = (1. - a) * (1. - b); phi[1] = a * (1. - b); phi[2] = a * b; phi[3] = (1. - a) * b;
+        dphi[0][0] = -(1. - b); dphi[0][1] = (1. - b); dphi[0][2] = b; dphi[0][3] = -b;
+        dphi[1][0] = -(1. - a); dphi[1][1] = -a; dphi[1][2] = a; dphi[1][3] = (1. - a);
+    } else {
+        const double ma = 1. - a, mb = 1. - b, mc = 1. - c;
+        phi[0] = ma * mb * mc; phi[1] = a * mb * mc; phi[2] = a * b * mc; phi[3] = ma * b * mc;
+        phi[4] = ma * mb * c;  phi[5] = a * mb * c;  phi[6] = a * b * c;  phi[7] = ma * b * c;
+        dphi[0][0] = -mb * mc; dphi[0][1] = mb * mc; dphi[0][2] = b * mc; dphi[0][3] = -b * mc;
+        dphi[0][4] = -mb * c;  dphi[0][5] = mb * c;  dphi[0][6] = b * c;  dphi[0][7] = -b * c;
+        dphi[1][0] = -ma * mc; dphi[1][1] = -a * mc; dphi[1][2] = a * mc; dphi[1][3] = ma * mc;
+        dphi[1][4] = -ma * c;  dphi[1][5] = -a * c;  dphi[1][6] = a * c;  dphi[1][7] = ma * c;
+        dphi[2][0] = -ma * mb; dphi[2][1] = -a * mb; dphi[2][2] = -a * b; dphi[2][3] = -ma * b;
+        dphi[2][4] = ma * mb;  dphi[2][5] = a * mb;  dphi[2][6] = a * b;  dphi[2][7] = ma * b;
+    }
+}
+
+// corlib/ElementBasic.ipp:177-205 with the square contravariant basis of
+// corlib/linalg.hpp:200-223: g = (J^T)^-1 dphi/dxi, returns det J.
+template <int KIND>
+__device__ inline double gderiv(const double (*X)[ET<KIND>::dim], const double xi[3], double* phi,
+                                double (*g)[ET<KIND>::nen]) {
+    constexpr int D = ET<KIND>::dim, NEN = ET<KIND>::nen;
+    double dphi[D][NEN];
+    shape<KIND>(xi, phi, dphi);
+    double m[D][D];   // m[al][i] = sum_K dphi[al][K] X[K][i]  (= J^T)
+#pragma unroll
+    for (int al = 0; al < D; ++al)
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int K = 0; K < NEN; ++K) s += X[K][i] * dphi[al][K];
+            m[al][i] = s;
+        }
+    double inv[D][D];
+    double det;
+    if (D == 1) {
+        det = m[0][0];
+        inv[0][0] = 1.0 / m[0][0];
+    } else if (D == 2) {
+        det = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+        inv[0][0] = m[1][1] / det; inv[0][1] = -m[0][1] / det;
+        inv[1][0] = -m[1][0] / det; inv[1][1] = m[0][0] / det;
+    } else {
+        const double* a = &m[0][0];
+        det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+              a[2] * (a[3] * a[7] - a[4] * a[6]);
+        double* v = &inv[0][0];
+        v[0] = (a[4] * a[8] - a[5] * a[7]) / det; v[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+        v[2] = (a[1] * a[5] - a[2] * a[4]) / det; v[3] = (a[5] * a[6] - a[3] * a[8]) / det;
+        v[4] = (a[0] * a[8] - a[2] * a[6]) / det; v[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+        v[6] = (a[3] * a[7] - a[4] * a[6]) / det; v[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+        v[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int K = 0; K < NEN; ++K) {
+            double s = 0.0;
+#pragma unroll
+            for (int al = 0; al < D; ++al) s += inv[i][al] * dphi[al][K];
+            g[i][K] = s;
+        }
+    return det;
+}
+
+template <int KIND>
+__device__ inline void load_X(const double* coords, const int* cn, double (*X)[ET<KIND>::dim]) {
+#pragma unroll
+    for (int K = 0; K < ET<KIND>::nen; ++K) {
+        const int n = cn[K];
+#pragma unroll
+        for (int i = 0; i < ET<KIND>::dim; ++i) X[K][i] = coords[(size_t)n * ET<KIND>::dim + i];
+    }
+}
+
+// ---------------------------------------------------------------- pattern
+__global__ void k_adj_count(int nel, int nen, const int* __restrict__ conn, int* cnt) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nel * nen) return;
+    atomicAdd(&cnt[conn[t]], 1);
+}
+
+__global__ void k_adj_fill(int nel, int nen, const int* __restrict__ conn, const int* ptr, int* fill,
+                           int* adj) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nel * nen) return;
+    const int n = conn[t];
+    const int pos = atomicAdd(&fill[n], 1);
+    const int e = (int)(t / nen), a = (int)(t % nen);
+    adj[ptr[n] + pos] = (e << 3) | a;
+}
+
+// sorts each node's adjacency by (element, local index): element order of the
+// reference's mesh.iterateOverElements, independent of atomic arrival order
+__global__ void k_adj_sort(int nnodes, const int* ptr, int* adj, int* err) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnodes) return;
+    const int b = ptr[n], e = ptr[n + 1];
+    if (e - b > kMaxAdj) { atomicOr(err, 1); return; }
+    for (int i = b + 1; i < e; ++i) {
+        const int v = adj[i];
+        int j = i - 1;
+        while (j >= b && adj[j] > v) { adj[j + 1] = adj[j]; --j; }
+        adj[j + 1] = v;
+    }
+}
+
+// distinct coupled nodes of node n in ascending order -> u[0..cnt)
+__device__ inline int node_row(int n, int nen, const int* conn, const int* ptr, const int* adj,
+                               int* u, int* err) {
+    int cnt = 0;
+    for (int s = ptr[n]; s < ptr[n + 1]; ++s) {
+        const int e = adj[s] >> 3;
+        for (int b = 0; b < nen; ++b) {
+            const int c = conn[(size_t)e * nen + b];
+            int pos = 0;
+            while (pos < cnt && u[pos] < c) ++pos;
+            if (pos < cnt && u[pos] == c) continue;
+            if (cnt == kMaxCand) { atomicOr(err, 2); return cnt; }
+            for (int q = cnt; q > pos; --q) u[q] = u[q - 1];
+            u[pos] = c;
+            ++cnt;
+        }
+    }
+    return cnt;
+}
+
+__global__ void k_row_count(int nnodes, int nen, const int* conn, const int* ptr, const int* adj,
+                            int* len, int* err) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnodes) return;
+    int u[kMaxCand];
+    len[n] = node_row(n, nen, conn, ptr, adj, u, err);
+}
+
+__global__ void k_row_fill(int nnodes, int nen, const int* conn, const int* ptr, const int* adj,
+                           const int* rowptr, int* colind, unsigned char* emap, int* err) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnodes) return;
+    int u[kMaxCand];
+    const int cnt = node_row(n, nen, conn, ptr, adj, u, err);
+    const int base = rowptr[n];
+    for (int p = 0; p < cnt; ++p) colind[base + p] = u[p];
+    for (int s = ptr[n]; s < ptr[n + 1]; ++s) {
+        const int e = adj[s] >> 3;
+        for (int b = 0; b < nen; ++b) {
+            const int c = conn[(size_t)e * nen + b];
+            int lo = 0, hi = cnt - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (u[mid] < c) lo = mid + 1; else hi = mid;
+            }
+            emap[(size_t)s * nen + b] = (unsigned char)lo;
+        }
+    }
+}
+
+// interleaved vector DOFs: DOF row 3i+c holds columns 3j+k for every coupled node j
+__global__ void k_expand_pattern(int nnodes, int d, const int* rpn, const int* cin, int* rp, int* ci,
+                                 int* diag) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nnodes * d) return;
+    const int i = r / d, c = r % d;
+    const int b = rpn[i], len = rpn[i + 1] - b;
+    const int start = d * d * b + c * d * len;
+    rp[r] = start;
+    if (r == nnodes * d - 1) rp[r + 1] = d * d * rpn[nnodes];
+    for (int p = 0; p < len; ++p) {
+        const int j = cin[b + p];
+        for (int k = 0; k < d; ++k) ci[start + d * p + k] = d * j + k;
+        if (j == i) diag[r] = start + d * p + c;
+    }
+}
+
+__global__ void k_diagpos(int n, const int* rp, const int* ci, int* diag) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int pos = -1;
+    for (int p = rp[r]; p < rp[r + 1]; ++p)
+        if (ci[p] == r) pos = p;
+    diag[r] = pos;
+}
+
+// exclusive scan of n ints (single block; setup only)
+__global__ void k_scan(int n, const int* in, int* out) {
+    __shared__ long long part[1024];
+    const int t = threadIdx.x, T = blockDim.x;
+    const long long chunk = ((long long)n + T - 1) / T;
+    const long long b = t * chunk, e = b + chunk < n ? b + chunk : n;
+    long long s = 0;
+    for (long long i = b; i < e; ++i) s += in[i];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        long long acc = 0;
+        for (int i = 0; i < T; ++i) { const long long v = part[i]; part[i] = acc; acc += v; }
+    }
+    __syncthreads();
+    long long acc = part[t];
+    for (long long i = b; i < e; ++i) { const int v = in[i]; out[i] = (int)acc; acc += v; }
+    if (t == T - 1) out[n] = (int)acc;
+}
+
+// -------------------------------------------------------------- assembly
+// Scalar integrands (Poisson, Matern). One thread per node row; the row lives
+// in LDS as buf[p * blockDim + tid] so that same-offset accesses of a wave hit
+// distinct banks.
+template <int KIND, int PHYS>
+__global__ __launch_bounds__(kBlock) void k_assemble_scalar(
+    int nnodes, int nq1d, const double* __restrict__ coords, const int* __restrict__ conn,
+    const int* __restrict__ n2e_ptr, const int* __restrict__ n2e, const unsigned char* __restrict__ emap,
+    const int* __restrict__ rowptr, const double* __restrict__ p1, const double* __restrict__ p2,
+    double s1, double s2, double factor, int rowmax, double* __restrict__ vals, float* __restrict__ vals32,
+    int* err) {
+    constexpr int NEN = ET<KIND>::nen, D = ET<KIND>::dim;
+    extern __shared__ double buf[];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + tid;
+    if (i >= nnodes) return;
+    const int r0 = rowptr[i], len = rowptr[i + 1] - r0;
+    for (int p = 0; p < len; ++p) buf[p * kBlock + tid] = 0.0;
+    const int nq = (KIND == SFH_TRI3) ? 1 : (D == 1 ? nq1d : (D == 2 ? nq1d * nq1d : nq1d * nq1d * nq1d));
+    for (int s = n2e_ptr[i]; s < n2e_ptr[i + 1]; ++s) {
+        const int pk = n2e[s];
+        const int e = pk >> 3, a = pk & 7;
+        double X[NEN][D];
+        load_X<KIND>(coords, conn + (size_t)e * NEN, X);
+        const double k1 = p1 ? p1[e] : s1;
+        const double k2 = p2 ? p2[e] : s2;
+        double row[NEN];
+#pragma unroll
+        for (int b = 0; b < NEN; ++b) row[b] = 0.0;
+        for (int q = 0; q < nq; ++q) {
+            double xi[3], w, phi[NEN], g[D][NEN];
+            quad_point<KIND>(q, nq1d, xi, w);
+            const double detJ = gderiv<KIND>(X, xi, phi, g);
+            if (!(detJ > 0.0)) atomicOr(err, 4);
+            if (PHYS == SFH_POISSON) {
+                // del2/fem/ElementPotential.ipp:71-72
+                const double sc = k1 * detJ * w;
+#pragma unroll
+                for (int b = 0; b < NEN; ++b) {
+                    double m = 0.0;
+#pragma unroll
+                    for (int dd = 0; dd < D; ++dd) m += g[dd][a] * g[dd][b];
+                    row[b] += sc * m;
+                }
+            } else {
+                // del2/fem/ElementMatern.ipp:68-79 (scalar diffusivity k2)
+#pragma unroll
+                for (int b = 0; b < NEN; ++b) {
+                    double m = 0.0;
+#pragma unroll
+                    for (int dd = 0; dd < D; ++dd) m += (k2 * g[dd][a]) * g[dd][b];
+                    row[b] += detJ * w * m;
+                }
+                row[a] += detJ * w * (k1 * k1 * phi[a]);
+            }
+        }
+        const unsigned char* em = emap + (size_t)s * NEN;
+#pragma unroll
+        for (int b = 0; b < NEN; ++b) buf[em[b] * kBlock + tid] += row[b] * factor;
+    }
+    for (int p = 0; p < len; ++p) {
+        const double v = buf[p * kBlock + tid];
+        vals[r0 + p] = v;
+        vals32[r0 + p] = (float)v;
+    }
+}
+
+// hex8 linear elasticity at F = I: one thread per DOF row (node i, component c).
+// K[(a,c),(n,k)] += [lambda g_c(a) g_k(n) + mu g_k(a) g_c(n) + mu delta_ck g(a).g(n)] detJ w,
+// i.e. sum_JL C_iJkL dphi_J dphi_L of solid/fem/ElementStatic.ipp:300-316 with the
+// SVenant tensor of solid/material/svenant/SVenant.ipp:107-140 at F = I.
+constexpr int kBlockEl = 128;
+__global__ __launch_bounds__(kBlockEl) void k_assemble_elastic(
+    int ndof, int nq1d, const double* __restrict__ coords, const int* __restrict__ conn,
+    const int* __restrict__ n2e_ptr, const int* __restrict__ n2e, const unsigned char* __restrict__ emap,
+    const int* __restrict__ rowptr, const double* __restrict__ p1, const double* __restrict__ p2,
+    double s1, double s2, double factor, double* __restrict__ vals, float* __restrict__ vals32, int* err) {
+    extern __shared__ double buf[];
+    const int tid = threadIdx.x;
+    const int r = blockIdx.x * blockDim.x + tid;
+    if (r >= ndof) return;
+    const int i = r / 3, c = r % 3;
+    const int r0 = rowptr[r], len = rowptr[r + 1] - r0;
+    for (int p = 0; p < len; ++p) buf[p * kBlockEl + tid] = 0.0;
+    const int nq = nq1d * nq1d * nq1d;
+    for (int s = n2e_ptr[i]; s < n2e_ptr[i + 1]; ++s) {
+        const int pk = n2e[s];
+        const int e = pk >> 3, a = pk & 7;
+        double X[8][3];
+        load_X<SFH_HEX8>(coords, conn + (size_t)e * 8, X);
+        const double E = p1 ? p1[e] : s1;
+        const double nu = p2 ? p2[e] : s2;
+        const double lambda = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu);   // SVenant.ipp:32-37
+        const double mu = E / 2.0 / (1.0 + nu);
+        double row[8][3];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) row[n][0] = row[n][1] = row[n][2] = 0.0;
+        for (int q = 0; q < nq; ++q) {
+            double xi[3], w, phi[8], g[3][8];
+            quad_point<SFH_HEX8>(q, nq1d, xi, w);
+            const double detJ = gderiv<SFH_HEX8>(X, xi, phi, g);
+            if (!(detJ > 0.0)) atomicOr(err, 4);
+            const double dw = detJ * w;
+            const double gca = g[c][a];
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                const double dot = g[0][a] * g[0][n] + g[1][a] * g[1][n] + g[2][a] * g[2][n];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    double kentry = lambda * gca * g[k][n] + mu * g[k][a] * g[c][n];
+                    if (k == c) kentry += mu * dot;
+                    row[n][k] += kentry * dw;
+                }
+            }
+        }
+        const unsigned char* em = emap + (size_t)s * 8;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const int pbase = 3 * em[n];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) buf[(pbase + k) * kBlockEl + tid] += row[n][k] * factor;
+        }
+    }
+    for (int p = 0; p < len; ++p) {
+        const double v = buf[p * kBlockEl + tid];
+        vals[r0 + p] = v;
+        vals32[r0 + p] = (float)v;
+    }
+}
+
+// Body force (del2/fem/ElementPotential.ipp:146-175) gathered per node:
+// F[i*d+c] += w detJ factor f_c(x_q) phi_a(xi_q), f interpolated from nodal values.
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_body_force(
+    int nnodes, int nq1d, int dofpn, const double* __restrict__ coords, const int* __restrict__ conn,
+    const int* __restrict__ n2e_ptr, const int* __restrict__ n2e, double f0, double f1, double f2,
+    const double* __restrict__ fnodal, double factor, int accumulate, double* __restrict__ F, int* err) {
+    constexpr int NEN = ET<KIND>::nen, D = ET<KIND>::dim;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    const int nq = (KIND == SFH_TRI3) ? 1 : (D == 1 ? nq1d : (D == 2 ? nq1d * nq1d : nq1d * nq1d * nq1d));
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int s = n2e_ptr[i]; s < n2e_ptr[i + 1]; ++s) {
+        const int pk = n2e[s];
+        const int e = pk >> 3, a = pk & 7;
+        const int* cn = conn + (size_t)e * NEN;
+        double X[NEN][D];
+        load_X<KIND>(coords, cn, X);
+        for (int q = 0; q < nq; ++q) {
+            double xi[3], w, phi[NEN], g[D][NEN];
+            quad_point<KIND>(q, nq1d, xi, w);
+            const double detJ = gderiv<KIND>(X, xi, phi, g);
+            if (!(detJ > 0.0)) atomicOr(err, 4);
+            for (int c = 0; c < dofpn; ++c) {
+                double f = (c == 0) ? f0 : (c == 1 ? f1 : f2);
+                if (fnodal) {
+#pragma unroll
+                    for (int K = 0; K < NEN; ++K) f += fnodal[(size_t)cn[K] * dofpn + c] * phi[K];
+                }
+                acc[c] += w * detJ * factor * (f * phi[a]);
+            }
+        }
+    }
+    for (int c = 0; c < dofpn; ++c) {
+        const size_t o = (size_t)i * dofpn + c;
+        F[o] = accumulate ? F[o] + acc[c] : acc[c];
+    }
+}
+
+// corlib/Triplet.hpp:219-260 zeroRow + SystemSolveEigenSparse.hpp:208-210
+__global__ void k_constraints(int nc, const int* __restrict__ dofs, const double* __restrict__ values,
+                              double factor, double scalar, const int* __restrict__ rowptr,
+                              const int* __restrict__ colind, double* vals, float* vals32, double* F,
+                              unsigned char* cmask) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc) return;
+    const int r = dofs[t];
+    for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+        const double v = (colind[p] == r) ? scalar : 0.0;
+        vals[p] = v;
+        vals32[p] = (float)v;
+    }
+    F[r] = factor * (values ? values[t] : 0.0) * scalar;
+    cmask[r] = 1;
+}
+
+inline int grid_for(long long n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+int build_pattern(sfh_system* s) {
+    hipStream_t st = s->ctx->stream;
+    const int N = s->nnodes, nen = s->nen;
+    const long long nadj = (long long)s->nel * nen;
+    DevBuf cnt, fill, err;
+    SFH_TRY(cnt.ensure(sizeof(int) * (N + 1)));
+    SFH_TRY(fill.ensure(sizeof(int) * (N + 1)));
+    SFH_TRY(err.ensure(sizeof(int)));
+    SFH_TRY(s->n2e_ptr.ensure(sizeof(int) * (N + 1)));
+    SFH_TRY(s->n2e.ensure(sizeof(int) * nadj));
+    SFH_TRY(s->emap.ensure(nadj * nen));
+    SFH_CHECK_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * (N + 1), st));
+    SFH_CHECK_HIP(hipMemsetAsync(fill.p, 0, sizeof(int) * (N + 1), st));
+    SFH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    const int* conn = s->conn.as<int>();
+    k_adj_count<<<grid_for(nadj, kBlock), kBlock, 0, st>>>(s->nel, nen, conn, cnt.as<int>());
+    SFH_LAUNCH_CHECK();
+    k_scan<<<1, 1024, 0, st>>>(N, cnt.as<int>(), s->n2e_ptr.as<int>());
+    SFH_LAUNCH_CHECK();
+    k_adj_fill<<<grid_for(nadj, kBlock), kBlock, 0, st>>>(s->nel, nen, conn, s->n2e_ptr.as<int>(),
+                                                          fill.as<int>(), s->n2e.as<int>());
+    SFH_LAUNCH_CHECK();
+    k_adj_sort<<<grid_for(N, kBlock), kBlock, 0, st>>>(N, s->n2e_ptr.as<int>(), s->n2e.as<int>(),
+                                                       err.as<int>());
+    SFH_LAUNCH_CHECK();
+    k_row_count<<<grid_for(N, 128), 128, 0, st>>>(N, nen, conn, s->n2e_ptr.as<int>(),
+                                                  s->n2e.as<int>(), cnt.as<int>(), err.as<int>());
+    SFH_LAUNCH_CHECK();
+    SFH_TRY(s->rowptr_node.ensure(sizeof(int) * (N + 1)));
+    k_scan<<<1, 1024, 0, st>>>(N, cnt.as<int>(), s->rowptr_node.as<int>());
+    SFH_LAUNCH_CHECK();
+    int nnz_node = 0, herr = 0;
+    SFH_CHECK_HIP(hipMemcpyAsync(&nnz_node, s->rowptr_node.as<int>() + N, sizeof(int),
+                                 hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (herr) {
+        set_error("mesh exceeds adjacency limits (%d elements per node, %d coupled nodes)", kMaxAdj,
+                  kMaxCand);
+        return SFH_EARG;
+    }
+    s->nnz_node = nnz_node;
+    SFH_TRY(s->colind_node.ensure(sizeof(int) * (size_t)nnz_node));
+    k_row_fill<<<grid_for(N, 128), 128, 0, st>>>(N, nen, conn, s->n2e_ptr.as<int>(), s->n2e.as<int>(),
+                                                 s->rowptr_node.as<int>(), s->colind_node.as<int>(),
+                                                 s->emap.as<unsigned char>(), err.as<int>());
+    SFH_LAUNCH_CHECK();
+    const int d = s->dofpn;
+    s->ndof = N * d;
+    s->nnz = (long long)nnz_node * d * d;
+    if (s->nnz > 0x7fffffffLL) {
+        set_error("nnz %lld exceeds 32-bit indexing", s->nnz);
+        return SFH_EARG;
+    }
+    SFH_TRY(s->diagpos.ensure(sizeof(int) * s->ndof));
+    if (d == 1) {
+        SFH_TRY(s->rowptr.ensure(sizeof(int) * (N + 1)));
+        SFH_TRY(s->colind.ensure(sizeof(int) * (size_t)nnz_node));
+        SFH_CHECK_HIP(hipMemcpyAsync(s->rowptr.p, s->rowptr_node.p, sizeof(int) * (N + 1),
+                                     hipMemcpyDeviceToDevice, st));
+        SFH_CHECK_HIP(hipMemcpyAsync(s->colind.p, s->colind_node.p, sizeof(int) * (size_t)nnz_node,
+                                     hipMemcpyDeviceToDevice, st));
+        k_diagpos<<<grid_for(N, kBlock), kBlock, 0, st>>>(N, s->rowptr.as<int>(), s->colind.as<int>(),
+                                                          s->diagpos.as<int>());
+        SFH_LAUNCH_CHECK();
+    } else {
+        SFH_TRY(s->rowptr.ensure(sizeof(int) * (s->ndof + 1)));
+        SFH_TRY(s->colind.ensure(sizeof(int) * (size_t)s->nnz));
+        k_expand_pattern<<<grid_for(s->ndof, kBlock), kBlock, 0, st>>>(
+            N, d, s->rowptr_node.as<int>(), s->colind_node.as<int>(), s->rowptr.as<int>(),
+            s->colind.as<int>(), s->diagpos.as<int>());
+        SFH_LAUNCH_CHECK();
+    }
+    // longest row bounds the LDS row buffer of the assembly kernels
+    std::vector<int> rp(s->ndof + 1);
+    SFH_CHECK_HIP(hipMemcpyAsync(rp.data(), s->rowptr.p, sizeof(int) * (s->ndof + 1),
+                                 hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (herr) {
+        set_error("pattern build failed (code %d)", herr);
+        return SFH_EARG;
+    }
+    int mx = 0;
+    for (int r = 0; r < s->ndof; ++r) {
+        if (rp[r + 1] - rp[r] <= 0) {
+            set_error("DOF %d belongs to no element", r);
+            return SFH_EARG;
+        }
+        mx = std::max(mx, rp[r + 1] - rp[r]);
+    }
+    s->maxrow = mx;
+    SFH_TRY(s->vals.ensure(sizeof(double) * (size_t)s->nnz));
+    SFH_TRY(s->vals32.ensure(sizeof(float) * (size_t)s->nnz));
+    SFH_TRY(s->F.ensure(sizeof(double) * s->ndof));
+    SFH_TRY(s->cmask.ensure(s->ndof));
+    SFH_TRY(s->dinv32.ensure(sizeof(float) * s->ndof));
+    SFH_TRY(s->dinv64.ensure(sizeof(double) * s->ndof));
+    SFH_CHECK_HIP(hipMemsetAsync(s->vals.p, 0, sizeof(double) * (size_t)s->nnz, st));
+    SFH_CHECK_HIP(hipMemsetAsync(s->vals32.p, 0, sizeof(float) * (size_t)s->nnz, st));
+    SFH_CHECK_HIP(hipMemsetAsync(s->F.p, 0, sizeof(double) * s->ndof, st));
+    SFH_CHECK_HIP(hipMemsetAsync(s->cmask.p, 0, s->ndof, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    return SFH_OK;
+}
+
+int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, const double* p2,
+                    double s1, double s2, double factor) {
+    hipStream_t st = s->ctx->stream;
+    if (nq1d < 1 || nq1d > 3) { set_error("nq1d must be 1..3"); return SFH_EARG; }
+    if (s->kind == SFH_TRI3 && nq1d != 1) { set_error("TRI3 uses the 1-point rule"); return SFH_EARG; }
+    DevBuf err;
+    SFH_TRY(err.ensure(sizeof(int)));
+    SFH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    const int* c = s->conn.as<int>();
+    const double* X = s->coords.as<double>();
+    if (physics == SFH_ELASTICITY) {
+        if (s->kind != SFH_HEX8 || s->dofpn != 3) {
+            set_error("elasticity needs HEX8 elements with 3 DOFs per node");
+            return SFH_EARG;
+        }
+        const size_t lds = sizeof(double) * kBlockEl * s->maxrow;
+        if (lds > s->ctx->lds_per_block) { set_error("row too long for LDS (%d)", s->maxrow); return SFH_EARG; }
+        k_assemble_elastic<<<grid_for(s->ndof, kBlockEl), kBlockEl, lds, st>>>(
+            s->ndof, nq1d, X, c, s->n2e_ptr.as<int>(), s->n2e.as<int>(), s->emap.as<unsigned char>(),
+            s->rowptr.as<int>(), p1, p2, s1, s2, factor, s->vals.as<double>(), s->vals32.as<float>(),
+            err.as<int>());
+        SFH_LAUNCH_CHECK();
+    } else if (physics == SFH_POISSON || physics == SFH_MATERN) {
+        if (s->dofpn != 1) { set_error("scalar integrand needs 1 DOF per node"); return SFH_EARG; }
+        const size_t lds = sizeof(double) * kBlock * s->maxrow;
+        if (lds > s->ctx->lds_per_block) { set_error("row too long for LDS (%d)", s->maxrow); return SFH_EARG; }
+        const int g = grid_for(s->nnodes, kBlock);
+#define SFH_ASM(K, P)                                                                           \
+    k_assemble_scalar<K, P><<<g, kBlock, lds, st>>>(                                            \
+        s->nnodes, nq1d, X, c, s->n2e_ptr.as<int>(), s->n2e.as<int>(), s->emap.as<unsigned char>(), \
+        s->rowptr.as<int>(), p1, p2, s1, s2, factor, s->maxrow, s->vals.as<double>(),          \
+        s->vals32.as<float>(), err.as<int>())
+        if (physics == SFH_POISSON) {
+            switch (s->kind) {
+                case SFH_LINE2: SFH_ASM(SFH_LINE2, SFH_POISSON); break;
+                case SFH_TRI3: SFH_ASM(SFH_TRI3, SFH_POISSON); break;
+                case SFH_QUAD4: SFH_ASM(SFH_QUAD4, SFH_POISSON); break;
+                case SFH_HEX8: SFH_ASM(SFH_HEX8, SFH_POISSON); break;
+            }
+        } else {
+            switch (s->kind) {
+                case SFH_LINE2: SFH_ASM(SFH_LINE2, SFH_MATERN); break;
+                case SFH_TRI3: SFH_ASM(SFH_TRI3, SFH_MATERN); break;
+                case SFH_QUAD4: SFH_ASM(SFH_QUAD4, SFH_MATERN); break;
+                case SFH_HEX8: SFH_ASM(SFH_HEX8, SFH_MATERN); break;
+            }
+        }
+#undef SFH_ASM
+        SFH_LAUNCH_CHECK();
+    } else {
+        set_error("unknown physics %d", physics);
+        return SFH_EARG;
+    }
+    int herr = 0;
+    SFH_CHECK_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (herr & 4) { set_error("Negative Jacobian."); return SFH_ENEGJAC; }
+    SFH_CHECK_HIP(hipMemsetAsync(s->cmask.p, 0, s->ndof, st));
+    s->nc = 0;
+    s->assembled = true;
+    s->dirty = true;
+    return SFH_OK;
+}
+
+int assemble_rhs(sfh_system* s, int nq1d, const double* fconst, const double* fnodal, double factor,
+                 int accumulate, double* out) {
+    hipStream_t st = s->ctx->stream;
+    if (nq1d < 1 || nq1d > 3) { set_error("nq1d must be 1..3"); return SFH_EARG; }
+    DevBuf err;
+    SFH_TRY(err.ensure(sizeof(int)));
+    SFH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    const double f0 = fconst ? fconst[0] : 0.0;
+    const double f1 = (fconst && s->dofpn > 1) ? fconst[1] : 0.0;
+    const double f2 = (fconst && s->dofpn > 2) ? fconst[2] : 0.0;
+    const int g = grid_for(s->nnodes, kBlock);
+#define SFH_BF(K)                                                                                 \
+    k_body_force<K><<<g, kBlock, 0, st>>>(s->nnodes, nq1d, s->dofpn, s->coords.as<double>(),      \
+                                          s->conn.as<int>(), s->n2e_ptr.as<int>(), s->n2e.as<int>(), \
+                                          f0, f1, f2, fnodal, factor, accumulate, out, err.as<int>())
+    switch (s->kind) {
+        case SFH_LINE2: SFH_BF(SFH_LINE2); break;
+        case SFH_TRI3: SFH_BF(SFH_TRI3); break;
+        case SFH_QUAD4: SFH_BF(SFH_QUAD4); break;
+        case SFH_HEX8: SFH_BF(SFH_HEX8); break;
+    }
+#undef SFH_BF
+    SFH_LAUNCH_CHECK();
+    int herr = 0;
+    SFH_CHECK_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (herr & 4) { set_error("Negative Jacobian."); return SFH_ENEGJAC; }
+    return SFH_OK;
+}
+
+int apply_constraints(sfh_system* s, int nc, const int* dofs, const double* values, double factor,
+                      double scalar) {
+    hipStream_t st = s->ctx->stream;
+    if (nc <= 0) return SFH_OK;
+    for (int i = 0; i < nc; ++i)
+        if (dofs[i] < 0 || dofs[i] >= s->ndof) { set_error("constraint dof %d out of range", dofs[i]); return SFH_EARG; }
+    DevBuf d, v;
+    SFH_TRY(d.ensure(sizeof(int) * nc));
+    SFH_CHECK_HIP(hipMemcpyAsync(d.p, dofs, sizeof(int) * nc, hipMemcpyHostToDevice, st));
+    if (values) {
+        SFH_TRY(v.ensure(sizeof(double) * nc));
+        SFH_CHECK_HIP(hipMemcpyAsync(v.p, values, sizeof(double) * nc, hipMemcpyHostToDevice, st));
+    }
+    k_constraints<<<grid_for(nc, kBlock), kBlock, 0, st>>>(nc, d.as<int>(), values ? v.as<double>() : nullptr,
+                                                           factor, scalar, s->rowptr.as<int>(),
+                                                           s->colind.as<int>(), s->vals.as<double>(),
+                                                           s->vals32.as<float>(), s->F.as<double>(),
+                                                           s->cmask.as<unsigned char>());
+    SFH_LAUNCH_CHECK();
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    s->dirty = true;
+    return SFH_OK;
+}
+
+}  // namespace sfh

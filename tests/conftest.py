@@ -18,3 +18,11 @@ def orc():
     from oracle import oracle
     oracle.lib()
     return oracle
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    from statfem_amd import hip
+    c = hip.Context(0)
+    yield c
+    c.close()
