@@ -4,7 +4,8 @@ Workload (one step): Poisson on the unit square, 512 x 512 bilinear Q1 cells
 (263,169 DOFs), element diffusivity exp(GP(ell=0.1, sigma=0.5)), homogeneous
 Dirichlet boundary, 256 sensors. A step is
   GPU assembly (matrix FP64 + FP32 shadow, body force) + row-zero constraints
-  + R^T = A^-T P^T           (256-RHS Jacobi-PCG, FP32 inner / FP64 refinement)
+  + R^T = A^-T P^T           (256-RHS Jacobi-PCG in column tiles of 64, FP32
+                               recurrences with FP64 residual replacement)
   + mean = R F, V = C_f R^T  (separable SE forcing covariance, FP64 MFMA)
   + C_u = R C_f R^T          (Gram, FP64 MFMA)
   + C_u P^T = A^-1 V (+reg)  (second 256-RHS solve) and ubar = A^-1 F.
@@ -44,7 +45,7 @@ def gpu_run(args):
     s.set_grid()
     kappa = ctx.to_device(cfg["kappa"])
     fg = hip.ForwardGaussRHSHip(s, cfg["cols"], cfg["vals"], reg_eps=cfg["reg"])
-    opts = hip.default_opts(tol=args.tol)
+    opts = hip.default_opts(tol=args.tol, rhs_tile=args.tile)
     bc = cfg["bc"]
 
     def step():
@@ -78,16 +79,18 @@ def gpu_run(args):
     asm_ms = ctx.timer_stop() / 10
     s.applyConstraints(bc)
     # per-launch times of the inner PCG kernels (HIP events, library stream)
-    kt = s.profile_pcg(ns, reps=args.kreps)
+    kk = args.tile if 0 < args.tile < ns else ns
+    kt = s.profile_pcg(kk, reps=args.kreps)
     N, nnz = s.ndof, s.nnz
-    kp = (ns + 3) // 4 * 4
+    kp = (kk + 3) // 4 * 4
+    mat = 8.0 * nnz + 4.0 * (N + 1) + 4.0 * N          # FP32 CSR values + columns + row pointers + D^-1
     algo = {
-        "spmv_dot": 8.0 * nnz + 4.0 * (N + 1) + 8.0 * N * kp,     # FP32 CSR + p read + q write
-        "update": 24.0 * N * kp + 4.0 * N,                        # x,r,p,q read; x,r write; D^-1
-        "pupdate": 12.0 * N * kp + 4.0 * N,                       # r,p read; p write; D^-1
+        "dir_spmv": mat + 12.0 * N * kp,                 # r, p_old read; p_new write
+        "cg_update": mat + 20.0 * N * kp,                # p_new, x, r read; x, r write
+        "reduce": 8.0 * kp * ((N + 1023) // 1024),       # per-block partial sums (approx.)
     }
     per = {k: {"ms": kt[k], "GBs": algo[k] / (kt[k] * 1e-3) / 1e9, "bytes": algo[k]} for k in kt}
-    dom = max(kt, key=lambda k: kt[k])
+    dom = max(("dir_spmv", "cg_update"), key=lambda k: kt[k])
     asm_bytes = m.nnodes * 2 * 8 + m.nel * 4 * 4 + m.nel * 8 + nnz * 8 + nnz * 4
     res = {
         "ms_per_step": ms_step, "value": value, "dev_ms_per_step": dev_ms / args.steps,
@@ -145,15 +148,27 @@ def main():
     ap.add_argument("--ns", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--kreps", type=int, default=20)
+    ap.add_argument("--tile", type=int, default=64, help="right-hand sides per PCG column tile")
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-rhs", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-cal", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(see the sharding notes in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     res, dom = gpu_run(args)
+    if args.copy_cal:
+        from statfem_amd import hip
+        c = hip.Context(0)
+        a, b = c.empty((1 << 27,)), c.empty((1 << 27,))
+        a.zero()
+        import ctypes
+        hip._check(hip.lib().sfh_memcpy_d2d(c.h, b.ptr, a.ptr, ctypes.c_size_t(1 << 30)))
+        c.sync()
+        log("copy calibration: 1 GiB device-to-device")
     cpu = None if args.no_cpu else cpu_baseline(args, 30.0)
     line = {
         "metric": "assembled+solved DOF-RHS/s",
@@ -170,7 +185,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"Poisson {args.n}x{args.n} Q1, {res['N']} DOFs, exp(GP) diffusivity, "
                                f"{args.ns} sensors/RHS: assembly + A^-T P^T + C_f + Gram + A^-1 C_f R^T",
-                   "dofs": res["N"], "rhs": args.ns, "nnz": res["nnz"], "tol": args.tol,
+                   "dofs": res["N"], "rhs": args.ns, "nnz": res["nnz"], "tol": args.tol, "rhs_tile": args.tile,
                    "inner_iters": res["inner_iters"], "max_relres": res["max_relres"],
                    "kernels": res["kernels"], "assembly": res["assembly"]},
         "roofline": {"kernel": res["kernel"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,

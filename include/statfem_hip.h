@@ -122,16 +122,18 @@ int sfh_system_device_ptrs(sfh_system* sys, const int** rowptr_dev, const int** 
 /* ------------------------------------------------------------------ solve */
 typedef struct {
     double tol;          /* FP64 relative residual target per RHS (default 1e-10) */
-    double inner_tol;    /* FP32 inner PCG relative reduction (default 1e-5) */
-    int max_outer;       /* refinement steps (default 8) */
-    int max_inner;       /* inner PCG iteration cap per refinement step (default 20000) */
+    double inner_tol;    /* FP64 residual replacement once the FP32 residual fell by this factor (default 0.01) */
+    int max_outer;       /* cap on residual replacements (default 500) */
+    int max_inner;       /* cap on PCG iterations (default 50000) */
     int check_every;     /* inner iterations between host convergence checks (default 32) */
-    int precision;       /* 0: FP32 inner + FP64 refinement; 1: FP64 PCG throughout */
+    int precision;       /* 0: FP32 recurrences + FP64 replacement; 1: FP64 throughout */
+    int rhs_tile;        /* >0: solve the right-hand sides in column tiles of this width (0: all at once) */
+    int layout;          /* 0: auto, 1: CSR, 2: stencil layout on declared 2D tensor grids */
 } sfh_solve_opts;
 
 typedef struct {
-    int outer;            /* refinement steps taken */
-    int inner_iters;      /* total inner PCG iterations */
+    int outer;            /* FP64 residual replacements */
+    int inner_iters;      /* PCG iterations */
     double max_relres;    /* max over RHS of ||b - A x|| / ||b|| in FP64 */
     double ms;            /* device time */
     int converged;
@@ -150,7 +152,8 @@ int sfh_solve_rhs(sfh_system* sys, double* X_dev, const sfh_solve_opts* opts,
                   sfh_solve_stats* stats);
 
 /* Instrumentation, no reference counterpart: average device time per launch
- * (ms) of the inner-PCG kernels (SpMV+dot, update+dots, direction update) on
+ * (ms) of the inner-PCG kernels (direction+SpMV+dot, SpMV+update+dots, one
+ * per-RHS reduction) on
  * the workspace of the last k-RHS FP32 solve, timed with HIP events on the
  * context stream. */
 int sfh_profile_pcg(sfh_system* sys, int k, int reps, double* ms_spmv, double* ms_update,

@@ -300,6 +300,7 @@ int sfh_system_set_grid(sfh_system* s, int nx, int ny, int nz, const double* gx,
         SFH_CHECK_HIP(hipMemcpyAsync(s->gaxis[a].p, g[a], sizeof(double) * n[a], hipMemcpyHostToDevice, s->ctx->stream));
         s->gn[a] = n[a];
     }
+    s->dia_state = 0;
     SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
     return SFH_OK;
 }
@@ -338,6 +339,7 @@ int sfh_clear_matrix(sfh_system* s) {
     SFH_CHECK_HIP(hipMemsetAsync(s->cmask.p, 0, s->ndof, s->ctx->stream));
     s->nc = 0;
     s->dirty = true;
+    if (s->dia_state > 0) s->dia_state = 0;
     return SFH_OK;
 }
 

@@ -107,6 +107,10 @@ struct sfh_system {
     int gn[3] = {0, 0, 0};
     sfh::DevBuf gaxis[3];
     int nq1d = 2;                // quadrature of the last matrix assembly
+    // stencil (DIA-9) copy of the operator for 2D tensor grids: coefficient k of
+    // node (x,y) couples (x + k%3 - 1, y + k/3 - 1); built lazily per assembly
+    sfh::DevBuf dia32, dia64;
+    int dia_state = 0;           // 0 stale, 1 valid, -1 mesh does not fit the stencil
     sfh::SolveWork work;
     sfh::DevBuf scratch0, scratch1, scratch2, scratch3;
     sfh::DevBuf fg_pt, fg_rt, fg_v, fg_cv, fg_cu, fg_gram, fg_cols, fg_vals;

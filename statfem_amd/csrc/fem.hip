@@ -635,6 +635,7 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
     s->nc = 0;
     s->assembled = true;
     s->dirty = true;
+    if (s->dia_state > 0) s->dia_state = 0;
     return SFH_OK;
 }
 
@@ -689,6 +690,7 @@ int apply_constraints(sfh_system* s, int nc, const int* dofs, const double* valu
     SFH_LAUNCH_CHECK();
     SFH_CHECK_HIP(hipStreamSynchronize(st));
     s->dirty = true;
+    if (s->dia_state > 0) s->dia_state = 0;
     return SFH_OK;
 }
 

@@ -4,8 +4,13 @@
 // (corlib/SystemSolveEigenSparse.hpp:273-404 called in a loop by
 // statX/infer/ForwardGaussRHS.hpp:105-112) with one batched iterative solve:
 //
-//   outer (FP64): r = b - A x, per-RHS ||r||/||b|| test, x += d / scale
-//   inner (FP32): Jacobi-PCG on A d = scale * r, every RHS with its own alpha/beta
+//   FP32: Jacobi-PCG recurrences, every RHS with its own alpha / beta, two
+//         fused kernels per iteration (direction + SpMV + p.Ap; SpMV + x/r
+//         update + r.z, r.r), 32 bytes of vector traffic per DOF-RHS;
+//   FP64: residual replacement -- the FP32 correction is flushed into the FP64
+//         iterate and r = b - A x is recomputed in FP64 whenever a column's
+//         recursive residual fell by inner_tol since the last replacement; the
+//         search direction is kept, so no Krylov information is lost.
 //
 // Layout: blocks are N x kp row-major (kp = k rounded up to 4), so one CSR row
 // feeds a contiguous run of right-hand sides; a group of LPR lanes handles one
@@ -24,8 +29,8 @@
 namespace sfh {
 namespace {
 
-constexpr int kB = 256;
-constexpr int kRedB = 256;
+constexpr int kB = 1024;   // 16 waves per work-group: 16 waves per SIMD at 1024 work-groups
+constexpr int kRedB = 1024;   // reduction blocks: 16 row groups x 64 columns
 
 template <class T> struct V4 { T x, y, z, w; };
 
@@ -87,54 +92,138 @@ __device__ inline void block_partial(const Tile& t, int kp, const V4<double>& a,
     }
 }
 
-// q = A p ; part0 = sum_rows p.q
+// row index of the lane group; wave-uniform (scalar) when one row spans the wave
+template <int LPR>
+__device__ inline int uniform_row(int r) {
+    if (LPR == 64) return __builtin_amdgcn_readfirstlane(r);
+    return r;
+}
+
+// preconditioned direction entry p = D^-1 r + beta p_old, evaluated identically
+// wherever it is needed (fused into the SpMV gathers)
+__device__ inline float pdir(float d, float r, float b, float po) { return fmaf(b, po, d * r); }
+__device__ inline double pdir(double d, double r, double b, double po) { return fma(b, po, d * r); }
+
+template <class T>
+__device__ inline V4<T> mul4(const V4<T>& a, const V4<T>& b) {
+    return {a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w};
+}
+
+template <class T>
+__device__ inline void pdir4(T d, const V4<T>& r, const V4<T>& b, const V4<T>& po, V4<T>& out) {
+    out.x = pdir(d, r.x, b.x, po.x); out.y = pdir(d, r.y, b.y, po.y);
+    out.z = pdir(d, r.z, b.z, po.z); out.w = pdir(d, r.w, b.w, po.w);
+}
+
+// K1: p_new = D^-1 r + beta p_old (every row, recomputed on the fly at the
+// neighbours the stencil touches), q = A p_new, part = sum_rows p_new . q.
+// q itself is not stored.
 template <class T, int LPR>
-__global__ __launch_bounds__(kB) void k_spmv_dot(int n, int kp, int rpb, const int* __restrict__ rp,
+__global__ __launch_bounds__(kB) void k_dir_spmv(int n, int kp, int rpb, const int* __restrict__ rp,
                                                  const int* __restrict__ ci, const T* __restrict__ va,
-                                                 const T* __restrict__ p, T* __restrict__ q,
+                                                 const T* __restrict__ dinv, const double* __restrict__ beta,
+                                                 const double* __restrict__ active, const T* __restrict__ r,
+                                                 const T* __restrict__ pold, T* __restrict__ pnew,
                                                  double* __restrict__ part) {
     __shared__ double red[kB * 4];
     const Tile t = tile<LPR>(n, kp, rpb);
     const bool ok = t.col0 < kp;
     V4<T> acc = {0, 0, 0, 0};
     if (ok) {
-        for (int r = t.r0 + t.grp; r < t.r1; r += t.ngrp) {
-            V4<T> s = {0, 0, 0, 0};
-            const int e1 = rp[r + 1];
-            for (int e = rp[r]; e < e1; ++e) {
-                const T a = va[e];
-                const V4<T> v = ld4(p + (size_t)ci[e] * kp + t.col0);
-                s.x += a * v.x; s.y += a * v.y; s.z += a * v.z; s.w += a * v.w;
+        // frozen columns (waiting for residual replacement) keep their direction:
+        // p = 1 * p_old + D^-1 (0 * r)
+        V4<T> b, m;
+        {
+            const double* bb = beta + t.col0;
+            const double* aa = active + t.col0;
+            b.x = aa[0] > 0.0 ? (T)bb[0] : (T)1; m.x = aa[0] > 0.0 ? (T)1 : (T)0;
+            b.y = aa[1] > 0.0 ? (T)bb[1] : (T)1; m.y = aa[1] > 0.0 ? (T)1 : (T)0;
+            b.z = aa[2] > 0.0 ? (T)bb[2] : (T)1; m.z = aa[2] > 0.0 ? (T)1 : (T)0;
+            b.w = aa[3] > 0.0 ? (T)bb[3] : (T)1; m.w = aa[3] > 0.0 ? (T)1 : (T)0;
+        }
+        for (int row0 = t.r0 + t.grp; row0 < t.r1; row0 += t.ngrp) {
+            const int row = uniform_row<LPR>(row0);
+            V4<T> q = {0, 0, 0, 0};
+            int e = rp[row];
+            const int e1 = rp[row + 1];
+            for (; e + 4 <= e1; e += 4) {
+                int c[4];
+                T a[4], d[4];
+                V4<T> rv[4], pv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) { c[u] = ci[e + u]; a[u] = va[e + u]; }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    d[u] = dinv[c[u]];
+                    rv[u] = mul4(ld4(r + (size_t)c[u] * kp + t.col0), m);
+                    pv[u] = ld4(pold + (size_t)c[u] * kp + t.col0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    V4<T> pj;
+                    pdir4(d[u], rv[u], b, pv[u], pj);
+                    q.x += a[u] * pj.x; q.y += a[u] * pj.y; q.z += a[u] * pj.z; q.w += a[u] * pj.w;
+                }
             }
-            st4(q + (size_t)r * kp + t.col0, s);
-            const V4<T> pr = ld4(p + (size_t)r * kp + t.col0);
-            acc.x += pr.x * s.x; acc.y += pr.y * s.y; acc.z += pr.z * s.z; acc.w += pr.w * s.w;
+            for (; e < e1; ++e) {
+                const int c = ci[e];
+                const T a = va[e];
+                V4<T> pj;
+                pdir4(dinv[c], mul4(ld4(r + (size_t)c * kp + t.col0), m), b, ld4(pold + (size_t)c * kp + t.col0), pj);
+                q.x += a * pj.x; q.y += a * pj.y; q.z += a * pj.z; q.w += a * pj.w;
+            }
+            const size_t o = (size_t)row * kp + t.col0;
+            V4<T> pi;
+            pdir4(dinv[row], mul4(ld4(r + o), m), b, ld4(pold + o), pi);
+            st4(pnew + o, pi);
+            acc.x += pi.x * q.x; acc.y += pi.y * q.y; acc.z += pi.z * q.z; acc.w += pi.w * q.w;
         }
     }
     const V4<double> ad = {(double)acc.x, (double)acc.y, (double)acc.z, (double)acc.w};
     block_partial<LPR>(t, kp, ad, red, part);
 }
 
-// x += alpha p ; r -= alpha q ; z = D^-1 r ; part0 = r.z ; part1 = r.r
+// K2: q = A p (recomputed), x += alpha p, r -= alpha q, part0 = r.D^-1 r, part1 = r.r
 template <class T, int LPR>
-__global__ __launch_bounds__(kB) void k_update(int n, int kp, int rpb, const double* __restrict__ alpha,
-                                               const T* __restrict__ dinv, T* __restrict__ x,
-                                               T* __restrict__ r, const T* __restrict__ p,
-                                               const T* __restrict__ q, double* __restrict__ part0,
-                                               double* __restrict__ part1) {
+__global__ __launch_bounds__(kB) void k_cg_update(int n, int kp, int rpb, const int* __restrict__ rp,
+                                                  const int* __restrict__ ci, const T* __restrict__ va,
+                                                  const T* __restrict__ dinv, const double* __restrict__ alpha,
+                                                  const T* __restrict__ p, T* __restrict__ x, T* __restrict__ r,
+                                                  double* __restrict__ part0, double* __restrict__ part1) {
     __shared__ double red[kB * 4];
     const Tile t = tile<LPR>(n, kp, rpb);
     const bool ok = t.col0 < kp;
     V4<T> rz = {0, 0, 0, 0}, rr = {0, 0, 0, 0};
     if (ok) {
-        const T a0 = (T)alpha[t.col0], a1 = (T)alpha[t.col0 + 1], a2 = (T)alpha[t.col0 + 2],
-                a3 = (T)alpha[t.col0 + 3];
-        for (int row = t.r0 + t.grp; row < t.r1; row += t.ngrp) {
+        const V4<T> al = {(T)alpha[t.col0], (T)alpha[t.col0 + 1], (T)alpha[t.col0 + 2], (T)alpha[t.col0 + 3]};
+        for (int row0 = t.r0 + t.grp; row0 < t.r1; row0 += t.ngrp) {
+            const int row = uniform_row<LPR>(row0);
+            V4<T> q = {0, 0, 0, 0};
+            int e = rp[row];
+            const int e1 = rp[row + 1];
+            for (; e + 4 <= e1; e += 4) {
+                int c[4];
+                T a[4];
+                V4<T> pv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) { c[u] = ci[e + u]; a[u] = va[e + u]; }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) pv[u] = ld4(p + (size_t)c[u] * kp + t.col0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    q.x += a[u] * pv[u].x; q.y += a[u] * pv[u].y; q.z += a[u] * pv[u].z; q.w += a[u] * pv[u].w;
+                }
+            }
+            for (; e < e1; ++e) {
+                const T a = va[e];
+                const V4<T> pv = ld4(p + (size_t)ci[e] * kp + t.col0);
+                q.x += a * pv.x; q.y += a * pv.y; q.z += a * pv.z; q.w += a * pv.w;
+            }
             const size_t o = (size_t)row * kp + t.col0;
+            const V4<T> pi = ld4(p + o);
             V4<T> xv = ld4(x + o), rv = ld4(r + o);
-            const V4<T> pv = ld4(p + o), qv = ld4(q + o);
-            xv.x += a0 * pv.x; xv.y += a1 * pv.y; xv.z += a2 * pv.z; xv.w += a3 * pv.w;
-            rv.x -= a0 * qv.x; rv.y -= a1 * qv.y; rv.z -= a2 * qv.z; rv.w -= a3 * qv.w;
+            xv.x += al.x * pi.x; xv.y += al.y * pi.y; xv.z += al.z * pi.z; xv.w += al.w * pi.w;
+            rv.x -= al.x * q.x; rv.y -= al.y * q.y; rv.z -= al.z * q.z; rv.w -= al.w * q.w;
             st4(x + o, xv);
             st4(r + o, rv);
             const T d = dinv[row];
@@ -145,62 +234,204 @@ __global__ __launch_bounds__(kB) void k_update(int n, int kp, int rpb, const dou
     V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
     block_partial<LPR>(t, kp, a, red, part0);
     __syncthreads();
-    V4<double> b = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
-    block_partial<LPR>(t, kp, b, red, part1);
+    V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
+    block_partial<LPR>(t, kp, bb, red, part1);
 }
 
-// p = D^-1 r + beta p
+// Residual replacement: r = s' r64, p *= ratio (s'/s), part0 = r.D^-1 r, part1 = r.r
 template <class T, int LPR>
-__global__ __launch_bounds__(kB) void k_pupdate(int n, int kp, int rpb, const double* __restrict__ beta,
-                                                const T* __restrict__ dinv, const T* __restrict__ r,
-                                                T* __restrict__ p) {
-    const Tile t = tile<LPR>(n, kp, rpb);
-    if (t.col0 >= kp) return;
-    const T b0 = (T)beta[t.col0], b1 = (T)beta[t.col0 + 1], b2 = (T)beta[t.col0 + 2],
-            b3 = (T)beta[t.col0 + 3];
-    for (int row = t.r0 + t.grp; row < t.r1; row += t.ngrp) {
-        const size_t o = (size_t)row * kp + t.col0;
-        const T d = dinv[row];
-        const V4<T> rv = ld4(r + o);
-        V4<T> pv = ld4(p + o);
-        pv.x = rv.x * d + b0 * pv.x; pv.y = rv.y * d + b1 * pv.y;
-        pv.z = rv.z * d + b2 * pv.z; pv.w = rv.w * d + b3 * pv.w;
-        st4(p + o, pv);
-    }
-}
-
-// inner start: r = scale * r64 ; x = 0 ; p = D^-1 r ; part0 = r.z ; part1 = r.r
-template <class T, int LPR>
-__global__ __launch_bounds__(kB) void k_init(int n, int kp, int rpb, const double* __restrict__ scale,
-                                             const T* __restrict__ dinv, const double* __restrict__ r64,
-                                             T* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
-                                             double* __restrict__ part0, double* __restrict__ part1) {
+__global__ __launch_bounds__(kB) void k_rescale(int n, int kp, int rpb, const double* __restrict__ scale,
+                                                const double* __restrict__ ratio, const T* __restrict__ dinv,
+                                                const double* __restrict__ r64, T* __restrict__ r, T* __restrict__ p,
+                                                double* __restrict__ part0, double* __restrict__ part1) {
     __shared__ double red[kB * 4];
     const Tile t = tile<LPR>(n, kp, rpb);
     const bool ok = t.col0 < kp;
     V4<T> rz = {0, 0, 0, 0}, rr = {0, 0, 0, 0};
     if (ok) {
-        const double s0 = scale[t.col0], s1 = scale[t.col0 + 1], s2 = scale[t.col0 + 2],
-                     s3 = scale[t.col0 + 3];
+        const double s0 = scale[t.col0], s1 = scale[t.col0 + 1], s2 = scale[t.col0 + 2], s3 = scale[t.col0 + 3];
+        const V4<T> q = {(T)ratio[t.col0], (T)ratio[t.col0 + 1], (T)ratio[t.col0 + 2], (T)ratio[t.col0 + 3]};
         for (int row = t.r0 + t.grp; row < t.r1; row += t.ngrp) {
             const size_t o = (size_t)row * kp + t.col0;
             const V4<double> rd = ld4(r64 + o);
             const V4<T> rv = {(T)(rd.x * s0), (T)(rd.y * s1), (T)(rd.z * s2), (T)(rd.w * s3)};
-            const T d = dinv[row];
-            const V4<T> z0 = {0, 0, 0, 0};
-            st4(x + o, z0);
+            V4<T> pv = ld4(p + o);
+            pv.x *= q.x; pv.y *= q.y; pv.z *= q.z; pv.w *= q.w;
             st4(r + o, rv);
-            const V4<T> pv = {rv.x * d, rv.y * d, rv.z * d, rv.w * d};
             st4(p + o, pv);
-            rz.x += rv.x * pv.x; rz.y += rv.y * pv.y; rz.z += rv.z * pv.z; rz.w += rv.w * pv.w;
+            const T d = dinv[row];
+            rz.x += rv.x * rv.x * d; rz.y += rv.y * rv.y * d; rz.z += rv.z * rv.z * d; rz.w += rv.w * rv.w * d;
             rr.x += rv.x * rv.x; rr.y += rv.y * rv.y; rr.z += rv.z * rv.z; rr.w += rv.w * rv.w;
         }
     }
     V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
     block_partial<LPR>(t, kp, a, red, part0);
     __syncthreads();
-    V4<double> b = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
-    block_partial<LPR>(t, kp, b, red, part1);
+    V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
+    block_partial<LPR>(t, kp, bb, red, part1);
+}
+
+// ------------------------------------------------ 2D tensor-grid (DIA-9) path
+// Nodes are numbered x fastest on a W x H grid. A lane group walks a segment of
+// one grid row with a 3 x 3 register window: each step loads the three nodes
+// of column x+1 (rows y-1, y, y+1) instead of gathering all nine neighbours.
+struct GridTile {
+    int lane, grp, col0, bid, seg;
+};
+
+template <int LPR>
+__device__ inline GridTile grid_tile() {
+    GridTile g;
+    g.lane = threadIdx.x % LPR;
+    g.grp = threadIdx.x / LPR;
+    g.col0 = (blockIdx.y * LPR + g.lane) * 4;
+    g.bid = xcd_block(blockIdx.x, gridDim.x);
+    g.seg = g.bid * (blockDim.x / LPR) + g.grp;
+    return g;
+}
+
+// Tile-compatible view for block_partial
+template <int LPR>
+__device__ inline Tile as_tile(const GridTile& g) {
+    Tile t;
+    t.lane = g.lane; t.grp = g.grp; t.ngrp = blockDim.x / LPR; t.col0 = g.col0; t.bid = g.bid;
+    t.r0 = t.r1 = 0;
+    return t;
+}
+
+template <class T, int LPR>
+__global__ __launch_bounds__(kB) void k_dir_spmv_g2(int W, int H, int kp, int L, int nsegx, const T* __restrict__ coef,
+                                                    const T* __restrict__ dinv, const double* __restrict__ beta,
+                                                    const double* __restrict__ active, const T* __restrict__ r,
+                                                    const T* __restrict__ pold, T* __restrict__ pnew,
+                                                    double* __restrict__ part) {
+    __shared__ double red[kB * 4];
+    const GridTile g = grid_tile<LPR>();
+    V4<T> acc = {0, 0, 0, 0};
+    if (g.col0 < kp && g.seg < nsegx * H) {
+        V4<T> b, m;
+        {
+            const double* bb = beta + g.col0;
+            const double* aa = active + g.col0;
+            b.x = aa[0] > 0.0 ? (T)bb[0] : (T)1; m.x = aa[0] > 0.0 ? (T)1 : (T)0;
+            b.y = aa[1] > 0.0 ? (T)bb[1] : (T)1; m.y = aa[1] > 0.0 ? (T)1 : (T)0;
+            b.z = aa[2] > 0.0 ? (T)bb[2] : (T)1; m.z = aa[2] > 0.0 ? (T)1 : (T)0;
+            b.w = aa[3] > 0.0 ? (T)bb[3] : (T)1; m.w = aa[3] > 0.0 ? (T)1 : (T)0;
+        }
+        const int y = uniform_row<LPR>(g.seg / nsegx);
+        const int xs = uniform_row<LPR>((g.seg % nsegx) * L);
+        const int xe = min(W, xs + L);
+        auto pval = [&](int x, int yy) -> V4<T> {
+            V4<T> v = {0, 0, 0, 0};
+            if (x >= 0 && x < W && yy >= 0 && yy < H) {
+                const int nd = yy * W + x;
+                const size_t o = (size_t)nd * kp + g.col0;
+                pdir4(dinv[nd], mul4(ld4(r + o), m), b, ld4(pold + o), v);
+            }
+            return v;
+        };
+        V4<T> w0[3], w1[3], w2[3];   // columns x-1, x, x+1; index = dy + 1
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { w0[d] = pval(xs - 1, y + d - 1); w1[d] = pval(xs, y + d - 1); }
+        for (int x = xs; x < xe; ++x) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) w2[d] = pval(x + 1, y + d - 1);
+            const int nd = y * W + x;
+            const T* c = coef + (size_t)nd * 9;
+            V4<T> q = {0, 0, 0, 0};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const T c0 = c[d * 3], c1 = c[d * 3 + 1], c2 = c[d * 3 + 2];
+                q.x += c0 * w0[d].x + c1 * w1[d].x + c2 * w2[d].x;
+                q.y += c0 * w0[d].y + c1 * w1[d].y + c2 * w2[d].y;
+                q.z += c0 * w0[d].z + c1 * w1[d].z + c2 * w2[d].z;
+                q.w += c0 * w0[d].w + c1 * w1[d].w + c2 * w2[d].w;
+            }
+            const V4<T> pi = w1[1];
+            st4(pnew + (size_t)nd * kp + g.col0, pi);
+            acc.x += pi.x * q.x; acc.y += pi.y * q.y; acc.z += pi.z * q.z; acc.w += pi.w * q.w;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) { w0[d] = w1[d]; w1[d] = w2[d]; }
+        }
+    }
+    const V4<double> ad = {(double)acc.x, (double)acc.y, (double)acc.z, (double)acc.w};
+    block_partial<LPR>(as_tile<LPR>(g), kp, ad, red, part);
+}
+
+template <class T, int LPR>
+__global__ __launch_bounds__(kB) void k_cg_update_g2(int W, int H, int kp, int L, int nsegx, const T* __restrict__ coef,
+                                                     const T* __restrict__ dinv, const double* __restrict__ alpha,
+                                                     const T* __restrict__ p, T* __restrict__ x, T* __restrict__ r,
+                                                     double* __restrict__ part0, double* __restrict__ part1) {
+    __shared__ double red[kB * 4];
+    const GridTile g = grid_tile<LPR>();
+    V4<T> rz = {0, 0, 0, 0}, rr = {0, 0, 0, 0};
+    if (g.col0 < kp && g.seg < nsegx * H) {
+        const V4<T> al = {(T)alpha[g.col0], (T)alpha[g.col0 + 1], (T)alpha[g.col0 + 2], (T)alpha[g.col0 + 3]};
+        const int y = uniform_row<LPR>(g.seg / nsegx);
+        const int xs = uniform_row<LPR>((g.seg % nsegx) * L);
+        const int xe = min(W, xs + L);
+        auto pv = [&](int xx, int yy) -> V4<T> {
+            V4<T> v = {0, 0, 0, 0};
+            if (xx >= 0 && xx < W && yy >= 0 && yy < H) v = ld4(p + (size_t)(yy * W + xx) * kp + g.col0);
+            return v;
+        };
+        V4<T> w0[3], w1[3], w2[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { w0[d] = pv(xs - 1, y + d - 1); w1[d] = pv(xs, y + d - 1); }
+        for (int xx = xs; xx < xe; ++xx) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) w2[d] = pv(xx + 1, y + d - 1);
+            const int nd = y * W + xx;
+            const T* c = coef + (size_t)nd * 9;
+            V4<T> q = {0, 0, 0, 0};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const T c0 = c[d * 3], c1 = c[d * 3 + 1], c2 = c[d * 3 + 2];
+                q.x += c0 * w0[d].x + c1 * w1[d].x + c2 * w2[d].x;
+                q.y += c0 * w0[d].y + c1 * w1[d].y + c2 * w2[d].y;
+                q.z += c0 * w0[d].z + c1 * w1[d].z + c2 * w2[d].z;
+                q.w += c0 * w0[d].w + c1 * w1[d].w + c2 * w2[d].w;
+            }
+            const size_t o = (size_t)nd * kp + g.col0;
+            const V4<T> pi = w1[1];
+            V4<T> xv = ld4(x + o), rv = ld4(r + o);
+            xv.x += al.x * pi.x; xv.y += al.y * pi.y; xv.z += al.z * pi.z; xv.w += al.w * pi.w;
+            rv.x -= al.x * q.x; rv.y -= al.y * q.y; rv.z -= al.z * q.z; rv.w -= al.w * q.w;
+            st4(x + o, xv);
+            st4(r + o, rv);
+            const T d = dinv[nd];
+            rz.x += rv.x * rv.x * d; rz.y += rv.y * rv.y * d; rz.z += rv.z * rv.z * d; rz.w += rv.w * rv.w * d;
+            rr.x += rv.x * rv.x; rr.y += rv.y * rv.y; rr.z += rv.z * rv.z; rr.w += rv.w * rv.w;
+#pragma unroll
+            for (int d2 = 0; d2 < 3; ++d2) { w0[d2] = w1[d2]; w1[d2] = w2[d2]; }
+        }
+    }
+    const Tile t = as_tile<LPR>(g);
+    V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
+    block_partial<LPR>(t, kp, a, red, part0);
+    __syncthreads();
+    V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
+    block_partial<LPR>(t, kp, bb, red, part1);
+}
+
+// CSR -> DIA-9. Every entry must couple a grid neighbour of its row's node;
+// anything else (err) keeps the solver on the CSR kernels.
+__global__ void k_build_dia9(int W, int H, const int* __restrict__ rp, const int* __restrict__ ci,
+                             const double* __restrict__ va, float* d32, double* d64, int* err) {
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nd >= W * H) return;
+    const int x = nd % W, y = nd / W;
+    for (int k = 0; k < 9; ++k) { d32[(size_t)nd * 9 + k] = 0.f; d64[(size_t)nd * 9 + k] = 0.0; }
+    for (int e = rp[nd]; e < rp[nd + 1]; ++e) {
+        const int c = ci[e];
+        const int cx = c % W, cy = c / W;
+        const int dx = cx - x, dy = cy - y;
+        if (dx < -1 || dx > 1 || dy < -1 || dy > 1) { atomicOr(err, 1); continue; }
+        const int k = (dy + 1) * 3 + (dx + 1);
+        d32[(size_t)nd * 9 + k] = (float)va[e];
+        d64[(size_t)nd * 9 + k] = va[e];
+    }
 }
 
 // FP64 residual r = b - A x ; part0 = r.r
@@ -229,30 +460,42 @@ __global__ __launch_bounds__(kB) void k_resid64(int n, int kp, int rpb, const in
     block_partial<LPR>(t, kp, acc, red, part);
 }
 
-// x64 += x_inner / scale on active columns (scale > 0)
+// x64 += x_inner / scale on active columns (scale > 0); x_inner restarts at 0
 template <class T>
-__global__ void k_refine(long long total, int kp, const double* __restrict__ scale,
-                         const T* __restrict__ xin, double* __restrict__ x64) {
+__global__ void k_refine(long long total, int kp, const double* __restrict__ scale, T* __restrict__ xin,
+                         double* __restrict__ x64) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const double s = scale[i % kp];
     if (s > 0.0) x64[i] += (double)xin[i] / s;
+    xin[i] = (T)0;
 }
 
 // ------------------------------------------------------------ reductions
-// scalar slots (each kp doubles)
-enum { S_RZ = 0, S_ALPHA, S_BETA, S_RR0, S_ACTIVE, S_SCALE, S_RN2, S_BN2, S_TOL2, S_NSLOT };
+// per-RHS scalar slots (each kp doubles)
+enum { S_RZ = 0, S_RZPREV, S_ALPHA, S_BETA, S_TRIG, S_TGT, S_ACTIVE, S_SCALE, S_RATIO, S_RN2, S_BN2,
+       S_NSLOT };
 
-__device__ inline double sum_parts(int nblk, int kp, const double* part, int col, double* red) {
+// Fixed-order sum over the nblk per-block partial rows of one column: 16
+// groups of 64 lanes, each group strides over the rows with 8 independent
+// loads in flight, then a fixed-order combine of the 16 group sums.
+__device__ inline double sum_parts(int nblk, int kp, const double* __restrict__ part, int col, double* red) {
+    constexpr int G = kRedB / 64;
     const int g = threadIdx.x / 64, l = threadIdx.x % 64;
-    double s = 0.0;
-    if (col < kp)
-        for (int b = g; b < nblk; b += kRedB / 64) s += part[(size_t)b * kp + col];
-    red[g * 64 + l] = s;
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (col < kp) {
+        int b = g;
+        for (; b + 7 * G < nblk; b += 8 * G) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] += part[(size_t)(b + u * G) * kp + col];
+        }
+        for (; b < nblk; b += G) a[0] += part[(size_t)b * kp + col];
+    }
+    red[g * 64 + l] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     __syncthreads();
     double tot = 0.0;
     if (g == 0)
-        for (int q = 0; q < kRedB / 64; ++q) tot += red[q * 64 + l];
+        for (int q = 0; q < G; ++q) tot += red[q * 64 + l];
     __syncthreads();
     return tot;
 }
@@ -267,42 +510,54 @@ __global__ __launch_bounds__(kRedB) void k_red_alpha(int kp, int nblk, const dou
     }
 }
 
+// beta, rz; a column whose recursive residual reached its trigger is frozen
+// (alpha = 0, direction kept) until the next FP64 replacement. Counts active
+// and frozen (pending) columns.
 __global__ __launch_bounds__(kRedB) void k_red_beta(int kp, int nblk, const double* part0,
-                                                    const double* part1, double* S, int* nact) {
+                                                    const double* part1, double* S, int* cnt2) {
     __shared__ double red[kRedB];
-    __shared__ int cnt;
-    if (threadIdx.x == 0) cnt = 0;
+    __shared__ int nact, npend;
+    if (threadIdx.x == 0) { nact = 0; npend = 0; }
     const int col = blockIdx.x * 64 + threadIdx.x % 64;
     const double rzn = sum_parts(nblk, kp, part0, col, red);
     const double rrn = sum_parts(nblk, kp, part1, col, red);
     if (threadIdx.x < 64 && col < kp) {
-        double* active = S + S_ACTIVE * kp;
-        if (active[col] > 0.0) {
+        if (S[S_ACTIVE * kp + col] > 0.0) {
             const double rz = S[S_RZ * kp + col];
             S[S_BETA * kp + col] = rz > 0.0 ? rzn / rz : 0.0;
+            S[S_RZPREV * kp + col] = rz;
             S[S_RZ * kp + col] = rzn;
-            if (!(rrn > S[S_TOL2 * kp + col] * S[S_RR0 * kp + col])) active[col] = 0.0;
-            else atomicAdd(&cnt, 1);
-        } else {
-            S[S_BETA * kp + col] = 0.0;
+            if (!(rrn > S[S_TRIG * kp + col])) {
+                S[S_ACTIVE * kp + col] = 0.0;
+                atomicAdd(&npend, 1);
+            } else {
+                atomicAdd(&nact, 1);
+            }
+        } else if (S[S_SCALE * kp + col] > 0.0) {
+            atomicAdd(&npend, 1);
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) nact[blockIdx.x] = cnt;
+    if (threadIdx.x == 0) { cnt2[2 * blockIdx.x] = nact; cnt2[2 * blockIdx.x + 1] = npend; }
 }
 
-__global__ __launch_bounds__(kRedB) void k_red_init(int kp, int nblk, const double* part0,
-                                                    const double* part1, double* S) {
+// after residual replacement: rz from the new residual, beta against the
+// rescaled previous rz (the direction is kept), next trigger level
+__global__ __launch_bounds__(kRedB) void k_red_reset(int kp, int nblk, const double* part0,
+                                                     const double* part1, double* S, double delta2) {
     __shared__ double red[kRedB];
     const int col = blockIdx.x * 64 + threadIdx.x % 64;
-    const double rz = sum_parts(nblk, kp, part0, col, red);
-    const double rr = sum_parts(nblk, kp, part1, col, red);
+    const double rzn = sum_parts(nblk, kp, part0, col, red);
+    const double rrn = sum_parts(nblk, kp, part1, col, red);
     if (threadIdx.x < 64 && col < kp) {
-        S[S_RZ * kp + col] = rz;
-        S[S_RR0 * kp + col] = rr;
-        S[S_BETA * kp + col] = 0.0;
+        const bool act = S[S_SCALE * kp + col] > 0.0 && rrn > 0.0;
+        const double q = S[S_RATIO * kp + col];
+        const double prev = S[S_RZPREV * kp + col] * q * q;
+        S[S_ACTIVE * kp + col] = act ? 1.0 : 0.0;
+        S[S_BETA * kp + col] = (act && prev > 0.0) ? rzn / prev : 0.0;
+        S[S_RZ * kp + col] = act ? rzn : 0.0;
         S[S_ALPHA * kp + col] = 0.0;
-        S[S_ACTIVE * kp + col] = (S[S_SCALE * kp + col] > 0.0 && rr > 0.0) ? 1.0 : 0.0;
+        S[S_TRIG * kp + col] = fmax(delta2 * rrn, S[S_TGT * kp + col]);
     }
 }
 
@@ -383,6 +638,8 @@ inline int pow2_at_least(int v) {
 
 struct Plan {
     int n, kp, lpr, ntile, rpb, nblk, nred;
+    // 2D tensor-grid path
+    int grid, W, H, L, nsegx, gblk;
 };
 
 Plan make_plan(int n, int k, int num_cu) {
@@ -397,25 +654,50 @@ Plan make_plan(int n, int k, int num_cu) {
     p.nblk = (n + p.rpb - 1) / p.rpb;
     p.nblk = (p.nblk + 7) / 8 * 8;
     p.nred = (p.kp + 63) / 64;
+    p.grid = 0;
+    p.W = p.H = p.L = p.nsegx = p.gblk = 0;
     return p;
+}
+
+// segments of L nodes along x, one per lane group
+void plan_grid(Plan& p, int W, int H) {
+    p.grid = 1;
+    p.W = W;
+    p.H = H;
+    p.L = 32;
+    p.nsegx = (W + p.L - 1) / p.L;
+    const int groups = kB / p.lpr;
+    p.gblk = (p.nsegx * H + groups - 1) / groups;
+    p.gblk = (p.gblk + 7) / 8 * 8;
 }
 
 template <class T, int LPR>
 struct Kern {
-    static void spmv(const Plan& P, hipStream_t st, const int* rp, const int* ci, const T* va, const T* p,
-                     T* q, double* part) {
-        k_spmv_dot<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, rp, ci, va, p, q, part);
+    static void dir_spmv(const Plan& P, hipStream_t st, const int* rp, const int* ci, const T* va, const T* dinv,
+                         const double* beta, const double* active, const T* r, const T* pold, T* pnew,
+                         double* part) {
+        k_dir_spmv<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, rp, ci, va, dinv, beta, active, r,
+                                                                 pold, pnew, part);
     }
-    static void update(const Plan& P, hipStream_t st, const double* alpha, const T* dinv, T* x, T* r,
-                       const T* p, const T* q, double* p0, double* p1) {
-        k_update<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, alpha, dinv, x, r, p, q, p0, p1);
+    static void update(const Plan& P, hipStream_t st, const int* rp, const int* ci, const T* va, const T* dinv,
+                       const double* alpha, const T* p, T* x, T* r, double* p0, double* p1) {
+        k_cg_update<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, rp, ci, va, dinv, alpha, p, x, r,
+                                                                  p0, p1);
     }
-    static void pupdate(const Plan& P, hipStream_t st, const double* beta, const T* dinv, const T* r, T* p) {
-        k_pupdate<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, beta, dinv, r, p);
+    static void dir_spmv_g(const Plan& P, hipStream_t st, const T* coef, const T* dinv, const double* beta,
+                           const double* active, const T* r, const T* pold, T* pnew, double* part) {
+        k_dir_spmv_g2<T, LPR><<<dim3(P.gblk, P.ntile), kB, 0, st>>>(P.W, P.H, P.kp, P.L, P.nsegx, coef, dinv, beta,
+                                                                    active, r, pold, pnew, part);
     }
-    static void init(const Plan& P, hipStream_t st, const double* scale, const T* dinv, const double* r64,
-                     T* x, T* r, T* p, double* p0, double* p1) {
-        k_init<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, scale, dinv, r64, x, r, p, p0, p1);
+    static void update_g(const Plan& P, hipStream_t st, const T* coef, const T* dinv, const double* alpha, const T* p,
+                         T* x, T* r, double* p0, double* p1) {
+        k_cg_update_g2<T, LPR><<<dim3(P.gblk, P.ntile), kB, 0, st>>>(P.W, P.H, P.kp, P.L, P.nsegx, coef, dinv, alpha,
+                                                                     p, x, r, p0, p1);
+    }
+    static void rescale(const Plan& P, hipStream_t st, const double* scale, const double* ratio, const T* dinv,
+                        const double* r64, T* r, T* p, double* p0, double* p1) {
+        k_rescale<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, scale, ratio, dinv, r64, r, p, p0,
+                                                                p1);
     }
     static void resid(const Plan& P, hipStream_t st, const int* rp, const int* ci, const double* va,
                       const double* x, const double* b, double* r, double* part) {
@@ -423,85 +705,39 @@ struct Kern {
     }
 };
 
-template <class T>
-int run_solve(sfh_system* s, const Plan& P, int trans, int k, const double* B, int ldb, double* X,
-              int ldx, const sfh_solve_opts& o, sfh_solve_stats* stats);
-
+// Mixed-precision PCG with residual replacement ("reliable updates"):
+// FP32 recurrences; whenever a column's recursive residual fell by 10x (or
+// below its target) the accumulated FP32 correction is flushed into the FP64
+// iterate, the true residual b - A x is recomputed in FP64 and replaces the
+// FP32 one (rescaled to unit norm), and the search direction is kept.
 template <class T, int LPR>
-int inner_loop(sfh_system* s, const Plan& P, const sfh_solve_opts& o, const T* va, const T* dinv,
-               int& iters) {
+int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, int ldb, double* X, int ldx,
+              const sfh_solve_opts& o, sfh_solve_stats* stats) {
     hipStream_t st = s->ctx->stream;
     SolveWork& w = s->work;
-    double* S = w.scal.as<double>();
-    const int kp = P.kp;
-    T* x = w.x32.as<T>();
-    T* r = w.r32.as<T>();
-    T* p = w.p32.as<T>();
-    T* q = w.q32.as<T>();
-    double* p0 = w.part0.as<double>();
-    double* p1 = w.part1.as<double>();
-    const int* rp = s->rowptr.as<int>();
-    const int* ci = s->colind.as<int>();
-    Kern<T, LPR>::init(P, st, S + S_SCALE * kp, dinv, w.r64.as<double>(), x, r, p, p0, p1);
-    k_red_init<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, p0, p1, S);
-    SFH_LAUNCH_CHECK();
-    std::vector<int> nact(P.nred);
-    const int check = std::max(1, o.check_every);
-    for (int it = 0; it < o.max_inner; ++it) {
-        Kern<T, LPR>::spmv(P, st, rp, ci, va, p, q, p0);
-        k_red_alpha<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, p0, S);
-        Kern<T, LPR>::update(P, st, S + S_ALPHA * kp, dinv, x, r, p, q, p0, p1);
-        k_red_beta<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, p0, p1, S, w.nact.as<int>());
-        Kern<T, LPR>::pupdate(P, st, S + S_BETA * kp, dinv, r, p);
-        ++iters;
-        if ((it + 1) % check == 0 || it + 1 == o.max_inner) {
-            SFH_LAUNCH_CHECK();
-            SFH_CHECK_HIP(hipMemcpyAsync(nact.data(), w.nact.p, sizeof(int) * P.nred,
-                                         hipMemcpyDeviceToHost, st));
-            SFH_CHECK_HIP(hipStreamSynchronize(st));
-            int tot = 0;
-            for (int v : nact) tot += v;
-            if (tot == 0) break;
-        }
-    }
-    SFH_LAUNCH_CHECK();
-    return SFH_OK;
-}
-
-template <class T>
-int dispatch_inner(sfh_system* s, const Plan& P, const sfh_solve_opts& o, const T* va, const T* dinv,
-                   int& iters) {
-    switch (P.lpr) {
-        case 1: return inner_loop<T, 1>(s, P, o, va, dinv, iters);
-        case 2: return inner_loop<T, 2>(s, P, o, va, dinv, iters);
-        case 4: return inner_loop<T, 4>(s, P, o, va, dinv, iters);
-        case 8: return inner_loop<T, 8>(s, P, o, va, dinv, iters);
-        case 16: return inner_loop<T, 16>(s, P, o, va, dinv, iters);
-        case 32: return inner_loop<T, 32>(s, P, o, va, dinv, iters);
-        default: return inner_loop<T, 64>(s, P, o, va, dinv, iters);
-    }
-}
-
-void launch_resid(const Plan& P, hipStream_t st, const int* rp, const int* ci, const double* va,
-                  const double* x, const double* b, double* r, double* part) {
-    switch (P.lpr) {
-        case 1: Kern<double, 1>::resid(P, st, rp, ci, va, x, b, r, part); break;
-        case 2: Kern<double, 2>::resid(P, st, rp, ci, va, x, b, r, part); break;
-        case 4: Kern<double, 4>::resid(P, st, rp, ci, va, x, b, r, part); break;
-        case 8: Kern<double, 8>::resid(P, st, rp, ci, va, x, b, r, part); break;
-        case 16: Kern<double, 16>::resid(P, st, rp, ci, va, x, b, r, part); break;
-        case 32: Kern<double, 32>::resid(P, st, rp, ci, va, x, b, r, part); break;
-        default: Kern<double, 64>::resid(P, st, rp, ci, va, x, b, r, part); break;
-    }
-}
-
-template <class T>
-int run_solve(sfh_system* s, const Plan& P, int trans, int k, const double* B, int ldb, double* X,
-              int ldx, const sfh_solve_opts& o, sfh_solve_stats* stats) {
-    hipStream_t st = s->ctx->stream;
-    SolveWork& w = s->work;
+    Plan P = P0;
     const int n = P.n, kp = P.kp;
     const size_t nk = (size_t)n * kp;
+    // 2D tensor grid: stencil layout and sliding-window kernels
+    if (o.layout == 2 && s->gn[0] >= 3 && s->dim == 2 && s->dofpn == 1 && s->gn[1] >= 3 && s->dia_state >= 0) {
+        if (s->dia_state == 0) {
+            SFH_TRY(s->dia32.ensure((size_t)n * 9 * 4));
+            SFH_TRY(s->dia64.ensure((size_t)n * 9 * 8));
+            DevBuf err;
+            SFH_TRY(err.ensure(sizeof(int)));
+            SFH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+            k_build_dia9<<<(n + 255) / 256, 256, 0, st>>>(s->gn[0], s->gn[1], s->rowptr.as<int>(), s->colind.as<int>(),
+                                                         s->vals.as<double>(), s->dia32.as<float>(),
+                                                         s->dia64.as<double>(), err.as<int>());
+            SFH_LAUNCH_CHECK();
+            int he = 0;
+            SFH_CHECK_HIP(hipMemcpyAsync(&he, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            SFH_CHECK_HIP(hipStreamSynchronize(st));
+            s->dia_state = he ? -1 : 1;
+        }
+        if (s->dia_state == 1) plan_grid(P, s->gn[0], s->gn[1]);
+    }
+    const int nbi = P.grid ? P.gblk : P.nblk;   // partial rows written by the iteration kernels
     SFH_TRY(w.b64.ensure(nk * 8));
     SFH_TRY(w.x64.ensure(nk * 8));
     SFH_TRY(w.r64.ensure(nk * 8));
@@ -509,14 +745,23 @@ int run_solve(sfh_system* s, const Plan& P, int trans, int k, const double* B, i
     SFH_TRY(w.r32.ensure(nk * sizeof(T)));
     SFH_TRY(w.p32.ensure(nk * sizeof(T)));
     SFH_TRY(w.q32.ensure(nk * sizeof(T)));
-    SFH_TRY(w.part0.ensure((size_t)P.nblk * kp * 8));
-    SFH_TRY(w.part1.ensure((size_t)P.nblk * kp * 8));
+    SFH_TRY(w.part0.ensure((size_t)std::max(P.nblk, nbi) * kp * 8));
+    SFH_TRY(w.part1.ensure((size_t)std::max(P.nblk, nbi) * kp * 8));
     SFH_TRY(w.scal.ensure((size_t)S_NSLOT * kp * 8));
-    SFH_TRY(w.nact.ensure(sizeof(int) * P.nred));
+    SFH_TRY(w.nact.ensure(sizeof(int) * 2 * P.nred));
     double* S = w.scal.as<double>();
+    const T* coef = std::is_same<T, float>::value ? (const T*)s->dia32.p : (const T*)s->dia64.p;
     const int* rp = s->rowptr.as<int>();
     const int* ci = s->colind.as<int>();
     const double* va = s->vals.as<double>();
+    const T* vaT = std::is_same<T, float>::value ? (const T*)s->vals32.p : (const T*)s->vals.p;
+    const T* dinv = std::is_same<T, float>::value ? (const T*)s->dinv32.p : (const T*)s->dinv64.p;
+    T* x = w.x32.as<T>();
+    T* r = w.r32.as<T>();
+    T* pa = w.p32.as<T>();
+    T* pb = w.q32.as<T>();
+    double* p0 = w.part0.as<double>();
+    double* p1 = w.part1.as<double>();
 
     SFH_CHECK_HIP(hipEventRecord(s->ctx->ev0, st));
     if (s->dirty) {
@@ -530,43 +775,79 @@ int run_solve(sfh_system* s, const Plan& P, int trans, int k, const double* B, i
                                                           w.b64.as<double>());
     SFH_LAUNCH_CHECK();
     SFH_CHECK_HIP(hipMemsetAsync(w.x64.p, 0, nk * 8, st));
-    launch_resid(P, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(),
-                 w.part0.as<double>());
-    k_red_norm<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, w.part0.as<double>(), S + S_BN2 * kp);
+    SFH_CHECK_HIP(hipMemsetAsync(w.x32.p, 0, nk * sizeof(T), st));
+    SFH_CHECK_HIP(hipMemsetAsync(w.p32.p, 0, nk * sizeof(T), st));
+    SFH_CHECK_HIP(hipMemsetAsync(w.scal.p, 0, (size_t)S_NSLOT * kp * 8, st));
+    Kern<T, LPR>::resid(P, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
+    k_red_norm<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, p0, S + S_BN2 * kp);
     SFH_LAUNCH_CHECK();
-    std::vector<double> bn2(kp), rn2(kp), scale(kp), tol2(kp, o.inner_tol * o.inner_tol);
+    std::vector<double> bn2(kp), rn2(kp), scale(kp, 0.0), ratio(kp, 0.0), tgt(kp, 0.0);
     SFH_CHECK_HIP(hipMemcpyAsync(bn2.data(), S + S_BN2 * kp, kp * 8, hipMemcpyDeviceToHost, st));
-    SFH_CHECK_HIP(hipMemcpyAsync(S + S_TOL2 * kp, tol2.data(), kp * 8, hipMemcpyHostToDevice, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
     rn2 = bn2;
-    const T* vaT = std::is_same<T, float>::value ? (const T*)s->vals32.p : (const T*)s->vals.p;
-    const T* dinvT = std::is_same<T, float>::value ? (const T*)s->dinv32.p : (const T*)s->dinv64.p;
-    int outer = 0, iters = 0;
+    const double tol2 = o.tol * o.tol;
+    const double delta2 = o.inner_tol * o.inner_tol;
+    int updates = 0, iters = 0;
     double maxrel = 0.0;
     bool done = false;
+    std::vector<int> cnt(2 * P.nred);
+    const int check = std::max(1, o.check_every);
     for (;;) {
+        // residual replacement: decide per column, rescale, keep directions
         int nact = 0;
         maxrel = 0.0;
         for (int c = 0; c < kp; ++c) {
             const double rel = bn2[c] > 0.0 ? std::sqrt(rn2[c] / bn2[c]) : 0.0;
             maxrel = std::max(maxrel, rel);
-            const bool act = bn2[c] > 0.0 && rel > o.tol;
-            scale[c] = act ? 1.0 / std::sqrt(rn2[c]) : 0.0;
+            const bool act = bn2[c] > 0.0 && rn2[c] > tol2 * bn2[c];
+            const double sn = act ? 1.0 / std::sqrt(rn2[c]) : 0.0;
+            ratio[c] = (act && scale[c] > 0.0) ? sn / scale[c] : 0.0;
+            scale[c] = sn;
+            // stop the FP32 recurrence a little below the target, then verify in FP64
+            tgt[c] = act ? 0.09 * tol2 * bn2[c] * sn * sn : 0.0;
             nact += act;
         }
         if (nact == 0) { done = true; break; }
-        if (outer >= o.max_outer) break;
+        if (updates > o.max_outer || iters >= o.max_inner) break;
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_SCALE * kp, scale.data(), kp * 8, hipMemcpyHostToDevice, st));
-        SFH_TRY(dispatch_inner<T>(s, P, o, vaT, dinvT, iters));
-        k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, S + S_SCALE * kp,
-                                                                 w.x32.as<T>(), w.x64.as<double>());
-        launch_resid(P, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(),
-                     w.part0.as<double>());
-        k_red_norm<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, w.part0.as<double>(), S + S_RN2 * kp);
+        SFH_CHECK_HIP(hipMemcpyAsync(S + S_RATIO * kp, ratio.data(), kp * 8, hipMemcpyHostToDevice, st));
+        SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * kp, tgt.data(), kp * 8, hipMemcpyHostToDevice, st));
+        Kern<T, LPR>::rescale(P, st, S + S_SCALE * kp, S + S_RATIO * kp, dinv, w.r64.as<double>(), r, pa, p0, p1);
+        k_red_reset<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, p0, p1, S, delta2);
+        SFH_LAUNCH_CHECK();
+        ++updates;
+        // FP32 CG until some column is due for replacement
+        for (int it = 1;; ++it) {
+            if (P.grid) {
+                Kern<T, LPR>::dir_spmv_g(P, st, coef, dinv, S + S_BETA * kp, S + S_ACTIVE * kp, r, pa, pb, p0);
+                k_red_alpha<<<P.nred, kRedB, 0, st>>>(kp, nbi, p0, S);
+                Kern<T, LPR>::update_g(P, st, coef, dinv, S + S_ALPHA * kp, pb, x, r, p0, p1);
+            } else {
+                Kern<T, LPR>::dir_spmv(P, st, rp, ci, vaT, dinv, S + S_BETA * kp, S + S_ACTIVE * kp, r, pa, pb, p0);
+                k_red_alpha<<<P.nred, kRedB, 0, st>>>(kp, nbi, p0, S);
+                Kern<T, LPR>::update(P, st, rp, ci, vaT, dinv, S + S_ALPHA * kp, pb, x, r, p0, p1);
+            }
+            k_red_beta<<<P.nred, kRedB, 0, st>>>(kp, nbi, p0, p1, S, w.nact.as<int>());
+            std::swap(pa, pb);
+            ++iters;
+            if (iters >= o.max_inner) break;
+            if (it % check == 0) {
+                SFH_LAUNCH_CHECK();
+                SFH_CHECK_HIP(hipMemcpyAsync(cnt.data(), w.nact.p, sizeof(int) * 2 * P.nred, hipMemcpyDeviceToHost, st));
+                SFH_CHECK_HIP(hipStreamSynchronize(st));
+                int na = 0, nu = 0;
+                for (int b = 0; b < P.nred; ++b) { na += cnt[2 * b]; nu += cnt[2 * b + 1]; }
+                if (na == 0 || nu > 0) break;
+            }
+        }
+        // flush the FP32 correction, true residual in FP64
+        k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, S + S_SCALE * kp, x,
+                                                                 w.x64.as<double>());
+        Kern<T, LPR>::resid(P, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
+        k_red_norm<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, p0, S + S_RN2 * kp);
         SFH_LAUNCH_CHECK();
         SFH_CHECK_HIP(hipMemcpyAsync(rn2.data(), S + S_RN2 * kp, kp * 8, hipMemcpyDeviceToHost, st));
         SFH_CHECK_HIP(hipStreamSynchronize(st));
-        ++outer;
     }
     k_finalize<<<(unsigned)(((size_t)n * k + 255) / 256), 256, 0, st>>>(
         n, k, kp, trans, B, ldb, w.x64.as<double>(), rp, ci, va, s->diagpos.as<int>(),
@@ -577,41 +858,70 @@ int run_solve(sfh_system* s, const Plan& P, int trans, int k, const double* B, i
     float ms = 0.f;
     SFH_CHECK_HIP(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
     if (stats) {
-        stats->outer = outer;
+        stats->outer = updates;
         stats->inner_iters = iters;
         stats->max_relres = maxrel;
         stats->ms = ms;
         stats->converged = done ? 1 : 0;
     }
     if (!done) {
-        set_error("solve did not reach tol %.3g (max rel. residual %.3g after %d steps, %d inner iterations)",
-                  o.tol, maxrel, outer, iters);
+        set_error("solve did not reach tol %.3g (max rel. residual %.3g after %d iterations, %d replacements)",
+                  o.tol, maxrel, iters, updates);
         return SFH_ENOCONV;
     }
     return SFH_OK;
 }
 
+template <class T>
+int dispatch_solve(sfh_system* s, const Plan& P, int trans, int k, const double* B, int ldb, double* X, int ldx,
+                   const sfh_solve_opts& o, sfh_solve_stats* st) {
+    switch (P.lpr) {
+        case 1: return run_solve<T, 1>(s, P, trans, k, B, ldb, X, ldx, o, st);
+        case 2: return run_solve<T, 2>(s, P, trans, k, B, ldb, X, ldx, o, st);
+        case 4: return run_solve<T, 4>(s, P, trans, k, B, ldb, X, ldx, o, st);
+        case 8: return run_solve<T, 8>(s, P, trans, k, B, ldb, X, ldx, o, st);
+        case 16: return run_solve<T, 16>(s, P, trans, k, B, ldb, X, ldx, o, st);
+        case 32: return run_solve<T, 32>(s, P, trans, k, B, ldb, X, ldx, o, st);
+        default: return run_solve<T, 64>(s, P, trans, k, B, ldb, X, ldx, o, st);
+    }
+}
+
 template <class T, int LPR>
-int time_kernels(sfh_system* s, const Plan& P, int reps, double* ms) {
+int time_kernels(sfh_system* s, const Plan& P0, int reps, double* ms) {
     hipStream_t st = s->ctx->stream;
     SolveWork& w = s->work;
+    Plan P = P0;
+    if (s->dia_state == 1 && s->dim == 2 && s->gn[0] >= 3) plan_grid(P, s->gn[0], s->gn[1]);
+    const int nbi = P.grid ? P.gblk : P.nblk;
     const int kp = P.kp;
     double* S = w.scal.as<double>();
     hipEvent_t e[4];
     for (auto& x : e) SFH_CHECK_HIP(hipEventCreate(&x));
     const T* va = (const T*)s->vals32.p;
     const T* dinv = (const T*)s->dinv32.p;
+    const T* coef = (const T*)s->dia32.p;
+    const int* rp = s->rowptr.as<int>();
+    const int* ci = s->colind.as<int>();
     SFH_CHECK_HIP(hipEventRecord(e[0], st));
-    for (int r = 0; r < reps; ++r)
-        Kern<T, LPR>::spmv(P, st, s->rowptr.as<int>(), s->colind.as<int>(), va, w.p32.as<T>(), w.q32.as<T>(),
-                           w.part0.as<double>());
+    for (int q = 0; q < reps; ++q) {
+        if (P.grid)
+            Kern<T, LPR>::dir_spmv_g(P, st, coef, dinv, S + S_BETA * kp, S + S_ACTIVE * kp, w.r32.as<T>(),
+                                     w.p32.as<T>(), w.q32.as<T>(), w.part0.as<double>());
+        else
+            Kern<T, LPR>::dir_spmv(P, st, rp, ci, va, dinv, S + S_BETA * kp, S + S_ACTIVE * kp, w.r32.as<T>(),
+                                   w.p32.as<T>(), w.q32.as<T>(), w.part0.as<double>());
+    }
     SFH_CHECK_HIP(hipEventRecord(e[1], st));
-    for (int r = 0; r < reps; ++r)
-        Kern<T, LPR>::update(P, st, S + S_ALPHA * kp, dinv, w.x32.as<T>(), w.r32.as<T>(), w.p32.as<T>(),
-                             w.q32.as<T>(), w.part0.as<double>(), w.part1.as<double>());
+    for (int q = 0; q < reps; ++q) {
+        if (P.grid)
+            Kern<T, LPR>::update_g(P, st, coef, dinv, S + S_ALPHA * kp, w.q32.as<T>(), w.x32.as<T>(), w.r32.as<T>(),
+                                   w.part0.as<double>(), w.part1.as<double>());
+        else
+            Kern<T, LPR>::update(P, st, rp, ci, va, dinv, S + S_ALPHA * kp, w.q32.as<T>(), w.x32.as<T>(),
+                                 w.r32.as<T>(), w.part0.as<double>(), w.part1.as<double>());
+    }
     SFH_CHECK_HIP(hipEventRecord(e[2], st));
-    for (int r = 0; r < reps; ++r)
-        Kern<T, LPR>::pupdate(P, st, S + S_BETA * kp, dinv, w.r32.as<T>(), w.p32.as<T>());
+    for (int q = 0; q < reps; ++q) k_red_alpha<<<P.nred, kRedB, 0, st>>>(kp, nbi, w.part0.as<double>(), S);
     SFH_CHECK_HIP(hipEventRecord(e[3], st));
     SFH_LAUNCH_CHECK();
     SFH_CHECK_HIP(hipEventSynchronize(e[3]));
@@ -626,8 +936,9 @@ int time_kernels(sfh_system* s, const Plan& P, int reps, double* ms) {
 
 }  // namespace
 
-// Average device time per launch of the three inner-PCG kernels on the
-// workspace of the last k-RHS solve (whose buffers hold finite values).
+// Average device time per launch of the inner-PCG kernels (fused direction +
+// SpMV, fused SpMV + update, one reduction) on the workspace of the last k-RHS
+// FP32 solve, whose buffers hold finite values.
 int profile_pcg(sfh_system* s, int k, int reps, double* ms) {
     const Plan P = make_plan(s->ndof, k, s->ctx->num_cu);
     if (s->work.x32.bytes < (size_t)P.n * P.kp * sizeof(float) || s->work.part0.bytes < (size_t)P.nblk * P.kp * 8) {
@@ -653,25 +964,48 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
     sfh_solve_opts_default(&o);
     if (opts) o = *opts;
     if (o.tol <= 0) o.tol = 1e-10;
-    if (o.inner_tol <= 0) o.inner_tol = 1e-5;
-    if (o.max_outer <= 0) o.max_outer = 8;
-    if (o.max_inner <= 0) o.max_inner = 20000;
+    if (o.inner_tol <= 0 || o.inner_tol >= 1) o.inner_tol = 0.01;
+    if (o.max_outer <= 0) o.max_outer = 500;
+    if (o.max_inner <= 0) o.max_inner = 50000;
     if (o.check_every <= 0) o.check_every = 32;
-    const Plan P = make_plan(s->ndof, k, s->ctx->num_cu);
-    if (o.precision == 1) {
-        if (opts == nullptr || opts->inner_tol <= 0) o.inner_tol = o.tol * 0.5;
-        return run_solve<double>(s, P, trans, k, B, ldb, X, ldx, o, stats);
+    if (o.layout == 0) o.layout = 1;
+    // column tiles: each tile is an independent multi-RHS solve whose working
+    // set can stay in the last-level cache
+    if (o.rhs_tile > 0 && o.rhs_tile < k) {
+        sfh_solve_stats acc = {0, 0, 0.0, 0.0, 1};
+        for (int c0 = 0; c0 < k; c0 += o.rhs_tile) {
+            const int kc = std::min(o.rhs_tile, k - c0);
+            sfh_solve_stats st1;
+            sfh_solve_opts o1 = o;
+            o1.rhs_tile = 0;
+            const int rc = solve(s, trans, kc, B + c0, ldb, X + c0, ldx, &o1, &st1);
+            acc.outer += st1.outer;
+            acc.inner_iters += st1.inner_iters;
+            acc.max_relres = std::max(acc.max_relres, st1.max_relres);
+            acc.ms += st1.ms;
+            acc.converged = acc.converged && st1.converged;
+            if (rc != SFH_OK) {
+                if (stats) *stats = acc;
+                return rc;
+            }
+        }
+        if (stats) *stats = acc;
+        return SFH_OK;
     }
-    return run_solve<float>(s, P, trans, k, B, ldb, X, ldx, o, stats);
+    const Plan P = make_plan(s->ndof, k, s->ctx->num_cu);
+    if (o.precision == 1) return dispatch_solve<double>(s, P, trans, k, B, ldb, X, ldx, o, stats);
+    return dispatch_solve<float>(s, P, trans, k, B, ldb, X, ldx, o, stats);
 }
 
 }  // namespace sfh
 
 extern "C" void sfh_solve_opts_default(sfh_solve_opts* o) {
     o->tol = 1e-10;
-    o->inner_tol = 1e-5;
-    o->max_outer = 8;
-    o->max_inner = 20000;
+    o->inner_tol = 0.01;
+    o->max_outer = 500;
+    o->max_inner = 50000;
     o->check_every = 32;
     o->precision = 0;
+    o->rhs_tile = 0;
+    o->layout = 1;
 }

@@ -32,7 +32,8 @@ class SfhError(RuntimeError):
 
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("inner_tol", C.c_double), ("max_outer", C.c_int),
-                ("max_inner", C.c_int), ("check_every", C.c_int), ("precision", C.c_int)]
+                ("max_inner", C.c_int), ("check_every", C.c_int), ("precision", C.c_int),
+                ("rhs_tile", C.c_int), ("layout", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -315,7 +316,7 @@ class SystemSolveHip:
     def profile_pcg(self, k, reps=20):
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         _check(lib().sfh_profile_pcg(self.h, k, reps, C.byref(a), C.byref(b), C.byref(c)))
-        return {"spmv_dot": a.value, "update": b.value, "pupdate": c.value}
+        return {"dir_spmv": a.value, "cg_update": b.value, "reduce": c.value}
 
     def covforcing_apply(self, covvec, sigma, ell, W, mode=0):
         Wd = W if isinstance(W, DeviceArray) else self.ctx.to_device(np.asarray(W).reshape(self.ndof, -1))

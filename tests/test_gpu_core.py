@@ -116,11 +116,12 @@ def _solve_problem(orc, n=40, kind=M.QUAD4, seed=4):
     return m, bc, kap, nq, rp, ci, va
 
 
+@pytest.mark.parametrize("grid", [False, True], ids=["csr", "stencil"])
 @pytest.mark.parametrize("trans", [False, True])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_multi_rhs_solve_matches_direct(ctx, orc, trans, precision):
+def test_multi_rhs_solve_matches_direct(ctx, orc, trans, precision, grid):
     from statfem_amd import hip
-    m, bc, kap, nq, rp, ci, va = _solve_problem(orc)
+    m, bc, kap, nq, rp, ci, va = _solve_problem(orc, kind=M.TRI3 if grid and trans else M.QUAD4)
     rng = np.random.default_rng(5)
     k = 37   # ragged RHS count (not a multiple of 4)
     B = rng.normal(size=(m.nnodes, k))
@@ -128,9 +129,12 @@ def test_multi_rhs_solve_matches_direct(ctx, orc, trans, precision):
     B[:, 4] = 0.0
     B[bc[5], 4] = 1.0              # a load on a constrained DOF only
     s = hip.SystemSolveHip(ctx, m)
+    if grid:
+        s.set_grid()
     s.assemble(hip.POISSON, kap, nq1d=nq)
     s.applyConstraints(bc)
-    X, st = s.solve(B, trans=trans, opts=hip.default_opts(tol=1e-12, precision=precision))
+    X, st = s.solve(B, trans=trans, opts=hip.default_opts(tol=1e-12, precision=precision, layout=2 if grid else 1,
+                                                          rhs_tile=16 if precision == 0 else 0))
     Xr = orc.LU(rp, ci, va).solve(B, trans=trans)
     Xg = X.download()
     err = np.linalg.norm(Xg - Xr, axis=0) / np.maximum(np.linalg.norm(Xr, axis=0), 1e-300)
@@ -155,7 +159,8 @@ def test_solve_inhomogeneous_dirichlet(ctx, orc):
     assert np.abs(X.download() - ref).max() < 1e-9 * np.abs(ref).max()
 
 
-def test_solve_residual_at_scale(ctx):
+@pytest.mark.parametrize("grid", [False, True], ids=["csr", "stencil"])
+def test_solve_residual_at_scale(ctx, grid):
     """Size-independent property at a larger grid: FP64 residual of the returned
     solution, recomputed on the host, meets the tolerance."""
     from statfem_amd import hip
@@ -164,12 +169,14 @@ def test_solve_residual_at_scale(ctx):
     cfg = configs.c2_inputs(seed=1, n=n, ns=64)
     m = cfg["mesh"]
     s = hip.SystemSolveHip(ctx, m)
+    if grid:
+        s.set_grid()
     s.assemble(hip.POISSON, cfg["kappa"], nq1d=2)
     s.applyConstraints(cfg["bc"])
     P = np.zeros((m.nnodes, 64))
     for j in range(64):
         P[cfg["cols"][j], j] += cfg["vals"][j]
-    X, st = s.solve(P, trans=True, opts=hip.default_opts(tol=1e-10))
+    X, st = s.solve(P, trans=True, opts=hip.default_opts(tol=1e-10, layout=2 if grid else 1))
     rp, ci = s.pattern()
     va = s.values()
     Xh = X.download()
