@@ -217,6 +217,28 @@ int sfh_mcmc(sfh_lik* lik, int chains, int steps, const double* theta0, const do
              uint64_t seed, int prior_mode, const double* prior_mu, const double* prior_var,
              double* samples, double* acc);
 
+/* LikelihoodKOH::giveTypicalReadings + NormalSampler::giveSample
+ * (statX/infer/LikelihoodKOH.ipp:42-71, statX/infer/NormalSampler.hpp:46-72):
+ * Y (nr x ns host) = rho mean + L z, L L^T = K_s + rho^2 Cu. z ~ N(0,1) from
+ * Philox4x32-10, counter {reading, index, 3, 0}, key {seed} (Box-Muller, cosine). */
+int sfh_lik_typical_readings(sfh_lik* lik, double rho, double sigma_d, double ell_d, int nr,
+                             uint64_t seed, double* Y);
+
+/* Y = A X for k right-hand sides with the assembled (row-zero) operator;
+ * the sparse products of statX/inferQ/PrecisionMatrixRational.ipp:278-298. */
+int sfh_spmv(sfh_system* sys, int k, const double* X_dev, double* Y_dev);
+/* PrecisionMatrixRational::giveTau_ (statX/inferQ/PrecisionMatrixRational.ipp:258-275):
+ * tau^2 = Gamma(nu) / (sigma^2 Gamma(nu + d/2) (4 pi)^{d/2} kappa^{2 nu}). */
+int sfh_matern_tau(double sigma, double kappa, double nu, int dim, double* tau);
+/* givePrecisionMatrix for integer beta = (nu + d/2)/2 applied to X
+ * (statX/inferQ/PrecisionMatrixRational.ipp:24-39,177-209,278-298):
+ * Y = tau^2 kappa^{4 beta} (A M^-1)^{2 alpha - 1} A X, alpha = max(1, floor(beta)),
+ * A the system's assembled Matern matrix (M + kappa^-2 G: SFH_MATERN with
+ * factor 1/kappa^2, DriverMaxLikelihood.hpp:199-229), mass_dev the lumped mass.
+ * Fractional beta returns SFH_EARG (the reference needs the baryrat package). */
+int sfh_precision_apply(sfh_system* sys, const double* mass_dev, double sigma, double kappa,
+                        double nu, int k, const double* X_dev, double* Y_dev);
+
 /* GaussianProcess<KERNEL,N> fit / giveLogLikelihood / predictMean / predictVariance
  * (statX/infer/GaussianProcess.hpp:83-242) on the GPU. Host arrays. */
 int sfh_gp(sfh_ctx* ctx, int n, int dim, const double* X, const double* y, double sigma,

@@ -61,6 +61,11 @@ int or_mcmc_koh(int ns, int no, int dim, const double* Cu, const double* mean, c
                 const double* xs, double eps, const double* theta0, const double* propsd,
                 int nsteps, uint64_t seed, int chain, int prior_mode, const double* prior_mu,
                 const double* prior_var, double* samples, double* acc);
+int or_typical_readings(int ns, int nr, int dim, const double* Cu, const double* mean, const double* xs,
+                        double eps, double rho, double sd, double ld, uint64_t seed, double* Y);
+double or_matern_tau(double sigma, double kappa, double nu, int dim);
+int or_precision_apply(int n, const int* rp, const int* ci, const double* va, const double* mass,
+                       double sigma, double kappa, double nu, int dim, int k, const double* X, double* Y);
 int or_mcmc_gauss(double mu, double sd, double x0, double propsd, int nsteps, uint64_t seed,
                   int chain, double* samples, double* acc);
 

@@ -63,6 +63,9 @@ def lib():
         "or_mcmc_koh": (i, [i, i, i, _dp, _dp, _dp, _dp, d, _dp, _dp, i, C.c_uint64, i, i,
                             _dp, _dp, _dp, C.POINTER(C.c_double)]),
         "or_mcmc_gauss": (i, [d, d, d, d, i, C.c_uint64, i, _dp, C.POINTER(C.c_double)]),
+        "or_typical_readings": (i, [i, i, i, _dp, _dp, _dp, d, d, d, d, C.c_uint64, _dp]),
+        "or_matern_tau": (d, [d, d, d, i]),
+        "or_precision_apply": (i, [i, _ip, _ip, _dp, _dp, d, d, d, i, i, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -329,3 +332,27 @@ def mcmc_gauss(mu, sd, x0, propsd, nsteps, seed=0, chain=0):
     acc = C.c_double(0.0)
     lib().or_mcmc_gauss(mu, sd, x0, propsd, nsteps, seed, chain, samples, C.byref(acc))
     return samples, acc.value
+
+
+def typical_readings(Cu, mean, xs, eps, rho, sd, ld, nr, seed=0):
+    ns = len(mean)
+    xs = _c(xs).reshape(ns, -1)
+    Y = np.zeros((nr, ns))
+    rc = lib().or_typical_readings(ns, nr, xs.shape[1], _c(Cu), _c(mean), xs, eps, rho, sd, ld, seed, Y)
+    if rc != 0:
+        raise RuntimeError(f"or_typical_readings: {rc}")
+    return Y
+
+
+def matern_tau(sigma, kappa, nu, dim):
+    return lib().or_matern_tau(sigma, kappa, nu, dim)
+
+
+def precision_apply(rp, ci, va, mass, sigma, kappa, nu, dim, X):
+    X = _c(X)
+    n = len(rp) - 1
+    X2 = X.reshape(n, -1)
+    Y = np.zeros_like(X2)
+    lib().or_precision_apply(n, _c(rp, np.int32), _c(ci, np.int32), _c(va), _c(mass), sigma, kappa, nu, dim,
+                             X2.shape[1], _c(X2), Y)
+    return Y.reshape(X.shape)

@@ -71,6 +71,16 @@ void mcmc_draws(uint64_t seed, int chain, int step, double* z, double* u) {
     *u = to_unit(c[0], c[1]);
 }
 
+// one N(0,1) per (reading, index): counter {reading, index, 3, 0}, Box-Muller (cos)
+double reading_normal(uint64_t seed, int reading, int idx) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t ctr[4] = {(uint32_t)reading, (uint32_t)idx, 3u, 0u};
+    uint32_t a[4];
+    philox(ctr, key, a);
+    const double u1 = to_unit(a[0], a[1]), u2 = to_unit(a[2], a[3]);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
 double se(int dim, const double* x, const double* y, double sigma, double ell) {
     double sq = 0.0;
     for (int i = 0; i < dim; ++i) {
@@ -458,6 +468,54 @@ int or_mcmc_koh(int ns, int no, int dim, const double* Cu, const double* mean, c
         s[0] = x[0]; s[1] = x[1]; s[2] = x[2]; s[3] = y;
     }
     *acc = accepted / (double)nsteps;
+    return OR_OK;
+}
+
+// LikelihoodKOH::giveTypicalReadings + NormalSampler::giveSample
+// (statX/infer/LikelihoodKOH.ipp:42-71, statX/infer/NormalSampler.hpp:46-72):
+// y_r = rho mean + L z_r with L L^T = K_s + rho^2 Cu; Y is nr x ns.
+int or_typical_readings(int ns, int nr, int dim, const double* Cu, const double* mean, const double* xs,
+                        double eps, double rho, double sd, double ld, uint64_t seed, double* Y) {
+    std::vector<double> C((size_t)ns * ns);
+    or_mismatch_cov(ns, dim, xs, sd, ld, eps, C.data());
+    for (size_t i = 0; i < (size_t)ns * ns; ++i) C[i] += rho * rho * Cu[i];
+    if (!llt(ns, C)) return OR_ENOTSPD;
+    std::vector<double> z(ns);
+    for (int r = 0; r < nr; ++r) {
+        for (int j = 0; j < ns; ++j) z[j] = reading_normal(seed, r, j);
+        for (int i = 0; i < ns; ++i) {
+            double acc = 0.0;
+            for (int j = 0; j <= i; ++j) acc += C[(size_t)i * ns + j] * z[j];
+            Y[(size_t)r * ns + i] = rho * mean[i] + acc;
+        }
+    }
+    return OR_OK;
+}
+
+// PrecisionMatrixRational::giveTau_ (statX/inferQ/PrecisionMatrixRational.ipp:258-275)
+double or_matern_tau(double sigma, double kappa, double nu, int dim) {
+    const double t2 = std::tgamma(nu) / ((sigma * sigma) * std::tgamma(nu + dim / 2.) *
+                                          std::pow(4. * M_PI, dim / 2.) * std::pow(kappa, 2. * nu));
+    return std::sqrt(t2);
+}
+
+// givePrecisionMatrixInteger_ applied to X (PrecisionMatrixRational.ipp:177-209,278-298):
+// Q X = tau^2 kappa^{4 beta} (A M^-1)^{2 alpha - 1} A X, alpha = max(1, floor(beta)),
+// beta = (nu + d/2)/2; A is the (scaled) Matern matrix in CSR, M the lumped mass.
+int or_precision_apply(int n, const int* rp, const int* ci, const double* va, const double* mass,
+                       double sigma, double kappa, double nu, int dim, int k, const double* X, double* Y) {
+    const double beta = (nu + dim / 2.) / 2.;
+    const int alpha = (int)std::max(1., std::floor(beta));
+    std::vector<double> t((size_t)n * k), u((size_t)n * k);
+    or_spmv(n, rp, ci, va, k, X, t.data());
+    for (int rep = 0; rep < 2 * alpha - 1; ++rep) {
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < k; ++c) u[(size_t)i * k + c] = t[(size_t)i * k + c] / mass[i];
+        or_spmv(n, rp, ci, va, k, u.data(), t.data());
+    }
+    const double tau = or_matern_tau(sigma, kappa, nu, dim);
+    const double sc = tau * tau * std::pow(kappa, 4. * beta);
+    for (size_t i = 0; i < (size_t)n * k; ++i) Y[i] = sc * t[i];
     return OR_OK;
 }
 

@@ -621,6 +621,43 @@ int sfh_mcmc(sfh_lik* L, int chains, int steps, const double* theta0, const doub
     return mcmc_run(L, chains, steps, theta0, propsd, seed, prior_mode, prior_mu, prior_var, samples, acc);
 }
 
+int sfh_lik_typical_readings(sfh_lik* L, double rho, double sd, double ld, int nr, uint64_t seed, double* Y) {
+    if (!L || nr <= 0 || !Y) { set_error("bad argument"); return SFH_EARG; }
+    return typical_readings(L, rho, sd, ld, nr, seed, Y);
+}
+
+int sfh_spmv(sfh_system* s, int k, const double* X, double* Y) {
+    if (!s || k <= 0 || !X || !Y) { set_error("bad argument"); return SFH_EARG; }
+    SFH_TRY(spmv(s, k, X, Y));
+    SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    return SFH_OK;
+}
+
+int sfh_matern_tau(double sigma, double kappa, double nu, int dim, double* tau) {
+    if (!(sigma > 0.0) || !(kappa > 0.0) || !(nu > 0.0) || dim < 1 || dim > 3 || !tau) {
+        set_error("Matern hyperparameters must be positive");
+        return SFH_EARG;
+    }
+    const double t2 = std::tgamma(nu) / ((sigma * sigma) * std::tgamma(nu + dim / 2.) *
+                                          std::pow(4. * M_PI, dim / 2.) * std::pow(kappa, 2. * nu));
+    *tau = std::sqrt(t2);
+    return SFH_OK;
+}
+
+int sfh_precision_apply(sfh_system* s, const double* mass, double sigma, double kappa, double nu, int k,
+                        const double* X, double* Y) {
+    if (!s || !mass || k <= 0 || !X || !Y || s->dofpn != 1) { set_error("bad argument"); return SFH_EARG; }
+    double tau = 0.0;
+    SFH_TRY(sfh_matern_tau(sigma, kappa, nu, s->dim, &tau));
+    const double beta = (nu + s->dim / 2.) / 2.;
+    const double alpha = std::max(1., std::floor(beta));
+    if (std::fabs(beta - alpha) >= 1e-12) {
+        set_error("fractional SPDE exponent (beta = %g) needs the rational approximation", beta);
+        return SFH_EARG;
+    }
+    return precision_apply(s, mass, tau * tau * std::pow(kappa, 4. * beta), (int)alpha, k, X, Y);
+}
+
 int sfh_gp(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell, double eps, int np,
            const double* Xp, double* logl, double* mean, double* var) {
     if (!c || n <= 0 || dim <= 0 || !X || !y || !logl || (np > 0 && (!Xp || !mean || !var))) {

@@ -148,6 +148,9 @@ int dense_spd_solve(sfh_ctx* c, int n, double* A_dev, int m, double* B_dev, int*
 int mcmc_run(sfh_lik* L, int C, int steps, const double* theta0, const double* propsd, uint64_t seed,
              int mode, const double* pm, const double* pv, double* samples, double* acc);
 int profile_pcg(sfh_system* s, int k, int reps, double* ms);
+int typical_readings(sfh_lik* L, double rho, double sd, double ld, int nr, uint64_t seed, double* Y);
+int spmv(sfh_system* s, int k, const double* X, double* Y);
+int precision_apply(sfh_system* s, const double* mass, double tau2k4b, int alpha, int k, const double* X, double* Y);
 int gp_run(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell,
            double eps, int np, const double* Xp, double* logl, double* mean, double* var);
 }  // namespace sfh

@@ -433,6 +433,49 @@ __global__ void k_mc_accept(int C, int step, int steps, uint64_t seed, double* _
     }
 }
 
+// ---------------------------------------------------------- typical readings
+// LikelihoodKOH::giveTypicalReadings (statX/infer/LikelihoodKOH.ipp:42-71) with
+// NormalSampler::giveSample (statX/infer/NormalSampler.hpp:46-72):
+// y_r = rho mean + L z_r, L L^T = K_s + rho^2 Cu; z from Philox counter
+// {reading, index, 3, 0} (Box-Muller, cosine branch). One work-group.
+__device__ inline double reading_normal(uint64_t seed, int reading, int idx) {
+    uint32_t a[4];
+    philox10((uint32_t)reading, (uint32_t)idx, 3u, 0u, (uint32_t)seed, (uint32_t)(seed >> 32), a);
+    const double u1 = to_unit(a[0], a[1]), u2 = to_unit(a[2], a[3]);
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+__global__ __launch_bounds__(256) void k_typical(int ns, int nr, int dim, const double* __restrict__ xs,
+                                                 const double* __restrict__ Cu, const double* __restrict__ mean,
+                                                 double eps, double rho, double sd, double ld, uint64_t seed,
+                                                 double* __restrict__ a, double* __restrict__ Y, int* status) {
+    extern __shared__ double col[];
+    __shared__ int sflag;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    for (int idx = tid; idx < ns * ns; idx += bs) {
+        const int i = idx / ns, k = idx % ns;
+        double v = 0.0;
+        if (k <= i) {
+            v = se_kernel(dim, xs + (size_t)i * dim, xs + (size_t)k * dim, sd, ld);
+            if (i == k) v += eps * eps;
+            v += rho * rho * Cu[idx];
+        }
+        a[idx] = v;
+    }
+    __syncthreads();
+    if (!chol_block(ns, a, ns, col, &sflag)) {
+        if (tid == 0) *status = SFH_ENOTSPD;
+        return;
+    }
+    for (int q = tid; q < nr * ns; q += bs) {
+        const int r = q / ns, i = q % ns;
+        double acc = 0.0;
+        for (int j = 0; j <= i; ++j) acc += a[(size_t)i * ns + j] * reading_normal(seed, r, j);
+        Y[(size_t)r * ns + i] = rho * mean[i] + acc;
+    }
+    if (tid == 0) *status = SFH_OK;
+}
+
 // ------------------------------------------------------------------- GP
 __global__ __launch_bounds__(256) void k_gp_fit(int n, int dim, const double* __restrict__ X,
                                                 const double* __restrict__ y, double sigma, double ell,
@@ -649,6 +692,25 @@ int mcmc_run(sfh_lik* L, int C, int steps, const double* theta0, const double* p
     SFH_CHECK_HIP(hipStreamSynchronize(st));
     if (acc)
         for (int c = 0; c < C; ++c) acc[c] = na[c] / (double)steps;
+    return SFH_OK;
+}
+
+int typical_readings(sfh_lik* L, double rho, double sd, double ld, int nr, uint64_t seed, double* Y) {
+    hipStream_t st = L->ctx->stream;
+    const int ns = L->ns;
+    DevBuf a, y, dst;
+    SFH_TRY(a.ensure(sizeof(double) * ns * ns));
+    SFH_TRY(y.ensure(sizeof(double) * (size_t)nr * ns));
+    SFH_TRY(dst.ensure(sizeof(int)));
+    k_typical<<<1, 256, sizeof(double) * ns, st>>>(ns, nr, L->dim, L->xs.as<double>(), L->Cu.as<double>(),
+                                                  L->mean.as<double>(), L->eps, rho, sd, ld, seed, a.as<double>(),
+                                                  y.as<double>(), dst.as<int>());
+    SFH_LAUNCH_CHECK();
+    int h = 0;
+    SFH_CHECK_HIP(hipMemcpyAsync(&h, dst.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(Y, y.p, sizeof(double) * (size_t)nr * ns, hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (h != SFH_OK) { set_error("total covariance not SPD"); return h; }
     return SFH_OK;
 }
 

@@ -463,6 +463,25 @@ __global__ void k_constraints(int nc, const int* __restrict__ dofs, const double
     cmask[r] = 1;
 }
 
+// Y = A X for k right-hand sides (FP64, row-zero operator as assembled)
+__global__ void k_spmv64(int n, int k, const int* __restrict__ rp, const int* __restrict__ ci,
+                         const double* __restrict__ va, const double* __restrict__ X, double* __restrict__ Y) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * k) return;
+    const int i = (int)(t / k), c = (int)(t % k);
+    double s = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) s += va[e] * X[(size_t)ci[e] * k + c];
+    Y[t] = s;
+}
+
+// Y = s X ./ m (row scaling)
+__global__ void k_rowdiv(long long total, int k, double s, const double* __restrict__ m,
+                         const double* __restrict__ X, double* __restrict__ Y) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    Y[t] = m ? s * X[t] / m[t / k] : s * X[t];
+}
+
 inline int grid_for(long long n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -691,6 +710,35 @@ int apply_constraints(sfh_system* s, int nc, const int* dofs, const double* valu
     SFH_CHECK_HIP(hipStreamSynchronize(st));
     s->dirty = true;
     if (s->dia_state > 0) s->dia_state = 0;
+    return SFH_OK;
+}
+
+int spmv(sfh_system* s, int k, const double* X, double* Y) {
+    hipStream_t st = s->ctx->stream;
+    k_spmv64<<<grid_for((long long)s->ndof * k, 256), 256, 0, st>>>(s->ndof, k, s->rowptr.as<int>(), s->colind.as<int>(),
+                                                                    s->vals.as<double>(), X, Y);
+    SFH_LAUNCH_CHECK();
+    return SFH_OK;
+}
+
+// Q X = tau^2 kappa^{4 beta} (A M^-1)^{2 alpha - 1} A X  (integer beta,
+// statX/inferQ/PrecisionMatrixRational.ipp:177-209,278-298)
+int precision_apply(sfh_system* s, const double* mass, double tau2k4b, int alpha, int k, const double* X, double* Y) {
+    hipStream_t st = s->ctx->stream;
+    const long long total = (long long)s->ndof * k;
+    DevBuf t;
+    SFH_TRY(t.ensure(sizeof(double) * total));
+    double* a = Y;
+    double* b = t.as<double>();
+    SFH_TRY(spmv(s, k, X, a));
+    for (int rep = 0; rep < 2 * alpha - 1; ++rep) {
+        k_rowdiv<<<grid_for(total, 256), 256, 0, st>>>(total, k, 1.0, mass, a, b);
+        SFH_LAUNCH_CHECK();
+        SFH_TRY(spmv(s, k, b, a));
+    }
+    k_rowdiv<<<grid_for(total, 256), 256, 0, st>>>(total, 1, tau2k4b, nullptr, a, a);
+    SFH_LAUNCH_CHECK();
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
     return SFH_OK;
 }
 

@@ -90,6 +90,10 @@ _SIGS = {
     "sfh_lik_post_proj": (i32, [v, dbl, dbl, dbl, v, v]),
     "sfh_lik_post_field": (i32, [v, dbl, dbl, dbl, i32, v, v, v]),
     "sfh_mcmc": (i32, [v, i32, i32, v, v, C.c_uint64, i32, v, v, v, v]),
+    "sfh_lik_typical_readings": (i32, [v, dbl, dbl, dbl, i32, C.c_uint64, v]),
+    "sfh_spmv": (i32, [v, i32, v, v]),
+    "sfh_matern_tau": (i32, [dbl, dbl, dbl, i32, C.POINTER(dbl)]),
+    "sfh_precision_apply": (i32, [v, v, dbl, dbl, dbl, i32, v, v]),
     "sfh_gp": (i32, [v, i32, i32, v, v, dbl, dbl, dbl, i32, v, v, v, v]),
 }
 
@@ -318,6 +322,22 @@ class SystemSolveHip:
         _check(lib().sfh_profile_pcg(self.h, k, reps, C.byref(a), C.byref(b), C.byref(c)))
         return {"dir_spmv": a.value, "cg_update": b.value, "reduce": c.value}
 
+    def spmv(self, X):
+        Xd = X if isinstance(X, DeviceArray) else self.ctx.to_device(np.asarray(X).reshape(self.ndof, -1))
+        k = int(np.prod(Xd.shape) // self.ndof)
+        Y = self.ctx.empty((self.ndof, k))
+        _check(lib().sfh_spmv(self.h, k, Xd.ptr, Y.ptr))
+        return Y
+
+    def precision_apply(self, mass, sigma, kappa, nu, X):
+        """PrecisionMatrixRational::givePrecisionMatrix (integer beta) applied to X."""
+        Xd = X if isinstance(X, DeviceArray) else self.ctx.to_device(np.asarray(X).reshape(self.ndof, -1))
+        k = int(np.prod(Xd.shape) // self.ndof)
+        md = mass if isinstance(mass, DeviceArray) else self.ctx.to_device(mass)
+        Y = self.ctx.empty((self.ndof, k))
+        _check(lib().sfh_precision_apply(self.h, md.ptr, sigma, kappa, nu, k, Xd.ptr, Y.ptr))
+        return Y
+
     def covforcing_apply(self, covvec, sigma, ell, W, mode=0):
         Wd = W if isinstance(W, DeviceArray) else self.ctx.to_device(np.asarray(W).reshape(self.ndof, -1))
         k = int(np.prod(Wd.shape) // self.ndof)
@@ -416,6 +436,11 @@ class LikelihoodKOHHip:
         _check(lib().sfh_lik_post_field(self.h, rho, sd, ld, N, field.ptr, ubar.ptr, out.ptr))
         return out
 
+    def giveTypicalReadings(self, rho, sd, ld, nr, seed=0):
+        Y = np.zeros((nr, self.ns))
+        _check(lib().sfh_lik_typical_readings(self.h, rho, sd, ld, nr, seed, _p(Y)))
+        return Y
+
     def mcmc(self, theta0, propsd, steps, seed=0, prior_mode=0, prior_mu=(0, 0, 0), prior_var=(1, 1, 1),
              keep_samples=True):
         th0 = _h(np.atleast_2d(theta0))
@@ -438,3 +463,9 @@ def gaussian_process(ctx, X, y, sigma, ell, eps, Xp):
     _check(lib().sfh_gp(ctx.h, len(y), dim, _p(X), _p(y), sigma, ell, eps, Xp.shape[0], _p(Xp), C.byref(logl),
                         _p(mean), _p(var)))
     return logl.value, mean, var
+
+
+def matern_tau(sigma, kappa, nu, dim):
+    t = C.c_double()
+    _check(lib().sfh_matern_tau(sigma, kappa, nu, dim, C.byref(t)))
+    return t.value

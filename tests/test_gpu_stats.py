@@ -180,3 +180,39 @@ def test_mcmc_chains_match_oracle(ctx, orc):
             assert acc[ch] == racc
             assert np.abs(samples[ch, :, :3] - ref[:, :3]).max() < 1e-12
             assert np.abs(samples[ch, :, 3] - ref[:, 3]).max() < 1e-8 * np.abs(ref[:, 3]).max()
+
+
+def test_typical_readings_match_oracle(ctx, orc):
+    from statfem_amd import hip
+    rng = np.random.default_rng(10)
+    ns = 40
+    xs = rng.uniform(size=(ns, 2))
+    A = rng.normal(size=(ns, ns)) * 0.05
+    Cu = A @ A.T + 1e-6 * np.eye(ns)
+    mean = rng.normal(size=ns)
+    lik = hip.LikelihoodKOHHip(ctx, xs, Cu, mean, np.zeros((1, ns)), 0.01)
+    Yg = lik.giveTypicalReadings(1.2, 0.05, 0.2, 17, seed=123)
+    Yo = orc.typical_readings(Cu, mean, xs, 0.01, 1.2, 0.05, 0.2, 17, seed=123)
+    assert np.abs(Yg - Yo).max() < 1e-12 * np.abs(Yo).max()
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_precision_operator_matches_oracle(ctx, orc, dim):
+    from statfem_amd import hip
+    m = {1: M.line1d(60), 2: M.grid2d(14, 11), 3: M.grid3d(5, 4, 6)}[dim]
+    nu = {1: 1.5, 2: 1.0, 3: 0.5}[dim]
+    kap = np.sqrt(2 * nu) / 0.2
+    s = hip.SystemSolveHip(ctx, m)
+    s.assemble(hip.MATERN, kap, 1.0, nq1d=2, factor=1.0 / kap ** 2)
+    mass = s.lumped_mass(2)
+    rng = np.random.default_rng(11)
+    X = rng.normal(size=(m.nnodes, 5))
+    Yg = s.precision_apply(mass, 0.8, kap, nu, X).download()
+    rp, ci, va, _ = orc.assemble(m, orc.MATERN, kap, 1.0, nq1d=2, factor=1.0 / kap ** 2)
+    Yo = orc.precision_apply(rp, ci, va, orc.body_force(m, nq1d=2, fconst=1.0), 0.8, kap, nu, dim, X)
+    assert np.abs(Yg - Yo).max() <= 1e-12 * np.abs(Yo).max()
+    assert abs(hip.matern_tau(0.8, kap, nu, dim) - orc.matern_tau(0.8, kap, nu, dim)) < 1e-15
+    Ys = s.spmv(X).download()
+    assert np.abs(Ys - orc.spmv(rp, ci, va, X)).max() <= 1e-13 * np.abs(Ys).max()
+    with pytest.raises(hip.SfhError):
+        s.precision_apply(mass, 0.8, kap, nu + 0.3, X)   # fractional beta

@@ -297,3 +297,50 @@ def test_mcmc_koh_detailed_balance_on_known_target(orc):
     w = np.exp(logp - logp.max()); w /= w.sum()
     assert abs(th.mean() - (w * grid).sum()) < 0.01
     assert 0.2 < acc < 0.9
+
+
+# ------------------------------------------------- SPDE precision, readings
+def test_matern_tau_closed_forms(orc):
+    # d = 1, nu = 1/2 (exponential kernel): tau^2 = 1 / (2 kappa)
+    for kap in (0.5, 3.0, 20.0):
+        assert abs(orc.matern_tau(1.0, kap, 0.5, 1) ** 2 - 1.0 / (2 * kap)) < 1e-14 / kap
+        # d = 3, nu = 1/2: tau^2 = 1 / (8 pi kappa)
+        assert abs(orc.matern_tau(1.0, kap, 0.5, 3) ** 2 - 1.0 / (8 * np.pi * kap)) < 1e-15 / kap
+    # sigma scaling
+    assert abs(orc.matern_tau(2.0, 3.0, 1.0, 2) - orc.matern_tau(1.0, 3.0, 1.0, 2) / 2.0) < 1e-15
+
+
+def _matern_1d(orc, n=400, ell=0.05, sigma=0.7, nu=1.5):
+    m = M.line1d(n)
+    kap = np.sqrt(2 * nu) / ell
+    rp, ci, va, _ = orc.assemble(m, orc.MATERN, kap, 1.0, nq1d=2, factor=1.0 / kap ** 2)
+    mass = orc.body_force(m, nq1d=2, fconst=1.0)
+    return m, kap, rp, ci, va, mass
+
+
+def test_precision_marginal_variance_is_sigma2(orc):
+    """The integer-beta SPDE precision Q = tau^2 kappa^4 A M^-1 A of a Matern
+    field (nu = 3/2 in 1D) has marginal variance diag(Q^-1) ~ sigma^2 away from
+    the boundary; a wrong tau, kappa power or mass placement breaks this."""
+    sigma, nu = 0.7, 1.5
+    m, kap, rp, ci, va, mass = _matern_1d(orc, sigma=sigma, nu=nu)
+    n = m.nnodes
+    Q = orc.precision_apply(rp, ci, va, mass, sigma, kap, nu, 1, np.eye(n))
+    assert np.abs(Q - Q.T).max() < 1e-9 * np.abs(Q).max()
+    var = np.diag(np.linalg.inv(Q))
+    mid = var[n // 2 - 40:n // 2 + 40]
+    assert np.all(np.abs(mid - sigma ** 2) < 0.03 * sigma ** 2), (mid.min(), mid.max())
+
+
+def test_typical_readings_statistics(orc):
+    rng = np.random.default_rng(9)
+    ns = 5
+    xs = rng.uniform(size=(ns, 2))
+    A = rng.normal(size=(ns, ns)) * 0.1
+    Cu = A @ A.T
+    mean = rng.normal(size=ns)
+    rho, sd, ld, eps = 1.3, 0.2, 0.3, 0.05
+    Y = orc.typical_readings(Cu, mean, xs, eps, rho, sd, ld, 40000, seed=4)
+    C = orc.mismatch_cov(xs, sd, ld, eps) + rho ** 2 * Cu
+    assert np.abs(Y.mean(0) - rho * mean).max() < 4 * np.sqrt(np.diag(C).max() / 40000)
+    assert np.abs(np.cov(Y.T) - C).max() < 0.03 * np.abs(C).max()
