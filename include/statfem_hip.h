@@ -48,6 +48,7 @@ enum {
 typedef struct sfh_ctx sfh_ctx;
 typedef struct sfh_system sfh_system;
 typedef struct sfh_lik sfh_lik;
+typedef struct sfh_mlp sfh_mlp;
 
 const char* sfh_status_string(int status);
 const char* sfh_last_error(void);
@@ -238,6 +239,33 @@ int sfh_matern_tau(double sigma, double kappa, double nu, int dim, double* tau);
  * Fractional beta returns SFH_EARG (the reference needs the baryrat package). */
 int sfh_precision_apply(sfh_system* sys, const double* mass_dev, double sigma, double kappa,
                         double nu, int k, const double* X_dev, double* Y_dev);
+
+/* ---------------------------------------------------- learned surrogate
+ * No reference counterpart. An MLP 3 -> 64 -> 64 -> 1 (tanh) of the
+ * log-likelihood over theta = log(rho, sigma_d, ell_d); BF16 weights, FP32
+ * accumulation, FP32 Adam masters; hand-written forward/backward kernels.
+ * Parameter layout (4481): W1[64][3], b1[64], W2[64][64], b2[64], W3[64], b3.
+ * norm8 = (mu_theta[3], sd_theta[3], mu_logL, sd_logL). */
+int sfh_mlp_create(sfh_ctx* ctx, uint64_t seed, sfh_mlp** out);
+int sfh_mlp_destroy(sfh_mlp* mlp);
+int sfh_mlp_set_normalization(sfh_mlp* mlp, const double* norm8);
+int sfh_mlp_params(sfh_mlp* mlp, float* master, uint16_t* bf16, double* norm8);
+int sfh_mlp_set_params(sfh_mlp* mlp, const float* master);
+/* gradient of (1/n) sum (net(x_s) - t_s)^2 in standardised units; grad: 4481 */
+int sfh_mlp_grad(sfh_mlp* mlp, int n, const double* theta, const double* logl, float* grad, double* loss);
+/* full-batch Adam with cosine-decayed learning rate; standardisation from the data */
+int sfh_mlp_train(sfh_mlp* mlp, int n, const double* theta, const double* logl, int steps, double lr,
+                  double* loss);
+int sfh_mlp_predict(sfh_mlp* mlp, int n, const double* theta, double* logl);
+/* Delayed-acceptance (surrogate-transition) MCMC: per outer step every chain
+ * runs `sub` random-walk steps on the surrogate target (NN + prior terms of
+ * sfh_mcmc), then the end point is accepted with the exact likelihood
+ * (sfh_lik_loglik_batch) by min(1, pi(y) pi~(x) / (pi(x) pi~(y))), which keeps
+ * the exact posterior invariant. Arguments as sfh_mcmc; sub_acc: surrogate
+ * acceptance rate per chain. */
+int sfh_mcmc_da(sfh_lik* lik, sfh_mlp* mlp, int chains, int steps, int sub, const double* theta0,
+                const double* propsd, uint64_t seed, int prior_mode, const double* prior_mu,
+                const double* prior_var, double* samples, double* acc, double* sub_acc);
 
 /* GaussianProcess<KERNEL,N> fit / giveLogLikelihood / predictMean / predictVariance
  * (statX/infer/GaussianProcess.hpp:83-242) on the GPU. Host arrays. */

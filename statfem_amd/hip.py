@@ -94,6 +94,15 @@ _SIGS = {
     "sfh_spmv": (i32, [v, i32, v, v]),
     "sfh_matern_tau": (i32, [dbl, dbl, dbl, i32, C.POINTER(dbl)]),
     "sfh_precision_apply": (i32, [v, v, dbl, dbl, dbl, i32, v, v]),
+    "sfh_mlp_create": (i32, [v, C.c_uint64, C.POINTER(v)]),
+    "sfh_mlp_destroy": (i32, [v]),
+    "sfh_mlp_set_normalization": (i32, [v, v]),
+    "sfh_mlp_params": (i32, [v, v, v, v]),
+    "sfh_mlp_set_params": (i32, [v, v]),
+    "sfh_mlp_grad": (i32, [v, i32, v, v, v, C.POINTER(dbl)]),
+    "sfh_mlp_train": (i32, [v, i32, v, v, i32, dbl, C.POINTER(dbl)]),
+    "sfh_mlp_predict": (i32, [v, i32, v, v]),
+    "sfh_mcmc_da": (i32, [v, v, i32, i32, i32, v, v, C.c_uint64, i32, v, v, v, v, v]),
     "sfh_gp": (i32, [v, i32, i32, v, v, dbl, dbl, dbl, i32, v, v, v, v]),
 }
 
@@ -469,3 +478,68 @@ def matern_tau(sigma, kappa, nu, dim):
     t = C.c_double()
     _check(lib().sfh_matern_tau(sigma, kappa, nu, dim, C.byref(t)))
     return t.value
+
+
+class SurrogateMLP:
+    """MLP 3 -> 64 -> 64 -> 1 (tanh) surrogate of the KOH log-likelihood;
+    BF16 weights, FP32 accumulation, trained on the GPU."""
+    NP = 4481
+
+    def __init__(self, ctx, seed=0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().sfh_mlp_create(ctx.h, seed, C.byref(h)))
+        self.h = h
+        ctx.adopt(self)
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().sfh_mlp_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def params(self):
+        m = np.zeros(self.NP, np.float32)
+        b = np.zeros(self.NP, np.uint16)
+        nrm = np.zeros(8)
+        _check(lib().sfh_mlp_params(self.h, _p(m), _p(b), _p(nrm)))
+        return m, b, nrm
+
+    def set_params(self, master):
+        _check(lib().sfh_mlp_set_params(self.h, _p(_h(master, np.float32))))
+
+    def set_normalization(self, norm8):
+        _check(lib().sfh_mlp_set_normalization(self.h, _p(_h(norm8))))
+
+    def grad(self, theta, logl):
+        th = _h(np.atleast_2d(theta))
+        g = np.zeros(self.NP, np.float32)
+        loss = C.c_double()
+        _check(lib().sfh_mlp_grad(self.h, th.shape[0], _p(th), _p(_h(logl)), _p(g), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, theta, logl, steps=2000, lr=3e-3):
+        th = _h(np.atleast_2d(theta))
+        loss = C.c_double()
+        _check(lib().sfh_mlp_train(self.h, th.shape[0], _p(th), _p(_h(logl)), steps, lr, C.byref(loss)))
+        return loss.value
+
+    def predict(self, theta):
+        th = _h(np.atleast_2d(theta))
+        out = np.zeros(th.shape[0])
+        _check(lib().sfh_mlp_predict(self.h, th.shape[0], _p(th), _p(out)))
+        return out
+
+
+def mcmc_da(lik, mlp, theta0, propsd, steps, sub, seed=0, prior_mode=0, prior_mu=(0, 0, 0), prior_var=(1, 1, 1),
+            keep_samples=True):
+    th0 = _h(np.atleast_2d(theta0))
+    chains = th0.shape[0]
+    samples = np.zeros((chains, steps, 4)) if keep_samples else None
+    acc = np.zeros(chains)
+    sacc = np.zeros(chains)
+    _check(lib().sfh_mcmc_da(lik.h, mlp.h, chains, steps, sub, _p(th0), _p(_h(propsd)), seed, prior_mode,
+                             _p(_h(prior_mu)), _p(_h(prior_var)), _p(samples), _p(acc), _p(sacc)))
+    return samples, acc, sacc

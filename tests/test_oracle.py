@@ -344,3 +344,38 @@ def test_typical_readings_statistics(orc):
     C = orc.mismatch_cov(xs, sd, ld, eps) + rho ** 2 * Cu
     assert np.abs(Y.mean(0) - rho * mean).max() < 4 * np.sqrt(np.diag(C).max() / 40000)
     assert np.abs(np.cov(Y.T) - C).max() < 0.03 * np.abs(C).max()
+
+
+def test_mlp_oracle_gradient_by_finite_differences():
+    from oracle import mlp
+    rng = np.random.default_rng(12)
+    p = rng.normal(size=mlp.NP) * 0.2
+    X = rng.normal(size=(7, 3))
+    t = rng.normal(size=7)
+    _, g = mlp.loss_grad(p, X, t)
+    for idx in list(range(0, 5)) + [200, 300, 4000, 4400, 4480]:
+        h = 1e-6
+        pp, pm = p.copy(), p.copy()
+        pp[idx] += h
+        pm[idx] -= h
+        fd = (mlp.loss_grad(pp, X, t)[0] - mlp.loss_grad(pm, X, t)[0]) / (2 * h)
+        assert abs(fd - g[idx]) < 1e-7 + 1e-6 * abs(g[idx])
+
+
+def test_ess_of_ar1_chains_matches_closed_form():
+    """AR(1) with coefficient a has integrated autocorrelation time
+    (1 + a) / (1 - a); independent draws have ESS = number of draws."""
+    from statfem_amd import diagnostics as D
+    rng = np.random.default_rng(21)
+    m, n, a = 8, 20000, 0.8
+    x = np.zeros((m, n))
+    e = rng.normal(size=(m, n))
+    for t in range(1, n):
+        x[:, t] = a * x[:, t - 1] + e[:, t]
+    expect = m * n * (1 - a) / (1 + a)
+    assert abs(D.ess(x) / expect - 1) < 0.1
+    iid = rng.normal(size=(m, n))
+    assert abs(D.ess(iid) / (m * n) - 1) < 0.1
+    assert D.split_rhat(iid) < 1.01
+    shifted = iid + np.arange(m)[:, None]
+    assert D.split_rhat(shifted) > 2
