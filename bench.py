@@ -103,6 +103,62 @@ def gpu_run(args):
     return res, per[dom]
 
 
+def c5_run(args):
+    """Hyperparameter MCMC with the learned surrogate (256 x 256 Q1 Poisson,
+    200 sensors, 8 snapshots, 64 chains). The statFEM prior at the sensors
+    comes from the GPU forward model; a tuned pilot sets the random-walk step
+    sizes shared by both samplers; the plain chain runs the reference's
+    random-walk Metropolis on the exact likelihood; the surrogate run pays for
+    its training design, its training and its sampling. ESS/s is the minimum
+    over the three hyperparameters."""
+    from statfem_amd import configs, hip, inference as I
+    cfg = configs.c5_inputs(seed=0, n=args.c5_n)
+    m = cfg["mesh"]
+    ctx = hip.Context(0)
+    s = hip.SystemSolveHip(ctx, m)
+    s.set_grid()
+    s.assemble(hip.POISSON, 1.0, nq1d=2)
+    s.assemble_rhs(nq1d=2, fconst=1.0)
+    s.applyConstraints(cfg["bc"])
+    fg = hip.ForwardGaussRHSHip(s, cfg["cols"], cfg["vals"], reg_eps=cfg["reg"])
+    t0 = time.perf_counter()
+    mean, Cu, _ = fg.updateSolution(cfg["sigma_f"], cfg["ell_f"], opts=hip.default_opts(tol=1e-10))
+    t_prior = time.perf_counter() - t0
+    ns, no = len(cfg["xs"]), cfg["no"]
+    Y = hip.LikelihoodKOHHip(ctx, cfg["xs"], Cu, mean, np.zeros((no, ns)), cfg["sigma_e"]).giveTypicalReadings(
+        *cfg["truth"], no, seed=cfg["data_seed"])
+    lik = hip.LikelihoodKOHHip(ctx, cfg["xs"], Cu, mean, Y, cfg["sigma_e"])
+    prior = dict(prior_mode=2, prior_mu=cfg["prior_mu"], prior_var=cfg["prior_var"])
+    rng = np.random.default_rng(3)
+    chains = args.c5_chains
+    th0 = cfg["prior_mu"] + 0.1 * rng.standard_normal((chains, 3))
+    prop, start, center, scale, t_pilot = I.tuned_proposal(lik, th0, steps=2000, **prior)
+    log(f"C5 prior {t_prior:.2f}s, pilot {t_pilot:.2f}s, proposal sd {prop}")
+    t0 = time.perf_counter()
+    plain, acc = lik.mcmc(start, prop, args.c5_steps, seed=1, **prior)
+    t_plain = time.perf_counter() - t0
+    ess_p, rate_p = I.ess_per_second(plain, t_plain)
+    log(f"C5 plain: {t_plain:.2f}s, acceptance {acc.mean():.3f}, ESS {ess_p.round(0)}")
+    da = I.surrogate_posterior(lik, start, prop, center, scale, args.c5_da_steps, args.c5_sub, seed=5, **prior)
+    t_da = sum(da["times"].values())
+    ess_d, rate_d = I.ess_per_second(da["samples"], t_da)
+    log(f"C5 surrogate: {da['times']}, acceptance {da['acc'].mean():.3f}, ESS {ess_d.round(0)}")
+    burn_p, burn_d = int(0.1 * plain.shape[1]), int(0.1 * da["samples"].shape[1])
+    em, es = I.moment_errors(da["samples"][:, burn_d:], plain[:, burn_p:])
+    return {
+        "workload": f"Poisson {args.c5_n}x{args.c5_n} Q1, {ns} sensors, {no} snapshots, {chains} chains",
+        "plain": {"steps": args.c5_steps, "s": t_plain, "acceptance": float(acc.mean()),
+                  "ess": ess_p.tolist(), "ess_per_s": rate_p},
+        "surrogate": {"steps": args.c5_da_steps, "sub_steps": args.c5_sub, "s": t_da,
+                      "phases_s": da["times"], "acceptance": float(da["acc"].mean()),
+                      "sub_acceptance": float(da["sub_acc"].mean()), "train_points": da["n_train"],
+                      "train_loss": da["train_loss"], "ess": ess_d.tolist(), "ess_per_s": rate_d},
+        "ess_per_s_ratio": rate_d / rate_p,
+        "moment_rel_err": {"mean": em.tolist(), "sd": es.tolist()},
+        "prior_setup_s": t_prior, "pilot_s": t_pilot,
+    }
+
+
 def cpu_baseline(args, budget_s):
     """The oracle (reference algorithm restated in C++, FP64, 1 thread):
     Triplet sort-merge assembly, banded Cholesky of the eliminated operator,
@@ -152,6 +208,12 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-rhs", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c5", action="store_true", help="skip the surrogate-MCMC leg")
+    ap.add_argument("--c5-n", type=int, default=256)
+    ap.add_argument("--c5-chains", type=int, default=64)
+    ap.add_argument("--c5-steps", type=int, default=20000)
+    ap.add_argument("--c5-da-steps", type=int, default=2000)
+    ap.add_argument("--c5-sub", type=int, default=16)
     ap.add_argument("--copy-cal", action="store_true",
                     help="also run a 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -169,9 +231,10 @@ def main():
         hip._check(hip.lib().sfh_memcpy_d2d(c.h, b.ptr, a.ptr, ctypes.c_size_t(1 << 30)))
         c.sync()
         log("copy calibration: 1 GiB device-to-device")
+    c5 = None if args.no_c5 else c5_run(args)
     cpu = None if args.no_cpu else cpu_baseline(args, 30.0)
     line = {
-        "metric": "assembled+solved DOF-RHS/s",
+        "metric": "assembled+solved DOF-RHS/s; MCMC ESS/s surrogate vs plain GPU at 1e-8 rel err",
         "value": res["value"],
         "unit": "DOF-RHS/s",
         "n_gpus": 1,
@@ -191,6 +254,7 @@ def main():
         "roofline": {"kernel": res["kernel"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
+        "mcmc": c5,
     }
     print(json.dumps(line), flush=True)
 

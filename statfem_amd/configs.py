@@ -75,3 +75,20 @@ def c2_inputs(seed=0, n=512, ns=256):
     cols, vals = M.sensor_rows(m, xs)
     return dict(mesh=m, bc=bc, kappa=kappa, xs=xs, cols=cols, vals=vals, sigma_f=0.1, ell_f=0.05,
                 nq1d=2, reg=1e-3)
+
+
+def c5_inputs(seed=0, n=256, ns=200, no=8):
+    """Hyperparameter MCMC problem: Poisson on n x n Q1 (kappa = 1), homogeneous
+    Dirichlet, mean forcing 1 with SE forcing covariance (0.1, 0.2), `ns`
+    interior sensors, `no` observation snapshots drawn from the statFEM data
+    model at the true hyperparameters (rho, sigma_d, ell_d) = (1.2, 0.05, 0.25)
+    with noise sigma_e = 1e-3. Log-normal priors: medians (1, 0.1, 0.3),
+    log-variance 0.5 each."""
+    m = M.grid2d(n, kind=M.QUAD4)
+    bc = M.dirichlet_all_boundary(m)
+    rng = np.random.default_rng(seed)
+    xs = rng.uniform(0.0, 1.0, size=(ns, 2)) * (1 - 2.0 / n) + 1.0 / n
+    cols, vals = M.sensor_rows(m, xs)
+    return dict(mesh=m, bc=bc, xs=xs, cols=cols, vals=vals, no=no, sigma_f=0.1, ell_f=0.2, nq1d=2, reg=1e-3,
+                truth=np.array([1.2, 0.05, 0.25]), sigma_e=1e-3, data_seed=seed + 1000,
+                prior_mu=np.log([1.0, 0.1, 0.3]), prior_var=np.array([0.5, 0.5, 0.5]))
