@@ -58,6 +58,8 @@ def gpu_run(args):
         _, _, st = step()
         log(f"warmup {w}: {st['inner_iters']} inner iterations, {st['outer']} refinements, "
             f"max rel. residual {st['max_relres']:.2e}, {st['ms']:.1f} ms")
+    kk = args.tile if 0 < args.tile < ns else ns
+    ctx.kernel_sampling(every=args.sample_every, max_samples=4096, min_rhs=kk)
     ctx.sync()
     t0 = time.perf_counter()
     ctx.timer_start()
@@ -72,24 +74,25 @@ def gpu_run(args):
     ms_step = 1000.0 * wall / args.steps
     value = m.nnodes * ns / (wall / args.steps)
 
+    kt = ctx.kernel_times()
+    log(f"sampled inner-PCG launches: {kt}")
     # assembly alone (kernel + its status check), averaged
     ctx.timer_start()
     for _ in range(10):
         s.assemble(hip.POISSON, kappa, nq1d=2)
     asm_ms = ctx.timer_stop() / 10
     s.applyConstraints(bc)
-    # per-launch times of the inner PCG kernels (HIP events, library stream)
     kk = args.tile if 0 < args.tile < ns else ns
-    kt = s.profile_pcg(kk, reps=args.kreps)
     N, nnz = s.ndof, s.nnz
     kp = (kk + 3) // 4 * 4
     mat = 8.0 * nnz + 4.0 * (N + 1) + 4.0 * N          # FP32 CSR values + columns + row pointers + D^-1
     algo = {
         "dir_spmv": mat + 12.0 * N * kp,                 # r, p_old read; p_new write
         "cg_update": mat + 20.0 * N * kp,                # p_new, x, r read; x, r write
-        "reduce": 8.0 * kp * ((N + 1023) // 1024),       # per-block partial sums (approx.)
     }
-    per = {k: {"ms": kt[k], "GBs": algo[k] / (kt[k] * 1e-3) / 1e9, "bytes": algo[k]} for k in kt}
+    per = {k: {"ms": kt[k], "GBs": algo[k] / (kt[k] * 1e-3) / 1e9, "bytes": algo[k], "samples": kt["samples"]}
+           for k in algo}
+    per["beta"] = {"ms": kt["beta"]}
     dom = max(("dir_spmv", "cg_update"), key=lambda k: kt[k])
     asm_bytes = m.nnodes * 2 * 8 + m.nel * 4 * 4 + m.nel * 8 + nnz * 8 + nnz * 4
     res = {
@@ -203,7 +206,8 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--ns", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--kreps", type=int, default=20)
+    ap.add_argument("--sample-every", type=int, default=16,
+                    help="time every n-th inner PCG iteration with HIP events inside the timed solves")
     ap.add_argument("--tile", type=int, default=64, help="right-hand sides per PCG column tile")
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-rhs", type=int, default=8)

@@ -152,13 +152,17 @@ int sfh_solve(sfh_system* sys, int trans, int k, const double* B_dev, double* X_
 int sfh_solve_rhs(sfh_system* sys, double* X_dev, const sfh_solve_opts* opts,
                   sfh_solve_stats* stats);
 
-/* Instrumentation, no reference counterpart: average device time per launch
- * (ms) of the inner-PCG kernels (direction+SpMV+dot, SpMV+update+dots, one
- * per-RHS reduction) on
- * the workspace of the last k-RHS FP32 solve, timed with HIP events on the
- * context stream. */
-int sfh_profile_pcg(sfh_system* sys, int k, int reps, double* ms_spmv, double* ms_update,
-                    double* ms_pupdate);
+/* Instrumentation, no reference counterpart: sampled device time of the
+ * inner-PCG launches inside real solves. With every > 0, every every-th inner
+ * iteration of the following solves on this context (at most max_samples per
+ * solve; only solves of at least min_rhs right-hand sides per column tile) is
+ * bracketed by HIP events on the context stream; the accumulators are reset.
+ * every = 0 turns sampling off. sfh_kernel_times returns the average device
+ * time (ms per launch) of ms[0] direction + SpMV + p.Ap, ms[1] alpha +
+ * SpMV + x/r update + r.z, r.r, ms[2] beta, and the number of samples since
+ * the last sfh_kernel_sampling call. */
+int sfh_kernel_sampling(sfh_ctx* ctx, int every, int max_samples, int min_rhs);
+int sfh_kernel_times(sfh_ctx* ctx, double* ms /* 3 */, long long* samples);
 
 /* ------------------------------------------------------------- statistics */
 /* CovarianceForcing::giveProjectedCovar building block (statX/infer/CovarianceForcing.ipp:37-109):

@@ -146,6 +146,7 @@ int sfh_ctx_destroy(sfh_ctx* c) {
     (void)hipEventDestroy(c->ev1);
     (void)hipEventDestroy(c->tev0);
     (void)hipEventDestroy(c->tev1);
+    for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return SFH_OK;
@@ -378,13 +379,31 @@ int sfh_solve_rhs(sfh_system* s, double* X, const sfh_solve_opts* o, sfh_solve_s
     return solve(s, 0, 1, s->F.as<double>(), 1, X, 1, o, st);
 }
 
-int sfh_profile_pcg(sfh_system* s, int k, int reps, double* ms_spmv, double* ms_update, double* ms_pupdate) {
-    if (!s || k <= 0 || reps <= 0) { set_error("bad argument"); return SFH_EARG; }
-    double ms[3] = {0, 0, 0};
-    SFH_TRY(profile_pcg(s, k, reps, ms));
-    if (ms_spmv) *ms_spmv = ms[0];
-    if (ms_update) *ms_update = ms[1];
-    if (ms_pupdate) *ms_pupdate = ms[2];
+int sfh_kernel_sampling(sfh_ctx* c, int every, int max_samples, int min_rhs) {
+    if (!c || every < 0 || max_samples < 0 || min_rhs < 0) { set_error("bad argument"); return SFH_EARG; }
+    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    const size_t want = every > 0 ? 4 * (size_t)max_samples : 0;
+    while (c->kev.size() > want) {
+        (void)hipEventDestroy(c->kev.back());
+        c->kev.pop_back();
+    }
+    while (c->kev.size() < want) {
+        hipEvent_t e;
+        SFH_CHECK_HIP(hipEventCreate(&e));
+        c->kev.push_back(e);
+    }
+    c->ksample = every;
+    c->kmin_rhs = min_rhs;
+    c->ksum[0] = c->ksum[1] = c->ksum[2] = 0.0;
+    c->kcount = 0;
+    return SFH_OK;
+}
+
+int sfh_kernel_times(sfh_ctx* c, double* ms, long long* n) {
+    if (!c || !ms) { set_error("bad argument"); return SFH_EARG; }
+    const double k = c->kcount > 0 ? (double)c->kcount : 1.0;
+    for (int j = 0; j < 3; ++j) ms[j] = c->ksum[j] / k;
+    if (n) *n = c->kcount;
     return SFH_OK;
 }
 

@@ -71,7 +71,8 @@ struct SolveWork {
     DevBuf x32, r32, p32, q32;       // FP32 inner PCG (or FP64 when precision == 1)
     DevBuf part0, part1;             // per-block partial dot products
     DevBuf scal;                     // per-RHS scalars
-    DevBuf nact;                     // active counters
+    DevBuf nact;                     // active / pending column counts
+    DevBuf acc;                      // per-column FP64 accumulators of the iteration kernels
 };
 
 }  // namespace sfh
@@ -84,6 +85,14 @@ struct sfh_ctx {
     int num_cu = 0;
     size_t lds_per_block = 0;
     std::vector<void*> allocs;
+    // sampled in-solve timing of the inner-PCG launches (direction + SpMV,
+    // SpMV + update, beta): every ksample-th iteration is bracketed by events
+    // (0 = off)
+    int ksample = 0;
+    int kmin_rhs = 0;
+    std::vector<hipEvent_t> kev;     // 4 events per sample
+    double ksum[3] = {0.0, 0.0, 0.0};
+    long long kcount = 0;
 };
 
 struct sfh_system {
@@ -98,6 +107,8 @@ struct sfh_system {
     sfh::DevBuf rowptr_node, colind_node;
     sfh::DevBuf rowptr, colind, diagpos;
     sfh::DevBuf vals, vals32, F, dinv32, dinv64;
+    sfh::DevBuf diag32, diag64;  // A_ii (the PCG keeps z = D^-1 r and needs both)
+    sfh::DevBuf cv32;            // FP32 operator as packed (column, value bits) pairs
     sfh::DevBuf cmask;           // 1 on constrained DOFs
     sfh::DevBuf cdofs;           // constrained DOF list
     int nc = 0;
@@ -147,7 +158,6 @@ int loglik_batch(sfh_lik* L, int M, const double* theta_dev, double* out_dev, in
 int dense_spd_solve(sfh_ctx* c, int n, double* A_dev, int m, double* B_dev, int* status_dev);
 int mcmc_run(sfh_lik* L, int C, int steps, const double* theta0, const double* propsd, uint64_t seed,
              int mode, const double* pm, const double* pv, double* samples, double* acc);
-int profile_pcg(sfh_system* s, int k, int reps, double* ms);
 int typical_readings(sfh_lik* L, double rho, double sd, double ld, int nr, uint64_t seed, double* Y);
 int spmv(sfh_system* s, int k, const double* X, double* Y);
 int precision_apply(sfh_system* s, const double* mass, double tau2k4b, int alpha, int k, const double* X, double* Y);

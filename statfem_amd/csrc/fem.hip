@@ -585,6 +585,9 @@ int build_pattern(sfh_system* s) {
     SFH_TRY(s->cmask.ensure(s->ndof));
     SFH_TRY(s->dinv32.ensure(sizeof(float) * s->ndof));
     SFH_TRY(s->dinv64.ensure(sizeof(double) * s->ndof));
+    SFH_TRY(s->diag32.ensure(sizeof(float) * s->ndof));
+    SFH_TRY(s->diag64.ensure(sizeof(double) * s->ndof));
+    SFH_TRY(s->cv32.ensure(sizeof(int2) * (size_t)s->nnz));
     SFH_CHECK_HIP(hipMemsetAsync(s->vals.p, 0, sizeof(double) * (size_t)s->nnz, st));
     SFH_CHECK_HIP(hipMemsetAsync(s->vals32.p, 0, sizeof(float) * (size_t)s->nnz, st));
     SFH_CHECK_HIP(hipMemsetAsync(s->F.p, 0, sizeof(double) * s->ndof, st));

@@ -23,7 +23,9 @@
 // eliminated: constrained components of the iterate stay exactly zero, so the
 // free block A_ff is what the PCG sees; constrained values and, for transposed
 // solves, r_c = (p_c - sum_f A[f][c] r_f) / A[c][c] are restored afterwards.
-// All reductions are deterministic (fixed-order two-stage sums).
+// The iteration kernels' dot products are added into per-column FP64
+// accumulators with hardware atomics (one per column and work-group); the
+// refinement-time reductions are fixed-order two-stage sums.
 #include "common.hpp"
 
 namespace sfh {
@@ -31,6 +33,7 @@ namespace {
 
 constexpr int kB = 1024;   // 16 waves per work-group: 16 waves per SIMD at 1024 work-groups
 constexpr int kRedB = 1024;   // reduction blocks: 16 row groups x 64 columns
+constexpr int kU = 3;         // gathered neighbours in flight per lane in the direction kernel (9-point rows = 3 x 3)
 
 template <class T> struct V4 { T x, y, z, w; };
 
@@ -92,6 +95,46 @@ __device__ inline void block_partial(const Tile& t, int kp, const V4<double>& a,
     }
 }
 
+// per-RHS scalar slots (each kp doubles)
+enum { S_RZ = 0, S_RZPREV, S_ALPHA, S_BETA, S_TRIG, S_TGT, S_ACTIVE, S_SCALE, S_RATIO, S_RN2, S_BN2,
+       S_NSLOT };
+
+// per-column FP64 accumulators of the iteration kernels (kp doubles each)
+enum { A_PQ = 0, A_RZ, A_RR, A_N };
+
+// block-level sum over row groups of per-lane 4-column partials, added into
+// acc[col] with one hardware FP64 atomic per column and block (the order of the
+// additions across blocks is not fixed)
+template <int LPR>
+__device__ inline void block_accumulate(const Tile& t, int kp, const V4<double>& a, double* red, double* acc) {
+    const int w = 4 * LPR;
+    red[t.grp * w + t.lane * 4 + 0] = a.x;
+    red[t.grp * w + t.lane * 4 + 1] = a.y;
+    red[t.grp * w + t.lane * 4 + 2] = a.z;
+    red[t.grp * w + t.lane * 4 + 3] = a.w;
+    __syncthreads();
+    for (int c = threadIdx.x; c < w; c += blockDim.x) {
+        double s = 0.0;
+        for (int g = 0; g < t.ngrp; ++g) s += red[g * w + c];
+        const int col = blockIdx.y * w + c;
+        if (col < kp) unsafeAtomicAdd(acc + col, s);
+    }
+}
+
+// step length of 4 consecutive columns: alpha = r.z / p.Ap on active columns,
+// from the p.Ap accumulated by the direction kernel
+template <class T>
+__device__ inline V4<T> step_alpha(int kp, int col0, const double* __restrict__ S, const double* __restrict__ acc) {
+    double a[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = col0 + j;
+        const double pq = acc[A_PQ * kp + col];
+        a[j] = (S[S_ACTIVE * kp + col] > 0.0 && pq > 0.0) ? S[S_RZ * kp + col] / pq : 0.0;
+    }
+    return {(T)a[0], (T)a[1], (T)a[2], (T)a[3]};
+}
+
 // row index of the lane group; wave-uniform (scalar) when one row spans the wave
 template <int LPR>
 __device__ inline int uniform_row(int r) {
@@ -99,39 +142,61 @@ __device__ inline int uniform_row(int r) {
     return r;
 }
 
-// preconditioned direction entry p = D^-1 r + beta p_old, evaluated identically
-// wherever it is needed (fused into the SpMV gathers)
-__device__ inline float pdir(float d, float r, float b, float po) { return fmaf(b, po, d * r); }
-__device__ inline double pdir(double d, double r, double b, double po) { return fma(b, po, d * r); }
+// operator entries of one CSR row: FP32 as packed (column, value bits) pairs
+// (one 8-byte load per entry), FP64 from the separate column / value arrays
+template <class T> struct Entries;
+template <> struct Entries<float> {
+    const int2* cv;
+    const int* ci;
+    const float* va;
+    __device__ inline void get(int e, int& c, float& a) const {
+        const int2 v = cv[e];
+        c = v.x;
+        a = __int_as_float(v.y);
+    }
+};
+template <> struct Entries<double> {
+    const int2* cv;
+    const int* ci;
+    const double* va;
+    __device__ inline void get(int e, int& c, double& a) const {
+        c = ci[e];
+        a = va[e];
+    }
+};
 
+// preconditioned direction entry p = m z + b p_old (m = 0 freezes a column),
+// evaluated identically wherever it is needed (fused into the SpMV gathers)
 template <class T>
-__device__ inline V4<T> mul4(const V4<T>& a, const V4<T>& b) {
-    return {a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w};
+__device__ inline void pdir4(const V4<T>& z, const V4<T>& m, const V4<T>& b, const V4<T>& po, V4<T>& out) {
+    out.x = b.x * po.x + m.x * z.x; out.y = b.y * po.y + m.y * z.y;
+    out.z = b.z * po.z + m.z * z.z; out.w = b.w * po.w + m.w * z.w;
 }
 
+// r.z and r.r of one row from z = D^-1 r (r = g z, g = A_ii)
 template <class T>
-__device__ inline void pdir4(T d, const V4<T>& r, const V4<T>& b, const V4<T>& po, V4<T>& out) {
-    out.x = pdir(d, r.x, b.x, po.x); out.y = pdir(d, r.y, b.y, po.y);
-    out.z = pdir(d, r.z, b.z, po.z); out.w = pdir(d, r.w, b.w, po.w);
+__device__ inline void res_dots(const V4<T>& z, T g, V4<T>& rz, V4<T>& rr) {
+    const V4<T> r = {g * z.x, g * z.y, g * z.z, g * z.w};
+    rz.x += r.x * z.x; rz.y += r.y * z.y; rz.z += r.z * z.z; rz.w += r.w * z.w;
+    rr.x += r.x * r.x; rr.y += r.y * r.y; rr.z += r.z * r.z; rr.w += r.w * r.w;
 }
 
-// K1: p_new = D^-1 r + beta p_old (every row, recomputed on the fly at the
-// neighbours the stencil touches), q = A p_new, part = sum_rows p_new . q.
-// q itself is not stored.
+// K1: p_new = z + beta p_old (z = D^-1 r is what the solver stores; every row,
+// recomputed on the fly at the neighbours the stencil touches), q = A p_new,
+// acc += sum_rows p_new . q. q itself is not stored.
 template <class T, int LPR>
 __global__ __launch_bounds__(kB) void k_dir_spmv(int n, int kp, int rpb, const int* __restrict__ rp,
-                                                 const int* __restrict__ ci, const T* __restrict__ va,
-                                                 const T* __restrict__ dinv, const double* __restrict__ beta,
-                                                 const double* __restrict__ active, const T* __restrict__ r,
+                                                 const Entries<T> M, const double* __restrict__ beta,
+                                                 const double* __restrict__ active, const T* __restrict__ z,
                                                  const T* __restrict__ pold, T* __restrict__ pnew,
-                                                 double* __restrict__ part) {
+                                                 double* __restrict__ acc) {
     __shared__ double red[kB * 4];
     const Tile t = tile<LPR>(n, kp, rpb);
     const bool ok = t.col0 < kp;
-    V4<T> acc = {0, 0, 0, 0};
+    V4<T> pq = {0, 0, 0, 0};
     if (ok) {
         // frozen columns (waiting for residual replacement) keep their direction:
-        // p = 1 * p_old + D^-1 (0 * r)
+        // p = 1 * p_old + 0 * z
         V4<T> b, m;
         {
             const double* bb = beta + t.col0;
@@ -146,56 +211,57 @@ __global__ __launch_bounds__(kB) void k_dir_spmv(int n, int kp, int rpb, const i
             V4<T> q = {0, 0, 0, 0};
             int e = rp[row];
             const int e1 = rp[row + 1];
-            for (; e + 4 <= e1; e += 4) {
-                int c[4];
-                T a[4], d[4];
-                V4<T> rv[4], pv[4];
+            for (; e + kU <= e1; e += kU) {
+                int c[kU];
+                T a[kU];
+                V4<T> zv[kU], pv[kU];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) { c[u] = ci[e + u]; a[u] = va[e + u]; }
+                for (int u = 0; u < kU; ++u) M.get(e + u, c[u], a[u]);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    d[u] = dinv[c[u]];
-                    rv[u] = mul4(ld4(r + (size_t)c[u] * kp + t.col0), m);
+                for (int u = 0; u < kU; ++u) {
+                    zv[u] = ld4(z + (size_t)c[u] * kp + t.col0);
                     pv[u] = ld4(pold + (size_t)c[u] * kp + t.col0);
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kU; ++u) {
                     V4<T> pj;
-                    pdir4(d[u], rv[u], b, pv[u], pj);
+                    pdir4(zv[u], m, b, pv[u], pj);
                     q.x += a[u] * pj.x; q.y += a[u] * pj.y; q.z += a[u] * pj.z; q.w += a[u] * pj.w;
                 }
             }
             for (; e < e1; ++e) {
-                const int c = ci[e];
-                const T a = va[e];
+                int c;
+                T a;
+                M.get(e, c, a);
                 V4<T> pj;
-                pdir4(dinv[c], mul4(ld4(r + (size_t)c * kp + t.col0), m), b, ld4(pold + (size_t)c * kp + t.col0), pj);
+                pdir4(ld4(z + (size_t)c * kp + t.col0), m, b, ld4(pold + (size_t)c * kp + t.col0), pj);
                 q.x += a * pj.x; q.y += a * pj.y; q.z += a * pj.z; q.w += a * pj.w;
             }
             const size_t o = (size_t)row * kp + t.col0;
             V4<T> pi;
-            pdir4(dinv[row], mul4(ld4(r + o), m), b, ld4(pold + o), pi);
+            pdir4(ld4(z + o), m, b, ld4(pold + o), pi);
             st4(pnew + o, pi);
-            acc.x += pi.x * q.x; acc.y += pi.y * q.y; acc.z += pi.z * q.z; acc.w += pi.w * q.w;
+            pq.x += pi.x * q.x; pq.y += pi.y * q.y; pq.z += pi.z * q.z; pq.w += pi.w * q.w;
         }
     }
-    const V4<double> ad = {(double)acc.x, (double)acc.y, (double)acc.z, (double)acc.w};
-    block_partial<LPR>(t, kp, ad, red, part);
+    const V4<double> ad = {(double)pq.x, (double)pq.y, (double)pq.z, (double)pq.w};
+    block_accumulate<LPR>(t, kp, ad, red, acc + A_PQ * kp);
 }
 
-// K2: q = A p (recomputed), x += alpha p, r -= alpha q, part0 = r.D^-1 r, part1 = r.r
+// K2: q = A p (recomputed), x += alpha p, z -= alpha D^-1 q (z = D^-1 r),
+// acc += r.z, r.r with r = diag z
 template <class T, int LPR>
 __global__ __launch_bounds__(kB) void k_cg_update(int n, int kp, int rpb, const int* __restrict__ rp,
-                                                  const int* __restrict__ ci, const T* __restrict__ va,
-                                                  const T* __restrict__ dinv, const double* __restrict__ alpha,
-                                                  const T* __restrict__ p, T* __restrict__ x, T* __restrict__ r,
-                                                  double* __restrict__ part0, double* __restrict__ part1) {
+                                                  const Entries<T> M, const T* __restrict__ dinv,
+                                                  const T* __restrict__ diag, const double* __restrict__ S,
+                                                  const T* __restrict__ p, T* __restrict__ x, T* __restrict__ z,
+                                                  double* __restrict__ acc) {
     __shared__ double red[kB * 4];
     const Tile t = tile<LPR>(n, kp, rpb);
     const bool ok = t.col0 < kp;
     V4<T> rz = {0, 0, 0, 0}, rr = {0, 0, 0, 0};
     if (ok) {
-        const V4<T> al = {(T)alpha[t.col0], (T)alpha[t.col0 + 1], (T)alpha[t.col0 + 2], (T)alpha[t.col0 + 3]};
+        const V4<T> al = step_alpha<T>(kp, t.col0, S, acc);
         for (int row0 = t.r0 + t.grp; row0 < t.r1; row0 += t.ngrp) {
             const int row = uniform_row<LPR>(row0);
             V4<T> q = {0, 0, 0, 0};
@@ -206,7 +272,7 @@ __global__ __launch_bounds__(kB) void k_cg_update(int n, int kp, int rpb, const 
                 T a[4];
                 V4<T> pv[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) { c[u] = ci[e + u]; a[u] = va[e + u]; }
+                for (int u = 0; u < 4; ++u) M.get(e + u, c[u], a[u]);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) pv[u] = ld4(p + (size_t)c[u] * kp + t.col0);
 #pragma unroll
@@ -215,35 +281,38 @@ __global__ __launch_bounds__(kB) void k_cg_update(int n, int kp, int rpb, const 
                 }
             }
             for (; e < e1; ++e) {
-                const T a = va[e];
-                const V4<T> pv = ld4(p + (size_t)ci[e] * kp + t.col0);
+                int c;
+                T a;
+                M.get(e, c, a);
+                const V4<T> pv = ld4(p + (size_t)c * kp + t.col0);
                 q.x += a * pv.x; q.y += a * pv.y; q.z += a * pv.z; q.w += a * pv.w;
             }
             const size_t o = (size_t)row * kp + t.col0;
             const V4<T> pi = ld4(p + o);
-            V4<T> xv = ld4(x + o), rv = ld4(r + o);
+            V4<T> xv = ld4(x + o), zv = ld4(z + o);
+            const T d = dinv[row], g = diag[row];
             xv.x += al.x * pi.x; xv.y += al.y * pi.y; xv.z += al.z * pi.z; xv.w += al.w * pi.w;
-            rv.x -= al.x * q.x; rv.y -= al.y * q.y; rv.z -= al.z * q.z; rv.w -= al.w * q.w;
+            zv.x -= al.x * d * q.x; zv.y -= al.y * d * q.y; zv.z -= al.z * d * q.z; zv.w -= al.w * d * q.w;
             st4(x + o, xv);
-            st4(r + o, rv);
-            const T d = dinv[row];
-            rz.x += rv.x * rv.x * d; rz.y += rv.y * rv.y * d; rz.z += rv.z * rv.z * d; rz.w += rv.w * rv.w * d;
-            rr.x += rv.x * rv.x; rr.y += rv.y * rv.y; rr.z += rv.z * rv.z; rr.w += rv.w * rv.w;
+            st4(z + o, zv);
+            res_dots(zv, g, rz, rr);
         }
     }
     V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
-    block_partial<LPR>(t, kp, a, red, part0);
+    block_accumulate<LPR>(t, kp, a, red, acc + A_RZ * kp);
     __syncthreads();
     V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
-    block_partial<LPR>(t, kp, bb, red, part1);
+    block_accumulate<LPR>(t, kp, bb, red, acc + A_RR * kp);
 }
 
-// Residual replacement: r = s' r64, p *= ratio (s'/s), part0 = r.D^-1 r, part1 = r.r
+// Residual replacement: z = D^-1 (s' r64), p *= ratio (s'/s), part0 = r.z,
+// part1 = r.r
 template <class T, int LPR>
 __global__ __launch_bounds__(kB) void k_rescale(int n, int kp, int rpb, const double* __restrict__ scale,
-                                                const double* __restrict__ ratio, const T* __restrict__ dinv,
-                                                const double* __restrict__ r64, T* __restrict__ r, T* __restrict__ p,
-                                                double* __restrict__ part0, double* __restrict__ part1) {
+                                                const double* __restrict__ ratio, const double* __restrict__ dinv64,
+                                                const T* __restrict__ diag, const double* __restrict__ r64,
+                                                T* __restrict__ z, T* __restrict__ p, double* __restrict__ part0,
+                                                double* __restrict__ part1) {
     __shared__ double red[kB * 4];
     const Tile t = tile<LPR>(n, kp, rpb);
     const bool ok = t.col0 < kp;
@@ -254,14 +323,13 @@ __global__ __launch_bounds__(kB) void k_rescale(int n, int kp, int rpb, const do
         for (int row = t.r0 + t.grp; row < t.r1; row += t.ngrp) {
             const size_t o = (size_t)row * kp + t.col0;
             const V4<double> rd = ld4(r64 + o);
-            const V4<T> rv = {(T)(rd.x * s0), (T)(rd.y * s1), (T)(rd.z * s2), (T)(rd.w * s3)};
+            const double di = dinv64[row];
+            const V4<T> zv = {(T)(rd.x * s0 * di), (T)(rd.y * s1 * di), (T)(rd.z * s2 * di), (T)(rd.w * s3 * di)};
             V4<T> pv = ld4(p + o);
             pv.x *= q.x; pv.y *= q.y; pv.z *= q.z; pv.w *= q.w;
-            st4(r + o, rv);
+            st4(z + o, zv);
             st4(p + o, pv);
-            const T d = dinv[row];
-            rz.x += rv.x * rv.x * d; rz.y += rv.y * rv.y * d; rz.z += rv.z * rv.z * d; rz.w += rv.w * rv.w * d;
-            rr.x += rv.x * rv.x; rr.y += rv.y * rv.y; rr.z += rv.z * rv.z; rr.w += rv.w * rv.w;
+            res_dots(zv, diag[row], rz, rr);
         }
     }
     V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
@@ -302,12 +370,12 @@ __device__ inline Tile as_tile(const GridTile& g) {
 template <class T, int LPR>
 __global__ __launch_bounds__(kB) void k_dir_spmv_g2(int W, int H, int kp, int L, int nsegx, const T* __restrict__ coef,
                                                     const T* __restrict__ dinv, const double* __restrict__ beta,
-                                                    const double* __restrict__ active, const T* __restrict__ r,
+                                                    const double* __restrict__ active, const T* __restrict__ z,
                                                     const T* __restrict__ pold, T* __restrict__ pnew,
-                                                    double* __restrict__ part) {
+                                                    double* __restrict__ acc) {
     __shared__ double red[kB * 4];
     const GridTile g = grid_tile<LPR>();
-    V4<T> acc = {0, 0, 0, 0};
+    V4<T> pq = {0, 0, 0, 0};
     if (g.col0 < kp && g.seg < nsegx * H) {
         V4<T> b, m;
         {
@@ -326,7 +394,7 @@ __global__ __launch_bounds__(kB) void k_dir_spmv_g2(int W, int H, int kp, int L,
             if (x >= 0 && x < W && yy >= 0 && yy < H) {
                 const int nd = yy * W + x;
                 const size_t o = (size_t)nd * kp + g.col0;
-                pdir4(dinv[nd], mul4(ld4(r + o), m), b, ld4(pold + o), v);
+                pdir4(ld4(z + o), m, b, ld4(pold + o), v);
             }
             return v;
         };
@@ -349,25 +417,25 @@ __global__ __launch_bounds__(kB) void k_dir_spmv_g2(int W, int H, int kp, int L,
             }
             const V4<T> pi = w1[1];
             st4(pnew + (size_t)nd * kp + g.col0, pi);
-            acc.x += pi.x * q.x; acc.y += pi.y * q.y; acc.z += pi.z * q.z; acc.w += pi.w * q.w;
+            pq.x += pi.x * q.x; pq.y += pi.y * q.y; pq.z += pi.z * q.z; pq.w += pi.w * q.w;
 #pragma unroll
             for (int d = 0; d < 3; ++d) { w0[d] = w1[d]; w1[d] = w2[d]; }
         }
     }
-    const V4<double> ad = {(double)acc.x, (double)acc.y, (double)acc.z, (double)acc.w};
-    block_partial<LPR>(as_tile<LPR>(g), kp, ad, red, part);
+    const V4<double> ad = {(double)pq.x, (double)pq.y, (double)pq.z, (double)pq.w};
+    block_accumulate<LPR>(as_tile<LPR>(g), kp, ad, red, acc + A_PQ * kp);
 }
 
 template <class T, int LPR>
 __global__ __launch_bounds__(kB) void k_cg_update_g2(int W, int H, int kp, int L, int nsegx, const T* __restrict__ coef,
-                                                     const T* __restrict__ dinv, const double* __restrict__ alpha,
-                                                     const T* __restrict__ p, T* __restrict__ x, T* __restrict__ r,
-                                                     double* __restrict__ part0, double* __restrict__ part1) {
+                                                     const T* __restrict__ dinv, const T* __restrict__ diag, const double* __restrict__ S,
+                                                     const T* __restrict__ p, T* __restrict__ x, T* __restrict__ z,
+                                                     double* __restrict__ acc) {
     __shared__ double red[kB * 4];
     const GridTile g = grid_tile<LPR>();
     V4<T> rz = {0, 0, 0, 0}, rr = {0, 0, 0, 0};
     if (g.col0 < kp && g.seg < nsegx * H) {
-        const V4<T> al = {(T)alpha[g.col0], (T)alpha[g.col0 + 1], (T)alpha[g.col0 + 2], (T)alpha[g.col0 + 3]};
+        const V4<T> al = step_alpha<T>(kp, g.col0, S, acc);
         const int y = uniform_row<LPR>(g.seg / nsegx);
         const int xs = uniform_row<LPR>((g.seg % nsegx) * L);
         const int xe = min(W, xs + L);
@@ -395,24 +463,23 @@ __global__ __launch_bounds__(kB) void k_cg_update_g2(int W, int H, int kp, int L
             }
             const size_t o = (size_t)nd * kp + g.col0;
             const V4<T> pi = w1[1];
-            V4<T> xv = ld4(x + o), rv = ld4(r + o);
+            V4<T> xv = ld4(x + o), zv = ld4(z + o);
+            const T dd = dinv[nd], g = diag[nd];
             xv.x += al.x * pi.x; xv.y += al.y * pi.y; xv.z += al.z * pi.z; xv.w += al.w * pi.w;
-            rv.x -= al.x * q.x; rv.y -= al.y * q.y; rv.z -= al.z * q.z; rv.w -= al.w * q.w;
+            zv.x -= al.x * dd * q.x; zv.y -= al.y * dd * q.y; zv.z -= al.z * dd * q.z; zv.w -= al.w * dd * q.w;
             st4(x + o, xv);
-            st4(r + o, rv);
-            const T d = dinv[nd];
-            rz.x += rv.x * rv.x * d; rz.y += rv.y * rv.y * d; rz.z += rv.z * rv.z * d; rz.w += rv.w * rv.w * d;
-            rr.x += rv.x * rv.x; rr.y += rv.y * rv.y; rr.z += rv.z * rv.z; rr.w += rv.w * rv.w;
+            st4(z + o, zv);
+            res_dots(zv, g, rz, rr);
 #pragma unroll
             for (int d2 = 0; d2 < 3; ++d2) { w0[d2] = w1[d2]; w1[d2] = w2[d2]; }
         }
     }
     const Tile t = as_tile<LPR>(g);
     V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
-    block_partial<LPR>(t, kp, a, red, part0);
+    block_accumulate<LPR>(t, kp, a, red, acc + A_RZ * kp);
     __syncthreads();
     V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
-    block_partial<LPR>(t, kp, bb, red, part1);
+    block_accumulate<LPR>(t, kp, bb, red, acc + A_RR * kp);
 }
 
 // CSR -> DIA-9. Every entry must couple a grid neighbour of its row's node;
@@ -472,10 +539,6 @@ __global__ void k_refine(long long total, int kp, const double* __restrict__ sca
 }
 
 // ------------------------------------------------------------ reductions
-// per-RHS scalar slots (each kp doubles)
-enum { S_RZ = 0, S_RZPREV, S_ALPHA, S_BETA, S_TRIG, S_TGT, S_ACTIVE, S_SCALE, S_RATIO, S_RN2, S_BN2,
-       S_NSLOT };
-
 // Fixed-order sum over the nblk per-block partial rows of one column: 16
 // groups of 64 lanes, each group strides over the rows with 8 independent
 // loads in flight, then a fixed-order combine of the 16 group sums.
@@ -500,28 +563,20 @@ __device__ inline double sum_parts(int nblk, int kp, const double* __restrict__ 
     return tot;
 }
 
-__global__ __launch_bounds__(kRedB) void k_red_alpha(int kp, int nblk, const double* part, double* S) {
-    __shared__ double red[kRedB];
-    const int col = blockIdx.x * 64 + threadIdx.x % 64;
-    const double pq = sum_parts(nblk, kp, part, col, red);
-    if (threadIdx.x < 64 && col < kp) {
-        const double rz = S[S_RZ * kp + col];
-        S[S_ALPHA * kp + col] = (S[S_ACTIVE * kp + col] > 0.0 && pq > 0.0) ? rz / pq : 0.0;
-    }
-}
-
-// beta, rz; a column whose recursive residual reached its trigger is frozen
-// (alpha = 0, direction kept) until the next FP64 replacement. Counts active
-// and frozen (pending) columns.
-__global__ __launch_bounds__(kRedB) void k_red_beta(int kp, int nblk, const double* part0,
-                                                    const double* part1, double* S, int* cnt2) {
-    __shared__ double red[kRedB];
+// beta, rz from the accumulated r.z and r.r; all accumulators are cleared for
+// the next iteration. A column whose recursive
+// residual reached its trigger is frozen (alpha = 0, direction kept) until the
+// next FP64 replacement. Counts active and frozen (pending) columns.
+__global__ void k_red_beta(int kp, double* __restrict__ acc, double* __restrict__ S, int* __restrict__ cnt2) {
     __shared__ int nact, npend;
     if (threadIdx.x == 0) { nact = 0; npend = 0; }
-    const int col = blockIdx.x * 64 + threadIdx.x % 64;
-    const double rzn = sum_parts(nblk, kp, part0, col, red);
-    const double rrn = sum_parts(nblk, kp, part1, col, red);
-    if (threadIdx.x < 64 && col < kp) {
+    __syncthreads();
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col < kp) {
+        const double rzn = acc[A_RZ * kp + col], rrn = acc[A_RR * kp + col];
+        acc[A_PQ * kp + col] = 0.0;
+        acc[A_RZ * kp + col] = 0.0;
+        acc[A_RR * kp + col] = 0.0;
         if (S[S_ACTIVE * kp + col] > 0.0) {
             const double rz = S[S_RZ * kp + col];
             S[S_BETA * kp + col] = rz > 0.0 ? rzn / rz : 0.0;
@@ -569,12 +624,21 @@ __global__ __launch_bounds__(kRedB) void k_red_norm(int kp, int nblk, const doub
 }
 
 // ---------------------------------------------------- constraint handling
-__global__ void k_dinv(int n, const int* diag, const double* va, float* d32, double* d64) {
+__global__ void k_dinv(int n, const int* diagpos, const double* va, float* d32, double* d64, float* g32,
+                       double* g64) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double d = va[diag[i]];
+    const double d = va[diagpos[i]];
     d64[i] = 1.0 / d;
     d32[i] = (float)(1.0 / d);
+    g64[i] = d;
+    g32[i] = (float)d;
+}
+
+__global__ void k_pack_entries(long long nnz, const int* __restrict__ ci, const float* __restrict__ va32,
+                               int2* __restrict__ cv) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nnz) cv[e] = make_int2(ci[e], __float_as_int(va32[e]));
 }
 
 // b64 (N x kp): free rows carry B (non-transposed: minus the constrained-column
@@ -673,31 +737,30 @@ void plan_grid(Plan& p, int W, int H) {
 
 template <class T, int LPR>
 struct Kern {
-    static void dir_spmv(const Plan& P, hipStream_t st, const int* rp, const int* ci, const T* va, const T* dinv,
-                         const double* beta, const double* active, const T* r, const T* pold, T* pnew,
-                         double* part) {
-        k_dir_spmv<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, rp, ci, va, dinv, beta, active, r,
-                                                                 pold, pnew, part);
+    static void dir_spmv(const Plan& P, hipStream_t st, const int* rp, const Entries<T>& M, const double* beta,
+                         const double* active, const T* z, const T* pold, T* pnew, double* acc) {
+        k_dir_spmv<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, rp, M, beta, active, z, pold, pnew,
+                                                                 acc);
     }
-    static void update(const Plan& P, hipStream_t st, const int* rp, const int* ci, const T* va, const T* dinv,
-                       const double* alpha, const T* p, T* x, T* r, double* p0, double* p1) {
-        k_cg_update<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, rp, ci, va, dinv, alpha, p, x, r,
-                                                                  p0, p1);
+    static void update(const Plan& P, hipStream_t st, const int* rp, const Entries<T>& M, const T* dinv,
+                       const T* diag, const double* S, const T* p, T* x, T* z, double* acc) {
+        k_cg_update<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, rp, M, dinv, diag, S, p, x, z,
+                                                                  acc);
     }
     static void dir_spmv_g(const Plan& P, hipStream_t st, const T* coef, const T* dinv, const double* beta,
-                           const double* active, const T* r, const T* pold, T* pnew, double* part) {
+                           const double* active, const T* z, const T* pold, T* pnew, double* acc) {
         k_dir_spmv_g2<T, LPR><<<dim3(P.gblk, P.ntile), kB, 0, st>>>(P.W, P.H, P.kp, P.L, P.nsegx, coef, dinv, beta,
-                                                                    active, r, pold, pnew, part);
+                                                                    active, z, pold, pnew, acc);
     }
-    static void update_g(const Plan& P, hipStream_t st, const T* coef, const T* dinv, const double* alpha, const T* p,
-                         T* x, T* r, double* p0, double* p1) {
-        k_cg_update_g2<T, LPR><<<dim3(P.gblk, P.ntile), kB, 0, st>>>(P.W, P.H, P.kp, P.L, P.nsegx, coef, dinv, alpha,
-                                                                     p, x, r, p0, p1);
+    static void update_g(const Plan& P, hipStream_t st, const T* coef, const T* dinv, const T* diag, const double* S,
+                         const T* p, T* x, T* z, double* acc) {
+        k_cg_update_g2<T, LPR><<<dim3(P.gblk, P.ntile), kB, 0, st>>>(P.W, P.H, P.kp, P.L, P.nsegx, coef, dinv, diag, S,
+                                                                     p, x, z, acc);
     }
-    static void rescale(const Plan& P, hipStream_t st, const double* scale, const double* ratio, const T* dinv,
-                        const double* r64, T* r, T* p, double* p0, double* p1) {
-        k_rescale<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, scale, ratio, dinv, r64, r, p, p0,
-                                                                p1);
+    static void rescale(const Plan& P, hipStream_t st, const double* scale, const double* ratio, const double* dinv64,
+                        const T* diag, const double* r64, T* z, T* p, double* p0, double* p1) {
+        k_rescale<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, scale, ratio, dinv64, diag, r64, z, p,
+                                                                p0, p1);
     }
     static void resid(const Plan& P, hipStream_t st, const int* rp, const int* ci, const double* va,
                       const double* x, const double* b, double* r, double* part) {
@@ -748,7 +811,9 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_TRY(w.part0.ensure((size_t)std::max(P.nblk, nbi) * kp * 8));
     SFH_TRY(w.part1.ensure((size_t)std::max(P.nblk, nbi) * kp * 8));
     SFH_TRY(w.scal.ensure((size_t)S_NSLOT * kp * 8));
-    SFH_TRY(w.nact.ensure(sizeof(int) * 2 * P.nred));
+    const int nred = (kp + 255) / 256;
+    SFH_TRY(w.nact.ensure(sizeof(int) * 2 * nred));
+    SFH_TRY(w.acc.ensure((size_t)A_N * kp * 8));
     double* S = w.scal.as<double>();
     const T* coef = std::is_same<T, float>::value ? (const T*)s->dia32.p : (const T*)s->dia64.p;
     const int* rp = s->rowptr.as<int>();
@@ -756,8 +821,10 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     const double* va = s->vals.as<double>();
     const T* vaT = std::is_same<T, float>::value ? (const T*)s->vals32.p : (const T*)s->vals.p;
     const T* dinv = std::is_same<T, float>::value ? (const T*)s->dinv32.p : (const T*)s->dinv64.p;
+    const T* diag = std::is_same<T, float>::value ? (const T*)s->diag32.p : (const T*)s->diag64.p;
+    const Entries<T> M = {s->cv32.as<int2>(), ci, vaT};
     T* x = w.x32.as<T>();
-    T* r = w.r32.as<T>();
+    T* z = w.r32.as<T>();   // z = D^-1 r
     T* pa = w.p32.as<T>();
     T* pb = w.q32.as<T>();
     double* p0 = w.part0.as<double>();
@@ -766,7 +833,10 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_CHECK_HIP(hipEventRecord(s->ctx->ev0, st));
     if (s->dirty) {
         k_dinv<<<(n + 255) / 256, 256, 0, st>>>(n, s->diagpos.as<int>(), va, s->dinv32.as<float>(),
-                                                s->dinv64.as<double>());
+                                                s->dinv64.as<double>(), s->diag32.as<float>(), s->diag64.as<double>());
+        SFH_LAUNCH_CHECK();
+        k_pack_entries<<<(unsigned)((s->nnz + 255) / 256), 256, 0, st>>>(s->nnz, ci, s->vals32.as<float>(),
+                                                                         s->cv32.as<int2>());
         SFH_LAUNCH_CHECK();
         s->dirty = false;
     }
@@ -778,6 +848,11 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_CHECK_HIP(hipMemsetAsync(w.x32.p, 0, nk * sizeof(T), st));
     SFH_CHECK_HIP(hipMemsetAsync(w.p32.p, 0, nk * sizeof(T), st));
     SFH_CHECK_HIP(hipMemsetAsync(w.scal.p, 0, (size_t)S_NSLOT * kp * 8, st));
+    SFH_CHECK_HIP(hipMemsetAsync(w.acc.p, 0, (size_t)A_N * kp * 8, st));
+    double* acc = w.acc.as<double>();
+    sfh_ctx* cx = s->ctx;
+    const int nsmp = (cx->ksample > 0 && k >= cx->kmin_rhs) ? (int)cx->kev.size() / 4 : 0;
+    int ismp = 0;
     Kern<T, LPR>::resid(P, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
     k_red_norm<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, p0, S + S_BN2 * kp);
     SFH_LAUNCH_CHECK();
@@ -790,7 +865,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     int updates = 0, iters = 0;
     double maxrel = 0.0;
     bool done = false;
-    std::vector<int> cnt(2 * P.nred);
+    std::vector<int> cnt(2 * nred);
     const int check = std::max(1, o.check_every);
     for (;;) {
         // residual replacement: decide per column, rescale, keep directions
@@ -812,31 +887,37 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_SCALE * kp, scale.data(), kp * 8, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_RATIO * kp, ratio.data(), kp * 8, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * kp, tgt.data(), kp * 8, hipMemcpyHostToDevice, st));
-        Kern<T, LPR>::rescale(P, st, S + S_SCALE * kp, S + S_RATIO * kp, dinv, w.r64.as<double>(), r, pa, p0, p1);
+        Kern<T, LPR>::rescale(P, st, S + S_SCALE * kp, S + S_RATIO * kp, s->dinv64.as<double>(), diag,
+                              w.r64.as<double>(), z, pa, p0, p1);
         k_red_reset<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, p0, p1, S, delta2);
         SFH_LAUNCH_CHECK();
         ++updates;
         // FP32 CG until some column is due for replacement
         for (int it = 1;; ++it) {
-            if (P.grid) {
-                Kern<T, LPR>::dir_spmv_g(P, st, coef, dinv, S + S_BETA * kp, S + S_ACTIVE * kp, r, pa, pb, p0);
-                k_red_alpha<<<P.nred, kRedB, 0, st>>>(kp, nbi, p0, S);
-                Kern<T, LPR>::update_g(P, st, coef, dinv, S + S_ALPHA * kp, pb, x, r, p0, p1);
-            } else {
-                Kern<T, LPR>::dir_spmv(P, st, rp, ci, vaT, dinv, S + S_BETA * kp, S + S_ACTIVE * kp, r, pa, pb, p0);
-                k_red_alpha<<<P.nred, kRedB, 0, st>>>(kp, nbi, p0, S);
-                Kern<T, LPR>::update(P, st, rp, ci, vaT, dinv, S + S_ALPHA * kp, pb, x, r, p0, p1);
-            }
-            k_red_beta<<<P.nred, kRedB, 0, st>>>(kp, nbi, p0, p1, S, w.nact.as<int>());
+            const bool smp = ismp < nsmp && iters % cx->ksample == 0;
+            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp], st));
+            if (P.grid)
+                Kern<T, LPR>::dir_spmv_g(P, st, coef, dinv, S + S_BETA * kp, S + S_ACTIVE * kp, z, pa, pb, acc);
+            else
+                Kern<T, LPR>::dir_spmv(P, st, rp, M, S + S_BETA * kp, S + S_ACTIVE * kp, z, pa, pb, acc);
+            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 1], st));
+            if (P.grid)
+                Kern<T, LPR>::update_g(P, st, coef, dinv, diag, S, pb, x, z, acc);
+            else
+                Kern<T, LPR>::update(P, st, rp, M, dinv, diag, S, pb, x, z, acc);
+            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 2], st));
+            k_red_beta<<<nred, 256, 0, st>>>(kp, acc, S, w.nact.as<int>());
+            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 3], st));
+            ismp += smp;
             std::swap(pa, pb);
             ++iters;
             if (iters >= o.max_inner) break;
             if (it % check == 0) {
                 SFH_LAUNCH_CHECK();
-                SFH_CHECK_HIP(hipMemcpyAsync(cnt.data(), w.nact.p, sizeof(int) * 2 * P.nred, hipMemcpyDeviceToHost, st));
+                SFH_CHECK_HIP(hipMemcpyAsync(cnt.data(), w.nact.p, sizeof(int) * 2 * nred, hipMemcpyDeviceToHost, st));
                 SFH_CHECK_HIP(hipStreamSynchronize(st));
                 int na = 0, nu = 0;
-                for (int b = 0; b < P.nred; ++b) { na += cnt[2 * b]; nu += cnt[2 * b + 1]; }
+                for (int b = 0; b < nred; ++b) { na += cnt[2 * b]; nu += cnt[2 * b + 1]; }
                 if (na == 0 || nu > 0) break;
             }
         }
@@ -857,6 +938,14 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_CHECK_HIP(hipEventSynchronize(s->ctx->ev1));
     float ms = 0.f;
     SFH_CHECK_HIP(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
+    for (int i = 0; i < ismp; ++i) {
+        for (int j = 0; j < 3; ++j) {
+            float f = 0.f;
+            SFH_CHECK_HIP(hipEventElapsedTime(&f, cx->kev[4 * i + j], cx->kev[4 * i + j + 1]));
+            cx->ksum[j] += f;
+        }
+        ++cx->kcount;
+    }
     if (stats) {
         stats->outer = updates;
         stats->inner_iters = iters;
@@ -886,75 +975,7 @@ int dispatch_solve(sfh_system* s, const Plan& P, int trans, int k, const double*
     }
 }
 
-template <class T, int LPR>
-int time_kernels(sfh_system* s, const Plan& P0, int reps, double* ms) {
-    hipStream_t st = s->ctx->stream;
-    SolveWork& w = s->work;
-    Plan P = P0;
-    if (s->dia_state == 1 && s->dim == 2 && s->gn[0] >= 3) plan_grid(P, s->gn[0], s->gn[1]);
-    const int nbi = P.grid ? P.gblk : P.nblk;
-    const int kp = P.kp;
-    double* S = w.scal.as<double>();
-    hipEvent_t e[4];
-    for (auto& x : e) SFH_CHECK_HIP(hipEventCreate(&x));
-    const T* va = (const T*)s->vals32.p;
-    const T* dinv = (const T*)s->dinv32.p;
-    const T* coef = (const T*)s->dia32.p;
-    const int* rp = s->rowptr.as<int>();
-    const int* ci = s->colind.as<int>();
-    SFH_CHECK_HIP(hipEventRecord(e[0], st));
-    for (int q = 0; q < reps; ++q) {
-        if (P.grid)
-            Kern<T, LPR>::dir_spmv_g(P, st, coef, dinv, S + S_BETA * kp, S + S_ACTIVE * kp, w.r32.as<T>(),
-                                     w.p32.as<T>(), w.q32.as<T>(), w.part0.as<double>());
-        else
-            Kern<T, LPR>::dir_spmv(P, st, rp, ci, va, dinv, S + S_BETA * kp, S + S_ACTIVE * kp, w.r32.as<T>(),
-                                   w.p32.as<T>(), w.q32.as<T>(), w.part0.as<double>());
-    }
-    SFH_CHECK_HIP(hipEventRecord(e[1], st));
-    for (int q = 0; q < reps; ++q) {
-        if (P.grid)
-            Kern<T, LPR>::update_g(P, st, coef, dinv, S + S_ALPHA * kp, w.q32.as<T>(), w.x32.as<T>(), w.r32.as<T>(),
-                                   w.part0.as<double>(), w.part1.as<double>());
-        else
-            Kern<T, LPR>::update(P, st, rp, ci, va, dinv, S + S_ALPHA * kp, w.q32.as<T>(), w.x32.as<T>(),
-                                 w.r32.as<T>(), w.part0.as<double>(), w.part1.as<double>());
-    }
-    SFH_CHECK_HIP(hipEventRecord(e[2], st));
-    for (int q = 0; q < reps; ++q) k_red_alpha<<<P.nred, kRedB, 0, st>>>(kp, nbi, w.part0.as<double>(), S);
-    SFH_CHECK_HIP(hipEventRecord(e[3], st));
-    SFH_LAUNCH_CHECK();
-    SFH_CHECK_HIP(hipEventSynchronize(e[3]));
-    for (int i = 0; i < 3; ++i) {
-        float f = 0.f;
-        SFH_CHECK_HIP(hipEventElapsedTime(&f, e[i], e[i + 1]));
-        ms[i] = f / reps;
-    }
-    for (auto& x : e) (void)hipEventDestroy(x);
-    return SFH_OK;
-}
-
 }  // namespace
-
-// Average device time per launch of the inner-PCG kernels (fused direction +
-// SpMV, fused SpMV + update, one reduction) on the workspace of the last k-RHS
-// FP32 solve, whose buffers hold finite values.
-int profile_pcg(sfh_system* s, int k, int reps, double* ms) {
-    const Plan P = make_plan(s->ndof, k, s->ctx->num_cu);
-    if (s->work.x32.bytes < (size_t)P.n * P.kp * sizeof(float) || s->work.part0.bytes < (size_t)P.nblk * P.kp * 8) {
-        set_error("run a %d-RHS FP32 solve before profiling", k);
-        return SFH_ESTATE;
-    }
-    switch (P.lpr) {
-        case 1: return time_kernels<float, 1>(s, P, reps, ms);
-        case 2: return time_kernels<float, 2>(s, P, reps, ms);
-        case 4: return time_kernels<float, 4>(s, P, reps, ms);
-        case 8: return time_kernels<float, 8>(s, P, reps, ms);
-        case 16: return time_kernels<float, 16>(s, P, reps, ms);
-        case 32: return time_kernels<float, 32>(s, P, reps, ms);
-        default: return time_kernels<float, 64>(s, P, reps, ms);
-    }
-}
 
 int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, int ldx,
           const sfh_solve_opts* opts, sfh_solve_stats* stats) {

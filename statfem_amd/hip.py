@@ -78,7 +78,8 @@ _SIGS = {
     "sfh_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "sfh_solve": (i32, [v, i32, i32, v, v, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "sfh_solve_rhs": (i32, [v, v, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
-    "sfh_profile_pcg": (i32, [v, i32, i32, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]),
+    "sfh_kernel_sampling": (i32, [v, i32, i32, i32]),
+    "sfh_kernel_times": (i32, [v, v, C.POINTER(C.c_longlong)]),
     "sfh_covforcing_apply": (i32, [v, v, dbl, dbl, i32, v, v, i32]),
     "sfh_gram": (i32, [v, i32, i32, i32, v, i32, v, i32, v]),
     "sfh_forward_gauss": (i32, [v, i32, i32, v, v, dbl, dbl, dbl, C.POINTER(SolveOpts), v, v, v, v,
@@ -174,6 +175,17 @@ class Context:
         d = DeviceArray(self, a.shape, dtype)
         d.upload(a)
         return d
+
+    def kernel_sampling(self, every=16, max_samples=4096, min_rhs=0):
+        """Sample the device time of the two inner-PCG kernels inside solves of
+        at least min_rhs right-hand sides (per column tile)."""
+        _check(lib().sfh_kernel_sampling(self.h, every, max_samples, min_rhs))
+
+    def kernel_times(self):
+        ms = np.zeros(3)
+        n = C.c_longlong()
+        _check(lib().sfh_kernel_times(self.h, _p(ms), C.byref(n)))
+        return {"dir_spmv": float(ms[0]), "cg_update": float(ms[1]), "beta": float(ms[2]), "samples": n.value}
 
     def timer_start(self):
         _check(lib().sfh_timer_start(self.h))
@@ -325,11 +337,6 @@ class SystemSolveHip:
     def solveSystemTranspose(self, rhs, opts=None):
         X, st = self.solve(np.asarray(rhs).reshape(-1, 1), trans=True, opts=opts)
         return X.download().ravel(), st
-
-    def profile_pcg(self, k, reps=20):
-        a, b, c = C.c_double(), C.c_double(), C.c_double()
-        _check(lib().sfh_profile_pcg(self.h, k, reps, C.byref(a), C.byref(b), C.byref(c)))
-        return {"dir_spmv": a.value, "cg_update": b.value, "reduce": c.value}
 
     def spmv(self, X):
         Xd = X if isinstance(X, DeviceArray) else self.ctx.to_device(np.asarray(X).reshape(self.ndof, -1))
