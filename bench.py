@@ -33,6 +33,22 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` (e.g. "k_dir_spmv<float, 16>") from the
+    committed counter summary (FETCH_SIZE and WRITE_SIZE passes scaled by a
+    copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "pcg_c2_counters.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(kernel + " "):
+            try:
+                return float(line.split()[-1])
+            except ValueError:
+                return None
+    return None
+
+
 def gpu_run(args):
     from statfem_amd import configs, hip
 
@@ -94,6 +110,9 @@ def gpu_run(args):
            for k in algo}
     per["beta"] = {"ms": kt["beta"]}
     dom = max(("dir_spmv", "cg_update"), key=lambda k: kt[k])
+    lpr = min(64, 1 << max(0, (kp // 4 - 1).bit_length()))
+    for k in algo:
+        per[k]["traffic"] = profiled_traffic(f"k_{k}<float, {lpr}>")
     asm_bytes = m.nnodes * 2 * 8 + m.nel * 4 * 4 + m.nel * 8 + nnz * 8 + nnz * 4
     res = {
         "ms_per_step": ms_step, "value": value, "dev_ms_per_step": dev_ms / args.steps,
@@ -256,7 +275,7 @@ def main():
                    "inner_iters": res["inner_iters"], "max_relres": res["max_relres"],
                    "kernels": res["kernels"], "assembly": res["assembly"]},
         "roofline": {"kernel": res["kernel"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": dom.get("traffic")},
         "cpu_baseline": cpu,
         "mcmc": c5,
     }
