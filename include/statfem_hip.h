@@ -66,6 +66,19 @@ int sfh_memcpy_h2d(sfh_ctx* ctx, void* dst_dev, const void* src, size_t bytes);
 int sfh_memcpy_d2h(sfh_ctx* ctx, void* dst, const void* src_dev, size_t bytes);
 int sfh_memcpy_d2d(sfh_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes);
 int sfh_memset(sfh_ctx* ctx, void* dst_dev, int value, size_t bytes);
+/* Load operator of a nodal scalar field acting on one displacement component,
+ * with per-node weights (the lumped form of BodyForce + VectorAssembler,
+ * corlib/BodyForce.hpp:66-71, corlib/VectorAssembler.hpp:101-114), and its
+ * transpose. Blocks are row-major with k columns.
+ * gather:  dst[j][c] = w[j] src[j*stride + comp][c]          (n x k from (n*stride) x k)
+ * scatter: dst[j*stride + q][c] = (q == comp) w[j] src[j][c]  ((n*stride) x k from n x k) */
+int sfh_dof_gather(sfh_ctx* ctx, int n, int k, int stride, int comp, const double* w_dev,
+                   const double* src_dev, double* dst_dev);
+int sfh_dof_scatter(sfh_ctx* ctx, int n, int k, int stride, int comp, const double* w_dev,
+                    const double* src_dev, double* dst_dev);
+/* Y = diag(w) X (n x k, may alias): the lumped-mass weights of the SPDE
+ * covariance products (statX/inferQ/PrecisionMatrixRational.ipp:278-298). */
+int sfh_row_scale(sfh_ctx* ctx, int n, int k, const double* w_dev, const double* X_dev, double* Y_dev);
 /* Elapsed device time between two points on the context stream, in ms. */
 int sfh_timer_start(sfh_ctx* ctx);
 int sfh_timer_stop(sfh_ctx* ctx, double* ms);
@@ -199,6 +212,9 @@ int sfh_lik_destroy(sfh_lik* lik);
 /* giveLogLikelihood (statX/infer/LikelihoodKOH.ipp:74-120) for M hyperparameter
  * sets theta = (rho, mismatch.sigma, mismatch.length), M x 3 host; out, status: M. */
 int sfh_lik_loglik_batch(sfh_lik* lik, int M, const double* theta, double* out, int* status);
+/* log det(K_s + rho^2 Cu) for M hyperparameter sets: the log-determinant term of
+ * giveLogLikelihood (statX/infer/LikelihoodKOH.ipp:93-100). Host arrays. */
+int sfh_lik_logdet_batch(sfh_lik* lik, int M, const double* theta, double* logdet, int* status);
 /* MismatchCovariance::giveCovariance (MismatchCovariance.hpp:76-99), ns x ns host */
 int sfh_lik_mismatch(sfh_lik* lik, double sigma_d, double ell_d, double* Ks);
 /* giveProjFEconditionedData (statX/infer/LikelihoodKOH.ipp:310-361) in the

@@ -92,3 +92,23 @@ def c5_inputs(seed=0, n=256, ns=200, no=8):
     return dict(mesh=m, bc=bc, xs=xs, cols=cols, vals=vals, no=no, sigma_f=0.1, ell_f=0.2, nq1d=2, reg=1e-3,
                 truth=np.array([1.2, 0.05, 0.25]), sigma_e=1e-3, data_seed=seed + 1000,
                 prior_mu=np.log([1.0, 0.1, 0.3]), prior_var=np.array([0.5, 0.5, 0.5]))
+
+
+def c3_inputs(seed=0, n=64, ns=128):
+    """3D linear elasticity: unit cube of n^3 HEX8 cells, E = 1, nu = 0.3, the
+    z = 0 face clamped. Vertical body force f_z = -1 + g with g a Matern field
+    (SPDE alpha = 2, so nu_m = 1/2 in 3D, ell = 0.15, kappa = sqrt(8 nu_m)/ell,
+    sigma_g = 0.1). `ns` sensors of the vertical displacement at seeded points
+    of the top face, noise sigma_e = 1e-4."""
+    m = M.grid3d(n)
+    bottom = M.fuzzy_boundary(m.coords, [2], sides=("lo",))
+    bc = np.sort(np.concatenate([3 * bottom, 3 * bottom + 1, 3 * bottom + 2])).astype(np.int32)
+    rng = np.random.default_rng(seed)
+    xy = rng.uniform(0.0, 1.0, size=(ns, 2))
+    pts = np.concatenate([xy, np.ones((ns, 1))], axis=1)
+    cols, vals = M.sensor_rows(m, pts)
+    nu_m = 2.0 - 1.5
+    ell = 0.15
+    return dict(mesh=m, bc=bc, bottom=bottom, xs=pts, cols=3 * cols + 2, vals=vals, E=1.0, nu=0.3,
+                fbar=-1.0, sigma_g=0.1, ell_g=ell, nu_m=nu_m, kappa=np.sqrt(8.0 * nu_m) / ell, sigma_e=1e-4,
+                nq1d=2, noise_seed=seed + 77)

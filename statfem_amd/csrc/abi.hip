@@ -51,6 +51,29 @@ __global__ void k_mask_zero(int n, const unsigned char* m, double* v) {
     if (i < n && m[i]) v[i] = 0.0;
 }
 
+__global__ void k_dof_gather(int n, int k, int stride, int comp, const double* __restrict__ w,
+                             const double* __restrict__ src, double* __restrict__ dst) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * k) return;
+    const int j = (int)(t / k), c = (int)(t % k);
+    dst[t] = w[j] * src[((size_t)j * stride + comp) * k + c];
+}
+
+__global__ void k_dof_scatter(int n, int k, int stride, int comp, const double* __restrict__ w,
+                              const double* __restrict__ src, double* __restrict__ dst) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * stride * k) return;
+    const long long row = t / k;
+    const int c = (int)(t % k), j = (int)(row / stride), q = (int)(row % stride);
+    dst[t] = q == comp ? w[j] * src[(size_t)j * k + c] : 0.0;
+}
+
+__global__ void k_rows_scale(long long nk, int k, const double* __restrict__ w, const double* __restrict__ X,
+                             double* __restrict__ Y) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nk) Y[t] = w[t / k] * X[t];
+}
+
 __global__ void k_axpy(long long n, double a, const double* x, double* y) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] += a * x[i];
@@ -197,6 +220,33 @@ int sfh_memcpy_d2d(sfh_ctx* c, void* dst, const void* src, size_t bytes) {
 
 int sfh_memset(sfh_ctx* c, void* dst, int v, size_t bytes) {
     SFH_CHECK_HIP(hipMemsetAsync(dst, v, bytes, c->stream));
+    return SFH_OK;
+}
+
+int sfh_dof_gather(sfh_ctx* c, int n, int k, int stride, int comp, const double* w, const double* src, double* dst) {
+    if (!c || n <= 0 || k <= 0 || stride <= 0 || comp < 0 || comp >= stride || !w || !src || !dst) {
+        set_error("bad argument");
+        return SFH_EARG;
+    }
+    k_dof_gather<<<nb((long long)n * k, 256), 256, 0, c->stream>>>(n, k, stride, comp, w, src, dst);
+    SFH_LAUNCH_CHECK();
+    return SFH_OK;
+}
+
+int sfh_dof_scatter(sfh_ctx* c, int n, int k, int stride, int comp, const double* w, const double* src, double* dst) {
+    if (!c || n <= 0 || k <= 0 || stride <= 0 || comp < 0 || comp >= stride || !w || !src || !dst) {
+        set_error("bad argument");
+        return SFH_EARG;
+    }
+    k_dof_scatter<<<nb((long long)n * stride * k, 256), 256, 0, c->stream>>>(n, k, stride, comp, w, src, dst);
+    SFH_LAUNCH_CHECK();
+    return SFH_OK;
+}
+
+int sfh_row_scale(sfh_ctx* c, int n, int k, const double* w, const double* X, double* Y) {
+    if (!c || n <= 0 || k <= 0 || !w || !X || !Y) { set_error("bad argument"); return SFH_EARG; }
+    k_rows_scale<<<nb((long long)n * k, 256), 256, 0, c->stream>>>((long long)n * k, k, w, X, Y);
+    SFH_LAUNCH_CHECK();
     return SFH_OK;
 }
 
@@ -628,6 +678,22 @@ int sfh_lik_post_field(sfh_lik* L, double rho, double sd, double ld, int N, cons
     SFH_TRY(dgemm_shared_a(L->ctx, N, 1, ns, field, ns, B.as<double>() + ns, ns + 1, 0, mean, 1, 0, 1));
     k_axpy<<<nb(N, 256), 256, 0, st>>>(N, 1.0, ubar, mean);
     SFH_LAUNCH_CHECK();
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    return SFH_OK;
+}
+
+int sfh_lik_logdet_batch(sfh_lik* L, int M, const double* theta, double* logdet, int* status) {
+    if (!L || M <= 0 || !theta || !logdet) { set_error("bad argument"); return SFH_EARG; }
+    hipStream_t st = L->ctx->stream;
+    SFH_TRY(L->theta.ensure(sizeof(double) * 3 * M));
+    SFH_TRY(L->out.ensure(sizeof(double) * M));
+    SFH_TRY(L->status.ensure(sizeof(int) * M));
+    DevBuf ld;
+    SFH_TRY(ld.ensure(sizeof(double) * M));
+    SFH_CHECK_HIP(hipMemcpyAsync(L->theta.p, theta, sizeof(double) * 3 * M, hipMemcpyHostToDevice, st));
+    SFH_TRY(loglik_batch(L, M, L->theta.as<double>(), L->out.as<double>(), L->status.as<int>(), ld.as<double>()));
+    SFH_CHECK_HIP(hipMemcpyAsync(logdet, ld.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+    if (status) SFH_CHECK_HIP(hipMemcpyAsync(status, L->status.p, sizeof(int) * M, hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
     return SFH_OK;
 }

@@ -154,7 +154,8 @@ int gram(sfh_ctx* c, int N, int m, int n, const double* X, int ldx, const double
          double* C, DevBuf& work);
 int dgemm_shared_a(sfh_ctx* c, int M, int N, int K, const double* A, int lda, const double* B,
                    int ldb, long long strideB, double* C, int ldc, long long strideC, int batch);
-int loglik_batch(sfh_lik* L, int M, const double* theta_dev, double* out_dev, int* status_dev);
+int loglik_batch(sfh_lik* L, int M, const double* theta_dev, double* out_dev, int* status_dev,
+                 double* logdet_dev = nullptr);
 int dense_spd_solve(sfh_ctx* c, int n, double* A_dev, int m, double* B_dev, int* status_dev);
 int mcmc_run(sfh_lik* L, int C, int steps, const double* theta0, const double* propsd, uint64_t seed,
              int mode, const double* pm, const double* pv, double* samples, double* acc);

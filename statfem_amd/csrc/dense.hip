@@ -267,7 +267,8 @@ __global__ __launch_bounds__(256) void k_loglik(int ns, int no, int dim, const d
                                                 const double* __restrict__ Cu, const double* __restrict__ mean,
                                                 const double* __restrict__ Y, double eps,
                                                 const double* __restrict__ theta, double* __restrict__ work,
-                                                double* __restrict__ out, int* __restrict__ status) {
+                                                double* __restrict__ out, int* __restrict__ status,
+                                                double* __restrict__ logdet_out) {
     extern __shared__ double sh[];
     double* col = sh;               // ns
     double* D = sh + ns;            // ns * no
@@ -295,6 +296,7 @@ __global__ __launch_bounds__(256) void k_loglik(int ns, int no, int dim, const d
     double lsum = 0.0;
     for (int i = tid; i < ns; i += bs) lsum += log(a[(size_t)i * ns + i]);
     const double logdet = block_sum(lsum, red);
+    if (logdet_out && tid == 0) logdet_out[b] = 2.0 * logdet;
     forward_block(ns, a, ns, D, no);
     int bad = 0;
     double tmp3 = 0.0;
@@ -580,14 +582,14 @@ int covforcing_apply(sfh_system* s, const double* cv, double sigma, double ell, 
     return SFH_OK;
 }
 
-int loglik_batch(sfh_lik* L, int M, const double* theta, double* out, int* status) {
+int loglik_batch(sfh_lik* L, int M, const double* theta, double* out, int* status, double* logdet) {
     hipStream_t st = L->ctx->stream;
     const int ns = L->ns, no = L->no;
     SFH_TRY(L->work.ensure((size_t)M * ns * ns * 8));
     const size_t lds = sizeof(double) * ((size_t)ns + (size_t)ns * no + 256);
     if (lds > L->ctx->lds_per_block) { set_error("ns * no too large for LDS"); return SFH_EARG; }
     k_loglik<<<M, 256, lds, st>>>(ns, no, L->dim, L->xs.as<double>(), L->Cu.as<double>(), L->mean.as<double>(),
-                                 L->Y.as<double>(), L->eps, theta, L->work.as<double>(), out, status);
+                                 L->Y.as<double>(), L->eps, theta, L->work.as<double>(), out, status, logdet);
     SFH_LAUNCH_CHECK();
     return SFH_OK;
 }

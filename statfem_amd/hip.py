@@ -59,6 +59,10 @@ _SIGS = {
     "sfh_memcpy_d2h": (i32, [v, v, v, sz]),
     "sfh_memcpy_d2d": (i32, [v, v, v, sz]),
     "sfh_memset": (i32, [v, v, i32, sz]),
+    "sfh_dof_gather": (i32, [v, i32, i32, i32, i32, v, v, v]),
+    "sfh_dof_scatter": (i32, [v, i32, i32, i32, i32, v, v, v]),
+    "sfh_row_scale": (i32, [v, i32, i32, v, v, v]),
+    "sfh_lik_logdet_batch": (i32, [v, i32, v, v, v]),
     "sfh_timer_start": (i32, [v]),
     "sfh_timer_stop": (i32, [v, C.POINTER(dbl)]),
     "sfh_system_create": (i32, [v, i32, i32, i32, v, i32, v, C.POINTER(v)]),
@@ -363,6 +367,33 @@ class SystemSolveHip:
         return V
 
 
+def dof_gather(ctx, w, src, stride, comp, out=None):
+    """out[j, c] = w[j] src[j*stride + comp, c] (device arrays)."""
+    n = w.shape[0]
+    k = int(np.prod(src.shape) // (n * stride))
+    out = out if out is not None else ctx.empty((n, k))
+    _check(lib().sfh_dof_gather(ctx.h, n, k, stride, comp, w.ptr, src.ptr, out.ptr))
+    return out
+
+
+def dof_scatter(ctx, w, src, stride, comp, out=None):
+    """out[j*stride + q, c] = (q == comp) w[j] src[j, c] (device arrays)."""
+    n = w.shape[0]
+    k = int(np.prod(src.shape) // n)
+    out = out if out is not None else ctx.empty((n * stride, k))
+    _check(lib().sfh_dof_scatter(ctx.h, n, k, stride, comp, w.ptr, src.ptr, out.ptr))
+    return out
+
+
+def row_scale(ctx, w, X, out=None):
+    """out = diag(w) X (device arrays; out may be X)."""
+    n = w.shape[0]
+    k = int(np.prod(X.shape) // n)
+    out = out if out is not None else ctx.empty(X.shape)
+    _check(lib().sfh_row_scale(ctx.h, n, k, w.ptr, X.ptr, out.ptr))
+    return out
+
+
 def gram(ctx, X, V):
     N, m = X.shape
     n = V.shape[1]
@@ -433,6 +464,15 @@ class LikelihoodKOHHip:
         out = np.zeros(M)
         st = np.zeros(M, np.int32)
         _check(lib().sfh_lik_loglik_batch(self.h, M, _p(th), _p(out), _p(st)))
+        return out, st
+
+    def giveLogDeterminant(self, theta):
+        """log det(K_s + rho^2 Cu) for (M, 3) natural-scale hyperparameters."""
+        th = _h(np.atleast_2d(theta))
+        M = th.shape[0]
+        out = np.zeros(M)
+        st = np.zeros(M, np.int32)
+        _check(lib().sfh_lik_logdet_batch(self.h, M, _p(th), _p(out), _p(st)))
         return out, st
 
     def mismatch(self, sd, ld):
