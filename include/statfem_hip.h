@@ -161,6 +161,16 @@ void sfh_solve_opts_default(sfh_solve_opts* o);
  * (statX/infer/ForwardGaussRHS.hpp:105-112). B_dev, X_dev: N x k. */
 int sfh_solve(sfh_system* sys, int trans, int k, const double* B_dev, double* X_dev,
               const sfh_solve_opts* opts, sfh_solve_stats* stats);
+/* A batch of independent problems in one solve: the system holds their
+ * block-diagonal union (disjoint meshes, rows seg_rows[q] .. seg_rows[q+1]-1 of
+ * problem q, seg_rows host, nseg + 1 entries), and every (problem, column) pair
+ * runs its own PCG scalars and convergence test in the same kernel launches.
+ * The per-problem form of solveSystemTranspose / solveSystem
+ * (corlib/SystemSolveEigenSparse.hpp:318-404) for many uneven meshes at once.
+ * A problem with fewer right-hand sides leaves its extra columns of B zero. */
+int sfh_solve_batched(sfh_system* sys, int nseg, const int* seg_rows, int trans, int k,
+                      const double* B_dev, double* X_dev, const sfh_solve_opts* opts,
+                      sfh_solve_stats* stats);
 /* solveSystem() on the assembled F (corlib/SystemSolveEigenSparse.hpp:273-311). */
 int sfh_solve_rhs(sfh_system* sys, double* X_dev, const sfh_solve_opts* opts,
                   sfh_solve_stats* stats);
@@ -189,6 +199,16 @@ int sfh_covforcing_apply(sfh_system* sys, const double* covvec_dev, double sigma
 int sfh_gram(sfh_ctx* ctx, int N, int m, int n, const double* X_dev, int ldx, const double* V_dev,
              int ldv, double* C_dev);
 
+/* Per-problem Gram products of a batch: C[q] = X_q^T V_q (m x n each, device,
+ * nseg blocks) over the row ranges seg_rows[q] .. seg_rows[q+1]-1 (host). */
+int sfh_gram_segmented(sfh_ctx* ctx, int nseg, const int* seg_rows, int m, int n, const double* X_dev,
+                       int ldx, const double* V_dev, int ldv, double* C_dev);
+/* The separable CovarianceForcing product (statX/infer/CovarianceForcing.ipp:37-109)
+ * on a tensor grid given by its axes (host, x fastest; ny = nz = 1 for lower
+ * dimensions), independent of any system: V = C_f W, W_dev, V_dev: n x k. */
+int sfh_covforcing_grid(sfh_ctx* ctx, int nx, int ny, int nz, const double* gx, const double* gy,
+                        const double* gz, const double* covvec_dev, double sigma, double ell, int k,
+                        const double* W_dev, double* V_dev);
 /* ForwardGaussRHS::updateSolution (statX/infer/ForwardGaussRHS.hpp:95-139) for a
  * sparse projection P given row-wise (ns rows of pn (col, val) pairs):
  *   R^T = A^-T P^T, mean = R F, Cu = R C_f R^T + (sigma_f reg_eps)^2 P P^T.

@@ -1,0 +1,126 @@
+"""Many independent statFEM problems of uneven size in one system.
+
+The meshes are joined into a disjoint union (block-diagonal operator), assembled
+and constrained once, and the multi-RHS transposed solve of every problem runs
+in the same kernel launches (`sfh_solve_batched`: per-problem, per-column PCG
+scalars and convergence). Each problem's forcing covariance is applied on its
+own tensor grid and its C_u = R C_f R^T comes from a per-problem Gram product,
+so the result per problem is ForwardGaussRHS::updateSolution
+(statX/infer/ForwardGaussRHS.hpp:95-139) on that problem alone."""
+import ctypes as C
+
+import numpy as np
+
+from . import hip
+from . import mesh as M
+
+
+class _View:
+    """Device pointer at an element offset into a DeviceArray."""
+
+    def __init__(self, arr, offset):
+        self.ptr = C.c_void_p(arr.ptr.value + int(offset) * arr.dtype.itemsize)
+
+
+class BatchedForwardGauss:
+    def __init__(self, ctx, problems, nq1d=2):
+        self.ctx = ctx
+        self.probs = problems
+        meshes = [p["mesh"] for p in problems]
+        self.mesh, off = M.union(meshes)
+        self.seg = np.ascontiguousarray(off, dtype=np.int32)
+        self.nseg = len(problems)
+        self.kmax = max(len(p["cols"]) for p in problems)
+        s = hip.SystemSolveHip(ctx, self.mesh)
+        s.assemble(hip.POISSON, 1.0, nq1d=nq1d)
+        s.assemble_rhs(nq1d=nq1d, fconst=1.0)
+        bc = np.concatenate([p["bc"] + off[q] for q, p in enumerate(problems)]).astype(np.int32)
+        s.applyConstraints(bc)
+        self.sys = s
+        cv = s.lumped_mass(nq1d).download()
+        cv[bc] = 0.0
+        self.cv = ctx.to_device(cv)
+        N = self.mesh.nnodes
+        Pt = np.zeros((N, self.kmax))
+        for q, p in enumerate(problems):
+            for j in range(len(p["cols"])):
+                np.add.at(Pt[:, j], p["cols"][j] + off[q], p["vals"][j])
+        self.Pt = ctx.to_device(Pt)
+        self.F = ctx.to_device(s.rhs().reshape(-1, 1))
+        self.PPt = [_sparse_ppt(p["cols"], p["vals"]) for p in problems]
+
+    def updateSolution(self, opts=None):
+        """Prior mean and covariance at each problem's sensors."""
+        ctx, k = self.ctx, self.kmax
+        N = self.mesh.nnodes
+        Rt = ctx.empty((N, k))
+        st = hip.SolveStats()
+        o = opts or hip.default_opts()
+        hip._check(hip.lib().sfh_solve_batched(self.sys.h, self.nseg, hip._p(self.seg), 1, k, self.Pt.ptr, Rt.ptr,
+                                               C.byref(o), C.byref(st)))
+        V = ctx.empty((N, k))
+        for q, p in enumerate(self.probs):
+            r0 = int(self.seg[q])
+            ax = hip._h(p["mesh"].grid[0])
+            ay = hip._h(p["mesh"].grid[1])
+            hip._check(hip.lib().sfh_covforcing_grid(ctx.h, len(ax), len(ay), 1, hip._p(ax), hip._p(ay), None,
+                                                     _View(self.cv, r0).ptr, p["sigma_f"], p["ell_f"], k,
+                                                     _View(Rt, r0 * k).ptr, _View(V, r0 * k).ptr))
+        G = ctx.empty((self.nseg, k, k))
+        hip._check(hip.lib().sfh_gram_segmented(ctx.h, self.nseg, hip._p(self.seg), k, k, Rt.ptr, k, V.ptr, k, G.ptr))
+        Mn = ctx.empty((self.nseg, k, 1))
+        hip._check(hip.lib().sfh_gram_segmented(ctx.h, self.nseg, hip._p(self.seg), k, 1, Rt.ptr, k, self.F.ptr, 1,
+                                                Mn.ptr))
+        G, Mn = G.download(), Mn.download()
+        means, Cus = [], []
+        for q, p in enumerate(self.probs):
+            ns = len(p["cols"])
+            reg = (p["sigma_f"] * p["reg"]) ** 2
+            Cus.append(G[q, :ns, :ns] + reg * self.PPt[q])
+            means.append(Mn[q, :ns, 0].copy())
+        self.Rt = Rt
+        return means, Cus, st.as_dict()
+
+
+def _sparse_ppt(cols, vals):
+    """P P^T of sensor rows given as (cols, vals) of shape (ns, nen)."""
+    ns = cols.shape[0]
+    out = np.zeros((ns, ns))
+    rows = [dict() for _ in range(ns)]
+    for i in range(ns):
+        for c, v in zip(cols[i], vals[i]):
+            rows[i][int(c)] = rows[i].get(int(c), 0.0) + float(v)
+    for i in range(ns):
+        for j in range(i, ns):
+            d = sum(v * rows[j].get(c, 0.0) for c, v in rows[i].items())
+            out[i, j] = out[j, i] = d
+    return out
+
+
+class SequentialForwardGauss:
+    """The same priors one problem at a time: one system, one ForwardGaussRHS
+    update and its own solves per problem."""
+
+    def __init__(self, ctx, problems, nq1d=2):
+        self.probs = problems
+        self.fgs = []
+        for p in problems:
+            s = hip.SystemSolveHip(ctx, p["mesh"])
+            s.assemble(hip.POISSON, 1.0, nq1d=nq1d)
+            s.assemble_rhs(nq1d=nq1d, fconst=1.0)
+            s.applyConstraints(p["bc"])
+            s.set_grid()
+            self.fgs.append(hip.ForwardGaussRHSHip(s, p["cols"], p["vals"], reg_eps=p["reg"]))
+
+    def updateSolution(self, opts=None):
+        means, Cus, iters = [], [], 0
+        for p, fg in zip(self.probs, self.fgs):
+            m, Cu, st = fg.updateSolution(p["sigma_f"], p["ell_f"], opts=opts)
+            means.append(m)
+            Cus.append(Cu)
+            iters += st["inner_iters"]
+        return means, Cus, iters
+
+    def close(self):
+        for fg in self.fgs:
+            fg.sys.close()

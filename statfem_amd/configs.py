@@ -112,3 +112,22 @@ def c3_inputs(seed=0, n=64, ns=128):
     return dict(mesh=m, bc=bc, bottom=bottom, xs=pts, cols=3 * cols + 2, vals=vals, E=1.0, nu=0.3,
                 fbar=-1.0, sigma_g=0.1, ell_g=ell, nu_m=nu_m, kappa=np.sqrt(8.0 * nu_m) / ell, sigma_e=1e-4,
                 nq1d=2, noise_seed=seed + 77)
+
+
+def c4_inputs(seed=0, nprob=64, nmin=32, nmax=256, smin=16, smax=128):
+    """Batched uneven statFEM: `nprob` independent Poisson problems on the unit
+    square with n x n Q1 cells, n ~ U{nmin..nmax}; forcing mean 1 with SE
+    covariance (sigma_f = 0.1, ell_f ~ U(0.05, 0.3)); n_s ~ U{smin..smax}
+    sensors at uniform interior points; homogeneous Dirichlet boundary."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for _ in range(nprob):
+        n = int(rng.integers(nmin, nmax + 1))
+        ell = float(rng.uniform(0.05, 0.3))
+        ns = int(rng.integers(smin, smax + 1))
+        m = M.grid2d(n, kind=M.QUAD4)
+        xs = rng.uniform(0.0, 1.0, size=(ns, 2)) * (1 - 2.0 / n) + 1.0 / n
+        cols, vals = M.sensor_rows(m, xs)
+        probs.append(dict(n=n, mesh=m, bc=M.dirichlet_all_boundary(m), xs=xs, cols=cols, vals=vals,
+                          sigma_f=0.1, ell_f=ell, reg=1e-3, sigma_e=1e-3))
+    return probs

@@ -424,6 +424,12 @@ int sfh_solve(sfh_system* s, int trans, int k, const double* B, double* X, const
     return solve(s, trans, k, B, k, X, k, o, st);
 }
 
+int sfh_solve_batched(sfh_system* s, int nseg, const int* seg_rows, int trans, int k, const double* B, double* X,
+                      const sfh_solve_opts* o, sfh_solve_stats* st) {
+    if (!s || nseg <= 0 || !seg_rows) { set_error("bad argument"); return SFH_EARG; }
+    return solve(s, trans, k, B, k, X, k, o, st, nseg, seg_rows);
+}
+
 int sfh_solve_rhs(sfh_system* s, double* X, const sfh_solve_opts* o, sfh_solve_stats* st) {
     if (!s) { set_error("null system"); return SFH_EARG; }
     return solve(s, 0, 1, s->F.as<double>(), 1, X, 1, o, st);
@@ -468,6 +474,44 @@ int sfh_gram(sfh_ctx* c, int N, int m, int n, const double* X, int ldx, const do
     if (!c || N <= 0 || m <= 0 || n <= 0) { set_error("bad argument"); return SFH_EARG; }
     DevBuf work;
     SFH_TRY(gram(c, N, m, n, X, ldx, V, ldv, C, work));
+    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SFH_OK;
+}
+
+int sfh_gram_segmented(sfh_ctx* c, int nseg, const int* seg_rows, int m, int n, const double* X, int ldx,
+                       const double* V, int ldv, double* C) {
+    if (!c || nseg <= 0 || !seg_rows || m <= 0 || n <= 0 || !X || !V || !C) { set_error("bad argument"); return SFH_EARG; }
+    DevBuf work;
+    for (int q = 0; q < nseg; ++q) {
+        const int r0 = seg_rows[q], nr = seg_rows[q + 1] - r0;
+        double* Cq = C + (size_t)q * m * n;
+        if (nr <= 0) {
+            SFH_CHECK_HIP(hipMemsetAsync(Cq, 0, sizeof(double) * m * n, c->stream));
+            continue;
+        }
+        SFH_TRY(gram(c, nr, m, n, X + (size_t)r0 * ldx, ldx, V + (size_t)r0 * ldv, ldv, Cq, work));
+    }
+    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SFH_OK;
+}
+
+int sfh_covforcing_grid(sfh_ctx* c, int nx, int ny, int nz, const double* gx, const double* gy, const double* gz,
+                        const double* cv, double sigma, double ell, int k, const double* W, double* V) {
+    if (!c || nx <= 0 || ny <= 0 || nz <= 0 || !gx || (ny > 1 && !gy) || (nz > 1 && !gz) || !cv || k <= 0 || !W ||
+        !V || !(ell > 0)) {
+        set_error("bad argument");
+        return SFH_EARG;
+    }
+    const int d = nz > 1 ? 3 : (ny > 1 ? 2 : 1);
+    const int gn[3] = {nx, ny, nz};
+    DevBuf axes, sc0, sc1, sc2;
+    SFH_TRY(axes.ensure(sizeof(double) * ((size_t)nx + ny + nz)));
+    double* a = axes.as<double>();
+    SFH_CHECK_HIP(hipMemcpyAsync(a, gx, sizeof(double) * nx, hipMemcpyHostToDevice, c->stream));
+    if (d > 1) SFH_CHECK_HIP(hipMemcpyAsync(a + nx, gy, sizeof(double) * ny, hipMemcpyHostToDevice, c->stream));
+    if (d > 2) SFH_CHECK_HIP(hipMemcpyAsync(a + nx + ny, gz, sizeof(double) * nz, hipMemcpyHostToDevice, c->stream));
+    const double* ax[3] = {a, a + nx, a + nx + ny};
+    SFH_TRY(covforcing_grid(c, d, gn, ax, cv, sigma, ell, k, W, V, sc0, sc1, sc2));
     SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
     return SFH_OK;
 }

@@ -73,6 +73,7 @@ struct SolveWork {
     DevBuf scal;                     // per-RHS scalars
     DevBuf nact;                     // active / pending column counts
     DevBuf acc;                      // per-column FP64 accumulators of the iteration kernels
+    DevBuf tab;                      // row-partition tables of the current solve
 };
 
 }  // namespace sfh
@@ -146,8 +147,14 @@ int assemble_rhs(sfh_system* s, int nq1d, const double* fconst, const double* fn
                  double factor, int accumulate, double* out);
 int apply_constraints(sfh_system* s, int nc, const int* dofs, const double* values, double factor,
                       double scalar);
+// nseg > 1: a block-diagonal batch of independent problems whose rows are
+// seg_rows[q] .. seg_rows[q+1]-1; every (problem, column) pair runs its own
+// PCG scalars and convergence test
 int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, int ldx,
-          const sfh_solve_opts* o, sfh_solve_stats* st);
+          const sfh_solve_opts* o, sfh_solve_stats* st, int nseg = 1, const int* seg_rows = nullptr);
+int covforcing_grid(sfh_ctx* c, int d, const int gn[3], const double* const ax[3], const double* cv, double sigma,
+                    double ell, int k, const double* W, double* V, sfh::DevBuf& sc0, sfh::DevBuf& sc1,
+                    sfh::DevBuf& sc2);
 int covforcing_apply(sfh_system* s, const double* covvec, double sigma, double ell, int k,
                      const double* W, double* V, int mode);
 int gram(sfh_ctx* c, int N, int m, int n, const double* X, int ldx, const double* V, int ldv,

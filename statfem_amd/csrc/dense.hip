@@ -537,25 +537,34 @@ int covforcing_apply(sfh_system* s, const double* cv, double sigma, double ell, 
         SFH_LAUNCH_CHECK();
         return SFH_OK;
     }
-    // separable: V = sigma^2 D (Kz (x) Ky (x) Kx) D W, axes x fastest
-    const int d = s->dim;
-    const int nx = s->gn[0], ny = d > 1 ? s->gn[1] : 1, nz = d > 2 ? s->gn[2] : 1;
-    const size_t total = (size_t)n * k;
-    SFH_TRY(s->scratch0.ensure(total * 8));
-    SFH_TRY(s->scratch1.ensure(total * 8));
+    const int gn[3] = {s->gn[0], s->dim > 1 ? s->gn[1] : 1, s->dim > 2 ? s->gn[2] : 1};
+    const double* ax[3] = {s->gaxis[0].as<double>(), s->gaxis[1].as<double>(), s->gaxis[2].as<double>()};
+    return covforcing_grid(s->ctx, s->dim, gn, ax, cv, sigma, ell, k, W, V, s->scratch0, s->scratch1, s->scratch2);
+}
+
+// separable: V = sigma^2 D (Kz (x) Ky (x) Kx) D W on a tensor grid, axes x
+// fastest; W, V: (nx ny nz) x k row-major
+int covforcing_grid(sfh_ctx* c, int d, const int gn_[3], const double* const ax[3], const double* cv, double sigma,
+                    double ell, int k, const double* W, double* V, DevBuf& sc0, DevBuf& sc1, DevBuf& sc2) {
+    hipStream_t st = c->stream;
+    const int nx = gn_[0], ny = d > 1 ? gn_[1] : 1, nz = d > 2 ? gn_[2] : 1;
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t total = n * k;
+    SFH_TRY(sc0.ensure(total * 8));
+    SFH_TRY(sc1.ensure(total * 8));
     size_t kb = (size_t)nx * nx + (size_t)ny * ny + (size_t)nz * nz;
-    SFH_TRY(s->scratch2.ensure(kb * 8));
-    double* Kx = s->scratch2.as<double>();
+    SFH_TRY(sc2.ensure(kb * 8));
+    double* Kx = sc2.as<double>();
     double* Ky = Kx + (size_t)nx * nx;
     double* Kz = Ky + (size_t)ny * ny;
     const int gn[3] = {nx, ny, nz};
     double* Ks[3] = {Kx, Ky, Kz};
     for (int a = 0; a < d; ++a) {
-        k_axis_kernel<<<nblocks((long long)gn[a] * gn[a], 256), 256, 0, st>>>(gn[a], s->gaxis[a].as<double>(), ell, Ks[a]);
+        k_axis_kernel<<<nblocks((long long)gn[a] * gn[a], 256), 256, 0, st>>>(gn[a], ax[a], ell, Ks[a]);
         SFH_LAUNCH_CHECK();
     }
-    double* t0 = s->scratch0.as<double>();
-    double* t1 = s->scratch1.as<double>();
+    double* t0 = sc0.as<double>();
+    double* t1 = sc1.as<double>();
     k_row_scale<<<nblocks((long long)total, 256), 256, 0, st>>>((long long)total, k, 1.0, cv, W, t0);
     SFH_LAUNCH_CHECK();
     // x axis: batch over (iy, iz): Kx (nx x nx) * t0[b] (nx x k)
@@ -563,18 +572,18 @@ int covforcing_apply(sfh_system* s, const double* cv, double sigma, double ell, 
         int batch = ny * nz;
         for (int b0 = 0; b0 < batch; b0 += 65535) {
             const int bb = std::min(65535, batch - b0);
-            SFH_TRY(dgemm_shared_a(s->ctx, nx, k, nx, Kx, nx, t0 + (size_t)b0 * nx * k, k, (long long)nx * k,
+            SFH_TRY(dgemm_shared_a(c, nx, k, nx, Kx, nx, t0 + (size_t)b0 * nx * k, k, (long long)nx * k,
                                    t1 + (size_t)b0 * nx * k, k, (long long)nx * k, bb));
         }
         std::swap(t0, t1);
     }
     if (d > 1) {   // y axis: batch over iz: Ky (ny x ny) * t0[iz] (ny x nx k)
         const long long st2 = (long long)ny * nx * k;
-        SFH_TRY(dgemm_shared_a(s->ctx, ny, nx * k, ny, Ky, ny, t0, nx * k, st2, t1, nx * k, st2, nz));
+        SFH_TRY(dgemm_shared_a(c, ny, nx * k, ny, Ky, ny, t0, nx * k, st2, t1, nx * k, st2, nz));
         std::swap(t0, t1);
     }
     if (d > 2) {   // z axis: Kz (nz x nz) * t0 (nz x ny nx k)
-        SFH_TRY(dgemm_shared_a(s->ctx, nz, ny * nx * k, nz, Kz, nz, t0, ny * nx * k, 0, t1, ny * nx * k, 0, 1));
+        SFH_TRY(dgemm_shared_a(c, nz, ny * nx * k, nz, Kz, nz, t0, ny * nx * k, 0, t1, ny * nx * k, 0, 1));
         std::swap(t0, t1);
     }
     k_row_scale<<<nblocks((long long)total, 256), 256, 0, st>>>((long long)total, k, sigma * sigma, cv, t0, V);

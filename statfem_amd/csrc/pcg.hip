@@ -61,19 +61,30 @@ __device__ inline int xcd_block(int b, int nb) {
 }
 
 struct Tile {
-    int lane, grp, ngrp, col0, r0, r1, bid;
+    int lane, grp, ngrp, col0, r0, r1, bid, so;
+};
+
+// Row partition of a (segmented) solve: work-group b (logical id) owns rows
+// [brow[b], brow[b+1]) of segment bseg[b]; a segment is one independent
+// problem of a batch, with its own per-column scalars at offset seg * kp of
+// each scalar slot (slot stride ks = nseg * kp).
+struct Seg {
+    const int* brow;
+    const int* bseg;
+    int ks;
 };
 
 template <int LPR>
-__device__ inline Tile tile(int n, int kp, int rpb) {
+__device__ inline Tile tile(int kp, const Seg& g) {
     Tile t;
     t.lane = threadIdx.x % LPR;
     t.grp = threadIdx.x / LPR;
     t.ngrp = blockDim.x / LPR;
     t.col0 = (blockIdx.y * LPR + t.lane) * 4;
     t.bid = xcd_block(blockIdx.x, gridDim.x);
-    t.r0 = t.bid * rpb;
-    t.r1 = min(n, t.r0 + rpb);
+    t.r0 = g.brow[t.bid];
+    t.r1 = g.brow[t.bid + 1];
+    t.so = g.bseg[t.bid] * kp;
     return t;
 }
 
@@ -124,15 +135,27 @@ __device__ inline void block_accumulate(const Tile& t, int kp, const V4<double>&
 // step length of 4 consecutive columns: alpha = r.z / p.Ap on active columns,
 // from the p.Ap accumulated by the direction kernel
 template <class T>
-__device__ inline V4<T> step_alpha(int kp, int col0, const double* __restrict__ S, const double* __restrict__ acc) {
+__device__ inline V4<T> step_alpha(int ks, int col0, const double* __restrict__ S, const double* __restrict__ acc) {
     double a[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int col = col0 + j;
-        const double pq = acc[A_PQ * kp + col];
-        a[j] = (S[S_ACTIVE * kp + col] > 0.0 && pq > 0.0) ? S[S_RZ * kp + col] / pq : 0.0;
+        const double pq = acc[A_PQ * ks + col];
+        a[j] = (S[S_ACTIVE * ks + col] > 0.0 && pq > 0.0) ? S[S_RZ * ks + col] / pq : 0.0;
     }
     return {(T)a[0], (T)a[1], (T)a[2], (T)a[3]};
+}
+
+// true when none of the columns the work-group handles is still being solved
+// (converged, zero or padding columns have scale 0): the whole work-group then
+// skips the iteration. Frozen columns (scale > 0) keep the work-group live.
+__device__ inline bool block_done(const Tile& t, int kp, int ks, const double* __restrict__ S) {
+    bool live = false;
+    if (t.col0 < kp) {
+        const double* sc = S + S_SCALE * ks + t.so + t.col0;
+        live = sc[0] > 0.0 || sc[1] > 0.0 || sc[2] > 0.0 || sc[3] > 0.0;
+    }
+    return !__syncthreads_or(live);
 }
 
 // row index of the lane group; wave-uniform (scalar) when one row spans the wave
@@ -185,13 +208,15 @@ __device__ inline void res_dots(const V4<T>& z, T g, V4<T>& rz, V4<T>& rr) {
 // recomputed on the fly at the neighbours the stencil touches), q = A p_new,
 // acc += sum_rows p_new . q. q itself is not stored.
 template <class T, int LPR>
-__global__ __launch_bounds__(kB) void k_dir_spmv(int n, int kp, int rpb, const int* __restrict__ rp,
-                                                 const Entries<T> M, const double* __restrict__ beta,
+__global__ __launch_bounds__(kB) void k_dir_spmv(int kp, Seg G, const int* __restrict__ rp,
+                                                 const Entries<T> M, const double* __restrict__ S,
+                                                 const double* __restrict__ beta,
                                                  const double* __restrict__ active, const T* __restrict__ z,
                                                  const T* __restrict__ pold, T* __restrict__ pnew,
                                                  double* __restrict__ acc) {
     __shared__ double red[kB * 4];
-    const Tile t = tile<LPR>(n, kp, rpb);
+    const Tile t = tile<LPR>(kp, G);
+    if (block_done(t, kp, G.ks, S)) return;
     const bool ok = t.col0 < kp;
     V4<T> pq = {0, 0, 0, 0};
     if (ok) {
@@ -199,8 +224,8 @@ __global__ __launch_bounds__(kB) void k_dir_spmv(int n, int kp, int rpb, const i
         // p = 1 * p_old + 0 * z
         V4<T> b, m;
         {
-            const double* bb = beta + t.col0;
-            const double* aa = active + t.col0;
+            const double* bb = beta + t.so + t.col0;
+            const double* aa = active + t.so + t.col0;
             b.x = aa[0] > 0.0 ? (T)bb[0] : (T)1; m.x = aa[0] > 0.0 ? (T)1 : (T)0;
             b.y = aa[1] > 0.0 ? (T)bb[1] : (T)1; m.y = aa[1] > 0.0 ? (T)1 : (T)0;
             b.z = aa[2] > 0.0 ? (T)bb[2] : (T)1; m.z = aa[2] > 0.0 ? (T)1 : (T)0;
@@ -245,23 +270,24 @@ __global__ __launch_bounds__(kB) void k_dir_spmv(int n, int kp, int rpb, const i
         }
     }
     const V4<double> ad = {(double)pq.x, (double)pq.y, (double)pq.z, (double)pq.w};
-    block_accumulate<LPR>(t, kp, ad, red, acc + A_PQ * kp);
+    block_accumulate<LPR>(t, kp, ad, red, acc + A_PQ * G.ks + t.so);
 }
 
 // K2: q = A p (recomputed), x += alpha p, z -= alpha D^-1 q (z = D^-1 r),
 // acc += r.z, r.r with r = diag z
 template <class T, int LPR>
-__global__ __launch_bounds__(kB) void k_cg_update(int n, int kp, int rpb, const int* __restrict__ rp,
+__global__ __launch_bounds__(kB) void k_cg_update(int kp, Seg G, const int* __restrict__ rp,
                                                   const Entries<T> M, const T* __restrict__ dinv,
                                                   const T* __restrict__ diag, const double* __restrict__ S,
                                                   const T* __restrict__ p, T* __restrict__ x, T* __restrict__ z,
                                                   double* __restrict__ acc) {
     __shared__ double red[kB * 4];
-    const Tile t = tile<LPR>(n, kp, rpb);
+    const Tile t = tile<LPR>(kp, G);
+    if (block_done(t, kp, G.ks, S)) return;
     const bool ok = t.col0 < kp;
     V4<T> rz = {0, 0, 0, 0}, rr = {0, 0, 0, 0};
     if (ok) {
-        const V4<T> al = step_alpha<T>(kp, t.col0, S, acc);
+        const V4<T> al = step_alpha<T>(G.ks, t.so + t.col0, S, acc);
         for (int row0 = t.r0 + t.grp; row0 < t.r1; row0 += t.ngrp) {
             const int row = uniform_row<LPR>(row0);
             V4<T> q = {0, 0, 0, 0};
@@ -299,27 +325,29 @@ __global__ __launch_bounds__(kB) void k_cg_update(int n, int kp, int rpb, const 
         }
     }
     V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
-    block_accumulate<LPR>(t, kp, a, red, acc + A_RZ * kp);
+    block_accumulate<LPR>(t, kp, a, red, acc + A_RZ * G.ks + t.so);
     __syncthreads();
     V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
-    block_accumulate<LPR>(t, kp, bb, red, acc + A_RR * kp);
+    block_accumulate<LPR>(t, kp, bb, red, acc + A_RR * G.ks + t.so);
 }
 
 // Residual replacement: z = D^-1 (s' r64), p *= ratio (s'/s), part0 = r.z,
 // part1 = r.r
 template <class T, int LPR>
-__global__ __launch_bounds__(kB) void k_rescale(int n, int kp, int rpb, const double* __restrict__ scale,
+__global__ __launch_bounds__(kB) void k_rescale(int kp, Seg G, const double* __restrict__ scale,
                                                 const double* __restrict__ ratio, const double* __restrict__ dinv64,
                                                 const T* __restrict__ diag, const double* __restrict__ r64,
                                                 T* __restrict__ z, T* __restrict__ p, double* __restrict__ part0,
                                                 double* __restrict__ part1) {
     __shared__ double red[kB * 4];
-    const Tile t = tile<LPR>(n, kp, rpb);
+    const Tile t = tile<LPR>(kp, G);
     const bool ok = t.col0 < kp;
     V4<T> rz = {0, 0, 0, 0}, rr = {0, 0, 0, 0};
     if (ok) {
-        const double s0 = scale[t.col0], s1 = scale[t.col0 + 1], s2 = scale[t.col0 + 2], s3 = scale[t.col0 + 3];
-        const V4<T> q = {(T)ratio[t.col0], (T)ratio[t.col0 + 1], (T)ratio[t.col0 + 2], (T)ratio[t.col0 + 3]};
+        const double* sc = scale + t.so + t.col0;
+        const double* ra = ratio + t.so + t.col0;
+        const double s0 = sc[0], s1 = sc[1], s2 = sc[2], s3 = sc[3];
+        const V4<T> q = {(T)ra[0], (T)ra[1], (T)ra[2], (T)ra[3]};
         for (int row = t.r0 + t.grp; row < t.r1; row += t.ngrp) {
             const size_t o = (size_t)row * kp + t.col0;
             const V4<double> rd = ld4(r64 + o);
@@ -364,6 +392,7 @@ __device__ inline Tile as_tile(const GridTile& g) {
     Tile t;
     t.lane = g.lane; t.grp = g.grp; t.ngrp = blockDim.x / LPR; t.col0 = g.col0; t.bid = g.bid;
     t.r0 = t.r1 = 0;
+    t.so = 0;
     return t;
 }
 
@@ -435,7 +464,7 @@ __global__ __launch_bounds__(kB) void k_cg_update_g2(int W, int H, int kp, int L
     const GridTile g = grid_tile<LPR>();
     V4<T> rz = {0, 0, 0, 0}, rr = {0, 0, 0, 0};
     if (g.col0 < kp && g.seg < nsegx * H) {
-        const V4<T> al = step_alpha<T>(kp, g.col0, S, acc);
+        const V4<T> al = step_alpha<T>(kp, g.col0, S, acc);   // one segment: ks = kp
         const int y = uniform_row<LPR>(g.seg / nsegx);
         const int xs = uniform_row<LPR>((g.seg % nsegx) * L);
         const int xe = min(W, xs + L);
@@ -503,12 +532,12 @@ __global__ void k_build_dia9(int W, int H, const int* __restrict__ rp, const int
 
 // FP64 residual r = b - A x ; part0 = r.r
 template <int LPR>
-__global__ __launch_bounds__(kB) void k_resid64(int n, int kp, int rpb, const int* __restrict__ rp,
+__global__ __launch_bounds__(kB) void k_resid64(int kp, Seg G, const int* __restrict__ rp,
                                                 const int* __restrict__ ci, const double* __restrict__ va,
                                                 const double* __restrict__ x, const double* __restrict__ b,
                                                 double* __restrict__ r, double* __restrict__ part) {
     __shared__ double red[kB * 4];
-    const Tile t = tile<LPR>(n, kp, rpb);
+    const Tile t = tile<LPR>(kp, G);
     const bool ok = t.col0 < kp;
     V4<double> acc = {0, 0, 0, 0};
     if (ok) {
@@ -527,13 +556,20 @@ __global__ __launch_bounds__(kB) void k_resid64(int n, int kp, int rpb, const in
     block_partial<LPR>(t, kp, acc, red, part);
 }
 
-// x64 += x_inner / scale on active columns (scale > 0); x_inner restarts at 0
+// x64 += x_inner / scale on active columns (scale > 0); x_inner restarts at 0.
+// srow: nseg + 1 segment row bounds (the row's segment selects its scales).
 template <class T>
-__global__ void k_refine(long long total, int kp, const double* __restrict__ scale, T* __restrict__ xin,
-                         double* __restrict__ x64) {
+__global__ void k_refine(long long total, int kp, int nseg, const int* __restrict__ srow,
+                         const double* __restrict__ scale, T* __restrict__ xin, double* __restrict__ x64) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    const double s = scale[i % kp];
+    const int row = (int)(i / kp);
+    int lo = 0, hi = nseg - 1;   // last segment with srow[seg] <= row
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (srow[mid] <= row) lo = mid; else hi = mid - 1;
+    }
+    const double s = scale[(size_t)lo * kp + i % kp];
     if (s > 0.0) x64[i] += (double)xin[i] / s;
     xin[i] = (T)0;
 }
@@ -542,12 +578,12 @@ __global__ void k_refine(long long total, int kp, const double* __restrict__ sca
 // Fixed-order sum over the nblk per-block partial rows of one column: 16
 // groups of 64 lanes, each group strides over the rows with 8 independent
 // loads in flight, then a fixed-order combine of the 16 group sums.
-__device__ inline double sum_parts(int nblk, int kp, const double* __restrict__ part, int col, double* red) {
+__device__ inline double sum_parts(int b0, int nblk, int kp, const double* __restrict__ part, int col, double* red) {
     constexpr int G = kRedB / 64;
     const int g = threadIdx.x / 64, l = threadIdx.x % 64;
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (col < kp) {
-        int b = g;
+        int b = b0 + g;
         for (; b + 7 * G < nblk; b += 8 * G) {
 #pragma unroll
             for (int u = 0; u < 8; ++u) a[u] += part[(size_t)(b + u * G) * kp + col];
@@ -598,29 +634,35 @@ __global__ void k_red_beta(int kp, double* __restrict__ acc, double* __restrict_
 
 // after residual replacement: rz from the new residual, beta against the
 // rescaled previous rz (the direction is kept), next trigger level
-__global__ __launch_bounds__(kRedB) void k_red_reset(int kp, int nblk, const double* part0,
-                                                     const double* part1, double* S, double delta2) {
+// (one column group of one segment per work-group: blockIdx.y = segment, whose
+// partial rows are the work-groups sb[seg] .. sb[seg+1]-1; slot stride ks)
+__global__ __launch_bounds__(kRedB) void k_red_reset(int kp, int ks, const int* __restrict__ sb, const double* part0,
+                                                     const double* part1, double* Sg, double delta2) {
     __shared__ double red[kRedB];
-    const int col = blockIdx.x * 64 + threadIdx.x % 64;
-    const double rzn = sum_parts(nblk, kp, part0, col, red);
-    const double rrn = sum_parts(nblk, kp, part1, col, red);
+    const int c = blockIdx.x * 64 + threadIdx.x % 64;
+    const int b0 = sb[blockIdx.y], b1 = sb[blockIdx.y + 1];
+    const double rzn = sum_parts(b0, b1, kp, part0, c, red);
+    const double rrn = sum_parts(b0, b1, kp, part1, c, red);
+    double* S = Sg + (size_t)blockIdx.y * kp;   // this segment's columns; slots keep stride ks
+    const int col = c;
     if (threadIdx.x < 64 && col < kp) {
-        const bool act = S[S_SCALE * kp + col] > 0.0 && rrn > 0.0;
-        const double q = S[S_RATIO * kp + col];
-        const double prev = S[S_RZPREV * kp + col] * q * q;
-        S[S_ACTIVE * kp + col] = act ? 1.0 : 0.0;
-        S[S_BETA * kp + col] = (act && prev > 0.0) ? rzn / prev : 0.0;
-        S[S_RZ * kp + col] = act ? rzn : 0.0;
-        S[S_ALPHA * kp + col] = 0.0;
-        S[S_TRIG * kp + col] = fmax(delta2 * rrn, S[S_TGT * kp + col]);
+        const bool act = S[S_SCALE * ks + col] > 0.0 && rrn > 0.0;
+        const double q = S[S_RATIO * ks + col];
+        const double prev = S[S_RZPREV * ks + col] * q * q;
+        S[S_ACTIVE * ks + col] = act ? 1.0 : 0.0;
+        S[S_BETA * ks + col] = (act && prev > 0.0) ? rzn / prev : 0.0;
+        S[S_RZ * ks + col] = act ? rzn : 0.0;
+        S[S_ALPHA * ks + col] = 0.0;
+        S[S_TRIG * ks + col] = fmax(delta2 * rrn, S[S_TGT * ks + col]);
     }
 }
 
-__global__ __launch_bounds__(kRedB) void k_red_norm(int kp, int nblk, const double* part, double* out) {
+__global__ __launch_bounds__(kRedB) void k_red_norm(int kp, const int* __restrict__ sb, const double* part,
+                                                    double* out) {
     __shared__ double red[kRedB];
     const int col = blockIdx.x * 64 + threadIdx.x % 64;
-    const double s = sum_parts(nblk, kp, part, col, red);
-    if (threadIdx.x < 64 && col < kp) out[col] = s;
+    const double s = sum_parts(sb[blockIdx.y], sb[blockIdx.y + 1], kp, part, col, red);
+    if (threadIdx.x < 64 && col < kp) out[(size_t)blockIdx.y * kp + col] = s;
 }
 
 // ---------------------------------------------------- constraint handling
@@ -701,12 +743,16 @@ inline int pow2_at_least(int v) {
 }
 
 struct Plan {
-    int n, kp, lpr, ntile, rpb, nblk, nred;
+    int n, kp, lpr, ntile, rpb, nblk, nred, nseg, ks;
+    // row partition: work-group b owns rows [brow[b], brow[b+1]) of segment
+    // bseg[b]; segment q's work-groups are sb[q] .. sb[q+1]-1, its rows
+    // srow[q] .. srow[q+1]-1
+    std::vector<int> brow, bseg, sb, srow;
     // 2D tensor-grid path
     int grid, W, H, L, nsegx, gblk;
 };
 
-Plan make_plan(int n, int k, int num_cu) {
+Plan make_plan(int n, int k, int num_cu, int nseg = 1, const int* seg_rows = nullptr) {
     Plan p;
     p.n = n;
     p.kp = (k + 3) / 4 * 4;
@@ -715,9 +761,25 @@ Plan make_plan(int n, int k, int num_cu) {
     const int rows_per_pass = kB / p.lpr;
     int target = std::max(8, 4 * num_cu / p.ntile);
     p.rpb = std::max(rows_per_pass, (n + target - 1) / target);
-    p.nblk = (n + p.rpb - 1) / p.rpb;
-    p.nblk = (p.nblk + 7) / 8 * 8;
-    p.nred = (p.kp + 63) / 64;
+    p.nseg = nseg;
+    p.ks = nseg * p.kp;
+    p.srow.assign(nseg + 1, 0);
+    for (int q = 0; q <= nseg; ++q) p.srow[q] = seg_rows ? seg_rows[q] : (q == 0 ? 0 : n);
+    for (int q = 0; q < nseg; ++q) {
+        p.sb.push_back((int)p.bseg.size());
+        for (int r = p.srow[q]; r < p.srow[q + 1]; r += p.rpb) {
+            p.brow.push_back(r);
+            p.bseg.push_back(q);
+        }
+    }
+    while (p.bseg.size() % 8 || p.bseg.empty()) {   // empty work-groups pad to whole XCD groups
+        p.brow.push_back(n);
+        p.bseg.push_back(nseg - 1);
+    }
+    p.brow.push_back(n);
+    p.sb.push_back((int)p.bseg.size());
+    p.nblk = (int)p.bseg.size();
+    p.nred = (p.ks + 255) / 256;
     p.grid = 0;
     p.W = p.H = p.L = p.nsegx = p.gblk = 0;
     return p;
@@ -737,15 +799,15 @@ void plan_grid(Plan& p, int W, int H) {
 
 template <class T, int LPR>
 struct Kern {
-    static void dir_spmv(const Plan& P, hipStream_t st, const int* rp, const Entries<T>& M, const double* beta,
-                         const double* active, const T* z, const T* pold, T* pnew, double* acc) {
-        k_dir_spmv<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, rp, M, beta, active, z, pold, pnew,
+    static void dir_spmv(const Plan& P, const Seg& G, hipStream_t st, const int* rp, const Entries<T>& M,
+                         const double* S, const double* beta, const double* active, const T* z, const T* pold,
+                         T* pnew, double* acc) {
+        k_dir_spmv<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.kp, G, rp, M, S, beta, active, z, pold, pnew,
                                                                  acc);
     }
-    static void update(const Plan& P, hipStream_t st, const int* rp, const Entries<T>& M, const T* dinv,
-                       const T* diag, const double* S, const T* p, T* x, T* z, double* acc) {
-        k_cg_update<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, rp, M, dinv, diag, S, p, x, z,
-                                                                  acc);
+    static void update(const Plan& P, const Seg& G, hipStream_t st, const int* rp, const Entries<T>& M,
+                       const T* dinv, const T* diag, const double* S, const T* p, T* x, T* z, double* acc) {
+        k_cg_update<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.kp, G, rp, M, dinv, diag, S, p, x, z, acc);
     }
     static void dir_spmv_g(const Plan& P, hipStream_t st, const T* coef, const T* dinv, const double* beta,
                            const double* active, const T* z, const T* pold, T* pnew, double* acc) {
@@ -757,14 +819,13 @@ struct Kern {
         k_cg_update_g2<T, LPR><<<dim3(P.gblk, P.ntile), kB, 0, st>>>(P.W, P.H, P.kp, P.L, P.nsegx, coef, dinv, diag, S,
                                                                      p, x, z, acc);
     }
-    static void rescale(const Plan& P, hipStream_t st, const double* scale, const double* ratio, const double* dinv64,
-                        const T* diag, const double* r64, T* z, T* p, double* p0, double* p1) {
-        k_rescale<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, scale, ratio, dinv64, diag, r64, z, p,
-                                                                p0, p1);
+    static void rescale(const Plan& P, const Seg& G, hipStream_t st, const double* scale, const double* ratio,
+                        const double* dinv64, const T* diag, const double* r64, T* z, T* p, double* p0, double* p1) {
+        k_rescale<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.kp, G, scale, ratio, dinv64, diag, r64, z, p, p0, p1);
     }
-    static void resid(const Plan& P, hipStream_t st, const int* rp, const int* ci, const double* va,
+    static void resid(const Plan& P, const Seg& G, hipStream_t st, const int* rp, const int* ci, const double* va,
                       const double* x, const double* b, double* r, double* part) {
-        k_resid64<LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.n, P.kp, P.rpb, rp, ci, va, x, b, r, part);
+        k_resid64<LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.kp, G, rp, ci, va, x, b, r, part);
     }
 };
 
@@ -781,8 +842,10 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     Plan P = P0;
     const int n = P.n, kp = P.kp;
     const size_t nk = (size_t)n * kp;
-    // 2D tensor grid: stencil layout and sliding-window kernels
-    if (o.layout == 2 && s->gn[0] >= 3 && s->dim == 2 && s->dofpn == 1 && s->gn[1] >= 3 && s->dia_state >= 0) {
+    const int ns = P.nseg, ks = P.ks;
+    // 2D tensor grid: stencil layout and sliding-window kernels (one segment)
+    if (ns == 1 && o.layout == 2 && s->gn[0] >= 3 && s->dim == 2 && s->dofpn == 1 && s->gn[1] >= 3 &&
+        s->dia_state >= 0) {
         if (s->dia_state == 0) {
             SFH_TRY(s->dia32.ensure((size_t)n * 9 * 4));
             SFH_TRY(s->dia64.ensure((size_t)n * 9 * 8));
@@ -810,10 +873,28 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_TRY(w.q32.ensure(nk * sizeof(T)));
     SFH_TRY(w.part0.ensure((size_t)std::max(P.nblk, nbi) * kp * 8));
     SFH_TRY(w.part1.ensure((size_t)std::max(P.nblk, nbi) * kp * 8));
-    SFH_TRY(w.scal.ensure((size_t)S_NSLOT * kp * 8));
-    const int nred = (kp + 255) / 256;
+    SFH_TRY(w.scal.ensure((size_t)S_NSLOT * ks * 8));
+    const int nred = P.nred;
     SFH_TRY(w.nact.ensure(sizeof(int) * 2 * nred));
-    SFH_TRY(w.acc.ensure((size_t)A_N * kp * 8));
+    SFH_TRY(w.acc.ensure((size_t)A_N * ks * 8));
+    // partition tables: brow (nblk + 1), bseg (nblk), sb (nseg + 1), srow (nseg + 1)
+    const size_t ntab = P.brow.size() + P.bseg.size() + P.sb.size() + P.srow.size();
+    SFH_TRY(w.tab.ensure(sizeof(int) * ntab));
+    {
+        std::vector<int> h;
+        h.reserve(ntab);
+        h.insert(h.end(), P.brow.begin(), P.brow.end());
+        h.insert(h.end(), P.bseg.begin(), P.bseg.end());
+        h.insert(h.end(), P.sb.begin(), P.sb.end());
+        h.insert(h.end(), P.srow.begin(), P.srow.end());
+        SFH_CHECK_HIP(hipMemcpy(w.tab.p, h.data(), sizeof(int) * ntab, hipMemcpyHostToDevice));
+    }
+    const int* d_brow = w.tab.as<int>();
+    const int* d_bseg = d_brow + P.brow.size();
+    const int* d_sb = d_bseg + P.bseg.size();
+    const int* d_srow = d_sb + P.sb.size();
+    const Seg G = {d_brow, d_bseg, ks};
+    const dim3 gred((kp + 63) / 64, ns);
     double* S = w.scal.as<double>();
     const T* coef = std::is_same<T, float>::value ? (const T*)s->dia32.p : (const T*)s->dia64.p;
     const int* rp = s->rowptr.as<int>();
@@ -847,17 +928,18 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_CHECK_HIP(hipMemsetAsync(w.x64.p, 0, nk * 8, st));
     SFH_CHECK_HIP(hipMemsetAsync(w.x32.p, 0, nk * sizeof(T), st));
     SFH_CHECK_HIP(hipMemsetAsync(w.p32.p, 0, nk * sizeof(T), st));
-    SFH_CHECK_HIP(hipMemsetAsync(w.scal.p, 0, (size_t)S_NSLOT * kp * 8, st));
-    SFH_CHECK_HIP(hipMemsetAsync(w.acc.p, 0, (size_t)A_N * kp * 8, st));
+    SFH_CHECK_HIP(hipMemsetAsync(w.scal.p, 0, (size_t)S_NSLOT * ks * 8, st));
+    SFH_CHECK_HIP(hipMemsetAsync(w.acc.p, 0, (size_t)A_N * ks * 8, st));
     double* acc = w.acc.as<double>();
     sfh_ctx* cx = s->ctx;
     const int nsmp = (cx->ksample > 0 && k >= cx->kmin_rhs) ? (int)cx->kev.size() / 4 : 0;
     int ismp = 0;
-    Kern<T, LPR>::resid(P, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
-    k_red_norm<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, p0, S + S_BN2 * kp);
+    Kern<T, LPR>::resid(P, G, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
+    k_red_norm<<<gred, kRedB, 0, st>>>(kp, d_sb, p0, S + S_BN2 * ks);
     SFH_LAUNCH_CHECK();
-    std::vector<double> bn2(kp), rn2(kp), scale(kp, 0.0), ratio(kp, 0.0), tgt(kp, 0.0);
-    SFH_CHECK_HIP(hipMemcpyAsync(bn2.data(), S + S_BN2 * kp, kp * 8, hipMemcpyDeviceToHost, st));
+    // per (segment, column) state, index seg * kp + col
+    std::vector<double> bn2(ks), rn2(ks), scale(ks, 0.0), ratio(ks, 0.0), tgt(ks, 0.0);
+    SFH_CHECK_HIP(hipMemcpyAsync(bn2.data(), S + S_BN2 * ks, ks * 8, hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
     rn2 = bn2;
     const double tol2 = o.tol * o.tol;
@@ -871,7 +953,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         // residual replacement: decide per column, rescale, keep directions
         int nact = 0;
         maxrel = 0.0;
-        for (int c = 0; c < kp; ++c) {
+        for (int c = 0; c < ks; ++c) {
             const double rel = bn2[c] > 0.0 ? std::sqrt(rn2[c] / bn2[c]) : 0.0;
             maxrel = std::max(maxrel, rel);
             const bool act = bn2[c] > 0.0 && rn2[c] > tol2 * bn2[c];
@@ -884,12 +966,12 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         }
         if (nact == 0) { done = true; break; }
         if (updates > o.max_outer || iters >= o.max_inner) break;
-        SFH_CHECK_HIP(hipMemcpyAsync(S + S_SCALE * kp, scale.data(), kp * 8, hipMemcpyHostToDevice, st));
-        SFH_CHECK_HIP(hipMemcpyAsync(S + S_RATIO * kp, ratio.data(), kp * 8, hipMemcpyHostToDevice, st));
-        SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * kp, tgt.data(), kp * 8, hipMemcpyHostToDevice, st));
-        Kern<T, LPR>::rescale(P, st, S + S_SCALE * kp, S + S_RATIO * kp, s->dinv64.as<double>(), diag,
+        SFH_CHECK_HIP(hipMemcpyAsync(S + S_SCALE * ks, scale.data(), ks * 8, hipMemcpyHostToDevice, st));
+        SFH_CHECK_HIP(hipMemcpyAsync(S + S_RATIO * ks, ratio.data(), ks * 8, hipMemcpyHostToDevice, st));
+        SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * ks, tgt.data(), ks * 8, hipMemcpyHostToDevice, st));
+        Kern<T, LPR>::rescale(P, G, st, S + S_SCALE * ks, S + S_RATIO * ks, s->dinv64.as<double>(), diag,
                               w.r64.as<double>(), z, pa, p0, p1);
-        k_red_reset<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, p0, p1, S, delta2);
+        k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb, p0, p1, S, delta2);
         SFH_LAUNCH_CHECK();
         ++updates;
         // FP32 CG until some column is due for replacement
@@ -897,16 +979,16 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             const bool smp = ismp < nsmp && iters % cx->ksample == 0;
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp], st));
             if (P.grid)
-                Kern<T, LPR>::dir_spmv_g(P, st, coef, dinv, S + S_BETA * kp, S + S_ACTIVE * kp, z, pa, pb, acc);
+                Kern<T, LPR>::dir_spmv_g(P, st, coef, dinv, S + S_BETA * ks, S + S_ACTIVE * ks, z, pa, pb, acc);
             else
-                Kern<T, LPR>::dir_spmv(P, st, rp, M, S + S_BETA * kp, S + S_ACTIVE * kp, z, pa, pb, acc);
+                Kern<T, LPR>::dir_spmv(P, G, st, rp, M, S, S + S_BETA * ks, S + S_ACTIVE * ks, z, pa, pb, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 1], st));
             if (P.grid)
                 Kern<T, LPR>::update_g(P, st, coef, dinv, diag, S, pb, x, z, acc);
             else
-                Kern<T, LPR>::update(P, st, rp, M, dinv, diag, S, pb, x, z, acc);
+                Kern<T, LPR>::update(P, G, st, rp, M, dinv, diag, S, pb, x, z, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 2], st));
-            k_red_beta<<<nred, 256, 0, st>>>(kp, acc, S, w.nact.as<int>());
+            k_red_beta<<<nred, 256, 0, st>>>(ks, acc, S, w.nact.as<int>());
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 3], st));
             ismp += smp;
             std::swap(pa, pb);
@@ -922,12 +1004,12 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             }
         }
         // flush the FP32 correction, true residual in FP64
-        k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, S + S_SCALE * kp, x,
-                                                                 w.x64.as<double>());
-        Kern<T, LPR>::resid(P, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
-        k_red_norm<<<P.nred, kRedB, 0, st>>>(kp, P.nblk, p0, S + S_RN2 * kp);
+        k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow, S + S_SCALE * ks,
+                                                                 x, w.x64.as<double>());
+        Kern<T, LPR>::resid(P, G, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
+        k_red_norm<<<gred, kRedB, 0, st>>>(kp, d_sb, p0, S + S_RN2 * ks);
         SFH_LAUNCH_CHECK();
-        SFH_CHECK_HIP(hipMemcpyAsync(rn2.data(), S + S_RN2 * kp, kp * 8, hipMemcpyDeviceToHost, st));
+        SFH_CHECK_HIP(hipMemcpyAsync(rn2.data(), S + S_RN2 * ks, ks * 8, hipMemcpyDeviceToHost, st));
         SFH_CHECK_HIP(hipStreamSynchronize(st));
     }
     k_finalize<<<(unsigned)(((size_t)n * k + 255) / 256), 256, 0, st>>>(
@@ -978,9 +1060,15 @@ int dispatch_solve(sfh_system* s, const Plan& P, int trans, int k, const double*
 }  // namespace
 
 int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, int ldx,
-          const sfh_solve_opts* opts, sfh_solve_stats* stats) {
+          const sfh_solve_opts* opts, sfh_solve_stats* stats, int nseg, const int* seg_rows) {
     if (!s->assembled) { set_error("solve before assembly"); return SFH_ESTATE; }
     if (k <= 0 || ldb < k || ldx < k || !B || !X) { set_error("bad solve arguments"); return SFH_EARG; }
+    if (nseg < 1 || (nseg > 1 && !seg_rows)) { set_error("bad segments"); return SFH_EARG; }
+    if (seg_rows) {
+        if (seg_rows[0] != 0 || seg_rows[nseg] != s->ndof) { set_error("segments must cover all rows"); return SFH_EARG; }
+        for (int q = 0; q < nseg; ++q)
+            if (seg_rows[q + 1] < seg_rows[q]) { set_error("segment bounds must not decrease"); return SFH_EARG; }
+    }
     sfh_solve_opts o;
     sfh_solve_opts_default(&o);
     if (opts) o = *opts;
@@ -999,7 +1087,7 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
             sfh_solve_stats st1;
             sfh_solve_opts o1 = o;
             o1.rhs_tile = 0;
-            const int rc = solve(s, trans, kc, B + c0, ldb, X + c0, ldx, &o1, &st1);
+            const int rc = solve(s, trans, kc, B + c0, ldb, X + c0, ldx, &o1, &st1, nseg, seg_rows);
             acc.outer += st1.outer;
             acc.inner_iters += st1.inner_iters;
             acc.max_relres = std::max(acc.max_relres, st1.max_relres);
@@ -1013,7 +1101,7 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
         if (stats) *stats = acc;
         return SFH_OK;
     }
-    const Plan P = make_plan(s->ndof, k, s->ctx->num_cu);
+    const Plan P = make_plan(s->ndof, k, s->ctx->num_cu, nseg, seg_rows);
     if (o.precision == 1) return dispatch_solve<double>(s, P, trans, k, B, ldb, X, ldx, o, stats);
     return dispatch_solve<float>(s, P, trans, k, B, ldb, X, ldx, o, stats);
 }

@@ -59,6 +59,9 @@ _SIGS = {
     "sfh_memcpy_d2h": (i32, [v, v, v, sz]),
     "sfh_memcpy_d2d": (i32, [v, v, v, sz]),
     "sfh_memset": (i32, [v, v, i32, sz]),
+    "sfh_solve_batched": (i32, [v, i32, v, i32, i32, v, v, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "sfh_gram_segmented": (i32, [v, i32, v, i32, i32, v, i32, v, i32, v]),
+    "sfh_covforcing_grid": (i32, [v, i32, i32, i32, v, v, v, v, dbl, dbl, i32, v, v]),
     "sfh_dof_gather": (i32, [v, i32, i32, i32, i32, v, v, v]),
     "sfh_dof_scatter": (i32, [v, i32, i32, i32, i32, v, v, v]),
     "sfh_row_scale": (i32, [v, i32, i32, v, v, v]),
@@ -365,6 +368,22 @@ class SystemSolveHip:
         V = self.ctx.empty((self.ndof, k))
         _check(lib().sfh_covforcing_apply(self.h, cv.ptr, sigma, ell, k, Wd.ptr, V.ptr, mode))
         return V
+
+
+def solve_probe(system, B, X, k, trans=False, opts=None, seg=None):
+    """Run a (possibly batched) solve that may stop at its iteration cap and
+    return its statistics (for timing probes): SFH_ENOCONV is not an error here."""
+    st = SolveStats()
+    o = opts or default_opts()
+    if seg is None:
+        rc = lib().sfh_solve(system.h, int(trans), k, B.ptr, X.ptr, C.byref(o), C.byref(st))
+    else:
+        seg = _h(seg, np.int32)
+        rc = lib().sfh_solve_batched(system.h, len(seg) - 1, _p(seg), int(trans), k, B.ptr, X.ptr, C.byref(o),
+                                     C.byref(st))
+    if rc not in (0, -6):
+        _check(rc)
+    return st.as_dict()
 
 
 def dof_gather(ctx, w, src, stride, comp, out=None):

@@ -190,3 +190,16 @@ def dense_projection(ncols, cols, vals, dofpn=1, comp=0):
     for s in range(ns):
         np.add.at(P[s], cols[s] * dofpn + comp, vals[s])
     return P
+
+
+def union(meshes):
+    """Disjoint union of meshes of one element kind: nodes and elements of mesh q
+    follow those of meshes 0..q-1, so the union's operator is block diagonal.
+    Returns the union mesh and the node offsets (len(meshes) + 1)."""
+    kinds = {m.kind for m in meshes}
+    assert len(kinds) == 1, "union needs one element kind"
+    off = np.zeros(len(meshes) + 1, dtype=np.int64)
+    off[1:] = np.cumsum([m.nnodes for m in meshes])
+    coords = np.concatenate([m.coords for m in meshes], axis=0)
+    conn = np.concatenate([m.conn + off[q] for q, m in enumerate(meshes)], axis=0).astype(np.int32)
+    return Mesh(kinds.pop(), coords, conn), off

@@ -1,0 +1,65 @@
+"""Batched uneven problems: one block-diagonal system, per-problem PCG scalars,
+against the oracle problem by problem."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batched_forward_gauss_matches_oracle_per_problem(ctx, orc):
+    from statfem_amd import configs, hip, mesh as M
+    from statfem_amd.batch import BatchedForwardGauss
+    probs = configs.c4_inputs(seed=5, nprob=5, nmin=4, nmax=14, smin=2, smax=9)
+    bf = BatchedForwardGauss(ctx, probs)
+    means, Cus, st = bf.updateSolution(hip.default_opts(tol=1e-12))
+    for p, mean, Cu in zip(probs, means, Cus):
+        m = p["mesh"]
+        F = orc.body_force(m, nq1d=2, fconst=1.0)
+        rp, ci, va, F = orc.assemble(m, orc.POISSON, 1.0, nq1d=2, bc_dofs=p["bc"], F=F)
+        P = M.dense_projection(m.nnodes, p["cols"], p["vals"])
+        R = orc.LU(rp, ci, va).solve(P.T.copy(), trans=True).T.copy()
+        cv = orc.body_force(m, nq1d=2, fconst=1.0)
+        cv[p["bc"]] = 0.0
+        Cu_ref = orc.covforcing_project(m.coords, cv, p["sigma_f"], p["ell_f"], R) \
+            + (p["sigma_f"] * p["reg"]) ** 2 * P @ P.T
+        mean_ref = R @ F
+        assert np.abs(mean - mean_ref).max() <= 1e-9 * np.abs(mean_ref).max()
+        assert np.abs(Cu - Cu_ref).max() <= 1e-8 * np.abs(Cu_ref).max()
+
+
+def test_batched_solve_forward_and_transposed(ctx, orc):
+    """Per-problem relative residual targets: problems whose right-hand sides
+    differ in scale by 1e6 still each reach the tolerance."""
+    from statfem_amd import configs, hip, mesh as M
+    probs = configs.c4_inputs(seed=6, nprob=4, nmin=3, nmax=11, smin=1, smax=3)
+    mu, off = M.union([p["mesh"] for p in probs])
+    s = hip.SystemSolveHip(ctx, mu)
+    kap = np.exp(np.random.default_rng(2).normal(size=mu.nel))
+    s.assemble(hip.POISSON, kap, nq1d=2)
+    bc = np.concatenate([p["bc"] + off[q] for q, p in enumerate(probs)]).astype(np.int32)
+    s.applyConstraints(bc)
+    rp, ci, va, _ = orc.assemble(mu, orc.POISSON, kap, nq1d=2, bc_dofs=bc)
+    lu = orc.LU(rp, ci, va)
+    rng = np.random.default_rng(3)
+    k = 6
+    B = rng.normal(size=(mu.nnodes, k))
+    for q in range(len(probs)):
+        B[off[q]:off[q + 1]] *= 10.0 ** (2 * q)
+    B[off[1]:off[2], 3:] = 0.0      # a problem with fewer right-hand sides
+    seg = np.ascontiguousarray(off, dtype=np.int32)
+    for trans in (0, 1):
+        X = ctx.empty((mu.nnodes, k))
+        st = hip.SolveStats()
+        o = hip.default_opts(tol=1e-11)
+        Bd = ctx.to_device(B)
+        hip._check(hip.lib().sfh_solve_batched(s.h, len(probs), hip._p(seg), trans, k, Bd.ptr, X.ptr, C.byref(o),
+                                               C.byref(st)))
+        Xh = X.download()
+        ref = lu.solve(B, trans=bool(trans))
+        for q in range(len(probs)):
+            a, b = off[q], off[q + 1]
+            scale = np.abs(ref[a:b]).max()
+            assert np.abs(Xh[a:b] - ref[a:b]).max() <= 1e-9 * scale
+        assert np.all(Xh[off[1]:off[2], 3:] == 0.0)
