@@ -94,6 +94,22 @@ int sfh_timer_stop(sfh_ctx* ctx, double* ms);
 int sfh_system_create(sfh_ctx* ctx, int kind, int dofpn, int nnodes, const double* coords,
                       int nel, const int* conn, sfh_system** out);
 int sfh_system_destroy(sfh_system* sys);
+/* A system from (row, col, value) triplets in insertion order, for callers that
+ * insert element matrices one at a time through the SystemSolve concept
+ * (SystemSolveEigenSparse::insertToMatrix, corlib/SystemSolveEigenSparse.hpp:171-182).
+ * The CSR pattern and values are Triplet::finishAssembly's
+ * (corlib/Triplet.hpp:126-170,264-339): rows in order, columns ascending,
+ * equal keys merged by summation (in insertion order), explicit zeros kept.
+ * Every row must hold a diagonal entry. Host arrays; no mesh, so the
+ * mesh-based assembly and covariance entry points return SFH_ESTATE. */
+int sfh_system_create_triplets(sfh_ctx* ctx, int ndof, long long n, const int* rows, const int* cols,
+                               const double* vals, sfh_system** out);
+/* New values for the same insertion sequence (clearMatrix + re-assembly,
+ * corlib/Triplet.hpp:351-363): merged without sorting again. Clears the
+ * constrained-DOF marks; apply constraints again afterwards. */
+int sfh_system_set_triplet_values(sfh_system* sys, long long n, const double* vals);
+/* The right-hand side F (host, ndof), replacing the assembled one. */
+int sfh_system_set_rhs(sfh_system* sys, const double* F);
 /* numNonZeros (corlib/SystemSolveEigenSparse.hpp:121) and size */
 int sfh_system_info(sfh_system* sys, int* ndof, long long* nnz);
 /* Triplet::setCompressedRowArrays pattern (corlib/Triplet.hpp:264-304), host copies */

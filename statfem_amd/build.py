@@ -50,7 +50,21 @@ def build(verbose=False):
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_cpp_client()
     return LIB
+
+
+CPP_CLIENT = os.path.join(OUT, "system_solve_hip")
+
+
+def build_cpp_client():
+    """The C++ drop-in test client (tests/cpp/system_solve_hip.cpp) against the
+    header-only SystemSolveHip and the library, with the host compiler."""
+    src = os.path.join(HERE, "..", "tests", "cpp", "system_solve_hip.cpp")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(HERE, "..", "include"), src,
+           "-L", HERE, "-lstatfem_hip", "-Wl,-rpath,$ORIGIN/..", "-o", CPP_CLIENT]
+    subprocess.run(cmd, check=True)
+    return CPP_CLIENT
 
 
 if __name__ == "__main__":

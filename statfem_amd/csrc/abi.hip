@@ -327,6 +327,7 @@ int sfh_system_pattern(sfh_system* s, int* rowptr, int* colind) {
 }
 
 int sfh_system_set_grid(sfh_system* s, int nx, int ny, int nz, const double* gx, const double* gy, const double* gz) {
+    if (s && s->kind == 0) { set_error("this system was built from triplets and has no mesh"); return SFH_ESTATE; }
     if (!s) { set_error("null system"); return SFH_EARG; }
     const int d = s->dim;
     const int n[3] = {nx, d > 1 ? ny : 1, d > 2 ? nz : 1};
@@ -358,6 +359,7 @@ int sfh_system_set_grid(sfh_system* s, int nx, int ny, int nz, const double* gx,
 
 int sfh_assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, const double* p2, double s1,
                         double s2, double factor) {
+    if (s && s->kind == 0) { set_error("this system was built from triplets and has no mesh"); return SFH_ESTATE; }
     if (!s) { set_error("null system"); return SFH_EARG; }
     SFH_TRY(assemble_matrix(s, physics, nq1d, p1, p2, s1, s2, factor));
     s->nq1d = nq1d;
@@ -365,11 +367,13 @@ int sfh_assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, 
 }
 
 int sfh_assemble_rhs(sfh_system* s, int nq1d, const double* fconst, const double* fnodal, double factor, int accumulate) {
+    if (s && s->kind == 0) { set_error("this system was built from triplets and has no mesh"); return SFH_ESTATE; }
     if (!s) { set_error("null system"); return SFH_EARG; }
     return assemble_rhs(s, nq1d, fconst, fnodal, factor, accumulate, s->F.as<double>());
 }
 
 int sfh_lumped_mass(sfh_system* s, int nq1d, double* out) {
+    if (s && s->kind == 0) { set_error("this system was built from triplets and has no mesh"); return SFH_ESTATE; }
     if (!s || !out) { set_error("null argument"); return SFH_EARG; }
     const double one[3] = {1.0, 1.0, 1.0};
     return assemble_rhs(s, nq1d, one, nullptr, 1.0, 0, out);
@@ -466,6 +470,7 @@ int sfh_kernel_times(sfh_ctx* c, double* ms, long long* n) {
 // ------------------------------------------------------------ statistics
 int sfh_covforcing_apply(sfh_system* s, const double* cv, double sigma, double ell, int k, const double* W, double* V,
                          int mode) {
+    if (s && s->kind == 0) { set_error("this system was built from triplets and has no mesh"); return SFH_ESTATE; }
     if (!s || !cv || !W || !V || k <= 0) { set_error("bad argument"); return SFH_EARG; }
     return covforcing_apply(s, cv, sigma, ell, k, W, V, mode);
 }
@@ -519,6 +524,7 @@ int sfh_covforcing_grid(sfh_ctx* c, int nx, int ny, int nz, const double* gx, co
 int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const double* pvals, double sigma_f,
                       double ell_f, double reg_eps, const sfh_solve_opts* opts, double* mean, double* Cu,
                       double* field, double* ubar, sfh_solve_stats* stats) {
+    if (s && s->kind == 0) { set_error("this system was built from triplets and has no mesh"); return SFH_ESTATE; }
     if (!s || ns <= 0 || pn <= 0 || !pcols || !pvals) { set_error("bad argument"); return SFH_EARG; }
     if (s->dofpn != 1) { set_error("forward model needs scalar DOFs"); return SFH_EARG; }
     if (!s->assembled) { set_error("forward model before assembly"); return SFH_ESTATE; }

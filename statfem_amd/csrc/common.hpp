@@ -110,6 +110,10 @@ struct sfh_system {
     sfh::DevBuf vals, vals32, F, dinv32, dinv64;
     sfh::DevBuf diag32, diag64;  // A_ii (the PCG keeps z = D^-1 r and needs both)
     sfh::DevBuf cv32;            // FP32 operator as packed (column, value bits) pairs
+    // systems built from triplets: sorted-to-insertion permutation and the
+    // first sorted position of every merged entry (nnz + 1)
+    sfh::DevBuf coo_perm, coo_start;
+    long long ncoo = 0;
     sfh::DevBuf cmask;           // 1 on constrained DOFs
     sfh::DevBuf cdofs;           // constrained DOF list
     int nc = 0;
@@ -152,6 +156,8 @@ int apply_constraints(sfh_system* s, int nc, const int* dofs, const double* valu
 // PCG scalars and convergence test
 int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, int ldx,
           const sfh_solve_opts* o, sfh_solve_stats* st, int nseg = 1, const int* seg_rows = nullptr);
+int triplets_build(sfh_system* s, long long n, const int* rows, const int* cols, const double* vals);
+int triplets_values(sfh_system* s, long long n, const double* vals);
 int covforcing_grid(sfh_ctx* c, int d, const int gn[3], const double* const ax[3], const double* cv, double sigma,
                     double ell, int k, const double* W, double* V, sfh::DevBuf& sc0, sfh::DevBuf& sc1,
                     sfh::DevBuf& sc2);

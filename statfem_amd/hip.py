@@ -59,6 +59,9 @@ _SIGS = {
     "sfh_memcpy_d2h": (i32, [v, v, v, sz]),
     "sfh_memcpy_d2d": (i32, [v, v, v, sz]),
     "sfh_memset": (i32, [v, v, i32, sz]),
+    "sfh_system_create_triplets": (i32, [v, i32, C.c_longlong, v, v, v, C.POINTER(v)]),
+    "sfh_system_set_triplet_values": (i32, [v, C.c_longlong, v]),
+    "sfh_system_set_rhs": (i32, [v, v]),
     "sfh_solve_batched": (i32, [v, i32, v, i32, i32, v, v, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "sfh_gram_segmented": (i32, [v, i32, v, i32, i32, v, i32, v, i32, v]),
     "sfh_covforcing_grid": (i32, [v, i32, i32, i32, v, v, v, v, dbl, dbl, i32, v, v]),

@@ -33,3 +33,15 @@ def test_status_strings_and_version():
 def test_python_mirror_binds_every_symbol():
     from statfem_amd import hip
     assert set(_declared()) == set(hip._SIGS)
+
+
+def test_cpp_dropin_header_compiles_and_client_is_built():
+    """include/statfem/SystemSolveHip.hpp compiles standalone with the host
+    compiler, and the C++ client used by the GPU test is built and linked."""
+    import subprocess
+    hdr = os.path.join(ROOT, "include", "statfem", "SystemSolveHip.hpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
+                        "-x", "c++", hdr], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    from statfem_amd import build
+    assert os.path.exists(build.CPP_CLIENT)
