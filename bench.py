@@ -101,24 +101,28 @@ def gpu_run(args):
     kk = args.tile if 0 < args.tile < ns else ns
     N, nnz = s.ndof, s.nnz
     kp = (kk + 3) // 4 * 4
-    mat = 8.0 * nnz + 4.0 * (N + 1) + 4.0 * N          # FP32 CSR values + columns + row pointers + D^-1
-    algo = {
-        "dir_spmv": mat + 12.0 * N * kp,                 # r, p_old read; p_new write
-        "cg_update": mat + 20.0 * N * kp,                # p_new, x, r read; x, r write
-    }
+    vec = 4.0 * N * kp                                    # one FP32 N x kp block
+    if stats[-1]["layout"] == 2:
+        # 9 FP32 stencil coefficients per node (+ D^-1 and diag for the update)
+        algo = {"dir_spmv": 36.0 * N + 3 * vec, "cg_update": 44.0 * N + 5 * vec}
+        kname = {"dir_spmv": "k_dir_spmv_t2<{lpr}>", "cg_update": "k_cg_update_t2<{lpr}>"}
+    else:
+        mat = 8.0 * nnz + 4.0 * (N + 1)                   # packed (column, value) entries + row pointers
+        algo = {"dir_spmv": mat + 3 * vec, "cg_update": mat + 8.0 * N + 5 * vec}
+        kname = {"dir_spmv": "k_dir_spmv<float, {lpr}>", "cg_update": "k_cg_update<float, {lpr}>"}
     per = {k: {"ms": kt[k], "GBs": algo[k] / (kt[k] * 1e-3) / 1e9, "bytes": algo[k], "samples": kt["samples"]}
            for k in algo}
     per["beta"] = {"ms": kt["beta"]}
     dom = max(("dir_spmv", "cg_update"), key=lambda k: kt[k])
     lpr = min(64, 1 << max(0, (kp // 4 - 1).bit_length()))
     for k in algo:
-        per[k]["traffic"] = profiled_traffic(f"k_{k}<float, {lpr}>")
+        per[k]["traffic"] = profiled_traffic(kname[k].format(lpr=lpr))
     asm_bytes = m.nnodes * 2 * 8 + m.nel * 4 * 4 + m.nel * 8 + nnz * 8 + nnz * 4
     res = {
         "ms_per_step": ms_step, "value": value, "dev_ms_per_step": dev_ms / args.steps,
         "inner_iters": stats[-1]["inner_iters"], "outer": stats[-1]["outer"],
         "max_relres": max(st["max_relres"] for st in stats),
-        "kernel": dom, "kernels": per,
+        "kernel": dom, "kernels": per, "layout": {1: "csr", 2: "stencil9"}.get(stats[-1]["layout"], "?"),
         "assembly": {"ms": asm_ms, "GBs": asm_bytes / (asm_ms * 1e-3) / 1e9, "bytes": asm_bytes},
         "N": N, "nnz": nnz,
     }
@@ -360,7 +364,7 @@ def main():
         "config": {"workload": f"Poisson {args.n}x{args.n} Q1, {res['N']} DOFs, exp(GP) diffusivity, "
                                f"{args.ns} sensors/RHS: assembly + A^-T P^T + C_f + Gram + A^-1 C_f R^T",
                    "dofs": res["N"], "rhs": args.ns, "nnz": res["nnz"], "tol": args.tol, "rhs_tile": args.tile,
-                   "inner_iters": res["inner_iters"], "max_relres": res["max_relres"],
+                   "inner_iters": res["inner_iters"], "max_relres": res["max_relres"], "operator_layout": res["layout"],
                    "kernels": res["kernels"], "assembly": res["assembly"]},
         "roofline": {"kernel": res["kernel"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": dom.get("traffic")},

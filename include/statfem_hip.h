@@ -158,7 +158,9 @@ typedef struct {
     int check_every;     /* inner iterations between host convergence checks (default 32) */
     int precision;       /* 0: FP32 recurrences + FP64 replacement; 1: FP64 throughout */
     int rhs_tile;        /* >0: solve the right-hand sides in column tiles of this width (0: all at once) */
-    int layout;          /* 0: auto, 1: CSR, 2: stencil layout on declared 2D tensor grids */
+    int layout;          /* 0 (default): auto, 1: CSR, 2: 9-coefficient stencil with LDS-tiled kernels on
+                            declared 2D tensor grids (FP32, at most 64 right-hand sides per tile);
+                            auto uses 2 where it applies, else CSR */
 } sfh_solve_opts;
 
 typedef struct {
@@ -167,6 +169,7 @@ typedef struct {
     double max_relres;    /* max over RHS of ||b - A x|| / ||b|| in FP64 */
     double ms;            /* device time */
     int converged;
+    int layout;           /* operator layout the iteration kernels used: 1 CSR, 2 2D stencil */
 } sfh_solve_stats;
 
 void sfh_solve_opts_default(sfh_solve_opts* o);
@@ -184,6 +187,11 @@ int sfh_solve(sfh_system* sys, int trans, int k, const double* B_dev, double* X_
  * The per-problem form of solveSystemTranspose / solveSystem
  * (corlib/SystemSolveEigenSparse.hpp:318-404) for many uneven meshes at once.
  * A problem with fewer right-hand sides leaves its extra columns of B zero. */
+/* Declares that the batch's problems are 2D tensor grids with nodes numbered x
+ * fastest: problem q has nx[q] x ny[q] nodes on rows seg_rows[q] ..
+ * seg_rows[q+1]-1 (host arrays), so the batched solve can use the stencil
+ * layout (sfh_solve_opts.layout) on every problem. */
+int sfh_system_set_segment_grids(sfh_system* sys, int nseg, const int* seg_rows, const int* nx, const int* ny);
 int sfh_solve_batched(sfh_system* sys, int nseg, const int* seg_rows, int trans, int k,
                       const double* B_dev, double* X_dev, const sfh_solve_opts* opts,
                       sfh_solve_stats* stats);

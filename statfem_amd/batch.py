@@ -36,6 +36,9 @@ class BatchedForwardGauss:
         s.assemble_rhs(nq1d=nq1d, fconst=1.0)
         bc = np.concatenate([p["bc"] + off[q] for q, p in enumerate(problems)]).astype(np.int32)
         s.applyConstraints(bc)
+        nx = np.array([len(p["mesh"].grid[0]) for p in problems], np.int32)
+        ny = np.array([len(p["mesh"].grid[1]) for p in problems], np.int32)
+        hip._check(hip.lib().sfh_system_set_segment_grids(s.h, self.nseg, hip._p(self.seg), hip._p(nx), hip._p(ny)))
         self.sys = s
         cv = s.lumped_mass(nq1d).download()
         cv[bc] = 0.0

@@ -428,6 +428,25 @@ int sfh_solve(sfh_system* s, int trans, int k, const double* B, double* X, const
     return solve(s, trans, k, B, k, X, k, o, st);
 }
 
+int sfh_system_set_segment_grids(sfh_system* s, int nseg, const int* seg_rows, const int* nx, const int* ny) {
+    if (!s || nseg <= 0 || !seg_rows || !nx || !ny) { set_error("bad argument"); return SFH_EARG; }
+    if (s->dim != 2 || s->dofpn != 1) { set_error("segment grids need a scalar 2D system"); return SFH_EARG; }
+    if (seg_rows[0] != 0 || seg_rows[nseg] != s->ndof) { set_error("segments must cover all rows"); return SFH_EARG; }
+    std::vector<int> g(3 * (size_t)nseg);
+    for (int q = 0; q < nseg; ++q) {
+        if (nx[q] < 1 || ny[q] < 1 || (long long)nx[q] * ny[q] != seg_rows[q + 1] - seg_rows[q]) {
+            set_error("grid %d (%d x %d) does not match its %d rows", q, nx[q], ny[q], seg_rows[q + 1] - seg_rows[q]);
+            return SFH_EARG;
+        }
+        g[3 * q] = seg_rows[q];
+        g[3 * q + 1] = nx[q];
+        g[3 * q + 2] = ny[q];
+    }
+    s->sgrid = g;
+    if (s->dia_state != 0) s->dia_state = 0;
+    return SFH_OK;
+}
+
 int sfh_solve_batched(sfh_system* s, int nseg, const int* seg_rows, int trans, int k, const double* B, double* X,
                       const sfh_solve_opts* o, sfh_solve_stats* st) {
     if (!s || nseg <= 0 || !seg_rows) { set_error("bad argument"); return SFH_EARG; }
@@ -602,6 +621,7 @@ int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const dou
         stats->max_relres = std::max(s1.max_relres, std::max(s2.max_relres, s3.max_relres));
         stats->ms = ms;
         stats->converged = 1;
+        stats->layout = s1.layout;
     }
     return SFH_OK;
 }

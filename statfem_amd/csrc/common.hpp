@@ -127,6 +127,7 @@ struct sfh_system {
     // node (x,y) couples (x + k%3 - 1, y + k/3 - 1); built lazily per assembly
     sfh::DevBuf dia32, dia64;
     int dia_state = 0;           // 0 stale, 1 valid, -1 mesh does not fit the stencil
+    std::vector<int> sgrid;      // batch of 2D grids: (first row, W, H) per problem
     sfh::SolveWork work;
     sfh::DevBuf scratch0, scratch1, scratch2, scratch3;
     sfh::DevBuf fg_pt, fg_rt, fg_v, fg_cv, fg_cu, fg_gram, fg_cols, fg_vals;

@@ -368,162 +368,180 @@ __global__ __launch_bounds__(kB) void k_rescale(int kp, Seg G, const double* __r
 }
 
 // ------------------------------------------------ 2D tensor-grid (DIA-9) path
-// Nodes are numbered x fastest on a W x H grid. A lane group walks a segment of
-// one grid row with a 3 x 3 register window: each step loads the three nodes
-// of column x+1 (rows y-1, y, y+1) instead of gathering all nine neighbours.
-struct GridTile {
-    int lane, grp, col0, bid, seg;
-};
+// Nodes are numbered x fastest on a W x H grid and the operator is stored as 9
+// coefficients per node (coefficient (dy+1)*3 + (dx+1) couples (x+dx, y+dy)).
+// A work-group owns a kTX x kTY tile of nodes: it stages the tile plus a
+// one-node halo of the gathered vector in LDS with coalesced 16-byte loads
+// (each node row of kp columns is contiguous), then forms the 9-point products
+// from LDS. Every vector element is read from memory about (kTX+2)(kTY+2) /
+// (kTX kTY) times instead of once per stencil neighbour, so the iteration is
+// bound by streaming bandwidth rather than by gather instructions. Column tiles
+// of up to 64 right-hand sides (LPR <= 16 lanes of 4 columns per node), FP32.
+constexpr int kTX = 32, kTY = 6, kG = 512;
+constexpr int kHN = (kTX + 2) * (kTY + 2);   // staged nodes per tile
 
-template <int LPR>
-__device__ inline GridTile grid_tile() {
-    GridTile g;
-    g.lane = threadIdx.x % LPR;
-    g.grp = threadIdx.x / LPR;
-    g.col0 = (blockIdx.y * LPR + g.lane) * 4;
-    g.bid = xcd_block(blockIdx.x, gridDim.x);
-    g.seg = g.bid * (blockDim.x / LPR) + g.grp;
-    return g;
+inline size_t grid_lds_bytes(int lpr) {
+    const size_t tile = (size_t)kHN * lpr * sizeof(float4);
+    const size_t red = (size_t)kG * 4 * sizeof(double);   // block_accumulate after the tile is consumed
+    return tile > red ? tile : red;
 }
 
-// Tile-compatible view for block_partial
+// Work-group tile table, 6 ints per (logical) work-group: first node of the
+// grid's rows, grid width W and height H (W = 0: idle padding work-group),
+// tile origin x0, y0, and the segment (problem) whose per-column scalars apply.
+constexpr int kGT = 6;
+
 template <int LPR>
-__device__ inline Tile as_tile(const GridTile& g) {
+__device__ inline Tile grid_tile_t(int kp, const int* __restrict__ gtab, int* d) {
     Tile t;
-    t.lane = g.lane; t.grp = g.grp; t.ngrp = blockDim.x / LPR; t.col0 = g.col0; t.bid = g.bid;
+    t.lane = threadIdx.x % LPR;
+    t.grp = threadIdx.x / LPR;
+    t.ngrp = blockDim.x / LPR;
+    t.col0 = t.lane * 4;
+    t.bid = xcd_block(blockIdx.x, gridDim.x);
+#pragma unroll
+    for (int i = 0; i < kGT; ++i) d[i] = gtab[kGT * t.bid + i];
     t.r0 = t.r1 = 0;
-    t.so = 0;
+    t.so = d[5] * kp;
     return t;
 }
 
-template <class T, int LPR>
-__global__ __launch_bounds__(kB) void k_dir_spmv_g2(int W, int H, int kp, int L, int nsegx, const T* __restrict__ coef,
-                                                    const T* __restrict__ dinv, const double* __restrict__ beta,
-                                                    const double* __restrict__ active, const T* __restrict__ z,
-                                                    const T* __restrict__ pold, T* __restrict__ pnew,
-                                                    double* __restrict__ acc) {
-    __shared__ double red[kB * 4];
-    const GridTile g = grid_tile<LPR>();
-    V4<T> pq = {0, 0, 0, 0};
-    if (g.col0 < kp && g.seg < nsegx * H) {
-        V4<T> b, m;
-        {
-            const double* bb = beta + g.col0;
-            const double* aa = active + g.col0;
-            b.x = aa[0] > 0.0 ? (T)bb[0] : (T)1; m.x = aa[0] > 0.0 ? (T)1 : (T)0;
-            b.y = aa[1] > 0.0 ? (T)bb[1] : (T)1; m.y = aa[1] > 0.0 ? (T)1 : (T)0;
-            b.z = aa[2] > 0.0 ? (T)bb[2] : (T)1; m.z = aa[2] > 0.0 ? (T)1 : (T)0;
-            b.w = aa[3] > 0.0 ? (T)bb[3] : (T)1; m.w = aa[3] > 0.0 ? (T)1 : (T)0;
-        }
-        const int y = uniform_row<LPR>(g.seg / nsegx);
-        const int xs = uniform_row<LPR>((g.seg % nsegx) * L);
-        const int xe = min(W, xs + L);
-        auto pval = [&](int x, int yy) -> V4<T> {
-            V4<T> v = {0, 0, 0, 0};
-            if (x >= 0 && x < W && yy >= 0 && yy < H) {
-                const int nd = yy * W + x;
-                const size_t o = (size_t)nd * kp + g.col0;
-                pdir4(ld4(z + o), m, b, ld4(pold + o), v);
-            }
-            return v;
-        };
-        V4<T> w0[3], w1[3], w2[3];   // columns x-1, x, x+1; index = dy + 1
+__device__ inline float4 f4(const V4<float>& v) { return make_float4(v.x, v.y, v.z, v.w); }
+__device__ inline V4<float> v4(const float4& v) { return {v.x, v.y, v.z, v.w}; }
+
+// q = 9-point product at tile node (ox, oy) from the staged values
+template <int LPR>
+__device__ inline V4<float> stencil9(const float4* sm, const float* __restrict__ c, int ox, int oy, int lane) {
+    V4<float> q = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int d = 0; d < 3; ++d) { w0[d] = pval(xs - 1, y + d - 1); w1[d] = pval(xs, y + d - 1); }
-        for (int x = xs; x < xe; ++x) {
+    for (int dy = 0; dy < 3; ++dy) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) w2[d] = pval(x + 1, y + d - 1);
-            const int nd = y * W + x;
-            const T* c = coef + (size_t)nd * 9;
-            V4<T> q = {0, 0, 0, 0};
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const T c0 = c[d * 3], c1 = c[d * 3 + 1], c2 = c[d * 3 + 2];
-                q.x += c0 * w0[d].x + c1 * w1[d].x + c2 * w2[d].x;
-                q.y += c0 * w0[d].y + c1 * w1[d].y + c2 * w2[d].y;
-                q.z += c0 * w0[d].z + c1 * w1[d].z + c2 * w2[d].z;
-                q.w += c0 * w0[d].w + c1 * w1[d].w + c2 * w2[d].w;
-            }
-            const V4<T> pi = w1[1];
-            st4(pnew + (size_t)nd * kp + g.col0, pi);
-            pq.x += pi.x * q.x; pq.y += pi.y * q.y; pq.z += pi.z * q.z; pq.w += pi.w * q.w;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) { w0[d] = w1[d]; w1[d] = w2[d]; }
+        for (int dx = 0; dx < 3; ++dx) {
+            const float a = c[dy * 3 + dx];
+            const float4 v = sm[((oy + dy) * (kTX + 2) + (ox + dx)) * LPR + lane];
+            q.x += a * v.x; q.y += a * v.y; q.z += a * v.z; q.w += a * v.w;
         }
     }
-    const V4<double> ad = {(double)pq.x, (double)pq.y, (double)pq.z, (double)pq.w};
-    block_accumulate<LPR>(as_tile<LPR>(g), kp, ad, red, acc + A_PQ * kp);
+    return q;
 }
 
-template <class T, int LPR>
-__global__ __launch_bounds__(kB) void k_cg_update_g2(int W, int H, int kp, int L, int nsegx, const T* __restrict__ coef,
-                                                     const T* __restrict__ dinv, const T* __restrict__ diag, const double* __restrict__ S,
-                                                     const T* __restrict__ p, T* __restrict__ x, T* __restrict__ z,
-                                                     double* __restrict__ acc) {
-    __shared__ double red[kB * 4];
-    const GridTile g = grid_tile<LPR>();
-    V4<T> rz = {0, 0, 0, 0}, rr = {0, 0, 0, 0};
-    if (g.col0 < kp && g.seg < nsegx * H) {
-        const V4<T> al = step_alpha<T>(kp, g.col0, S, acc);   // one segment: ks = kp
-        const int y = uniform_row<LPR>(g.seg / nsegx);
-        const int xs = uniform_row<LPR>((g.seg % nsegx) * L);
-        const int xe = min(W, xs + L);
-        auto pv = [&](int xx, int yy) -> V4<T> {
-            V4<T> v = {0, 0, 0, 0};
-            if (xx >= 0 && xx < W && yy >= 0 && yy < H) v = ld4(p + (size_t)(yy * W + xx) * kp + g.col0);
-            return v;
-        };
-        V4<T> w0[3], w1[3], w2[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) { w0[d] = pv(xs - 1, y + d - 1); w1[d] = pv(xs, y + d - 1); }
-        for (int xx = xs; xx < xe; ++xx) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) w2[d] = pv(xx + 1, y + d - 1);
-            const int nd = y * W + xx;
-            const T* c = coef + (size_t)nd * 9;
-            V4<T> q = {0, 0, 0, 0};
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const T c0 = c[d * 3], c1 = c[d * 3 + 1], c2 = c[d * 3 + 2];
-                q.x += c0 * w0[d].x + c1 * w1[d].x + c2 * w2[d].x;
-                q.y += c0 * w0[d].y + c1 * w1[d].y + c2 * w2[d].y;
-                q.z += c0 * w0[d].z + c1 * w1[d].z + c2 * w2[d].z;
-                q.w += c0 * w0[d].w + c1 * w1[d].w + c2 * w2[d].w;
-            }
-            const size_t o = (size_t)nd * kp + g.col0;
-            const V4<T> pi = w1[1];
-            V4<T> xv = ld4(x + o), zv = ld4(z + o);
-            const T dd = dinv[nd], g = diag[nd];
-            xv.x += al.x * pi.x; xv.y += al.y * pi.y; xv.z += al.z * pi.z; xv.w += al.w * pi.w;
-            zv.x -= al.x * dd * q.x; zv.y -= al.y * dd * q.y; zv.z -= al.z * dd * q.z; zv.w -= al.w * dd * q.w;
-            st4(x + o, xv);
-            st4(z + o, zv);
-            res_dots(zv, g, rz, rr);
-#pragma unroll
-            for (int d2 = 0; d2 < 3; ++d2) { w0[d2] = w1[d2]; w1[d2] = w2[d2]; }
+template <int LPR>
+__global__ __launch_bounds__(kG) void k_dir_spmv_t2(const int* __restrict__ gtab, int kp, int ks,
+                                                    const float* __restrict__ coef, const double* __restrict__ S,
+                                                    const double* __restrict__ beta, const double* __restrict__ active,
+                                                    const float* __restrict__ z, const float* __restrict__ pold,
+                                                    float* __restrict__ pnew, double* __restrict__ acc) {
+    extern __shared__ float4 sm[];
+    int d[kGT];
+    const Tile t = grid_tile_t<LPR>(kp, gtab, d);
+    const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
+    if (W == 0) return;   // padding work-groups (whole group)
+    if (block_done(t, kp, ks, S)) return;
+    const bool ok = t.col0 < kp;
+    V4<float> b = {1.f, 1.f, 1.f, 1.f}, m = {0.f, 0.f, 0.f, 0.f};
+    if (ok) {
+        const double* bb = beta + t.so + t.col0;
+        const double* aa = active + t.so + t.col0;
+        b.x = aa[0] > 0.0 ? (float)bb[0] : 1.f; m.x = aa[0] > 0.0 ? 1.f : 0.f;
+        b.y = aa[1] > 0.0 ? (float)bb[1] : 1.f; m.y = aa[1] > 0.0 ? 1.f : 0.f;
+        b.z = aa[2] > 0.0 ? (float)bb[2] : 1.f; m.z = aa[2] > 0.0 ? 1.f : 0.f;
+        b.w = aa[3] > 0.0 ? (float)bb[3] : 1.f; m.w = aa[3] > 0.0 ? 1.f : 0.f;
+    }
+    // stage p_new = m z + b p_old on the tile and its halo; write the tile's own nodes
+    for (int h = t.grp; h < kHN; h += t.ngrp) {
+        const int hx = h % (kTX + 2), hy = h / (kTX + 2);
+        const int gx = tx0 + hx - 1, gy = ty0 + hy - 1;
+        V4<float> v = {0.f, 0.f, 0.f, 0.f};
+        if (ok && gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const size_t o = (size_t)(base + gy * W + gx) * kp + t.col0;
+            pdir4(ld4(z + o), m, b, ld4(pold + o), v);
+            if (hx >= 1 && hx <= kTX && hy >= 1 && hy <= kTY) st4(pnew + o, v);
+        }
+        sm[h * LPR + t.lane] = f4(v);
+    }
+    __syncthreads();
+    V4<float> pq = {0.f, 0.f, 0.f, 0.f};
+    if (ok) {
+        for (int o = t.grp; o < kTX * kTY; o += t.ngrp) {
+            const int ox = o % kTX, oy = o / kTX;
+            const int gx = tx0 + ox, gy = ty0 + oy;
+            if (gx >= W || gy >= H) continue;
+            const V4<float> q = stencil9<LPR>(sm, coef + (size_t)(base + gy * W + gx) * 9, ox, oy, t.lane);
+            const V4<float> pi = v4(sm[((oy + 1) * (kTX + 2) + (ox + 1)) * LPR + t.lane]);
+            pq.x += pi.x * q.x; pq.y += pi.y * q.y; pq.z += pi.z * q.z; pq.w += pi.w * q.w;
         }
     }
-    const Tile t = as_tile<LPR>(g);
+    __syncthreads();   // the staging area becomes the reduction buffer
+    const V4<double> ad = {(double)pq.x, (double)pq.y, (double)pq.z, (double)pq.w};
+    block_accumulate<LPR>(t, kp, ad, reinterpret_cast<double*>(sm), acc + A_PQ * ks + t.so);
+}
+
+template <int LPR>
+__global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gtab, int kp, int ks,
+                                                     const float* __restrict__ coef, const float* __restrict__ dinv,
+                                                     const float* __restrict__ diag, const double* __restrict__ S,
+                                                     const float* __restrict__ p, float* __restrict__ x,
+                                                     float* __restrict__ z, double* __restrict__ acc) {
+    extern __shared__ float4 sm[];
+    int d[kGT];
+    const Tile t = grid_tile_t<LPR>(kp, gtab, d);
+    const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
+    if (W == 0) return;
+    if (block_done(t, kp, ks, S)) return;
+    const bool ok = t.col0 < kp;
+    for (int h = t.grp; h < kHN; h += t.ngrp) {
+        const int hx = h % (kTX + 2), hy = h / (kTX + 2);
+        const int gx = tx0 + hx - 1, gy = ty0 + hy - 1;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok && gx >= 0 && gx < W && gy >= 0 && gy < H)
+            v = *reinterpret_cast<const float4*>(p + (size_t)(base + gy * W + gx) * kp + t.col0);
+        sm[h * LPR + t.lane] = v;
+    }
+    __syncthreads();
+    V4<float> rz = {0.f, 0.f, 0.f, 0.f}, rr = {0.f, 0.f, 0.f, 0.f};
+    if (ok) {
+        const V4<float> al = step_alpha<float>(ks, t.so + t.col0, S, acc);
+        for (int o = t.grp; o < kTX * kTY; o += t.ngrp) {
+            const int ox = o % kTX, oy = o / kTX;
+            const int gx = tx0 + ox, gy = ty0 + oy;
+            if (gx >= W || gy >= H) continue;
+            const int nd = base + gy * W + gx;
+            const V4<float> q = stencil9<LPR>(sm, coef + (size_t)nd * 9, ox, oy, t.lane);
+            const V4<float> pi = v4(sm[((oy + 1) * (kTX + 2) + (ox + 1)) * LPR + t.lane]);
+            const size_t off = (size_t)nd * kp + t.col0;
+            V4<float> xv = ld4(x + off), zv = ld4(z + off);
+            const float dd = dinv[nd], g = diag[nd];
+            xv.x += al.x * pi.x; xv.y += al.y * pi.y; xv.z += al.z * pi.z; xv.w += al.w * pi.w;
+            zv.x -= al.x * dd * q.x; zv.y -= al.y * dd * q.y; zv.z -= al.z * dd * q.z; zv.w -= al.w * dd * q.w;
+            st4(x + off, xv);
+            st4(z + off, zv);
+            res_dots(zv, g, rz, rr);
+        }
+    }
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(sm);
     V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
-    block_accumulate<LPR>(t, kp, a, red, acc + A_RZ * kp);
+    block_accumulate<LPR>(t, kp, a, red, acc + A_RZ * ks + t.so);
     __syncthreads();
     V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
-    block_accumulate<LPR>(t, kp, bb, red, acc + A_RR * kp);
+    block_accumulate<LPR>(t, kp, bb, red, acc + A_RR * ks + t.so);
 }
 
 // CSR -> DIA-9. Every entry must couple a grid neighbour of its row's node;
 // anything else (err) keeps the solver on the CSR kernels.
-__global__ void k_build_dia9(int W, int H, const int* __restrict__ rp, const int* __restrict__ ci,
+__global__ void k_build_dia9(int base, int W, int H, const int* __restrict__ rp, const int* __restrict__ ci,
                              const double* __restrict__ va, float* d32, double* d64, int* err) {
-    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-    if (nd >= W * H) return;
-    const int x = nd % W, y = nd / W;
+    const int loc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (loc >= W * H) return;
+    const int nd = base + loc;
+    const int x = loc % W, y = loc / W;
     for (int k = 0; k < 9; ++k) { d32[(size_t)nd * 9 + k] = 0.f; d64[(size_t)nd * 9 + k] = 0.0; }
     for (int e = rp[nd]; e < rp[nd + 1]; ++e) {
         const int c = ci[e];
-        const int cx = c % W, cy = c / W;
+        const int cl = c - base;
+        const int cx = cl % W, cy = cl / W;
         const int dx = cx - x, dy = cy - y;
-        if (dx < -1 || dx > 1 || dy < -1 || dy > 1) { atomicOr(err, 1); continue; }
+        if (cl < 0 || cl >= W * H || dx < -1 || dx > 1 || dy < -1 || dy > 1) { atomicOr(err, 1); continue; }
         const int k = (dy + 1) * 3 + (dx + 1);
         d32[(size_t)nd * 9 + k] = (float)va[e];
         d64[(size_t)nd * 9 + k] = va[e];
@@ -748,8 +766,10 @@ struct Plan {
     // bseg[b]; segment q's work-groups are sb[q] .. sb[q+1]-1, its rows
     // srow[q] .. srow[q+1]-1
     std::vector<int> brow, bseg, sb, srow;
+    std::vector<int> gtab;   // 2D-grid tile table (kGT ints per work-group)
     // 2D tensor-grid path
     int grid, W, H, L, nsegx, gblk;
+    const int* d_gtab = nullptr;
 };
 
 Plan make_plan(int n, int k, int num_cu, int nseg = 1, const int* seg_rows = nullptr) {
@@ -786,15 +806,25 @@ Plan make_plan(int n, int k, int num_cu, int nseg = 1, const int* seg_rows = nul
 }
 
 // segments of L nodes along x, one per lane group
-void plan_grid(Plan& p, int W, int H) {
+// one work-group per kTX x kTY node tile of every grid (segments in order, tiles
+// row-major within a grid), padded to whole XCD groups; grids: (first row, W, H)
+void plan_grid(Plan& p, const std::vector<int>& grids) {
     p.grid = 1;
-    p.W = W;
-    p.H = H;
-    p.L = 32;
-    p.nsegx = (W + p.L - 1) / p.L;
-    const int groups = kB / p.lpr;
-    p.gblk = (p.nsegx * H + groups - 1) / groups;
-    p.gblk = (p.gblk + 7) / 8 * 8;
+    p.gtab.clear();
+    const int ng = (int)grids.size() / 3;
+    for (int q = 0; q < ng; ++q) {
+        const int base = grids[3 * q], W = grids[3 * q + 1], H = grids[3 * q + 2];
+        for (int ty = 0; ty < H; ty += kTY)
+            for (int tx = 0; tx < W; tx += kTX) {
+                const int e[kGT] = {base, W, H, tx, ty, q};
+                p.gtab.insert(p.gtab.end(), e, e + kGT);
+            }
+    }
+    while ((p.gtab.size() / kGT) % 8) {
+        const int e[kGT] = {0, 0, 0, 0, 0, 0};
+        p.gtab.insert(p.gtab.end(), e, e + kGT);
+    }
+    p.gblk = (int)(p.gtab.size() / kGT);
 }
 
 template <class T, int LPR>
@@ -809,15 +839,18 @@ struct Kern {
                        const T* dinv, const T* diag, const double* S, const T* p, T* x, T* z, double* acc) {
         k_cg_update<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.kp, G, rp, M, dinv, diag, S, p, x, z, acc);
     }
-    static void dir_spmv_g(const Plan& P, hipStream_t st, const T* coef, const T* dinv, const double* beta,
+    // 2D tensor-grid kernels: FP32, at most 64 columns (16 lanes) per column tile
+    static void dir_spmv_g(const Plan& P, hipStream_t st, const T* coef, const double* S, const double* beta,
                            const double* active, const T* z, const T* pold, T* pnew, double* acc) {
-        k_dir_spmv_g2<T, LPR><<<dim3(P.gblk, P.ntile), kB, 0, st>>>(P.W, P.H, P.kp, P.L, P.nsegx, coef, dinv, beta,
-                                                                    active, z, pold, pnew, acc);
+        if constexpr (std::is_same<T, float>::value && LPR <= 16)
+            k_dir_spmv_t2<LPR><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, coef, S, beta, active,
+                                                                       z, pold, pnew, acc);
     }
     static void update_g(const Plan& P, hipStream_t st, const T* coef, const T* dinv, const T* diag, const double* S,
                          const T* p, T* x, T* z, double* acc) {
-        k_cg_update_g2<T, LPR><<<dim3(P.gblk, P.ntile), kB, 0, st>>>(P.W, P.H, P.kp, P.L, P.nsegx, coef, dinv, diag, S,
-                                                                     p, x, z, acc);
+        if constexpr (std::is_same<T, float>::value && LPR <= 16)
+            k_cg_update_t2<LPR><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, coef, dinv, diag, S,
+                                                                        p, x, z, acc);
     }
     static void rescale(const Plan& P, const Seg& G, hipStream_t st, const double* scale, const double* ratio,
                         const double* dinv64, const T* diag, const double* r64, T* z, T* p, double* p0, double* p1) {
@@ -843,25 +876,45 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     const int n = P.n, kp = P.kp;
     const size_t nk = (size_t)n * kp;
     const int ns = P.nseg, ks = P.ks;
-    // 2D tensor grid: stencil layout and sliding-window kernels (one segment)
-    if (ns == 1 && o.layout == 2 && s->gn[0] >= 3 && s->dim == 2 && s->dofpn == 1 && s->gn[1] >= 3 &&
-        s->dia_state >= 0) {
+    // 2D tensor grids: 9-coefficient stencil layout and LDS-tiled kernels (FP32,
+    // at most 64 columns per tile); layout 0 chooses it whenever every problem is
+    // a declared 2D grid: the system's grid for one problem, the declared
+    // segment grids for a batch
+    std::vector<int> grids;   // (first row, W, H) per problem
+    if (s->dim == 2 && s->dofpn == 1) {
+        if (ns == 1 && s->gn[0] >= 3 && s->gn[1] >= 3 && (size_t)s->gn[0] * s->gn[1] == (size_t)n) {
+            grids = {0, s->gn[0], s->gn[1]};
+        } else if (ns > 1 && (int)s->sgrid.size() == 3 * ns) {
+            bool match = true;
+            for (int q = 0; q < ns; ++q)
+                match = match && s->sgrid[3 * q] == P.srow[q] &&
+                        (long long)s->sgrid[3 * q + 1] * s->sgrid[3 * q + 2] == P.srow[q + 1] - P.srow[q] &&
+                        s->sgrid[3 * q + 1] >= 3 && s->sgrid[3 * q + 2] >= 3;
+            if (match) grids = s->sgrid;
+        }
+    }
+    if (!grids.empty() && (o.layout == 2 || o.layout == 0) && std::is_same<T, float>::value && P.lpr <= 16 &&
+        P.ntile == 1 && s->dia_state >= 0) {
         if (s->dia_state == 0) {
             SFH_TRY(s->dia32.ensure((size_t)n * 9 * 4));
             SFH_TRY(s->dia64.ensure((size_t)n * 9 * 8));
             DevBuf err;
             SFH_TRY(err.ensure(sizeof(int)));
             SFH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
-            k_build_dia9<<<(n + 255) / 256, 256, 0, st>>>(s->gn[0], s->gn[1], s->rowptr.as<int>(), s->colind.as<int>(),
-                                                         s->vals.as<double>(), s->dia32.as<float>(),
-                                                         s->dia64.as<double>(), err.as<int>());
-            SFH_LAUNCH_CHECK();
+            for (size_t q = 0; q < grids.size() / 3; ++q) {
+                const int base = grids[3 * q], W = grids[3 * q + 1], H = grids[3 * q + 2];
+                k_build_dia9<<<(W * H + 255) / 256, 256, 0, st>>>(base, W, H, s->rowptr.as<int>(),
+                                                                   s->colind.as<int>(), s->vals.as<double>(),
+                                                                   s->dia32.as<float>(), s->dia64.as<double>(),
+                                                                   err.as<int>());
+                SFH_LAUNCH_CHECK();
+            }
             int he = 0;
             SFH_CHECK_HIP(hipMemcpyAsync(&he, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
             SFH_CHECK_HIP(hipStreamSynchronize(st));
             s->dia_state = he ? -1 : 1;
         }
-        if (s->dia_state == 1) plan_grid(P, s->gn[0], s->gn[1]);
+        if (s->dia_state == 1) plan_grid(P, grids);
     }
     const int nbi = P.grid ? P.gblk : P.nblk;   // partial rows written by the iteration kernels
     SFH_TRY(w.b64.ensure(nk * 8));
@@ -877,8 +930,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     const int nred = P.nred;
     SFH_TRY(w.nact.ensure(sizeof(int) * 2 * nred));
     SFH_TRY(w.acc.ensure((size_t)A_N * ks * 8));
-    // partition tables: brow (nblk + 1), bseg (nblk), sb (nseg + 1), srow (nseg + 1)
-    const size_t ntab = P.brow.size() + P.bseg.size() + P.sb.size() + P.srow.size();
+    // partition tables: brow (nblk + 1), bseg (nblk), sb (nseg + 1), srow (nseg + 1), gtab
+    const size_t ntab = P.brow.size() + P.bseg.size() + P.sb.size() + P.srow.size() + P.gtab.size();
     SFH_TRY(w.tab.ensure(sizeof(int) * ntab));
     {
         std::vector<int> h;
@@ -887,12 +940,14 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         h.insert(h.end(), P.bseg.begin(), P.bseg.end());
         h.insert(h.end(), P.sb.begin(), P.sb.end());
         h.insert(h.end(), P.srow.begin(), P.srow.end());
+        h.insert(h.end(), P.gtab.begin(), P.gtab.end());
         SFH_CHECK_HIP(hipMemcpy(w.tab.p, h.data(), sizeof(int) * ntab, hipMemcpyHostToDevice));
     }
     const int* d_brow = w.tab.as<int>();
     const int* d_bseg = d_brow + P.brow.size();
     const int* d_sb = d_bseg + P.bseg.size();
     const int* d_srow = d_sb + P.sb.size();
+    P.d_gtab = d_srow + P.srow.size();
     const Seg G = {d_brow, d_bseg, ks};
     const dim3 gred((kp + 63) / 64, ns);
     double* S = w.scal.as<double>();
@@ -979,7 +1034,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             const bool smp = ismp < nsmp && iters % cx->ksample == 0;
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp], st));
             if (P.grid)
-                Kern<T, LPR>::dir_spmv_g(P, st, coef, dinv, S + S_BETA * ks, S + S_ACTIVE * ks, z, pa, pb, acc);
+                Kern<T, LPR>::dir_spmv_g(P, st, coef, S, S + S_BETA * ks, S + S_ACTIVE * ks, z, pa, pb, acc);
             else
                 Kern<T, LPR>::dir_spmv(P, G, st, rp, M, S, S + S_BETA * ks, S + S_ACTIVE * ks, z, pa, pb, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 1], st));
@@ -1034,6 +1089,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         stats->max_relres = maxrel;
         stats->ms = ms;
         stats->converged = done ? 1 : 0;
+        stats->layout = P.grid ? 2 : 1;
     }
     if (!done) {
         set_error("solve did not reach tol %.3g (max rel. residual %.3g after %d iterations, %d replacements)",
@@ -1077,11 +1133,11 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
     if (o.max_outer <= 0) o.max_outer = 500;
     if (o.max_inner <= 0) o.max_inner = 50000;
     if (o.check_every <= 0) o.check_every = 32;
-    if (o.layout == 0) o.layout = 1;
+    if (o.layout < 0 || o.layout > 2) o.layout = 0;
     // column tiles: each tile is an independent multi-RHS solve whose working
     // set can stay in the last-level cache
     if (o.rhs_tile > 0 && o.rhs_tile < k) {
-        sfh_solve_stats acc = {0, 0, 0.0, 0.0, 1};
+        sfh_solve_stats acc = {0, 0, 0.0, 0.0, 1, 0};
         for (int c0 = 0; c0 < k; c0 += o.rhs_tile) {
             const int kc = std::min(o.rhs_tile, k - c0);
             sfh_solve_stats st1;
@@ -1093,6 +1149,7 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
             acc.max_relres = std::max(acc.max_relres, st1.max_relres);
             acc.ms += st1.ms;
             acc.converged = acc.converged && st1.converged;
+            acc.layout = st1.layout;
             if (rc != SFH_OK) {
                 if (stats) *stats = acc;
                 return rc;
@@ -1116,5 +1173,5 @@ extern "C" void sfh_solve_opts_default(sfh_solve_opts* o) {
     o->check_every = 32;
     o->precision = 0;
     o->rhs_tile = 0;
-    o->layout = 1;
+    o->layout = 0;
 }

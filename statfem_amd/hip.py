@@ -38,7 +38,7 @@ class SolveOpts(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("outer", C.c_int), ("inner_iters", C.c_int), ("max_relres", C.c_double),
-                ("ms", C.c_double), ("converged", C.c_int)]
+                ("ms", C.c_double), ("converged", C.c_int), ("layout", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -62,6 +62,7 @@ _SIGS = {
     "sfh_system_create_triplets": (i32, [v, i32, C.c_longlong, v, v, v, C.POINTER(v)]),
     "sfh_system_set_triplet_values": (i32, [v, C.c_longlong, v]),
     "sfh_system_set_rhs": (i32, [v, v]),
+    "sfh_system_set_segment_grids": (i32, [v, i32, v, v, v]),
     "sfh_solve_batched": (i32, [v, i32, v, i32, i32, v, v, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "sfh_gram_segmented": (i32, [v, i32, v, i32, i32, v, i32, v, i32, v]),
     "sfh_covforcing_grid": (i32, [v, i32, i32, i32, v, v, v, v, dbl, dbl, i32, v, v]),
