@@ -2,6 +2,9 @@
 // No exception or abort crosses this boundary: every call returns a status.
 #include <cstring>
 
+#include <string>
+#include <thread>
+
 #include "common.hpp"
 
 namespace sfh {
@@ -151,6 +154,7 @@ int sfh_ctx_create(int device, sfh_ctx** out) {
     c->num_cu = prop.multiProcessorCount;
     c->lds_per_block = prop.sharedMemPerBlock;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->tev0) != hipSuccess || hipEventCreate(&c->tev1) != hipSuccess) {
         set_error("stream/event creation failed");
@@ -171,6 +175,7 @@ int sfh_ctx_destroy(sfh_ctx* c) {
     (void)hipEventDestroy(c->tev1);
     for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream2);
     delete c;
     return SFH_OK;
 }
@@ -580,6 +585,28 @@ int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const dou
     k_scatter_pt<<<nb(ns, 64), 64, 0, st>>>(ns, pn, s->fg_cols.as<int>(), s->fg_vals.as<double>(), ns, Pt);
     SFH_LAUNCH_CHECK();
     sfh_solve_stats s1 = {}, s2 = {}, s3 = {};
+    // ubar = A^-1 F: a one-column solve of latency-bound iterations, run on the
+    // second stream from a helper thread while the multi-RHS solves run here
+    struct Background {
+        std::thread t;
+        int rc = SFH_OK;
+        std::string err;
+        ~Background() { if (t.joinable()) t.join(); }
+        int join() {
+            if (t.joinable()) t.join();
+            if (rc != SFH_OK) set_error("%s", err.c_str());
+            return rc;
+        }
+    } bg;
+    if (ubar) {
+        SFH_TRY(prepare_operator(s));
+        SFH_CHECK_HIP(hipStreamSynchronize(st));
+        bg.t = std::thread([&bg, s, ubar, opts, &s3] {
+            hipSetDevice(s->ctx->device);
+            bg.rc = solve_on(s, s->ctx->stream2, s->work2, 0, 1, s->F.as<double>(), 1, ubar, 1, opts, &s3);
+            if (bg.rc != SFH_OK) bg.err = sfh_last_error();
+        });
+    }
     // R^T = A^-T P^T (ForwardGaussRHS.hpp:105-112)
     SFH_TRY(solve(s, 1, ns, Pt, ns, Rt, ns, opts, &s1));
     // mean = R F (:124)
@@ -594,7 +621,7 @@ int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const dou
         k_axpy<<<nb((long long)nk, 256), 256, 0, st>>>((long long)nk, reg * reg, Pt, field);
         SFH_LAUNCH_CHECK();
     }
-    if (ubar) SFH_TRY(solve(s, 0, 1, s->F.as<double>(), 1, ubar, 1, opts, &s3));
+    SFH_TRY(bg.join());
     SFH_CHECK_HIP(hipEventRecord(e1, st));
     std::vector<double> hCu((size_t)ns * ns), hm(ns);
     SFH_CHECK_HIP(hipMemcpyAsync(hCu.data(), dCu, hCu.size() * 8, hipMemcpyDeviceToHost, st));

@@ -81,6 +81,7 @@ struct SolveWork {
 struct sfh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;               // concurrent second solves
     hipEvent_t ev0 = nullptr, ev1 = nullptr;    // internal (solve timing)
     hipEvent_t tev0 = nullptr, tev1 = nullptr;  // public timer
     int num_cu = 0;
@@ -129,6 +130,7 @@ struct sfh_system {
     int dia_state = 0;           // 0 stale, 1 valid, -1 mesh does not fit the stencil
     std::vector<int> sgrid;      // batch of 2D grids: (first row, W, H) per problem
     sfh::SolveWork work;
+    sfh::SolveWork work2;        // workspace of a concurrent second solve
     sfh::DevBuf scratch0, scratch1, scratch2, scratch3;
     sfh::DevBuf fg_pt, fg_rt, fg_v, fg_cv, fg_cu, fg_gram, fg_cols, fg_vals;
 };
@@ -157,6 +159,12 @@ int apply_constraints(sfh_system* s, int nc, const int* dofs, const double* valu
 // PCG scalars and convergence test
 int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, int ldx,
           const sfh_solve_opts* o, sfh_solve_stats* st, int nseg = 1, const int* seg_rows = nullptr);
+// the same solve on another stream and workspace (a concurrent solve on the
+// system from another host thread); prepare_operator first, on the context
+// stream, builds what both read
+int solve_on(sfh_system* s, hipStream_t st, SolveWork& w, int trans, int k, const double* B, int ldb, double* X,
+             int ldx, const sfh_solve_opts* o, sfh_solve_stats* stats);
+int prepare_operator(sfh_system* s, int nseg = 1, const int* seg_rows = nullptr);
 int triplets_build(sfh_system* s, long long n, const int* rows, const int* cols, const double* vals);
 int triplets_values(sfh_system* s, long long n, const double* vals);
 int covforcing_grid(sfh_ctx* c, int d, const int gn[3], const double* const ax[3], const double* cv, double sigma,

@@ -862,6 +862,66 @@ struct Kern {
     }
 };
 
+// Stream and workspace of a solve: the context's by default; a concurrent
+// solve on the same system (issued from another host thread) brings its own.
+struct SolveEnv {
+    hipStream_t st;
+    SolveWork* w;
+};
+thread_local const SolveEnv* tl_env = nullptr;
+
+// (first row, W, H) of every problem when all are declared 2D grids, else empty
+std::vector<int> grids_of(const sfh_system* s, int nseg, const std::vector<int>& srow) {
+    std::vector<int> grids;
+    if (s->dim != 2 || s->dofpn != 1) return grids;
+    const int n = s->ndof;
+    if (nseg == 1 && s->gn[0] >= 3 && s->gn[1] >= 3 && (size_t)s->gn[0] * s->gn[1] == (size_t)n) {
+        grids = {0, s->gn[0], s->gn[1]};
+    } else if (nseg > 1 && (int)s->sgrid.size() == 3 * nseg) {
+        bool match = true;
+        for (int q = 0; q < nseg; ++q)
+            match = match && s->sgrid[3 * q] == srow[q] &&
+                    (long long)s->sgrid[3 * q + 1] * s->sgrid[3 * q + 2] == srow[q + 1] - srow[q] &&
+                    s->sgrid[3 * q + 1] >= 3 && s->sgrid[3 * q + 2] >= 3;
+        if (match) grids = s->sgrid;
+    }
+    return grids;
+}
+
+// operator data the iteration kernels read besides the CSR arrays: D^-1, diag,
+// packed FP32 entries, and the stencil coefficients of declared grids
+int prepare_operator_on(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
+    const int n = s->ndof;
+    if (s->dirty) {
+        k_dinv<<<(n + 255) / 256, 256, 0, st>>>(n, s->diagpos.as<int>(), s->vals.as<double>(), s->dinv32.as<float>(),
+                                                s->dinv64.as<double>(), s->diag32.as<float>(), s->diag64.as<double>());
+        SFH_LAUNCH_CHECK();
+        k_pack_entries<<<(unsigned)((s->nnz + 255) / 256), 256, 0, st>>>(s->nnz, s->colind.as<int>(),
+                                                                         s->vals32.as<float>(), s->cv32.as<int2>());
+        SFH_LAUNCH_CHECK();
+        s->dirty = false;
+    }
+    if (!grids.empty() && s->dia_state == 0) {
+        SFH_TRY(s->dia32.ensure((size_t)n * 9 * 4));
+        SFH_TRY(s->dia64.ensure((size_t)n * 9 * 8));
+        DevBuf err;
+        SFH_TRY(err.ensure(sizeof(int)));
+        SFH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+        for (size_t q = 0; q < grids.size() / 3; ++q) {
+            const int base = grids[3 * q], W = grids[3 * q + 1], H = grids[3 * q + 2];
+            k_build_dia9<<<(W * H + 255) / 256, 256, 0, st>>>(base, W, H, s->rowptr.as<int>(), s->colind.as<int>(),
+                                                               s->vals.as<double>(), s->dia32.as<float>(),
+                                                               s->dia64.as<double>(), err.as<int>());
+            SFH_LAUNCH_CHECK();
+        }
+        int he = 0;
+        SFH_CHECK_HIP(hipMemcpyAsync(&he, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        SFH_CHECK_HIP(hipStreamSynchronize(st));
+        s->dia_state = he ? -1 : 1;
+    }
+    return SFH_OK;
+}
+
 // Mixed-precision PCG with residual replacement ("reliable updates"):
 // FP32 recurrences; whenever a column's recursive residual fell by 10x (or
 // below its target) the accumulated FP32 correction is flushed into the FP64
@@ -870,52 +930,21 @@ struct Kern {
 template <class T, int LPR>
 int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, int ldb, double* X, int ldx,
               const sfh_solve_opts& o, sfh_solve_stats* stats) {
-    hipStream_t st = s->ctx->stream;
-    SolveWork& w = s->work;
+    hipStream_t st = tl_env ? tl_env->st : s->ctx->stream;
+    SolveWork& w = tl_env ? *tl_env->w : s->work;
     Plan P = P0;
     const int n = P.n, kp = P.kp;
     const size_t nk = (size_t)n * kp;
     const int ns = P.nseg, ks = P.ks;
     // 2D tensor grids: 9-coefficient stencil layout and LDS-tiled kernels (FP32,
-    // at most 64 columns per tile); layout 0 chooses it whenever every problem is
+    // 16 to 64 columns per tile); layout 0 chooses it whenever every problem is
     // a declared 2D grid: the system's grid for one problem, the declared
     // segment grids for a batch
-    std::vector<int> grids;   // (first row, W, H) per problem
-    if (s->dim == 2 && s->dofpn == 1) {
-        if (ns == 1 && s->gn[0] >= 3 && s->gn[1] >= 3 && (size_t)s->gn[0] * s->gn[1] == (size_t)n) {
-            grids = {0, s->gn[0], s->gn[1]};
-        } else if (ns > 1 && (int)s->sgrid.size() == 3 * ns) {
-            bool match = true;
-            for (int q = 0; q < ns; ++q)
-                match = match && s->sgrid[3 * q] == P.srow[q] &&
-                        (long long)s->sgrid[3 * q + 1] * s->sgrid[3 * q + 2] == P.srow[q + 1] - P.srow[q] &&
-                        s->sgrid[3 * q + 1] >= 3 && s->sgrid[3 * q + 2] >= 3;
-            if (match) grids = s->sgrid;
-        }
-    }
-    if (!grids.empty() && (o.layout == 2 || o.layout == 0) && std::is_same<T, float>::value && P.lpr <= 16 &&
-        P.ntile == 1 && s->dia_state >= 0) {
-        if (s->dia_state == 0) {
-            SFH_TRY(s->dia32.ensure((size_t)n * 9 * 4));
-            SFH_TRY(s->dia64.ensure((size_t)n * 9 * 8));
-            DevBuf err;
-            SFH_TRY(err.ensure(sizeof(int)));
-            SFH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
-            for (size_t q = 0; q < grids.size() / 3; ++q) {
-                const int base = grids[3 * q], W = grids[3 * q + 1], H = grids[3 * q + 2];
-                k_build_dia9<<<(W * H + 255) / 256, 256, 0, st>>>(base, W, H, s->rowptr.as<int>(),
-                                                                   s->colind.as<int>(), s->vals.as<double>(),
-                                                                   s->dia32.as<float>(), s->dia64.as<double>(),
-                                                                   err.as<int>());
-                SFH_LAUNCH_CHECK();
-            }
-            int he = 0;
-            SFH_CHECK_HIP(hipMemcpyAsync(&he, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
-            SFH_CHECK_HIP(hipStreamSynchronize(st));
-            s->dia_state = he ? -1 : 1;
-        }
-        if (s->dia_state == 1) plan_grid(P, grids);
-    }
+    const std::vector<int> grids = grids_of(s, ns, P.srow);
+    const bool use_grid = !grids.empty() && (o.layout == 2 || o.layout == 0) && std::is_same<T, float>::value &&
+                          P.lpr >= 4 && P.lpr <= 16 && P.ntile == 1;
+    SFH_TRY(prepare_operator_on(s, st, use_grid ? grids : std::vector<int>()));
+    if (use_grid && s->dia_state == 1) plan_grid(P, grids);
     const int nbi = P.grid ? P.gblk : P.nblk;   // partial rows written by the iteration kernels
     SFH_TRY(w.b64.ensure(nk * 8));
     SFH_TRY(w.x64.ensure(nk * 8));
@@ -941,7 +970,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         h.insert(h.end(), P.sb.begin(), P.sb.end());
         h.insert(h.end(), P.srow.begin(), P.srow.end());
         h.insert(h.end(), P.gtab.begin(), P.gtab.end());
-        SFH_CHECK_HIP(hipMemcpy(w.tab.p, h.data(), sizeof(int) * ntab, hipMemcpyHostToDevice));
+        SFH_CHECK_HIP(hipMemcpyAsync(w.tab.p, h.data(), sizeof(int) * ntab, hipMemcpyHostToDevice, st));
+        SFH_CHECK_HIP(hipStreamSynchronize(st));
     }
     const int* d_brow = w.tab.as<int>();
     const int* d_bseg = d_brow + P.brow.size();
@@ -966,16 +996,10 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     double* p0 = w.part0.as<double>();
     double* p1 = w.part1.as<double>();
 
-    SFH_CHECK_HIP(hipEventRecord(s->ctx->ev0, st));
-    if (s->dirty) {
-        k_dinv<<<(n + 255) / 256, 256, 0, st>>>(n, s->diagpos.as<int>(), va, s->dinv32.as<float>(),
-                                                s->dinv64.as<double>(), s->diag32.as<float>(), s->diag64.as<double>());
-        SFH_LAUNCH_CHECK();
-        k_pack_entries<<<(unsigned)((s->nnz + 255) / 256), 256, 0, st>>>(s->nnz, ci, s->vals32.as<float>(),
-                                                                         s->cv32.as<int2>());
-        SFH_LAUNCH_CHECK();
-        s->dirty = false;
-    }
+    hipEvent_t ev0, ev1;
+    SFH_CHECK_HIP(hipEventCreate(&ev0));
+    SFH_CHECK_HIP(hipEventCreate(&ev1));
+    SFH_CHECK_HIP(hipEventRecord(ev0, st));
     k_prepare<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>(n, k, kp, trans, B, ldb, rp, ci, va,
                                                           s->diagpos.as<int>(), s->cmask.as<unsigned char>(),
                                                           w.b64.as<double>());
@@ -987,7 +1011,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_CHECK_HIP(hipMemsetAsync(w.acc.p, 0, (size_t)A_N * ks * 8, st));
     double* acc = w.acc.as<double>();
     sfh_ctx* cx = s->ctx;
-    const int nsmp = (cx->ksample > 0 && k >= cx->kmin_rhs) ? (int)cx->kev.size() / 4 : 0;
+    const int nsmp = (!tl_env && cx->ksample > 0 && k >= cx->kmin_rhs) ? (int)cx->kev.size() / 4 : 0;
     int ismp = 0;
     Kern<T, LPR>::resid(P, G, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
     k_red_norm<<<gred, kRedB, 0, st>>>(kp, d_sb, p0, S + S_BN2 * ks);
@@ -1071,10 +1095,12 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         n, k, kp, trans, B, ldb, w.x64.as<double>(), rp, ci, va, s->diagpos.as<int>(),
         s->cmask.as<unsigned char>(), X, ldx);
     SFH_LAUNCH_CHECK();
-    SFH_CHECK_HIP(hipEventRecord(s->ctx->ev1, st));
-    SFH_CHECK_HIP(hipEventSynchronize(s->ctx->ev1));
+    SFH_CHECK_HIP(hipEventRecord(ev1, st));
+    SFH_CHECK_HIP(hipEventSynchronize(ev1));
     float ms = 0.f;
-    SFH_CHECK_HIP(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
+    SFH_CHECK_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
     for (int i = 0; i < ismp; ++i) {
         for (int j = 0; j < 3; ++j) {
             float f = 0.f;
@@ -1161,6 +1187,24 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
     const Plan P = make_plan(s->ndof, k, s->ctx->num_cu, nseg, seg_rows);
     if (o.precision == 1) return dispatch_solve<double>(s, P, trans, k, B, ldb, X, ldx, o, stats);
     return dispatch_solve<float>(s, P, trans, k, B, ldb, X, ldx, o, stats);
+}
+
+int prepare_operator(sfh_system* s, int nseg, const int* seg_rows) {
+    std::vector<int> srow(nseg + 1);
+    for (int q = 0; q <= nseg; ++q) srow[q] = seg_rows ? seg_rows[q] : (q == 0 ? 0 : s->ndof);
+    SFH_TRY(prepare_operator_on(s, s->ctx->stream, grids_of(s, nseg, srow)));
+    SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    return SFH_OK;
+}
+
+int solve_on(sfh_system* s, hipStream_t st, SolveWork& w, int trans, int k, const double* B, int ldb, double* X,
+             int ldx, const sfh_solve_opts* opts, sfh_solve_stats* stats) {
+    const SolveEnv env = {st, &w};
+    const SolveEnv* prev = tl_env;
+    tl_env = &env;
+    const int rc = solve(s, trans, k, B, ldb, X, ldx, opts, stats);
+    tl_env = prev;
+    return rc;
 }
 
 }  // namespace sfh
