@@ -112,7 +112,6 @@ def gpu_run(args):
         kname = {"dir_spmv": "k_dir_spmv<float, {lpr}>", "cg_update": "k_cg_update<float, {lpr}>"}
     per = {k: {"ms": kt[k], "GBs": algo[k] / (kt[k] * 1e-3) / 1e9, "bytes": algo[k], "samples": kt["samples"]}
            for k in algo}
-    per["beta"] = {"ms": kt["beta"]}
     dom = max(("dir_spmv", "cg_update"), key=lambda k: kt[k])
     lpr = min(64, 1 << max(0, (kp // 4 - 1).bit_length()))
     for k in algo:

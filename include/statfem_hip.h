@@ -205,11 +205,11 @@ int sfh_solve_rhs(sfh_system* sys, double* X_dev, const sfh_solve_opts* opts,
  * solve; only solves of at least min_rhs right-hand sides per column tile) is
  * bracketed by HIP events on the context stream; the accumulators are reset.
  * every = 0 turns sampling off. sfh_kernel_times returns the average device
- * time (ms per launch) of ms[0] direction + SpMV + p.Ap, ms[1] alpha +
- * SpMV + x/r update + r.z, r.r, ms[2] beta, and the number of samples since
- * the last sfh_kernel_sampling call. */
+ * time (ms per launch) of ms[0] direction + SpMV + p.Ap and ms[1] SpMV +
+ * x/r update + r.z, r.r (the two launches of an iteration), and the number of
+ * samples since the last sfh_kernel_sampling call. */
 int sfh_kernel_sampling(sfh_ctx* ctx, int every, int max_samples, int min_rhs);
-int sfh_kernel_times(sfh_ctx* ctx, double* ms /* 3 */, long long* samples);
+int sfh_kernel_times(sfh_ctx* ctx, double* ms /* 2 */, long long* samples);
 
 /* ------------------------------------------------------------- statistics */
 /* CovarianceForcing::giveProjectedCovar building block (statX/infer/CovarianceForcing.ipp:37-109):

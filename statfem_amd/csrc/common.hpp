@@ -71,7 +71,6 @@ struct SolveWork {
     DevBuf x32, r32, p32, q32;       // FP32 inner PCG (or FP64 when precision == 1)
     DevBuf part0, part1;             // per-block partial dot products
     DevBuf scal;                     // per-RHS scalars
-    DevBuf nact;                     // active / pending column counts
     DevBuf acc;                      // per-column FP64 accumulators of the iteration kernels
     DevBuf tab;                      // row-partition tables of the current solve
 };
@@ -87,13 +86,13 @@ struct sfh_ctx {
     int num_cu = 0;
     size_t lds_per_block = 0;
     std::vector<void*> allocs;
-    // sampled in-solve timing of the inner-PCG launches (direction + SpMV,
-    // SpMV + update, beta): every ksample-th iteration is bracketed by events
+    // sampled in-solve timing of the two inner-PCG launches (direction +
+    // SpMV, SpMV + update): every ksample-th iteration is bracketed by events
     // (0 = off)
     int ksample = 0;
     int kmin_rhs = 0;
-    std::vector<hipEvent_t> kev;     // 4 events per sample
-    double ksum[3] = {0.0, 0.0, 0.0};
+    std::vector<hipEvent_t> kev;     // 3 events per sample
+    double ksum[2] = {0.0, 0.0};
     long long kcount = 0;
 };
 

@@ -107,11 +107,88 @@ __device__ inline void block_partial(const Tile& t, int kp, const V4<double>& a,
 }
 
 // per-RHS scalar slots (each kp doubles)
-enum { S_RZ = 0, S_RZPREV, S_ALPHA, S_BETA, S_TRIG, S_TGT, S_ACTIVE, S_SCALE, S_RATIO, S_RN2, S_BN2,
+enum { S_RZ = 0, S_RZPREV, S_FROZEN, S_BETA, S_TRIG, S_TGT, S_ACTIVE, S_SCALE, S_RATIO, S_RN2, S_BN2,
        S_NSLOT };
 
-// per-column FP64 accumulators of the iteration kernels (kp doubles each)
-enum { A_PQ = 0, A_RZ, A_RR, A_N };
+// Per-column FP64 accumulators of the iteration kernels (ks doubles each),
+// rotating with the iteration index j of the current refinement cycle:
+// p.Ap of iteration j in A_PQ + j % 2, r.z and r.r after iteration j in
+// A_RZ + j % 3 and A_RR + j % 3. A slot is cleared by work-group 0 of the
+// kernel launched before the one that accumulates into it, and no work-group of
+// that launch reads it.
+enum { A_PQ = 0, A_RZ = 2, A_RR = 5, A_N = 8 };
+
+// r.z that the step length of iteration j uses
+__device__ inline double rz_at(int j, int ks, int c, const double* __restrict__ S, const double* __restrict__ acc) {
+    return j == 0 ? S[S_RZ * ks + c] : acc[(A_RZ + (j - 1) % 3) * ks + c];
+}
+
+// Direction coefficients of column c at iteration j: p = m z + b p_old with
+// b = beta, m = 1 for a live column and b = 1, m = 0 for a frozen or inactive
+// one. A live column whose recursive residual r.r reached its trigger is
+// frozen here (until the next FP64 replacement); every work-group takes the
+// same decision from the same inputs, so the flag and the r.z of the column's
+// last step are written with the same values by all of them.
+template <class T>
+__device__ inline void dir_coef(int j, int ks, int c, double* S, const double* __restrict__ acc, T& b, T& m) {
+    const bool act = S[S_ACTIVE * ks + c] > 0.0;
+    bool frozen = false;
+    double beta = S[S_BETA * ks + c];
+    if (j > 0 && act) {
+        frozen = S[S_FROZEN * ks + c] > 0.0;
+        if (!frozen && !(acc[(A_RR + (j - 1) % 3) * ks + c] > S[S_TRIG * ks + c])) {
+            frozen = true;
+            S[S_FROZEN * ks + c] = 1.0;
+            S[S_RZPREV * ks + c] = rz_at(j - 1, ks, c, S, acc);
+        }
+        const double rzo = rz_at(j - 1, ks, c, S, acc);
+        beta = rzo > 0.0 ? rz_at(j, ks, c, S, acc) / rzo : 0.0;
+    }
+    const bool live = act && !frozen;
+    b = live ? (T)beta : (T)1;
+    m = live ? (T)1 : (T)0;
+}
+
+// Per-column direction coefficients (b, m) or step lengths of the work-group's
+// 4*LPR columns, computed once per work-group into LDS and read by every row
+// group (the inputs are a handful of scattered scalars per column).
+template <class T, int LPR>
+__device__ inline void shared_dir_coef(int it, int kp, int ks, int so, double* S, const double* __restrict__ acc,
+                                       T* cb, T* cm) {
+    const int w = 4 * LPR;
+    for (int c = threadIdx.x; c < w; c += blockDim.x) {
+        const int col = blockIdx.y * w + c;
+        T b = (T)1, m = (T)0;
+        if (col < kp) dir_coef<T>(it, ks, so + col, S, acc, b, m);
+        cb[c] = b;
+        cm[c] = m;
+    }
+    __syncthreads();
+}
+
+template <class T, int LPR>
+__device__ inline void shared_alpha(int it, int kp, int ks, int so, const double* __restrict__ S,
+                                    const double* __restrict__ acc, T* ca) {
+    const int w = 4 * LPR;
+    for (int c = threadIdx.x; c < w; c += blockDim.x) {
+        const int col = blockIdx.y * w + c;
+        double a = 0.0;
+        if (col < kp) {
+            const int k = so + col;
+            const double pq = acc[(A_PQ + (it & 1)) * ks + k];
+            const bool live = S[S_ACTIVE * ks + k] > 0.0 && !(S[S_FROZEN * ks + k] > 0.0);
+            a = (live && pq > 0.0) ? rz_at(it, ks, k, S, acc) / pq : 0.0;
+        }
+        ca[c] = (T)a;
+    }
+    __syncthreads();
+}
+
+// work-group 0 clears `nslot` consecutive accumulator slots starting at slot a
+__device__ inline void clear_slots(double* acc, int a, int nslot, int ks) {
+    if (blockIdx.x != 0 || blockIdx.y != 0) return;
+    for (int i = threadIdx.x; i < nslot * ks; i += blockDim.x) acc[(size_t)a * ks + i] = 0.0;
+}
 
 // block-level sum over row groups of per-lane 4-column partials, added into
 // acc[col] with one hardware FP64 atomic per column and block (the order of the
@@ -130,20 +207,6 @@ __device__ inline void block_accumulate(const Tile& t, int kp, const V4<double>&
         const int col = blockIdx.y * w + c;
         if (col < kp) unsafeAtomicAdd(acc + col, s);
     }
-}
-
-// step length of 4 consecutive columns: alpha = r.z / p.Ap on active columns,
-// from the p.Ap accumulated by the direction kernel
-template <class T>
-__device__ inline V4<T> step_alpha(int ks, int col0, const double* __restrict__ S, const double* __restrict__ acc) {
-    double a[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int col = col0 + j;
-        const double pq = acc[A_PQ * ks + col];
-        a[j] = (S[S_ACTIVE * ks + col] > 0.0 && pq > 0.0) ? S[S_RZ * ks + col] / pq : 0.0;
-    }
-    return {(T)a[0], (T)a[1], (T)a[2], (T)a[3]};
 }
 
 // true when none of the columns the work-group handles is still being solved
@@ -208,29 +271,25 @@ __device__ inline void res_dots(const V4<T>& z, T g, V4<T>& rz, V4<T>& rr) {
 // recomputed on the fly at the neighbours the stencil touches), q = A p_new,
 // acc += sum_rows p_new . q. q itself is not stored.
 template <class T, int LPR>
-__global__ __launch_bounds__(kB) void k_dir_spmv(int kp, Seg G, const int* __restrict__ rp,
-                                                 const Entries<T> M, const double* __restrict__ S,
-                                                 const double* __restrict__ beta,
-                                                 const double* __restrict__ active, const T* __restrict__ z,
-                                                 const T* __restrict__ pold, T* __restrict__ pnew,
-                                                 double* __restrict__ acc) {
+__global__ __launch_bounds__(kB) void k_dir_spmv(int kp, Seg G, int it, const int* __restrict__ rp,
+                                                 const Entries<T> M, double* __restrict__ S,
+                                                 const T* __restrict__ z, const T* __restrict__ pold,
+                                                 T* __restrict__ pnew, double* __restrict__ acc) {
     __shared__ double red[kB * 4];
     const Tile t = tile<LPR>(kp, G);
+    clear_slots(acc, A_RZ + it % 3, 1, G.ks);
+    clear_slots(acc, A_RR + it % 3, 1, G.ks);
     if (block_done(t, kp, G.ks, S)) return;
+    __shared__ T cb[4 * LPR], cm[4 * LPR];
+    shared_dir_coef<T, LPR>(it, kp, G.ks, t.so, S, acc, cb, cm);
     const bool ok = t.col0 < kp;
     V4<T> pq = {0, 0, 0, 0};
     if (ok) {
         // frozen columns (waiting for residual replacement) keep their direction:
         // p = 1 * p_old + 0 * z
-        V4<T> b, m;
-        {
-            const double* bb = beta + t.so + t.col0;
-            const double* aa = active + t.so + t.col0;
-            b.x = aa[0] > 0.0 ? (T)bb[0] : (T)1; m.x = aa[0] > 0.0 ? (T)1 : (T)0;
-            b.y = aa[1] > 0.0 ? (T)bb[1] : (T)1; m.y = aa[1] > 0.0 ? (T)1 : (T)0;
-            b.z = aa[2] > 0.0 ? (T)bb[2] : (T)1; m.z = aa[2] > 0.0 ? (T)1 : (T)0;
-            b.w = aa[3] > 0.0 ? (T)bb[3] : (T)1; m.w = aa[3] > 0.0 ? (T)1 : (T)0;
-        }
+        const int lc = t.lane * 4;
+        const V4<T> b = {cb[lc], cb[lc + 1], cb[lc + 2], cb[lc + 3]};
+        const V4<T> m = {cm[lc], cm[lc + 1], cm[lc + 2], cm[lc + 3]};
         for (int row0 = t.r0 + t.grp; row0 < t.r1; row0 += t.ngrp) {
             const int row = uniform_row<LPR>(row0);
             V4<T> q = {0, 0, 0, 0};
@@ -270,24 +329,27 @@ __global__ __launch_bounds__(kB) void k_dir_spmv(int kp, Seg G, const int* __res
         }
     }
     const V4<double> ad = {(double)pq.x, (double)pq.y, (double)pq.z, (double)pq.w};
-    block_accumulate<LPR>(t, kp, ad, red, acc + A_PQ * G.ks + t.so);
+    block_accumulate<LPR>(t, kp, ad, red, acc + (A_PQ + (it & 1)) * G.ks + t.so);
 }
 
 // K2: q = A p (recomputed), x += alpha p, z -= alpha D^-1 q (z = D^-1 r),
 // acc += r.z, r.r with r = diag z
 template <class T, int LPR>
-__global__ __launch_bounds__(kB) void k_cg_update(int kp, Seg G, const int* __restrict__ rp,
+__global__ __launch_bounds__(kB) void k_cg_update(int kp, Seg G, int it, const int* __restrict__ rp,
                                                   const Entries<T> M, const T* __restrict__ dinv,
                                                   const T* __restrict__ diag, const double* __restrict__ S,
                                                   const T* __restrict__ p, T* __restrict__ x, T* __restrict__ z,
                                                   double* __restrict__ acc) {
     __shared__ double red[kB * 4];
     const Tile t = tile<LPR>(kp, G);
+    clear_slots(acc, A_PQ + ((it + 1) & 1), 1, G.ks);
     if (block_done(t, kp, G.ks, S)) return;
+    __shared__ T ca[4 * LPR];
+    shared_alpha<T, LPR>(it, kp, G.ks, t.so, S, acc, ca);
     const bool ok = t.col0 < kp;
     V4<T> rz = {0, 0, 0, 0}, rr = {0, 0, 0, 0};
     if (ok) {
-        const V4<T> al = step_alpha<T>(G.ks, t.so + t.col0, S, acc);
+        const V4<T> al = {ca[t.lane * 4], ca[t.lane * 4 + 1], ca[t.lane * 4 + 2], ca[t.lane * 4 + 3]};
         for (int row0 = t.r0 + t.grp; row0 < t.r1; row0 += t.ngrp) {
             const int row = uniform_row<LPR>(row0);
             V4<T> q = {0, 0, 0, 0};
@@ -325,10 +387,10 @@ __global__ __launch_bounds__(kB) void k_cg_update(int kp, Seg G, const int* __re
         }
     }
     V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
-    block_accumulate<LPR>(t, kp, a, red, acc + A_RZ * G.ks + t.so);
+    block_accumulate<LPR>(t, kp, a, red, acc + (A_RZ + it % 3) * G.ks + t.so);
     __syncthreads();
     V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
-    block_accumulate<LPR>(t, kp, bb, red, acc + A_RR * G.ks + t.so);
+    block_accumulate<LPR>(t, kp, bb, red, acc + (A_RR + it % 3) * G.ks + t.so);
 }
 
 // Residual replacement: z = D^-1 (s' r64), p *= ratio (s'/s), part0 = r.z,
@@ -426,27 +488,24 @@ __device__ inline V4<float> stencil9(const float4* sm, const float* __restrict__
 }
 
 template <int LPR>
-__global__ __launch_bounds__(kG) void k_dir_spmv_t2(const int* __restrict__ gtab, int kp, int ks,
-                                                    const float* __restrict__ coef, const double* __restrict__ S,
-                                                    const double* __restrict__ beta, const double* __restrict__ active,
+__global__ __launch_bounds__(kG) void k_dir_spmv_t2(const int* __restrict__ gtab, int kp, int ks, int it,
+                                                    const float* __restrict__ coef, double* __restrict__ S,
                                                     const float* __restrict__ z, const float* __restrict__ pold,
                                                     float* __restrict__ pnew, double* __restrict__ acc) {
     extern __shared__ float4 sm[];
     int d[kGT];
+    clear_slots(acc, A_RZ + it % 3, 1, ks);
+    clear_slots(acc, A_RR + it % 3, 1, ks);
     const Tile t = grid_tile_t<LPR>(kp, gtab, d);
     const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
     if (W == 0) return;   // padding work-groups (whole group)
     if (block_done(t, kp, ks, S)) return;
+    __shared__ float cb[4 * LPR], cm[4 * LPR];
+    shared_dir_coef<float, LPR>(it, kp, ks, t.so, S, acc, cb, cm);
     const bool ok = t.col0 < kp;
-    V4<float> b = {1.f, 1.f, 1.f, 1.f}, m = {0.f, 0.f, 0.f, 0.f};
-    if (ok) {
-        const double* bb = beta + t.so + t.col0;
-        const double* aa = active + t.so + t.col0;
-        b.x = aa[0] > 0.0 ? (float)bb[0] : 1.f; m.x = aa[0] > 0.0 ? 1.f : 0.f;
-        b.y = aa[1] > 0.0 ? (float)bb[1] : 1.f; m.y = aa[1] > 0.0 ? 1.f : 0.f;
-        b.z = aa[2] > 0.0 ? (float)bb[2] : 1.f; m.z = aa[2] > 0.0 ? 1.f : 0.f;
-        b.w = aa[3] > 0.0 ? (float)bb[3] : 1.f; m.w = aa[3] > 0.0 ? 1.f : 0.f;
-    }
+    const int lc = t.lane * 4;
+    const V4<float> b = {cb[lc], cb[lc + 1], cb[lc + 2], cb[lc + 3]};
+    const V4<float> m = {cm[lc], cm[lc + 1], cm[lc + 2], cm[lc + 3]};
     // stage p_new = m z + b p_old on the tile and its halo; write the tile's own nodes
     for (int h = t.grp; h < kHN; h += t.ngrp) {
         const int hx = h % (kTX + 2), hy = h / (kTX + 2);
@@ -473,21 +532,24 @@ __global__ __launch_bounds__(kG) void k_dir_spmv_t2(const int* __restrict__ gtab
     }
     __syncthreads();   // the staging area becomes the reduction buffer
     const V4<double> ad = {(double)pq.x, (double)pq.y, (double)pq.z, (double)pq.w};
-    block_accumulate<LPR>(t, kp, ad, reinterpret_cast<double*>(sm), acc + A_PQ * ks + t.so);
+    block_accumulate<LPR>(t, kp, ad, reinterpret_cast<double*>(sm), acc + (A_PQ + (it & 1)) * ks + t.so);
 }
 
 template <int LPR>
-__global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gtab, int kp, int ks,
+__global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gtab, int kp, int ks, int it,
                                                      const float* __restrict__ coef, const float* __restrict__ dinv,
                                                      const float* __restrict__ diag, const double* __restrict__ S,
                                                      const float* __restrict__ p, float* __restrict__ x,
                                                      float* __restrict__ z, double* __restrict__ acc) {
     extern __shared__ float4 sm[];
     int d[kGT];
+    clear_slots(acc, A_PQ + ((it + 1) & 1), 1, ks);
     const Tile t = grid_tile_t<LPR>(kp, gtab, d);
     const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
     if (W == 0) return;
     if (block_done(t, kp, ks, S)) return;
+    __shared__ float ca[4 * LPR];
+    shared_alpha<float, LPR>(it, kp, ks, t.so, S, acc, ca);
     const bool ok = t.col0 < kp;
     for (int h = t.grp; h < kHN; h += t.ngrp) {
         const int hx = h % (kTX + 2), hy = h / (kTX + 2);
@@ -500,7 +562,7 @@ __global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gta
     __syncthreads();
     V4<float> rz = {0.f, 0.f, 0.f, 0.f}, rr = {0.f, 0.f, 0.f, 0.f};
     if (ok) {
-        const V4<float> al = step_alpha<float>(ks, t.so + t.col0, S, acc);
+        const V4<float> al = {ca[t.lane * 4], ca[t.lane * 4 + 1], ca[t.lane * 4 + 2], ca[t.lane * 4 + 3]};
         for (int o = t.grp; o < kTX * kTY; o += t.ngrp) {
             const int ox = o % kTX, oy = o / kTX;
             const int gx = tx0 + ox, gy = ty0 + oy;
@@ -521,10 +583,10 @@ __global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gta
     __syncthreads();
     double* red = reinterpret_cast<double*>(sm);
     V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
-    block_accumulate<LPR>(t, kp, a, red, acc + A_RZ * ks + t.so);
+    block_accumulate<LPR>(t, kp, a, red, acc + (A_RZ + it % 3) * ks + t.so);
     __syncthreads();
     V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
-    block_accumulate<LPR>(t, kp, bb, red, acc + A_RR * ks + t.so);
+    block_accumulate<LPR>(t, kp, bb, red, acc + (A_RR + it % 3) * ks + t.so);
 }
 
 // CSR -> DIA-9. Every entry must couple a grid neighbour of its row's node;
@@ -617,61 +679,36 @@ __device__ inline double sum_parts(int b0, int nblk, int kp, const double* __res
     return tot;
 }
 
-// beta, rz from the accumulated r.z and r.r; all accumulators are cleared for
-// the next iteration. A column whose recursive
-// residual reached its trigger is frozen (alpha = 0, direction kept) until the
-// next FP64 replacement. Counts active and frozen (pending) columns.
-__global__ void k_red_beta(int kp, double* __restrict__ acc, double* __restrict__ S, int* __restrict__ cnt2) {
-    __shared__ int nact, npend;
-    if (threadIdx.x == 0) { nact = 0; npend = 0; }
-    __syncthreads();
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col < kp) {
-        const double rzn = acc[A_RZ * kp + col], rrn = acc[A_RR * kp + col];
-        acc[A_PQ * kp + col] = 0.0;
-        acc[A_RZ * kp + col] = 0.0;
-        acc[A_RR * kp + col] = 0.0;
-        if (S[S_ACTIVE * kp + col] > 0.0) {
-            const double rz = S[S_RZ * kp + col];
-            S[S_BETA * kp + col] = rz > 0.0 ? rzn / rz : 0.0;
-            S[S_RZPREV * kp + col] = rz;
-            S[S_RZ * kp + col] = rzn;
-            if (!(rrn > S[S_TRIG * kp + col])) {
-                S[S_ACTIVE * kp + col] = 0.0;
-                atomicAdd(&npend, 1);
-            } else {
-                atomicAdd(&nact, 1);
-            }
-        } else if (S[S_SCALE * kp + col] > 0.0) {
-            atomicAdd(&npend, 1);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) { cnt2[2 * blockIdx.x] = nact; cnt2[2 * blockIdx.x + 1] = npend; }
-}
-
-// after residual replacement: rz from the new residual, beta against the
-// rescaled previous rz (the direction is kept), next trigger level
-// (one column group of one segment per work-group: blockIdx.y = segment, whose
-// partial rows are the work-groups sb[seg] .. sb[seg+1]-1; slot stride ks)
+// After residual replacement: rz from the new residual; beta against the
+// rescaled r.z of the column's last step (jlast = last iteration of the cycle
+// that ended, -1 for none; a frozen column's is kept in S_RZPREV) since the
+// direction is kept; next trigger level; a fresh cycle (no frozen columns, all
+// accumulator slots cleared). One column group of one segment per work-group
+// (blockIdx.y = segment, whose partial rows are work-groups sb[seg] ..
+// sb[seg+1]-1); slot stride ks.
 __global__ __launch_bounds__(kRedB) void k_red_reset(int kp, int ks, const int* __restrict__ sb, const double* part0,
-                                                     const double* part1, double* Sg, double delta2) {
+                                                     const double* part1, double* S, double* acc, double delta2,
+                                                     int jlast) {
     __shared__ double red[kRedB];
     const int c = blockIdx.x * 64 + threadIdx.x % 64;
     const int b0 = sb[blockIdx.y], b1 = sb[blockIdx.y + 1];
     const double rzn = sum_parts(b0, b1, kp, part0, c, red);
     const double rrn = sum_parts(b0, b1, kp, part1, c, red);
-    double* S = Sg + (size_t)blockIdx.y * kp;   // this segment's columns; slots keep stride ks
-    const int col = c;
-    if (threadIdx.x < 64 && col < kp) {
+    if (threadIdx.x < 64 && c < kp) {
+        const int col = blockIdx.y * kp + c;
+        double last = 0.0;
+        if (jlast >= 0 && S[S_ACTIVE * ks + col] > 0.0)
+            last = S[S_FROZEN * ks + col] > 0.0 ? S[S_RZPREV * ks + col] : rz_at(jlast, ks, col, S, acc);
         const bool act = S[S_SCALE * ks + col] > 0.0 && rrn > 0.0;
         const double q = S[S_RATIO * ks + col];
-        const double prev = S[S_RZPREV * ks + col] * q * q;
+        const double prev = last * q * q;
         S[S_ACTIVE * ks + col] = act ? 1.0 : 0.0;
         S[S_BETA * ks + col] = (act && prev > 0.0) ? rzn / prev : 0.0;
         S[S_RZ * ks + col] = act ? rzn : 0.0;
-        S[S_ALPHA * ks + col] = 0.0;
+        S[S_RZPREV * ks + col] = 0.0;
+        S[S_FROZEN * ks + col] = 0.0;
         S[S_TRIG * ks + col] = fmax(delta2 * rrn, S[S_TGT * ks + col]);
+        for (int a = 0; a < A_N; ++a) acc[(size_t)a * ks + col] = 0.0;
     }
 }
 
@@ -829,28 +866,26 @@ void plan_grid(Plan& p, const std::vector<int>& grids) {
 
 template <class T, int LPR>
 struct Kern {
-    static void dir_spmv(const Plan& P, const Seg& G, hipStream_t st, const int* rp, const Entries<T>& M,
-                         const double* S, const double* beta, const double* active, const T* z, const T* pold,
-                         T* pnew, double* acc) {
-        k_dir_spmv<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.kp, G, rp, M, S, beta, active, z, pold, pnew,
-                                                                 acc);
+    static void dir_spmv(const Plan& P, const Seg& G, hipStream_t st, int it, const int* rp, const Entries<T>& M,
+                         double* S, const T* z, const T* pold, T* pnew, double* acc) {
+        k_dir_spmv<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.kp, G, it, rp, M, S, z, pold, pnew, acc);
     }
-    static void update(const Plan& P, const Seg& G, hipStream_t st, const int* rp, const Entries<T>& M,
+    static void update(const Plan& P, const Seg& G, hipStream_t st, int it, const int* rp, const Entries<T>& M,
                        const T* dinv, const T* diag, const double* S, const T* p, T* x, T* z, double* acc) {
-        k_cg_update<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.kp, G, rp, M, dinv, diag, S, p, x, z, acc);
+        k_cg_update<T, LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.kp, G, it, rp, M, dinv, diag, S, p, x, z, acc);
     }
     // 2D tensor-grid kernels: FP32, at most 64 columns (16 lanes) per column tile
-    static void dir_spmv_g(const Plan& P, hipStream_t st, const T* coef, const double* S, const double* beta,
-                           const double* active, const T* z, const T* pold, T* pnew, double* acc) {
+    static void dir_spmv_g(const Plan& P, hipStream_t st, int it, const T* coef, double* S, const T* z, const T* pold,
+                           T* pnew, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_dir_spmv_t2<LPR><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, coef, S, beta, active,
-                                                                       z, pold, pnew, acc);
+            k_dir_spmv_t2<LPR><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, S, z, pold,
+                                                                       pnew, acc);
     }
-    static void update_g(const Plan& P, hipStream_t st, const T* coef, const T* dinv, const T* diag, const double* S,
-                         const T* p, T* x, T* z, double* acc) {
+    static void update_g(const Plan& P, hipStream_t st, int it, const T* coef, const T* dinv, const T* diag,
+                         const double* S, const T* p, T* x, T* z, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_cg_update_t2<LPR><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, coef, dinv, diag, S,
-                                                                        p, x, z, acc);
+            k_cg_update_t2<LPR><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, dinv, diag,
+                                                                        S, p, x, z, acc);
     }
     static void rescale(const Plan& P, const Seg& G, hipStream_t st, const double* scale, const double* ratio,
                         const double* dinv64, const T* diag, const double* r64, T* z, T* p, double* p0, double* p1) {
@@ -956,8 +991,6 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_TRY(w.part0.ensure((size_t)std::max(P.nblk, nbi) * kp * 8));
     SFH_TRY(w.part1.ensure((size_t)std::max(P.nblk, nbi) * kp * 8));
     SFH_TRY(w.scal.ensure((size_t)S_NSLOT * ks * 8));
-    const int nred = P.nred;
-    SFH_TRY(w.nact.ensure(sizeof(int) * 2 * nred));
     SFH_TRY(w.acc.ensure((size_t)A_N * ks * 8));
     // partition tables: brow (nblk + 1), bseg (nblk), sb (nseg + 1), srow (nseg + 1), gtab
     const size_t ntab = P.brow.size() + P.bseg.size() + P.sb.size() + P.srow.size() + P.gtab.size();
@@ -1011,7 +1044,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_CHECK_HIP(hipMemsetAsync(w.acc.p, 0, (size_t)A_N * ks * 8, st));
     double* acc = w.acc.as<double>();
     sfh_ctx* cx = s->ctx;
-    const int nsmp = (!tl_env && cx->ksample > 0 && k >= cx->kmin_rhs) ? (int)cx->kev.size() / 4 : 0;
+    const int nsmp = (!tl_env && cx->ksample > 0 && k >= cx->kmin_rhs) ? (int)cx->kev.size() / 3 : 0;
     int ismp = 0;
     Kern<T, LPR>::resid(P, G, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
     k_red_norm<<<gred, kRedB, 0, st>>>(kp, d_sb, p0, S + S_BN2 * ks);
@@ -1026,8 +1059,9 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     int updates = 0, iters = 0;
     double maxrel = 0.0;
     bool done = false;
-    std::vector<int> cnt(2 * nred);
+    std::vector<double> hs(5 * (size_t)ks);   // ACTIVE, FROZEN, SCALE, TRIG and the latest r.r per column
     const int check = std::max(1, o.check_every);
+    int jlast = -1;                           // last iteration of the previous cycle
     for (;;) {
         // residual replacement: decide per column, rescale, keep directions
         int nact = 0;
@@ -1050,38 +1084,51 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * ks, tgt.data(), ks * 8, hipMemcpyHostToDevice, st));
         Kern<T, LPR>::rescale(P, G, st, S + S_SCALE * ks, S + S_RATIO * ks, s->dinv64.as<double>(), diag,
                               w.r64.as<double>(), z, pa, p0, p1);
-        k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb, p0, p1, S, delta2);
+        k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb, p0, p1, S, acc, delta2, jlast);
         SFH_LAUNCH_CHECK();
         ++updates;
-        // FP32 CG until some column is due for replacement
-        for (int it = 1;; ++it) {
+        // FP32 CG until some column is due for replacement: two kernels per
+        // iteration, the step lengths and directions from the rotating slots
+        int j = 0;
+        for (;; ++j) {
             const bool smp = ismp < nsmp && iters % cx->ksample == 0;
-            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp], st));
+            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[3 * ismp], st));
             if (P.grid)
-                Kern<T, LPR>::dir_spmv_g(P, st, coef, S, S + S_BETA * ks, S + S_ACTIVE * ks, z, pa, pb, acc);
+                Kern<T, LPR>::dir_spmv_g(P, st, j, coef, S, z, pa, pb, acc);
             else
-                Kern<T, LPR>::dir_spmv(P, G, st, rp, M, S, S + S_BETA * ks, S + S_ACTIVE * ks, z, pa, pb, acc);
-            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 1], st));
+                Kern<T, LPR>::dir_spmv(P, G, st, j, rp, M, S, z, pa, pb, acc);
+            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[3 * ismp + 1], st));
             if (P.grid)
-                Kern<T, LPR>::update_g(P, st, coef, dinv, diag, S, pb, x, z, acc);
+                Kern<T, LPR>::update_g(P, st, j, coef, dinv, diag, S, pb, x, z, acc);
             else
-                Kern<T, LPR>::update(P, G, st, rp, M, dinv, diag, S, pb, x, z, acc);
-            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 2], st));
-            k_red_beta<<<nred, 256, 0, st>>>(ks, acc, S, w.nact.as<int>());
-            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 3], st));
+                Kern<T, LPR>::update(P, G, st, j, rp, M, dinv, diag, S, pb, x, z, acc);
+            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[3 * ismp + 2], st));
             ismp += smp;
             std::swap(pa, pb);
             ++iters;
             if (iters >= o.max_inner) break;
-            if (it % check == 0) {
+            if ((j + 1) % check == 0) {
+                // live columns left, and columns waiting for replacement (frozen
+                // now or by the r.r of this iteration, or inactive and unconverged)
                 SFH_LAUNCH_CHECK();
-                SFH_CHECK_HIP(hipMemcpyAsync(cnt.data(), w.nact.p, sizeof(int) * 2 * nred, hipMemcpyDeviceToHost, st));
+                const int slot[4] = {S_ACTIVE, S_FROZEN, S_SCALE, S_TRIG};
+                for (int q = 0; q < 4; ++q)
+                    SFH_CHECK_HIP(hipMemcpyAsync(hs.data() + (size_t)q * ks, S + (size_t)slot[q] * ks, ks * 8,
+                                                 hipMemcpyDeviceToHost, st));
+                SFH_CHECK_HIP(hipMemcpyAsync(hs.data() + 4 * (size_t)ks, acc + (size_t)(A_RR + j % 3) * ks, ks * 8,
+                                             hipMemcpyDeviceToHost, st));
                 SFH_CHECK_HIP(hipStreamSynchronize(st));
                 int na = 0, nu = 0;
-                for (int b = 0; b < nred; ++b) { na += cnt[2 * b]; nu += cnt[2 * b + 1]; }
+                for (int c = 0; c < ks; ++c) {
+                    const bool act = hs[c] > 0.0;
+                    const bool fz = hs[ks + c] > 0.0 || !(hs[4 * (size_t)ks + c] > hs[3 * (size_t)ks + c]);
+                    if (act && !fz) ++na;
+                    else if (act || hs[2 * (size_t)ks + c] > 0.0) ++nu;
+                }
                 if (na == 0 || nu > 0) break;
             }
         }
+        jlast = j;
         // flush the FP32 correction, true residual in FP64
         k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow, S + S_SCALE * ks,
                                                                  x, w.x64.as<double>());
@@ -1102,10 +1149,10 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     for (int i = 0; i < ismp; ++i) {
-        for (int j = 0; j < 3; ++j) {
+        for (int jj = 0; jj < 2; ++jj) {
             float f = 0.f;
-            SFH_CHECK_HIP(hipEventElapsedTime(&f, cx->kev[4 * i + j], cx->kev[4 * i + j + 1]));
-            cx->ksum[j] += f;
+            SFH_CHECK_HIP(hipEventElapsedTime(&f, cx->kev[3 * i + jj], cx->kev[3 * i + jj + 1]));
+            cx->ksum[jj] += f;
         }
         ++cx->kcount;
     }

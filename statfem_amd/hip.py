@@ -193,10 +193,10 @@ class Context:
         _check(lib().sfh_kernel_sampling(self.h, every, max_samples, min_rhs))
 
     def kernel_times(self):
-        ms = np.zeros(3)
+        ms = np.zeros(2)
         n = C.c_longlong()
         _check(lib().sfh_kernel_times(self.h, _p(ms), C.byref(n)))
-        return {"dir_spmv": float(ms[0]), "cg_update": float(ms[1]), "beta": float(ms[2]), "samples": n.value}
+        return {"dir_spmv": float(ms[0]), "cg_update": float(ms[1]), "samples": n.value}
 
     def timer_start(self):
         _check(lib().sfh_timer_start(self.h))
