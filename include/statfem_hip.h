@@ -160,7 +160,8 @@ typedef struct {
     int rhs_tile;        /* >0: solve the right-hand sides in column tiles of this width (0: all at once) */
     int layout;          /* 0 (default): auto, 1: CSR, 2: 9-coefficient stencil with LDS-tiled kernels on
                             declared 2D tensor grids (FP32, at most 64 right-hand sides per tile);
-                            auto uses 2 where it applies, else CSR */
+                            3: 3 x 3 node blocks for vector-DOF meshes (FP32, one problem);
+                            auto uses 2 or 3 where they apply, else CSR */
 } sfh_solve_opts;
 
 typedef struct {
@@ -169,7 +170,7 @@ typedef struct {
     double max_relres;    /* max over RHS of ||b - A x|| / ||b|| in FP64 */
     double ms;            /* device time */
     int converged;
-    int layout;           /* operator layout the iteration kernels used: 1 CSR, 2 2D stencil */
+    int layout;           /* operator layout the iteration kernels used: 1 CSR, 2 2D stencil, 3 3x3 blocks */
 } sfh_solve_stats;
 
 void sfh_solve_opts_default(sfh_solve_opts* o);

@@ -110,6 +110,7 @@ struct sfh_system {
     sfh::DevBuf vals, vals32, F, dinv32, dinv64;
     sfh::DevBuf diag32, diag64;  // A_ii (the PCG keeps z = D^-1 r and needs both)
     sfh::DevBuf cv32;            // FP32 operator as packed (column, value bits) pairs
+    sfh::DevBuf bsr_val;         // vector DOFs: 9 FP32 values per node pair (node-level pattern)
     // systems built from triplets: sorted-to-insertion permutation and the
     // first sorted position of every merged entry (nnz + 1)
     sfh::DevBuf coo_perm, coo_start;

@@ -589,6 +589,179 @@ __global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gta
     block_accumulate<LPR>(t, kp, bb, red, acc + (A_RR + it % 3) * ks + t.so);
 }
 
+// ------------------------------------------------ vector DOFs: 3 x 3 blocks
+// Interleaved vector DOFs (3 per node) as a node-level block operator: for node
+// pair (i, j) one column index and 9 FP32 values, a(row comp, col comp). A lane
+// group owns a node and forms its 3 rows at once, so each neighbour node's 3
+// DOFs are gathered once instead of once per row.
+__global__ void k_pack_bsr3(int nnodes, const int* __restrict__ rpn, const float* __restrict__ va32,
+                            float* __restrict__ bval) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    const int b = rpn[i], len = rpn[i + 1] - b;
+    for (int q = 0; q < len; ++q)
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) bval[(size_t)(b + q) * 9 + a * 3 + c] = va32[(size_t)9 * b + a * 3 * len + 3 * q + c];
+}
+
+struct NodeTile {
+    int lane, grp, ngrp, col0, n0, n1;
+};
+
+template <int LPR>
+__device__ inline NodeTile node_tile(int nnodes, int npb) {
+    NodeTile t;
+    t.lane = threadIdx.x % LPR;
+    t.grp = threadIdx.x / LPR;
+    t.ngrp = blockDim.x / LPR;
+    t.col0 = (blockIdx.y * LPR + t.lane) * 4;
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    t.n0 = bid * npb;
+    t.n1 = min(nnodes, t.n0 + npb);
+    return t;
+}
+
+template <int LPR>
+__device__ inline Tile as_row_tile(const NodeTile& n) {
+    Tile t;
+    t.lane = n.lane; t.grp = n.grp; t.ngrp = n.ngrp; t.col0 = n.col0;
+    t.bid = xcd_block(blockIdx.x, gridDim.x);
+    t.r0 = 3 * n.n0; t.r1 = 3 * n.n1; t.so = 0;
+    return t;
+}
+
+__device__ inline void fma4(V4<float>& q, float a, const V4<float>& v) {
+    q.x += a * v.x; q.y += a * v.y; q.z += a * v.z; q.w += a * v.w;
+}
+
+// Direction update as a streaming pass: p_new = m z + b p_old on every row
+// (the 27-node neighbourhoods of 3D meshes outgrow the L2, so gathering both
+// z and p_old at every neighbour is replaced by one gathered vector below).
+template <int LPR>
+__global__ __launch_bounds__(kB) void k_pdir_stream(int kp, int nrow, int rpb, int it, double* __restrict__ S,
+                                                    const float* __restrict__ z, const float* __restrict__ pold,
+                                                    float* __restrict__ pnew, double* __restrict__ acc) {
+    Tile t;
+    t.lane = threadIdx.x % LPR;
+    t.grp = threadIdx.x / LPR;
+    t.ngrp = blockDim.x / LPR;
+    t.col0 = (blockIdx.y * LPR + t.lane) * 4;
+    t.bid = xcd_block(blockIdx.x, gridDim.x);
+    t.r0 = t.bid * rpb;
+    t.r1 = min(nrow, t.r0 + rpb);
+    t.so = 0;
+    clear_slots(acc, A_RZ + it % 3, 1, kp);
+    clear_slots(acc, A_RR + it % 3, 1, kp);
+    if (block_done(t, kp, kp, S)) return;
+    __shared__ float cb[4 * LPR], cm[4 * LPR];
+    shared_dir_coef<float, LPR>(it, kp, kp, 0, S, acc, cb, cm);
+    if (t.col0 >= kp) return;
+    const int lc = t.lane * 4;
+    const V4<float> b = {cb[lc], cb[lc + 1], cb[lc + 2], cb[lc + 3]};
+    const V4<float> m = {cm[lc], cm[lc + 1], cm[lc + 2], cm[lc + 3]};
+    for (int r = t.r0 + t.grp; r < t.r1; r += t.ngrp) {
+        const size_t o = (size_t)r * kp + t.col0;
+        V4<float> v;
+        pdir4(ld4(z + o), m, b, ld4(pold + o), v);
+        st4(pnew + o, v);
+    }
+}
+
+// q = A p (3 x 3 blocks, one gathered vector), acc += p . q
+template <int LPR>
+__global__ __launch_bounds__(kB) void k_spmv_pq_b3(int kp, int nnodes, int npb, int it, const int* __restrict__ rpn,
+                                                   const int* __restrict__ cin, const float* __restrict__ bval,
+                                                   const double* __restrict__ S, const float* __restrict__ p,
+                                                   double* __restrict__ acc) {
+    __shared__ double red[kB * 4];
+    const NodeTile n = node_tile<LPR>(nnodes, npb);
+    const Tile t = as_row_tile<LPR>(n);
+    if (block_done(t, kp, kp, S)) return;
+    V4<float> pq = {0.f, 0.f, 0.f, 0.f};
+    if (n.col0 < kp) {
+        for (int i0 = n.n0 + n.grp; i0 < n.n1; i0 += n.ngrp) {
+            const int i = uniform_row<LPR>(i0);
+            V4<float> q[3] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+            for (int e = rpn[i]; e < rpn[i + 1]; ++e) {
+                const int j = cin[e];
+                const float* A = bval + (size_t)e * 9;
+                V4<float> pj[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) pj[c] = ld4(p + (size_t)(3 * j + c) * kp + n.col0);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    fma4(q[a], A[a * 3], pj[0]);
+                    fma4(q[a], A[a * 3 + 1], pj[1]);
+                    fma4(q[a], A[a * 3 + 2], pj[2]);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const V4<float> pi = ld4(p + (size_t)(3 * i + a) * kp + n.col0);
+                pq.x += pi.x * q[a].x; pq.y += pi.y * q[a].y; pq.z += pi.z * q[a].z; pq.w += pi.w * q[a].w;
+            }
+        }
+    }
+    const V4<double> ad = {(double)pq.x, (double)pq.y, (double)pq.z, (double)pq.w};
+    block_accumulate<LPR>(t, kp, ad, red, acc + (A_PQ + (it & 1)) * kp);
+}
+
+template <int LPR>
+__global__ __launch_bounds__(kB) void k_cg_update_b3(int kp, int nnodes, int npb, int it, const int* __restrict__ rpn,
+                                                     const int* __restrict__ cin, const float* __restrict__ bval,
+                                                     const float* __restrict__ dinv, const float* __restrict__ diag,
+                                                     const double* __restrict__ S, const float* __restrict__ p,
+                                                     float* __restrict__ x, float* __restrict__ z,
+                                                     double* __restrict__ acc) {
+    __shared__ double red[kB * 4];
+    const NodeTile n = node_tile<LPR>(nnodes, npb);
+    const Tile t = as_row_tile<LPR>(n);
+    clear_slots(acc, A_PQ + ((it + 1) & 1), 1, kp);
+    if (block_done(t, kp, kp, S)) return;
+    __shared__ float ca[4 * LPR];
+    shared_alpha<float, LPR>(it, kp, kp, 0, S, acc, ca);
+    V4<float> rz = {0.f, 0.f, 0.f, 0.f}, rr = {0.f, 0.f, 0.f, 0.f};
+    if (n.col0 < kp) {
+        const V4<float> al = {ca[n.lane * 4], ca[n.lane * 4 + 1], ca[n.lane * 4 + 2], ca[n.lane * 4 + 3]};
+        for (int i0 = n.n0 + n.grp; i0 < n.n1; i0 += n.ngrp) {
+            const int i = uniform_row<LPR>(i0);
+            V4<float> q[3] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+            for (int e = rpn[i]; e < rpn[i + 1]; ++e) {
+                const int j = cin[e];
+                const float* A = bval + (size_t)e * 9;
+                V4<float> pj[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) pj[c] = ld4(p + (size_t)(3 * j + c) * kp + n.col0);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    fma4(q[a], A[a * 3], pj[0]);
+                    fma4(q[a], A[a * 3 + 1], pj[1]);
+                    fma4(q[a], A[a * 3 + 2], pj[2]);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int r = 3 * i + a;
+                const size_t o = (size_t)r * kp + n.col0;
+                const V4<float> pi = ld4(p + o);
+                V4<float> xv = ld4(x + o), zv = ld4(z + o);
+                const float dd = dinv[r], g = diag[r];
+                xv.x += al.x * pi.x; xv.y += al.y * pi.y; xv.z += al.z * pi.z; xv.w += al.w * pi.w;
+                zv.x -= al.x * dd * q[a].x; zv.y -= al.y * dd * q[a].y; zv.z -= al.z * dd * q[a].z;
+                zv.w -= al.w * dd * q[a].w;
+                st4(x + o, xv);
+                st4(z + o, zv);
+                res_dots(zv, g, rz, rr);
+            }
+        }
+    }
+    V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
+    block_accumulate<LPR>(t, kp, a, red, acc + (A_RZ + it % 3) * kp);
+    __syncthreads();
+    V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
+    block_accumulate<LPR>(t, kp, bb, red, acc + (A_RR + it % 3) * kp);
+}
+
 // CSR -> DIA-9. Every entry must couple a grid neighbour of its row's node;
 // anything else (err) keeps the solver on the CSR kernels.
 __global__ void k_build_dia9(int base, int W, int H, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -807,6 +980,8 @@ struct Plan {
     // 2D tensor-grid path
     int grid, W, H, L, nsegx, gblk;
     const int* d_gtab = nullptr;
+    // 3 x 3 block path (vector DOFs): node ranges of npb nodes per work-group
+    int bsr = 0, nnodes = 0, npb = 0, bblk = 0;
 };
 
 Plan make_plan(int n, int k, int num_cu, int nseg = 1, const int* seg_rows = nullptr) {
@@ -840,6 +1015,16 @@ Plan make_plan(int n, int k, int num_cu, int nseg = 1, const int* seg_rows = nul
     p.grid = 0;
     p.W = p.H = p.L = p.nsegx = p.gblk = 0;
     return p;
+}
+
+void plan_bsr3(Plan& p, int nnodes, int num_cu) {
+    p.bsr = 1;
+    p.nnodes = nnodes;
+    const int nodes_per_pass = kB / p.lpr;
+    const int target = std::max(8, 4 * num_cu / p.ntile);
+    p.npb = std::max(nodes_per_pass, (nnodes + target - 1) / target);
+    p.bblk = (nnodes + p.npb - 1) / p.npb;
+    p.bblk = (p.bblk + 7) / 8 * 8;
 }
 
 // segments of L nodes along x, one per lane group
@@ -886,6 +1071,20 @@ struct Kern {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
             k_cg_update_t2<LPR><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, dinv, diag,
                                                                         S, p, x, z, acc);
+    }
+    static void dir_spmv_b(const Plan& P, hipStream_t st, int it, const int* rpn, const int* cin, const float* bval,
+                           double* S, const T* z, const T* pold, T* pnew, double* acc) {
+        if constexpr (std::is_same<T, float>::value) {
+            k_pdir_stream<LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(P.kp, P.n, P.rpb, it, S, z, pold, pnew, acc);
+            k_spmv_pq_b3<LPR><<<dim3(P.bblk, P.ntile), kB, 0, st>>>(P.kp, P.nnodes, P.npb, it, rpn, cin, bval, S,
+                                                                    pnew, acc);
+        }
+    }
+    static void update_b(const Plan& P, hipStream_t st, int it, const int* rpn, const int* cin, const float* bval,
+                         const T* dinv, const T* diag, const double* S, const T* p, T* x, T* z, double* acc) {
+        if constexpr (std::is_same<T, float>::value)
+            k_cg_update_b3<LPR><<<dim3(P.bblk, P.ntile), kB, 0, st>>>(P.kp, P.nnodes, P.npb, it, rpn, cin, bval, dinv,
+                                                                      diag, S, p, x, z, acc);
     }
     static void rescale(const Plan& P, const Seg& G, hipStream_t st, const double* scale, const double* ratio,
                         const double* dinv64, const T* diag, const double* r64, T* z, T* p, double* p0, double* p1) {
@@ -934,6 +1133,12 @@ int prepare_operator_on(sfh_system* s, hipStream_t st, const std::vector<int>& g
         k_pack_entries<<<(unsigned)((s->nnz + 255) / 256), 256, 0, st>>>(s->nnz, s->colind.as<int>(),
                                                                          s->vals32.as<float>(), s->cv32.as<int2>());
         SFH_LAUNCH_CHECK();
+        if (s->dofpn == 3 && s->kind != 0) {
+            SFH_TRY(s->bsr_val.ensure(sizeof(float) * 9 * (size_t)s->nnz_node));
+            k_pack_bsr3<<<(s->nnodes + 255) / 256, 256, 0, st>>>(s->nnodes, s->rowptr_node.as<int>(),
+                                                                 s->vals32.as<float>(), s->bsr_val.as<float>());
+            SFH_LAUNCH_CHECK();
+        }
         s->dirty = false;
     }
     if (!grids.empty() && s->dia_state == 0) {
@@ -980,6 +1185,10 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                           P.lpr >= 4 && P.lpr <= 16 && P.ntile == 1;
     SFH_TRY(prepare_operator_on(s, st, use_grid ? grids : std::vector<int>()));
     if (use_grid && s->dia_state == 1) plan_grid(P, grids);
+    // vector DOFs of a mesh: 3 x 3 block layout (one problem, FP32)
+    if (!P.grid && ns == 1 && s->dofpn == 3 && s->kind != 0 && (o.layout == 3 || o.layout == 0) &&
+        std::is_same<T, float>::value && P.ntile == 1)
+        plan_bsr3(P, s->nnodes, s->ctx->num_cu);
     const int nbi = P.grid ? P.gblk : P.nblk;   // partial rows written by the iteration kernels
     SFH_TRY(w.b64.ensure(nk * 8));
     SFH_TRY(w.x64.ensure(nk * 8));
@@ -1095,11 +1304,17 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[3 * ismp], st));
             if (P.grid)
                 Kern<T, LPR>::dir_spmv_g(P, st, j, coef, S, z, pa, pb, acc);
+            else if (P.bsr)
+                Kern<T, LPR>::dir_spmv_b(P, st, j, s->rowptr_node.as<int>(), s->colind_node.as<int>(),
+                                         s->bsr_val.as<float>(), S, z, pa, pb, acc);
             else
                 Kern<T, LPR>::dir_spmv(P, G, st, j, rp, M, S, z, pa, pb, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[3 * ismp + 1], st));
             if (P.grid)
                 Kern<T, LPR>::update_g(P, st, j, coef, dinv, diag, S, pb, x, z, acc);
+            else if (P.bsr)
+                Kern<T, LPR>::update_b(P, st, j, s->rowptr_node.as<int>(), s->colind_node.as<int>(),
+                                       s->bsr_val.as<float>(), dinv, diag, S, pb, x, z, acc);
             else
                 Kern<T, LPR>::update(P, G, st, j, rp, M, dinv, diag, S, pb, x, z, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[3 * ismp + 2], st));
@@ -1162,7 +1377,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         stats->max_relres = maxrel;
         stats->ms = ms;
         stats->converged = done ? 1 : 0;
-        stats->layout = P.grid ? 2 : 1;
+        stats->layout = P.grid ? 2 : (P.bsr ? 3 : 1);
     }
     if (!done) {
         set_error("solve did not reach tol %.3g (max rel. residual %.3g after %d iterations, %d replacements)",
@@ -1206,7 +1421,7 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
     if (o.max_outer <= 0) o.max_outer = 500;
     if (o.max_inner <= 0) o.max_inner = 50000;
     if (o.check_every <= 0) o.check_every = 32;
-    if (o.layout < 0 || o.layout > 2) o.layout = 0;
+    if (o.layout < 0 || o.layout > 3) o.layout = 0;
     // column tiles: each tile is an independent multi-RHS solve whose working
     // set can stay in the last-level cache
     if (o.rhs_tile > 0 && o.rhs_tile < k) {
