@@ -117,15 +117,57 @@ def gpu_run(args):
     for k in algo:
         per[k]["traffic"] = profiled_traffic(kname[k].format(lpr=lpr))
     asm_bytes = m.nnodes * 2 * 8 + m.nel * 4 * 4 + m.nel * 8 + nnz * 8 + nnz * 4
+    dense = dense_stages(ctx, s, ns, cfg)
     res = {
         "ms_per_step": ms_step, "value": value, "dev_ms_per_step": dev_ms / args.steps,
         "inner_iters": stats[-1]["inner_iters"], "outer": stats[-1]["outer"],
         "max_relres": max(st["max_relres"] for st in stats),
         "kernel": dom, "kernels": per, "layout": {1: "csr", 2: "stencil9"}.get(stats[-1]["layout"], "?"),
         "assembly": {"ms": asm_ms, "GBs": asm_bytes / (asm_ms * 1e-3) / 1e9, "bytes": asm_bytes},
+        "dense": dense,
         "N": N, "nnz": nnz,
     }
     return res, per[dom]
+
+
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def dense_stages(ctx, s, k, cfg, reps=5):
+    """FP64-MFMA stages of the step at its sizes, timed alone on the library
+    stream: the Gram product C_u = R V (N x k by N x k) and the separable
+    forcing covariance V = C_f R^T (one batched GEMM per grid axis)."""
+    from statfem_amd import hip
+    N = s.ndof
+    X = ctx.empty((N, k))
+    V = ctx.empty((N, k))
+    X.zero()
+    V.zero()
+    hip.gram(ctx, X, V)
+    ctx.timer_start()
+    for _ in range(reps):
+        hip.gram(ctx, X, V)
+    t_gram = ctx.timer_stop() / reps
+    cv = s.lumped_mass(2)
+
+    def cf():
+        hip._check(hip.lib().sfh_covforcing_apply(s.h, cv.ptr, cfg["sigma_f"], cfg["ell_f"], k, X.ptr, V.ptr, 2))
+
+    cf()
+    ctx.timer_start()
+    for _ in range(reps):
+        cf()
+    t_cf = ctx.timer_stop() / reps
+    nx, ny = (len(a) for a in s.mesh.grid)
+    f_gram = 2.0 * N * k * k
+    f_cf = 2.0 * k * nx * ny * (nx + ny)
+    out = {}
+    for name, f, t in (("gram", f_gram, t_gram), ("covforcing_separable", f_cf, t_cf)):
+        tf = f / (t * 1e-3) / 1e12
+        out[name] = {"ms": t, "gflop": f / 1e9, "tflops": tf, "peak_tflops": FP64_MFMA_PEAK_TFLOPS,
+                     "frac": tf / FP64_MFMA_PEAK_TFLOPS, "bound": "mfma"}
+    log(f"dense stages: {out}")
+    return out
 
 
 def c5_run(args):

@@ -315,6 +315,131 @@ __global__ __launch_bounds__(256) void k_loglik(int ns, int no, int dim, const d
     }
 }
 
+// Register-resident form of the same log-likelihood for n_s + n_o <= 32 NB.
+// One 1024-thread work-group per hyperparameter set; thread (tr, tc) of the
+// 32 x 32 grid holds the lower-triangle entries (i, j), i = tr + 32 ii,
+// j = tc + 32 jj, jj <= ii, of the augmented matrix
+//     [ K_s + rho^2 Cu      .  ]    (n_s x n_s)
+//     [ (y_m - rho mean)^T  0  ]    (n_o data rows)
+// built in place. A right-looking Cholesky over the first n_s columns leaves
+// -|L^-1 d_m|^2 on the data rows' diagonal (the Schur complement), so the
+// factor, the log-determinant and all quadratic forms come from one pass with
+// no global-memory work matrix. Step k reads the current column k from an LDS
+// buffer (every thread forms the pivot's square root itself), updates its
+// entries, and the owners of column k+1 publish it into the other buffer: one
+// barrier per step.
+template <int NB>
+__global__ __launch_bounds__(1024) void k_loglik_reg(int ns, int no, int dim, const double* __restrict__ xs,
+                                                     const double* __restrict__ Cu, const double* __restrict__ mean,
+                                                     const double* __restrict__ Y, double eps,
+                                                     const double* __restrict__ theta, double* __restrict__ out,
+                                                     int* __restrict__ status, double* __restrict__ logdet_out) {
+    __shared__ double colb[2][32 * NB];
+    __shared__ double piv[32 * NB];
+    __shared__ double red[1024 / 64];
+    const int b = blockIdx.x;
+    const int tr = threadIdx.x >> 5, tc = threadIdx.x & 31;
+    const int na = ns + no;
+    const double rho = theta[3 * b], sd = theta[3 * b + 1], ld = theta[3 * b + 2];
+    double a[NB][NB];
+#pragma unroll
+    for (int ii = 0; ii < NB; ++ii)
+#pragma unroll
+        for (int jj = 0; jj <= ii; ++jj) {
+            const int i = tr + 32 * ii, j = tc + 32 * jj;
+            double v = 0.0;
+            if (i < na && j <= i) {
+                if (i < ns) {
+                    v = se_kernel(dim, xs + (size_t)i * dim, xs + (size_t)j * dim, sd, ld) + rho * rho * Cu[(size_t)i * ns + j];
+                    if (i == j) v += eps * eps;
+                } else if (j < ns) {
+                    v = Y[(size_t)(i - ns) * ns + j] - rho * mean[j];
+                }
+            }
+            a[ii][jj] = v;
+        }
+    // publish column 0
+    if (tc == 0) {
+#pragma unroll
+        for (int ii = 0; ii < NB; ++ii) {
+            const int i = tr + 32 * ii;
+            if (i < na) colb[0][i] = a[ii][0];
+        }
+    }
+    __syncthreads();
+    bool bad = false;
+    for (int k = 0; k < ns; ++k) {
+        const double* cb = colb[k & 1];
+        const double d = cb[k];
+        if (!(d > 0.0)) { bad = true; break; }   // the same value in every thread
+        const double l = sqrt(d);
+        const double inv = 1.0 / l;
+        if (threadIdx.x == 0) piv[k] = l;
+        double cj[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int j = tc + 32 * q;
+            cj[q] = (j > k && j < na) ? cb[j] * inv : 0.0;
+        }
+#pragma unroll
+        for (int ii = 0; ii < NB; ++ii) {
+            const int i = tr + 32 * ii;
+            const double ci = (i > k && i < na) ? cb[i] * inv : 0.0;
+#pragma unroll
+            for (int jj = 0; jj <= ii; ++jj) a[ii][jj] -= ci * cj[jj];
+        }
+        // owners of column k + 1 publish it (entries below and on the diagonal)
+        const int kn = k + 1;
+        if (kn < ns && tc == (kn & 31)) {
+            double* nb = colb[kn & 1];
+#pragma unroll
+            for (int ii = 0; ii < NB; ++ii)
+#pragma unroll
+                for (int jj = 0; jj <= ii; ++jj)
+                    if (jj == (kn >> 5)) {
+                        const int i = tr + 32 * ii;
+                        if (i >= kn && i < na) nb[i] = a[ii][jj];
+                    }
+        }
+        __syncthreads();
+    }
+    if (bad) {
+        if (threadIdx.x == 0) {
+            out[b] = -INFINITY;
+            status[b] = SFH_ENOTSPD;
+            if (logdet_out) logdet_out[b] = NAN;
+        }
+        return;
+    }
+    // sum of log pivots (fixed order: per-thread strided sums, then per wave)
+    double ls = 0.0;
+    for (int k = threadIdx.x; k < ns; k += 1024) ls += log(piv[k]);
+    for (int off = 32; off > 0; off >>= 1) ls += __shfl_xor(ls, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ls;
+    // data rows: diagonal entries hold -|L^-1 d_m|^2
+#pragma unroll
+    for (int ii = 0; ii < NB; ++ii) {
+        const int i = tr + 32 * ii;
+        if (tr == tc && i >= ns && i < na) colb[0][i - ns] = a[ii][ii];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double slog = 0.0;
+        for (int w = 0; w < 1024 / 64; ++w) slog += red[w];
+        double tmp3 = 0.0;
+        int badq = 0;
+        for (int m = 0; m < no; ++m) {
+            const double c = 0.5 * colb[0][m];
+            if (!(c < 0.0)) badq = 1;
+            tmp3 += c;
+        }
+        const double tmp1 = -0.5 * ns * log(2. * M_PI);
+        out[b] = no * (tmp1 - slog) + tmp3;
+        status[b] = badq ? SFH_EQUADFORM : SFH_OK;
+        if (logdet_out) logdet_out[b] = 2.0 * slog;
+    }
+}
+
 // in-place SPD solve A X = B (A n x n lower used, B n x m), one work-group
 __global__ __launch_bounds__(256) void k_spd_solve(int n, double* A, int m, double* B, int* status) {
     extern __shared__ double col[];
@@ -594,6 +719,24 @@ int covforcing_grid(sfh_ctx* c, int d, const int gn_[3], const double* const ax[
 int loglik_batch(sfh_lik* L, int M, const double* theta, double* out, int* status, double* logdet) {
     hipStream_t st = L->ctx->stream;
     const int ns = L->ns, no = L->no;
+    const int na = ns + no;
+    if (na <= 224 && M > 0) {
+        // register-resident factorisation (sizes up to 224 augmented rows)
+        if (na <= 64)
+            k_loglik_reg<2><<<M, 1024, 0, st>>>(ns, no, L->dim, L->xs.as<double>(), L->Cu.as<double>(),
+                                                L->mean.as<double>(), L->Y.as<double>(), L->eps, theta, out, status,
+                                                logdet);
+        else if (na <= 128)
+            k_loglik_reg<4><<<M, 1024, 0, st>>>(ns, no, L->dim, L->xs.as<double>(), L->Cu.as<double>(),
+                                                L->mean.as<double>(), L->Y.as<double>(), L->eps, theta, out, status,
+                                                logdet);
+        else
+            k_loglik_reg<7><<<M, 1024, 0, st>>>(ns, no, L->dim, L->xs.as<double>(), L->Cu.as<double>(),
+                                                L->mean.as<double>(), L->Y.as<double>(), L->eps, theta, out, status,
+                                                logdet);
+        SFH_LAUNCH_CHECK();
+        return SFH_OK;
+    }
     SFH_TRY(L->work.ensure((size_t)M * ns * ns * 8));
     const size_t lds = sizeof(double) * ((size_t)ns + (size_t)ns * no + 256);
     if (lds > L->ctx->lds_per_block) { set_error("ns * no too large for LDS"); return SFH_EARG; }

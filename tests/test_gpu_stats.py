@@ -216,3 +216,30 @@ def test_precision_operator_matches_oracle(ctx, orc, dim):
     assert np.abs(Ys - orc.spmv(rp, ci, va, X)).max() <= 1e-13 * np.abs(Ys).max()
     with pytest.raises(hip.SfhError):
         s.precision_apply(mass, 0.8, kap, nu + 0.3, X)   # fractional beta
+
+
+@pytest.mark.parametrize("ns,no", [(20, 3), (100, 4), (150, 9), (215, 9), (240, 6)])
+def test_loglik_sizes_match_oracle(ctx, orc, ns, no):
+    """Every factorisation path of the batched log-likelihood (register-resident
+    up to 224 augmented rows, work-matrix form beyond) against the oracle,
+    with its log-determinant, over several hyperparameter sets."""
+    from statfem_amd import hip
+    rng = np.random.default_rng(ns)
+    xs = rng.uniform(size=(ns, 2))
+    A = rng.normal(size=(ns, ns)) * 0.05
+    Cu = A @ A.T + 1e-4 * np.eye(ns)
+    mean = rng.normal(size=ns)
+    Y = rng.normal(size=(no, ns))
+    lik = hip.LikelihoodKOHHip(ctx, xs, Cu, mean, Y, 0.01)
+    thetas = np.array([[1.0, 0.05, 0.3], [0.7, 0.2, 0.1], [1.4, 0.01, 0.6]])
+    ll, st = lik.giveLogLikelihood(thetas)
+    ld, _ = lik.giveLogDeterminant(thetas)
+    assert np.all(st == 0)
+    for t, l, d in zip(thetas, ll, ld):
+        Ks = orc.mismatch_cov(xs, t[1], t[2], 0.01)
+        ref = orc.loglik(Cu, mean, Y, Ks, t[0])
+        assert abs(l - ref) <= 1e-10 * abs(ref)
+        assert abs(d - np.linalg.slogdet(Ks + t[0] ** 2 * Cu)[1]) <= 1e-10 * abs(d)
+    bad = hip.LikelihoodKOHHip(ctx, xs, -np.eye(ns), mean, Y, 0.0)
+    llb, stb = bad.giveLogLikelihood([[1.0, 0.0, 0.3]])
+    assert stb[0] == -4 and np.isinf(llb[0])
