@@ -367,7 +367,7 @@ def main():
     ap.add_argument("--c5-chains", type=int, default=64)
     ap.add_argument("--c5-steps", type=int, default=20000)
     ap.add_argument("--c5-da-steps", type=int, default=2000)
-    ap.add_argument("--c5-sub", type=int, default=16)
+    ap.add_argument("--c5-sub", type=int, default=32)
     ap.add_argument("--copy-cal", action="store_true",
                     help="also run a 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
