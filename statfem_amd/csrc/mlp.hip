@@ -29,8 +29,13 @@ constexpr int HID = 64;
 constexpr int NP = 3 * HID + HID + HID * HID + HID + HID + 1;   // 4481
 constexpr int OFF_W1 = 0, OFF_B1 = 3 * HID, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + HID * HID,
               OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + HID;
-constexpr int SPB = 64;   // training samples per work-group
-constexpr int HS = HID + 1;   // padded LDS row stride (no bank conflicts across samples)
+constexpr int SPB = 64;        // training samples per work-group
+constexpr int RS = SPB / 16;   // samples per thread of the 16 x 16 gradient tile
+static_assert(SPB % 16 == 0, "the gradient tile spreads SPB samples over 16 thread rows");
+constexpr int LS = HID + 4;    // LDS row stride: 16-byte rows, successive rows on other banks
+constexpr int PSTRIDE = (NP + 1 + 3) / 4 * 4;   // per-work-group partial row (gradient, SSE), 16-byte aligned
+// small parameters staged in LDS by the gradient kernel
+constexpr int S_B1 = 3 * HID, S_B2 = S_B1 + HID, S_W3 = S_B2 + HID, S_B3 = S_W3 + HID, SMALL = S_B3 + 4;
 
 __host__ __device__ inline uint16_t f2bf(float f) {
     uint32_t u;
@@ -67,75 +72,202 @@ __device__ inline float mlp_forward(const float* W, const float x[3], float* h1,
     return y;
 }
 
-// per-work-group gradient of sum_s w (y_s - t_s)^2 and SSE: part[blk][NP + 1]
+// Per-work-group gradient of sum_s w (y_s - t_s)^2 and the SSE, written to
+// part[blk][PSTRIDE] in parameter order (SSE at index NP). The three 64-wide
+// products per sample block (layer-2 forward, layer-2 backward, layer-2 weight
+// gradient) run as register-tiled FP32 products over LDS: the 256 threads form
+// a 16 x 16 grid, thread (r, c) owning samples r*RS .. r*RS+RS-1 and hidden
+// units 4c .. 4c+3 (rows 4r .. 4r+3 of the weight gradient). Weight tiles are
+// read as 16-byte vectors that are contiguous across c; activation rows are
+// shared by the 16 threads of one r. Every sum runs in a fixed order.
 __global__ __launch_bounds__(256) void k_mlp_grad(int n, const float* __restrict__ X, const float* __restrict__ T,
                                                   float w, const uint16_t* __restrict__ bf, float* __restrict__ part) {
-    extern __shared__ float sh[];
-    float* W = sh;                       // NP
-    float* Sh1 = W + NP;                 // SPB x HS
-    float* Sh2 = Sh1 + SPB * HS;
-    float* Sd2 = Sh2 + SPB * HS;         // dL/da2
-    float* Sd1 = Sd2 + SPB * HS;         // dL/da1
-    float* Sx = Sd1 + SPB * HS;          // SPB x 3
-    float* Sdy = Sx + SPB * 3;           // SPB
-    float* Sse = Sdy + SPB;              // SPB
-    load_weights(bf, W);
-    __syncthreads();
+    extern __shared__ float4 sh4[];
+    float* const W2 = reinterpret_cast<float*>(sh4);   // HID x LS, W2[j][k]
+    float* const W2T = W2 + HID * LS;                   // HID x LS, W2T[k][j]
+    float* const Ssm = W2T + HID * LS;                  // W1 (3 HID), b1, b2, W3, b3
+    float* const H1 = Ssm + SMALL;                      // SPB x LS
+    float* const H2 = H1 + SPB * LS;
+    float* const D2 = H2 + SPB * LS;                    // dL/da2
+    float* const D1 = D2 + SPB * LS;                    // dL/da1
+    float* const Sx = D1 + SPB * LS;                    // SPB x 4
+    float* const Sdy = Sx + SPB * 4;                    // SPB
+    float* const Sse = Sdy + SPB;                       // SPB
+    const int t = threadIdx.x, r = t >> 4, c = t & 15;
     const int s0 = blockIdx.x * SPB;
-    if (threadIdx.x < SPB) {
-        const int s = threadIdx.x, gs = s0 + s;
-        float* h1 = Sh1 + s * HS;
-        float* h2 = Sh2 + s * HS;
-        float* d2 = Sd2 + s * HS;
-        float* d1 = Sd1 + s * HS;
-        if (gs < n) {
-            float x[3] = {X[3 * gs], X[3 * gs + 1], X[3 * gs + 2]};
-            const float y = mlp_forward(W, x, h1, h2);
-            const float e = y - T[gs];
-            const float dy = 2.0f * w * e;
-            for (int j = 0; j < HID; ++j) d2[j] = dy * W[OFF_W3 + j] * (1.0f - h2[j] * h2[j]);
-            for (int k = 0; k < HID; ++k) {
-                float a = 0.f;
-                for (int j = 0; j < HID; ++j) a += W[OFF_W2 + HID * j + k] * d2[j];
-                d1[k] = a * (1.0f - h1[k] * h1[k]);
+    for (int p = t; p < HID * HID; p += 256) {
+        const float v = bf2f(bf[OFF_W2 + p]);
+        const int j = p / HID, k = p % HID;
+        W2[j * LS + k] = v;
+        W2T[k * LS + j] = v;
+    }
+    for (int p = t; p < 3 * HID; p += 256) Ssm[p] = bf2f(bf[OFF_W1 + p]);
+    for (int p = t; p < HID; p += 256) {
+        Ssm[S_B1 + p] = bf2f(bf[OFF_B1 + p]);
+        Ssm[S_B2 + p] = bf2f(bf[OFF_B2 + p]);
+        Ssm[S_W3 + p] = bf2f(bf[OFF_W3 + p]);
+    }
+    if (t == 0) Ssm[S_B3] = bf2f(bf[OFF_B3]);
+    if (t < SPB) {
+        const int gs = s0 + t;
+        const bool ok = gs < n;
+        Sx[4 * t] = ok ? X[3 * gs] : 0.f;
+        Sx[4 * t + 1] = ok ? X[3 * gs + 1] : 0.f;
+        Sx[4 * t + 2] = ok ? X[3 * gs + 2] : 0.f;
+        Sx[4 * t + 3] = 0.f;
+    }
+    __syncthreads();
+
+    // layer 1: h1[s][j] = tanh(b1[j] + W1[j] . x[s])
+    float h1r[RS][4];
+#pragma unroll
+    for (int a = 0; a < RS; ++a) {
+        const int s = r * RS + a;
+        const float x0 = Sx[4 * s], x1 = Sx[4 * s + 1], x2 = Sx[4 * s + 2];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int j = 4 * c + b;
+            h1r[a][b] = tanhf(Ssm[S_B1 + j] + Ssm[3 * j] * x0 + Ssm[3 * j + 1] * x1 + Ssm[3 * j + 2] * x2);
+        }
+        *reinterpret_cast<float4*>(H1 + s * LS + 4 * c) = make_float4(h1r[a][0], h1r[a][1], h1r[a][2], h1r[a][3]);
+    }
+    __syncthreads();
+
+    // layer 2: a2[s][j] = b2[j] + sum_k h1[s][k] W2[j][k]
+    float acc[RS][4];
+#pragma unroll
+    for (int a = 0; a < RS; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = Ssm[S_B2 + 4 * c + b];
+    for (int k = 0; k < HID; k += 4) {
+        float4 wv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wv[q] = *reinterpret_cast<const float4*>(W2T + (k + q) * LS + 4 * c);
+#pragma unroll
+        for (int a = 0; a < RS; ++a) {
+            const float4 hv = *reinterpret_cast<const float4*>(H1 + (r * RS + a) * LS + k);
+            const float hq[4] = {hv.x, hv.y, hv.z, hv.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                acc[a][0] += hq[q] * wv[q].x;
+                acc[a][1] += hq[q] * wv[q].y;
+                acc[a][2] += hq[q] * wv[q].z;
+                acc[a][3] += hq[q] * wv[q].w;
             }
-            Sx[3 * s] = x[0]; Sx[3 * s + 1] = x[1]; Sx[3 * s + 2] = x[2];
+        }
+    }
+    // h2, output layer, error, dL/da2
+    const float b3 = Ssm[S_B3];
+#pragma unroll
+    for (int a = 0; a < RS; ++a) {
+        const int s = r * RS + a, gs = s0 + s;
+        float h2[4], yp = 0.f;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            h2[b] = tanhf(acc[a][b]);
+            yp += Ssm[S_W3 + 4 * c + b] * h2[b];
+        }
+        // sum over the 16 threads of this r (lanes 16m .. 16m+15 of the wave)
+#pragma unroll
+        for (int m = 8; m > 0; m >>= 1) yp += __shfl_xor(yp, m, 64);
+        const bool ok = gs < n;
+        const float e = ok ? yp + b3 - T[gs] : 0.f;
+        const float dy = 2.0f * w * e;
+        *reinterpret_cast<float4*>(H2 + s * LS + 4 * c) = make_float4(h2[0], h2[1], h2[2], h2[3]);
+        float d2[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) d2[b] = dy * Ssm[S_W3 + 4 * c + b] * (1.0f - h2[b] * h2[b]);
+        *reinterpret_cast<float4*>(D2 + s * LS + 4 * c) = make_float4(d2[0], d2[1], d2[2], d2[3]);
+        if (c == 0) {
             Sdy[s] = dy;
             Sse[s] = w * e * e;
-        } else {
-            for (int j = 0; j < HID; ++j) { h1[j] = 0.f; h2[j] = 0.f; d2[j] = 0.f; d1[j] = 0.f; }
-            Sx[3 * s] = Sx[3 * s + 1] = Sx[3 * s + 2] = 0.f;
-            Sdy[s] = 0.f;
-            Sse[s] = 0.f;
         }
     }
     __syncthreads();
-    float* out = part + (size_t)blockIdx.x * (NP + 1);
-    for (int p = threadIdx.x; p < NP; p += blockDim.x) {
-        float g = 0.f;
-        if (p < OFF_B1) {
-            const int j = p / 3, i = p % 3;
-            for (int s = 0; s < SPB; ++s) g += Sd1[s * HS + j] * Sx[3 * s + i];
-        } else if (p < OFF_W2) {
-            const int j = p - OFF_B1;
-            for (int s = 0; s < SPB; ++s) g += Sd1[s * HS + j];
-        } else if (p < OFF_B2) {
-            const int j = (p - OFF_W2) / HID, k = (p - OFF_W2) % HID;
-            for (int s = 0; s < SPB; ++s) g += Sd2[s * HS + j] * Sh1[s * HS + k];
-        } else if (p < OFF_W3) {
-            const int j = p - OFF_B2;
-            for (int s = 0; s < SPB; ++s) g += Sd2[s * HS + j];
-        } else if (p < OFF_B3) {
-            const int j = p - OFF_W3;
-            for (int s = 0; s < SPB; ++s) g += Sdy[s] * Sh2[s * HS + j];
-        } else {
-            for (int s = 0; s < SPB; ++s) g += Sdy[s];
+
+    // dL/da1[s][k] = (1 - h1^2) sum_j d2[s][j] W2[j][k]
+#pragma unroll
+    for (int a = 0; a < RS; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+    for (int j = 0; j < HID; j += 4) {
+        float4 wv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wv[q] = *reinterpret_cast<const float4*>(W2 + (j + q) * LS + 4 * c);
+#pragma unroll
+        for (int a = 0; a < RS; ++a) {
+            const float4 dv = *reinterpret_cast<const float4*>(D2 + (r * RS + a) * LS + j);
+            const float dq[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                acc[a][0] += dq[q] * wv[q].x;
+                acc[a][1] += dq[q] * wv[q].y;
+                acc[a][2] += dq[q] * wv[q].z;
+                acc[a][3] += dq[q] * wv[q].w;
+            }
         }
-        out[p] = g;
     }
-    if (threadIdx.x == 0) {
-        float e = 0.f;
-        for (int s = 0; s < SPB; ++s) e += Sse[s];
+#pragma unroll
+    for (int a = 0; a < RS; ++a) {
+        float d1[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) d1[b] = acc[a][b] * (1.0f - h1r[a][b] * h1r[a][b]);
+        *reinterpret_cast<float4*>(D1 + (r * RS + a) * LS + 4 * c) = make_float4(d1[0], d1[1], d1[2], d1[3]);
+    }
+    __syncthreads();
+
+    // weight gradient of layer 2: gW2[j][k] = sum_s d2[s][j] h1[s][k], j = 4r + a, k = 4c + b
+    float* out = part + (size_t)blockIdx.x * PSTRIDE;
+    {
+        float g[4][4] = {};
+        for (int s = 0; s < SPB; ++s) {
+            const float4 dv = *reinterpret_cast<const float4*>(D2 + s * LS + 4 * r);
+            const float4 hv = *reinterpret_cast<const float4*>(H1 + s * LS + 4 * c);
+            const float dq[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                g[a][0] += dq[a] * hv.x;
+                g[a][1] += dq[a] * hv.y;
+                g[a][2] += dq[a] * hv.z;
+                g[a][3] += dq[a] * hv.w;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            *reinterpret_cast<float4*>(out + OFF_W2 + (4 * r + a) * HID + 4 * c) =
+                make_float4(g[a][0], g[a][1], g[a][2], g[a][3]);
+    }
+    // the small sums: W1 and b1 (threads 0-63), b2 and W3 (64-127), b3 and SSE (128)
+    if (t < HID) {
+        const int j = t;
+        float g0 = 0.f, g1 = 0.f, g2 = 0.f, gb = 0.f;
+        for (int s = 0; s < SPB; ++s) {
+            const float d = D1[s * LS + j];
+            g0 += d * Sx[4 * s];
+            g1 += d * Sx[4 * s + 1];
+            g2 += d * Sx[4 * s + 2];
+            gb += d;
+        }
+        out[OFF_W1 + 3 * j] = g0;
+        out[OFF_W1 + 3 * j + 1] = g1;
+        out[OFF_W1 + 3 * j + 2] = g2;
+        out[OFF_B1 + j] = gb;
+    } else if (t < 2 * HID) {
+        const int j = t - HID;
+        float gb = 0.f, gw = 0.f;
+        for (int s = 0; s < SPB; ++s) {
+            gb += D2[s * LS + j];
+            gw += Sdy[s] * H2[s * LS + j];
+        }
+        out[OFF_B2 + j] = gb;
+        out[OFF_W3 + j] = gw;
+    } else if (t == 2 * HID) {
+        float gb = 0.f, e = 0.f;
+        for (int s = 0; s < SPB; ++s) {
+            gb += Sdy[s];
+            e += Sse[s];
+        }
+        out[OFF_B3] = gb;
         out[NP] = e;
     }
 }
@@ -147,8 +279,17 @@ __global__ void k_mlp_update(int nblk, const float* __restrict__ part, float* __
                              float c1, float c2) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p > NP) return;
+    // loads issued eight at a time, added in block order
     float g = 0.f;
-    for (int b = 0; b < nblk; ++b) g += part[(size_t)b * (NP + 1) + p];
+    int b = 0;
+    for (; b + 8 <= nblk; b += 8) {
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = part[(size_t)(b + q) * PSTRIDE + p];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) g += v[q];
+    }
+    for (; b < nblk; ++b) g += part[(size_t)b * PSTRIDE + p];
     grad[p] = g;
     if (!adam || p == NP) return;
     const float mm = b1 * m[p] + (1.f - b1) * g;
@@ -291,7 +432,7 @@ using namespace sfh;
 
 namespace {
 
-const size_t kGradLds = sizeof(float) * (NP + 4 * SPB * HS + SPB * 3 + 2 * SPB);
+const size_t kGradLds = sizeof(float) * (2 * HID * LS + SMALL + 4 * SPB * LS + 4 * SPB + 2 * SPB);
 
 int upload_training(sfh_mlp* M, int n, const double* th, const double* y) {
     std::vector<float> xf((size_t)n * 3), tf(n);
@@ -312,7 +453,7 @@ int grad_pass(sfh_mlp* M, int n, int adam, float lr) {
     hipStream_t st = M->ctx->stream;
     const int nblk = (n + SPB - 1) / SPB;
     if (kGradLds > M->ctx->lds_per_block) { set_error("MLP gradient needs %zu B of LDS", kGradLds); return SFH_EARG; }
-    SFH_TRY(M->part.ensure(sizeof(float) * (size_t)nblk * (NP + 1)));
+    SFH_TRY(M->part.ensure(sizeof(float) * (size_t)nblk * PSTRIDE));
     k_mlp_grad<<<nblk, 256, kGradLds, st>>>(n, M->X.as<float>(), M->T.as<float>(), 1.0f / n, M->bf.as<uint16_t>(),
                                             M->part.as<float>());
     SFH_LAUNCH_CHECK();
