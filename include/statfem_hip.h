@@ -152,16 +152,22 @@ int sfh_system_device_ptrs(sfh_system* sys, const int** rowptr_dev, const int** 
 /* ------------------------------------------------------------------ solve */
 typedef struct {
     double tol;          /* FP64 relative residual target per RHS (default 1e-10) */
-    double inner_tol;    /* FP64 residual replacement once the FP32 residual fell by this factor (default 0.01) */
+    double inner_tol;    /* FP64 residual replacement once the FP32 residual fell by this factor
+                            (default 0: 0.01 with the Jacobi preconditioner, 1e-4 with multigrid) */
     int max_outer;       /* cap on residual replacements (default 500) */
     int max_inner;       /* cap on PCG iterations (default 50000) */
-    int check_every;     /* inner iterations between host convergence checks (default 32) */
+    int check_every;     /* inner iterations between host convergence checks
+                            (default 0: 32 with the Jacobi preconditioner, 4 with multigrid) */
     int precision;       /* 0: FP32 recurrences + FP64 replacement; 1: FP64 throughout */
     int rhs_tile;        /* >0: solve the right-hand sides in column tiles of this width (0: all at once) */
     int layout;          /* 0 (default): auto, 1: CSR, 2: 9-coefficient stencil with LDS-tiled kernels on
                             declared 2D tensor grids (FP32, at most 64 right-hand sides per tile);
                             3: 3 x 3 node blocks for vector-DOF meshes (FP32, one problem);
                             auto uses 2 or 3 where they apply, else CSR */
+    int precond;         /* 0 (default): auto, 1: Jacobi, 2: geometric multigrid V(1,1)-cycle (bilinear
+                            prolongation, Galerkin coarse stencils, l1-Jacobi smoothing, dense coarsest
+                            solve) on the stencil layout of a single declared 2D grid; auto uses 2
+                            wherever it applies, else 1 */
 } sfh_solve_opts;
 
 typedef struct {
@@ -171,6 +177,8 @@ typedef struct {
     double ms;            /* device time */
     int converged;
     int layout;           /* operator layout the iteration kernels used: 1 CSR, 2 2D stencil, 3 3x3 blocks */
+    int precond;          /* preconditioner used: 1 Jacobi, 2 multigrid */
+    int levels;           /* multigrid levels including the fine grid (1 for Jacobi) */
 } sfh_solve_stats;
 
 void sfh_solve_opts_default(sfh_solve_opts* o);
@@ -206,11 +214,12 @@ int sfh_solve_rhs(sfh_system* sys, double* X_dev, const sfh_solve_opts* opts,
  * solve; only solves of at least min_rhs right-hand sides per column tile) is
  * bracketed by HIP events on the context stream; the accumulators are reset.
  * every = 0 turns sampling off. sfh_kernel_times returns the average device
- * time (ms per launch) of ms[0] direction + SpMV + p.Ap and ms[1] SpMV +
- * x/r update + r.z, r.r (the two launches of an iteration), and the number of
- * samples since the last sfh_kernel_sampling call. */
+ * time (ms per iteration) of ms[0] direction + SpMV + p.Ap, ms[1] SpMV +
+ * x/r update + r.z, r.r (the two launches of an iteration) and ms[2] the
+ * multigrid V-cycle (0 with the Jacobi preconditioner, which is fused into
+ * ms[1]), and the number of samples since the last sfh_kernel_sampling call. */
 int sfh_kernel_sampling(sfh_ctx* ctx, int every, int max_samples, int min_rhs);
-int sfh_kernel_times(sfh_ctx* ctx, double* ms /* 2 */, long long* samples);
+int sfh_kernel_times(sfh_ctx* ctx, double* ms /* 3 */, long long* samples);
 
 /* ------------------------------------------------------------- statistics */
 /* CovarianceForcing::giveProjectedCovar building block (statX/infer/CovarianceForcing.ipp:37-109):

@@ -466,7 +466,7 @@ int sfh_solve_rhs(sfh_system* s, double* X, const sfh_solve_opts* o, sfh_solve_s
 int sfh_kernel_sampling(sfh_ctx* c, int every, int max_samples, int min_rhs) {
     if (!c || every < 0 || max_samples < 0 || min_rhs < 0) { set_error("bad argument"); return SFH_EARG; }
     SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
-    const size_t want = every > 0 ? 3 * (size_t)max_samples : 0;
+    const size_t want = every > 0 ? 4 * (size_t)max_samples : 0;
     while (c->kev.size() > want) {
         (void)hipEventDestroy(c->kev.back());
         c->kev.pop_back();
@@ -478,7 +478,7 @@ int sfh_kernel_sampling(sfh_ctx* c, int every, int max_samples, int min_rhs) {
     }
     c->ksample = every;
     c->kmin_rhs = min_rhs;
-    c->ksum[0] = c->ksum[1] = 0.0;
+    c->ksum[0] = c->ksum[1] = c->ksum[2] = 0.0;
     c->kcount = 0;
     return SFH_OK;
 }
@@ -486,7 +486,7 @@ int sfh_kernel_sampling(sfh_ctx* c, int every, int max_samples, int min_rhs) {
 int sfh_kernel_times(sfh_ctx* c, double* ms, long long* n) {
     if (!c || !ms) { set_error("bad argument"); return SFH_EARG; }
     const double k = c->kcount > 0 ? (double)c->kcount : 1.0;
-    for (int j = 0; j < 2; ++j) ms[j] = c->ksum[j] / k;
+    for (int j = 0; j < 3; ++j) ms[j] = c->ksum[j] / k;
     if (n) *n = c->kcount;
     return SFH_OK;
 }
@@ -649,6 +649,8 @@ int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const dou
         stats->ms = ms;
         stats->converged = 1;
         stats->layout = s1.layout;
+        stats->precond = s1.precond;
+        stats->levels = s1.levels;
     }
     return SFH_OK;
 }

@@ -73,6 +73,22 @@ struct SolveWork {
     DevBuf scal;                     // per-RHS scalars
     DevBuf acc;                      // per-column FP64 accumulators of the iteration kernels
     DevBuf tab;                      // row-partition tables of the current solve
+    DevBuf s32;                      // multigrid-preconditioned PCG: the FP32 residual r (z = M r lives in r32)
+    DevBuf mgv;                      // multigrid: restricted residuals and corrections of the coarse levels
+};
+
+// Geometric multigrid hierarchy of a single 2D tensor grid: level 0 is the
+// operator's W x H grid (stencil in sfh_system::dia32 / dia64), level l + 1
+// keeps every other node of level l; per level the FP32 stencil (levels >= 1)
+// and the l1-Jacobi smoother weights; the coarsest level's dense FP32 inverse.
+constexpr int kMGMaxLev = 16;
+constexpr int kMGCoarse = 160;       // coarsest level: at most this many nodes
+struct MGHier {
+    bool built = false;              // current for the stencil copy (reset when that is rebuilt)
+    int L = 0;                       // coarse levels below the grid (0: no hierarchy)
+    int W[kMGMaxLev + 1] = {}, H[kMGMaxLev + 1] = {};
+    DevBuf coef64[kMGMaxLev + 1], coef32[kMGMaxLev + 1], dsm[kMGMaxLev + 1];
+    DevBuf cinv;
 };
 
 }  // namespace sfh
@@ -86,13 +102,13 @@ struct sfh_ctx {
     int num_cu = 0;
     size_t lds_per_block = 0;
     std::vector<void*> allocs;
-    // sampled in-solve timing of the two inner-PCG launches (direction +
-    // SpMV, SpMV + update): every ksample-th iteration is bracketed by events
+    // sampled in-solve timing of the inner-PCG launches (direction + SpMV,
+    // SpMV + update, multigrid V-cycle): every ksample-th iteration is bracketed by events
     // (0 = off)
     int ksample = 0;
     int kmin_rhs = 0;
-    std::vector<hipEvent_t> kev;     // 3 events per sample
-    double ksum[2] = {0.0, 0.0};
+    std::vector<hipEvent_t> kev;     // 4 events per sample
+    double ksum[3] = {0.0, 0.0, 0.0};
     long long kcount = 0;
 };
 
@@ -129,6 +145,7 @@ struct sfh_system {
     sfh::DevBuf dia32, dia64;
     int dia_state = 0;           // 0 stale, 1 valid, -1 mesh does not fit the stencil
     std::vector<int> sgrid;      // batch of 2D grids: (first row, W, H) per problem
+    sfh::MGHier mg;              // built with the stencil copy of a single grid
     sfh::SolveWork work;
     sfh::SolveWork work2;        // workspace of a concurrent second solve
     sfh::DevBuf scratch0, scratch1, scratch2, scratch3;
@@ -165,6 +182,9 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
 int solve_on(sfh_system* s, hipStream_t st, SolveWork& w, int trans, int k, const double* B, int ldb, double* X,
              int ldx, const sfh_solve_opts* o, sfh_solve_stats* stats);
 int prepare_operator(sfh_system* s, int nseg = 1, const int* seg_rows = nullptr);
+// multigrid hierarchy of the system's W x H grid from its FP64 stencil copy
+// (s->mg.L stays 0 when the grid is too small or a level is not SPD)
+int mg_build(sfh_system* s, hipStream_t st, int W, int H);
 int triplets_build(sfh_system* s, long long n, const int* rows, const int* cols, const double* vals);
 int triplets_values(sfh_system* s, long long n, const double* vals);
 int covforcing_grid(sfh_ctx* c, int d, const int gn[3], const double* const ax[3], const double* cv, double sigma,

@@ -535,7 +535,10 @@ __global__ __launch_bounds__(kG) void k_dir_spmv_t2(const int* __restrict__ gtab
     block_accumulate<LPR>(t, kp, ad, reinterpret_cast<double*>(sm), acc + (A_PQ + (it & 1)) * ks + t.so);
 }
 
-template <int LPR>
+// MG = false: z = D^-1 r is the stored vector (Jacobi); MG = true: the FP32
+// residual r itself is updated and only r.r is summed (r.z comes from the
+// multigrid preconditioner that follows)
+template <int LPR, bool MG>
 __global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gtab, int kp, int ks, int it,
                                                      const float* __restrict__ coef, const float* __restrict__ dinv,
                                                      const float* __restrict__ diag, const double* __restrict__ S,
@@ -572,21 +575,273 @@ __global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gta
             const V4<float> pi = v4(sm[((oy + 1) * (kTX + 2) + (ox + 1)) * LPR + t.lane]);
             const size_t off = (size_t)nd * kp + t.col0;
             V4<float> xv = ld4(x + off), zv = ld4(z + off);
-            const float dd = dinv[nd], g = diag[nd];
             xv.x += al.x * pi.x; xv.y += al.y * pi.y; xv.z += al.z * pi.z; xv.w += al.w * pi.w;
-            zv.x -= al.x * dd * q.x; zv.y -= al.y * dd * q.y; zv.z -= al.z * dd * q.z; zv.w -= al.w * dd * q.w;
+            if constexpr (MG) {
+                zv.x -= al.x * q.x; zv.y -= al.y * q.y; zv.z -= al.z * q.z; zv.w -= al.w * q.w;
+                rr.x += zv.x * zv.x; rr.y += zv.y * zv.y; rr.z += zv.z * zv.z; rr.w += zv.w * zv.w;
+            } else {
+                const float dd = dinv[nd], g = diag[nd];
+                zv.x -= al.x * dd * q.x; zv.y -= al.y * dd * q.y; zv.z -= al.z * dd * q.z; zv.w -= al.w * dd * q.w;
+                res_dots(zv, g, rz, rr);
+            }
             st4(x + off, xv);
             st4(z + off, zv);
-            res_dots(zv, g, rz, rr);
         }
     }
     __syncthreads();
     double* red = reinterpret_cast<double*>(sm);
-    V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
-    block_accumulate<LPR>(t, kp, a, red, acc + (A_RZ + it % 3) * ks + t.so);
-    __syncthreads();
+    if constexpr (!MG) {
+        V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
+        block_accumulate<LPR>(t, kp, a, red, acc + (A_RZ + it % 3) * ks + t.so);
+        __syncthreads();
+    }
     V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
     block_accumulate<LPR>(t, kp, bb, red, acc + (A_RR + it % 3) * ks + t.so);
+}
+
+// ------------------------------------------------ multigrid V-cycle (2D grids)
+// z = M r with M one V(1,1)-cycle of the hierarchy built by mg_build: on each
+// level l, pre-smoothing from zero e = S r (S = omega D_l1^-1), restriction of
+// the residual r - A e to level l+1, recursion, prolongation of the coarse
+// correction, one post-smoothing sweep; a dense solve on the coarsest level.
+// Two launches per level: the restriction kernel recomputes e = S r on the
+// fly, the prolongation kernel rebuilds it and adds P e_{l+1} before the post
+// sweep, so no level stores its pre-smoothed iterate. Level-0 constrained
+// nodes carry neither residual nor correction, so M acts on the free block.
+// Tiles stage their vectors (kp columns per node, 4 per lane) with a halo in
+// LDS exactly like the iteration kernels above.
+constexpr int kRX = 16, kRY = 2;                      // coarse nodes per restriction tile
+constexpr int kRW = 2 * kRX + 3, kRH = 2 * kRY + 3;   // fine nodes it stages (residual support + halo)
+constexpr int kPX = 32, kPY = 4;                      // fine nodes per prolongation tile
+constexpr int kPW = kPX + 2, kPH = kPY + 2;
+constexpr int kCW = kPX / 2 + 2, kCH = kPY / 2 + 2;   // coarse nodes a prolongation tile stages
+// tile table, kMT ints per work-group: fine level first node, W, H, coarse
+// level first node, Wc, Hc, tile origin x, y (restriction: coarse nodes;
+// prolongation: fine nodes), segment; W = 0 marks padding work-groups
+constexpr int kMT = 9;
+
+inline size_t mg_restrict_lds(int lpr) { return (size_t)kRW * kRH * (lpr * sizeof(float4) + sizeof(float)); }
+inline size_t mg_prolong_lds(int lpr) {
+    const size_t tile = (size_t)(kPW * kPH + kCW * kCH) * lpr * sizeof(float4);
+    const size_t red = (size_t)kG * 4 * sizeof(double);
+    return tile > red ? tile : red;
+}
+
+// weight of coarse index X in fine index x of an n-node line (bilinear P,
+// constant continuation at an odd last node)
+__device__ inline float mg_w(int x, int X, int n) {
+    if ((x & 1) == 0) return x == 2 * X ? 1.f : 0.f;
+    if (x == n - 1) return (x - 1) / 2 == X ? 1.f : 0.f;
+    return ((x - 1) / 2 == X || (x + 1) / 2 == X) ? 0.5f : 0.f;
+}
+
+__device__ inline void fma4(float a, const float4& v, float4& s) {
+    s.x += a * v.x; s.y += a * v.y; s.z += a * v.z; s.w += a * v.w;
+}
+
+// r_{l+1} = P^T (r - A (S r)) on one tile of coarse nodes
+template <int LPR>
+__global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab, int kp, const float* __restrict__ coef,
+                                                    const float* __restrict__ dsm,
+                                                    const unsigned char* __restrict__ cmask,
+                                                    const float* __restrict__ r, float* __restrict__ rc) {
+    extern __shared__ float4 sm[];
+    float* sd = reinterpret_cast<float*>(sm + kRW * kRH * LPR);
+    constexpr int ngrp = kG / LPR;
+    constexpr int kIW = 2 * kRX + 1, kIn = kIW * (2 * kRY + 1);   // fine nodes whose residual is restricted
+    constexpr int kPer = (kIn + ngrp - 1) / ngrp;
+    const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
+    const int* d = tab + kMT * xcd_block(blockIdx.x, gridDim.x);
+    const int fbase = d[0], W = d[1], H = d[2], cbase = d[3], Wc = d[4], Hc = d[5], cx0 = d[6], cy0 = d[7];
+    if (W == 0) return;
+    const int col0 = lane * 4;
+    const bool ok = col0 < kp;
+    const int fx0 = 2 * cx0 - 2, fy0 = 2 * cy0 - 2;
+    for (int h = grp; h < kRW * kRH; h += ngrp) {
+        const int gx = fx0 + h % kRW, gy = fy0 + h / kRW;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        float w = 0.f;
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const int nd = fbase + gy * W + gx;
+            if (ok) v = *reinterpret_cast<const float4*>(r + (size_t)nd * kp + col0);
+            w = dsm[nd];
+        }
+        sm[h * LPR + lane] = v;
+        if (lane == 0) sd[h] = w;
+    }
+    __syncthreads();
+    float4 res[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int i = grp + u * ngrp;
+        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < kIn) {
+            const int hx = i % kIW + 1, hy = i / kIW + 1;
+            const int gx = fx0 + hx, gy = fy0 + hy;
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+                const int nd = fbase + gy * W + gx;
+                if (!(cmask && cmask[nd])) {
+                    const float* c = coef + (size_t)nd * 9;
+                    q = sm[(hy * kRW + hx) * LPR + lane];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        const int h = (hy + k / 3 - 1) * kRW + hx + k % 3 - 1;
+                        fma4(-c[k] * sd[h], sm[h * LPR + lane], q);
+                    }
+                }
+            }
+        }
+        res[u] = q;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int i = grp + u * ngrp;
+        if (i < kIn) sm[((i / kIW + 1) * kRW + i % kIW + 1) * LPR + lane] = res[u];
+    }
+    __syncthreads();
+    if (!ok) return;
+    for (int o = grp; o < kRX * kRY; o += ngrp) {
+        const int X = cx0 + o % kRX, Y = cy0 + o / kRX;
+        if (X >= Wc || Y >= Hc) continue;
+        float4 acc4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int fy = 2 * Y + dy;
+            if (fy < 0 || fy >= H) continue;
+            const float wy = mg_w(fy, Y, H);
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int fx = 2 * X + dx;
+                if (fx < 0 || fx >= W) continue;
+                const float w = wy * mg_w(fx, X, W);
+                if (w != 0.f) fma4(w, sm[((fy - fy0) * kRW + (fx - fx0)) * LPR + lane], acc4);
+            }
+        }
+        *reinterpret_cast<float4*>(rc + (size_t)(cbase + Y * Wc + X) * kp + col0) = acc4;
+    }
+}
+
+// e = e0 + S (r - A e0), e0 = S r + P e_{l+1}, on one tile of fine nodes. TOP
+// (level 0): P acts on free nodes only, e is zero on constrained nodes, and the
+// column sums r.e (and r.r) are formed: mode 0 adds r.e into the iteration's
+// accumulator slot, mode 1 writes per-tile partial rows of r.e and r.r (after
+// a residual replacement; padding tiles write zeros)
+template <int LPR, bool TOP>
+__global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, int kp, int ks, int it, int mode,
+                                                   const float* __restrict__ coef, const float* __restrict__ dsm,
+                                                   const unsigned char* __restrict__ cmask,
+                                                   const float* __restrict__ r, const float* __restrict__ ec,
+                                                   float* __restrict__ e, double* __restrict__ acc,
+                                                   double* __restrict__ part0, double* __restrict__ part1) {
+    extern __shared__ float4 sm[];
+    float4* sc = sm + kPW * kPH * LPR;
+    Tile t;
+    t.lane = threadIdx.x % LPR;
+    t.grp = threadIdx.x / LPR;
+    t.ngrp = kG / LPR;
+    t.col0 = t.lane * 4;
+    t.bid = xcd_block(blockIdx.x, gridDim.x);
+    t.r0 = t.r1 = 0;
+    const int* d = tab + kMT * t.bid;
+    const int fbase = d[0], W = d[1], H = d[2], cbase = d[3], Wc = d[4], Hc = d[5], tx0 = d[6], ty0 = d[7];
+    t.so = d[8] * kp;
+    if (W == 0 && !(TOP && mode == 1)) return;
+    const bool ok = t.col0 < kp;
+    V4<float> rz = {0.f, 0.f, 0.f, 0.f}, rr = {0.f, 0.f, 0.f, 0.f};
+    if (W > 0) {
+        const int cx0 = tx0 / 2 - 1, cy0 = ty0 / 2 - 1;
+        for (int h = t.grp; h < kCW * kCH; h += t.ngrp) {
+            const int X = cx0 + h % kCW, Y = cy0 + h / kCW;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ok && X >= 0 && X < Wc && Y >= 0 && Y < Hc)
+                v = *reinterpret_cast<const float4*>(ec + (size_t)(cbase + Y * Wc + X) * kp + t.col0);
+            sc[h * LPR + t.lane] = v;
+        }
+        __syncthreads();
+        for (int h = t.grp; h < kPW * kPH; h += t.ngrp) {
+            const int gx = tx0 - 1 + h % kPW, gy = ty0 - 1 + h / kPW;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ok && gx >= 0 && gx < W && gy >= 0 && gy < H) {
+                const int nd = fbase + gy * W + gx;
+                if (!(TOP && cmask[nd])) {
+                    const float4 rv = *reinterpret_cast<const float4*>(r + (size_t)nd * kp + t.col0);
+                    const float w = dsm[nd];
+                    v = make_float4(w * rv.x, w * rv.y, w * rv.z, w * rv.w);
+                    const int ax = (gx & 1) ? (gx - 1) / 2 : gx / 2;   // first coarse parent
+                    const int ay = (gy & 1) ? (gy - 1) / 2 : gy / 2;
+                    const int nx = ((gx & 1) && gx != W - 1) ? 2 : 1;
+                    const int ny = ((gy & 1) && gy != H - 1) ? 2 : 1;
+                    const float fx = nx == 2 ? 0.5f : 1.f, fy = ny == 2 ? 0.5f : 1.f;
+                    for (int u = 0; u < ny; ++u)
+                        for (int q = 0; q < nx; ++q)
+                            fma4(fx * fy, sc[((ay + u - cy0) * kCW + (ax + q - cx0)) * LPR + t.lane], v);
+                }
+            }
+            sm[h * LPR + t.lane] = v;
+        }
+        __syncthreads();
+        if (ok) {
+            for (int o = t.grp; o < kPX * kPY; o += t.ngrp) {
+                const int ox = o % kPX, oy = o / kPX;
+                const int gx = tx0 + ox, gy = ty0 + oy;
+                if (gx >= W || gy >= H) continue;
+                const int nd = fbase + gy * W + gx;
+                const float* c = coef + (size_t)nd * 9;
+                const float4 rv = *reinterpret_cast<const float4*>(r + (size_t)nd * kp + t.col0);
+                float4 q = rv;
+#pragma unroll
+                for (int k = 0; k < 9; ++k)
+                    fma4(-c[k], sm[((oy + k / 3) * kPW + ox + k % 3) * LPR + t.lane], q);
+                const float w = dsm[nd];
+                float4 z = sm[((oy + 1) * kPW + ox + 1) * LPR + t.lane];
+                fma4(w, q, z);
+                if (TOP && cmask[nd]) z = make_float4(0.f, 0.f, 0.f, 0.f);
+                *reinterpret_cast<float4*>(e + (size_t)nd * kp + t.col0) = z;
+                if (TOP) {
+                    rz.x += rv.x * z.x; rz.y += rv.y * z.y; rz.z += rv.z * z.z; rz.w += rv.w * z.w;
+                    rr.x += rv.x * rv.x; rr.y += rv.y * rv.y; rr.z += rv.z * rv.z; rr.w += rv.w * rv.w;
+                }
+            }
+        }
+    }
+    if constexpr (TOP) {
+        __syncthreads();   // the staging area becomes the reduction buffer
+        double* red = reinterpret_cast<double*>(sm);
+        const V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
+        if (mode == 0) {
+            block_accumulate<LPR>(t, kp, a, red, acc + (A_RZ + it % 3) * ks + t.so);
+        } else {
+            block_partial<LPR>(t, kp, a, red, part0);
+            __syncthreads();
+            const V4<double> b = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
+            block_partial<LPR>(t, kp, b, red, part1);
+        }
+    }
+}
+
+// e = C^-1 r on the coarsest level (n nodes; dense FP32 inverse, row-major)
+__global__ void k_mg_coarse(int n, int kp, const float* __restrict__ cinv, const float* __restrict__ r,
+                            float* __restrict__ e) {
+    const int nl = kp / 4;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nl) return;
+    const int i = t / nl, c = (t % nl) * 4;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int j = 0; j < n; ++j) fma4(cinv[(size_t)i * n + j], *reinterpret_cast<const float4*>(r + (size_t)j * kp + c), s);
+    *reinterpret_cast<float4*>(e + (size_t)i * kp + c) = s;
+}
+
+// residual replacement with the multigrid preconditioner: r = s' r64 (FP32),
+// p *= s'/s (one problem: scalars indexed by column)
+__global__ void k_mg_rescale(long long total, int kp, const double* __restrict__ scale,
+                             const double* __restrict__ ratio, const double* __restrict__ r64, float* __restrict__ r,
+                             float* __restrict__ p) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = (int)(i % kp);
+    r[i] = (float)(r64[i] * scale[c]);
+    p[i] *= (float)ratio[c];
 }
 
 // ------------------------------------------------ vector DOFs: 3 x 3 blocks
@@ -982,12 +1237,21 @@ struct Plan {
     const int* d_gtab = nullptr;
     // 3 x 3 block path (vector DOFs): node ranges of npb nodes per work-group
     int bsr = 0, nnodes = 0, npb = 0, bblk = 0;
+    // multigrid V-cycle: tile tables (kMT ints per work-group) of the
+    // restriction (coarse tiles of level l+1) and prolongation (tiles of level
+    // l) launches for l < mgL, each padded to whole XCD groups; level l >= 1
+    // vectors in the coarse-level buffer: r_l at mgv_off[l], e_l after it
+    int mgL = 0;
+    std::vector<int> mgtab;
+    int mg_roff[kMGMaxLev] = {}, mg_rn[kMGMaxLev] = {}, mg_poff[kMGMaxLev] = {}, mg_pn[kMGMaxLev] = {};
+    size_t mgv_off[kMGMaxLev + 1] = {}, mgv_total = 0;
+    const int* d_mgtab = nullptr;
 };
 
-Plan make_plan(int n, int k, int num_cu, int nseg = 1, const int* seg_rows = nullptr) {
+Plan make_plan(int n, int k, int num_cu, int nseg = 1, const int* seg_rows = nullptr, int min_kp = 4) {
     Plan p;
     p.n = n;
-    p.kp = (k + 3) / 4 * 4;
+    p.kp = std::max((k + 3) / 4 * 4, min_kp);
     p.lpr = std::min(64, pow2_at_least(p.kp / 4));
     p.ntile = (p.kp + 4 * p.lpr - 1) / (4 * p.lpr);
     const int rows_per_pass = kB / p.lpr;
@@ -1049,6 +1313,45 @@ void plan_grid(Plan& p, const std::vector<int>& grids) {
     p.gblk = (int)(p.gtab.size() / kGT);
 }
 
+void pad_mgtab(std::vector<int>& tab, int& cnt) {
+    while (cnt % 8) {
+        tab.insert(tab.end(), kMT, 0);
+        ++cnt;
+    }
+}
+
+void plan_mg(Plan& p, const MGHier& m) {
+    p.mgL = m.L;
+    p.mgtab.clear();
+    for (int l = 0; l < m.L; ++l) {
+        const int W = m.W[l], H = m.H[l], Wc = m.W[l + 1], Hc = m.H[l + 1];
+        p.mg_roff[l] = (int)(p.mgtab.size() / kMT);
+        int cnt = 0;
+        for (int cy = 0; cy < Hc; cy += kRY)
+            for (int cx = 0; cx < Wc; cx += kRX, ++cnt) {
+                const int e[kMT] = {0, W, H, 0, Wc, Hc, cx, cy, 0};
+                p.mgtab.insert(p.mgtab.end(), e, e + kMT);
+            }
+        pad_mgtab(p.mgtab, cnt);
+        p.mg_rn[l] = cnt;
+        p.mg_poff[l] = (int)(p.mgtab.size() / kMT);
+        cnt = 0;
+        for (int ty = 0; ty < H; ty += kPY)
+            for (int tx = 0; tx < W; tx += kPX, ++cnt) {
+                const int e[kMT] = {0, W, H, 0, Wc, Hc, tx, ty, 0};
+                p.mgtab.insert(p.mgtab.end(), e, e + kMT);
+            }
+        pad_mgtab(p.mgtab, cnt);
+        p.mg_pn[l] = cnt;
+    }
+    size_t off = 0;
+    for (int l = 1; l <= m.L; ++l) {
+        p.mgv_off[l] = off;
+        off += 2 * (size_t)m.W[l] * m.H[l] * p.kp;
+    }
+    p.mgv_total = off;
+}
+
 template <class T, int LPR>
 struct Kern {
     static void dir_spmv(const Plan& P, const Seg& G, hipStream_t st, int it, const int* rp, const Entries<T>& M,
@@ -1069,8 +1372,40 @@ struct Kern {
     static void update_g(const Plan& P, hipStream_t st, int it, const T* coef, const T* dinv, const T* diag,
                          const double* S, const T* p, T* x, T* z, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_cg_update_t2<LPR><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, dinv, diag,
-                                                                        S, p, x, z, acc);
+            k_cg_update_t2<LPR, false><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, dinv,
+                                                                               diag, S, p, x, z, acc);
+    }
+    // multigrid-preconditioned variant: x += alpha p, r -= alpha A p, r.r
+    static void update_g_mg(const Plan& P, hipStream_t st, int it, const T* coef, const double* S, const T* p, T* x,
+                            T* r, double* acc) {
+        if constexpr (std::is_same<T, float>::value && LPR <= 16)
+            k_cg_update_t2<LPR, true><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, nullptr,
+                                                                              nullptr, S, p, x, r, acc);
+    }
+    // z = M r (one V-cycle) with r.z into the iteration's slot (mode 0) or r.z,
+    // r.r as per-tile partial rows (mode 1)
+    static void vcycle(const Plan& P, const MGHier& m, hipStream_t st, int it, int mode, const T* coef0,
+                       const unsigned char* cmask, const T* r0, T* z, T* mgv, double* acc, double* p0, double* p1) {
+        if constexpr (std::is_same<T, float>::value && LPR >= 4 && LPR <= 16) {
+            const int L = P.mgL, kp = P.kp;
+            auto rvec = [&](int l) -> float* { return l == 0 ? const_cast<float*>(r0) : mgv + P.mgv_off[l]; };
+            auto evec = [&](int l) -> float* {
+                return l == 0 ? z : mgv + P.mgv_off[l] + (size_t)m.W[l] * m.H[l] * kp;
+            };
+            for (int l = 0; l < L; ++l)
+                k_mg_restrict<LPR><<<P.mg_rn[l], kG, mg_restrict_lds(LPR), st>>>(
+                    P.d_mgtab + (size_t)kMT * P.mg_roff[l], kp, l == 0 ? coef0 : m.coef32[l].as<float>(),
+                    m.dsm[l].as<float>(), l == 0 ? cmask : nullptr, rvec(l), rvec(l + 1));
+            const int nL = m.W[L] * m.H[L];
+            k_mg_coarse<<<(nL * (kp / 4) + 255) / 256, 256, 0, st>>>(nL, kp, m.cinv.as<float>(), rvec(L), evec(L));
+            for (int l = L - 1; l >= 1; --l)
+                k_mg_prolong<LPR, false><<<P.mg_pn[l], kG, mg_prolong_lds(LPR), st>>>(
+                    P.d_mgtab + (size_t)kMT * P.mg_poff[l], kp, P.ks, it, mode, m.coef32[l].as<float>(),
+                    m.dsm[l].as<float>(), nullptr, rvec(l), evec(l + 1), evec(l), nullptr, nullptr, nullptr);
+            k_mg_prolong<LPR, true><<<P.mg_pn[0], kG, mg_prolong_lds(LPR), st>>>(
+                P.d_mgtab + (size_t)kMT * P.mg_poff[0], kp, P.ks, it, mode, coef0, m.dsm[0].as<float>(), cmask, r0,
+                evec(1), z, acc, p0, p1);
+        }
     }
     static void dir_spmv_b(const Plan& P, hipStream_t st, int it, const int* rpn, const int* cin, const float* bval,
                            double* S, const T* z, const T* pold, T* pnew, double* acc) {
@@ -1158,6 +1493,13 @@ int prepare_operator_on(sfh_system* s, hipStream_t st, const std::vector<int>& g
         SFH_CHECK_HIP(hipMemcpyAsync(&he, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
         SFH_CHECK_HIP(hipStreamSynchronize(st));
         s->dia_state = he ? -1 : 1;
+        s->mg.built = false;
+        s->mg.L = 0;
+    }
+    // multigrid hierarchy of a single grid, from the stencil copy
+    if (grids.size() == 3 && s->dia_state == 1 && !s->mg.built) {
+        SFH_TRY(mg_build(s, st, grids[1], grids[2]));
+        s->mg.built = true;
     }
     return SFH_OK;
 }
@@ -1189,7 +1531,12 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     if (!P.grid && ns == 1 && s->dofpn == 3 && s->kind != 0 && (o.layout == 3 || o.layout == 0) &&
         std::is_same<T, float>::value && P.ntile == 1)
         plan_bsr3(P, s->nnodes, s->ctx->num_cu);
-    const int nbi = P.grid ? P.gblk : P.nblk;   // partial rows written by the iteration kernels
+    // multigrid preconditioner: stencil layout of one grid with a hierarchy
+    const MGHier& mg = s->mg;
+    const bool use_mg = P.grid && ns == 1 && o.precond != 1 && mg.built && mg.L > 0 &&
+                        (long long)mg.W[0] * mg.H[0] == n && P.lpr >= 4;
+    if (use_mg) plan_mg(P, mg);
+    const int nbi = std::max(P.grid ? P.gblk : P.nblk, use_mg ? P.mg_pn[0] : 0);   // partial rows per launch
     SFH_TRY(w.b64.ensure(nk * 8));
     SFH_TRY(w.x64.ensure(nk * 8));
     SFH_TRY(w.r64.ensure(nk * 8));
@@ -1201,8 +1548,15 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_TRY(w.part1.ensure((size_t)std::max(P.nblk, nbi) * kp * 8));
     SFH_TRY(w.scal.ensure((size_t)S_NSLOT * ks * 8));
     SFH_TRY(w.acc.ensure((size_t)A_N * ks * 8));
-    // partition tables: brow (nblk + 1), bseg (nblk), sb (nseg + 1), srow (nseg + 1), gtab
-    const size_t ntab = P.brow.size() + P.bseg.size() + P.sb.size() + P.srow.size() + P.gtab.size();
+    if (use_mg) {
+        SFH_TRY(w.s32.ensure(nk * sizeof(T)));
+        SFH_TRY(w.mgv.ensure(P.mgv_total * sizeof(T)));
+    }
+    // partition tables: brow (nblk + 1), bseg (nblk), sb (nseg + 1), srow (nseg + 1), gtab, multigrid tiles,
+    // and the partial-row bounds {0, level-0 prolongation tiles} of a multigrid residual replacement
+    const int sbmg[2] = {0, use_mg ? P.mg_pn[0] : 0};
+    const size_t ntab =
+        P.brow.size() + P.bseg.size() + P.sb.size() + P.srow.size() + P.gtab.size() + P.mgtab.size() + 2;
     SFH_TRY(w.tab.ensure(sizeof(int) * ntab));
     {
         std::vector<int> h;
@@ -1212,6 +1566,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         h.insert(h.end(), P.sb.begin(), P.sb.end());
         h.insert(h.end(), P.srow.begin(), P.srow.end());
         h.insert(h.end(), P.gtab.begin(), P.gtab.end());
+        h.insert(h.end(), P.mgtab.begin(), P.mgtab.end());
+        h.insert(h.end(), sbmg, sbmg + 2);
         SFH_CHECK_HIP(hipMemcpyAsync(w.tab.p, h.data(), sizeof(int) * ntab, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipStreamSynchronize(st));
     }
@@ -1220,6 +1576,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     const int* d_sb = d_bseg + P.bseg.size();
     const int* d_srow = d_sb + P.sb.size();
     P.d_gtab = d_srow + P.srow.size();
+    P.d_mgtab = P.d_gtab + P.gtab.size();
+    const int* d_sbmg = P.d_mgtab + P.mgtab.size();
     const Seg G = {d_brow, d_bseg, ks};
     const dim3 gred((kp + 63) / 64, ns);
     double* S = w.scal.as<double>();
@@ -1232,7 +1590,10 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     const T* diag = std::is_same<T, float>::value ? (const T*)s->diag32.p : (const T*)s->diag64.p;
     const Entries<T> M = {s->cv32.as<int2>(), ci, vaT};
     T* x = w.x32.as<T>();
-    T* z = w.r32.as<T>();   // z = D^-1 r
+    T* z = w.r32.as<T>();   // Jacobi: z = D^-1 r (r itself is not kept); multigrid: z = M r
+    T* r = use_mg ? w.s32.as<T>() : nullptr;
+    T* mgv = use_mg ? w.mgv.as<T>() : nullptr;
+    const unsigned char* cmask = s->cmask.as<unsigned char>();
     T* pa = w.p32.as<T>();
     T* pb = w.q32.as<T>();
     double* p0 = w.part0.as<double>();
@@ -1253,7 +1614,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_CHECK_HIP(hipMemsetAsync(w.acc.p, 0, (size_t)A_N * ks * 8, st));
     double* acc = w.acc.as<double>();
     sfh_ctx* cx = s->ctx;
-    const int nsmp = (!tl_env && cx->ksample > 0 && k >= cx->kmin_rhs) ? (int)cx->kev.size() / 3 : 0;
+    const int nsmp = (!tl_env && cx->ksample > 0 && k >= cx->kmin_rhs) ? (int)cx->kev.size() / 4 : 0;
     int ismp = 0;
     Kern<T, LPR>::resid(P, G, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
     k_red_norm<<<gred, kRedB, 0, st>>>(kp, d_sb, p0, S + S_BN2 * ks);
@@ -1264,12 +1625,16 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_CHECK_HIP(hipStreamSynchronize(st));
     rn2 = bn2;
     const double tol2 = o.tol * o.tol;
-    const double delta2 = o.inner_tol * o.inner_tol;
+    // FP32 reduction per refinement cycle and host check interval: a multigrid
+    // iteration reduces the residual by about an order of magnitude, a Jacobi
+    // one by far less
+    const double inner = (o.inner_tol > 0.0 && o.inner_tol < 1.0) ? o.inner_tol : (use_mg ? 1e-4 : 0.01);
+    const double delta2 = inner * inner;
     int updates = 0, iters = 0;
     double maxrel = 0.0;
     bool done = false;
     std::vector<double> hs(5 * (size_t)ks);   // ACTIVE, FROZEN, SCALE, TRIG and the latest r.r per column
-    const int check = std::max(1, o.check_every);
+    const int check = o.check_every > 0 ? o.check_every : (use_mg ? 4 : 32);
     int jlast = -1;                           // last iteration of the previous cycle
     for (;;) {
         // residual replacement: decide per column, rescale, keep directions
@@ -1291,9 +1656,17 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_SCALE * ks, scale.data(), ks * 8, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_RATIO * ks, ratio.data(), ks * 8, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * ks, tgt.data(), ks * 8, hipMemcpyHostToDevice, st));
-        Kern<T, LPR>::rescale(P, G, st, S + S_SCALE * ks, S + S_RATIO * ks, s->dinv64.as<double>(), diag,
-                              w.r64.as<double>(), z, pa, p0, p1);
-        k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb, p0, p1, S, acc, delta2, jlast);
+        if (use_mg) {
+            k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, S + S_SCALE * ks,
+                                                                      S + S_RATIO * ks, w.r64.as<double>(),
+                                                                      (float*)r, (float*)pa);
+            Kern<T, LPR>::vcycle(P, mg, st, 0, 1, coef, cmask, r, z, mgv, acc, p0, p1);
+            k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sbmg, p0, p1, S, acc, delta2, jlast);
+        } else {
+            Kern<T, LPR>::rescale(P, G, st, S + S_SCALE * ks, S + S_RATIO * ks, s->dinv64.as<double>(), diag,
+                                  w.r64.as<double>(), z, pa, p0, p1);
+            k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb, p0, p1, S, acc, delta2, jlast);
+        }
         SFH_LAUNCH_CHECK();
         ++updates;
         // FP32 CG until some column is due for replacement: two kernels per
@@ -1301,7 +1674,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         int j = 0;
         for (;; ++j) {
             const bool smp = ismp < nsmp && iters % cx->ksample == 0;
-            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[3 * ismp], st));
+            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp], st));
             if (P.grid)
                 Kern<T, LPR>::dir_spmv_g(P, st, j, coef, S, z, pa, pb, acc);
             else if (P.bsr)
@@ -1309,15 +1682,19 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                                          s->bsr_val.as<float>(), S, z, pa, pb, acc);
             else
                 Kern<T, LPR>::dir_spmv(P, G, st, j, rp, M, S, z, pa, pb, acc);
-            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[3 * ismp + 1], st));
-            if (P.grid)
+            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 1], st));
+            if (use_mg)
+                Kern<T, LPR>::update_g_mg(P, st, j, coef, S, pb, x, r, acc);
+            else if (P.grid)
                 Kern<T, LPR>::update_g(P, st, j, coef, dinv, diag, S, pb, x, z, acc);
             else if (P.bsr)
                 Kern<T, LPR>::update_b(P, st, j, s->rowptr_node.as<int>(), s->colind_node.as<int>(),
                                        s->bsr_val.as<float>(), dinv, diag, S, pb, x, z, acc);
             else
                 Kern<T, LPR>::update(P, G, st, j, rp, M, dinv, diag, S, pb, x, z, acc);
-            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[3 * ismp + 2], st));
+            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 2], st));
+            if (use_mg) Kern<T, LPR>::vcycle(P, mg, st, j, 0, coef, cmask, r, z, mgv, acc, p0, p1);
+            if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 3], st));
             ismp += smp;
             std::swap(pa, pb);
             ++iters;
@@ -1364,9 +1741,9 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     for (int i = 0; i < ismp; ++i) {
-        for (int jj = 0; jj < 2; ++jj) {
+        for (int jj = 0; jj < 3; ++jj) {
             float f = 0.f;
-            SFH_CHECK_HIP(hipEventElapsedTime(&f, cx->kev[3 * i + jj], cx->kev[3 * i + jj + 1]));
+            SFH_CHECK_HIP(hipEventElapsedTime(&f, cx->kev[4 * i + jj], cx->kev[4 * i + jj + 1]));
             cx->ksum[jj] += f;
         }
         ++cx->kcount;
@@ -1378,6 +1755,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         stats->ms = ms;
         stats->converged = done ? 1 : 0;
         stats->layout = P.grid ? 2 : (P.bsr ? 3 : 1);
+        stats->precond = use_mg ? 2 : 1;
+        stats->levels = use_mg ? P.mgL + 1 : 1;
     }
     if (!done) {
         set_error("solve did not reach tol %.3g (max rel. residual %.3g after %d iterations, %d replacements)",
@@ -1417,15 +1796,16 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
     sfh_solve_opts_default(&o);
     if (opts) o = *opts;
     if (o.tol <= 0) o.tol = 1e-10;
-    if (o.inner_tol <= 0 || o.inner_tol >= 1) o.inner_tol = 0.01;
+    if (o.inner_tol < 0 || o.inner_tol >= 1) o.inner_tol = 0;   // automatic
     if (o.max_outer <= 0) o.max_outer = 500;
     if (o.max_inner <= 0) o.max_inner = 50000;
-    if (o.check_every <= 0) o.check_every = 32;
+    if (o.check_every < 0) o.check_every = 0;
     if (o.layout < 0 || o.layout > 3) o.layout = 0;
+    if (o.precond < 0 || o.precond > 2) o.precond = 0;
     // column tiles: each tile is an independent multi-RHS solve whose working
     // set can stay in the last-level cache
     if (o.rhs_tile > 0 && o.rhs_tile < k) {
-        sfh_solve_stats acc = {0, 0, 0.0, 0.0, 1, 0};
+        sfh_solve_stats acc = {0, 0, 0.0, 0.0, 1, 0, 1, 1};
         for (int c0 = 0; c0 < k; c0 += o.rhs_tile) {
             const int kc = std::min(o.rhs_tile, k - c0);
             sfh_solve_stats st1;
@@ -1438,6 +1818,8 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
             acc.ms += st1.ms;
             acc.converged = acc.converged && st1.converged;
             acc.layout = st1.layout;
+            acc.precond = st1.precond;
+            acc.levels = st1.levels;
             if (rc != SFH_OK) {
                 if (stats) *stats = acc;
                 return rc;
@@ -1446,7 +1828,15 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
         if (stats) *stats = acc;
         return SFH_OK;
     }
-    const Plan P = make_plan(s->ndof, k, s->ctx->num_cu, nseg, seg_rows);
+    // narrow solves on a grid with a multigrid hierarchy run as 16-column tiles
+    // (the stencil kernels' narrowest): a dozen multigrid iterations on padded
+    // columns beat thousands of Jacobi ones on the CSR kernels
+    int min_kp = 4;
+    if (o.precision == 0 && o.precond != 1 && (o.layout == 0 || o.layout == 2) && nseg == 1 && k < 16) {
+        const std::vector<int> g = grids_of(s, 1, std::vector<int>{0, s->ndof});
+        if (g.size() == 3 && (long long)g[1] * g[2] > kMGCoarse) min_kp = 16;
+    }
+    const Plan P = make_plan(s->ndof, k, s->ctx->num_cu, nseg, seg_rows, min_kp);
     if (o.precision == 1) return dispatch_solve<double>(s, P, trans, k, B, ldb, X, ldx, o, stats);
     return dispatch_solve<float>(s, P, trans, k, B, ldb, X, ldx, o, stats);
 }
@@ -1473,11 +1863,12 @@ int solve_on(sfh_system* s, hipStream_t st, SolveWork& w, int trans, int k, cons
 
 extern "C" void sfh_solve_opts_default(sfh_solve_opts* o) {
     o->tol = 1e-10;
-    o->inner_tol = 0.01;
+    o->inner_tol = 0.0;
     o->max_outer = 500;
     o->max_inner = 50000;
-    o->check_every = 32;
+    o->check_every = 0;
     o->precision = 0;
     o->rhs_tile = 0;
     o->layout = 0;
+    o->precond = 0;
 }

@@ -33,12 +33,13 @@ class SfhError(RuntimeError):
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("inner_tol", C.c_double), ("max_outer", C.c_int),
                 ("max_inner", C.c_int), ("check_every", C.c_int), ("precision", C.c_int),
-                ("rhs_tile", C.c_int), ("layout", C.c_int)]
+                ("rhs_tile", C.c_int), ("layout", C.c_int), ("precond", C.c_int)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("outer", C.c_int), ("inner_iters", C.c_int), ("max_relres", C.c_double),
-                ("ms", C.c_double), ("converged", C.c_int), ("layout", C.c_int)]
+                ("ms", C.c_double), ("converged", C.c_int), ("layout", C.c_int), ("precond", C.c_int),
+                ("levels", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -193,10 +194,10 @@ class Context:
         _check(lib().sfh_kernel_sampling(self.h, every, max_samples, min_rhs))
 
     def kernel_times(self):
-        ms = np.zeros(2)
+        ms = np.zeros(3)
         n = C.c_longlong()
         _check(lib().sfh_kernel_times(self.h, _p(ms), C.byref(n)))
-        return {"dir_spmv": float(ms[0]), "cg_update": float(ms[1]), "samples": n.value}
+        return {"dir_spmv": float(ms[0]), "cg_update": float(ms[1]), "vcycle": float(ms[2]), "samples": n.value}
 
     def timer_start(self):
         _check(lib().sfh_timer_start(self.h))

@@ -1,0 +1,126 @@
+"""Multigrid-preconditioned solves on 2D tensor grids against the CPU FP64
+oracle's direct solve (1e-8 relative), on odd and even node counts, square and
+rectangular grids, fully and partly constrained boundaries, ragged RHS counts
+and both orientations; the Jacobi path and the hierarchy rebuild after a new
+assembly as cross-checks."""
+import numpy as np
+import pytest
+
+from statfem_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (elements x, elements y, element kind, boundary), nodes (nx + 1) x (ny + 1)
+    ("q4_48x48_all", 48, 48, M.QUAD4, "all"),     # 49 x 49 nodes (odd)
+    ("q4_45x30_left", 45, 30, M.QUAD4, "left"),   # 46 x 31 nodes, Neumann on three sides
+    ("q4_63x47_lb", 63, 47, M.QUAD4, "lb"),       # 64 x 48 nodes (even x even)
+    ("t3_40x40_all", 40, 40, M.TRI3, "all"),      # 7-point stencil inside the 9-point layout
+]
+
+
+def _bc(m, which):
+    X = m.coords
+    if which == "all":
+        return M.dirichlet_all_boundary(m)
+    if which == "left":
+        return np.where(X[:, 0] < 1e-9)[0]
+    return np.where((X[:, 0] < 1e-9) | (X[:, 1] < 1e-9))[0]
+
+
+def _problem(orc, nx, ny, kind, which, seed=3):
+    m = M.grid2d(nx, ny, kind=kind)
+    bc = _bc(m, which)
+    rng = np.random.default_rng(seed)
+    kap = np.exp(rng.normal(size=m.nel) * 0.7)
+    nq = 2 if kind == M.QUAD4 else 1
+    rp, ci, va, _ = orc.assemble(m, orc.POISSON, kap, nq1d=nq, bc_dofs=bc)
+    return m, bc, kap, nq, rp, ci, va
+
+
+def _system(ctx, m, kap, nq, bc):
+    from statfem_amd import hip
+    s = hip.SystemSolveHip(ctx, m)
+    s.set_grid()
+    s.assemble(hip.POISSON, kap, nq1d=nq)
+    s.applyConstraints(bc)
+    return s
+
+
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("name,nx,ny,kind,which", CASES, ids=[c[0] for c in CASES])
+def test_multigrid_solve_matches_direct(ctx, orc, name, nx, ny, kind, which, trans):
+    from statfem_amd import hip
+    m, bc, kap, nq, rp, ci, va = _problem(orc, nx, ny, kind, which)
+    rng = np.random.default_rng(11)
+    k = 37
+    B = rng.normal(size=(m.nnodes, k))
+    B[:, 2] = 0.0                  # an all-zero right-hand side
+    B[:, 5] = 0.0
+    B[bc[0], 5] = 1.0              # a load on a constrained DOF only
+    s = _system(ctx, m, kap, nq, bc)
+    X, st = s.solve(B, trans=trans, opts=hip.default_opts(tol=1e-12, rhs_tile=16))
+    Xr = orc.LU(rp, ci, va).solve(B, trans=trans)
+    Xg = X.download()
+    err = np.linalg.norm(Xg - Xr, axis=0) / np.maximum(np.linalg.norm(Xr, axis=0), 1e-300)
+    assert st["precond"] == 2 and st["layout"] == 2 and st["levels"] >= 3, st
+    assert st["converged"] == 1 and st["max_relres"] <= 1e-12, st
+    assert err.max() < 1e-8, (err.max(), st)
+    assert np.all(Xg[:, 2] == 0.0)
+    # three column tiles; a multigrid iteration cuts the residual by about 10x
+    assert st["inner_iters"] < 3 * 60, st
+
+
+def test_multigrid_agrees_with_jacobi_and_needs_fewer_iterations(ctx, orc):
+    from statfem_amd import hip
+    m, bc, kap, nq, rp, ci, va = _problem(orc, 64, 64, M.QUAD4, "all", seed=7)
+    rng = np.random.default_rng(12)
+    B = rng.normal(size=(m.nnodes, 64))
+    s = _system(ctx, m, kap, nq, bc)
+    Xm, sm = s.solve(B, trans=True, opts=hip.default_opts(tol=1e-11, precond=2))
+    Xj, sj = s.solve(B, trans=True, opts=hip.default_opts(tol=1e-11, precond=1))
+    assert sm["precond"] == 2 and sj["precond"] == 1 and sj["levels"] == 1
+    a, b = Xm.download(), Xj.download()
+    assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+    assert sm["inner_iters"] * 8 < sj["inner_iters"], (sm, sj)
+
+
+def test_multigrid_not_used_where_it_does_not_apply(ctx, orc):
+    from statfem_amd import hip
+    # CSR layout: Jacobi whatever precond asks
+    m, bc, kap, nq, rp, ci, va = _problem(orc, 40, 40, M.QUAD4, "all")
+    s = _system(ctx, m, kap, nq, bc)
+    B = np.random.default_rng(1).normal(size=(m.nnodes, 16))
+    X, st = s.solve(B, opts=hip.default_opts(tol=1e-10, layout=1, precond=2))
+    assert st["precond"] == 1 and st["layout"] == 1
+    Xr = orc.LU(rp, ci, va).solve(B)
+    assert np.abs(X.download() - Xr).max() <= 1e-8 * np.abs(Xr).max()
+    # a grid of at most 160 nodes has no coarse level
+    m, bc, kap, nq, rp, ci, va = _problem(orc, 10, 10, M.QUAD4, "all")
+    s = _system(ctx, m, kap, nq, bc)
+    B = np.random.default_rng(2).normal(size=(m.nnodes, 16))
+    X, st = s.solve(B, opts=hip.default_opts(tol=1e-10))
+    assert st["precond"] == 1 and st["layout"] == 2
+    Xr = orc.LU(rp, ci, va).solve(B)
+    assert np.abs(X.download() - Xr).max() <= 1e-8 * np.abs(Xr).max()
+
+
+def test_multigrid_hierarchy_follows_reassembly(ctx, orc):
+    """A new assembly (other coefficients) must rebuild the coarse operators:
+    the second solve matches the second operator's direct solve."""
+    from statfem_amd import hip
+    m = M.grid2d(50, 50)
+    bc = M.dirichlet_all_boundary(m)
+    rng = np.random.default_rng(21)
+    k1, k2 = np.exp(rng.normal(size=m.nel)), np.exp(2.0 * rng.normal(size=m.nel))
+    B = rng.normal(size=(m.nnodes, 16))
+    s = hip.SystemSolveHip(ctx, m)
+    s.set_grid()
+    for kap in (k1, k2):
+        s.assemble(hip.POISSON, kap, nq1d=2)   # overwrites the values and the constraints
+        s.applyConstraints(bc)
+        X, st = s.solve(B, opts=hip.default_opts(tol=1e-12))
+        rp, ci, va, _ = orc.assemble(m, orc.POISSON, kap, nq1d=2, bc_dofs=bc)
+        Xr = orc.LU(rp, ci, va).solve(B)
+        assert st["precond"] == 2
+        assert np.abs(X.download() - Xr).max() <= 1e-8 * np.abs(Xr).max(), st
