@@ -102,7 +102,23 @@ def gpu_run(args):
     N, nnz = s.ndof, s.nnz
     kp = (kk + 3) // 4 * 4
     vec = 4.0 * N * kp                                    # one FP32 N x kp block
-    if stats[-1]["layout"] == 2:
+    if stats[-1]["layout"] == 2 and stats[-1]["precond"] == 2:
+        # multigrid-preconditioned iteration: 9 FP64 stencil coefficients per
+        # node; direction: z, p_old read, p written (FP32); update: p read
+        # (FP32), FP64 x and r read and written
+        algo = {"dir_spmv": 72.0 * N + 3 * vec, "cg_update": 72.0 * N + 9 * vec}
+        kname = {"dir_spmv": "k_dir_spmv_mg<{lpr}>", "cg_update": "k_cg_update_mg<{lpr}>"}
+        # V-cycle, all levels: per level FP32 stencil + smoother weight (40 B
+        # per node, read twice); level 0 reads the FP64 residual twice and
+        # writes z; coarse level l reads r_l twice and writes r_l, e_l; the
+        # restricted residual and the coarse correction add a quarter vector
+        vc, Nl = 0.0, float(N)
+        for lev in range(stats[-1]["levels"]):
+            v = 4.0 * Nl * kp
+            vc += 80.0 * Nl + (5.0 * v if lev == 0 else 4.0 * v) + 0.5 * v
+            Nl = Nl / 4.0
+        algo_vc = vc
+    elif stats[-1]["layout"] == 2:
         # 9 FP32 stencil coefficients per node (+ D^-1 and diag for the update)
         algo = {"dir_spmv": 36.0 * N + 3 * vec, "cg_update": 44.0 * N + 5 * vec}
         kname = {"dir_spmv": "k_dir_spmv_t2<{lpr}>", "cg_update": "k_cg_update_t2<{lpr}>"}
@@ -112,8 +128,11 @@ def gpu_run(args):
         kname = {"dir_spmv": "k_dir_spmv<float, {lpr}>", "cg_update": "k_cg_update<float, {lpr}>"}
     per = {k: {"ms": kt[k], "GBs": algo[k] / (kt[k] * 1e-3) / 1e9, "bytes": algo[k], "samples": kt["samples"]}
            for k in algo}
+    if stats[-1]["precond"] == 2 and kt.get("vcycle", 0) > 0:
+        per["vcycle"] = {"ms": kt["vcycle"], "GBs": algo_vc / (kt["vcycle"] * 1e-3) / 1e9, "bytes": algo_vc,
+                         "samples": kt["samples"], "launches": 2 * (stats[-1]["levels"] - 1) + 1}
     dom = max(("dir_spmv", "cg_update"), key=lambda k: kt[k])
-    lpr = min(64, 1 << max(0, (kp // 4 - 1).bit_length()))
+    lpr = min(16 if stats[-1]["layout"] == 2 else 64, 1 << max(0, (kp // 4 - 1).bit_length()))
     for k in algo:
         per[k]["traffic"] = profiled_traffic(kname[k].format(lpr=lpr))
     asm_bytes = m.nnodes * 2 * 8 + m.nel * 4 * 4 + m.nel * 8 + nnz * 8 + nnz * 4
@@ -123,6 +142,7 @@ def gpu_run(args):
         "inner_iters": stats[-1]["inner_iters"], "outer": stats[-1]["outer"],
         "max_relres": max(st["max_relres"] for st in stats),
         "kernel": dom, "kernels": per, "layout": {1: "csr", 2: "stencil9"}.get(stats[-1]["layout"], "?"),
+        "precond": {1: "jacobi", 2: "multigrid"}.get(stats[-1]["precond"], "?"), "mg_levels": stats[-1]["levels"],
         "assembly": {"ms": asm_ms, "GBs": asm_bytes / (asm_ms * 1e-3) / 1e9, "bytes": asm_bytes},
         "dense": dense,
         "N": N, "nnz": nnz,
@@ -354,7 +374,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--sample-every", type=int, default=16,
                     help="time every n-th inner PCG iteration with HIP events inside the timed solves")
-    ap.add_argument("--tile", type=int, default=64, help="right-hand sides per PCG column tile")
+    ap.add_argument("--tile", type=int, default=0, help="right-hand sides per solve (0: all in one solve)")
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-rhs", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
@@ -406,6 +426,7 @@ def main():
                                f"{args.ns} sensors/RHS: assembly + A^-T P^T + C_f + Gram + A^-1 C_f R^T",
                    "dofs": res["N"], "rhs": args.ns, "nnz": res["nnz"], "tol": args.tol, "rhs_tile": args.tile,
                    "inner_iters": res["inner_iters"], "max_relres": res["max_relres"], "operator_layout": res["layout"],
+                   "precond": res["precond"], "mg_levels": res["mg_levels"],
                    "kernels": res["kernels"], "assembly": res["assembly"]},
         "roofline": {"kernel": res["kernel"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": dom.get("traffic")},

@@ -125,10 +125,11 @@ __device__ inline double rz_at(int j, int ks, int c, const double* __restrict__ 
 
 // Direction coefficients of column c at iteration j: p = m z + b p_old with
 // b = beta, m = 1 for a live column and b = 1, m = 0 for a frozen or inactive
-// one. A live column whose recursive residual r.r reached its trigger is
-// frozen here (until the next FP64 replacement); every work-group takes the
-// same decision from the same inputs, so the flag and the r.z of the column's
-// last step are written with the same values by all of them.
+// one. A live column whose recursive residual r.r reached its trigger, or whose
+// r.z is no longer positive (the FP32 recurrence has reached its rounding
+// floor), is frozen here (until the next FP64 replacement); every work-group
+// takes the same decision from the same inputs, so the flag and the r.z of the
+// column's last step are written with the same values by all of them.
 template <class T>
 __device__ inline void dir_coef(int j, int ks, int c, double* S, const double* __restrict__ acc, T& b, T& m) {
     const bool act = S[S_ACTIVE * ks + c] > 0.0;
@@ -136,7 +137,8 @@ __device__ inline void dir_coef(int j, int ks, int c, double* S, const double* _
     double beta = S[S_BETA * ks + c];
     if (j > 0 && act) {
         frozen = S[S_FROZEN * ks + c] > 0.0;
-        if (!frozen && !(acc[(A_RR + (j - 1) % 3) * ks + c] > S[S_TRIG * ks + c])) {
+        if (!frozen && (!(acc[(A_RR + (j - 1) % 3) * ks + c] > S[S_TRIG * ks + c]) ||
+                        !(rz_at(j, ks, c, S, acc) > 0.0))) {
             frozen = true;
             S[S_FROZEN * ks + c] = 1.0;
             S[S_RZPREV * ks + c] = rz_at(j - 1, ks, c, S, acc);
@@ -459,7 +461,7 @@ __device__ inline Tile grid_tile_t(int kp, const int* __restrict__ gtab, int* d)
     t.lane = threadIdx.x % LPR;
     t.grp = threadIdx.x / LPR;
     t.ngrp = blockDim.x / LPR;
-    t.col0 = t.lane * 4;
+    t.col0 = (blockIdx.y * LPR + t.lane) * 4;   // column tile blockIdx.y of 4 * LPR columns
     t.bid = xcd_block(blockIdx.x, gridDim.x);
 #pragma unroll
     for (int i = 0; i < kGT; ++i) d[i] = gtab[kGT * t.bid + i];
@@ -535,10 +537,7 @@ __global__ __launch_bounds__(kG) void k_dir_spmv_t2(const int* __restrict__ gtab
     block_accumulate<LPR>(t, kp, ad, reinterpret_cast<double*>(sm), acc + (A_PQ + (it & 1)) * ks + t.so);
 }
 
-// MG = false: z = D^-1 r is the stored vector (Jacobi); MG = true: the FP32
-// residual r itself is updated and only r.r is summed (r.z comes from the
-// multigrid preconditioner that follows)
-template <int LPR, bool MG>
+template <int LPR>
 __global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gtab, int kp, int ks, int it,
                                                      const float* __restrict__ coef, const float* __restrict__ dinv,
                                                      const float* __restrict__ diag, const double* __restrict__ S,
@@ -575,28 +574,144 @@ __global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gta
             const V4<float> pi = v4(sm[((oy + 1) * (kTX + 2) + (ox + 1)) * LPR + t.lane]);
             const size_t off = (size_t)nd * kp + t.col0;
             V4<float> xv = ld4(x + off), zv = ld4(z + off);
+            const float dd = dinv[nd], g = diag[nd];
             xv.x += al.x * pi.x; xv.y += al.y * pi.y; xv.z += al.z * pi.z; xv.w += al.w * pi.w;
-            if constexpr (MG) {
-                zv.x -= al.x * q.x; zv.y -= al.y * q.y; zv.z -= al.z * q.z; zv.w -= al.w * q.w;
-                rr.x += zv.x * zv.x; rr.y += zv.y * zv.y; rr.z += zv.z * zv.z; rr.w += zv.w * zv.w;
-            } else {
-                const float dd = dinv[nd], g = diag[nd];
-                zv.x -= al.x * dd * q.x; zv.y -= al.y * dd * q.y; zv.z -= al.z * dd * q.z; zv.w -= al.w * dd * q.w;
-                res_dots(zv, g, rz, rr);
-            }
+            zv.x -= al.x * dd * q.x; zv.y -= al.y * dd * q.y; zv.z -= al.z * dd * q.z; zv.w -= al.w * dd * q.w;
             st4(x + off, xv);
             st4(z + off, zv);
+            res_dots(zv, g, rz, rr);
         }
     }
     __syncthreads();
     double* red = reinterpret_cast<double*>(sm);
-    if constexpr (!MG) {
-        V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
-        block_accumulate<LPR>(t, kp, a, red, acc + (A_RZ + it % 3) * ks + t.so);
-        __syncthreads();
-    }
+    V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
+    block_accumulate<LPR>(t, kp, a, red, acc + (A_RZ + it % 3) * ks + t.so);
+    __syncthreads();
     V4<double> bb = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
     block_accumulate<LPR>(t, kp, bb, red, acc + (A_RR + it % 3) * ks + t.so);
+}
+
+// ------------------------------------------------ multigrid-preconditioned iteration (2D grids)
+// With the multigrid preconditioner one iteration reduces the residual about
+// tenfold, so the recurrences must stay accurate far below FP32's reach: for a
+// smooth right-hand side on a fine grid, FP32 iterates and FP32 stencil
+// products (whose 9 terms cancel down to the small A p of a smooth p) lose
+// the true residual at about eps32 times the condition number. Here the
+// iterate x and residual r are FP64 and q = A p is formed from the FP64
+// stencil with FP64 sums; the direction p and the preconditioned residual z
+// stay FP32 (p is exact as stored, so x += alpha p and r -= alpha A p remain
+// consistent to FP64 rounding whatever p's precision). Vector traffic per
+// DOF-RHS and iteration: p (staged), x and r read and written.
+
+// q = FP64 9-point product at tile node (ox, oy) from the staged FP32 values
+template <int LPR>
+__device__ inline V4<double> stencil9d(const float4* sm, const double* __restrict__ c, int ox, int oy, int lane) {
+    V4<double> q = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            const double a = c[dy * 3 + dx];
+            const float4 v = sm[((oy + dy) * (kTX + 2) + (ox + dx)) * LPR + lane];
+            q.x += a * v.x; q.y += a * v.y; q.z += a * v.z; q.w += a * v.w;
+        }
+    }
+    return q;
+}
+
+// p_new = m z + b p_old (FP32), p.Ap in FP64 from the FP64 stencil
+template <int LPR>
+__global__ __launch_bounds__(kG) void k_dir_spmv_mg(const int* __restrict__ gtab, int kp, int ks, int it,
+                                                    const double* __restrict__ coef, double* __restrict__ S,
+                                                    const float* __restrict__ z, const float* __restrict__ pold,
+                                                    float* __restrict__ pnew, double* __restrict__ acc) {
+    extern __shared__ float4 sm[];
+    int d[kGT];
+    clear_slots(acc, A_RZ + it % 3, 1, ks);
+    clear_slots(acc, A_RR + it % 3, 1, ks);
+    const Tile t = grid_tile_t<LPR>(kp, gtab, d);
+    const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
+    if (W == 0) return;
+    if (block_done(t, kp, ks, S)) return;
+    __shared__ float cb[4 * LPR], cm[4 * LPR];
+    shared_dir_coef<float, LPR>(it, kp, ks, t.so, S, acc, cb, cm);
+    const bool ok = t.col0 < kp;
+    const int lc = t.lane * 4;
+    const V4<float> b = {cb[lc], cb[lc + 1], cb[lc + 2], cb[lc + 3]};
+    const V4<float> m = {cm[lc], cm[lc + 1], cm[lc + 2], cm[lc + 3]};
+    for (int h = t.grp; h < kHN; h += t.ngrp) {
+        const int hx = h % (kTX + 2), hy = h / (kTX + 2);
+        const int gx = tx0 + hx - 1, gy = ty0 + hy - 1;
+        V4<float> v = {0.f, 0.f, 0.f, 0.f};
+        if (ok && gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const size_t o = (size_t)(base + gy * W + gx) * kp + t.col0;
+            pdir4(ld4(z + o), m, b, ld4(pold + o), v);
+            if (hx >= 1 && hx <= kTX && hy >= 1 && hy <= kTY) st4(pnew + o, v);
+        }
+        sm[h * LPR + t.lane] = f4(v);
+    }
+    __syncthreads();
+    V4<double> pq = {0.0, 0.0, 0.0, 0.0};
+    if (ok) {
+        for (int o = t.grp; o < kTX * kTY; o += t.ngrp) {
+            const int ox = o % kTX, oy = o / kTX;
+            const int gx = tx0 + ox, gy = ty0 + oy;
+            if (gx >= W || gy >= H) continue;
+            const V4<double> q = stencil9d<LPR>(sm, coef + (size_t)(base + gy * W + gx) * 9, ox, oy, t.lane);
+            const float4 pi = sm[((oy + 1) * (kTX + 2) + (ox + 1)) * LPR + t.lane];
+            pq.x += pi.x * q.x; pq.y += pi.y * q.y; pq.z += pi.z * q.z; pq.w += pi.w * q.w;
+        }
+    }
+    __syncthreads();
+    block_accumulate<LPR>(t, kp, pq, reinterpret_cast<double*>(sm), acc + (A_PQ + (it & 1)) * ks + t.so);
+}
+
+// x += alpha p, r -= alpha A p (FP64 x, r, stencil and sums), r.r
+template <int LPR>
+__global__ __launch_bounds__(kG) void k_cg_update_mg(const int* __restrict__ gtab, int kp, int ks, int it,
+                                                     const double* __restrict__ coef, const double* __restrict__ S,
+                                                     const float* __restrict__ p, double* __restrict__ x,
+                                                     double* __restrict__ r, double* __restrict__ acc) {
+    extern __shared__ float4 sm[];
+    int d[kGT];
+    clear_slots(acc, A_PQ + ((it + 1) & 1), 1, ks);
+    const Tile t = grid_tile_t<LPR>(kp, gtab, d);
+    const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
+    if (W == 0) return;
+    if (block_done(t, kp, ks, S)) return;
+    __shared__ double ca[4 * LPR];
+    shared_alpha<double, LPR>(it, kp, ks, t.so, S, acc, ca);
+    const bool ok = t.col0 < kp;
+    for (int h = t.grp; h < kHN; h += t.ngrp) {
+        const int hx = h % (kTX + 2), hy = h / (kTX + 2);
+        const int gx = tx0 + hx - 1, gy = ty0 + hy - 1;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok && gx >= 0 && gx < W && gy >= 0 && gy < H)
+            v = *reinterpret_cast<const float4*>(p + (size_t)(base + gy * W + gx) * kp + t.col0);
+        sm[h * LPR + t.lane] = v;
+    }
+    __syncthreads();
+    V4<double> rr = {0.0, 0.0, 0.0, 0.0};
+    if (ok) {
+        const V4<double> al = {ca[t.lane * 4], ca[t.lane * 4 + 1], ca[t.lane * 4 + 2], ca[t.lane * 4 + 3]};
+        for (int o = t.grp; o < kTX * kTY; o += t.ngrp) {
+            const int ox = o % kTX, oy = o / kTX;
+            const int gx = tx0 + ox, gy = ty0 + oy;
+            if (gx >= W || gy >= H) continue;
+            const int nd = base + gy * W + gx;
+            const V4<double> q = stencil9d<LPR>(sm, coef + (size_t)nd * 9, ox, oy, t.lane);
+            const float4 pi = sm[((oy + 1) * (kTX + 2) + (ox + 1)) * LPR + t.lane];
+            const size_t off = (size_t)nd * kp + t.col0;
+            V4<double> xv = ld4(x + off), rv = ld4(r + off);
+            xv.x += al.x * pi.x; xv.y += al.y * pi.y; xv.z += al.z * pi.z; xv.w += al.w * pi.w;
+            rv.x -= al.x * q.x; rv.y -= al.y * q.y; rv.z -= al.z * q.z; rv.w -= al.w * q.w;
+            st4(x + off, xv);
+            st4(r + off, rv);
+            rr.x += rv.x * rv.x; rr.y += rv.y * rv.y; rr.z += rv.z * rv.z; rr.w += rv.w * rv.w;
+        }
+    }
+    __syncthreads();
+    block_accumulate<LPR>(t, kp, rr, reinterpret_cast<double*>(sm), acc + (A_RR + it % 3) * ks + t.so);
 }
 
 // ------------------------------------------------ multigrid V-cycle (2D grids)
@@ -640,11 +755,17 @@ __device__ inline void fma4(float a, const float4& v, float4& s) {
 }
 
 // r_{l+1} = P^T (r - A (S r)) on one tile of coarse nodes
-template <int LPR>
+__device__ inline float4 ldf4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ inline float4 ldf4(const double* p) {
+    const V4<double> v = ld4(p);
+    return make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+}
+
+template <int LPR, class TR>
 __global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab, int kp, const float* __restrict__ coef,
                                                     const float* __restrict__ dsm,
                                                     const unsigned char* __restrict__ cmask,
-                                                    const float* __restrict__ r, float* __restrict__ rc) {
+                                                    const TR* __restrict__ r, float* __restrict__ rc) {
     extern __shared__ float4 sm[];
     float* sd = reinterpret_cast<float*>(sm + kRW * kRH * LPR);
     constexpr int ngrp = kG / LPR;
@@ -654,7 +775,7 @@ __global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab,
     const int* d = tab + kMT * xcd_block(blockIdx.x, gridDim.x);
     const int fbase = d[0], W = d[1], H = d[2], cbase = d[3], Wc = d[4], Hc = d[5], cx0 = d[6], cy0 = d[7];
     if (W == 0) return;
-    const int col0 = lane * 4;
+    const int col0 = (blockIdx.y * LPR + lane) * 4;   // column tile blockIdx.y of 4 * LPR columns
     const bool ok = col0 < kp;
     const int fx0 = 2 * cx0 - 2, fy0 = 2 * cy0 - 2;
     for (int h = grp; h < kRW * kRH; h += ngrp) {
@@ -663,7 +784,7 @@ __global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab,
         float w = 0.f;
         if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
             const int nd = fbase + gy * W + gx;
-            if (ok) v = *reinterpret_cast<const float4*>(r + (size_t)nd * kp + col0);
+            if (ok) v = ldf4(r + (size_t)nd * kp + col0);
             w = dsm[nd];
         }
         sm[h * LPR + lane] = v;
@@ -723,15 +844,16 @@ __global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab,
 }
 
 // e = e0 + S (r - A e0), e0 = S r + P e_{l+1}, on one tile of fine nodes. TOP
-// (level 0): P acts on free nodes only, e is zero on constrained nodes, and the
-// column sums r.e (and r.r) are formed: mode 0 adds r.e into the iteration's
-// accumulator slot, mode 1 writes per-tile partial rows of r.e and r.r (after
-// a residual replacement; padding tiles write zeros)
-template <int LPR, bool TOP>
+// (level 0, FP64 residual): P acts on free nodes only, e is zero on
+// constrained nodes, and the column sums r.e (and r.r) are formed in FP64:
+// mode 0 adds r.e into the iteration's accumulator slot, mode 1 writes
+// per-tile partial rows of r.e and r.r (after a residual replacement; padding
+// tiles write zeros)
+template <int LPR, bool TOP, class TR>
 __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, int kp, int ks, int it, int mode,
                                                    const float* __restrict__ coef, const float* __restrict__ dsm,
                                                    const unsigned char* __restrict__ cmask,
-                                                   const float* __restrict__ r, const float* __restrict__ ec,
+                                                   const TR* __restrict__ r, const float* __restrict__ ec,
                                                    float* __restrict__ e, double* __restrict__ acc,
                                                    double* __restrict__ part0, double* __restrict__ part1) {
     extern __shared__ float4 sm[];
@@ -740,7 +862,7 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
     t.lane = threadIdx.x % LPR;
     t.grp = threadIdx.x / LPR;
     t.ngrp = kG / LPR;
-    t.col0 = t.lane * 4;
+    t.col0 = (blockIdx.y * LPR + t.lane) * 4;
     t.bid = xcd_block(blockIdx.x, gridDim.x);
     t.r0 = t.r1 = 0;
     const int* d = tab + kMT * t.bid;
@@ -748,7 +870,7 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
     t.so = d[8] * kp;
     if (W == 0 && !(TOP && mode == 1)) return;
     const bool ok = t.col0 < kp;
-    V4<float> rz = {0.f, 0.f, 0.f, 0.f}, rr = {0.f, 0.f, 0.f, 0.f};
+    V4<double> rz = {0.0, 0.0, 0.0, 0.0}, rr = {0.0, 0.0, 0.0, 0.0};
     if (W > 0) {
         const int cx0 = tx0 / 2 - 1, cy0 = ty0 / 2 - 1;
         for (int h = t.grp; h < kCW * kCH; h += t.ngrp) {
@@ -765,7 +887,7 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
             if (ok && gx >= 0 && gx < W && gy >= 0 && gy < H) {
                 const int nd = fbase + gy * W + gx;
                 if (!(TOP && cmask[nd])) {
-                    const float4 rv = *reinterpret_cast<const float4*>(r + (size_t)nd * kp + t.col0);
+                    const float4 rv = ldf4(r + (size_t)nd * kp + t.col0);
                     const float w = dsm[nd];
                     v = make_float4(w * rv.x, w * rv.y, w * rv.z, w * rv.w);
                     const int ax = (gx & 1) ? (gx - 1) / 2 : gx / 2;   // first coarse parent
@@ -788,7 +910,7 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
                 if (gx >= W || gy >= H) continue;
                 const int nd = fbase + gy * W + gx;
                 const float* c = coef + (size_t)nd * 9;
-                const float4 rv = *reinterpret_cast<const float4*>(r + (size_t)nd * kp + t.col0);
+                const float4 rv = ldf4(r + (size_t)nd * kp + t.col0);
                 float4 q = rv;
 #pragma unroll
                 for (int k = 0; k < 9; ++k)
@@ -798,9 +920,10 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
                 fma4(w, q, z);
                 if (TOP && cmask[nd]) z = make_float4(0.f, 0.f, 0.f, 0.f);
                 *reinterpret_cast<float4*>(e + (size_t)nd * kp + t.col0) = z;
-                if (TOP) {
-                    rz.x += rv.x * z.x; rz.y += rv.y * z.y; rz.z += rv.z * z.z; rz.w += rv.w * z.w;
-                    rr.x += rv.x * rv.x; rr.y += rv.y * rv.y; rr.z += rv.z * rv.z; rr.w += rv.w * rv.w;
+                if constexpr (TOP) {
+                    const V4<double> rd = ld4(r + (size_t)nd * kp + t.col0);   // cache hit: just loaded
+                    rz.x += rd.x * z.x; rz.y += rd.y * z.y; rz.z += rd.z * z.z; rz.w += rd.w * z.w;
+                    rr.x += rd.x * rd.x; rr.y += rd.y * rd.y; rr.z += rd.z * rd.z; rr.w += rd.w * rd.w;
                 }
             }
         }
@@ -808,14 +931,12 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
     if constexpr (TOP) {
         __syncthreads();   // the staging area becomes the reduction buffer
         double* red = reinterpret_cast<double*>(sm);
-        const V4<double> a = {(double)rz.x, (double)rz.y, (double)rz.z, (double)rz.w};
         if (mode == 0) {
-            block_accumulate<LPR>(t, kp, a, red, acc + (A_RZ + it % 3) * ks + t.so);
+            block_accumulate<LPR>(t, kp, rz, red, acc + (A_RZ + it % 3) * ks + t.so);
         } else {
-            block_partial<LPR>(t, kp, a, red, part0);
+            block_partial<LPR>(t, kp, rz, red, part0);
             __syncthreads();
-            const V4<double> b = {(double)rr.x, (double)rr.y, (double)rr.z, (double)rr.w};
-            block_partial<LPR>(t, kp, b, red, part1);
+            block_partial<LPR>(t, kp, rr, red, part1);
         }
     }
 }
@@ -832,15 +953,14 @@ __global__ void k_mg_coarse(int n, int kp, const float* __restrict__ cinv, const
     *reinterpret_cast<float4*>(e + (size_t)i * kp + c) = s;
 }
 
-// residual replacement with the multigrid preconditioner: r = s' r64 (FP32),
-// p *= s'/s (one problem: scalars indexed by column)
+// residual replacement with the multigrid preconditioner: r *= s' in place
+// (FP64), p *= s'/s (one problem: scalars indexed by column)
 __global__ void k_mg_rescale(long long total, int kp, const double* __restrict__ scale,
-                             const double* __restrict__ ratio, const double* __restrict__ r64, float* __restrict__ r,
-                             float* __restrict__ p) {
+                             const double* __restrict__ ratio, double* __restrict__ r, float* __restrict__ p) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const int c = (int)(i % kp);
-    r[i] = (float)(r64[i] * scale[c]);
+    r[i] *= scale[c];
     p[i] *= (float)ratio[c];
 }
 
@@ -1110,10 +1230,15 @@ __device__ inline double sum_parts(int b0, int nblk, int kp, const double* __res
 // After residual replacement: rz from the new residual; beta against the
 // rescaled r.z of the column's last step (jlast = last iteration of the cycle
 // that ended, -1 for none; a frozen column's is kept in S_RZPREV) since the
-// direction is kept; next trigger level; a fresh cycle (no frozen columns, all
-// accumulator slots cleared). One column group of one segment per work-group
+// direction is kept -- unless beta exceeds kBetaRestart: the true residual is
+// then far above the recursive one the direction was built for (the FP32
+// recurrence ran into its rounding floor) and the cycle restarts from p = z;
+// next trigger level; a fresh cycle (no frozen columns, all accumulator slots
+// cleared). One column group of one segment per work-group
 // (blockIdx.y = segment, whose partial rows are work-groups sb[seg] ..
 // sb[seg+1]-1); slot stride ks.
+constexpr double kBetaRestart = 4.0;
+
 __global__ __launch_bounds__(kRedB) void k_red_reset(int kp, int ks, const int* __restrict__ sb, const double* part0,
                                                      const double* part1, double* S, double* acc, double delta2,
                                                      int jlast) {
@@ -1131,7 +1256,8 @@ __global__ __launch_bounds__(kRedB) void k_red_reset(int kp, int ks, const int* 
         const double q = S[S_RATIO * ks + col];
         const double prev = last * q * q;
         S[S_ACTIVE * ks + col] = act ? 1.0 : 0.0;
-        S[S_BETA * ks + col] = (act && prev > 0.0) ? rzn / prev : 0.0;
+        const double beta = (act && prev > 0.0) ? rzn / prev : 0.0;
+        S[S_BETA * ks + col] = beta > kBetaRestart ? 0.0 : beta;
         S[S_RZ * ks + col] = act ? rzn : 0.0;
         S[S_RZPREV * ks + col] = 0.0;
         S[S_FROZEN * ks + col] = 0.0;
@@ -1248,11 +1374,12 @@ struct Plan {
     const int* d_mgtab = nullptr;
 };
 
-Plan make_plan(int n, int k, int num_cu, int nseg = 1, const int* seg_rows = nullptr, int min_kp = 4) {
+Plan make_plan(int n, int k, int num_cu, int nseg = 1, const int* seg_rows = nullptr, int min_kp = 4,
+               int max_lpr = 64) {
     Plan p;
     p.n = n;
     p.kp = std::max((k + 3) / 4 * 4, min_kp);
-    p.lpr = std::min(64, pow2_at_least(p.kp / 4));
+    p.lpr = std::min(max_lpr, pow2_at_least(p.kp / 4));
     p.ntile = (p.kp + 4 * p.lpr - 1) / (4 * p.lpr);
     const int rows_per_pass = kB / p.lpr;
     int target = std::max(8, 4 * num_cu / p.ntile);
@@ -1366,43 +1493,52 @@ struct Kern {
     static void dir_spmv_g(const Plan& P, hipStream_t st, int it, const T* coef, double* S, const T* z, const T* pold,
                            T* pnew, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_dir_spmv_t2<LPR><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, S, z, pold,
+            k_dir_spmv_t2<LPR><<<dim3(P.gblk, P.ntile), kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, S, z, pold,
                                                                        pnew, acc);
     }
     static void update_g(const Plan& P, hipStream_t st, int it, const T* coef, const T* dinv, const T* diag,
                          const double* S, const T* p, T* x, T* z, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_cg_update_t2<LPR, false><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, dinv,
-                                                                               diag, S, p, x, z, acc);
+            k_cg_update_t2<LPR><<<dim3(P.gblk, P.ntile), kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, dinv, diag,
+                                                                        S, p, x, z, acc);
     }
-    // multigrid-preconditioned variant: x += alpha p, r -= alpha A p, r.r
-    static void update_g_mg(const Plan& P, hipStream_t st, int it, const T* coef, const double* S, const T* p, T* x,
-                            T* r, double* acc) {
+    // multigrid-preconditioned iteration: FP64 x, r and stencil products
+    static void dir_spmv_mg(const Plan& P, hipStream_t st, int it, const double* coef64, double* S, const T* z,
+                            const T* pold, T* pnew, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_cg_update_t2<LPR, true><<<P.gblk, kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, nullptr,
-                                                                              nullptr, S, p, x, r, acc);
+            k_dir_spmv_mg<LPR><<<dim3(P.gblk, P.ntile), kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef64, S, z, pold,
+                                                                       pnew, acc);
+    }
+    static void update_mg(const Plan& P, hipStream_t st, int it, const double* coef64, const double* S, const T* p,
+                          double* x, double* r, double* acc) {
+        if constexpr (std::is_same<T, float>::value && LPR <= 16)
+            k_cg_update_mg<LPR><<<dim3(P.gblk, P.ntile), kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef64, S, p, x, r,
+                                                                        acc);
     }
     // z = M r (one V-cycle) with r.z into the iteration's slot (mode 0) or r.z,
     // r.r as per-tile partial rows (mode 1)
     static void vcycle(const Plan& P, const MGHier& m, hipStream_t st, int it, int mode, const T* coef0,
-                       const unsigned char* cmask, const T* r0, T* z, T* mgv, double* acc, double* p0, double* p1) {
+                       const unsigned char* cmask, const double* r0, T* z, T* mgv, double* acc, double* p0,
+                       double* p1) {
         if constexpr (std::is_same<T, float>::value && LPR >= 4 && LPR <= 16) {
             const int L = P.mgL, kp = P.kp;
-            auto rvec = [&](int l) -> float* { return l == 0 ? const_cast<float*>(r0) : mgv + P.mgv_off[l]; };
+            auto rvec = [&](int l) -> float* { return mgv + P.mgv_off[l]; };
             auto evec = [&](int l) -> float* {
                 return l == 0 ? z : mgv + P.mgv_off[l] + (size_t)m.W[l] * m.H[l] * kp;
             };
-            for (int l = 0; l < L; ++l)
-                k_mg_restrict<LPR><<<P.mg_rn[l], kG, mg_restrict_lds(LPR), st>>>(
-                    P.d_mgtab + (size_t)kMT * P.mg_roff[l], kp, l == 0 ? coef0 : m.coef32[l].as<float>(),
-                    m.dsm[l].as<float>(), l == 0 ? cmask : nullptr, rvec(l), rvec(l + 1));
+            k_mg_restrict<LPR, double><<<dim3(P.mg_rn[0], P.ntile), kG, mg_restrict_lds(LPR), st>>>(
+                P.d_mgtab + (size_t)kMT * P.mg_roff[0], kp, coef0, m.dsm[0].as<float>(), cmask, r0, rvec(1));
+            for (int l = 1; l < L; ++l)
+                k_mg_restrict<LPR, float><<<dim3(P.mg_rn[l], P.ntile), kG, mg_restrict_lds(LPR), st>>>(
+                    P.d_mgtab + (size_t)kMT * P.mg_roff[l], kp, m.coef32[l].as<float>(), m.dsm[l].as<float>(),
+                    nullptr, rvec(l), rvec(l + 1));
             const int nL = m.W[L] * m.H[L];
             k_mg_coarse<<<(nL * (kp / 4) + 255) / 256, 256, 0, st>>>(nL, kp, m.cinv.as<float>(), rvec(L), evec(L));
             for (int l = L - 1; l >= 1; --l)
-                k_mg_prolong<LPR, false><<<P.mg_pn[l], kG, mg_prolong_lds(LPR), st>>>(
+                k_mg_prolong<LPR, false, float><<<dim3(P.mg_pn[l], P.ntile), kG, mg_prolong_lds(LPR), st>>>(
                     P.d_mgtab + (size_t)kMT * P.mg_poff[l], kp, P.ks, it, mode, m.coef32[l].as<float>(),
                     m.dsm[l].as<float>(), nullptr, rvec(l), evec(l + 1), evec(l), nullptr, nullptr, nullptr);
-            k_mg_prolong<LPR, true><<<P.mg_pn[0], kG, mg_prolong_lds(LPR), st>>>(
+            k_mg_prolong<LPR, true, double><<<dim3(P.mg_pn[0], P.ntile), kG, mg_prolong_lds(LPR), st>>>(
                 P.d_mgtab + (size_t)kMT * P.mg_poff[0], kp, P.ks, it, mode, coef0, m.dsm[0].as<float>(), cmask, r0,
                 evec(1), z, acc, p0, p1);
         }
@@ -1524,7 +1660,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     // segment grids for a batch
     const std::vector<int> grids = grids_of(s, ns, P.srow);
     const bool use_grid = !grids.empty() && (o.layout == 2 || o.layout == 0) && std::is_same<T, float>::value &&
-                          P.lpr >= 4 && P.lpr <= 16 && P.ntile == 1;
+                          P.lpr >= 4 && P.lpr <= 16;
     SFH_TRY(prepare_operator_on(s, st, use_grid ? grids : std::vector<int>()));
     if (use_grid && s->dia_state == 1) plan_grid(P, grids);
     // vector DOFs of a mesh: 3 x 3 block layout (one problem, FP32)
@@ -1549,7 +1685,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_TRY(w.scal.ensure((size_t)S_NSLOT * ks * 8));
     SFH_TRY(w.acc.ensure((size_t)A_N * ks * 8));
     if (use_mg) {
-        SFH_TRY(w.s32.ensure(nk * sizeof(T)));
+        SFH_TRY(w.s32.ensure(nk * sizeof(double)));
         SFH_TRY(w.mgv.ensure(P.mgv_total * sizeof(T)));
     }
     // partition tables: brow (nblk + 1), bseg (nblk), sb (nseg + 1), srow (nseg + 1), gtab, multigrid tiles,
@@ -1591,8 +1727,12 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     const Entries<T> M = {s->cv32.as<int2>(), ci, vaT};
     T* x = w.x32.as<T>();
     T* z = w.r32.as<T>();   // Jacobi: z = D^-1 r (r itself is not kept); multigrid: z = M r
-    T* r = use_mg ? w.s32.as<T>() : nullptr;
+    // multigrid: the cycle's FP64 correction and, in place of the true
+    // residual after each replacement, the FP64 recursive residual
+    double* xmg = use_mg ? w.s32.as<double>() : nullptr;
+    double* rmg = w.r64.as<double>();
     T* mgv = use_mg ? w.mgv.as<T>() : nullptr;
+    const double* coef64 = s->dia64.as<double>();
     const unsigned char* cmask = s->cmask.as<unsigned char>();
     T* pa = w.p32.as<T>();
     T* pb = w.q32.as<T>();
@@ -1609,6 +1749,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_LAUNCH_CHECK();
     SFH_CHECK_HIP(hipMemsetAsync(w.x64.p, 0, nk * 8, st));
     SFH_CHECK_HIP(hipMemsetAsync(w.x32.p, 0, nk * sizeof(T), st));
+    if (use_mg) SFH_CHECK_HIP(hipMemsetAsync(xmg, 0, nk * sizeof(double), st));
     SFH_CHECK_HIP(hipMemsetAsync(w.p32.p, 0, nk * sizeof(T), st));
     SFH_CHECK_HIP(hipMemsetAsync(w.scal.p, 0, (size_t)S_NSLOT * ks * 8, st));
     SFH_CHECK_HIP(hipMemsetAsync(w.acc.p, 0, (size_t)A_N * ks * 8, st));
@@ -1628,14 +1769,22 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     // FP32 reduction per refinement cycle and host check interval: a multigrid
     // iteration reduces the residual by about an order of magnitude, a Jacobi
     // one by far less
-    const double inner = (o.inner_tol > 0.0 && o.inner_tol < 1.0) ? o.inner_tol : (use_mg ? 1e-4 : 0.01);
-    const double delta2 = inner * inner;
+    // (the multigrid recurrences are FP64: one cycle down to the target)
+    const double inner = (o.inner_tol > 0.0 && o.inner_tol < 1.0) ? o.inner_tol : (use_mg ? 1e-14 : 0.01);
     int updates = 0, iters = 0;
     double maxrel = 0.0;
     bool done = false;
     std::vector<double> hs(5 * (size_t)ks);   // ACTIVE, FROZEN, SCALE, TRIG and the latest r.r per column
-    const int check = o.check_every > 0 ? o.check_every : (use_mg ? 4 : 32);
+    const int check = o.check_every > 0 ? o.check_every : (use_mg ? 2 : 32);
     int jlast = -1;                           // last iteration of the previous cycle
+    std::vector<double> rrchk(ks);            // r.r of every column at the previous check of the cycle
+    // per-column FP32 reduction target of a cycle: starts at `inner`; a column
+    // whose last cycle reached its target while the true (FP64) residual fell
+    // by much less has met the FP32 rounding floor of its right-hand side
+    // (about eps32 times the condition number for smooth ones), and its next
+    // cycles aim at what the floor allows instead of iterating below it
+    std::vector<double> dlt(ks, inner), rn2start(ks, 0.0);
+    std::vector<char> reached(ks, 0);
     for (;;) {
         // residual replacement: decide per column, rescale, keep directions
         int nact = 0;
@@ -1646,11 +1795,19 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             const bool act = bn2[c] > 0.0 && rn2[c] > tol2 * bn2[c];
             const double sn = act ? 1.0 / std::sqrt(rn2[c]) : 0.0;
             ratio[c] = (act && scale[c] > 0.0) ? sn / scale[c] : 0.0;
+            if (act && reached[c] && rn2start[c] > 0.0) {
+                const double got = std::sqrt(rn2[c] / rn2start[c]);
+                if (got > 2.0 * dlt[c]) dlt[c] = std::min(0.1, got);
+            }
+            rn2start[c] = rn2[c];
             scale[c] = sn;
-            // stop the FP32 recurrence a little below the target, then verify in FP64
-            tgt[c] = act ? 0.09 * tol2 * bn2[c] * sn * sn : 0.0;
+            // stop the FP32 recurrence at the cycle's target (the residual is
+            // rescaled to unit norm), or a little below the final target, then
+            // verify in FP64
+            tgt[c] = act ? std::max(dlt[c] * dlt[c], 0.09 * tol2 * bn2[c] * sn * sn) : 0.0;
             nact += act;
         }
+        std::fill(reached.begin(), reached.end(), 0);
         if (nact == 0) { done = true; break; }
         if (updates > o.max_outer || iters >= o.max_inner) break;
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_SCALE * ks, scale.data(), ks * 8, hipMemcpyHostToDevice, st));
@@ -1658,24 +1815,26 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * ks, tgt.data(), ks * 8, hipMemcpyHostToDevice, st));
         if (use_mg) {
             k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, S + S_SCALE * ks,
-                                                                      S + S_RATIO * ks, w.r64.as<double>(),
-                                                                      (float*)r, (float*)pa);
-            Kern<T, LPR>::vcycle(P, mg, st, 0, 1, coef, cmask, r, z, mgv, acc, p0, p1);
-            k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sbmg, p0, p1, S, acc, delta2, jlast);
+                                                                      S + S_RATIO * ks, rmg, (float*)pa);
+            Kern<T, LPR>::vcycle(P, mg, st, 0, 1, coef, cmask, rmg, z, mgv, acc, p0, p1);
+            k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sbmg, p0, p1, S, acc, 0.0, jlast);
         } else {
             Kern<T, LPR>::rescale(P, G, st, S + S_SCALE * ks, S + S_RATIO * ks, s->dinv64.as<double>(), diag,
                                   w.r64.as<double>(), z, pa, p0, p1);
-            k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb, p0, p1, S, acc, delta2, jlast);
+            k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb, p0, p1, S, acc, 0.0, jlast);
         }
         SFH_LAUNCH_CHECK();
         ++updates;
+        std::fill(rrchk.begin(), rrchk.end(), HUGE_VAL);
         // FP32 CG until some column is due for replacement: two kernels per
         // iteration, the step lengths and directions from the rotating slots
         int j = 0;
         for (;; ++j) {
             const bool smp = ismp < nsmp && iters % cx->ksample == 0;
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp], st));
-            if (P.grid)
+            if (use_mg)
+                Kern<T, LPR>::dir_spmv_mg(P, st, j, coef64, S, z, pa, pb, acc);
+            else if (P.grid)
                 Kern<T, LPR>::dir_spmv_g(P, st, j, coef, S, z, pa, pb, acc);
             else if (P.bsr)
                 Kern<T, LPR>::dir_spmv_b(P, st, j, s->rowptr_node.as<int>(), s->colind_node.as<int>(),
@@ -1684,7 +1843,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                 Kern<T, LPR>::dir_spmv(P, G, st, j, rp, M, S, z, pa, pb, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 1], st));
             if (use_mg)
-                Kern<T, LPR>::update_g_mg(P, st, j, coef, S, pb, x, r, acc);
+                Kern<T, LPR>::update_mg(P, st, j, coef64, S, pb, xmg, rmg, acc);
             else if (P.grid)
                 Kern<T, LPR>::update_g(P, st, j, coef, dinv, diag, S, pb, x, z, acc);
             else if (P.bsr)
@@ -1693,7 +1852,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             else
                 Kern<T, LPR>::update(P, G, st, j, rp, M, dinv, diag, S, pb, x, z, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 2], st));
-            if (use_mg) Kern<T, LPR>::vcycle(P, mg, st, j, 0, coef, cmask, r, z, mgv, acc, p0, p1);
+            if (use_mg) Kern<T, LPR>::vcycle(P, mg, st, j, 0, coef, cmask, rmg, z, mgv, acc, p0, p1);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 3], st));
             ismp += smp;
             std::swap(pa, pb);
@@ -1710,20 +1869,34 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                 SFH_CHECK_HIP(hipMemcpyAsync(hs.data() + 4 * (size_t)ks, acc + (size_t)(A_RR + j % 3) * ks, ks * 8,
                                              hipMemcpyDeviceToHost, st));
                 SFH_CHECK_HIP(hipStreamSynchronize(st));
+                // a live column whose r.r did not fall over a whole check
+                // interval has stalled in FP32: replace its residual now
                 int na = 0, nu = 0;
+                bool stall = false;
                 for (int c = 0; c < ks; ++c) {
                     const bool act = hs[c] > 0.0;
-                    const bool fz = hs[ks + c] > 0.0 || !(hs[4 * (size_t)ks + c] > hs[3 * (size_t)ks + c]);
-                    if (act && !fz) ++na;
-                    else if (act || hs[2 * (size_t)ks + c] > 0.0) ++nu;
+                    const double rrc = hs[4 * (size_t)ks + c];
+                    const bool fz = hs[ks + c] > 0.0 || !(rrc > hs[3 * (size_t)ks + c]);
+                    reached[c] = act && fz;
+                    if (act && !fz) {
+                        ++na;
+                        stall = stall || !(rrc < rrchk[c]);
+                        rrchk[c] = rrc;
+                    } else if (act || hs[2 * (size_t)ks + c] > 0.0) {
+                        ++nu;
+                    }
                 }
-                if (na == 0 || nu > 0) break;
+                if (na == 0 || nu > 0 || stall) break;
             }
         }
         jlast = j;
         // flush the FP32 correction, true residual in FP64
-        k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow, S + S_SCALE * ks,
-                                                                 x, w.x64.as<double>());
+        if (use_mg)
+            k_refine<double><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
+                                                                          S + S_SCALE * ks, xmg, w.x64.as<double>());
+        else
+            k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
+                                                                     S + S_SCALE * ks, x, w.x64.as<double>());
         Kern<T, LPR>::resid(P, G, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
         k_red_norm<<<gred, kRedB, 0, st>>>(kp, d_sb, p0, S + S_RN2 * ks);
         SFH_LAUNCH_CHECK();
@@ -1828,15 +2001,21 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
         if (stats) *stats = acc;
         return SFH_OK;
     }
-    // narrow solves on a grid with a multigrid hierarchy run as 16-column tiles
-    // (the stencil kernels' narrowest): a dozen multigrid iterations on padded
-    // columns beat thousands of Jacobi ones on the CSR kernels
-    int min_kp = 4;
-    if (o.precision == 0 && o.precond != 1 && (o.layout == 0 || o.layout == 2) && nseg == 1 && k < 16) {
-        const std::vector<int> g = grids_of(s, 1, std::vector<int>{0, s->ndof});
-        if (g.size() == 3 && (long long)g[1] * g[2] > kMGCoarse) min_kp = 16;
+    // Declared 2D grids (stencil kernels): at most 16 lanes (64 columns) per
+    // column tile, wider solves run their column tiles side by side in the
+    // same launches; narrow solves on a grid with a multigrid hierarchy run as
+    // 16-column tiles (the stencil kernels' narrowest): a dozen multigrid
+    // iterations on padded columns beat thousands of Jacobi ones on the CSR
+    // kernels
+    int min_kp = 4, max_lpr = 64;
+    if (o.precision == 0 && (o.layout == 0 || o.layout == 2)) {
+        std::vector<int> srow(nseg + 1);
+        for (int q = 0; q <= nseg; ++q) srow[q] = seg_rows ? seg_rows[q] : (q == 0 ? 0 : s->ndof);
+        const std::vector<int> g = grids_of(s, nseg, srow);
+        if (!g.empty()) max_lpr = 16;
+        if (g.size() == 3 && o.precond != 1 && k < 16 && (long long)g[1] * g[2] > kMGCoarse) min_kp = 16;
     }
-    const Plan P = make_plan(s->ndof, k, s->ctx->num_cu, nseg, seg_rows, min_kp);
+    const Plan P = make_plan(s->ndof, k, s->ctx->num_cu, nseg, seg_rows, min_kp, max_lpr);
     if (o.precision == 1) return dispatch_solve<double>(s, P, trans, k, B, ldb, X, ldx, o, stats);
     return dispatch_solve<float>(s, P, trans, k, B, ldb, X, ldx, o, stats);
 }
