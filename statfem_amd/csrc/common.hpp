@@ -73,7 +73,6 @@ struct SolveWork {
     DevBuf scal;                     // per-RHS scalars
     DevBuf acc;                      // per-column FP64 accumulators of the iteration kernels
     DevBuf tab;                      // row-partition tables of the current solve
-    DevBuf s32;                      // multigrid-preconditioned PCG: the cycle's FP64 correction (z = M r lives in r32)
     DevBuf mgv;                      // multigrid: restricted residuals and corrections of the coarse levels
 };
 

@@ -603,6 +603,28 @@ __global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gta
 // consistent to FP64 rounding whatever p's precision). Vector traffic per
 // DOF-RHS and iteration: p (staged), x and r read and written.
 
+// stage an FP32 block on a kTX x kTY tile and its one-node halo (zeros off the
+// grid and on idle lanes); all of a lane group's loads are issued first
+template <int LPR>
+__device__ inline void stage_grid_f32(float4* sm, const float* __restrict__ v, int kp, int lc, bool ok, int base,
+                                      int W, int H, int tx0, int ty0, int grp, int lane) {
+    constexpr int ngrp = kG / LPR, kSt = (kHN + ngrp - 1) / ngrp;
+    float4 a[kSt];
+    bool in[kSt];
+#pragma unroll
+    for (int u = 0; u < kSt; ++u) {
+        const int h = grp + u * ngrp;
+        const int gx = tx0 + h % (kTX + 2) - 1, gy = ty0 + h / (kTX + 2) - 1;
+        in[u] = ok && h < kHN && gx >= 0 && gx < W && gy >= 0 && gy < H;
+        a[u] = *reinterpret_cast<const float4*>(v + (size_t)(in[u] ? base + gy * W + gx : 0) * kp + lc);
+    }
+#pragma unroll
+    for (int u = 0; u < kSt; ++u) {
+        const int h = grp + u * ngrp;
+        if (h < kHN) sm[h * LPR + lane] = in[u] ? a[u] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
 // q = FP64 9-point product at tile node (ox, oy) from the staged FP32 values
 template <int LPR>
 __device__ inline V4<double> stencil9d(const float4* sm, const double* __restrict__ c, int ox, int oy, int lane) {
@@ -639,16 +661,33 @@ __global__ __launch_bounds__(kG) void k_dir_spmv_mg(const int* __restrict__ gtab
     const int lc = t.lane * 4;
     const V4<float> b = {cb[lc], cb[lc + 1], cb[lc + 2], cb[lc + 3]};
     const V4<float> m = {cm[lc], cm[lc + 1], cm[lc + 2], cm[lc + 3]};
-    for (int h = t.grp; h < kHN; h += t.ngrp) {
-        const int hx = h % (kTX + 2), hy = h / (kTX + 2);
-        const int gx = tx0 + hx - 1, gy = ty0 + hy - 1;
-        V4<float> v = {0.f, 0.f, 0.f, 0.f};
-        if (ok && gx >= 0 && gx < W && gy >= 0 && gy < H) {
-            const size_t o = (size_t)(base + gy * W + gx) * kp + t.col0;
-            pdir4(ld4(z + o), m, b, ld4(pold + o), v);
-            if (hx >= 1 && hx <= kTX && hy >= 1 && hy <= kTY) st4(pnew + o, v);
+    {
+        constexpr int ngrp = kG / LPR, kSt = (kHN + ngrp - 1) / ngrp;
+        const int l0 = ok ? t.col0 : 0;
+        V4<float> za[kSt], pa[kSt];
+        bool in[kSt];
+#pragma unroll
+        for (int u = 0; u < kSt; ++u) {
+            const int h = t.grp + u * ngrp;
+            const int gx = tx0 + h % (kTX + 2) - 1, gy = ty0 + h / (kTX + 2) - 1;
+            in[u] = ok && h < kHN && gx >= 0 && gx < W && gy >= 0 && gy < H;
+            const size_t o = (size_t)(in[u] ? base + gy * W + gx : 0) * kp + l0;
+            za[u] = ld4(z + o);
+            pa[u] = ld4(pold + o);
         }
-        sm[h * LPR + t.lane] = f4(v);
+#pragma unroll
+        for (int u = 0; u < kSt; ++u) {
+            const int h = t.grp + u * ngrp;
+            if (h >= kHN) continue;
+            const int hx = h % (kTX + 2), hy = h / (kTX + 2);
+            V4<float> v = {0.f, 0.f, 0.f, 0.f};
+            if (in[u]) {
+                pdir4(za[u], m, b, pa[u], v);
+                if (hx >= 1 && hx <= kTX && hy >= 1 && hy <= kTY)
+                    st4(pnew + (size_t)(base + (ty0 + hy - 1) * W + tx0 + hx - 1) * kp + t.col0, v);
+            }
+            sm[h * LPR + t.lane] = f4(v);
+        }
     }
     __syncthreads();
     V4<double> pq = {0.0, 0.0, 0.0, 0.0};
@@ -666,7 +705,9 @@ __global__ __launch_bounds__(kG) void k_dir_spmv_mg(const int* __restrict__ gtab
     block_accumulate<LPR>(t, kp, pq, reinterpret_cast<double*>(sm), acc + (A_PQ + (it & 1)) * ks + t.so);
 }
 
-// x += alpha p, r -= alpha A p (FP64 x, r, stencil and sums), r.r
+// x += (alpha / s) p, r -= alpha A p, r.r: FP64 x, r, stencil and sums; r is
+// the residual scaled by the column's s (unit norm at the last replacement), x
+// the iterate itself, so no correction has to be flushed into it
 template <int LPR>
 __global__ __launch_bounds__(kG) void k_cg_update_mg(const int* __restrict__ gtab, int kp, int ks, int it,
                                                      const double* __restrict__ coef, const double* __restrict__ S,
@@ -679,21 +720,20 @@ __global__ __launch_bounds__(kG) void k_cg_update_mg(const int* __restrict__ gta
     const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
     if (W == 0) return;
     if (block_done(t, kp, ks, S)) return;
-    __shared__ double ca[4 * LPR];
+    __shared__ double ca[4 * LPR], cx[4 * LPR];
     shared_alpha<double, LPR>(it, kp, ks, t.so, S, acc, ca);
-    const bool ok = t.col0 < kp;
-    for (int h = t.grp; h < kHN; h += t.ngrp) {
-        const int hx = h % (kTX + 2), hy = h / (kTX + 2);
-        const int gx = tx0 + hx - 1, gy = ty0 + hy - 1;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ok && gx >= 0 && gx < W && gy >= 0 && gy < H)
-            v = *reinterpret_cast<const float4*>(p + (size_t)(base + gy * W + gx) * kp + t.col0);
-        sm[h * LPR + t.lane] = v;
+    for (int c = threadIdx.x; c < 4 * LPR; c += blockDim.x) {
+        const int col = blockIdx.y * 4 * LPR + c;
+        const double sc = col < kp ? S[S_SCALE * ks + t.so + col] : 0.0;
+        cx[c] = sc > 0.0 ? ca[c] / sc : 0.0;
     }
+    const bool ok = t.col0 < kp;
+    stage_grid_f32<LPR>(sm, p, kp, ok ? t.col0 : 0, ok, base, W, H, tx0, ty0, t.grp, t.lane);
     __syncthreads();
     V4<double> rr = {0.0, 0.0, 0.0, 0.0};
     if (ok) {
         const V4<double> al = {ca[t.lane * 4], ca[t.lane * 4 + 1], ca[t.lane * 4 + 2], ca[t.lane * 4 + 3]};
+        const V4<double> ax = {cx[t.lane * 4], cx[t.lane * 4 + 1], cx[t.lane * 4 + 2], cx[t.lane * 4 + 3]};
         for (int o = t.grp; o < kTX * kTY; o += t.ngrp) {
             const int ox = o % kTX, oy = o / kTX;
             const int gx = tx0 + ox, gy = ty0 + oy;
@@ -703,7 +743,7 @@ __global__ __launch_bounds__(kG) void k_cg_update_mg(const int* __restrict__ gta
             const float4 pi = sm[((oy + 1) * (kTX + 2) + (ox + 1)) * LPR + t.lane];
             const size_t off = (size_t)nd * kp + t.col0;
             V4<double> xv = ld4(x + off), rv = ld4(r + off);
-            xv.x += al.x * pi.x; xv.y += al.y * pi.y; xv.z += al.z * pi.z; xv.w += al.w * pi.w;
+            xv.x += ax.x * pi.x; xv.y += ax.y * pi.y; xv.z += ax.z * pi.z; xv.w += ax.w * pi.w;
             rv.x -= al.x * q.x; rv.y -= al.y * q.y; rv.z -= al.z * q.z; rv.w -= al.w * q.w;
             st4(x + off, xv);
             st4(r + off, rv);
@@ -754,13 +794,19 @@ __device__ inline void fma4(float a, const float4& v, float4& s) {
     s.x += a * v.x; s.y += a * v.y; s.z += a * v.z; s.w += a * v.w;
 }
 
-// r_{l+1} = P^T (r - A (S r)) on one tile of coarse nodes
 __device__ inline float4 ldf4(const float* p) { return *reinterpret_cast<const float4*>(p); }
 __device__ inline float4 ldf4(const double* p) {
     const V4<double> v = ld4(p);
     return make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
 }
+__device__ inline float4 zero4() { return make_float4(0.f, 0.f, 0.f, 0.f); }
 
+// The staging loops below issue all of a thread's loads of a phase before
+// using any of them: out-of-grid nodes and idle lanes load from node 0 /
+// column 0 (always valid) and are masked afterwards, so no load sits behind a
+// branch and the memory latencies of a phase overlap.
+
+// r_{l+1} = P^T (r - A (S r)) on one tile of coarse nodes
 template <int LPR, class TR>
 __global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab, int kp, const float* __restrict__ coef,
                                                     const float* __restrict__ dsm,
@@ -771,48 +817,68 @@ __global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab,
     constexpr int ngrp = kG / LPR;
     constexpr int kIW = 2 * kRX + 1, kIn = kIW * (2 * kRY + 1);   // fine nodes whose residual is restricted
     constexpr int kPer = (kIn + ngrp - 1) / ngrp;
+    constexpr int kSt = (kRW * kRH + ngrp - 1) / ngrp;
     const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
     const int* d = tab + kMT * xcd_block(blockIdx.x, gridDim.x);
     const int fbase = d[0], W = d[1], H = d[2], cbase = d[3], Wc = d[4], Hc = d[5], cx0 = d[6], cy0 = d[7];
     if (W == 0) return;
     const int col0 = (blockIdx.y * LPR + lane) * 4;   // column tile blockIdx.y of 4 * LPR columns
     const bool ok = col0 < kp;
+    const int lc = ok ? col0 : 0;
     const int fx0 = 2 * cx0 - 2, fy0 = 2 * cy0 - 2;
-    for (int h = grp; h < kRW * kRH; h += ngrp) {
-        const int gx = fx0 + h % kRW, gy = fy0 + h / kRW;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        float w = 0.f;
-        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-            const int nd = fbase + gy * W + gx;
-            if (ok) v = ldf4(r + (size_t)nd * kp + col0);
-            w = dsm[nd];
+    {
+        float4 v[kSt];
+        float w[kSt];
+        bool in[kSt];
+#pragma unroll
+        for (int u = 0; u < kSt; ++u) {
+            const int h = grp + u * ngrp;
+            const int gx = fx0 + h % kRW, gy = fy0 + h / kRW;
+            in[u] = h < kRW * kRH && gx >= 0 && gx < W && gy >= 0 && gy < H;
+            const int nd = in[u] ? fbase + gy * W + gx : 0;
+            v[u] = ldf4(r + (size_t)nd * kp + lc);
+            w[u] = dsm[nd];
         }
-        sm[h * LPR + lane] = v;
-        if (lane == 0) sd[h] = w;
+#pragma unroll
+        for (int u = 0; u < kSt; ++u) {
+            const int h = grp + u * ngrp;
+            if (h < kRW * kRH) {
+                sm[h * LPR + lane] = (in[u] && ok) ? v[u] : zero4();
+                if (lane == 0) sd[h] = in[u] ? w[u] : 0.f;
+            }
+        }
     }
     __syncthreads();
     float4 res[kPer];
+    {
+        float c[kPer][9];
+        bool fr[kPer];
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int i = grp + u * ngrp;
-        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < kIn) {
+        for (int u = 0; u < kPer; ++u) {
+            const int i = grp + u * ngrp;
             const int hx = i % kIW + 1, hy = i / kIW + 1;
             const int gx = fx0 + hx, gy = fy0 + hy;
-            if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-                const int nd = fbase + gy * W + gx;
-                if (!(cmask && cmask[nd])) {
-                    const float* c = coef + (size_t)nd * 9;
-                    q = sm[(hy * kRW + hx) * LPR + lane];
+            const bool in = i < kIn && gx >= 0 && gx < W && gy >= 0 && gy < H;
+            const int nd = in ? fbase + gy * W + gx : 0;
+            fr[u] = in && !(cmask && cmask[nd]);
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) {
-                        const int h = (hy + k / 3 - 1) * kRW + hx + k % 3 - 1;
-                        fma4(-c[k] * sd[h], sm[h * LPR + lane], q);
-                    }
+            for (int k = 0; k < 9; ++k) c[u][k] = coef[(size_t)nd * 9 + k];
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int i = grp + u * ngrp;
+            const int hx = i % kIW + 1, hy = i / kIW + 1;
+            float4 q = zero4();
+            if (fr[u]) {
+                q = sm[(hy * kRW + hx) * LPR + lane];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    const int h = (hy + k / 3 - 1) * kRW + hx + k % 3 - 1;
+                    fma4(-c[u][k] * sd[h], sm[h * LPR + lane], q);
                 }
             }
+            res[u] = q;
         }
-        res[u] = q;
     }
     __syncthreads();
 #pragma unroll
@@ -825,7 +891,7 @@ __global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab,
     for (int o = grp; o < kRX * kRY; o += ngrp) {
         const int X = cx0 + o % kRX, Y = cy0 + o / kRX;
         if (X >= Wc || Y >= Hc) continue;
-        float4 acc4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 acc4 = zero4();
 #pragma unroll
         for (int dy = -1; dy <= 1; ++dy) {
             const int fy = 2 * Y + dy;
@@ -858,10 +924,14 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
                                                    double* __restrict__ part0, double* __restrict__ part1) {
     extern __shared__ float4 sm[];
     float4* sc = sm + kPW * kPH * LPR;
+    constexpr int ngrp = kG / LPR;
+    constexpr int kCs = (kCW * kCH + ngrp - 1) / ngrp;   // staged coarse nodes per lane group
+    constexpr int kSt = (kPW * kPH + ngrp - 1) / ngrp;   // staged fine nodes per lane group
+    constexpr int kOw = (kPX * kPY + ngrp - 1) / ngrp;   // tile nodes per lane group
     Tile t;
     t.lane = threadIdx.x % LPR;
     t.grp = threadIdx.x / LPR;
-    t.ngrp = kG / LPR;
+    t.ngrp = ngrp;
     t.col0 = (blockIdx.y * LPR + t.lane) * 4;
     t.bid = xcd_block(blockIdx.x, gridDim.x);
     t.r0 = t.r1 = 0;
@@ -870,60 +940,101 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
     t.so = d[8] * kp;
     if (W == 0 && !(TOP && mode == 1)) return;
     const bool ok = t.col0 < kp;
+    const int lc = ok ? t.col0 : 0;
     V4<double> rz = {0.0, 0.0, 0.0, 0.0}, rr = {0.0, 0.0, 0.0, 0.0};
     if (W > 0) {
         const int cx0 = tx0 / 2 - 1, cy0 = ty0 / 2 - 1;
-        for (int h = t.grp; h < kCW * kCH; h += t.ngrp) {
-            const int X = cx0 + h % kCW, Y = cy0 + h / kCW;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ok && X >= 0 && X < Wc && Y >= 0 && Y < Hc)
-                v = *reinterpret_cast<const float4*>(ec + (size_t)(cbase + Y * Wc + X) * kp + t.col0);
-            sc[h * LPR + t.lane] = v;
+        {
+            float4 v[kCs];
+            bool in[kCs];
+#pragma unroll
+            for (int u = 0; u < kCs; ++u) {
+                const int h = t.grp + u * ngrp;
+                const int X = cx0 + h % kCW, Y = cy0 + h / kCW;
+                in[u] = h < kCW * kCH && X >= 0 && X < Wc && Y >= 0 && Y < Hc;
+                v[u] = ldf4(ec + (size_t)(in[u] ? cbase + Y * Wc + X : 0) * kp + lc);
+            }
+#pragma unroll
+            for (int u = 0; u < kCs; ++u) {
+                const int h = t.grp + u * ngrp;
+                if (h < kCW * kCH) sc[h * LPR + t.lane] = (in[u] && ok) ? v[u] : zero4();
+            }
         }
-        __syncthreads();
-        for (int h = t.grp; h < kPW * kPH; h += t.ngrp) {
-            const int gx = tx0 - 1 + h % kPW, gy = ty0 - 1 + h / kPW;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ok && gx >= 0 && gx < W && gy >= 0 && gy < H) {
+        {
+            float4 v[kSt];
+            float w[kSt];
+            bool in[kSt];
+#pragma unroll
+            for (int u = 0; u < kSt; ++u) {
+                const int h = t.grp + u * ngrp;
+                const int gx = tx0 - 1 + h % kPW, gy = ty0 - 1 + h / kPW;
                 const int nd = fbase + gy * W + gx;
-                if (!(TOP && cmask[nd])) {
-                    const float4 rv = ldf4(r + (size_t)nd * kp + t.col0);
-                    const float w = dsm[nd];
-                    v = make_float4(w * rv.x, w * rv.y, w * rv.z, w * rv.w);
+                in[u] = h < kPW * kPH && gx >= 0 && gx < W && gy >= 0 && gy < H;
+                v[u] = ldf4(r + (size_t)(in[u] ? nd : 0) * kp + lc);
+                w[u] = dsm[in[u] ? nd : 0];
+            }
+            __syncthreads();   // the coarse values are staged
+#pragma unroll
+            for (int u = 0; u < kSt; ++u) {
+                const int h = t.grp + u * ngrp;
+                if (h >= kPW * kPH) continue;
+                const int gx = tx0 - 1 + h % kPW, gy = ty0 - 1 + h / kPW;
+                float4 x = zero4();
+                if (in[u] && ok && !(TOP && cmask[fbase + gy * W + gx])) {
+                    x = make_float4(w[u] * v[u].x, w[u] * v[u].y, w[u] * v[u].z, w[u] * v[u].w);
                     const int ax = (gx & 1) ? (gx - 1) / 2 : gx / 2;   // first coarse parent
                     const int ay = (gy & 1) ? (gy - 1) / 2 : gy / 2;
                     const int nx = ((gx & 1) && gx != W - 1) ? 2 : 1;
                     const int ny = ((gy & 1) && gy != H - 1) ? 2 : 1;
                     const float fx = nx == 2 ? 0.5f : 1.f, fy = ny == 2 ? 0.5f : 1.f;
-                    for (int u = 0; u < ny; ++u)
-                        for (int q = 0; q < nx; ++q)
-                            fma4(fx * fy, sc[((ay + u - cy0) * kCW + (ax + q - cx0)) * LPR + t.lane], v);
+                    for (int a = 0; a < ny; ++a)
+                        for (int b = 0; b < nx; ++b)
+                            fma4(fx * fy, sc[((ay + a - cy0) * kCW + (ax + b - cx0)) * LPR + t.lane], x);
                 }
+                sm[h * LPR + t.lane] = x;
             }
-            sm[h * LPR + t.lane] = v;
         }
         __syncthreads();
-        if (ok) {
-            for (int o = t.grp; o < kPX * kPY; o += t.ngrp) {
+        {
+            float c[kOw][9], w[kOw];
+            bool in[kOw], fr[kOw];
+            V4<double> rd[kOw];
+#pragma unroll
+            for (int u = 0; u < kOw; ++u) {
+                const int o = t.grp + u * ngrp;
+                const int gx = tx0 + o % kPX, gy = ty0 + o / kPX;
+                in[u] = o < kPX * kPY && gx < W && gy < H;
+                const int nd = in[u] ? fbase + gy * W + gx : 0;
+                fr[u] = !(TOP && cmask[nd]);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) c[u][k] = coef[(size_t)nd * 9 + k];
+                w[u] = dsm[nd];
+                if constexpr (TOP) {
+                    rd[u] = ld4(r + (size_t)nd * kp + lc);
+                } else {
+                    const float4 f = ldf4(r + (size_t)nd * kp + lc);
+                    rd[u] = {f.x, f.y, f.z, f.w};
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kOw; ++u) {
+                if (!in[u] || !ok) continue;
+                const int o = t.grp + u * ngrp;
                 const int ox = o % kPX, oy = o / kPX;
-                const int gx = tx0 + ox, gy = ty0 + oy;
-                if (gx >= W || gy >= H) continue;
-                const int nd = fbase + gy * W + gx;
-                const float* c = coef + (size_t)nd * 9;
-                const float4 rv = ldf4(r + (size_t)nd * kp + t.col0);
+                const int nd = fbase + (ty0 + oy) * W + tx0 + ox;
+                const float4 rv = make_float4((float)rd[u].x, (float)rd[u].y, (float)rd[u].z, (float)rd[u].w);
                 float4 q = rv;
 #pragma unroll
                 for (int k = 0; k < 9; ++k)
-                    fma4(-c[k], sm[((oy + k / 3) * kPW + ox + k % 3) * LPR + t.lane], q);
-                const float w = dsm[nd];
+                    fma4(-c[u][k], sm[((oy + k / 3) * kPW + ox + k % 3) * LPR + t.lane], q);
                 float4 z = sm[((oy + 1) * kPW + ox + 1) * LPR + t.lane];
-                fma4(w, q, z);
-                if (TOP && cmask[nd]) z = make_float4(0.f, 0.f, 0.f, 0.f);
+                fma4(w[u], q, z);
+                if (!fr[u]) z = zero4();
                 *reinterpret_cast<float4*>(e + (size_t)nd * kp + t.col0) = z;
                 if constexpr (TOP) {
-                    const V4<double> rd = ld4(r + (size_t)nd * kp + t.col0);   // cache hit: just loaded
-                    rz.x += rd.x * z.x; rz.y += rd.y * z.y; rz.z += rd.z * z.z; rz.w += rd.w * z.w;
-                    rr.x += rd.x * rd.x; rr.y += rd.y * rd.y; rr.z += rd.z * rd.z; rr.w += rd.w * rd.w;
+                    const V4<double> a = rd[u];
+                    rz.x += a.x * z.x; rz.y += a.y * z.y; rz.z += a.z * z.z; rz.w += a.w * z.w;
+                    rr.x += a.x * a.x; rr.y += a.y * a.y; rr.z += a.z * a.z; rr.w += a.w * a.w;
                 }
             }
         }
@@ -941,16 +1052,32 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
     }
 }
 
-// e = C^-1 r on the coarsest level (n nodes; dense FP32 inverse, row-major)
-__global__ void k_mg_coarse(int n, int kp, const float* __restrict__ cinv, const float* __restrict__ r,
-                            float* __restrict__ e) {
-    const int nl = kp / 4;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * nl) return;
-    const int i = t / nl, c = (t % nl) * 4;
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int j = 0; j < n; ++j) fma4(cinv[(size_t)i * n + j], *reinterpret_cast<const float4*>(r + (size_t)j * kp + c), s);
-    *reinterpret_cast<float4*>(e + (size_t)i * kp + c) = s;
+// e = C^-1 r on the coarsest level (n nodes, dense FP32 inverse, row-major):
+// one work-group per 16 columns stages C^-1 and its residual columns in LDS
+constexpr int kCoarseCols = 16;
+inline size_t mg_coarse_lds(int n) {
+    return sizeof(float) * (((size_t)n * n + 3) / 4 * 4) + sizeof(float4) * (size_t)n * (kCoarseCols / 4);
+}
+__global__ __launch_bounds__(256) void k_mg_coarse(int n, int kp, const float* __restrict__ cinv,
+                                                   const float* __restrict__ r, float* __restrict__ e) {
+    extern __shared__ float4 sm[];
+    float* ci = reinterpret_cast<float*>(sm);
+    float4* rs = sm + ((size_t)n * n + 3) / 4;
+    constexpr int nq = kCoarseCols / 4;
+    const int c0 = blockIdx.x * kCoarseCols;
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) ci[i] = cinv[i];
+    for (int i = threadIdx.x; i < n * nq; i += blockDim.x) {
+        const int j = i / nq, col = c0 + 4 * (i % nq);
+        rs[i] = col < kp ? *reinterpret_cast<const float4*>(r + (size_t)j * kp + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < n * nq; o += blockDim.x) {
+        const int i = o / nq, q = o % nq, col = c0 + 4 * q;
+        if (col >= kp) continue;
+        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int j = 0; j < n; ++j) fma4(ci[i * n + j], rs[j * nq + q], s);
+        *reinterpret_cast<float4*>(e + (size_t)i * kp + col) = s;
+    }
 }
 
 // residual replacement with the multigrid preconditioner: r *= s' in place
@@ -1158,7 +1285,7 @@ __global__ void k_build_dia9(int base, int W, int H, const int* __restrict__ rp,
     }
 }
 
-// FP64 residual r = b - A x ; part0 = r.r
+// FP64 residual r = b - A x (x == NULL: a zero iterate); part0 = r.r
 template <int LPR>
 __global__ __launch_bounds__(kB) void k_resid64(int kp, Seg G, const int* __restrict__ rp,
                                                 const int* __restrict__ ci, const double* __restrict__ va,
@@ -1171,8 +1298,8 @@ __global__ __launch_bounds__(kB) void k_resid64(int kp, Seg G, const int* __rest
     if (ok) {
         for (int row = t.r0 + t.grp; row < t.r1; row += t.ngrp) {
             V4<double> s = ld4(b + (size_t)row * kp + t.col0);
-            const int e1 = rp[row + 1];
-            for (int e = rp[row]; e < e1; ++e) {
+            const int e1 = x ? rp[row + 1] : 0;   // x == NULL: a zero iterate, r = b
+            for (int e = x ? rp[row] : 0; e < e1; ++e) {
                 const double a = va[e];
                 const V4<double> v = ld4(x + (size_t)ci[e] * kp + t.col0);
                 s.x -= a * v.x; s.y -= a * v.y; s.z -= a * v.z; s.w -= a * v.w;
@@ -1521,19 +1648,23 @@ struct Kern {
                        const unsigned char* cmask, const double* r0, T* z, T* mgv, double* acc, double* p0,
                        double* p1) {
         if constexpr (std::is_same<T, float>::value && LPR >= 4 && LPR <= 16) {
-            const int L = P.mgL, kp = P.kp;
+            // restriction: 32 columns per work-group (half the LDS of a
+            // 64-column tile, so four work-groups share a CU)
+            constexpr int ML = LPR > 8 ? 8 : LPR;
+            const int L = P.mgL, kp = P.kp, ntm = (kp + 4 * ML - 1) / (4 * ML);
             auto rvec = [&](int l) -> float* { return mgv + P.mgv_off[l]; };
             auto evec = [&](int l) -> float* {
                 return l == 0 ? z : mgv + P.mgv_off[l] + (size_t)m.W[l] * m.H[l] * kp;
             };
-            k_mg_restrict<LPR, double><<<dim3(P.mg_rn[0], P.ntile), kG, mg_restrict_lds(LPR), st>>>(
+            k_mg_restrict<ML, double><<<dim3(P.mg_rn[0], ntm), kG, mg_restrict_lds(ML), st>>>(
                 P.d_mgtab + (size_t)kMT * P.mg_roff[0], kp, coef0, m.dsm[0].as<float>(), cmask, r0, rvec(1));
             for (int l = 1; l < L; ++l)
-                k_mg_restrict<LPR, float><<<dim3(P.mg_rn[l], P.ntile), kG, mg_restrict_lds(LPR), st>>>(
+                k_mg_restrict<ML, float><<<dim3(P.mg_rn[l], ntm), kG, mg_restrict_lds(ML), st>>>(
                     P.d_mgtab + (size_t)kMT * P.mg_roff[l], kp, m.coef32[l].as<float>(), m.dsm[l].as<float>(),
                     nullptr, rvec(l), rvec(l + 1));
             const int nL = m.W[L] * m.H[L];
-            k_mg_coarse<<<(nL * (kp / 4) + 255) / 256, 256, 0, st>>>(nL, kp, m.cinv.as<float>(), rvec(L), evec(L));
+            k_mg_coarse<<<(kp + kCoarseCols - 1) / kCoarseCols, 256, mg_coarse_lds(nL), st>>>(
+                nL, kp, m.cinv.as<float>(), rvec(L), evec(L));
             for (int l = L - 1; l >= 1; --l)
                 k_mg_prolong<LPR, false, float><<<dim3(P.mg_pn[l], P.ntile), kG, mg_prolong_lds(LPR), st>>>(
                     P.d_mgtab + (size_t)kMT * P.mg_poff[l], kp, P.ks, it, mode, m.coef32[l].as<float>(),
@@ -1684,10 +1815,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_TRY(w.part1.ensure((size_t)std::max(P.nblk, nbi) * kp * 8));
     SFH_TRY(w.scal.ensure((size_t)S_NSLOT * ks * 8));
     SFH_TRY(w.acc.ensure((size_t)A_N * ks * 8));
-    if (use_mg) {
-        SFH_TRY(w.s32.ensure(nk * sizeof(double)));
-        SFH_TRY(w.mgv.ensure(P.mgv_total * sizeof(T)));
-    }
+    if (use_mg) SFH_TRY(w.mgv.ensure(P.mgv_total * sizeof(T)));
     // partition tables: brow (nblk + 1), bseg (nblk), sb (nseg + 1), srow (nseg + 1), gtab, multigrid tiles,
     // and the partial-row bounds {0, level-0 prolongation tiles} of a multigrid residual replacement
     const int sbmg[2] = {0, use_mg ? P.mg_pn[0] : 0};
@@ -1727,9 +1855,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     const Entries<T> M = {s->cv32.as<int2>(), ci, vaT};
     T* x = w.x32.as<T>();
     T* z = w.r32.as<T>();   // Jacobi: z = D^-1 r (r itself is not kept); multigrid: z = M r
-    // multigrid: the cycle's FP64 correction and, in place of the true
-    // residual after each replacement, the FP64 recursive residual
-    double* xmg = use_mg ? w.s32.as<double>() : nullptr;
+    // multigrid: the iteration updates the FP64 iterate itself and, in place
+    // of the true residual after each replacement, the FP64 recursive residual
     double* rmg = w.r64.as<double>();
     T* mgv = use_mg ? w.mgv.as<T>() : nullptr;
     const double* coef64 = s->dia64.as<double>();
@@ -1748,8 +1875,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                                                           w.b64.as<double>());
     SFH_LAUNCH_CHECK();
     SFH_CHECK_HIP(hipMemsetAsync(w.x64.p, 0, nk * 8, st));
-    SFH_CHECK_HIP(hipMemsetAsync(w.x32.p, 0, nk * sizeof(T), st));
-    if (use_mg) SFH_CHECK_HIP(hipMemsetAsync(xmg, 0, nk * sizeof(double), st));
+    if (!use_mg) SFH_CHECK_HIP(hipMemsetAsync(w.x32.p, 0, nk * sizeof(T), st));
     SFH_CHECK_HIP(hipMemsetAsync(w.p32.p, 0, nk * sizeof(T), st));
     SFH_CHECK_HIP(hipMemsetAsync(w.scal.p, 0, (size_t)S_NSLOT * ks * 8, st));
     SFH_CHECK_HIP(hipMemsetAsync(w.acc.p, 0, (size_t)A_N * ks * 8, st));
@@ -1757,7 +1883,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     sfh_ctx* cx = s->ctx;
     const int nsmp = (!tl_env && cx->ksample > 0 && k >= cx->kmin_rhs) ? (int)cx->kev.size() / 4 : 0;
     int ismp = 0;
-    Kern<T, LPR>::resid(P, G, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
+    Kern<T, LPR>::resid(P, G, st, rp, ci, va, nullptr, w.b64.as<double>(), w.r64.as<double>(), p0);
     k_red_norm<<<gred, kRedB, 0, st>>>(kp, d_sb, p0, S + S_BN2 * ks);
     SFH_LAUNCH_CHECK();
     // per (segment, column) state, index seg * kp + col
@@ -1843,7 +1969,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                 Kern<T, LPR>::dir_spmv(P, G, st, j, rp, M, S, z, pa, pb, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 1], st));
             if (use_mg)
-                Kern<T, LPR>::update_mg(P, st, j, coef64, S, pb, xmg, rmg, acc);
+                Kern<T, LPR>::update_mg(P, st, j, coef64, S, pb, w.x64.as<double>(), rmg, acc);
             else if (P.grid)
                 Kern<T, LPR>::update_g(P, st, j, coef, dinv, diag, S, pb, x, z, acc);
             else if (P.bsr)
@@ -1891,10 +2017,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         }
         jlast = j;
         // flush the FP32 correction, true residual in FP64
-        if (use_mg)
-            k_refine<double><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
-                                                                          S + S_SCALE * ks, xmg, w.x64.as<double>());
-        else
+        if (!use_mg)   // (the multigrid iteration updates x64 itself)
             k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
                                                                      S + S_SCALE * ks, x, w.x64.as<double>());
         Kern<T, LPR>::resid(P, G, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
