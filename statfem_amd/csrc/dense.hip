@@ -22,20 +22,28 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int TM = 64, TN = 64, TK = 16;
 
 // C[z] (=|+=) op(A) B[z]. op(A) = A (M x K, lda) or, with TRANSA, X^T for X K x M (lda).
-// blockIdx.z selects a batch (B, C strides) and/or a K chunk [z*kstep, z*kstep + kz).
+// z selects a batch (B, C strides) and/or a K chunk [z*kstep, z*kstep + kz).
+// One-dimensional grid of nwg (a multiple of 8) work-groups over nM x nN x nZ
+// tiles, M tiles fastest: consecutive logical tiles share their B panel (and,
+// for a K-chunked Gram product, their A panel), and the XCD remap below puts
+// consecutive logical tiles on the same XCD, so a panel is fetched from HBM
+// into that XCD's L2 once rather than by every XCD.
 template <bool TRANSA>
 __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
                                                const double* __restrict__ B, int ldb, long long strideB,
                                                double* __restrict__ C, int ldc, long long strideC,
-                                               int kz, int kstep) {
+                                               int kz, int kstep, int nM, int nN, int nZ) {
     __shared__ double As[TM][TK + 1];
     __shared__ double Bs[TK][TN + 2];
-    const int z = blockIdx.z;
+    const int nwg = gridDim.x;
+    const int lid = (nwg % 8) ? (int)blockIdx.x : ((int)blockIdx.x % 8) * (nwg / 8) + (int)blockIdx.x / 8;
+    if (lid >= nM * nN * nZ) return;
+    const int mt = lid % nM, nt = (lid / nM) % nN, z = lid / (nM * nN);
     const double* Bz = B + (size_t)z * strideB;
     double* Cz = C + (size_t)z * strideC;
     const int kbeg = z * kstep;
     const int kend = min(K, kbeg + kz);
-    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int m0 = mt * TM, n0 = nt * TN;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w >> 1, wn = w & 1;
     v4d acc[2][2];
@@ -623,11 +631,15 @@ inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 1) / b)
 }  // namespace
 
 // ---------------------------------------------------------------- host
+// work-groups for nt tiles, padded to whole XCD groups
+inline unsigned dgemm_grid(long long nt) { return (unsigned)((nt + 7) / 8 * 8); }
+
 int dgemm_shared_a(sfh_ctx* c, int M, int N, int K, const double* A, int lda, const double* B, int ldb,
                    long long strideB, double* C, int ldc, long long strideC, int batch) {
-    if (batch > 65535) { set_error("batch too large"); return SFH_EARG; }
-    dim3 g((N + TN - 1) / TN, (M + TM - 1) / TM, batch);
-    k_dgemm<false><<<g, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, strideB, C, ldc, strideC, K, 0);
+    const int nM = (M + TM - 1) / TM, nN = (N + TN - 1) / TN;
+    if ((long long)nM * nN * batch > (1LL << 30)) { set_error("product too large"); return SFH_EARG; }
+    k_dgemm<false><<<dgemm_grid((long long)nM * nN * batch), 256, 0, c->stream>>>(
+        M, N, K, A, lda, B, ldb, strideB, C, ldc, strideC, K, 0, nM, nN, batch);
     SFH_LAUNCH_CHECK();
     return SFH_OK;
 }
@@ -641,8 +653,9 @@ int gram(sfh_ctx* c, int N, int m, int n, const double* X, int ldx, const double
     nz = (N + kz - 1) / kz;
     const long long mn = (long long)m * n;
     SFH_TRY(work.ensure((size_t)nz * mn * 8));
-    dim3 g((n + TN - 1) / TN, (m + TM - 1) / TM, nz);
-    k_dgemm<true><<<g, 256, 0, c->stream>>>(m, n, N, X, ldx, V, ldv, 0, work.as<double>(), n, mn, kz, kz);
+    const int nM = (m + TM - 1) / TM, nN = (n + TN - 1) / TN;
+    k_dgemm<true><<<dgemm_grid((long long)nM * nN * nz), 256, 0, c->stream>>>(
+        m, n, N, X, ldx, V, ldv, 0, work.as<double>(), n, mn, kz, kz, nM, nN, nz);
     SFH_LAUNCH_CHECK();
     k_sum_partials<<<nblocks(mn, 256), 256, 0, c->stream>>>(mn, nz, work.as<double>(), C);
     SFH_LAUNCH_CHECK();
