@@ -910,8 +910,8 @@ __global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab,
 }
 
 // e = e0 + S (r - A e0), e0 = S r + P e_{l+1}, on one tile of fine nodes. TOP
-// (level 0, FP64 residual): P acts on free nodes only, e is zero on
-// constrained nodes, and the column sums r.e (and r.r) are formed in FP64:
+// (level 0, FP64 residual read as FP32): P acts on free nodes only, e is zero
+// on constrained nodes, and the column sums r.e (and r.r) are added in FP64:
 // mode 0 adds r.e into the iteration's accumulator slot, mode 1 writes
 // per-tile partial rows of r.e and r.r (after a residual replacement; padding
 // tiles write zeros)
@@ -927,7 +927,6 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
     constexpr int ngrp = kG / LPR;
     constexpr int kCs = (kCW * kCH + ngrp - 1) / ngrp;   // staged coarse nodes per lane group
     constexpr int kSt = (kPW * kPH + ngrp - 1) / ngrp;   // staged fine nodes per lane group
-    constexpr int kOw = (kPX * kPY + ngrp - 1) / ngrp;   // tile nodes per lane group
     Tile t;
     t.lane = threadIdx.x % LPR;
     t.grp = threadIdx.x / LPR;
@@ -960,82 +959,65 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
                 if (h < kCW * kCH) sc[h * LPR + t.lane] = (in[u] && ok) ? v[u] : zero4();
             }
         }
-        {
-            float4 v[kSt];
-            float w[kSt];
-            bool in[kSt];
+        // phase 1 and 2 walk the same staged nodes (h = grp + u * ngrp): each
+        // lane group keeps its nodes' residuals in registers for the smoothing
+        // step instead of fetching them again
+        float4 v[kSt];
+        float w[kSt];
+        bool in[kSt];
 #pragma unroll
-            for (int u = 0; u < kSt; ++u) {
-                const int h = t.grp + u * ngrp;
-                const int gx = tx0 - 1 + h % kPW, gy = ty0 - 1 + h / kPW;
-                const int nd = fbase + gy * W + gx;
-                in[u] = h < kPW * kPH && gx >= 0 && gx < W && gy >= 0 && gy < H;
-                v[u] = ldf4(r + (size_t)(in[u] ? nd : 0) * kp + lc);
-                w[u] = dsm[in[u] ? nd : 0];
-            }
-            __syncthreads();   // the coarse values are staged
+        for (int u = 0; u < kSt; ++u) {
+            const int h = t.grp + u * ngrp;
+            const int gx = tx0 - 1 + h % kPW, gy = ty0 - 1 + h / kPW;
+            const int nd = fbase + gy * W + gx;
+            in[u] = h < kPW * kPH && gx >= 0 && gx < W && gy >= 0 && gy < H;
+            v[u] = ldf4(r + (size_t)(in[u] ? nd : 0) * kp + lc);
+            w[u] = dsm[in[u] ? nd : 0];
+        }
+        __syncthreads();   // the coarse values are staged
 #pragma unroll
-            for (int u = 0; u < kSt; ++u) {
-                const int h = t.grp + u * ngrp;
-                if (h >= kPW * kPH) continue;
-                const int gx = tx0 - 1 + h % kPW, gy = ty0 - 1 + h / kPW;
-                float4 x = zero4();
-                if (in[u] && ok && !(TOP && cmask[fbase + gy * W + gx])) {
-                    x = make_float4(w[u] * v[u].x, w[u] * v[u].y, w[u] * v[u].z, w[u] * v[u].w);
-                    const int ax = (gx & 1) ? (gx - 1) / 2 : gx / 2;   // first coarse parent
-                    const int ay = (gy & 1) ? (gy - 1) / 2 : gy / 2;
-                    const int nx = ((gx & 1) && gx != W - 1) ? 2 : 1;
-                    const int ny = ((gy & 1) && gy != H - 1) ? 2 : 1;
-                    const float fx = nx == 2 ? 0.5f : 1.f, fy = ny == 2 ? 0.5f : 1.f;
-                    for (int a = 0; a < ny; ++a)
-                        for (int b = 0; b < nx; ++b)
-                            fma4(fx * fy, sc[((ay + a - cy0) * kCW + (ax + b - cx0)) * LPR + t.lane], x);
-                }
-                sm[h * LPR + t.lane] = x;
+        for (int u = 0; u < kSt; ++u) {
+            const int h = t.grp + u * ngrp;
+            if (h >= kPW * kPH) continue;
+            const int gx = tx0 - 1 + h % kPW, gy = ty0 - 1 + h / kPW;
+            float4 x = zero4();
+            if (in[u] && ok && !(TOP && cmask[fbase + gy * W + gx])) {
+                x = make_float4(w[u] * v[u].x, w[u] * v[u].y, w[u] * v[u].z, w[u] * v[u].w);
+                const int ax = (gx & 1) ? (gx - 1) / 2 : gx / 2;   // first coarse parent
+                const int ay = (gy & 1) ? (gy - 1) / 2 : gy / 2;
+                const int nx = ((gx & 1) && gx != W - 1) ? 2 : 1;
+                const int ny = ((gy & 1) && gy != H - 1) ? 2 : 1;
+                const float fx = nx == 2 ? 0.5f : 1.f, fy = ny == 2 ? 0.5f : 1.f;
+                for (int a = 0; a < ny; ++a)
+                    for (int b = 0; b < nx; ++b)
+                        fma4(fx * fy, sc[((ay + a - cy0) * kCW + (ax + b - cx0)) * LPR + t.lane], x);
             }
+            sm[h * LPR + t.lane] = x;
         }
         __syncthreads();
-        {
-            float c[kOw][9], w[kOw];
-            bool in[kOw], fr[kOw];
-            V4<double> rd[kOw];
+        // smoothing on the tile's own nodes (the inner part of the staged region)
 #pragma unroll
-            for (int u = 0; u < kOw; ++u) {
-                const int o = t.grp + u * ngrp;
-                const int gx = tx0 + o % kPX, gy = ty0 + o / kPX;
-                in[u] = o < kPX * kPY && gx < W && gy < H;
-                const int nd = in[u] ? fbase + gy * W + gx : 0;
-                fr[u] = !(TOP && cmask[nd]);
+        for (int u = 0; u < kSt; ++u) {
+            const int h = t.grp + u * ngrp;
+            const int hx = h % kPW, hy = h / kPW;
+            if (!(in[u] && ok && h < kPW * kPH && hx >= 1 && hx <= kPX && hy >= 1 && hy <= kPY)) continue;
+            const int ox = hx - 1, oy = hy - 1;
+            const int nd = fbase + (ty0 + oy) * W + tx0 + ox;
+            const float* c = coef + (size_t)nd * 9;
+            float4 q = v[u];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) c[u][k] = coef[(size_t)nd * 9 + k];
-                w[u] = dsm[nd];
-                if constexpr (TOP) {
-                    rd[u] = ld4(r + (size_t)nd * kp + lc);
-                } else {
-                    const float4 f = ldf4(r + (size_t)nd * kp + lc);
-                    rd[u] = {f.x, f.y, f.z, f.w};
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kOw; ++u) {
-                if (!in[u] || !ok) continue;
-                const int o = t.grp + u * ngrp;
-                const int ox = o % kPX, oy = o / kPX;
-                const int nd = fbase + (ty0 + oy) * W + tx0 + ox;
-                const float4 rv = make_float4((float)rd[u].x, (float)rd[u].y, (float)rd[u].z, (float)rd[u].w);
-                float4 q = rv;
-#pragma unroll
-                for (int k = 0; k < 9; ++k)
-                    fma4(-c[u][k], sm[((oy + k / 3) * kPW + ox + k % 3) * LPR + t.lane], q);
-                float4 z = sm[((oy + 1) * kPW + ox + 1) * LPR + t.lane];
-                fma4(w[u], q, z);
-                if (!fr[u]) z = zero4();
-                *reinterpret_cast<float4*>(e + (size_t)nd * kp + t.col0) = z;
-                if constexpr (TOP) {
-                    const V4<double> a = rd[u];
-                    rz.x += a.x * z.x; rz.y += a.y * z.y; rz.z += a.z * z.z; rz.w += a.w * z.w;
-                    rr.x += a.x * a.x; rr.y += a.y * a.y; rr.z += a.z * a.z; rr.w += a.w * a.w;
-                }
+            for (int k = 0; k < 9; ++k)
+                fma4(-c[k], sm[((oy + k / 3) * kPW + ox + k % 3) * LPR + t.lane], q);
+            float4 z = sm[h * LPR + t.lane];
+            fma4(w[u], q, z);
+            if (TOP && cmask[nd]) z = zero4();
+            *reinterpret_cast<float4*>(e + (size_t)nd * kp + t.col0) = z;
+            if constexpr (TOP) {
+                const float4 a = v[u];
+                rz.x += (double)a.x * z.x; rz.y += (double)a.y * z.y; rz.z += (double)a.z * z.z;
+                rz.w += (double)a.w * z.w;
+                rr.x += (double)a.x * a.x; rr.y += (double)a.y * a.y; rr.z += (double)a.z * a.z;
+                rr.w += (double)a.w * a.w;
             }
         }
     }

@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 CSV output for this library's kernels.
 
-    python -m statfem_amd.rocprof_summary stats  <kernel_stats.csv>
+    python -m statfem_amd.rocprof_summary stats  <kernel_stats.csv> [rows]
     python -m statfem_amd.rocprof_summary traffic <fetch counter csv> <write counter csv> [kernel substring]
 
 `traffic` reports mean FETCH_SIZE / WRITE_SIZE (KiB per dispatch) per kernel
@@ -68,7 +68,7 @@ def traffic(fetch_csv, write_csv, copy_bytes=1 << 30, match=""):
 
 def main(argv):
     if argv[0] == "stats":
-        print(stats(argv[1]))
+        print(stats(argv[1], top=int(argv[2]) if len(argv) > 2 else 20))
     elif argv[0] == "traffic":
         t = traffic(argv[1], argv[2], match=argv[3] if len(argv) > 3 else "")
         print(f"calibration: FETCH_SIZE x {t['fetch_scale']:.3f}, WRITE_SIZE x {t['write_scale']:.3f} "
