@@ -76,18 +76,27 @@ struct SolveWork {
     DevBuf mgv;                      // multigrid: restricted residuals and corrections of the coarse levels
 };
 
-// Geometric multigrid hierarchy of a single 2D tensor grid: level 0 is the
-// operator's W x H grid (stencil in sfh_system::dia32 / dia64), level l + 1
-// keeps every other node of level l; per level the FP32 stencil (levels >= 1)
-// and the l1-Jacobi smoother weights; the coarsest level's dense FP32 inverse.
+// Geometric multigrid hierarchies of a batch of 2D tensor grids (one grid for
+// a single problem): level 0 of grid q is its W x H node grid in the operator
+// (stencil in sfh_system::dia32 / dia64), level l + 1 keeps every other node of
+// level l. The levels of all grids are stored side by side: grid q's level-l
+// nodes start at node gbase[l * nseg + q] of level l's arrays (level 0: the
+// grid's first row of the operator). Per level the FP32 stencil (levels >= 1)
+// and the l1-Jacobi smoother weights; per grid the dense FP32 inverse of its
+// coarsest level.
 constexpr int kMGMaxLev = 16;
 constexpr int kMGCoarse = 160;       // coarsest level: at most this many nodes
 struct MGHier {
     bool built = false;              // current for the stencil copy (reset when that is rebuilt)
-    int L = 0;                       // coarse levels below the grid (0: no hierarchy)
-    int W[kMGMaxLev + 1] = {}, H[kMGMaxLev + 1] = {};
+    int L = 0;                       // coarse levels of the deepest grid (0: no hierarchy)
+    int nseg = 0;
+    std::vector<int> segL;           // coarse levels of every grid (at least 1 when L > 0)
+    std::vector<int> gW, gH, gbase;  // per (level, grid), index l * nseg + q
+    long long N[kMGMaxLev + 1] = {}; // nodes of every level (all grids)
     DevBuf coef64[kMGMaxLev + 1], coef32[kMGMaxLev + 1], dsm[kMGMaxLev + 1];
-    DevBuf cinv;
+    DevBuf cinv;                     // coarsest-level inverses, grid after grid
+    std::vector<int> cinv_off;
+    int cmax = 0;                    // nodes of the largest coarsest level
 };
 
 }  // namespace sfh
@@ -181,9 +190,10 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
 int solve_on(sfh_system* s, hipStream_t st, SolveWork& w, int trans, int k, const double* B, int ldb, double* X,
              int ldx, const sfh_solve_opts* o, sfh_solve_stats* stats);
 int prepare_operator(sfh_system* s, int nseg = 1, const int* seg_rows = nullptr);
-// multigrid hierarchy of the system's W x H grid from its FP64 stencil copy
-// (s->mg.L stays 0 when the grid is too small or a level is not SPD)
-int mg_build(sfh_system* s, hipStream_t st, int W, int H);
+// multigrid hierarchies of the system's grids (first row, W, H per grid) from
+// its FP64 stencil copy (s->mg.L stays 0 when every grid is small enough to be
+// preconditioned by Jacobi alone, or when a coarsest level is not SPD)
+int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids);
 int triplets_build(sfh_system* s, long long n, const int* rows, const int* cols, const double* vals);
 int triplets_values(sfh_system* s, long long n, const double* vals);
 int covforcing_grid(sfh_ctx* c, int d, const int gn[3], const double* const ax[3], const double* cv, double sigma,

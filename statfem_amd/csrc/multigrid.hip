@@ -5,7 +5,7 @@
 // makes the iteration count independent of the grid size.
 //
 // Levels: level 0 is the W x H node grid of the operator (its 9-coefficient
-// stencil). Level l+1 keeps every other node of level l in each direction,
+// stencil); a batch of grids gets one hierarchy per grid, levels side by side. Level l+1 keeps every other node of level l in each direction,
 // ((W+1)/2) x ((H+1)/2) nodes. Prolongation P is bilinear: a fine node at an
 // even index copies its coarse parent, one at an odd index averages the two
 // neighbouring parents, and an odd last node (even W) copies its single parent
@@ -149,43 +149,89 @@ bool coarse_inverse(int W, int H, const std::vector<double>& a9, std::vector<flo
 
 }  // namespace
 
-int mg_build(sfh_system* s, hipStream_t st, int W, int H) {
+int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
     MGHier& m = s->mg;
+    const int ng = (int)grids.size() / 3;
     m.L = 0;
-    m.W[0] = W;
-    m.H[0] = H;
-    SFH_TRY(m.dsm[0].ensure(sizeof(float) * (size_t)W * H));
-    k_mg_level<<<(W * H + 255) / 256, 256, 0, st>>>(W * H, s->dia64.as<double>(), m.dsm[0].as<float>(), nullptr);
+    m.nseg = ng;
+    m.segL.assign(ng, 0);
+    bool big = false;
+    for (int q = 0; q < ng; ++q) big = big || (long long)grids[3 * q + 1] * grids[3 * q + 2] > kMGCoarse;
+    if (!big || ng == 0) return SFH_OK;   // Jacobi alone for small grids
+    // level sizes: coarsen every grid until it has at most kMGCoarse nodes
+    // (at least once, so that every grid of the batch has a coarse level)
+    std::vector<int> W(grids.size() / 3), H(W.size());
+    for (int q = 0; q < ng; ++q) {
+        int w = grids[3 * q + 1], h = grids[3 * q + 2], l = 0;
+        while (l < kMGMaxLev && (l == 0 || (long long)w * h > kMGCoarse) && (w > 2 || h > 2)) {
+            w = (w + 1) / 2;
+            h = (h + 1) / 2;
+            ++l;
+        }
+        m.segL[q] = l;
+        m.L = std::max(m.L, l);
+    }
+    m.gW.assign((size_t)(m.L + 1) * ng, 0);
+    m.gH.assign(m.gW.size(), 0);
+    m.gbase.assign(m.gW.size(), 0);
+    for (int l = 0; l <= m.L; ++l) {
+        long long base = 0;
+        for (int q = 0; q < ng; ++q) {
+            if (l > m.segL[q]) continue;
+            int w = grids[3 * q + 1], h = grids[3 * q + 2];
+            for (int u = 0; u < l; ++u) { w = (w + 1) / 2; h = (h + 1) / 2; }
+            const size_t o = (size_t)l * ng + q;
+            m.gW[o] = w;
+            m.gH[o] = h;
+            m.gbase[o] = l == 0 ? grids[3 * q] : (int)base;
+            base += (long long)w * h;
+        }
+        m.N[l] = l == 0 ? s->ndof : base;
+    }
+    SFH_TRY(m.dsm[0].ensure(sizeof(float) * (size_t)s->ndof));
+    k_mg_level<<<(s->ndof + 255) / 256, 256, 0, st>>>(s->ndof, s->dia64.as<double>(), m.dsm[0].as<float>(), nullptr);
     SFH_LAUNCH_CHECK();
-    const double* A = s->dia64.as<double>();
-    int l = 0;
-    while ((long long)m.W[l] * m.H[l] > kMGCoarse && l < kMGMaxLev && (m.W[l] > 2 || m.H[l] > 2)) {
-        const int Wf = m.W[l], Hf = m.H[l], Wc = (Wf + 1) / 2, Hc = (Hf + 1) / 2, nc = Wc * Hc;
-        m.W[l + 1] = Wc;
-        m.H[l + 1] = Hc;
+    for (int l = 0; l < m.L; ++l) {
+        const long long nc = m.N[l + 1];
         SFH_TRY(m.coef64[l + 1].ensure(sizeof(double) * 9 * (size_t)nc));
         SFH_TRY(m.coef32[l + 1].ensure(sizeof(float) * 9 * (size_t)nc));
         SFH_TRY(m.dsm[l + 1].ensure(sizeof(float) * (size_t)nc));
-        k_mg_galerkin<<<(nc + 255) / 256, 256, 0, st>>>(Wf, Hf, Wc, Hc, A, l == 0 ? s->cmask.as<unsigned char>() : nullptr,
-                                                        m.coef64[l + 1].as<double>());
+        const double* Af = l == 0 ? s->dia64.as<double>() : m.coef64[l].as<double>();
+        for (int q = 0; q < ng; ++q) {
+            if (m.segL[q] <= l) continue;
+            const size_t o = (size_t)l * ng + q, oc = (size_t)(l + 1) * ng + q;
+            const int ncq = m.gW[oc] * m.gH[oc];
+            k_mg_galerkin<<<(ncq + 255) / 256, 256, 0, st>>>(
+                m.gW[o], m.gH[o], m.gW[oc], m.gH[oc], Af + (size_t)m.gbase[o] * 9,
+                l == 0 ? s->cmask.as<unsigned char>() + m.gbase[o] : nullptr,
+                m.coef64[l + 1].as<double>() + (size_t)m.gbase[oc] * 9);
+            SFH_LAUNCH_CHECK();
+        }
+        k_mg_level<<<(unsigned)((nc + 255) / 256), 256, 0, st>>>((int)nc, m.coef64[l + 1].as<double>(),
+                                                                 m.dsm[l + 1].as<float>(),
+                                                                 m.coef32[l + 1].as<float>());
         SFH_LAUNCH_CHECK();
-        k_mg_level<<<(nc + 255) / 256, 256, 0, st>>>(nc, m.coef64[l + 1].as<double>(), m.dsm[l + 1].as<float>(),
-                                                     m.coef32[l + 1].as<float>());
-        SFH_LAUNCH_CHECK();
-        A = m.coef64[l + 1].as<double>();
-        ++l;
     }
-    if (l == 0) return SFH_OK;   // grid already small: no hierarchy
-    const int nL = m.W[l] * m.H[l];
-    std::vector<double> a9((size_t)nL * 9);
-    SFH_CHECK_HIP(hipMemcpyAsync(a9.data(), A, sizeof(double) * 9 * (size_t)nL, hipMemcpyDeviceToHost, st));
+    // dense inverses of the coarsest levels
+    std::vector<float> all;
+    m.cinv_off.assign(ng, 0);
+    m.cmax = 0;
+    for (int q = 0; q < ng; ++q) {
+        const size_t o = (size_t)m.segL[q] * ng + q;
+        const int w = m.gW[o], h = m.gH[o], n = w * h;
+        std::vector<double> a9((size_t)n * 9);
+        SFH_CHECK_HIP(hipMemcpyAsync(a9.data(), m.coef64[m.segL[q]].as<double>() + (size_t)m.gbase[o] * 9,
+                                     sizeof(double) * 9 * (size_t)n, hipMemcpyDeviceToHost, st));
+        SFH_CHECK_HIP(hipStreamSynchronize(st));
+        std::vector<float> inv;
+        if (!coarse_inverse(w, h, a9, inv)) { m.L = 0; return SFH_OK; }   // not SPD: Jacobi only
+        m.cinv_off[q] = (int)all.size();
+        all.insert(all.end(), inv.begin(), inv.end());
+        m.cmax = std::max(m.cmax, n);
+    }
+    SFH_TRY(m.cinv.ensure(sizeof(float) * all.size()));
+    SFH_CHECK_HIP(hipMemcpyAsync(m.cinv.p, all.data(), sizeof(float) * all.size(), hipMemcpyHostToDevice, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
-    std::vector<float> inv;
-    if (!coarse_inverse(m.W[l], m.H[l], a9, inv)) return SFH_OK;   // not SPD: Jacobi only
-    SFH_TRY(m.cinv.ensure(sizeof(float) * inv.size()));
-    SFH_CHECK_HIP(hipMemcpyAsync(m.cinv.p, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice, st));
-    SFH_CHECK_HIP(hipStreamSynchronize(st));
-    m.L = l;
     return SFH_OK;
 }
 

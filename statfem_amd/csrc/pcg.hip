@@ -1034,20 +1034,27 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
     }
 }
 
-// e = C^-1 r on the coarsest level (n nodes, dense FP32 inverse, row-major):
-// one work-group per 16 columns stages C^-1 and its residual columns in LDS
+// e = C^-1 r on the coarsest level of every grid (dense FP32 inverses,
+// row-major): work-group (column group x, grid y) stages C^-1 and its 16
+// residual columns in LDS. Table per grid: nodes, C^-1 offset, first float of
+// the grid's residual and correction in the coarse-level buffer.
 constexpr int kCoarseCols = 16;
 inline size_t mg_coarse_lds(int n) {
     return sizeof(float) * (((size_t)n * n + 3) / 4 * 4) + sizeof(float4) * (size_t)n * (kCoarseCols / 4);
 }
-__global__ __launch_bounds__(256) void k_mg_coarse(int n, int kp, const float* __restrict__ cinv,
-                                                   const float* __restrict__ r, float* __restrict__ e) {
+__global__ __launch_bounds__(256) void k_mg_coarse(const int* __restrict__ ctab, int kp, const float* __restrict__ cinv,
+                                                   float* __restrict__ mgv) {
     extern __shared__ float4 sm[];
+    const int* d = ctab + 4 * blockIdx.y;
+    const int n = d[0];
+    const float* ci_g = cinv + d[1];
+    const float* r = mgv + d[2];
+    float* e = mgv + d[3];
     float* ci = reinterpret_cast<float*>(sm);
     float4* rs = sm + ((size_t)n * n + 3) / 4;
     constexpr int nq = kCoarseCols / 4;
     const int c0 = blockIdx.x * kCoarseCols;
-    for (int i = threadIdx.x; i < n * n; i += blockDim.x) ci[i] = cinv[i];
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) ci[i] = ci_g[i];
     for (int i = threadIdx.x; i < n * nq; i += blockDim.x) {
         const int j = i / nq, col = c0 + 4 * (i % nq);
         rs[i] = col < kp ? *reinterpret_cast<const float4*>(r + (size_t)j * kp + col) : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1063,12 +1070,20 @@ __global__ __launch_bounds__(256) void k_mg_coarse(int n, int kp, const float* _
 }
 
 // residual replacement with the multigrid preconditioner: r *= s' in place
-// (FP64), p *= s'/s (one problem: scalars indexed by column)
-__global__ void k_mg_rescale(long long total, int kp, const double* __restrict__ scale,
-                             const double* __restrict__ ratio, double* __restrict__ r, float* __restrict__ p) {
+// (FP64), p *= s'/s; the row's problem (srow: nseg + 1 row bounds) selects the
+// scalars
+__global__ void k_mg_rescale(long long total, int kp, int nseg, const int* __restrict__ srow,
+                             const double* __restrict__ scale, const double* __restrict__ ratio,
+                             double* __restrict__ r, float* __restrict__ p) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    const int c = (int)(i % kp);
+    const int row = (int)(i / kp);
+    int lo = 0, hi = nseg - 1;   // last segment with srow[seg] <= row
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (srow[mid] <= row) lo = mid; else hi = mid - 1;
+    }
+    const size_t c = (size_t)lo * kp + i % kp;
     r[i] *= scale[c];
     p[i] *= (float)ratio[c];
 }
@@ -1481,6 +1496,11 @@ struct Plan {
     int mg_roff[kMGMaxLev] = {}, mg_rn[kMGMaxLev] = {}, mg_poff[kMGMaxLev] = {}, mg_pn[kMGMaxLev] = {};
     size_t mgv_off[kMGMaxLev + 1] = {}, mgv_total = 0;
     const int* d_mgtab = nullptr;
+    // coarse solves (4 ints per grid, see k_mg_coarse) and the level-0
+    // prolongation tiles of every grid (nseg + 1 bounds: partial-row ranges
+    // of a residual replacement)
+    std::vector<int> mgctab, mgsb;
+    const int* d_mgctab = nullptr;
 };
 
 Plan make_plan(int n, int k, int num_cu, int nseg = 1, const int* seg_rows = nullptr, int min_kp = 4,
@@ -1559,33 +1579,56 @@ void pad_mgtab(std::vector<int>& tab, int& cnt) {
 void plan_mg(Plan& p, const MGHier& m) {
     p.mgL = m.L;
     p.mgtab.clear();
+    const int ng = m.nseg;
     for (int l = 0; l < m.L; ++l) {
-        const int W = m.W[l], H = m.H[l], Wc = m.W[l + 1], Hc = m.H[l + 1];
         p.mg_roff[l] = (int)(p.mgtab.size() / kMT);
         int cnt = 0;
-        for (int cy = 0; cy < Hc; cy += kRY)
-            for (int cx = 0; cx < Wc; cx += kRX, ++cnt) {
-                const int e[kMT] = {0, W, H, 0, Wc, Hc, cx, cy, 0};
-                p.mgtab.insert(p.mgtab.end(), e, e + kMT);
-            }
+        for (int q = 0; q < ng; ++q) {
+            if (m.segL[q] <= l) continue;
+            const size_t o = (size_t)l * ng + q, oc = (size_t)(l + 1) * ng + q;
+            const int W = m.gW[o], H = m.gH[o], Wc = m.gW[oc], Hc = m.gH[oc];
+            for (int cy = 0; cy < Hc; cy += kRY)
+                for (int cx = 0; cx < Wc; cx += kRX, ++cnt) {
+                    const int e[kMT] = {m.gbase[o], W, H, m.gbase[oc], Wc, Hc, cx, cy, q};
+                    p.mgtab.insert(p.mgtab.end(), e, e + kMT);
+                }
+        }
         pad_mgtab(p.mgtab, cnt);
         p.mg_rn[l] = cnt;
         p.mg_poff[l] = (int)(p.mgtab.size() / kMT);
         cnt = 0;
-        for (int ty = 0; ty < H; ty += kPY)
-            for (int tx = 0; tx < W; tx += kPX, ++cnt) {
-                const int e[kMT] = {0, W, H, 0, Wc, Hc, tx, ty, 0};
-                p.mgtab.insert(p.mgtab.end(), e, e + kMT);
-            }
-        pad_mgtab(p.mgtab, cnt);
+        if (l == 0) p.mgsb.assign(ng + 1, 0);
+        for (int q = 0; q < ng; ++q) {
+            if (l == 0) p.mgsb[q] = cnt;
+            if (m.segL[q] <= l) continue;
+            const size_t o = (size_t)l * ng + q, oc = (size_t)(l + 1) * ng + q;
+            const int W = m.gW[o], H = m.gH[o], Wc = m.gW[oc], Hc = m.gH[oc];
+            for (int ty = 0; ty < H; ty += kPY)
+                for (int tx = 0; tx < W; tx += kPX, ++cnt) {
+                    const int e[kMT] = {m.gbase[o], W, H, m.gbase[oc], Wc, Hc, tx, ty, q};
+                    p.mgtab.insert(p.mgtab.end(), e, e + kMT);
+                }
+        }
+        pad_mgtab(p.mgtab, cnt);   // padding tiles count with the last grid
         p.mg_pn[l] = cnt;
+        if (l == 0) p.mgsb[ng] = cnt;
     }
     size_t off = 0;
     for (int l = 1; l <= m.L; ++l) {
         p.mgv_off[l] = off;
-        off += 2 * (size_t)m.W[l] * m.H[l] * p.kp;
+        off += 2 * (size_t)m.N[l] * p.kp;
     }
     p.mgv_total = off;
+    p.mgctab.clear();
+    for (int q = 0; q < ng; ++q) {
+        const int l = m.segL[q];
+        const size_t o = (size_t)l * ng + q;
+        const int n = m.gW[o] * m.gH[o];
+        const size_t r = p.mgv_off[l] + (size_t)m.gbase[o] * p.kp;
+        const size_t e = p.mgv_off[l] + ((size_t)m.N[l] + m.gbase[o]) * p.kp;
+        const int c[4] = {n, m.cinv_off[q], (int)r, (int)e};
+        p.mgctab.insert(p.mgctab.end(), c, c + 4);
+    }
 }
 
 template <class T, int LPR>
@@ -1635,18 +1678,15 @@ struct Kern {
             constexpr int ML = LPR > 8 ? 8 : LPR;
             const int L = P.mgL, kp = P.kp, ntm = (kp + 4 * ML - 1) / (4 * ML);
             auto rvec = [&](int l) -> float* { return mgv + P.mgv_off[l]; };
-            auto evec = [&](int l) -> float* {
-                return l == 0 ? z : mgv + P.mgv_off[l] + (size_t)m.W[l] * m.H[l] * kp;
-            };
+            auto evec = [&](int l) -> float* { return l == 0 ? z : mgv + P.mgv_off[l] + (size_t)m.N[l] * kp; };
             k_mg_restrict<ML, double><<<dim3(P.mg_rn[0], ntm), kG, mg_restrict_lds(ML), st>>>(
                 P.d_mgtab + (size_t)kMT * P.mg_roff[0], kp, coef0, m.dsm[0].as<float>(), cmask, r0, rvec(1));
             for (int l = 1; l < L; ++l)
                 k_mg_restrict<ML, float><<<dim3(P.mg_rn[l], ntm), kG, mg_restrict_lds(ML), st>>>(
                     P.d_mgtab + (size_t)kMT * P.mg_roff[l], kp, m.coef32[l].as<float>(), m.dsm[l].as<float>(),
                     nullptr, rvec(l), rvec(l + 1));
-            const int nL = m.W[L] * m.H[L];
-            k_mg_coarse<<<(kp + kCoarseCols - 1) / kCoarseCols, 256, mg_coarse_lds(nL), st>>>(
-                nL, kp, m.cinv.as<float>(), rvec(L), evec(L));
+            k_mg_coarse<<<dim3((kp + kCoarseCols - 1) / kCoarseCols, m.nseg), 256, mg_coarse_lds(m.cmax), st>>>(
+                P.d_mgctab, kp, m.cinv.as<float>(), mgv);
             for (int l = L - 1; l >= 1; --l)
                 k_mg_prolong<LPR, false, float><<<dim3(P.mg_pn[l], P.ntile), kG, mg_prolong_lds(LPR), st>>>(
                     P.d_mgtab + (size_t)kMT * P.mg_poff[l], kp, P.ks, it, mode, m.coef32[l].as<float>(),
@@ -1745,9 +1785,9 @@ int prepare_operator_on(sfh_system* s, hipStream_t st, const std::vector<int>& g
         s->mg.built = false;
         s->mg.L = 0;
     }
-    // multigrid hierarchy of a single grid, from the stencil copy
-    if (grids.size() == 3 && s->dia_state == 1 && !s->mg.built) {
-        SFH_TRY(mg_build(s, st, grids[1], grids[2]));
+    // multigrid hierarchies of the grids, from the stencil copy
+    if (!grids.empty() && s->dia_state == 1 && !s->mg.built) {
+        SFH_TRY(mg_build(s, st, grids));
         s->mg.built = true;
     }
     return SFH_OK;
@@ -1782,8 +1822,10 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         plan_bsr3(P, s->nnodes, s->ctx->num_cu);
     // multigrid preconditioner: stencil layout of one grid with a hierarchy
     const MGHier& mg = s->mg;
-    const bool use_mg = P.grid && ns == 1 && o.precond != 1 && mg.built && mg.L > 0 &&
-                        (long long)mg.W[0] * mg.H[0] == n && P.lpr >= 4;
+    bool use_mg = P.grid && o.precond != 1 && mg.built && mg.L > 0 && mg.nseg == (int)grids.size() / 3 &&
+                  P.lpr >= 4;
+    for (int q = 0; use_mg && q < mg.nseg; ++q)
+        use_mg = mg.gbase[q] == grids[3 * q] && mg.gW[q] == grids[3 * q + 1] && mg.gH[q] == grids[3 * q + 2];
     if (use_mg) plan_mg(P, mg);
     const int nbi = std::max(P.grid ? P.gblk : P.nblk, use_mg ? P.mg_pn[0] : 0);   // partial rows per launch
     SFH_TRY(w.b64.ensure(nk * 8));
@@ -1800,9 +1842,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     if (use_mg) SFH_TRY(w.mgv.ensure(P.mgv_total * sizeof(T)));
     // partition tables: brow (nblk + 1), bseg (nblk), sb (nseg + 1), srow (nseg + 1), gtab, multigrid tiles,
     // and the partial-row bounds {0, level-0 prolongation tiles} of a multigrid residual replacement
-    const int sbmg[2] = {0, use_mg ? P.mg_pn[0] : 0};
-    const size_t ntab =
-        P.brow.size() + P.bseg.size() + P.sb.size() + P.srow.size() + P.gtab.size() + P.mgtab.size() + 2;
+    const size_t ntab = P.brow.size() + P.bseg.size() + P.sb.size() + P.srow.size() + P.gtab.size() +
+                        P.mgtab.size() + P.mgctab.size() + P.mgsb.size();
     SFH_TRY(w.tab.ensure(sizeof(int) * ntab));
     {
         std::vector<int> h;
@@ -1813,7 +1854,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         h.insert(h.end(), P.srow.begin(), P.srow.end());
         h.insert(h.end(), P.gtab.begin(), P.gtab.end());
         h.insert(h.end(), P.mgtab.begin(), P.mgtab.end());
-        h.insert(h.end(), sbmg, sbmg + 2);
+        h.insert(h.end(), P.mgctab.begin(), P.mgctab.end());
+        h.insert(h.end(), P.mgsb.begin(), P.mgsb.end());
         SFH_CHECK_HIP(hipMemcpyAsync(w.tab.p, h.data(), sizeof(int) * ntab, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipStreamSynchronize(st));
     }
@@ -1823,7 +1865,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     const int* d_srow = d_sb + P.sb.size();
     P.d_gtab = d_srow + P.srow.size();
     P.d_mgtab = P.d_gtab + P.gtab.size();
-    const int* d_sbmg = P.d_mgtab + P.mgtab.size();
+    P.d_mgctab = P.d_mgtab + P.mgtab.size();
+    const int* d_sbmg = P.d_mgctab + P.mgctab.size();
     const Seg G = {d_brow, d_bseg, ks};
     const dim3 gred((kp + 63) / 64, ns);
     double* S = w.scal.as<double>();
@@ -1922,8 +1965,9 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_RATIO * ks, ratio.data(), ks * 8, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * ks, tgt.data(), ks * 8, hipMemcpyHostToDevice, st));
         if (use_mg) {
-            k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, S + S_SCALE * ks,
-                                                                      S + S_RATIO * ks, rmg, (float*)pa);
+            k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
+                                                                      S + S_SCALE * ks, S + S_RATIO * ks, rmg,
+                                                                      (float*)pa);
             Kern<T, LPR>::vcycle(P, mg, st, 0, 1, coef, cmask, rmg, z, mgv, acc, p0, p1);
             k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sbmg, p0, p1, S, acc, 0.0, jlast);
         } else {

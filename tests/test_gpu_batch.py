@@ -63,3 +63,47 @@ def test_batched_solve_forward_and_transposed(ctx, orc):
             scale = np.abs(ref[a:b]).max()
             assert np.abs(Xh[a:b] - ref[a:b]).max() <= 1e-9 * scale
         assert np.all(Xh[off[1]:off[2], 3:] == 0.0)
+
+
+def test_batched_multigrid_on_declared_grids(ctx, orc):
+    """A batch of declared grids of different sizes (odd and even node counts,
+    one small enough for a single coarse level) runs the multigrid-
+    preconditioned solve in shared launches; every problem matches its own
+    direct solve, whatever the scale of its right-hand sides."""
+    from statfem_amd import hip, mesh as M
+    sizes = [(40, 40), (9, 9), (57, 31), (24, 24)]
+    meshes = [M.grid2d(a, b) for a, b in sizes]
+    mu, off = M.union(meshes)
+    s = hip.SystemSolveHip(ctx, mu)
+    kap = np.exp(0.6 * np.random.default_rng(4).normal(size=mu.nel))
+    s.assemble(hip.POISSON, kap, nq1d=2)
+    bc = np.concatenate([M.dirichlet_all_boundary(m) + off[q] for q, m in enumerate(meshes)]).astype(np.int32)
+    s.applyConstraints(bc)
+    seg = np.ascontiguousarray(off, dtype=np.int32)
+    nx = np.array([a + 1 for a, b in sizes], np.int32)
+    ny = np.array([b + 1 for a, b in sizes], np.int32)
+    hip._check(hip.lib().sfh_system_set_segment_grids(s.h, len(sizes), hip._p(seg), hip._p(nx), hip._p(ny)))
+    rp, ci, va, _ = orc.assemble(mu, orc.POISSON, kap, nq1d=2, bc_dofs=bc)
+    lu = orc.LU(rp, ci, va)
+    rng = np.random.default_rng(5)
+    k = 20
+    B = rng.normal(size=(mu.nnodes, k))
+    for q in range(len(sizes)):
+        B[off[q]:off[q + 1]] *= 10.0 ** (2 * q)
+    B[off[2]:off[3], 7] = 0.0
+    for trans in (0, 1):
+        X = ctx.empty((mu.nnodes, k))
+        st = hip.SolveStats()
+        o = hip.default_opts(tol=1e-11)
+        Bd = ctx.to_device(B)
+        hip._check(hip.lib().sfh_solve_batched(s.h, len(sizes), hip._p(seg), trans, k, Bd.ptr, X.ptr, C.byref(o),
+                                               C.byref(st)))
+        assert st.precond == 2 and st.layout == 2 and st.converged == 1, st.as_dict()
+        assert st.inner_iters < 40, st.as_dict()
+        Xh = X.download()
+        ref = lu.solve(B, trans=bool(trans))
+        for q in range(len(sizes)):
+            a, b = off[q], off[q + 1]
+            scale = np.abs(ref[a:b]).max()
+            assert np.abs(Xh[a:b] - ref[a:b]).max() <= 1e-9 * scale, (q, trans)
+        assert np.all(Xh[off[2]:off[3], 7] == 0.0)
