@@ -257,9 +257,9 @@ def c3_run(args):
     from statfem_amd.spde_elasticity import SPDEElasticity
     cfg = configs.c3_inputs(seed=0, n=args.c3_n)
     ctx = hip.Context(0)
-    opts = hip.default_opts(tol=1e-10, rhs_tile=64)
+    opts = hip.default_opts(tol=1e-10)
     t0 = time.perf_counter()
-    pipe = SPDEElasticity(ctx, cfg, opts=opts, opts_matern=hip.default_opts(tol=1e-10, rhs_tile=64))
+    pipe = SPDEElasticity(ctx, cfg, opts=opts, opts_matern=hip.default_opts(tol=1e-10))
     mean, Cu = pipe.prior()
     t_prior = time.perf_counter() - t0
     y, g, u = pipe.synthesize()
@@ -307,10 +307,10 @@ def c4_run(args):
     means, Cus, st = bf.updateSolution(opts)
     t_batch = time.perf_counter() - t0
     seq = SequentialForwardGauss(ctx, probs)
-    seq.updateSolution(hip.default_opts(tol=1e-10, rhs_tile=64))              # warm-up
+    seq.updateSolution(hip.default_opts(tol=1e-10, rhs_tile=0))              # warm-up
     ctx.sync()
     t0 = time.perf_counter()
-    sm, sC, sit = seq.updateSolution(hip.default_opts(tol=1e-10, rhs_tile=64))
+    sm, sC, sit = seq.updateSolution(hip.default_opts(tol=1e-10, rhs_tile=0))
     t_seq = time.perf_counter() - t0
     rm, rC, _ = seq.updateSolution(hip.default_opts(tol=1e-12, precision=1))
     seq.close()

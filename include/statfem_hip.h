@@ -163,11 +163,13 @@ typedef struct {
     int layout;          /* 0 (default): auto, 1: CSR, 2: 9-coefficient stencil with LDS-tiled kernels on
                             declared 2D tensor grids (FP32, at most 64 right-hand sides per tile);
                             3: 3 x 3 node blocks for vector-DOF meshes (FP32, one problem);
-                            auto uses 2 or 3 where they apply, else CSR */
-    int precond;         /* 0 (default): auto, 1: Jacobi, 2: geometric multigrid V(1,1)-cycle (bilinear
-                            prolongation, Galerkin coarse stencils, l1-Jacobi smoothing, dense coarsest
-                            solve) on the stencil layout of a single declared 2D grid; auto uses 2
-                            wherever it applies, else 1 */
+                            4: 27-point block stencil on a declared 3D grid (1 or 3 DOFs per node, one
+                            problem), with the multigrid preconditioner;
+                            auto uses 2, 4 or 3 where they apply, else CSR */
+    int precond;         /* 0 (default): auto, 1: Jacobi, 2: geometric multigrid V(1,1)-cycle (bi- or
+                            trilinear prolongation, Galerkin coarse stencils, l1-Jacobi smoothing, dense
+                            coarsest solve) with FP64 recurrences, on declared 2D grids (one or a batch)
+                            and declared 3D grids; auto uses 2 wherever it applies, else 1 */
 } sfh_solve_opts;
 
 typedef struct {
@@ -176,7 +178,8 @@ typedef struct {
     double max_relres;    /* max over RHS of ||b - A x|| / ||b|| in FP64 */
     double ms;            /* device time */
     int converged;
-    int layout;           /* operator layout the iteration kernels used: 1 CSR, 2 2D stencil, 3 3x3 blocks */
+    int layout;           /* operator layout the iteration kernels used: 1 CSR, 2 2D stencil, 3 3x3 blocks,
+                             4 3D block stencil */
     int precond;          /* preconditioner used: 1 Jacobi, 2 multigrid */
     int levels;           /* multigrid levels including the fine grid (1 for Jacobi) */
 } sfh_solve_stats;

@@ -99,6 +99,28 @@ struct MGHier {
     int cmax = 0;                    // nodes of the largest coarsest level
 };
 
+// Multigrid hierarchy of a declared 3D grid (1 or 3 DOFs per node): the
+// operator as a 27-point block stencil (FP64 and FP32), the coarse levels'
+// Galerkin stencils, l1-Jacobi weights, and the coarsest level's inverse.
+struct MG3Hier {
+    bool built = false;              // current for the assembled operator
+    int L = 0, dpn = 0;              // coarse levels (0: no hierarchy), DOFs per node
+    int W[kMGMaxLev + 1] = {}, H[kMGMaxLev + 1] = {}, D[kMGMaxLev + 1] = {};
+    DevBuf dia64, dia32;
+    DevBuf coef64[kMGMaxLev + 1], coef32[kMGMaxLev + 1], dsm[kMGMaxLev + 1];
+    DevBuf cinv;
+    int nc = 0;                      // DOFs of the coarsest level
+};
+
+// per-solve layout of the 3D V-cycle: coarse-level vectors in the
+// workspace's multigrid buffer, coarse-solve table, launch shapes
+struct G3Plan {
+    int kp = 0, lpr = 0, ntile = 0, nsmooth = 0;
+    size_t off[kMGMaxLev + 1] = {}, total = 0;
+    int ctab[4] = {};
+    const int* d_ctab = nullptr;
+};
+
 }  // namespace sfh
 
 struct sfh_ctx {
@@ -153,7 +175,8 @@ struct sfh_system {
     sfh::DevBuf dia32, dia64;
     int dia_state = 0;           // 0 stale, 1 valid, -1 mesh does not fit the stencil
     std::vector<int> sgrid;      // batch of 2D grids: (first row, W, H) per problem
-    sfh::MGHier mg;              // built with the stencil copy of a single grid
+    sfh::MGHier mg;              // built with the stencil copy of declared 2D grids
+    sfh::MG3Hier mg3;            // declared 3D grid
     sfh::SolveWork work;
     sfh::SolveWork work2;        // workspace of a concurrent second solve
     sfh::DevBuf scratch0, scratch1, scratch2, scratch3;
@@ -194,6 +217,18 @@ int prepare_operator(sfh_system* s, int nseg = 1, const int* seg_rows = nullptr)
 // its FP64 stencil copy (s->mg.L stays 0 when every grid is small enough to be
 // preconditioned by Jacobi alone, or when a coarsest level is not SPD)
 int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids);
+// 3D grids: whether the system is a declared 3D grid the hierarchy applies to;
+// build it (s->mg3.L stays 0 when the operator is not 27-point on the grid or
+// the coarsest level is not SPD); plan and launch the iteration and V-cycle
+bool g3_applies(const sfh_system* s);
+int g3_prepare(sfh_system* s, hipStream_t st);
+void g3_plan(G3Plan& P, const MG3Hier& m, int kp, int lpr);
+void g3_dir(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, double* S, const float* z, const float* pold,
+            float* pnew, double* acc);
+void g3_update(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, const double* S, const float* p, double* x,
+               double* r, double* acc);
+void g3_vcycle(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, int mode, const unsigned char* cmask,
+               const double* r, float* z, float* mgv, double* acc, double* p0, double* p1);
 int triplets_build(sfh_system* s, long long n, const int* rows, const int* cols, const double* vals);
 int triplets_values(sfh_system* s, long long n, const double* vals);
 int covforcing_grid(sfh_ctx* c, int d, const int gn[3], const double* const ax[3], const double* cv, double sigma,

@@ -1,0 +1,698 @@
+// 3D tensor-grid path of the solver: the operator as a 27-point block stencil
+// (1 or 3 DOFs per node), its geometric multigrid hierarchy, one V(1,1)-cycle
+// as the preconditioner, and the PCG iteration with FP64 recurrences.
+// No reference counterpart: the reference factorises the assembled operator
+// (corlib/SystemSolveEigenSparse.hpp:273-311); on a declared 3D grid (hex8
+// elasticity, hex8 scalar operators) this library iterates instead.
+//
+// Layout: node (x, y, z) of a W x H x D grid is n = (z H + y) W + x (the mesh's
+// node order), DOF (n, a) is row n DPN + a (interleaved components, the
+// reference's numbering). Stencil coefficient (k, a, b) of node n couples DOF
+// (n, a) with DOF (n + off(k), b), off(k) = (k % 3 - 1, k / 3 % 3 - 1,
+// k / 9 - 1); stored at ((n 27 + k) DPN + a) DPN + b, FP64 and FP32.
+//
+// Hierarchy: level l + 1 keeps every other node of level l in each direction;
+// trilinear prolongation per component (odd last nodes copy their single
+// parent), restricted to free DOFs on the finest level; Galerkin coarse
+// operators P^T A P in FP64, 27-point again; l1-Jacobi smoothing (omega 4/3,
+// D_l1 = row sums of |a|) as in the 2D hierarchy; dense FP32 inverse of the
+// coarsest level (at most kMGCoarse DOFs).
+//
+// Kernels gather their 27 neighbours from global memory (the node-major
+// layout makes each neighbour's kp columns one contiguous run); a lane group
+// of LPR lanes owns one node and 4 columns per lane of each of its DPN rows.
+#include <cmath>
+
+#include "pcg_dev.hpp"
+
+namespace sfh {
+namespace {
+
+constexpr int kB3 = 256;
+constexpr double kOmega3 = 4.0 / 3.0;
+
+struct G3 {
+    int W, H, D;
+};
+
+__host__ __device__ inline int g3n(const G3& g) { return g.W * g.H * g.D; }
+
+// node index of neighbour k of (x, y, z), -1 off the grid
+__device__ inline int nb27(int k, int x, int y, int z, const G3& g) {
+    const int a = x + k % 3 - 1, b = y + (k / 3) % 3 - 1, c = z + k / 9 - 1;
+    if (a < 0 || a >= g.W || b < 0 || b >= g.H || c < 0 || c >= g.D) return -1;
+    return (c * g.H + b) * g.W + a;
+}
+
+// 1D trilinear weight of coarse index X in fine index x (n fine nodes)
+__host__ __device__ inline float w1(int x, int X, int n) {
+    if ((x & 1) == 0) return x == 2 * X ? 1.f : 0.f;
+    if (x == n - 1) return (x - 1) / 2 == X ? 1.f : 0.f;
+    return ((x - 1) / 2 == X || (x + 1) / 2 == X) ? 0.5f : 0.f;
+}
+
+// coarse parents of fine index x: first parent, count, weight of each
+__host__ __device__ inline int parents1(int x, int n, int& p0, float& w) {
+    if ((x & 1) == 0) { p0 = x / 2; w = 1.f; return 1; }
+    if (x == n - 1) { p0 = (x - 1) / 2; w = 1.f; return 1; }
+    p0 = (x - 1) / 2; w = 0.5f;
+    return 2;
+}
+
+// node tile of work-group (logical) b: nodes [b npb, min(n, (b + 1) npb))
+template <int LPR>
+__device__ inline Tile tile3(int npb, int n) {
+    Tile t;
+    t.lane = threadIdx.x % LPR;
+    t.grp = threadIdx.x / LPR;
+    t.ngrp = blockDim.x / LPR;
+    t.col0 = (blockIdx.y * LPR + t.lane) * 4;
+    t.bid = xcd_block(blockIdx.x, gridDim.x);
+    t.r0 = min(n, t.bid * npb);
+    t.r1 = min(n, t.r0 + npb);
+    t.so = 0;
+    return t;
+}
+
+// ---------------------------------------------------------------- set-up
+// CSR -> 27-point block stencil; an entry that couples no grid neighbour sets err
+template <int DPN>
+__global__ void k_build_dia27(G3 g, const int* __restrict__ rp, const int* __restrict__ ci,
+                              const double* __restrict__ va, double* __restrict__ d64, float* __restrict__ d32,
+                              int* err) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= g3n(g) * DPN) return;
+    const int n = row / DPN, a = row % DPN;
+    const int x = n % g.W, y = (n / g.W) % g.H, z = n / (g.W * g.H);
+    for (int k = 0; k < 27; ++k)
+        for (int b = 0; b < DPN; ++b) {
+            const size_t o = (((size_t)n * 27 + k) * DPN + a) * DPN + b;
+            d64[o] = 0.0;
+            d32[o] = 0.f;
+        }
+    for (int e = rp[row]; e < rp[row + 1]; ++e) {
+        const int c = ci[e], m = c / DPN, b = c % DPN;
+        const int dx = m % g.W - x, dy = (m / g.W) % g.H - y, dz = m / (g.W * g.H) - z;
+        if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) { atomicOr(err, 1); continue; }
+        const int k = (dz + 1) * 9 + (dy + 1) * 3 + dx + 1;
+        const size_t o = (((size_t)n * 27 + k) * DPN + a) * DPN + b;
+        d64[o] = va[e];
+        d32[o] = (float)va[e];
+    }
+}
+
+// A_c = P^T A P for coarse row (I, a): every fine node i of I's support, every
+// stencil neighbour j of i and every coarse parent J of j. cmask (finest level,
+// per DOF) drops constrained DOFs from P. A row without free support becomes
+// a decoupled unit row.
+template <int DPN>
+__global__ __launch_bounds__(64) void k_g3_galerkin(G3 gf, G3 gc, const double* __restrict__ Af,
+                                                    const unsigned char* __restrict__ cmask,
+                                                    double* __restrict__ Ac) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= g3n(gc) * DPN) return;
+    const int I = row / DPN, a = row % DPN;
+    const int IX = I % gc.W, IY = (I / gc.W) % gc.H, IZ = I / (gc.W * gc.H);
+    double acc[27 * DPN];
+    for (int q = 0; q < 27 * DPN; ++q) acc[q] = 0.0;
+    for (int di = 0; di < 27; ++di) {
+        const int ix = 2 * IX + di % 3 - 1, iy = 2 * IY + (di / 3) % 3 - 1, iz = 2 * IZ + di / 9 - 1;
+        if (ix < 0 || ix >= gf.W || iy < 0 || iy >= gf.H || iz < 0 || iz >= gf.D) continue;
+        const double wi = (double)w1(ix, IX, gf.W) * w1(iy, IY, gf.H) * w1(iz, IZ, gf.D);
+        const int i = (iz * gf.H + iy) * gf.W + ix;
+        if (wi == 0.0 || (cmask && cmask[(size_t)i * DPN + a])) continue;
+        for (int k = 0; k < 27; ++k) {
+            const int j = nb27(k, ix, iy, iz, gf);
+            if (j < 0) continue;
+            const int jx = j % gf.W, jy = (j / gf.W) % gf.H, jz = j / (gf.W * gf.H);
+            int px, py, pz;
+            float fx, fy, fz;
+            const int nx = parents1(jx, gf.W, px, fx), ny = parents1(jy, gf.H, py, fy),
+                      nz = parents1(jz, gf.D, pz, fz);
+            for (int b = 0; b < DPN; ++b) {
+                const double v = Af[(((size_t)i * 27 + k) * DPN + a) * DPN + b];
+                if (v == 0.0 || (cmask && cmask[(size_t)j * DPN + b])) continue;
+                for (int uz = 0; uz < nz; ++uz)
+                    for (int uy = 0; uy < ny; ++uy)
+                        for (int ux = 0; ux < nx; ++ux) {
+                            const int K = (pz + uz - IZ + 1) * 9 + (py + uy - IY + 1) * 3 + (px + ux - IX + 1);
+                            acc[K * DPN + b] += wi * v * ((double)fx * fy * fz);
+                        }
+            }
+        }
+    }
+    if (!(acc[13 * DPN + a] > 0.0)) {
+        for (int q = 0; q < 27 * DPN; ++q) acc[q] = 0.0;
+        acc[13 * DPN + a] = 1.0;
+    }
+    for (int K = 0; K < 27; ++K)
+        for (int b = 0; b < DPN; ++b) Ac[(((size_t)I * 27 + K) * DPN + a) * DPN + b] = acc[K * DPN + b];
+}
+
+// l1-Jacobi weights omega / sum |a| per row, and the FP32 stencil copy
+__global__ void k_g3_level(int nrow, int dpn, const double* __restrict__ A, float* __restrict__ dsm,
+                           float* __restrict__ A32) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrow) return;
+    const int n = row / dpn, a = row % dpn;
+    double s = 0.0;
+    for (int k = 0; k < 27; ++k)
+        for (int b = 0; b < dpn; ++b) {
+            const size_t o = (((size_t)n * 27 + k) * dpn + a) * dpn + b;
+            s += fabs(A[o]);
+            if (A32) A32[o] = (float)A[o];
+        }
+    dsm[row] = s > 0.0 ? (float)(kOmega3 / s) : 0.f;
+}
+
+// dense FP64 Cholesky inverse of the coarsest level's stencil, FP32 result
+bool coarse_inverse3(const G3& g, int dpn, const std::vector<double>& a, std::vector<float>& inv) {
+    const int nn = g3n(g), n = nn * dpn;
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int m = 0; m < nn; ++m) {
+        const int x = m % g.W, y = (m / g.W) % g.H, z = m / (g.W * g.H);
+        for (int k = 0; k < 27; ++k) {
+            const int jx = x + k % 3 - 1, jy = y + (k / 3) % 3 - 1, jz = z + k / 9 - 1;
+            if (jx < 0 || jx >= g.W || jy < 0 || jy >= g.H || jz < 0 || jz >= g.D) continue;
+            const int j = (jz * g.H + jy) * g.W + jx;
+            for (int ca = 0; ca < dpn; ++ca)
+                for (int cb = 0; cb < dpn; ++cb)
+                    A[(size_t)(m * dpn + ca) * n + j * dpn + cb] = a[(((size_t)m * 27 + k) * dpn + ca) * dpn + cb];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j)
+            A[(size_t)i * n + j] = A[(size_t)j * n + i] = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
+    for (int j = 0; j < n; ++j) {
+        double d = A[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+        if (!(d > 0.0)) return false;
+        d = std::sqrt(d);
+        A[(size_t)j * n + j] = d;
+        for (int i = j + 1; i < n; ++i) {
+            double s = A[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+            A[(size_t)i * n + j] = s / d;
+        }
+    }
+    inv.assign((size_t)n * n, 0.f);
+    std::vector<double> col(n);
+    for (int c = 0; c < n; ++c) {
+        for (int i = 0; i < n; ++i) {
+            double s = i == c ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) s -= A[(size_t)i * n + k] * col[k];
+            col[i] = s / A[(size_t)i * n + i];
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double s = col[i];
+            for (int k = i + 1; k < n; ++k) s -= A[(size_t)k * n + i] * col[k];
+            col[i] = s / A[(size_t)i * n + i];
+        }
+        for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = (float)col[i];
+    }
+    return true;
+}
+
+// ------------------------------------------------------------ iteration
+// p_new = m z + b p_old for every row (streaming); clears the r.z and r.r
+// slots of this iteration and takes the freeze decisions
+template <int LPR>
+__global__ __launch_bounds__(kB3) void k_g3_pdir(int kp, int nrow, int rpb, int it, double* __restrict__ S,
+                                                 const float* __restrict__ z, const float* __restrict__ pold,
+                                                 float* __restrict__ pnew, double* __restrict__ acc) {
+    Tile t = tile3<LPR>(rpb, nrow);
+    clear_slots(acc, A_RZ + it % 3, 1, kp);
+    clear_slots(acc, A_RR + it % 3, 1, kp);
+    if (block_done(t, kp, kp, S)) return;
+    __shared__ float cb[4 * LPR], cm[4 * LPR];
+    shared_dir_coef<float, LPR>(it, kp, kp, 0, S, acc, cb, cm);
+    if (t.col0 >= kp) return;
+    const int lc = t.lane * 4;
+    const V4<float> b = {cb[lc], cb[lc + 1], cb[lc + 2], cb[lc + 3]};
+    const V4<float> m = {cm[lc], cm[lc + 1], cm[lc + 2], cm[lc + 3]};
+    for (int r = t.r0 + t.grp; r < t.r1; r += t.ngrp) {
+        const size_t o = (size_t)r * kp + t.col0;
+        V4<float> v;
+        pdir4(ld4(z + o), m, b, ld4(pold + o), v);
+        st4(pnew + o, v);
+    }
+}
+
+// q = A p at node n (FP64 stencil and sums, FP32 p)
+template <int DPN>
+__device__ inline void apply_p64(const G3& g, int n, const double* __restrict__ coef, const float* __restrict__ p,
+                                 int kp, int col, V4<double> q[DPN]) {
+    const int x = n % g.W, y = (n / g.W) % g.H, z = n / (g.W * g.H);
+#pragma unroll
+    for (int a = 0; a < DPN; ++a) q[a] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < 27; ++k) {
+        const int j = nb27(k, x, y, z, g);
+        if (j < 0) continue;
+        const double* c = coef + ((size_t)n * 27 + k) * DPN * DPN;
+        V4<float> v[DPN];
+#pragma unroll
+        for (int b = 0; b < DPN; ++b) v[b] = ld4(p + ((size_t)j * DPN + b) * kp + col);
+#pragma unroll
+        for (int a = 0; a < DPN; ++a)
+#pragma unroll
+            for (int b = 0; b < DPN; ++b) {
+                const double w = c[a * DPN + b];
+                q[a].x += w * v[b].x; q[a].y += w * v[b].y; q[a].z += w * v[b].z; q[a].w += w * v[b].w;
+            }
+    }
+}
+
+// p.Ap into the iteration's accumulator slot
+template <int DPN, int LPR>
+__global__ __launch_bounds__(kB3) void k_g3_pq(G3 g, int kp, int npb, int it, const double* __restrict__ coef,
+                                               const double* __restrict__ S, const float* __restrict__ p,
+                                               double* __restrict__ acc) {
+    __shared__ double red[kB3 * 4];
+    const Tile t = tile3<LPR>(npb, g3n(g));
+    if (block_done(t, kp, kp, S)) return;
+    V4<double> pq = {0.0, 0.0, 0.0, 0.0};
+    if (t.col0 < kp) {
+        for (int n = t.r0 + t.grp; n < t.r1; n += t.ngrp) {
+            V4<double> q[DPN];
+            apply_p64<DPN>(g, n, coef, p, kp, t.col0, q);
+#pragma unroll
+            for (int a = 0; a < DPN; ++a) {
+                const V4<float> pi = ld4(p + ((size_t)n * DPN + a) * kp + t.col0);
+                pq.x += pi.x * q[a].x; pq.y += pi.y * q[a].y; pq.z += pi.z * q[a].z; pq.w += pi.w * q[a].w;
+            }
+        }
+    }
+    block_accumulate<LPR>(t, kp, pq, red, acc + (A_PQ + (it & 1)) * kp);
+}
+
+// x += (alpha / s) p, r -= alpha A p (FP64 x, r, stencil and sums), r.r
+template <int DPN, int LPR>
+__global__ __launch_bounds__(kB3) void k_g3_update(G3 g, int kp, int npb, int it, const double* __restrict__ coef,
+                                                   const double* __restrict__ S, const float* __restrict__ p,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ acc) {
+    __shared__ double red[kB3 * 4];
+    __shared__ double ca[4 * LPR], cx[4 * LPR];
+    clear_slots(acc, A_PQ + ((it + 1) & 1), 1, kp);
+    const Tile t = tile3<LPR>(npb, g3n(g));
+    if (block_done(t, kp, kp, S)) return;
+    shared_alpha<double, LPR>(it, kp, kp, 0, S, acc, ca);
+    for (int c = threadIdx.x; c < 4 * LPR; c += blockDim.x) {
+        const int col = blockIdx.y * 4 * LPR + c;
+        const double sc = col < kp ? S[S_SCALE * kp + col] : 0.0;
+        cx[c] = sc > 0.0 ? ca[c] / sc : 0.0;
+    }
+    __syncthreads();
+    V4<double> rr = {0.0, 0.0, 0.0, 0.0};
+    if (t.col0 < kp) {
+        const int l4 = t.lane * 4;
+        const V4<double> al = {ca[l4], ca[l4 + 1], ca[l4 + 2], ca[l4 + 3]};
+        const V4<double> ax = {cx[l4], cx[l4 + 1], cx[l4 + 2], cx[l4 + 3]};
+        for (int n = t.r0 + t.grp; n < t.r1; n += t.ngrp) {
+            V4<double> q[DPN];
+            apply_p64<DPN>(g, n, coef, p, kp, t.col0, q);
+#pragma unroll
+            for (int a = 0; a < DPN; ++a) {
+                const size_t o = ((size_t)n * DPN + a) * kp + t.col0;
+                const V4<float> pi = ld4(p + o);
+                V4<double> xv = ld4(x + o), rv = ld4(r + o);
+                xv.x += ax.x * pi.x; xv.y += ax.y * pi.y; xv.z += ax.z * pi.z; xv.w += ax.w * pi.w;
+                rv.x -= al.x * q[a].x; rv.y -= al.y * q[a].y; rv.z -= al.z * q[a].z; rv.w -= al.w * q[a].w;
+                st4(x + o, xv);
+                st4(r + o, rv);
+                rr.x += rv.x * rv.x; rr.y += rv.y * rv.y; rr.z += rv.z * rv.z; rr.w += rv.w * rv.w;
+            }
+        }
+    }
+    block_accumulate<LPR>(t, kp, rr, red, acc + (A_RR + it % 3) * kp);
+}
+
+// -------------------------------------------------------------- V-cycle
+// res = r - A (S r) on free DOFs (zero on constrained ones)
+template <int DPN, int LPR, class TR>
+__global__ __launch_bounds__(kB3) void k_g3_resid(G3 g, int kp, int npb, const float* __restrict__ coef,
+                                                  const float* __restrict__ dsm, const unsigned char* __restrict__ cmask,
+                                                  const TR* __restrict__ r, float* __restrict__ res) {
+    const Tile t = tile3<LPR>(npb, g3n(g));
+    if (t.col0 >= kp) return;
+    for (int n = t.r0 + t.grp; n < t.r1; n += t.ngrp) {
+        const int x = n % g.W, y = (n / g.W) % g.H, z = n / (g.W * g.H);
+        float4 q[DPN];
+#pragma unroll
+        for (int a = 0; a < DPN; ++a) q[a] = ldf4(r + ((size_t)n * DPN + a) * kp + t.col0);
+        for (int k = 0; k < 27; ++k) {
+            const int j = nb27(k, x, y, z, g);
+            if (j < 0) continue;
+            const float* c = coef + ((size_t)n * 27 + k) * DPN * DPN;
+            float4 e[DPN];
+#pragma unroll
+            for (int b = 0; b < DPN; ++b) {
+                const float4 v = ldf4(r + ((size_t)j * DPN + b) * kp + t.col0);
+                const float w = dsm[(size_t)j * DPN + b];
+                e[b] = make_float4(w * v.x, w * v.y, w * v.z, w * v.w);
+            }
+#pragma unroll
+            for (int a = 0; a < DPN; ++a)
+#pragma unroll
+                for (int b = 0; b < DPN; ++b) fma4(-c[a * DPN + b], e[b], q[a]);
+        }
+#pragma unroll
+        for (int a = 0; a < DPN; ++a) {
+            const size_t row = (size_t)n * DPN + a;
+            *reinterpret_cast<float4*>(res + row * kp + t.col0) = (cmask && cmask[row]) ? zero4() : q[a];
+        }
+    }
+}
+
+// r_c = P^T res on coarse nodes
+template <int DPN, int LPR>
+__global__ __launch_bounds__(kB3) void k_g3_restrict(G3 gf, G3 gc, int kp, int npb, const float* __restrict__ res,
+                                                     float* __restrict__ rc) {
+    const Tile t = tile3<LPR>(npb, g3n(gc));
+    if (t.col0 >= kp) return;
+    for (int I = t.r0 + t.grp; I < t.r1; I += t.ngrp) {
+        const int X = I % gc.W, Y = (I / gc.W) % gc.H, Z = I / (gc.W * gc.H);
+        float4 s[DPN];
+#pragma unroll
+        for (int a = 0; a < DPN; ++a) s[a] = zero4();
+        for (int d = 0; d < 27; ++d) {
+            const int fx = 2 * X + d % 3 - 1, fy = 2 * Y + (d / 3) % 3 - 1, fz = 2 * Z + d / 9 - 1;
+            if (fx < 0 || fx >= gf.W || fy < 0 || fy >= gf.H || fz < 0 || fz >= gf.D) continue;
+            const float w = w1(fx, X, gf.W) * w1(fy, Y, gf.H) * w1(fz, Z, gf.D);
+            if (w == 0.f) continue;
+            const int i = (fz * gf.H + fy) * gf.W + fx;
+#pragma unroll
+            for (int a = 0; a < DPN; ++a) fma4(w, ldf4(res + ((size_t)i * DPN + a) * kp + t.col0), s[a]);
+        }
+#pragma unroll
+        for (int a = 0; a < DPN; ++a)
+            *reinterpret_cast<float4*>(rc + ((size_t)I * DPN + a) * kp + t.col0) = s[a];
+    }
+}
+
+// e0 = S r + P e_c on fine nodes (P on free DOFs only on the finest level)
+template <int DPN, int LPR, class TR>
+__global__ __launch_bounds__(kB3) void k_g3_prolong(G3 gf, G3 gc, int kp, int npb, const float* __restrict__ dsm,
+                                                    const unsigned char* __restrict__ cmask,
+                                                    const TR* __restrict__ r, const float* __restrict__ ec,
+                                                    float* __restrict__ e0) {
+    const Tile t = tile3<LPR>(npb, g3n(gf));
+    if (t.col0 >= kp) return;
+    for (int n = t.r0 + t.grp; n < t.r1; n += t.ngrp) {
+        const int x = n % gf.W, y = (n / gf.W) % gf.H, z = n / (gf.W * gf.H);
+        int px, py, pz;
+        float fx, fy, fz;
+        const int nx = parents1(x, gf.W, px, fx), ny = parents1(y, gf.H, py, fy), nz = parents1(z, gf.D, pz, fz);
+        float4 v[DPN];
+#pragma unroll
+        for (int a = 0; a < DPN; ++a) {
+            const float4 rv = ldf4(r + ((size_t)n * DPN + a) * kp + t.col0);
+            const float w = dsm[(size_t)n * DPN + a];
+            v[a] = make_float4(w * rv.x, w * rv.y, w * rv.z, w * rv.w);
+        }
+        for (int uz = 0; uz < nz; ++uz)
+            for (int uy = 0; uy < ny; ++uy)
+                for (int ux = 0; ux < nx; ++ux) {
+                    const int J = ((pz + uz) * gc.H + py + uy) * gc.W + px + ux;
+                    const float w = fx * fy * fz;
+#pragma unroll
+                    for (int a = 0; a < DPN; ++a) fma4(w, ldf4(ec + ((size_t)J * DPN + a) * kp + t.col0), v[a]);
+                }
+#pragma unroll
+        for (int a = 0; a < DPN; ++a) {
+            const size_t row = (size_t)n * DPN + a;
+            *reinterpret_cast<float4*>(e0 + row * kp + t.col0) = (cmask && cmask[row]) ? zero4() : v[a];
+        }
+    }
+}
+
+// e = e0 + S (r - A e0); TOP (finest level): e is zero on constrained DOFs and
+// r.e (mode 0: into the iteration's slot; mode 1: r.e and r.r as per-work-group
+// partial rows) is summed in FP64
+template <int DPN, int LPR, class TR, bool TOP>
+__global__ __launch_bounds__(kB3) void k_g3_smooth(G3 g, int kp, int npb, int it, int mode,
+                                                   const float* __restrict__ coef, const float* __restrict__ dsm,
+                                                   const unsigned char* __restrict__ cmask,
+                                                   const TR* __restrict__ r, const float* __restrict__ e0,
+                                                   float* __restrict__ e, double* __restrict__ acc,
+                                                   double* __restrict__ part0, double* __restrict__ part1) {
+    __shared__ double red[kB3 * 4];
+    const Tile t = tile3<LPR>(npb, g3n(g));
+    V4<double> rz = {0.0, 0.0, 0.0, 0.0}, rr = {0.0, 0.0, 0.0, 0.0};
+    if (t.col0 < kp) {
+        for (int n = t.r0 + t.grp; n < t.r1; n += t.ngrp) {
+            const int x = n % g.W, y = (n / g.W) % g.H, z = n / (g.W * g.H);
+            float4 q[DPN];
+#pragma unroll
+            for (int a = 0; a < DPN; ++a) q[a] = ldf4(r + ((size_t)n * DPN + a) * kp + t.col0);
+            for (int k = 0; k < 27; ++k) {
+                const int j = nb27(k, x, y, z, g);
+                if (j < 0) continue;
+                const float* c = coef + ((size_t)n * 27 + k) * DPN * DPN;
+                float4 v[DPN];
+#pragma unroll
+                for (int b = 0; b < DPN; ++b) v[b] = ldf4(e0 + ((size_t)j * DPN + b) * kp + t.col0);
+#pragma unroll
+                for (int a = 0; a < DPN; ++a)
+#pragma unroll
+                    for (int b = 0; b < DPN; ++b) fma4(-c[a * DPN + b], v[b], q[a]);
+            }
+#pragma unroll
+            for (int a = 0; a < DPN; ++a) {
+                const size_t row = (size_t)n * DPN + a;
+                float4 zz = ldf4(e0 + row * kp + t.col0);
+                fma4(dsm[row], q[a], zz);
+                if (TOP && cmask[row]) zz = zero4();
+                *reinterpret_cast<float4*>(e + row * kp + t.col0) = zz;
+                if constexpr (TOP) {
+                    const V4<double> rd = ld4(r + row * kp + t.col0);
+                    rz.x += rd.x * zz.x; rz.y += rd.y * zz.y; rz.z += rd.z * zz.z; rz.w += rd.w * zz.w;
+                    rr.x += rd.x * rd.x; rr.y += rd.y * rd.y; rr.z += rd.z * rd.z; rr.w += rd.w * rd.w;
+                }
+            }
+        }
+    }
+    if constexpr (TOP) {
+        if (mode == 0) {
+            block_accumulate<LPR>(t, kp, rz, red, acc + (A_RZ + it % 3) * kp);
+        } else {
+            block_partial<LPR>(t, kp, rz, red, part0);
+            __syncthreads();
+            block_partial<LPR>(t, kp, rr, red, part1);
+        }
+    }
+}
+
+inline unsigned blocks8(long long n, int per) {
+    const long long b = (n + per - 1) / per;
+    return (unsigned)((b + 7) / 8 * 8);
+}
+
+// nodes per work-group: 4 per lane group
+inline int npb_of(int lpr) { return 4 * (kB3 / lpr); }
+
+template <int DPN, int LPR>
+struct K3 {
+    static void dir(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, double* S, const float* z,
+                    const float* pold, float* pnew, double* acc) {
+        const G3 g = {m.W[0], m.H[0], m.D[0]};
+        const int nrow = g3n(g) * DPN, rpb = 4 * (kB3 / LPR);
+        k_g3_pdir<LPR><<<dim3(blocks8(nrow, rpb), P.ntile), kB3, 0, st>>>(P.kp, nrow, rpb, it, S, z, pold, pnew, acc);
+        const int npb = npb_of(LPR);
+        k_g3_pq<DPN, LPR><<<dim3(blocks8(g3n(g), npb), P.ntile), kB3, 0, st>>>(g, P.kp, npb, it, m.dia64.as<double>(),
+                                                                               S, pnew, acc);
+    }
+    static void update(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, const double* S, const float* p,
+                       double* x, double* r, double* acc) {
+        const G3 g = {m.W[0], m.H[0], m.D[0]};
+        const int npb = npb_of(LPR);
+        k_g3_update<DPN, LPR><<<dim3(blocks8(g3n(g), npb), P.ntile), kB3, 0, st>>>(g, P.kp, npb, it,
+                                                                                   m.dia64.as<double>(), S, p, x, r,
+                                                                                   acc);
+    }
+    static void vcycle(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, int mode,
+                       const unsigned char* cmask, const double* r0, float* z, float* mgv, double* acc, double* p0,
+                       double* p1) {
+        const int L = m.L, kp = P.kp, npb = npb_of(LPR);
+        auto g = [&](int l) { return G3{m.W[l], m.H[l], m.D[l]}; };
+        auto rv = [&](int l) { return mgv + P.off[l]; };
+        auto ev = [&](int l) { return mgv + P.off[l] + (size_t)g3n(g(l)) * DPN * kp; };
+        auto tv = [&](int l) { return l == 0 ? mgv : mgv + P.off[l] + 2 * (size_t)g3n(g(l)) * DPN * kp; };
+        const dim3 b0(blocks8(g3n(g(0)), npb), P.ntile);
+        k_g3_resid<DPN, LPR, double><<<b0, kB3, 0, st>>>(g(0), kp, npb, m.dia32.as<float>(), m.dsm[0].as<float>(),
+                                                          cmask, r0, tv(0));
+        for (int l = 0; l < L; ++l) {
+            if (l > 0)
+                k_g3_resid<DPN, LPR, float><<<dim3(blocks8(g3n(g(l)), npb), P.ntile), kB3, 0, st>>>(
+                    g(l), kp, npb, m.coef32[l].as<float>(), m.dsm[l].as<float>(), nullptr, rv(l), tv(l));
+            k_g3_restrict<DPN, LPR><<<dim3(blocks8(g3n(g(l + 1)), npb), P.ntile), kB3, 0, st>>>(
+                g(l), g(l + 1), kp, npb, tv(l), rv(l + 1));
+        }
+        k_mg_coarse<<<dim3((kp + kCoarseCols - 1) / kCoarseCols, 1), 256, mg_coarse_lds(m.nc), st>>>(
+            P.d_ctab, kp, m.cinv.as<float>(), mgv);
+        for (int l = L - 1; l >= 0; --l) {
+            const dim3 bl(blocks8(g3n(g(l)), npb), P.ntile);
+            if (l > 0) {
+                k_g3_prolong<DPN, LPR, float><<<bl, kB3, 0, st>>>(g(l), g(l + 1), kp, npb, m.dsm[l].as<float>(),
+                                                                   nullptr, rv(l), ev(l + 1), tv(l));
+                k_g3_smooth<DPN, LPR, float, false><<<bl, kB3, 0, st>>>(g(l), kp, npb, it, mode,
+                                                                         m.coef32[l].as<float>(),
+                                                                         m.dsm[l].as<float>(), nullptr, rv(l), tv(l),
+                                                                         ev(l), nullptr, nullptr, nullptr);
+            } else {
+                k_g3_prolong<DPN, LPR, double><<<bl, kB3, 0, st>>>(g(0), g(1), kp, npb, m.dsm[0].as<float>(), cmask,
+                                                                    r0, ev(1), tv(0));
+                k_g3_smooth<DPN, LPR, double, true><<<bl, kB3, 0, st>>>(g(0), kp, npb, it, mode,
+                                                                         m.dia32.as<float>(), m.dsm[0].as<float>(),
+                                                                         cmask, r0, tv(0), z, acc, p0, p1);
+            }
+        }
+    }
+};
+
+template <int DPN>
+void dispatch_lpr(int lpr, int what, const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, int mode,
+                  double* S, const float* z, const float* pold, float* pnew, double* x, double* r,
+                  const unsigned char* cmask, float* mgv, double* acc, double* p0, double* p1) {
+#define SFH_G3(L)                                                                                \
+    do {                                                                                         \
+        if (what == 0) K3<DPN, L>::dir(P, m, st, it, S, z, pold, pnew, acc);                     \
+        else if (what == 1) K3<DPN, L>::update(P, m, st, it, S, pold, x, r, acc);                \
+        else K3<DPN, L>::vcycle(P, m, st, it, mode, cmask, r, pnew, mgv, acc, p0, p1);           \
+    } while (0)
+    if (lpr == 4) SFH_G3(4);
+    else if (lpr == 8) SFH_G3(8);
+    else SFH_G3(16);
+#undef SFH_G3
+}
+
+void g3_launch(int what, const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, int mode, double* S,
+               const float* z, const float* pold, float* pnew, double* x, double* r, const unsigned char* cmask,
+               float* mgv, double* acc, double* p0, double* p1) {
+    if (m.dpn == 3)
+        dispatch_lpr<3>(P.lpr, what, P, m, st, it, mode, S, z, pold, pnew, x, r, cmask, mgv, acc, p0, p1);
+    else
+        dispatch_lpr<1>(P.lpr, what, P, m, st, it, mode, S, z, pold, pnew, x, r, cmask, mgv, acc, p0, p1);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+bool g3_applies(const sfh_system* s) {
+    return s->dim == 3 && (s->dofpn == 1 || s->dofpn == 3) && s->gn[0] >= 2 && s->gn[1] >= 2 && s->gn[2] >= 2 &&
+           (long long)s->gn[0] * s->gn[1] * s->gn[2] == s->nnodes &&
+           (long long)s->nnodes * s->dofpn > kMGCoarse;
+}
+
+int g3_prepare(sfh_system* s, hipStream_t st) {
+    MG3Hier& m = s->mg3;
+    if (m.built) return SFH_OK;
+    m.L = 0;
+    m.built = true;
+    if (!g3_applies(s)) return SFH_OK;
+    const int dpn = s->dofpn;
+    m.dpn = dpn;
+    m.W[0] = s->gn[0];
+    m.H[0] = s->gn[1];
+    m.D[0] = s->gn[2];
+    const G3 g0 = {m.W[0], m.H[0], m.D[0]};
+    const long long n0 = (long long)g3n(g0), nrow0 = n0 * dpn;
+    const size_t ncoef0 = (size_t)n0 * 27 * dpn * dpn;
+    SFH_TRY(m.dia64.ensure(ncoef0 * 8));
+    SFH_TRY(m.dia32.ensure(ncoef0 * 4));
+    DevBuf err;
+    SFH_TRY(err.ensure(sizeof(int)));
+    SFH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    if (dpn == 3)
+        k_build_dia27<3><<<(unsigned)((nrow0 + 255) / 256), 256, 0, st>>>(g0, s->rowptr.as<int>(), s->colind.as<int>(),
+                                                                        s->vals.as<double>(), m.dia64.as<double>(),
+                                                                        m.dia32.as<float>(), err.as<int>());
+    else
+        k_build_dia27<1><<<(unsigned)((nrow0 + 255) / 256), 256, 0, st>>>(g0, s->rowptr.as<int>(), s->colind.as<int>(),
+                                                                        s->vals.as<double>(), m.dia64.as<double>(),
+                                                                        m.dia32.as<float>(), err.as<int>());
+    SFH_LAUNCH_CHECK();
+    int he = 0;
+    SFH_CHECK_HIP(hipMemcpyAsync(&he, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (he) return SFH_OK;   // not a 27-point operator on this grid
+    SFH_TRY(m.dsm[0].ensure(sizeof(float) * (size_t)nrow0));
+    k_g3_level<<<(unsigned)((nrow0 + 255) / 256), 256, 0, st>>>((int)nrow0, dpn, m.dia64.as<double>(),
+                                                               m.dsm[0].as<float>(), nullptr);
+    SFH_LAUNCH_CHECK();
+    int l = 0;
+    while ((long long)m.W[l] * m.H[l] * m.D[l] * dpn > kMGCoarse && l < kMGMaxLev &&
+           (m.W[l] > 2 || m.H[l] > 2 || m.D[l] > 2)) {
+        const G3 gf = {m.W[l], m.H[l], m.D[l]};
+        const G3 gc = {(gf.W + 1) / 2, (gf.H + 1) / 2, (gf.D + 1) / 2};
+        m.W[l + 1] = gc.W;
+        m.H[l + 1] = gc.H;
+        m.D[l + 1] = gc.D;
+        const long long nrc = (long long)g3n(gc) * dpn;
+        SFH_TRY(m.coef64[l + 1].ensure((size_t)g3n(gc) * 27 * dpn * dpn * 8));
+        SFH_TRY(m.coef32[l + 1].ensure((size_t)g3n(gc) * 27 * dpn * dpn * 4));
+        SFH_TRY(m.dsm[l + 1].ensure(sizeof(float) * (size_t)nrc));
+        const double* Af = l == 0 ? m.dia64.as<double>() : m.coef64[l].as<double>();
+        const unsigned char* cm = l == 0 ? s->cmask.as<unsigned char>() : nullptr;
+        if (dpn == 3)
+            k_g3_galerkin<3><<<(unsigned)((nrc + 63) / 64), 64, 0, st>>>(gf, gc, Af, cm, m.coef64[l + 1].as<double>());
+        else
+            k_g3_galerkin<1><<<(unsigned)((nrc + 63) / 64), 64, 0, st>>>(gf, gc, Af, cm, m.coef64[l + 1].as<double>());
+        SFH_LAUNCH_CHECK();
+        k_g3_level<<<(unsigned)((nrc + 255) / 256), 256, 0, st>>>((int)nrc, dpn, m.coef64[l + 1].as<double>(),
+                                                                 m.dsm[l + 1].as<float>(), m.coef32[l + 1].as<float>());
+        SFH_LAUNCH_CHECK();
+        ++l;
+    }
+    if (l == 0) return SFH_OK;
+    const G3 gL = {m.W[l], m.H[l], m.D[l]};
+    if ((long long)g3n(gL) * dpn > kMGCoarse) return SFH_OK;   // coarsest inverse must fit the LDS-staged solve
+    std::vector<double> a((size_t)g3n(gL) * 27 * dpn * dpn);
+    SFH_CHECK_HIP(hipMemcpyAsync(a.data(), m.coef64[l].p, a.size() * 8, hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    std::vector<float> inv;
+    if (!coarse_inverse3(gL, dpn, a, inv)) return SFH_OK;
+    SFH_TRY(m.cinv.ensure(inv.size() * 4));
+    SFH_CHECK_HIP(hipMemcpyAsync(m.cinv.p, inv.data(), inv.size() * 4, hipMemcpyHostToDevice, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    m.nc = g3n(gL) * dpn;
+    m.L = l;
+    return SFH_OK;
+}
+
+void g3_plan(G3Plan& P, const MG3Hier& m, int kp, int lpr) {
+    P.kp = kp;
+    P.lpr = lpr;
+    P.ntile = (kp + 4 * lpr - 1) / (4 * lpr);
+    size_t off = (size_t)m.W[0] * m.H[0] * m.D[0] * m.dpn * kp;   // level-0 temporary
+    for (int l = 1; l <= m.L; ++l) {
+        P.off[l] = off;
+        off += 3 * (size_t)m.W[l] * m.H[l] * m.D[l] * m.dpn * kp;   // r_l, e_l, t_l
+    }
+    P.total = off;
+    const int L = m.L;
+    P.ctab[0] = m.nc;
+    P.ctab[1] = 0;
+    P.ctab[2] = (int)P.off[L];
+    P.ctab[3] = (int)(P.off[L] + (size_t)m.W[L] * m.H[L] * m.D[L] * m.dpn * kp);
+    P.nsmooth = (int)blocks8((long long)m.W[0] * m.H[0] * m.D[0], npb_of(lpr));
+}
+
+void g3_dir(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, double* S, const float* z, const float* pold,
+            float* pnew, double* acc) {
+    g3_launch(0, P, m, st, it, 0, S, z, pold, pnew, nullptr, nullptr, nullptr, nullptr, acc, nullptr, nullptr);
+}
+
+void g3_update(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, const double* S, const float* p, double* x,
+               double* r, double* acc) {
+    g3_launch(1, P, m, st, it, 0, const_cast<double*>(S), nullptr, p, nullptr, x, r, nullptr, nullptr, acc, nullptr,
+              nullptr);
+}
+
+void g3_vcycle(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, int mode, const unsigned char* cmask,
+               const double* r, float* z, float* mgv, double* acc, double* p0, double* p1) {
+    g3_launch(2, P, m, st, it, mode, nullptr, nullptr, nullptr, z, nullptr, const_cast<double*>(r), cmask, mgv, acc,
+              p0, p1);
+}
+
+}  // namespace sfh

@@ -26,248 +26,10 @@
 // The iteration kernels' dot products are added into per-column FP64
 // accumulators with hardware atomics (one per column and work-group); the
 // refinement-time reductions are fixed-order two-stage sums.
-#include "common.hpp"
+#include "pcg_dev.hpp"
 
 namespace sfh {
 namespace {
-
-constexpr int kB = 1024;   // 16 waves per work-group: 16 waves per SIMD at 1024 work-groups
-constexpr int kRedB = 1024;   // reduction blocks: 16 row groups x 64 columns
-constexpr int kU = 3;         // gathered neighbours in flight per lane in the direction kernel (9-point rows = 3 x 3)
-
-template <class T> struct V4 { T x, y, z, w; };
-
-__device__ inline V4<float> ld4(const float* p) {
-    const float4 v = *reinterpret_cast<const float4*>(p);
-    return {v.x, v.y, v.z, v.w};
-}
-__device__ inline void st4(float* p, const V4<float>& v) {
-    *reinterpret_cast<float4*>(p) = make_float4(v.x, v.y, v.z, v.w);
-}
-__device__ inline V4<double> ld4(const double* p) {
-    const double2 a = *reinterpret_cast<const double2*>(p);
-    const double2 b = *reinterpret_cast<const double2*>(p + 2);
-    return {a.x, a.y, b.x, b.y};
-}
-__device__ inline void st4(double* p, const V4<double>& v) {
-    *reinterpret_cast<double2*>(p) = make_double2(v.x, v.y);
-    *reinterpret_cast<double2*>(p + 2) = make_double2(v.z, v.w);
-}
-
-// logical block id: XCD x (hardware round-robin b % 8) gets a contiguous range
-__device__ inline int xcd_block(int b, int nb) {
-    if (nb % 8) return b;
-    return (b % 8) * (nb / 8) + b / 8;
-}
-
-struct Tile {
-    int lane, grp, ngrp, col0, r0, r1, bid, so;
-};
-
-// Row partition of a (segmented) solve: work-group b (logical id) owns rows
-// [brow[b], brow[b+1]) of segment bseg[b]; a segment is one independent
-// problem of a batch, with its own per-column scalars at offset seg * kp of
-// each scalar slot (slot stride ks = nseg * kp).
-struct Seg {
-    const int* brow;
-    const int* bseg;
-    int ks;
-};
-
-template <int LPR>
-__device__ inline Tile tile(int kp, const Seg& g) {
-    Tile t;
-    t.lane = threadIdx.x % LPR;
-    t.grp = threadIdx.x / LPR;
-    t.ngrp = blockDim.x / LPR;
-    t.col0 = (blockIdx.y * LPR + t.lane) * 4;
-    t.bid = xcd_block(blockIdx.x, gridDim.x);
-    t.r0 = g.brow[t.bid];
-    t.r1 = g.brow[t.bid + 1];
-    t.so = g.bseg[t.bid] * kp;
-    return t;
-}
-
-// block-level sum over row groups of per-lane 4-column partials -> part[bid*kp + col]
-template <int LPR>
-__device__ inline void block_partial(const Tile& t, int kp, const V4<double>& a, double* red,
-                                     double* part) {
-    const int w = 4 * LPR;
-    red[t.grp * w + t.lane * 4 + 0] = a.x;
-    red[t.grp * w + t.lane * 4 + 1] = a.y;
-    red[t.grp * w + t.lane * 4 + 2] = a.z;
-    red[t.grp * w + t.lane * 4 + 3] = a.w;
-    __syncthreads();
-    for (int c = threadIdx.x; c < w; c += blockDim.x) {
-        double s = 0.0;
-        for (int g = 0; g < t.ngrp; ++g) s += red[g * w + c];
-        const int col = blockIdx.y * w + c;
-        if (col < kp) part[(size_t)t.bid * kp + col] = s;
-    }
-}
-
-// per-RHS scalar slots (each kp doubles)
-enum { S_RZ = 0, S_RZPREV, S_FROZEN, S_BETA, S_TRIG, S_TGT, S_ACTIVE, S_SCALE, S_RATIO, S_RN2, S_BN2,
-       S_NSLOT };
-
-// Per-column FP64 accumulators of the iteration kernels (ks doubles each),
-// rotating with the iteration index j of the current refinement cycle:
-// p.Ap of iteration j in A_PQ + j % 2, r.z and r.r after iteration j in
-// A_RZ + j % 3 and A_RR + j % 3. A slot is cleared by work-group 0 of the
-// kernel launched before the one that accumulates into it, and no work-group of
-// that launch reads it.
-enum { A_PQ = 0, A_RZ = 2, A_RR = 5, A_N = 8 };
-
-// r.z that the step length of iteration j uses
-__device__ inline double rz_at(int j, int ks, int c, const double* __restrict__ S, const double* __restrict__ acc) {
-    return j == 0 ? S[S_RZ * ks + c] : acc[(A_RZ + (j - 1) % 3) * ks + c];
-}
-
-// Direction coefficients of column c at iteration j: p = m z + b p_old with
-// b = beta, m = 1 for a live column and b = 1, m = 0 for a frozen or inactive
-// one. A live column whose recursive residual r.r reached its trigger, or whose
-// r.z is no longer positive (the FP32 recurrence has reached its rounding
-// floor), is frozen here (until the next FP64 replacement); every work-group
-// takes the same decision from the same inputs, so the flag and the r.z of the
-// column's last step are written with the same values by all of them.
-template <class T>
-__device__ inline void dir_coef(int j, int ks, int c, double* S, const double* __restrict__ acc, T& b, T& m) {
-    const bool act = S[S_ACTIVE * ks + c] > 0.0;
-    bool frozen = false;
-    double beta = S[S_BETA * ks + c];
-    if (j > 0 && act) {
-        frozen = S[S_FROZEN * ks + c] > 0.0;
-        if (!frozen && (!(acc[(A_RR + (j - 1) % 3) * ks + c] > S[S_TRIG * ks + c]) ||
-                        !(rz_at(j, ks, c, S, acc) > 0.0))) {
-            frozen = true;
-            S[S_FROZEN * ks + c] = 1.0;
-            S[S_RZPREV * ks + c] = rz_at(j - 1, ks, c, S, acc);
-        }
-        const double rzo = rz_at(j - 1, ks, c, S, acc);
-        beta = rzo > 0.0 ? rz_at(j, ks, c, S, acc) / rzo : 0.0;
-    }
-    const bool live = act && !frozen;
-    b = live ? (T)beta : (T)1;
-    m = live ? (T)1 : (T)0;
-}
-
-// Per-column direction coefficients (b, m) or step lengths of the work-group's
-// 4*LPR columns, computed once per work-group into LDS and read by every row
-// group (the inputs are a handful of scattered scalars per column).
-template <class T, int LPR>
-__device__ inline void shared_dir_coef(int it, int kp, int ks, int so, double* S, const double* __restrict__ acc,
-                                       T* cb, T* cm) {
-    const int w = 4 * LPR;
-    for (int c = threadIdx.x; c < w; c += blockDim.x) {
-        const int col = blockIdx.y * w + c;
-        T b = (T)1, m = (T)0;
-        if (col < kp) dir_coef<T>(it, ks, so + col, S, acc, b, m);
-        cb[c] = b;
-        cm[c] = m;
-    }
-    __syncthreads();
-}
-
-template <class T, int LPR>
-__device__ inline void shared_alpha(int it, int kp, int ks, int so, const double* __restrict__ S,
-                                    const double* __restrict__ acc, T* ca) {
-    const int w = 4 * LPR;
-    for (int c = threadIdx.x; c < w; c += blockDim.x) {
-        const int col = blockIdx.y * w + c;
-        double a = 0.0;
-        if (col < kp) {
-            const int k = so + col;
-            const double pq = acc[(A_PQ + (it & 1)) * ks + k];
-            const bool live = S[S_ACTIVE * ks + k] > 0.0 && !(S[S_FROZEN * ks + k] > 0.0);
-            a = (live && pq > 0.0) ? rz_at(it, ks, k, S, acc) / pq : 0.0;
-        }
-        ca[c] = (T)a;
-    }
-    __syncthreads();
-}
-
-// work-group 0 clears `nslot` consecutive accumulator slots starting at slot a
-__device__ inline void clear_slots(double* acc, int a, int nslot, int ks) {
-    if (blockIdx.x != 0 || blockIdx.y != 0) return;
-    for (int i = threadIdx.x; i < nslot * ks; i += blockDim.x) acc[(size_t)a * ks + i] = 0.0;
-}
-
-// block-level sum over row groups of per-lane 4-column partials, added into
-// acc[col] with one hardware FP64 atomic per column and block (the order of the
-// additions across blocks is not fixed)
-template <int LPR>
-__device__ inline void block_accumulate(const Tile& t, int kp, const V4<double>& a, double* red, double* acc) {
-    const int w = 4 * LPR;
-    red[t.grp * w + t.lane * 4 + 0] = a.x;
-    red[t.grp * w + t.lane * 4 + 1] = a.y;
-    red[t.grp * w + t.lane * 4 + 2] = a.z;
-    red[t.grp * w + t.lane * 4 + 3] = a.w;
-    __syncthreads();
-    for (int c = threadIdx.x; c < w; c += blockDim.x) {
-        double s = 0.0;
-        for (int g = 0; g < t.ngrp; ++g) s += red[g * w + c];
-        const int col = blockIdx.y * w + c;
-        if (col < kp) unsafeAtomicAdd(acc + col, s);
-    }
-}
-
-// true when none of the columns the work-group handles is still being solved
-// (converged, zero or padding columns have scale 0): the whole work-group then
-// skips the iteration. Frozen columns (scale > 0) keep the work-group live.
-__device__ inline bool block_done(const Tile& t, int kp, int ks, const double* __restrict__ S) {
-    bool live = false;
-    if (t.col0 < kp) {
-        const double* sc = S + S_SCALE * ks + t.so + t.col0;
-        live = sc[0] > 0.0 || sc[1] > 0.0 || sc[2] > 0.0 || sc[3] > 0.0;
-    }
-    return !__syncthreads_or(live);
-}
-
-// row index of the lane group; wave-uniform (scalar) when one row spans the wave
-template <int LPR>
-__device__ inline int uniform_row(int r) {
-    if (LPR == 64) return __builtin_amdgcn_readfirstlane(r);
-    return r;
-}
-
-// operator entries of one CSR row: FP32 as packed (column, value bits) pairs
-// (one 8-byte load per entry), FP64 from the separate column / value arrays
-template <class T> struct Entries;
-template <> struct Entries<float> {
-    const int2* cv;
-    const int* ci;
-    const float* va;
-    __device__ inline void get(int e, int& c, float& a) const {
-        const int2 v = cv[e];
-        c = v.x;
-        a = __int_as_float(v.y);
-    }
-};
-template <> struct Entries<double> {
-    const int2* cv;
-    const int* ci;
-    const double* va;
-    __device__ inline void get(int e, int& c, double& a) const {
-        c = ci[e];
-        a = va[e];
-    }
-};
-
-// preconditioned direction entry p = m z + b p_old (m = 0 freezes a column),
-// evaluated identically wherever it is needed (fused into the SpMV gathers)
-template <class T>
-__device__ inline void pdir4(const V4<T>& z, const V4<T>& m, const V4<T>& b, const V4<T>& po, V4<T>& out) {
-    out.x = b.x * po.x + m.x * z.x; out.y = b.y * po.y + m.y * z.y;
-    out.z = b.z * po.z + m.z * z.z; out.w = b.w * po.w + m.w * z.w;
-}
-
-// r.z and r.r of one row from z = D^-1 r (r = g z, g = A_ii)
-template <class T>
-__device__ inline void res_dots(const V4<T>& z, T g, V4<T>& rz, V4<T>& rr) {
-    const V4<T> r = {g * z.x, g * z.y, g * z.z, g * z.w};
-    rz.x += r.x * z.x; rz.y += r.y * z.y; rz.z += r.z * z.z; rz.w += r.w * z.w;
-    rr.x += r.x * r.x; rr.y += r.y * r.y; rr.z += r.z * r.z; rr.w += r.w * r.w;
-}
 
 // K1: p_new = z + beta p_old (z = D^-1 r is what the solver stores; every row,
 // recomputed on the fly at the neighbours the stencil touches), q = A p_new,
@@ -790,16 +552,7 @@ __device__ inline float mg_w(int x, int X, int n) {
     return ((x - 1) / 2 == X || (x + 1) / 2 == X) ? 0.5f : 0.f;
 }
 
-__device__ inline void fma4(float a, const float4& v, float4& s) {
-    s.x += a * v.x; s.y += a * v.y; s.z += a * v.z; s.w += a * v.w;
-}
 
-__device__ inline float4 ldf4(const float* p) { return *reinterpret_cast<const float4*>(p); }
-__device__ inline float4 ldf4(const double* p) {
-    const V4<double> v = ld4(p);
-    return make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
-}
-__device__ inline float4 zero4() { return make_float4(0.f, 0.f, 0.f, 0.f); }
 
 // The staging loops below issue all of a thread's loads of a phase before
 // using any of them: out-of-grid nodes and idle lanes load from node 0 /
@@ -1031,41 +784,6 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
             __syncthreads();
             block_partial<LPR>(t, kp, rr, red, part1);
         }
-    }
-}
-
-// e = C^-1 r on the coarsest level of every grid (dense FP32 inverses,
-// row-major): work-group (column group x, grid y) stages C^-1 and its 16
-// residual columns in LDS. Table per grid: nodes, C^-1 offset, first float of
-// the grid's residual and correction in the coarse-level buffer.
-constexpr int kCoarseCols = 16;
-inline size_t mg_coarse_lds(int n) {
-    return sizeof(float) * (((size_t)n * n + 3) / 4 * 4) + sizeof(float4) * (size_t)n * (kCoarseCols / 4);
-}
-__global__ __launch_bounds__(256) void k_mg_coarse(const int* __restrict__ ctab, int kp, const float* __restrict__ cinv,
-                                                   float* __restrict__ mgv) {
-    extern __shared__ float4 sm[];
-    const int* d = ctab + 4 * blockIdx.y;
-    const int n = d[0];
-    const float* ci_g = cinv + d[1];
-    const float* r = mgv + d[2];
-    float* e = mgv + d[3];
-    float* ci = reinterpret_cast<float*>(sm);
-    float4* rs = sm + ((size_t)n * n + 3) / 4;
-    constexpr int nq = kCoarseCols / 4;
-    const int c0 = blockIdx.x * kCoarseCols;
-    for (int i = threadIdx.x; i < n * n; i += blockDim.x) ci[i] = ci_g[i];
-    for (int i = threadIdx.x; i < n * nq; i += blockDim.x) {
-        const int j = i / nq, col = c0 + 4 * (i % nq);
-        rs[i] = col < kp ? *reinterpret_cast<const float4*>(r + (size_t)j * kp + col) : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    __syncthreads();
-    for (int o = threadIdx.x; o < n * nq; o += blockDim.x) {
-        const int i = o / nq, q = o % nq, col = c0 + 4 * q;
-        if (col >= kp) continue;
-        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int j = 0; j < n; ++j) fma4(ci[i * n + j], rs[j * nq + q], s);
-        *reinterpret_cast<float4*>(e + (size_t)i * kp + col) = s;
     }
 }
 
@@ -1751,6 +1469,7 @@ std::vector<int> grids_of(const sfh_system* s, int nseg, const std::vector<int>&
 int prepare_operator_on(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
     const int n = s->ndof;
     if (s->dirty) {
+        s->mg3.built = false;
         k_dinv<<<(n + 255) / 256, 256, 0, st>>>(n, s->diagpos.as<int>(), s->vals.as<double>(), s->dinv32.as<float>(),
                                                 s->dinv64.as<double>(), s->diag32.as<float>(), s->diag64.as<double>());
         SFH_LAUNCH_CHECK();
@@ -1816,8 +1535,18 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                           P.lpr >= 4 && P.lpr <= 16;
     SFH_TRY(prepare_operator_on(s, st, use_grid ? grids : std::vector<int>()));
     if (use_grid && s->dia_state == 1) plan_grid(P, grids);
+    // declared 3D grid: 27-point block stencil with its multigrid hierarchy
+    // (FP32 inner type, one problem)
+    bool use_g3 = false;
+    G3Plan P3;
+    if (!P.grid && ns == 1 && std::is_same<T, float>::value && o.precond != 1 && (o.layout == 0 || o.layout == 4) &&
+        P.lpr >= 4 && P.lpr <= 16 && g3_applies(s)) {
+        SFH_TRY(g3_prepare(s, st));
+        use_g3 = s->mg3.L > 0;
+        if (use_g3) g3_plan(P3, s->mg3, kp, P.lpr);
+    }
     // vector DOFs of a mesh: 3 x 3 block layout (one problem, FP32)
-    if (!P.grid && ns == 1 && s->dofpn == 3 && s->kind != 0 && (o.layout == 3 || o.layout == 0) &&
+    if (!P.grid && !use_g3 && ns == 1 && s->dofpn == 3 && s->kind != 0 && (o.layout == 3 || o.layout == 0) &&
         std::is_same<T, float>::value && P.ntile == 1)
         plan_bsr3(P, s->nnodes, s->ctx->num_cu);
     // multigrid preconditioner: stencil layout of one grid with a hierarchy
@@ -1827,7 +1556,9 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     for (int q = 0; use_mg && q < mg.nseg; ++q)
         use_mg = mg.gbase[q] == grids[3 * q] && mg.gW[q] == grids[3 * q + 1] && mg.gH[q] == grids[3 * q + 2];
     if (use_mg) plan_mg(P, mg);
-    const int nbi = std::max(P.grid ? P.gblk : P.nblk, use_mg ? P.mg_pn[0] : 0);   // partial rows per launch
+    const bool fp64_rec = use_mg || use_g3;   // multigrid: FP64 iterate and residual
+    const int nbi = std::max(std::max(P.grid ? P.gblk : P.nblk, use_mg ? P.mg_pn[0] : 0),
+                             use_g3 ? P3.nsmooth : 0);   // partial rows per launch
     SFH_TRY(w.b64.ensure(nk * 8));
     SFH_TRY(w.x64.ensure(nk * 8));
     SFH_TRY(w.r64.ensure(nk * 8));
@@ -1840,10 +1571,13 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_TRY(w.scal.ensure((size_t)S_NSLOT * ks * 8));
     SFH_TRY(w.acc.ensure((size_t)A_N * ks * 8));
     if (use_mg) SFH_TRY(w.mgv.ensure(P.mgv_total * sizeof(T)));
+    if (use_g3) SFH_TRY(w.mgv.ensure(P3.total * sizeof(float)));
     // partition tables: brow (nblk + 1), bseg (nblk), sb (nseg + 1), srow (nseg + 1), gtab, multigrid tiles,
     // and the partial-row bounds {0, level-0 prolongation tiles} of a multigrid residual replacement
+    // (3D multigrid: its coarse-solve table and the partial-row bounds {0, smoothing work-groups})
+    const int sb3[2] = {0, use_g3 ? P3.nsmooth : 0};
     const size_t ntab = P.brow.size() + P.bseg.size() + P.sb.size() + P.srow.size() + P.gtab.size() +
-                        P.mgtab.size() + P.mgctab.size() + P.mgsb.size();
+                        P.mgtab.size() + P.mgctab.size() + P.mgsb.size() + 4 + 2;
     SFH_TRY(w.tab.ensure(sizeof(int) * ntab));
     {
         std::vector<int> h;
@@ -1856,6 +1590,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         h.insert(h.end(), P.mgtab.begin(), P.mgtab.end());
         h.insert(h.end(), P.mgctab.begin(), P.mgctab.end());
         h.insert(h.end(), P.mgsb.begin(), P.mgsb.end());
+        h.insert(h.end(), P3.ctab, P3.ctab + 4);
+        h.insert(h.end(), sb3, sb3 + 2);
         SFH_CHECK_HIP(hipMemcpyAsync(w.tab.p, h.data(), sizeof(int) * ntab, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipStreamSynchronize(st));
     }
@@ -1867,6 +1603,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     P.d_mgtab = P.d_gtab + P.gtab.size();
     P.d_mgctab = P.d_mgtab + P.mgtab.size();
     const int* d_sbmg = P.d_mgctab + P.mgctab.size();
+    P3.d_ctab = d_sbmg + P.mgsb.size();
+    const int* d_sb3 = P3.d_ctab + 4;
     const Seg G = {d_brow, d_bseg, ks};
     const dim3 gred((kp + 63) / 64, ns);
     double* S = w.scal.as<double>();
@@ -1900,7 +1638,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                                                           w.b64.as<double>());
     SFH_LAUNCH_CHECK();
     SFH_CHECK_HIP(hipMemsetAsync(w.x64.p, 0, nk * 8, st));
-    if (!use_mg) SFH_CHECK_HIP(hipMemsetAsync(w.x32.p, 0, nk * sizeof(T), st));
+    if (!fp64_rec) SFH_CHECK_HIP(hipMemsetAsync(w.x32.p, 0, nk * sizeof(T), st));
     SFH_CHECK_HIP(hipMemsetAsync(w.p32.p, 0, nk * sizeof(T), st));
     SFH_CHECK_HIP(hipMemsetAsync(w.scal.p, 0, (size_t)S_NSLOT * ks * 8, st));
     SFH_CHECK_HIP(hipMemsetAsync(w.acc.p, 0, (size_t)A_N * ks * 8, st));
@@ -1921,12 +1659,12 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     // iteration reduces the residual by about an order of magnitude, a Jacobi
     // one by far less
     // (the multigrid recurrences are FP64: one cycle down to the target)
-    const double inner = (o.inner_tol > 0.0 && o.inner_tol < 1.0) ? o.inner_tol : (use_mg ? 1e-14 : 0.01);
+    const double inner = (o.inner_tol > 0.0 && o.inner_tol < 1.0) ? o.inner_tol : (fp64_rec ? 1e-14 : 0.01);
     int updates = 0, iters = 0;
     double maxrel = 0.0;
     bool done = false;
     std::vector<double> hs(5 * (size_t)ks);   // ACTIVE, FROZEN, SCALE, TRIG and the latest r.r per column
-    const int check = o.check_every > 0 ? o.check_every : (use_mg ? 2 : 32);
+    const int check = o.check_every > 0 ? o.check_every : (fp64_rec ? 2 : 32);
     int jlast = -1;                           // last iteration of the previous cycle
     std::vector<double> rrchk(ks);            // r.r of every column at the previous check of the cycle
     // per-column FP32 reduction target of a cycle: starts at `inner`; a column
@@ -1964,7 +1702,13 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_SCALE * ks, scale.data(), ks * 8, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_RATIO * ks, ratio.data(), ks * 8, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * ks, tgt.data(), ks * 8, hipMemcpyHostToDevice, st));
-        if (use_mg) {
+        if (use_g3) {
+            k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, 1, d_srow,
+                                                                      S + S_SCALE * ks, S + S_RATIO * ks, rmg,
+                                                                      (float*)pa);
+            g3_vcycle(P3, s->mg3, st, 0, 1, cmask, rmg, (float*)z, (float*)w.mgv.p, acc, p0, p1);
+            k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb3, p0, p1, S, acc, 0.0, jlast);
+        } else if (use_mg) {
             k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
                                                                       S + S_SCALE * ks, S + S_RATIO * ks, rmg,
                                                                       (float*)pa);
@@ -1984,7 +1728,9 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         for (;; ++j) {
             const bool smp = ismp < nsmp && iters % cx->ksample == 0;
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp], st));
-            if (use_mg)
+            if (use_g3)
+                g3_dir(P3, s->mg3, st, j, S, (const float*)z, (const float*)pa, (float*)pb, acc);
+            else if (use_mg)
                 Kern<T, LPR>::dir_spmv_mg(P, st, j, coef64, S, z, pa, pb, acc);
             else if (P.grid)
                 Kern<T, LPR>::dir_spmv_g(P, st, j, coef, S, z, pa, pb, acc);
@@ -1994,7 +1740,9 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             else
                 Kern<T, LPR>::dir_spmv(P, G, st, j, rp, M, S, z, pa, pb, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 1], st));
-            if (use_mg)
+            if (use_g3)
+                g3_update(P3, s->mg3, st, j, S, (const float*)pb, w.x64.as<double>(), rmg, acc);
+            else if (use_mg)
                 Kern<T, LPR>::update_mg(P, st, j, coef64, S, pb, w.x64.as<double>(), rmg, acc);
             else if (P.grid)
                 Kern<T, LPR>::update_g(P, st, j, coef, dinv, diag, S, pb, x, z, acc);
@@ -2005,6 +1753,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                 Kern<T, LPR>::update(P, G, st, j, rp, M, dinv, diag, S, pb, x, z, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 2], st));
             if (use_mg) Kern<T, LPR>::vcycle(P, mg, st, j, 0, coef, cmask, rmg, z, mgv, acc, p0, p1);
+            if (use_g3) g3_vcycle(P3, s->mg3, st, j, 0, cmask, rmg, (float*)z, (float*)w.mgv.p, acc, p0, p1);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 3], st));
             ismp += smp;
             std::swap(pa, pb);
@@ -2043,7 +1792,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         }
         jlast = j;
         // flush the FP32 correction, true residual in FP64
-        if (!use_mg)   // (the multigrid iteration updates x64 itself)
+        if (!fp64_rec)   // (the multigrid iterations update x64 themselves)
             k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
                                                                      S + S_SCALE * ks, x, w.x64.as<double>());
         Kern<T, LPR>::resid(P, G, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
@@ -2076,9 +1825,9 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         stats->max_relres = maxrel;
         stats->ms = ms;
         stats->converged = done ? 1 : 0;
-        stats->layout = P.grid ? 2 : (P.bsr ? 3 : 1);
-        stats->precond = use_mg ? 2 : 1;
-        stats->levels = use_mg ? P.mgL + 1 : 1;
+        stats->layout = use_g3 ? 4 : (P.grid ? 2 : (P.bsr ? 3 : 1));
+        stats->precond = fp64_rec ? 2 : 1;
+        stats->levels = use_mg ? P.mgL + 1 : (use_g3 ? s->mg3.L + 1 : 1);
     }
     if (!done) {
         set_error("solve did not reach tol %.3g (max rel. residual %.3g after %d iterations, %d replacements)",
@@ -2122,7 +1871,7 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
     if (o.max_outer <= 0) o.max_outer = 500;
     if (o.max_inner <= 0) o.max_inner = 50000;
     if (o.check_every < 0) o.check_every = 0;
-    if (o.layout < 0 || o.layout > 3) o.layout = 0;
+    if (o.layout < 0 || o.layout > 4) o.layout = 0;
     if (o.precond < 0 || o.precond > 2) o.precond = 0;
     // column tiles: each tile is an independent multi-RHS solve whose working
     // set can stay in the last-level cache
@@ -2164,6 +1913,10 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
         if (!g.empty()) max_lpr = 16;
         if (g.size() == 3 && o.precond != 1 && k < 16 && (long long)g[1] * g[2] > kMGCoarse) min_kp = 16;
     }
+    if (o.precision == 0 && (o.layout == 0 || o.layout == 4) && nseg == 1 && g3_applies(s)) {
+        max_lpr = 16;
+        if (o.precond != 1 && k < 16) min_kp = 16;
+    }
     const Plan P = make_plan(s->ndof, k, s->ctx->num_cu, nseg, seg_rows, min_kp, max_lpr);
     if (o.precision == 1) return dispatch_solve<double>(s, P, trans, k, B, ldb, X, ldx, o, stats);
     return dispatch_solve<float>(s, P, trans, k, B, ldb, X, ldx, o, stats);
@@ -2173,6 +1926,7 @@ int prepare_operator(sfh_system* s, int nseg, const int* seg_rows) {
     std::vector<int> srow(nseg + 1);
     for (int q = 0; q <= nseg; ++q) srow[q] = seg_rows ? seg_rows[q] : (q == 0 ? 0 : s->ndof);
     SFH_TRY(prepare_operator_on(s, s->ctx->stream, grids_of(s, nseg, srow)));
+    if (nseg == 1 && g3_applies(s)) SFH_TRY(g3_prepare(s, s->ctx->stream));
     SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
     return SFH_OK;
 }
