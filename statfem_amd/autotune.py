@@ -14,14 +14,15 @@ from . import hip
 
 def tune_rhs_tile(system, B, k, trans=False, seg=None, candidates=(16, 32, 64, 128, 256), probe_iters=64,
                   repeats=2):
-    """B: device block (N x k). Returns (best tile, {tile: ms per column-iteration})."""
+    """B: device block (N x k). Returns (best tile, 0 for one whole-width solve;
+    {width: ms per column-iteration})."""
     ctx = system.ctx
     X = ctx.empty((system.ndof, k))
     cost = {}
-    for t in candidates:
-        if t > k and t != candidates[0]:
-            continue
-        kt = min(t, k)
+    # widths at or above k all mean one whole-width solve: probe it once
+    cands = sorted({min(t, k) for t in candidates})
+    for t in cands:
+        kt = t
         best = np.inf
         for _ in range(repeats):
             o = hip.default_opts(rhs_tile=0, max_inner=probe_iters, check_every=probe_iters)

@@ -560,11 +560,25 @@ __device__ inline float mg_w(int x, int X, int n) {
 // branch and the memory latencies of a phase overlap.
 
 // r_{l+1} = P^T (r - A (S r)) on one tile of coarse nodes
+// work-group skip of the V-cycle kernels: none of the work-group's columns of
+// its problem is still being solved (scale 0: converged, zero or padding
+// columns of a narrower problem of a batch); S == NULL never skips
+__device__ inline bool mg_cols_done(const double* __restrict__ S, int ks, int so, int col0, bool ok) {
+    if (!S) return false;
+    bool live = false;
+    if (ok) {
+        const double* sc = S + S_SCALE * ks + so + col0;
+        live = sc[0] > 0.0 || sc[1] > 0.0 || sc[2] > 0.0 || sc[3] > 0.0;
+    }
+    return !__syncthreads_or(live);
+}
+
 template <int LPR, class TR>
 __global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab, int kp, const float* __restrict__ coef,
                                                     const float* __restrict__ dsm,
                                                     const unsigned char* __restrict__ cmask,
-                                                    const TR* __restrict__ r, float* __restrict__ rc) {
+                                                    const TR* __restrict__ r, float* __restrict__ rc,
+                                                    const double* __restrict__ S, int ks) {
     extern __shared__ float4 sm[];
     float* sd = reinterpret_cast<float*>(sm + kRW * kRH * LPR);
     constexpr int ngrp = kG / LPR;
@@ -577,6 +591,7 @@ __global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab,
     if (W == 0) return;
     const int col0 = (blockIdx.y * LPR + lane) * 4;   // column tile blockIdx.y of 4 * LPR columns
     const bool ok = col0 < kp;
+    if (mg_cols_done(S, ks, d[8] * kp, col0, ok)) return;
     const int lc = ok ? col0 : 0;
     const int fx0 = 2 * cx0 - 2, fy0 = 2 * cy0 - 2;
     {
@@ -674,7 +689,8 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
                                                    const unsigned char* __restrict__ cmask,
                                                    const TR* __restrict__ r, const float* __restrict__ ec,
                                                    float* __restrict__ e, double* __restrict__ acc,
-                                                   double* __restrict__ part0, double* __restrict__ part1) {
+                                                   double* __restrict__ part0, double* __restrict__ part1,
+                                                   const double* __restrict__ S) {
     extern __shared__ float4 sm[];
     float4* sc = sm + kPW * kPH * LPR;
     constexpr int ngrp = kG / LPR;
@@ -692,6 +708,7 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
     t.so = d[8] * kp;
     if (W == 0 && !(TOP && mode == 1)) return;
     const bool ok = t.col0 < kp;
+    if (mg_cols_done(S, ks, t.so, t.col0, ok)) return;   // (S == NULL for residual replacements)
     const int lc = ok ? t.col0 : 0;
     V4<double> rz = {0.0, 0.0, 0.0, 0.0}, rr = {0.0, 0.0, 0.0, 0.0};
     if (W > 0) {
@@ -1389,7 +1406,7 @@ struct Kern {
     // r.r as per-tile partial rows (mode 1)
     static void vcycle(const Plan& P, const MGHier& m, hipStream_t st, int it, int mode, const T* coef0,
                        const unsigned char* cmask, const double* r0, T* z, T* mgv, double* acc, double* p0,
-                       double* p1) {
+                       double* p1, const double* S) {
         if constexpr (std::is_same<T, float>::value && LPR >= 4 && LPR <= 16) {
             // restriction: 32 columns per work-group (half the LDS of a
             // 64-column tile, so four work-groups share a CU)
@@ -1397,21 +1414,23 @@ struct Kern {
             const int L = P.mgL, kp = P.kp, ntm = (kp + 4 * ML - 1) / (4 * ML);
             auto rvec = [&](int l) -> float* { return mgv + P.mgv_off[l]; };
             auto evec = [&](int l) -> float* { return l == 0 ? z : mgv + P.mgv_off[l] + (size_t)m.N[l] * kp; };
+            const double* Sk = mode == 0 ? S : nullptr;   // work-group skips only inside an iteration
             k_mg_restrict<ML, double><<<dim3(P.mg_rn[0], ntm), kG, mg_restrict_lds(ML), st>>>(
-                P.d_mgtab + (size_t)kMT * P.mg_roff[0], kp, coef0, m.dsm[0].as<float>(), cmask, r0, rvec(1));
+                P.d_mgtab + (size_t)kMT * P.mg_roff[0], kp, coef0, m.dsm[0].as<float>(), cmask, r0, rvec(1), Sk,
+                P.ks);
             for (int l = 1; l < L; ++l)
                 k_mg_restrict<ML, float><<<dim3(P.mg_rn[l], ntm), kG, mg_restrict_lds(ML), st>>>(
                     P.d_mgtab + (size_t)kMT * P.mg_roff[l], kp, m.coef32[l].as<float>(), m.dsm[l].as<float>(),
-                    nullptr, rvec(l), rvec(l + 1));
+                    nullptr, rvec(l), rvec(l + 1), Sk, P.ks);
             k_mg_coarse<<<dim3((kp + kCoarseCols - 1) / kCoarseCols, m.nseg), 256, mg_coarse_lds(m.cmax), st>>>(
                 P.d_mgctab, kp, m.cinv.as<float>(), mgv);
             for (int l = L - 1; l >= 1; --l)
                 k_mg_prolong<LPR, false, float><<<dim3(P.mg_pn[l], P.ntile), kG, mg_prolong_lds(LPR), st>>>(
                     P.d_mgtab + (size_t)kMT * P.mg_poff[l], kp, P.ks, it, mode, m.coef32[l].as<float>(),
-                    m.dsm[l].as<float>(), nullptr, rvec(l), evec(l + 1), evec(l), nullptr, nullptr, nullptr);
+                    m.dsm[l].as<float>(), nullptr, rvec(l), evec(l + 1), evec(l), nullptr, nullptr, nullptr, Sk);
             k_mg_prolong<LPR, true, double><<<dim3(P.mg_pn[0], P.ntile), kG, mg_prolong_lds(LPR), st>>>(
                 P.d_mgtab + (size_t)kMT * P.mg_poff[0], kp, P.ks, it, mode, coef0, m.dsm[0].as<float>(), cmask, r0,
-                evec(1), z, acc, p0, p1);
+                evec(1), z, acc, p0, p1, Sk);
         }
     }
     static void dir_spmv_b(const Plan& P, hipStream_t st, int it, const int* rpn, const int* cin, const float* bval,
@@ -1712,7 +1731,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
                                                                       S + S_SCALE * ks, S + S_RATIO * ks, rmg,
                                                                       (float*)pa);
-            Kern<T, LPR>::vcycle(P, mg, st, 0, 1, coef, cmask, rmg, z, mgv, acc, p0, p1);
+            Kern<T, LPR>::vcycle(P, mg, st, 0, 1, coef, cmask, rmg, z, mgv, acc, p0, p1, S);
             k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sbmg, p0, p1, S, acc, 0.0, jlast);
         } else {
             Kern<T, LPR>::rescale(P, G, st, S + S_SCALE * ks, S + S_RATIO * ks, s->dinv64.as<double>(), diag,
@@ -1752,7 +1771,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             else
                 Kern<T, LPR>::update(P, G, st, j, rp, M, dinv, diag, S, pb, x, z, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 2], st));
-            if (use_mg) Kern<T, LPR>::vcycle(P, mg, st, j, 0, coef, cmask, rmg, z, mgv, acc, p0, p1);
+            if (use_mg) Kern<T, LPR>::vcycle(P, mg, st, j, 0, coef, cmask, rmg, z, mgv, acc, p0, p1, S);
             if (use_g3) g3_vcycle(P3, s->mg3, st, j, 0, cmask, rmg, (float*)z, (float*)w.mgv.p, acc, p0, p1);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 3], st));
             ismp += smp;
