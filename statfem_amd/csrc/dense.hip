@@ -19,7 +19,7 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int TM = 64, TN = 64, TK = 16;
+constexpr int TM = 64, TN = 128, TK = 16;
 
 // C[z] (=|+=) op(A) B[z]. op(A) = A (M x K, lda) or, with TRANSA, X^T for X K x M (lda).
 // z selects a batch (B, C strides) and/or a K chunk [z*kstep, z*kstep + kz).
@@ -28,13 +28,19 @@ constexpr int TM = 64, TN = 64, TK = 16;
 // for a K-chunked Gram product, their A panel), and the XCD remap below puts
 // consecutive logical tiles on the same XCD, so a panel is fetched from HBM
 // into that XCD's L2 once rather than by every XCD.
+//
+// A 64 x 128 tile per work-group: each of the 4 waves owns 32 x 64 of C as
+// 2 x 4 blocks of v_mfma_f64_16x16x4f64. The K loop is double-buffered: the
+// next 16-deep panels of A and B are loaded into registers while the current
+// ones are multiplied from LDS, then stored to the other LDS buffer (one
+// barrier per step). Interior tiles load 16-byte pairs.
 template <bool TRANSA>
 __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
                                                const double* __restrict__ B, int ldb, long long strideB,
                                                double* __restrict__ C, int ldc, long long strideC,
                                                int kz, int kstep, int nM, int nN, int nZ) {
-    __shared__ double As[TM][TK + 1];
-    __shared__ double Bs[TK][TN + 2];
+    __shared__ double As[2][TM][TK + 1];
+    __shared__ double Bs[2][TK][TN + 2];
     const int nwg = gridDim.x;
     const int lid = (nwg % 8) ? (int)blockIdx.x : ((int)blockIdx.x % 8) * (nwg / 8) + (int)blockIdx.x / 8;
     if (lid >= nM * nN * nZ) return;
@@ -46,57 +52,134 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double
     const int m0 = mt * TM, n0 = nt * TN;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w >> 1, wn = w & 1;
-    v4d acc[2][2];
+    // 16-byte loads when every row of the tile's panels is in range and aligned
+    const bool vec = m0 + TM <= M && n0 + TN <= N && ((lda | ldb) & 1) == 0 &&
+                     (((uintptr_t)A | (uintptr_t)Bz) & 15) == 0;
+    v4d acc[2][4];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 4; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+    // register staging of one K panel: A 64 x 16 (4 per thread), B 16 x 128 (8 per thread)
+    double ra[4], rb[8];
+    auto load = [&](int k0) {
+        if (vec && k0 + TK <= kend) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int idx = tid + q * 256;   // 512 pairs
+                double2 v;
+                if (TRANSA) {   // A^T panel: X rows k, columns m (pairs along m)
+                    const int c = idx / (TM / 2), r2 = idx % (TM / 2);
+                    v = *reinterpret_cast<const double2*>(A + (size_t)(k0 + c) * lda + m0 + 2 * r2);
+                } else {        // rows m, pairs along k
+                    const int r = idx / (TK / 2), c2 = idx % (TK / 2);
+                    v = *reinterpret_cast<const double2*>(A + (size_t)(m0 + r) * lda + k0 + 2 * c2);
+                }
+                ra[2 * q] = v.x;
+                ra[2 * q + 1] = v.y;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int idx = tid + q * 256;   // 1024 pairs
+                const int r = idx / (TN / 2), c2 = idx % (TN / 2);
+                const double2 v = *reinterpret_cast<const double2*>(Bz + (size_t)(k0 + r) * ldb + n0 + 2 * c2);
+                rb[2 * q] = v.x;
+                rb[2 * q + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int idx = tid + q * 256;
+                int r, c;
+                if (TRANSA) { c = idx / TM; r = idx % TM; }   // coalesced along M
+                else { r = idx / TK; c = idx % TK; }
+                const int gm = m0 + r, gk = k0 + c;
+                double v = 0.0;
+                if (gm < M && gk < kend) v = TRANSA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+                ra[q] = v;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int idx = tid + q * 256;
+                const int r = idx / TN, c = idx % TN;
+                const int gk = k0 + r, gn = n0 + c;
+                rb[q] = (gk < kend && gn < N) ? Bz[(size_t)gk * ldb + gn] : 0.0;
+            }
+        }
+    };
+    auto store = [&](int buf, int k0) {
+        if (vec && k0 + TK <= kend) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int idx = tid + q * 256;
+                if (TRANSA) {
+                    const int c = idx / (TM / 2), r2 = idx % (TM / 2);
+                    As[buf][2 * r2][c] = ra[2 * q];
+                    As[buf][2 * r2 + 1][c] = ra[2 * q + 1];
+                } else {
+                    const int r = idx / (TK / 2), c2 = idx % (TK / 2);
+                    As[buf][r][2 * c2] = ra[2 * q];
+                    As[buf][r][2 * c2 + 1] = ra[2 * q + 1];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int idx = tid + q * 256;
+                const int r = idx / (TN / 2), c2 = idx % (TN / 2);
+                Bs[buf][r][2 * c2] = rb[2 * q];
+                Bs[buf][r][2 * c2 + 1] = rb[2 * q + 1];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int idx = tid + q * 256;
+                int r, c;
+                if (TRANSA) { c = idx / TM; r = idx % TM; }
+                else { r = idx / TK; c = idx % TK; }
+                As[buf][r][c] = ra[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int idx = tid + q * 256;
+                Bs[buf][idx / TN][idx % TN] = rb[q];
+            }
+        }
+    };
+    int buf = 0;
+    if (kbeg < kend) {
+        load(kbeg);
+        store(0, kbeg);
+    }
+    __syncthreads();
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
-        // A tile TM x TK
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int idx = tid + q * 256;
-            int r, c;
-            if (TRANSA) { c = idx / TM; r = idx % TM; }   // coalesced along M
-            else { r = idx / TK; c = idx % TK; }
-            const int gm = m0 + r, gk = k0 + c;
-            double v = 0.0;
-            if (gm < M && gk < kend) v = TRANSA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-            As[r][c] = v;
-        }
-        // B tile TK x TN
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int idx = tid + q * 256;
-            const int r = idx / TN, c = idx % TN;
-            const int gk = k0 + r, gn = n0 + c;
-            Bs[r][c] = (gk < kend && gn < N) ? Bz[(size_t)gk * ldb + gn] : 0.0;
-        }
-        __syncthreads();
+        const bool more = k0 + TK < kend;
+        if (more) load(k0 + TK);   // in flight while the current panel is multiplied
 #pragma unroll
         for (int kk = 0; kk < TK / 4; ++kk) {
             const int kr = kk * 4 + (lane >> 4);
+            double a[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const double a = As[wm * 32 + i * 16 + (lane & 15)][kr];
+            for (int i = 0; i < 2; ++i) a[i] = As[buf][wm * 32 + i * 16 + (lane & 15)][kr];
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const double b = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i][j], 0, 0, 0);
-                }
+            for (int j = 0; j < 4; ++j) {
+                const double b = Bs[buf][kr][wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b, acc[i][j], 0, 0, 0);
             }
         }
+        if (more) store(buf ^ 1, k0 + TK);
         __syncthreads();
+        buf ^= 1;
     }
     // v_mfma_f64_16x16x4f64 result: register r of lane l holds D[l/16 + 4r][l%16]
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gm = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
-                const int gn = n0 + wn * 32 + j * 16 + (lane & 15);
+                const int gn = n0 + wn * 64 + j * 16 + (lane & 15);
                 if (gm < M && gn < N) Cz[(size_t)gm * ldc + gn] = acc[i][j][r];
             }
 }
