@@ -226,9 +226,9 @@ void g3_plan(G3Plan& P, const MG3Hier& m, int kp, int lpr);
 void g3_dir(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, double* S, const float* z, const float* pold,
             float* pnew, double* acc);
 void g3_update(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, const double* S, const float* p, double* x,
-               double* r, double* acc);
+               double* r, float* rf, double* acc);
 void g3_vcycle(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, int mode, const unsigned char* cmask,
-               const double* r, float* z, float* mgv, double* acc, double* p0, double* p1);
+               const float* r, float* z, float* mgv, double* acc, double* p0, double* p1);
 int triplets_build(sfh_system* s, long long n, const int* rows, const int* cols, const double* vals);
 int triplets_values(sfh_system* s, long long n, const double* vals);
 int covforcing_grid(sfh_ctx* c, int d, const int gn[3], const double* const ax[3], const double* cv, double sigma,

@@ -285,12 +285,13 @@ __global__ __launch_bounds__(kB3) void k_g3_pq(G3 g, int kp, int npb, int it, co
     block_accumulate<LPR>(t, kp, pq, red, acc + (A_PQ + (it & 1)) * kp);
 }
 
-// x += (alpha / s) p, r -= alpha A p (FP64 x, r, stencil and sums), r.r
+// x += (alpha / s) p, r -= alpha A p (FP64 x, r, stencil and sums), r.r; rf is
+// the FP32 copy of r that the V-cycle reads
 template <int DPN, int LPR>
 __global__ __launch_bounds__(kB3) void k_g3_update(G3 g, int kp, int npb, int it, const double* __restrict__ coef,
                                                    const double* __restrict__ S, const float* __restrict__ p,
                                                    double* __restrict__ x, double* __restrict__ r,
-                                                   double* __restrict__ acc) {
+                                                   float* __restrict__ rf, double* __restrict__ acc) {
     __shared__ double red[kB3 * 4];
     __shared__ double ca[4 * LPR], cx[4 * LPR];
     clear_slots(acc, A_PQ + ((it + 1) & 1), 1, kp);
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(kB3) void k_g3_update(G3 g, int kp, int npb, int it
                 rv.x -= al.x * q[a].x; rv.y -= al.y * q[a].y; rv.z -= al.z * q[a].z; rv.w -= al.w * q[a].w;
                 st4(x + o, xv);
                 st4(r + o, rv);
+                *reinterpret_cast<float4*>(rf + o) = make_float4((float)rv.x, (float)rv.y, (float)rv.z, (float)rv.w);
                 rr.x += rv.x * rv.x; rr.y += rv.y * rv.y; rr.z += rv.z * rv.z; rr.w += rv.w * rv.w;
             }
         }
@@ -465,9 +467,11 @@ __global__ __launch_bounds__(kB3) void k_g3_smooth(G3 g, int kp, int npb, int it
                 if (TOP && cmask[row]) zz = zero4();
                 *reinterpret_cast<float4*>(e + row * kp + t.col0) = zz;
                 if constexpr (TOP) {
-                    const V4<double> rd = ld4(r + row * kp + t.col0);
-                    rz.x += rd.x * zz.x; rz.y += rd.y * zz.y; rz.z += rd.z * zz.z; rz.w += rd.w * zz.w;
-                    rr.x += rd.x * rd.x; rr.y += rd.y * rd.y; rr.z += rd.z * rd.z; rr.w += rd.w * rd.w;
+                    const float4 rd = ldf4(r + row * kp + t.col0);
+                    rz.x += (double)rd.x * zz.x; rz.y += (double)rd.y * zz.y; rz.z += (double)rd.z * zz.z;
+                    rz.w += (double)rd.w * zz.w;
+                    rr.x += (double)rd.x * rd.x; rr.y += (double)rd.y * rd.y; rr.z += (double)rd.z * rd.z;
+                    rr.w += (double)rd.w * rd.w;
                 }
             }
         }
@@ -503,15 +507,15 @@ struct K3 {
                                                                                S, pnew, acc);
     }
     static void update(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, const double* S, const float* p,
-                       double* x, double* r, double* acc) {
+                       double* x, double* r, float* rf, double* acc) {
         const G3 g = {m.W[0], m.H[0], m.D[0]};
         const int npb = npb_of(LPR);
         k_g3_update<DPN, LPR><<<dim3(blocks8(g3n(g), npb), P.ntile), kB3, 0, st>>>(g, P.kp, npb, it,
                                                                                    m.dia64.as<double>(), S, p, x, r,
-                                                                                   acc);
+                                                                                   rf, acc);
     }
     static void vcycle(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, int mode,
-                       const unsigned char* cmask, const double* r0, float* z, float* mgv, double* acc, double* p0,
+                       const unsigned char* cmask, const float* r0, float* z, float* mgv, double* acc, double* p0,
                        double* p1) {
         const int L = m.L, kp = P.kp, npb = npb_of(LPR);
         auto g = [&](int l) { return G3{m.W[l], m.H[l], m.D[l]}; };
@@ -519,7 +523,7 @@ struct K3 {
         auto ev = [&](int l) { return mgv + P.off[l] + (size_t)g3n(g(l)) * DPN * kp; };
         auto tv = [&](int l) { return l == 0 ? mgv : mgv + P.off[l] + 2 * (size_t)g3n(g(l)) * DPN * kp; };
         const dim3 b0(blocks8(g3n(g(0)), npb), P.ntile);
-        k_g3_resid<DPN, LPR, double><<<b0, kB3, 0, st>>>(g(0), kp, npb, m.dia32.as<float>(), m.dsm[0].as<float>(),
+        k_g3_resid<DPN, LPR, float><<<b0, kB3, 0, st>>>(g(0), kp, npb, m.dia32.as<float>(), m.dsm[0].as<float>(),
                                                           cmask, r0, tv(0));
         for (int l = 0; l < L; ++l) {
             if (l > 0)
@@ -540,9 +544,9 @@ struct K3 {
                                                                          m.dsm[l].as<float>(), nullptr, rv(l), tv(l),
                                                                          ev(l), nullptr, nullptr, nullptr);
             } else {
-                k_g3_prolong<DPN, LPR, double><<<bl, kB3, 0, st>>>(g(0), g(1), kp, npb, m.dsm[0].as<float>(), cmask,
+                k_g3_prolong<DPN, LPR, float><<<bl, kB3, 0, st>>>(g(0), g(1), kp, npb, m.dsm[0].as<float>(), cmask,
                                                                     r0, ev(1), tv(0));
-                k_g3_smooth<DPN, LPR, double, true><<<bl, kB3, 0, st>>>(g(0), kp, npb, it, mode,
+                k_g3_smooth<DPN, LPR, float, true><<<bl, kB3, 0, st>>>(g(0), kp, npb, it, mode,
                                                                          m.dia32.as<float>(), m.dsm[0].as<float>(),
                                                                          cmask, r0, tv(0), z, acc, p0, p1);
             }
@@ -552,13 +556,13 @@ struct K3 {
 
 template <int DPN>
 void dispatch_lpr(int lpr, int what, const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, int mode,
-                  double* S, const float* z, const float* pold, float* pnew, double* x, double* r,
+                  double* S, const float* z, const float* pold, float* pnew, double* x, double* r, float* rf,
                   const unsigned char* cmask, float* mgv, double* acc, double* p0, double* p1) {
 #define SFH_G3(L)                                                                                \
     do {                                                                                         \
         if (what == 0) K3<DPN, L>::dir(P, m, st, it, S, z, pold, pnew, acc);                     \
-        else if (what == 1) K3<DPN, L>::update(P, m, st, it, S, pold, x, r, acc);                \
-        else K3<DPN, L>::vcycle(P, m, st, it, mode, cmask, r, pnew, mgv, acc, p0, p1);           \
+        else if (what == 1) K3<DPN, L>::update(P, m, st, it, S, pold, x, r, rf, acc);            \
+        else K3<DPN, L>::vcycle(P, m, st, it, mode, cmask, rf, pnew, mgv, acc, p0, p1);          \
     } while (0)
     if (lpr == 4) SFH_G3(4);
     else if (lpr == 8) SFH_G3(8);
@@ -567,12 +571,12 @@ void dispatch_lpr(int lpr, int what, const G3Plan& P, const MG3Hier& m, hipStrea
 }
 
 void g3_launch(int what, const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, int mode, double* S,
-               const float* z, const float* pold, float* pnew, double* x, double* r, const unsigned char* cmask,
-               float* mgv, double* acc, double* p0, double* p1) {
+               const float* z, const float* pold, float* pnew, double* x, double* r, float* rf,
+               const unsigned char* cmask, float* mgv, double* acc, double* p0, double* p1) {
     if (m.dpn == 3)
-        dispatch_lpr<3>(P.lpr, what, P, m, st, it, mode, S, z, pold, pnew, x, r, cmask, mgv, acc, p0, p1);
+        dispatch_lpr<3>(P.lpr, what, P, m, st, it, mode, S, z, pold, pnew, x, r, rf, cmask, mgv, acc, p0, p1);
     else
-        dispatch_lpr<1>(P.lpr, what, P, m, st, it, mode, S, z, pold, pnew, x, r, cmask, mgv, acc, p0, p1);
+        dispatch_lpr<1>(P.lpr, what, P, m, st, it, mode, S, z, pold, pnew, x, r, rf, cmask, mgv, acc, p0, p1);
 }
 
 }  // namespace
@@ -680,19 +684,20 @@ void g3_plan(G3Plan& P, const MG3Hier& m, int kp, int lpr) {
 
 void g3_dir(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, double* S, const float* z, const float* pold,
             float* pnew, double* acc) {
-    g3_launch(0, P, m, st, it, 0, S, z, pold, pnew, nullptr, nullptr, nullptr, nullptr, acc, nullptr, nullptr);
-}
-
-void g3_update(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, const double* S, const float* p, double* x,
-               double* r, double* acc) {
-    g3_launch(1, P, m, st, it, 0, const_cast<double*>(S), nullptr, p, nullptr, x, r, nullptr, nullptr, acc, nullptr,
+    g3_launch(0, P, m, st, it, 0, S, z, pold, pnew, nullptr, nullptr, nullptr, nullptr, nullptr, acc, nullptr,
               nullptr);
 }
 
+void g3_update(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, const double* S, const float* p, double* x,
+               double* r, float* rf, double* acc) {
+    g3_launch(1, P, m, st, it, 0, const_cast<double*>(S), nullptr, p, nullptr, x, r, rf, nullptr, nullptr, acc,
+              nullptr, nullptr);
+}
+
 void g3_vcycle(const G3Plan& P, const MG3Hier& m, hipStream_t st, int it, int mode, const unsigned char* cmask,
-               const double* r, float* z, float* mgv, double* acc, double* p0, double* p1) {
-    g3_launch(2, P, m, st, it, mode, nullptr, nullptr, nullptr, z, nullptr, const_cast<double*>(r), cmask, mgv, acc,
-              p0, p1);
+               const float* r, float* z, float* mgv, double* acc, double* p0, double* p1) {
+    g3_launch(2, P, m, st, it, mode, nullptr, nullptr, nullptr, z, nullptr, nullptr, const_cast<float*>(r), cmask,
+              mgv, acc, p0, p1);
 }
 
 }  // namespace sfh

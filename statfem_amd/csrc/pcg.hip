@@ -805,11 +805,11 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
 }
 
 // residual replacement with the multigrid preconditioner: r *= s' in place
-// (FP64), p *= s'/s; the row's problem (srow: nseg + 1 row bounds) selects the
-// scalars
+// (FP64) and, when rf is given, its FP32 copy, p *= s'/s; the row's problem
+// (srow: nseg + 1 row bounds) selects the scalars
 __global__ void k_mg_rescale(long long total, int kp, int nseg, const int* __restrict__ srow,
                              const double* __restrict__ scale, const double* __restrict__ ratio,
-                             double* __restrict__ r, float* __restrict__ p) {
+                             double* __restrict__ r, float* __restrict__ rf, float* __restrict__ p) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const int row = (int)(i / kp);
@@ -819,7 +819,9 @@ __global__ void k_mg_rescale(long long total, int kp, int nseg, const int* __res
         if (srow[mid] <= row) lo = mid; else hi = mid - 1;
     }
     const size_t c = (size_t)lo * kp + i % kp;
-    r[i] *= scale[c];
+    const double v = r[i] * scale[c];
+    r[i] = v;
+    if (rf) rf[i] = (float)v;
     p[i] *= (float)ratio[c];
 }
 
@@ -1640,6 +1642,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     // multigrid: the iteration updates the FP64 iterate itself and, in place
     // of the true residual after each replacement, the FP64 recursive residual
     double* rmg = w.r64.as<double>();
+    // 3D: the FP32 copy of r that the gather-bound 3D V-cycle reads (x32 is free here)
+    float* rfm = use_g3 ? w.x32.as<float>() : nullptr;
     T* mgv = use_mg ? w.mgv.as<T>() : nullptr;
     const double* coef64 = s->dia64.as<double>();
     const unsigned char* cmask = s->cmask.as<unsigned char>();
@@ -1723,13 +1727,13 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * ks, tgt.data(), ks * 8, hipMemcpyHostToDevice, st));
         if (use_g3) {
             k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, 1, d_srow,
-                                                                      S + S_SCALE * ks, S + S_RATIO * ks, rmg,
+                                                                      S + S_SCALE * ks, S + S_RATIO * ks, rmg, rfm,
                                                                       (float*)pa);
-            g3_vcycle(P3, s->mg3, st, 0, 1, cmask, rmg, (float*)z, (float*)w.mgv.p, acc, p0, p1);
+            g3_vcycle(P3, s->mg3, st, 0, 1, cmask, rfm, (float*)z, (float*)w.mgv.p, acc, p0, p1);
             k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb3, p0, p1, S, acc, 0.0, jlast);
         } else if (use_mg) {
             k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
-                                                                      S + S_SCALE * ks, S + S_RATIO * ks, rmg,
+                                                                      S + S_SCALE * ks, S + S_RATIO * ks, rmg, nullptr,
                                                                       (float*)pa);
             Kern<T, LPR>::vcycle(P, mg, st, 0, 1, coef, cmask, rmg, z, mgv, acc, p0, p1, S);
             k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sbmg, p0, p1, S, acc, 0.0, jlast);
@@ -1760,7 +1764,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                 Kern<T, LPR>::dir_spmv(P, G, st, j, rp, M, S, z, pa, pb, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 1], st));
             if (use_g3)
-                g3_update(P3, s->mg3, st, j, S, (const float*)pb, w.x64.as<double>(), rmg, acc);
+                g3_update(P3, s->mg3, st, j, S, (const float*)pb, w.x64.as<double>(), rmg, rfm, acc);
             else if (use_mg)
                 Kern<T, LPR>::update_mg(P, st, j, coef64, S, pb, w.x64.as<double>(), rmg, acc);
             else if (P.grid)
@@ -1772,7 +1776,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                 Kern<T, LPR>::update(P, G, st, j, rp, M, dinv, diag, S, pb, x, z, acc);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 2], st));
             if (use_mg) Kern<T, LPR>::vcycle(P, mg, st, j, 0, coef, cmask, rmg, z, mgv, acc, p0, p1, S);
-            if (use_g3) g3_vcycle(P3, s->mg3, st, j, 0, cmask, rmg, (float*)z, (float*)w.mgv.p, acc, p0, p1);
+            if (use_g3) g3_vcycle(P3, s->mg3, st, j, 0, cmask, rfm, (float*)z, (float*)w.mgv.p, acc, p0, p1);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp + 3], st));
             ismp += smp;
             std::swap(pa, pb);
