@@ -143,3 +143,25 @@ def test_wide_solve_runs_column_tiles_side_by_side(ctx, orc, precond):
     assert st["layout"] == 2 and st["precond"] == precond and st["converged"] == 1, st
     assert err.max() < 1e-8, (err.max(), st)
     assert np.all(Xg[:, 140] == 0.0)
+
+
+def test_multigrid_on_a_graded_grid(ctx, orc):
+    """Non-uniform axes (geometric grading, cells 12x apart in width, so
+    anisotropic near one side): the Galerkin hierarchy still yields a
+    convergent preconditioner and the oracle's solution."""
+    from statfem_amd import hip
+    g = M.grid2d(48, 40)
+    xs = np.cumsum(np.r_[0.0, 1.055 ** np.arange(48)])
+    xs /= xs[-1]
+    ys = g.grid[1]
+    X, Y = np.meshgrid(xs, ys)
+    m = M.Mesh(M.QUAD4, np.stack([X.ravel(), Y.ravel()], axis=1), g.conn, grid=(xs, ys))
+    bc = M.dirichlet_all_boundary(m)
+    kap = np.exp(0.5 * np.random.default_rng(8).normal(size=m.nel))
+    rp, ci, va, _ = orc.assemble(m, orc.POISSON, kap, nq1d=2, bc_dofs=bc)
+    s = _system(ctx, m, kap, 2, bc)
+    B = np.random.default_rng(9).normal(size=(m.nnodes, 24))
+    Xd, st = s.solve(B, trans=True, opts=hip.default_opts(tol=1e-12))
+    Xr = orc.LU(rp, ci, va).solve(B, trans=True)
+    assert st["precond"] == 2 and st["converged"] == 1, st
+    assert np.abs(Xd.download() - Xr).max() <= 1e-8 * np.abs(Xr).max(), st

@@ -84,3 +84,16 @@ def test_scalar_3d_multigrid_matches_direct(ctx, orc, physics):
         err = np.linalg.norm(Xd.download() - Xr, axis=0) / np.linalg.norm(Xr, axis=0)
         assert st["layout"] == 4 and st["precond"] == 2 and st["converged"] == 1, st
         assert err.max() < 1e-8, (err.max(), st)
+
+
+def test_elasticity_multigrid_on_anisotropic_cells(ctx, orc):
+    """Cells 4 : 1 : 1 (a 24 x 6 x 6 hex grid of the unit cube): slower
+    smoothing, same answer as the direct solve."""
+    from statfem_amd import hip
+    m, E, bc, rp, ci, va = _elastic(orc, 24, 6, 6, seed=7)
+    s = _elastic_system(ctx, m, E, bc)
+    B = np.random.default_rng(8).normal(size=(3 * m.nnodes, 8))
+    X, st = s.solve(B, opts=hip.default_opts(tol=1e-11))
+    Xr = orc.LU(rp, ci, va).solve(B)
+    assert st["layout"] == 4 and st["converged"] == 1, st
+    assert np.abs(X.download() - Xr).max() <= 1e-8 * np.abs(Xr).max(), st
