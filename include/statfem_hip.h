@@ -246,6 +246,16 @@ int sfh_gram_segmented(sfh_ctx* ctx, int nseg, const int* seg_rows, int m, int n
 int sfh_covforcing_grid(sfh_ctx* ctx, int nx, int ny, int nz, const double* gx, const double* gy,
                         const double* gz, const double* covvec_dev, double sigma, double ell, int k,
                         const double* W_dev, double* V_dev);
+/* The same product for every problem of a batch of 2D grids in one call (the
+ * per-problem loop of CovarianceForcing over the problems of a union system,
+ * statX/infer/CovarianceForcing.ipp:37-109): problem q owns rows
+ * seg_rows[q] .. seg_rows[q+1]-1 of W_dev / V_dev / covvec_dev (row-major,
+ * k columns), an nx[q] x ny[q] grid whose axes follow each other in axes
+ * (host: nx[q] x coordinates, then ny[q] y coordinates, problem after
+ * problem), and its own sigma[q], ell[q] (host). */
+int sfh_covforcing_batched(sfh_ctx* ctx, int nseg, const int* seg_rows, const int* nx, const int* ny,
+                           const double* axes, const double* covvec_dev, const double* sigma,
+                           const double* ell, int k, const double* W_dev, double* V_dev);
 /* ForwardGaussRHS::updateSolution (statX/infer/ForwardGaussRHS.hpp:95-139) for a
  * sparse projection P given row-wise (ns rows of pn (col, val) pairs):
  *   R^T = A^-T P^T, mean = R F, Cu = R C_f R^T + (sigma_f reg_eps)^2 P P^T.

@@ -15,13 +15,6 @@ from . import hip
 from . import mesh as M
 
 
-class _View:
-    """Device pointer at an element offset into a DeviceArray."""
-
-    def __init__(self, arr, offset):
-        self.ptr = C.c_void_p(arr.ptr.value + int(offset) * arr.dtype.itemsize)
-
-
 class BatchedForwardGauss:
     def __init__(self, ctx, problems, nq1d=2):
         self.ctx = ctx
@@ -50,6 +43,13 @@ class BatchedForwardGauss:
                 np.add.at(Pt[:, j], p["cols"][j] + off[q], p["vals"][j])
         self.Pt = ctx.to_device(Pt)
         self.F = ctx.to_device(s.rhs().reshape(-1, 1))
+        # per-problem forcing covariance: grid shapes, axes (x then y, problem
+        # after problem), sigma_f and ell_f, for one batched C_f call
+        self.nx = np.array([len(p["mesh"].grid[0]) for p in problems], np.int32)
+        self.ny = np.array([len(p["mesh"].grid[1]) for p in problems], np.int32)
+        self.axes = np.concatenate([np.r_[p["mesh"].grid[0], p["mesh"].grid[1]] for p in problems]).astype(np.float64)
+        self.sig = np.array([p["sigma_f"] for p in problems], np.float64)
+        self.ell = np.array([p["ell_f"] for p in problems], np.float64)
         self.PPt = [_sparse_ppt(p["cols"], p["vals"]) for p in problems]
 
     def updateSolution(self, opts=None):
@@ -62,13 +62,9 @@ class BatchedForwardGauss:
         hip._check(hip.lib().sfh_solve_batched(self.sys.h, self.nseg, hip._p(self.seg), 1, k, self.Pt.ptr, Rt.ptr,
                                                C.byref(o), C.byref(st)))
         V = ctx.empty((N, k))
-        for q, p in enumerate(self.probs):
-            r0 = int(self.seg[q])
-            ax = hip._h(p["mesh"].grid[0])
-            ay = hip._h(p["mesh"].grid[1])
-            hip._check(hip.lib().sfh_covforcing_grid(ctx.h, len(ax), len(ay), 1, hip._p(ax), hip._p(ay), None,
-                                                     _View(self.cv, r0).ptr, p["sigma_f"], p["ell_f"], k,
-                                                     _View(Rt, r0 * k).ptr, _View(V, r0 * k).ptr))
+        hip._check(hip.lib().sfh_covforcing_batched(ctx.h, self.nseg, hip._p(self.seg), hip._p(self.nx),
+                                                    hip._p(self.ny), hip._p(self.axes), self.cv.ptr,
+                                                    hip._p(self.sig), hip._p(self.ell), k, Rt.ptr, V.ptr))
         G = ctx.empty((self.nseg, k, k))
         hip._check(hip.lib().sfh_gram_segmented(ctx.h, self.nseg, hip._p(self.seg), k, k, Rt.ptr, k, V.ptr, k, G.ptr))
         Mn = ctx.empty((self.nseg, k, 1))

@@ -545,6 +545,39 @@ int sfh_covforcing_grid(sfh_ctx* c, int nx, int ny, int nz, const double* gx, co
     return SFH_OK;
 }
 
+int sfh_covforcing_batched(sfh_ctx* c, int nseg, const int* seg_rows, const int* nx, const int* ny,
+                           const double* axes, const double* cv, const double* sigma, const double* ell, int k,
+                           const double* W, double* V) {
+    if (!c || nseg <= 0 || !seg_rows || !nx || !ny || !axes || !cv || !sigma || !ell || k <= 0 || !W || !V) {
+        set_error("bad argument");
+        return SFH_EARG;
+    }
+    size_t na = 0;
+    for (int q = 0; q < nseg; ++q) {
+        if (nx[q] <= 0 || ny[q] <= 0 || !(ell[q] > 0) ||
+            (long long)nx[q] * ny[q] != (long long)seg_rows[q + 1] - seg_rows[q]) {
+            set_error("problem %d: grid %d x %d does not match its rows", q, nx[q], ny[q]);
+            return SFH_EARG;
+        }
+        na += (size_t)nx[q] + ny[q];
+    }
+    DevBuf dax, sc0, sc1, sc2;
+    SFH_TRY(dax.ensure(sizeof(double) * na));
+    SFH_CHECK_HIP(hipMemcpyAsync(dax.p, axes, sizeof(double) * na, hipMemcpyHostToDevice, c->stream));
+    size_t off = 0;
+    for (int q = 0; q < nseg; ++q) {
+        const int gn[3] = {nx[q], ny[q], 1};
+        const double* a = dax.as<double>() + off;
+        const double* ax[3] = {a, a + nx[q], nullptr};
+        const size_t r0 = (size_t)seg_rows[q];
+        SFH_TRY(covforcing_grid(c, ny[q] > 1 ? 2 : 1, gn, ax, cv + r0, sigma[q], ell[q], k, W + r0 * k, V + r0 * k,
+                                sc0, sc1, sc2));
+        off += (size_t)nx[q] + ny[q];
+    }
+    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SFH_OK;
+}
+
 int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const double* pvals, double sigma_f,
                       double ell_f, double reg_eps, const sfh_solve_opts* opts, double* mean, double* Cu,
                       double* field, double* ubar, sfh_solve_stats* stats) {

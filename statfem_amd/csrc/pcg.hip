@@ -560,15 +560,19 @@ __device__ inline float mg_w(int x, int X, int n) {
 // branch and the memory latencies of a phase overlap.
 
 // r_{l+1} = P^T (r - A (S r)) on one tile of coarse nodes
-// work-group skip of the V-cycle kernels: none of the work-group's columns of
-// its problem is still being solved (scale 0: converged, zero or padding
-// columns of a narrower problem of a batch); S == NULL never skips
+// work-group skip of the V-cycle kernels inside an iteration: none of the
+// work-group's columns of its problem is still iterating (scale 0: converged,
+// zero or padding columns of a narrower problem of a batch; frozen: at the
+// target, its z unused until the next replacement recomputes it); S == NULL
+// never skips
 __device__ inline bool mg_cols_done(const double* __restrict__ S, int ks, int so, int col0, bool ok) {
     if (!S) return false;
     bool live = false;
     if (ok) {
         const double* sc = S + S_SCALE * ks + so + col0;
-        live = sc[0] > 0.0 || sc[1] > 0.0 || sc[2] > 0.0 || sc[3] > 0.0;
+        const double* fz = S + S_FROZEN * ks + so + col0;
+        live = (sc[0] > 0.0 && !(fz[0] > 0.0)) || (sc[1] > 0.0 && !(fz[1] > 0.0)) ||
+               (sc[2] > 0.0 && !(fz[2] > 0.0)) || (sc[3] > 0.0 && !(fz[3] > 0.0));
     }
     return !__syncthreads_or(live);
 }
@@ -1810,7 +1814,9 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                         ++nu;
                     }
                 }
-                if (na == 0 || nu > 0 || stall) break;
+                // (FP64 recurrences: a column at its target is done, not
+                // waiting for a replacement; the cycle runs until none is live)
+                if (na == 0 || (nu > 0 && !fp64_rec) || stall) break;
             }
         }
         jlast = j;

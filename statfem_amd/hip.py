@@ -67,6 +67,7 @@ _SIGS = {
     "sfh_solve_batched": (i32, [v, i32, v, i32, i32, v, v, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "sfh_gram_segmented": (i32, [v, i32, v, i32, i32, v, i32, v, i32, v]),
     "sfh_covforcing_grid": (i32, [v, i32, i32, i32, v, v, v, v, dbl, dbl, i32, v, v]),
+    "sfh_covforcing_batched": (i32, [v, i32, v, v, v, v, v, v, v, i32, v, v]),
     "sfh_dof_gather": (i32, [v, i32, i32, i32, i32, v, v, v]),
     "sfh_dof_scatter": (i32, [v, i32, i32, i32, i32, v, v, v]),
     "sfh_row_scale": (i32, [v, i32, i32, v, v, v]),

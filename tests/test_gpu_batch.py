@@ -107,3 +107,33 @@ def test_batched_multigrid_on_declared_grids(ctx, orc):
             scale = np.abs(ref[a:b]).max()
             assert np.abs(Xh[a:b] - ref[a:b]).max() <= 1e-9 * scale, (q, trans)
         assert np.all(Xh[off[2]:off[3], 7] == 0.0)
+
+
+def test_batched_forcing_covariance_equals_per_problem(ctx):
+    """sfh_covforcing_batched over a union of grids gives what the per-grid
+    sfh_covforcing_grid gives on each block."""
+    from statfem_amd import hip, mesh as M
+    sizes = [(12, 9), (20, 20), (7, 15)]
+    meshes = [M.grid2d(a, b) for a, b in sizes]
+    mu, off = M.union(meshes)
+    rng = np.random.default_rng(14)
+    k = 5
+    W = rng.normal(size=(mu.nnodes, k))
+    cv = rng.uniform(0.5, 1.5, size=mu.nnodes)
+    sig = np.array([0.3, 1.1, 0.7])
+    ell = np.array([0.1, 0.25, 0.4])
+    nx = np.array([a + 1 for a, b in sizes], np.int32)
+    ny = np.array([b + 1 for a, b in sizes], np.int32)
+    axes = np.concatenate([np.r_[m.grid[0], m.grid[1]] for m in meshes])
+    seg = np.ascontiguousarray(off, dtype=np.int32)
+    Wd, cvd, V = ctx.to_device(W), ctx.to_device(cv), ctx.empty((mu.nnodes, k))
+    hip._check(hip.lib().sfh_covforcing_batched(ctx.h, 3, hip._p(seg), hip._p(nx), hip._p(ny), hip._p(axes),
+                                                cvd.ptr, hip._p(sig), hip._p(ell), k, Wd.ptr, V.ptr))
+    Vb = V.download()
+    for q, m in enumerate(meshes):
+        a, b = off[q], off[q + 1]
+        Wq, cq, Vq = ctx.to_device(W[a:b]), ctx.to_device(cv[a:b]), ctx.empty((b - a, k))
+        ax, ay = np.ascontiguousarray(m.grid[0]), np.ascontiguousarray(m.grid[1])
+        hip._check(hip.lib().sfh_covforcing_grid(ctx.h, len(ax), len(ay), 1, hip._p(ax), hip._p(ay), None, cq.ptr,
+                                                 sig[q], ell[q], k, Wq.ptr, Vq.ptr))
+        assert np.array_equal(Vb[a:b], Vq.download())
