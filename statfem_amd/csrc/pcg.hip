@@ -527,9 +527,9 @@ __global__ __launch_bounds__(kG) void k_cg_update_mg(const int* __restrict__ gta
 // nodes carry neither residual nor correction, so M acts on the free block.
 // Tiles stage their vectors (kp columns per node, 4 per lane) with a halo in
 // LDS exactly like the iteration kernels above.
-constexpr int kRX = 16, kRY = 2;                      // coarse nodes per restriction tile
+constexpr int kRX = 8, kRY = 4;                       // coarse nodes per restriction tile
 constexpr int kRW = 2 * kRX + 3, kRH = 2 * kRY + 3;   // fine nodes it stages (residual support + halo)
-constexpr int kPX = 32, kPY = 4;                      // fine nodes per prolongation tile
+constexpr int kPX = 16, kPY = 8;                      // fine nodes per prolongation tile
 constexpr int kPW = kPX + 2, kPH = kPY + 2;
 constexpr int kCW = kPX / 2 + 2, kCH = kPY / 2 + 2;   // coarse nodes a prolongation tile stages
 // tile table, kMT ints per work-group: fine level first node, W, H, coarse
