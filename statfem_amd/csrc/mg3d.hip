@@ -31,6 +31,32 @@ namespace {
 constexpr int kB3 = 256;
 constexpr double kOmega3 = 4.0 / 3.0;
 
+// Node kernels step through their nodes in batches of one node per lane group
+// (kB3 / LPR nodes). With 16 lane groups (LPR 16) each batch's 27 DPN^2
+// stencil coefficients are first staged in LDS with coalesced loads (every
+// lane of a group needs all of them); with more groups the staging would cost
+// occupancy, and the lanes read the coefficients directly.
+template <int LPR>
+constexpr bool kStage3 = kB3 / LPR == 16;
+
+template <int LPR, int NC, class TC>
+struct CoefStage {
+    TC s[kStage3<LPR> ? (kB3 / LPR) * NC : 1];
+    // coefficients of batch node b0 + grp (cnt nodes in the batch); contains
+    // the barriers of the staged form, so every thread of the block calls it
+    __device__ const TC* load(const TC* __restrict__ coef, int b0, int cnt, int grp) {
+        if constexpr (kStage3<LPR>) {
+            __syncthreads();
+            const TC* src = coef + (size_t)b0 * NC;
+            for (int i = threadIdx.x; i < cnt * NC; i += blockDim.x) s[i] = src[i];
+            __syncthreads();
+            return s + grp * NC;
+        } else {
+            return coef + (size_t)(b0 + grp) * NC;
+        }
+    }
+};
+
 struct G3 {
     int W, H, D;
 };
@@ -238,9 +264,10 @@ __global__ __launch_bounds__(kB3) void k_g3_pdir(int kp, int nrow, int rpb, int 
     }
 }
 
-// q = A p at node n (FP64 stencil and sums, FP32 p)
+// q = A p at node n (FP64 stencil and sums, FP32 p); cn: the node's 27 DPN^2
+// coefficients (staged in LDS)
 template <int DPN>
-__device__ inline void apply_p64(const G3& g, int n, const double* __restrict__ coef, const float* __restrict__ p,
+__device__ inline void apply_p64(const G3& g, int n, const double* cn, const float* __restrict__ p,
                                  int kp, int col, V4<double> q[DPN]) {
     const int x = n % g.W, y = (n / g.W) % g.H, z = n / (g.W * g.H);
 #pragma unroll
@@ -248,7 +275,7 @@ __device__ inline void apply_p64(const G3& g, int n, const double* __restrict__ 
     for (int k = 0; k < 27; ++k) {
         const int j = nb27(k, x, y, z, g);
         if (j < 0) continue;
-        const double* c = coef + ((size_t)n * 27 + k) * DPN * DPN;
+        const double* c = cn + k * DPN * DPN;
         V4<float> v[DPN];
 #pragma unroll
         for (int b = 0; b < DPN; ++b) v[b] = ld4(p + ((size_t)j * DPN + b) * kp + col);
@@ -267,14 +294,19 @@ template <int DPN, int LPR>
 __global__ __launch_bounds__(kB3) void k_g3_pq(G3 g, int kp, int npb, int it, const double* __restrict__ coef,
                                                const double* __restrict__ S, const float* __restrict__ p,
                                                double* __restrict__ acc) {
+    constexpr int NC = 27 * DPN * DPN;
     __shared__ double red[kB3 * 4];
+    __shared__ CoefStage<LPR, NC, double> cs;
     const Tile t = tile3<LPR>(npb, g3n(g));
     if (block_done(t, kp, kp, S)) return;
     V4<double> pq = {0.0, 0.0, 0.0, 0.0};
-    if (t.col0 < kp) {
-        for (int n = t.r0 + t.grp; n < t.r1; n += t.ngrp) {
+    for (int b0 = t.r0; b0 < t.r1; b0 += t.ngrp) {
+        const int cnt = min(t.ngrp, t.r1 - b0);
+        const double* cn = cs.load(coef, b0, cnt, t.grp);
+        const int n = b0 + t.grp;
+        if (t.grp < cnt && t.col0 < kp) {
             V4<double> q[DPN];
-            apply_p64<DPN>(g, n, coef, p, kp, t.col0, q);
+            apply_p64<DPN>(g, n, cn, p, kp, t.col0, q);
 #pragma unroll
             for (int a = 0; a < DPN; ++a) {
                 const V4<float> pi = ld4(p + ((size_t)n * DPN + a) * kp + t.col0);
@@ -292,7 +324,9 @@ __global__ __launch_bounds__(kB3) void k_g3_update(G3 g, int kp, int npb, int it
                                                    const double* __restrict__ S, const float* __restrict__ p,
                                                    double* __restrict__ x, double* __restrict__ r,
                                                    float* __restrict__ rf, double* __restrict__ acc) {
+    constexpr int NC = 27 * DPN * DPN;
     __shared__ double red[kB3 * 4];
+    __shared__ CoefStage<LPR, NC, double> cs;
     __shared__ double ca[4 * LPR], cx[4 * LPR];
     clear_slots(acc, A_PQ + ((it + 1) & 1), 1, kp);
     const Tile t = tile3<LPR>(npb, g3n(g));
@@ -300,18 +334,21 @@ __global__ __launch_bounds__(kB3) void k_g3_update(G3 g, int kp, int npb, int it
     shared_alpha<double, LPR>(it, kp, kp, 0, S, acc, ca);
     for (int c = threadIdx.x; c < 4 * LPR; c += blockDim.x) {
         const int col = blockIdx.y * 4 * LPR + c;
-        const double sc = col < kp ? S[S_SCALE * kp + col] : 0.0;
-        cx[c] = sc > 0.0 ? ca[c] / sc : 0.0;
+        const double s = col < kp ? S[S_SCALE * kp + col] : 0.0;
+        cx[c] = s > 0.0 ? ca[c] / s : 0.0;
     }
     __syncthreads();
     V4<double> rr = {0.0, 0.0, 0.0, 0.0};
-    if (t.col0 < kp) {
-        const int l4 = t.lane * 4;
-        const V4<double> al = {ca[l4], ca[l4 + 1], ca[l4 + 2], ca[l4 + 3]};
-        const V4<double> ax = {cx[l4], cx[l4 + 1], cx[l4 + 2], cx[l4 + 3]};
-        for (int n = t.r0 + t.grp; n < t.r1; n += t.ngrp) {
+    const int l4 = t.lane * 4;
+    const V4<double> al = {ca[l4], ca[l4 + 1], ca[l4 + 2], ca[l4 + 3]};
+    const V4<double> ax = {cx[l4], cx[l4 + 1], cx[l4 + 2], cx[l4 + 3]};
+    for (int b0 = t.r0; b0 < t.r1; b0 += t.ngrp) {
+        const int cnt = min(t.ngrp, t.r1 - b0);
+        const double* cn = cs.load(coef, b0, cnt, t.grp);
+        const int n = b0 + t.grp;
+        if (t.grp < cnt && t.col0 < kp) {
             V4<double> q[DPN];
-            apply_p64<DPN>(g, n, coef, p, kp, t.col0, q);
+            apply_p64<DPN>(g, n, cn, p, kp, t.col0, q);
 #pragma unroll
             for (int a = 0; a < DPN; ++a) {
                 const size_t o = ((size_t)n * DPN + a) * kp + t.col0;
@@ -335,9 +372,14 @@ template <int DPN, int LPR, class TR>
 __global__ __launch_bounds__(kB3) void k_g3_resid(G3 g, int kp, int npb, const float* __restrict__ coef,
                                                   const float* __restrict__ dsm, const unsigned char* __restrict__ cmask,
                                                   const TR* __restrict__ r, float* __restrict__ res) {
+    constexpr int NC = 27 * DPN * DPN;
+    __shared__ CoefStage<LPR, NC, float> cs;
     const Tile t = tile3<LPR>(npb, g3n(g));
-    if (t.col0 >= kp) return;
-    for (int n = t.r0 + t.grp; n < t.r1; n += t.ngrp) {
+    for (int b0 = t.r0; b0 < t.r1; b0 += t.ngrp) {
+        const int cnt = min(t.ngrp, t.r1 - b0);
+        const float* cn = cs.load(coef, b0, cnt, t.grp);
+        const int n = b0 + t.grp;
+        if (!(t.grp < cnt && t.col0 < kp)) continue;
         const int x = n % g.W, y = (n / g.W) % g.H, z = n / (g.W * g.H);
         float4 q[DPN];
 #pragma unroll
@@ -345,7 +387,7 @@ __global__ __launch_bounds__(kB3) void k_g3_resid(G3 g, int kp, int npb, const f
         for (int k = 0; k < 27; ++k) {
             const int j = nb27(k, x, y, z, g);
             if (j < 0) continue;
-            const float* c = coef + ((size_t)n * 27 + k) * DPN * DPN;
+            const float* c = cn + k * DPN * DPN;
             float4 e[DPN];
 #pragma unroll
             for (int b = 0; b < DPN; ++b) {
@@ -438,11 +480,16 @@ __global__ __launch_bounds__(kB3) void k_g3_smooth(G3 g, int kp, int npb, int it
                                                    const TR* __restrict__ r, const float* __restrict__ e0,
                                                    float* __restrict__ e, double* __restrict__ acc,
                                                    double* __restrict__ part0, double* __restrict__ part1) {
+    constexpr int NC = 27 * DPN * DPN;
     __shared__ double red[kB3 * 4];
+    __shared__ CoefStage<LPR, NC, float> cs;
     const Tile t = tile3<LPR>(npb, g3n(g));
     V4<double> rz = {0.0, 0.0, 0.0, 0.0}, rr = {0.0, 0.0, 0.0, 0.0};
-    if (t.col0 < kp) {
-        for (int n = t.r0 + t.grp; n < t.r1; n += t.ngrp) {
+    for (int b0 = t.r0; b0 < t.r1; b0 += t.ngrp) {
+        const int cnt = min(t.ngrp, t.r1 - b0);
+        const float* cn = cs.load(coef, b0, cnt, t.grp);
+        const int n = b0 + t.grp;
+        if (t.grp < cnt && t.col0 < kp) {
             const int x = n % g.W, y = (n / g.W) % g.H, z = n / (g.W * g.H);
             float4 q[DPN];
 #pragma unroll
@@ -450,7 +497,7 @@ __global__ __launch_bounds__(kB3) void k_g3_smooth(G3 g, int kp, int npb, int it
             for (int k = 0; k < 27; ++k) {
                 const int j = nb27(k, x, y, z, g);
                 if (j < 0) continue;
-                const float* c = coef + ((size_t)n * 27 + k) * DPN * DPN;
+                const float* c = cn + k * DPN * DPN;
                 float4 v[DPN];
 #pragma unroll
                 for (int b = 0; b < DPN; ++b) v[b] = ldf4(e0 + ((size_t)j * DPN + b) * kp + t.col0);
