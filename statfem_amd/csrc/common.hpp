@@ -97,6 +97,9 @@ struct MGHier {
     DevBuf cinv;                     // coarsest-level inverses, grid after grid
     std::vector<int> cinv_off;
     int cmax = 0;                    // nodes of the largest coarsest level
+    // set-up tables (per level: the grids coarsened from it; per coarsest
+    // level: the grids whose inverse it holds), FP64 work and SPD flags
+    DevBuf gtab, cwork, cbad;
 };
 
 // Multigrid hierarchy of a declared 3D grid (1 or 3 DOFs per node): the
@@ -202,6 +205,7 @@ int assemble_rhs(sfh_system* s, int nq1d, const double* fconst, const double* fn
                  double factor, int accumulate, double* out);
 int apply_constraints(sfh_system* s, int nc, const int* dofs, const double* values, double factor,
                       double scalar);
+int apply_constraints_dev(sfh_system* s, int nc, const int* dofs_dev, double factor, double scalar);
 // nseg > 1: a block-diagonal batch of independent problems whose rows are
 // seg_rows[q] .. seg_rows[q+1]-1; every (problem, column) pair runs its own
 // PCG scalars and convergence test
@@ -242,6 +246,12 @@ int dgemm_shared_a(sfh_ctx* c, int M, int N, int K, const double* A, int lda, co
                    int ldb, long long strideB, double* C, int ldc, long long strideC, int batch);
 int loglik_batch(sfh_lik* L, int M, const double* theta_dev, double* out_dev, int* status_dev,
                  double* logdet_dev = nullptr);
+// the log-likelihood of M parameter sets on the device: set b uses
+// theta[3 b ..] = (rho, mismatch sigma, mismatch length), mean + b mstride and
+// eps_v[b] (eps when eps_v is null); K_s + rho^2 Cu with Cu shared
+int loglik_sets(sfh_ctx* c, int ns, int no, int dim, const double* xs, const double* Cu, const double* mean,
+                int mstride, const double* Y, double eps, const double* eps_v, int M, const double* theta,
+                double* out, int* status, double* logdet, DevBuf& work);
 int dense_spd_solve(sfh_ctx* c, int n, double* A_dev, int m, double* B_dev, int* status_dev);
 int mcmc_run(sfh_lik* L, int C, int steps, const double* theta0, const double* propsd, uint64_t seed,
              int mode, const double* pm, const double* pv, double* samples, double* acc);

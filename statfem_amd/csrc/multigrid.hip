@@ -19,7 +19,7 @@
 // symmetric A, so the sweep contracts on every level whatever the
 // coefficients; on a constant-coefficient interior row it is 2/3-damped
 // Jacobi. The coarsest level (at most kMGCoarse nodes) is solved with a dense
-// inverse computed in FP64 on the host.
+// inverse, computed in FP64 on the device.
 #include <cmath>
 
 #include "common.hpp"
@@ -49,10 +49,8 @@ __device__ inline int parents(int x, int n, int* P, double* w) {
 // their stencil neighbours j of P[i][I] a_ij P[j][J]. cmask (level 0 only)
 // drops constrained fine nodes from P. A coarse node without free support
 // gets a decoupled unit row.
-__global__ void k_mg_galerkin(int W, int H, int Wc, int Hc, const double* __restrict__ A,
-                              const unsigned char* __restrict__ cmask, double* __restrict__ Ac) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= Wc * Hc) return;
+__device__ inline void galerkin_node(int W, int H, int Wc, int I, const double* __restrict__ A,
+                                     const unsigned char* __restrict__ cmask, double* __restrict__ Ac) {
     const int Ix = I % Wc, Iy = I / Wc;
     double a9[9];
     for (int k = 0; k < 9; ++k) a9[k] = 0.0;
@@ -88,6 +86,18 @@ __global__ void k_mg_galerkin(int W, int H, int Wc, int Hc, const double* __rest
     for (int k = 0; k < 9; ++k) Ac[(size_t)I * 9 + k] = a9[k];
 }
 
+// one coarsening step of every grid that has level l + 1, in one launch:
+// work-group row blockIdx.y is grid tab[6 q ..] = {W, H, Wc, Hc, fine base,
+// coarse base} (bases: first node of the grid in its level's arrays)
+__global__ void k_mg_galerkin(const int* __restrict__ tab, const double* __restrict__ A,
+                              const unsigned char* __restrict__ cmask, double* __restrict__ Ac) {
+    const int* t = tab + 6 * blockIdx.y;
+    const int W = t[0], H = t[1], Wc = t[2], Hc = t[3], fb = t[4], cb = t[5];
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= Wc * Hc) return;
+    galerkin_node(W, H, Wc, I, A + (size_t)fb * 9, cmask ? cmask + fb : nullptr, Ac + (size_t)cb * 9);
+}
+
 // smoother weights omega / sum_j |a_ij| and (levels >= 1) the FP32 stencil
 __global__ void k_mg_level(int n, const double* __restrict__ A, float* __restrict__ dsm, float* __restrict__ A32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,50 +111,67 @@ __global__ void k_mg_level(int n, const double* __restrict__ A, float* __restric
     dsm[i] = s > 0.0 ? (float)(kOmega / s) : 0.f;
 }
 
-// dense inverse of the coarsest stencil operator (Cholesky, FP64); false when
-// the operator is not positive definite
-bool coarse_inverse(int W, int H, const std::vector<double>& a9, std::vector<float>& inv) {
-    const int n = W * H;
-    std::vector<double> A((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i) {
-        const int x = i % W, y = i / W;
-        for (int k = 0; k < 9; ++k) {
-            const int jx = x + k % 3 - 1, jy = y + k / 3 - 1;
-            if (jx < 0 || jx >= W || jy < 0 || jy >= H) continue;
-            A[(size_t)i * n + jy * W + jx] = a9[(size_t)i * 9 + k];
+// Dense inverse of each grid's coarsest level, one work-group per grid (grid
+// tab[6 r ..] = {W, H, first node in the level's arrays, offset of its n x n
+// inverse in inv, offset of its 2 n^2 FP64 work area, grid index q}): the symmetrised FP64
+// matrix of the 9-point stencil, its lower Cholesky factor in place (column
+// by column, the trailing update spread over the work-group), then one thread
+// per column c solves L L^T x = e_c and writes x in FP32. bad[q] = 1 when the
+// operator is not positive definite, 0 otherwise.
+__global__ __launch_bounds__(256) void k_mg_coarse_inverse(const int* __restrict__ tab,
+                                                           const double* __restrict__ a9,
+                                                           double* __restrict__ work, float* __restrict__ inv,
+                                                           int* __restrict__ bad) {
+    const int* t = tab + 6 * blockIdx.x;
+    const int W = t[0], H = t[1], n = W * H, q = t[5];
+    const double* s9 = a9 + (size_t)t[2] * 9;
+    float* out = inv + t[3];
+    double* A = work + t[4];
+    double* X = A + (size_t)n * n;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    for (int e = tid; e < n * n; e += bs) {
+        const int i = e / n, j = e % n;
+        const int dx = j % W - i % W, dy = j / W - i / W;
+        double v = 0.0;
+        if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) {
+            const int k = (dy + 1) * 3 + (dx + 1);
+            v = i == j ? s9[(size_t)i * 9 + 4] : 0.5 * (s9[(size_t)i * 9 + k] + s9[(size_t)j * 9 + (8 - k)]);
         }
+        A[e] = v;
     }
-    // symmetrise the FP64 rounding of the Galerkin sums
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < i; ++j) A[(size_t)i * n + j] = A[(size_t)j * n + i] = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
-    for (int j = 0; j < n; ++j) {   // lower Cholesky factor in place
-        double d = A[(size_t)j * n + j];
-        for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
-        if (!(d > 0.0)) return false;
-        d = std::sqrt(d);
-        A[(size_t)j * n + j] = d;
-        for (int i = j + 1; i < n; ++i) {
-            double s = A[(size_t)i * n + j];
-            for (int k = 0; k < j; ++k) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
-            A[(size_t)i * n + j] = s / d;
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        const double d = A[(size_t)j * n + j];
+        if (!(d > 0.0)) {   // the same value in every thread
+            if (tid == 0) bad[q] = 1;
+            return;
         }
+        const double l = sqrt(d);
+        __syncthreads();
+        if (tid == 0) A[(size_t)j * n + j] = l;
+        for (int i = j + 1 + tid; i < n; i += bs) A[(size_t)i * n + j] /= l;
+        __syncthreads();
+        const int m = n - j - 1;   // trailing lower triangle, rows and columns j+1 .. n-1
+        for (int e = tid; e < m * m; e += bs) {
+            const int i = j + 1 + e / m, k = j + 1 + e % m;
+            if (k <= i) A[(size_t)i * n + k] -= A[(size_t)i * n + j] * A[(size_t)k * n + j];
+        }
+        __syncthreads();
     }
-    inv.assign((size_t)n * n, 0.f);
-    std::vector<double> col(n);
-    for (int c = 0; c < n; ++c) {
+    if (tid == 0) bad[q] = 0;
+    for (int c = tid; c < n; c += bs) {
         for (int i = 0; i < n; ++i) {   // L y = e_c
             double s = i == c ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) s -= A[(size_t)i * n + k] * col[k];
-            col[i] = s / A[(size_t)i * n + i];
+            for (int k = 0; k < i; ++k) s -= A[(size_t)i * n + k] * X[(size_t)k * n + c];
+            X[(size_t)i * n + c] = s / A[(size_t)i * n + i];
         }
         for (int i = n - 1; i >= 0; --i) {   // L^T x = y
-            double s = col[i];
-            for (int k = i + 1; k < n; ++k) s -= A[(size_t)k * n + i] * col[k];
-            col[i] = s / A[(size_t)i * n + i];
+            double s = X[(size_t)i * n + c];
+            for (int k = i + 1; k < n; ++k) s -= A[(size_t)k * n + i] * X[(size_t)k * n + c];
+            X[(size_t)i * n + c] = s / A[(size_t)i * n + i];
         }
-        for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = (float)col[i];
+        for (int i = 0; i < n; ++i) out[(size_t)i * n + c] = (float)X[(size_t)i * n + c];
     }
-    return true;
 }
 
 }  // namespace
@@ -191,20 +218,57 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
     SFH_TRY(m.dsm[0].ensure(sizeof(float) * (size_t)s->ndof));
     k_mg_level<<<(s->ndof + 255) / 256, 256, 0, st>>>(s->ndof, s->dia64.as<double>(), m.dsm[0].as<float>(), nullptr);
     SFH_LAUNCH_CHECK();
+    // set-up tables for every level, uploaded once: the grids coarsened from
+    // level l ({W, H, Wc, Hc, fine base, coarse base}) and the grids whose
+    // coarsest level is l ({W, H, base, inverse offset, work offset, grid})
+    m.cinv_off.assign(ng, 0);
+    m.cmax = 0;
+    size_t ctot = 0, wtot = 0;
+    std::vector<size_t> woff(ng);
+    for (int q = 0; q < ng; ++q) {
+        const size_t o = (size_t)m.segL[q] * ng + q;
+        const int n = m.gW[o] * m.gH[o];
+        m.cinv_off[q] = (int)ctot;
+        woff[q] = wtot;
+        ctot += (size_t)n * n;
+        wtot += 2 * (size_t)n * n;
+        m.cmax = std::max(m.cmax, n);
+    }
+    if (ctot > (size_t)INT32_MAX || wtot > (size_t)INT32_MAX) { set_error("coarse levels too large"); return SFH_EARG; }
+    std::vector<int> tab;
+    std::vector<int> grow(m.L + 1, 0), gofs(m.L + 1, 0), gmax(m.L + 1, 0), crow(m.L + 1, 0), cofs(m.L + 1, 0);
+    for (int l = 0; l < m.L; ++l) {
+        gofs[l] = (int)tab.size();
+        for (int q = 0; q < ng; ++q) {
+            if (m.segL[q] <= l) continue;
+            const size_t o = (size_t)l * ng + q, oc = (size_t)(l + 1) * ng + q;
+            tab.insert(tab.end(), {m.gW[o], m.gH[o], m.gW[oc], m.gH[oc], m.gbase[o], m.gbase[oc]});
+            gmax[l] = std::max(gmax[l], m.gW[oc] * m.gH[oc]);
+            ++grow[l];
+        }
+    }
+    for (int l = 1; l <= m.L; ++l) {
+        cofs[l] = (int)tab.size();
+        for (int q = 0; q < ng; ++q) {
+            if (m.segL[q] != l) continue;
+            const size_t o = (size_t)l * ng + q;
+            tab.insert(tab.end(), {m.gW[o], m.gH[o], m.gbase[o], m.cinv_off[q], (int)woff[q], q});
+            ++crow[l];
+        }
+    }
+    SFH_TRY(m.gtab.ensure(sizeof(int) * tab.size()));
+    SFH_CHECK_HIP(hipMemcpyAsync(m.gtab.p, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice, st));
     for (int l = 0; l < m.L; ++l) {
         const long long nc = m.N[l + 1];
         SFH_TRY(m.coef64[l + 1].ensure(sizeof(double) * 9 * (size_t)nc));
         SFH_TRY(m.coef32[l + 1].ensure(sizeof(float) * 9 * (size_t)nc));
         SFH_TRY(m.dsm[l + 1].ensure(sizeof(float) * (size_t)nc));
         const double* Af = l == 0 ? s->dia64.as<double>() : m.coef64[l].as<double>();
-        for (int q = 0; q < ng; ++q) {
-            if (m.segL[q] <= l) continue;
-            const size_t o = (size_t)l * ng + q, oc = (size_t)(l + 1) * ng + q;
-            const int ncq = m.gW[oc] * m.gH[oc];
-            k_mg_galerkin<<<(ncq + 255) / 256, 256, 0, st>>>(
-                m.gW[o], m.gH[o], m.gW[oc], m.gH[oc], Af + (size_t)m.gbase[o] * 9,
-                l == 0 ? s->cmask.as<unsigned char>() + m.gbase[o] : nullptr,
-                m.coef64[l + 1].as<double>() + (size_t)m.gbase[oc] * 9);
+        for (int r0 = 0; r0 < grow[l]; r0 += 65535) {
+            const int rn = std::min(65535, grow[l] - r0);
+            k_mg_galerkin<<<dim3((gmax[l] + 255) / 256, rn), 256, 0, st>>>(
+                m.gtab.as<int>() + gofs[l] + 6 * (size_t)r0, Af, l == 0 ? s->cmask.as<unsigned char>() : nullptr,
+                m.coef64[l + 1].as<double>());
             SFH_LAUNCH_CHECK();
         }
         k_mg_level<<<(unsigned)((nc + 255) / 256), 256, 0, st>>>((int)nc, m.coef64[l + 1].as<double>(),
@@ -212,26 +276,22 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
                                                                  m.coef32[l + 1].as<float>());
         SFH_LAUNCH_CHECK();
     }
-    // dense inverses of the coarsest levels
-    std::vector<float> all;
-    m.cinv_off.assign(ng, 0);
-    m.cmax = 0;
-    for (int q = 0; q < ng; ++q) {
-        const size_t o = (size_t)m.segL[q] * ng + q;
-        const int w = m.gW[o], h = m.gH[o], n = w * h;
-        std::vector<double> a9((size_t)n * 9);
-        SFH_CHECK_HIP(hipMemcpyAsync(a9.data(), m.coef64[m.segL[q]].as<double>() + (size_t)m.gbase[o] * 9,
-                                     sizeof(double) * 9 * (size_t)n, hipMemcpyDeviceToHost, st));
-        SFH_CHECK_HIP(hipStreamSynchronize(st));
-        std::vector<float> inv;
-        if (!coarse_inverse(w, h, a9, inv)) { m.L = 0; return SFH_OK; }   // not SPD: Jacobi only
-        m.cinv_off[q] = (int)all.size();
-        all.insert(all.end(), inv.begin(), inv.end());
-        m.cmax = std::max(m.cmax, n);
+    // dense inverses of the coarsest levels, on the device: one launch per
+    // distinct coarsest level, one work-group per grid
+    SFH_TRY(m.cinv.ensure(sizeof(float) * ctot));
+    SFH_TRY(m.cwork.ensure(sizeof(double) * wtot));
+    SFH_TRY(m.cbad.ensure(sizeof(int) * (size_t)ng));
+    for (int l = 1; l <= m.L; ++l) {
+        if (crow[l] == 0) continue;
+        k_mg_coarse_inverse<<<crow[l], 256, 0, st>>>(m.gtab.as<int>() + cofs[l], m.coef64[l].as<double>(),
+                                                     m.cwork.as<double>(), m.cinv.as<float>(), m.cbad.as<int>());
+        SFH_LAUNCH_CHECK();
     }
-    SFH_TRY(m.cinv.ensure(sizeof(float) * all.size()));
-    SFH_CHECK_HIP(hipMemcpyAsync(m.cinv.p, all.data(), sizeof(float) * all.size(), hipMemcpyHostToDevice, st));
+    std::vector<int> bad(ng);
+    SFH_CHECK_HIP(hipMemcpyAsync(bad.data(), m.cbad.p, sizeof(int) * (size_t)ng, hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
+    for (int q = 0; q < ng; ++q)
+        if (bad[q]) { m.L = 0; return SFH_OK; }   // not SPD: Jacobi only
     return SFH_OK;
 }
 
