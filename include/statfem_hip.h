@@ -49,6 +49,7 @@ typedef struct sfh_ctx sfh_ctx;
 typedef struct sfh_system sfh_system;
 typedef struct sfh_lik sfh_lik;
 typedef struct sfh_mlp sfh_mlp;
+typedef struct sfh_lhs sfh_lhs;
 
 const char* sfh_status_string(int status);
 const char* sfh_last_error(void);
@@ -359,6 +360,66 @@ int sfh_mcmc_da(sfh_lik* lik, sfh_mlp* mlp, int chains, int steps, int sub, cons
 int sfh_gp(sfh_ctx* ctx, int n, int dim, const double* X, const double* y, double sigma,
            double ell, double eps, int np, const double* Xp, double* logl, double* mean,
            double* var);
+
+/* ------------------------------------------- LHS (conductivity) uncertainty */
+/* ForwardGaussLHS (statX/infer/ForwardGaussLHS.hpp:41-105) with the solve
+ * function of laplaceInverse (del2/apps/reference/laplaceInverse/inference.hpp:72-120)
+ * for nch conductivity fields at once. sys is the disjoint union of nch copies
+ * of one scalar mesh (copy c: rows seg_rows[c] .. seg_rows[c+1]-1, elements
+ * c nel .. (c+1) nel - 1), with its right-hand side assembled (the unit body
+ * force, fem.hpp:19-27) and constrained; bc (host, nbc union DOFs) are the
+ * homogeneous Dirichlet DOFs re-applied after every assembly (fem.hpp:31-43).
+ * A hyperparameter set theta (log space) has na + 1 entries: the anchor values
+ * w = exp(theta[0 .. na-1]) of the GaussianProcess<ExpKernel, na> whose mean at
+ * each element centroid is the element's conductivity (inference.hpp:72-88;
+ * GaussianProcess.hpp:117-135 with anchors Xa (na x dim, host), gp_sigma,
+ * gp_ell, gp_eps), and the sensor noise mismatch.sensorEps = exp(theta[na]).
+ * The projection P has ns rows of pn (column, value) pairs in one copy's DOF
+ * numbering (readMatrixPhi, ioHelpers.hpp:88-123). na + 1 <= 16. */
+int sfh_lhs_create(sfh_system* sys, int nch, const int* seg_rows, int nel, int na, const double* Xa,
+                   double gp_sigma, double gp_ell, double gp_eps, int nq1d, int nbc, const int* bc, int ns,
+                   int pn, const int* pcols, const double* pvals, sfh_lhs** out);
+int sfh_lhs_destroy(sfh_lhs* lhs);
+/* LikelihoodKOH + MismatchCovariance data and the prior of hyperParamLearning
+ * (del2/apps/reference/laplaceInverse/hyperParamLearning.hpp:126-150): readings
+ * Y (no x ns, host, reading-major), sensor coordinates xs (ns x sdim, host),
+ * rho, mismatch.sigma, mismatch.length; prior != 0 adds the normal prior
+ * density of (theta[0 .. na-1], exp(theta[na])) with means prior_mu and
+ * variances prior_var (na + 1 each, host; enrichWithPrior). */
+int sfh_lhs_set_data(sfh_lhs* lhs, int no, const double* Y, int sdim, const double* xs, double rho,
+                     double mis_sigma, double mis_ell, int prior, const double* prior_mu,
+                     const double* prior_var);
+/* ForwardGaussLHS::updateSolution (ForwardGaussLHS.hpp:83-92) for the nch sets
+ * theta (nch x (na + 1), host): conductivity, assembly, constraints, one
+ * batched solve, projection. mean (host, nch x ns) may be NULL. SFH_ENOTSPD
+ * when a set interpolates a non-positive conductivity. */
+int sfh_lhs_update(sfh_lhs* lhs, const double* theta, const sfh_solve_opts* opts, double* mean,
+                   sfh_solve_stats* stats);
+/* keep copy `copy`'s projected mean of the last update as the mean of every
+ * later evaluation with resolve = 0: the reference's laplaceInverse computes
+ * the FE solution once, at the hyperparameters it starts from
+ * (inference.hpp:139), and its likelihood reuses it at every MCMC step
+ * (LikelihoodKOH.ipp:82-84). */
+int sfh_lhs_fix_mean(sfh_lhs* lhs, int copy);
+/* the log target posteriorGivenData (hyperParamLearning.hpp:172-190) of M sets
+ * theta (M x (na + 1), host): the KOH log-likelihood with C_u = 0, plus
+ * theta[na], plus the log prior. resolve = 1 re-solves each set's field
+ * (M must equal nch); resolve = 0 uses the fixed mean (any M). out, status:
+ * M (host); -inf where the likelihood or the conductivity failed. */
+int sfh_lhs_log_target(sfh_lhs* lhs, int M, const double* theta, int resolve, const sfh_solve_opts* opts,
+                       double* out, int* status, sfh_solve_stats* stats);
+/* MCMC<na + 1>::sample (statX/infer/MCMC.hpp:123-158) of that target for nch
+ * chains in lockstep from theta0 (nch x (na + 1), host), random-walk proposal
+ * theta + propsd .* z; chain c's draws come from Philox counters
+ * {step, c, 32 + i, 0} (normals, Box-Muller pairs) and {step, c, 31, 0}
+ * (acceptance uniform) under the key `seed`. resolve = 1 re-interpolates,
+ * re-assembles and re-solves every proposal (all chains in one batched solve);
+ * resolve = 0 keeps the fixed mean as the reference does. samples (host,
+ * nch x steps x (na + 2): the state after each step and its log target) may
+ * be NULL; acc (host, nch) acceptance ratios; stats sums the solves. */
+int sfh_lhs_mcmc(sfh_lhs* lhs, int steps, const double* theta0, const double* propsd, uint64_t seed,
+                 int resolve, const sfh_solve_opts* opts, double* samples, double* acc,
+                 sfh_solve_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,7 @@ int or_spmv(int n, const int* rowptr, const int* colind, const double* vals, int
 // statistics half (oracle_stat.cpp)
 void or_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
 int or_mcmc_normals(uint64_t seed, int chain, int step, double* z /*3*/, double* u /*1*/);
+int or_lhs_draws(uint64_t seed, int chain, int step, int np, double* z /*np*/, double* u /*1*/);
 double or_se_kernel(int dim, const double* x, const double* y, double sigma, double ell);
 int or_covforcing_project(int n, int dim, const double* coords, const double* covvec,
                           double sigma, double ell, double cut_eps, int ns, const double* R,

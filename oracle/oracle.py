@@ -52,6 +52,7 @@ def lib():
         "or_spmv": (i, [i, _ip, _ip, _dp, i, _dp, _dp]),
         "or_philox4x32_10": (None, [_up, _up, _up]),
         "or_mcmc_normals": (i, [C.c_uint64, i, i, _dp, _dp]),
+        "or_lhs_draws": (i, [C.c_uint64, i, i, i, _dp, _dp]),
         "or_se_kernel": (d, [i, _dp, _dp, d, d]),
         "or_covforcing_project": (i, [i, i, _dp, _dp, d, d, d, i, _dp, _dp]),
         "or_covforcing_apply": (i, [i, i, _dp, _dp, d, d, i, _dp, _dp]),
@@ -239,6 +240,13 @@ def mcmc_draws(seed, chain, step):
     return z, u[0]
 
 
+def lhs_draws(seed, chain, step, np_):
+    z = np.zeros(np_)
+    u = np.zeros(1)
+    lib().or_lhs_draws(seed, chain, step, np_, z, u)
+    return z, u[0]
+
+
 def covforcing_project(coords, covvec, sigma, ell, R, cut_eps=0.0):
     coords = _c(coords)
     n, dim = coords.shape
@@ -356,3 +364,64 @@ def precision_apply(rp, ci, va, mass, sigma, kappa, nu, dim, X):
     lib().or_precision_apply(n, _c(rp, np.int32), _c(ci, np.int32), _c(va), _c(mass), sigma, kappa, nu, dim,
                              X2.shape[1], _c(X2), Y)
     return Y.reshape(X.shape)
+
+
+# ------------------------------------------- conductivity (LHS) uncertainty
+def lhs_conductivity(mesh, anchors, w, gp_sigma, gp_ell, gp_eps):
+    """Element conductivities of laplaceInverse (del2/apps/reference/laplaceInverse/
+    inference.hpp:72-88): the GaussianProcess mean (statX/infer/GaussianProcess.hpp:117-135)
+    at each element centroid given anchor values w."""
+    cen = mesh.coords[mesh.conn].mean(axis=1)
+    A = np.asarray(anchors, np.float64).reshape(len(w), -1)
+    _, kap, _ = gp(A, np.asarray(w, np.float64), gp_sigma, gp_ell, gp_eps, cen.reshape(len(cen), A.shape[1]))
+    return kap
+
+
+def lhs_solution(mesh, bc, anchors, w, gp_sigma, gp_ell, gp_eps, nq1d=2, fconst=1.0):
+    """The solve function of laplaceInverse (inference.hpp:90-120, fem.hpp:47-98):
+    conductivity, Poisson assembly with the unit body force, row-zero Dirichlet,
+    direct solve. Returns (u, kappa)."""
+    kap = lhs_conductivity(mesh, anchors, w, gp_sigma, gp_ell, gp_eps)
+    F = body_force(mesh, nq1d=nq1d, fconst=fconst)
+    rp, ci, va, F = assemble(mesh, POISSON, kap, nq1d=nq1d, bc_dofs=bc, F=F)
+    return LU(rp, ci, va).solve(F), kap
+
+
+def lhs_log_target(theta, mean, Y, xs, rho, mis_sigma, mis_ell, prior_mu=None, prior_var=None):
+    """posteriorGivenData (del2/apps/reference/laplaceInverse/hyperParamLearning.hpp:172-190)
+    of one set theta = (log w_0 .. log w_{na-1}, log sensorEps) given the projected
+    mean: the KOH log-likelihood with C_u = 0 (ForwardGaussLHS.hpp:71-79), plus
+    theta[na], plus (prior_mu given) the log normal density of
+    (theta[:na], exp(theta[na])) with diagonal covariance prior_var."""
+    theta = np.asarray(theta, np.float64)
+    na = len(theta) - 1
+    eps = np.exp(theta[na])
+    ns = len(mean)
+    Ks = mismatch_cov(np.asarray(xs, np.float64).reshape(ns, -1), mis_sigma, mis_ell, eps)
+    ll = loglik(np.zeros((ns, ns)), np.asarray(mean, np.float64), Y, Ks, rho)
+    y = ll + theta[na]
+    if prior_mu is not None:
+        hp = np.r_[theta[:na], eps]
+        pv = np.asarray(prior_var, np.float64)
+        y += -0.5 * np.sum((hp - np.asarray(prior_mu)) ** 2 / pv) - 0.5 * (len(hp) * np.log(2 * np.pi) + np.log(pv).sum())
+    return y
+
+
+def lhs_chain(theta0, propsd, steps, seed, chain, target):
+    """MCMC<np>::sample (statX/infer/MCMC.hpp:123-158) with the Philox draws of
+    or_lhs_draws; target(theta) -> log target. Returns (steps x (np + 1)
+    states and targets, acceptance ratio)."""
+    x = np.array(theta0, np.float64)
+    y = target(x)
+    out = np.zeros((steps, len(x) + 1))
+    acc = 0
+    for t in range(steps):
+        z, u = lhs_draws(seed, chain, t, len(x))
+        xp = x + np.asarray(propsd) * z
+        yp = target(xp)
+        if np.log(u) <= min(0.0, yp - y):
+            x, y = xp, yp
+            acc += 1
+        out[t, :-1] = x
+        out[t, -1] = y
+    return out, acc / max(steps, 1)

@@ -10,6 +10,7 @@
 //   statX/infer/GaussianProcess.hpp:117-242       dense GP fit / logL / mean / variance
 //   statX/infer/MCMC.hpp:123-158,186-197          random-walk Metropolis
 //   del2/apps/reference/laplaceRHSinfer/hyperParamLearning.hpp:60-82,172-190   target density
+//   del2/apps/reference/laplaceInverse/hyperParamLearning.hpp:172-190         (draws of its chain)
 // The random streams are Philox4x32-10 (Salmon et al., SC'11), not the
 // reference's Boost mt19937 stream, which cannot be rebuilt here.
 #include <cmath>
@@ -205,6 +206,27 @@ void or_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
 
 int or_mcmc_normals(uint64_t seed, int chain, int step, double* z, double* u) {
     mcmc_draws(seed, chain, step, z, u);
+    return OR_OK;
+}
+
+// Draws of the conductivity-uncertainty chain (np proposal increments and the
+// accept uniform): counters {step, chain, 32 + i, 0} give the Box-Muller pair
+// (cosine, sine) for components 2i, 2i+1; {step, chain, 31, 0} the uniform.
+int or_lhs_draws(uint64_t seed, int chain, int step, int np, double* z, double* u) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t a[4];
+    const double two_pi = 6.283185307179586;
+    for (int i = 0; 2 * i < np; ++i) {
+        const uint32_t ctr[4] = {(uint32_t)step, (uint32_t)chain, 32u + (uint32_t)i, 0u};
+        philox(ctr, key, a);
+        const double u1 = to_unit(a[0], a[1]), u2 = to_unit(a[2], a[3]);
+        const double r = std::sqrt(-2.0 * std::log(u1));
+        z[2 * i] = r * std::cos(two_pi * u2);
+        if (2 * i + 1 < np) z[2 * i + 1] = r * std::sin(two_pi * u2);
+    }
+    const uint32_t ctr[4] = {(uint32_t)step, (uint32_t)chain, 31u, 0u};
+    philox(ctr, key, a);
+    *u = to_unit(a[0], a[1]);
     return OR_OK;
 }
 

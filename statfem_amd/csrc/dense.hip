@@ -194,15 +194,6 @@ __global__ void k_sum_partials(long long mn, int nz, const double* __restrict__ 
 }
 
 // ----------------------------------------------------- forcing covariance
-__device__ inline double se_kernel(int dim, const double* x, const double* y, double sigma, double ell) {
-    double sq = 0.0;
-    for (int i = 0; i < dim; ++i) {
-        const double d = x[i] - y[i];
-        sq += d * d;
-    }
-    return sigma * sigma * exp(-sq / (2. * ell * ell));
-}
-
 // V = C_f W, matrix-free: 16 rows x 64 columns per block, kernel tiles in LDS
 __global__ __launch_bounds__(256) void k_cf_dense(int n, int dim, int k, const double* __restrict__ X,
                                                   const double* __restrict__ cv, double sigma, double ell,
@@ -355,8 +346,9 @@ __device__ double block_sum(double v, double* red) {
 // C = K_d(sigma_d, ell_d) + eps^2 I + rho^2 Cu; LLT; logL = no(-ns/2 log 2pi - sum log L_ii)
 // - 1/2 sum_k |L^-1 (y_k - rho mean)|^2   (statX/infer/LikelihoodKOH.ipp:74-120)
 __global__ __launch_bounds__(256) void k_loglik(int ns, int no, int dim, const double* __restrict__ xs,
-                                                const double* __restrict__ Cu, const double* __restrict__ mean,
-                                                const double* __restrict__ Y, double eps,
+                                                const double* __restrict__ Cu, const double* __restrict__ mean0,
+                                                int mstride, const double* __restrict__ Y, double eps0,
+                                                const double* __restrict__ eps_v,
                                                 const double* __restrict__ theta, double* __restrict__ work,
                                                 double* __restrict__ out, int* __restrict__ status,
                                                 double* __restrict__ logdet_out) {
@@ -367,6 +359,8 @@ __global__ __launch_bounds__(256) void k_loglik(int ns, int no, int dim, const d
     __shared__ int sflag;
     const int b = blockIdx.x, tid = threadIdx.x, bs = blockDim.x;
     const double rho = theta[3 * b], sd = theta[3 * b + 1], ld = theta[3 * b + 2];
+    const double* mean = mean0 + (size_t)b * mstride;
+    const double eps = eps_v ? eps_v[b] : eps0;
     double* a = work + (size_t)b * ns * ns;
     for (int idx = tid; idx < ns * ns; idx += bs) {
         const int i = idx / ns, k = idx % ns;
@@ -421,8 +415,9 @@ __global__ __launch_bounds__(256) void k_loglik(int ns, int no, int dim, const d
 // barrier per step.
 template <int NB>
 __global__ __launch_bounds__(1024) void k_loglik_reg(int ns, int no, int dim, const double* __restrict__ xs,
-                                                     const double* __restrict__ Cu, const double* __restrict__ mean,
-                                                     const double* __restrict__ Y, double eps,
+                                                     const double* __restrict__ Cu, const double* __restrict__ mean0,
+                                                     int mstride, const double* __restrict__ Y, double eps0,
+                                                     const double* __restrict__ eps_v,
                                                      const double* __restrict__ theta, double* __restrict__ out,
                                                      int* __restrict__ status, double* __restrict__ logdet_out) {
     __shared__ double colb[2][32 * NB];
@@ -432,6 +427,8 @@ __global__ __launch_bounds__(1024) void k_loglik_reg(int ns, int no, int dim, co
     const int tr = threadIdx.x >> 5, tc = threadIdx.x & 31;
     const int na = ns + no;
     const double rho = theta[3 * b], sd = theta[3 * b + 1], ld = theta[3 * b + 2];
+    const double* mean = mean0 + (size_t)b * mstride;
+    const double eps = eps_v ? eps_v[b] : eps0;
     double a[NB][NB];
 #pragma unroll
     for (int ii = 0; ii < NB; ++ii)
@@ -812,34 +809,36 @@ int covforcing_grid(sfh_ctx* c, int d, const int gn_[3], const double* const ax[
     return SFH_OK;
 }
 
-int loglik_batch(sfh_lik* L, int M, const double* theta, double* out, int* status, double* logdet) {
-    hipStream_t st = L->ctx->stream;
-    const int ns = L->ns, no = L->no;
+int loglik_sets(sfh_ctx* c, int ns, int no, int dim, const double* xs, const double* Cu, const double* mean,
+                int mstride, const double* Y, double eps, const double* eps_v, int M, const double* theta,
+                double* out, int* status, double* logdet, DevBuf& work) {
+    hipStream_t st = c->stream;
     const int na = ns + no;
-    if (na <= 224 && M > 0) {
+    if (M <= 0) return SFH_OK;
+    if (na <= 224) {
         // register-resident factorisation (sizes up to 224 augmented rows)
-        if (na <= 64)
-            k_loglik_reg<2><<<M, 1024, 0, st>>>(ns, no, L->dim, L->xs.as<double>(), L->Cu.as<double>(),
-                                                L->mean.as<double>(), L->Y.as<double>(), L->eps, theta, out, status,
-                                                logdet);
-        else if (na <= 128)
-            k_loglik_reg<4><<<M, 1024, 0, st>>>(ns, no, L->dim, L->xs.as<double>(), L->Cu.as<double>(),
-                                                L->mean.as<double>(), L->Y.as<double>(), L->eps, theta, out, status,
-                                                logdet);
-        else
-            k_loglik_reg<7><<<M, 1024, 0, st>>>(ns, no, L->dim, L->xs.as<double>(), L->Cu.as<double>(),
-                                                L->mean.as<double>(), L->Y.as<double>(), L->eps, theta, out, status,
-                                                logdet);
+#define SFH_LLREG(NB)                                                                                           \
+    k_loglik_reg<NB><<<M, 1024, 0, st>>>(ns, no, dim, xs, Cu, mean, mstride, Y, eps, eps_v, theta, out, status, \
+                                         logdet)
+        if (na <= 64) SFH_LLREG(2);
+        else if (na <= 128) SFH_LLREG(4);
+        else SFH_LLREG(7);
+#undef SFH_LLREG
         SFH_LAUNCH_CHECK();
         return SFH_OK;
     }
-    SFH_TRY(L->work.ensure((size_t)M * ns * ns * 8));
+    SFH_TRY(work.ensure((size_t)M * ns * ns * 8));
     const size_t lds = sizeof(double) * ((size_t)ns + (size_t)ns * no + 256);
-    if (lds > L->ctx->lds_per_block) { set_error("ns * no too large for LDS"); return SFH_EARG; }
-    k_loglik<<<M, 256, lds, st>>>(ns, no, L->dim, L->xs.as<double>(), L->Cu.as<double>(), L->mean.as<double>(),
-                                 L->Y.as<double>(), L->eps, theta, L->work.as<double>(), out, status, logdet);
+    if (lds > c->lds_per_block) { set_error("ns * no too large for LDS"); return SFH_EARG; }
+    k_loglik<<<M, 256, lds, st>>>(ns, no, dim, xs, Cu, mean, mstride, Y, eps, eps_v, theta, work.as<double>(), out,
+                                 status, logdet);
     SFH_LAUNCH_CHECK();
     return SFH_OK;
+}
+
+int loglik_batch(sfh_lik* L, int M, const double* theta, double* out, int* status, double* logdet) {
+    return loglik_sets(L->ctx, L->ns, L->no, L->dim, L->xs.as<double>(), L->Cu.as<double>(), L->mean.as<double>(), 0,
+                       L->Y.as<double>(), L->eps, nullptr, M, theta, out, status, logdet, L->work);
 }
 
 int dense_spd_solve(sfh_ctx* c, int n, double* A, int m, double* B, int* status) {

@@ -716,6 +716,19 @@ int apply_constraints(sfh_system* s, int nc, const int* dofs, const double* valu
     return SFH_OK;
 }
 
+// the same constraints from a DOF list already on the device (values 0), with
+// no allocation and no wait: for loops that re-assemble and re-constrain
+int apply_constraints_dev(sfh_system* s, int nc, const int* dofs_dev, double factor, double scalar) {
+    if (nc <= 0) return SFH_OK;
+    k_constraints<<<grid_for(nc, kBlock), kBlock, 0, s->ctx->stream>>>(
+        nc, dofs_dev, nullptr, factor, scalar, s->rowptr.as<int>(), s->colind.as<int>(), s->vals.as<double>(),
+        s->vals32.as<float>(), s->F.as<double>(), s->cmask.as<unsigned char>());
+    SFH_LAUNCH_CHECK();
+    s->dirty = true;
+    if (s->dia_state > 0) s->dia_state = 0;
+    return SFH_OK;
+}
+
 int spmv(sfh_system* s, int k, const double* X, double* Y) {
     hipStream_t st = s->ctx->stream;
     k_spmv64<<<grid_for((long long)s->ndof * k, 256), 256, 0, st>>>(s->ndof, k, s->rowptr.as<int>(), s->colind.as<int>(),

@@ -50,6 +50,17 @@ __device__ inline void mcmc_draws(uint64_t seed, int chain, int step, double* z,
     draws3(seed, (uint32_t)step, (uint32_t)chain, 0u, z, u);
 }
 
+// squared-exponential kernel sigma^2 exp(-|x - y|^2 / (2 ell^2)) of
+// ExpKernel::giveValue (statX/infer/ExpKernel.hpp:74-92)
+__host__ __device__ inline double se_kernel(int dim, const double* x, const double* y, double sigma, double ell) {
+    double sq = 0.0;
+    for (int i = 0; i < dim; ++i) {
+        const double d = x[i] - y[i];
+        sq += d * d;
+    }
+    return sigma * sigma * exp(-sq / (2. * ell * ell));
+}
+
 // prior / Jacobian terms of the log target, hyperParamLearning.hpp:172-190 in
 // log space: mode 0 flat in x (+ sum theta), 1 the reference's enrichWithPrior
 // (normal density evaluated at e^theta, lines 60-82), 2 log-normal N(theta; mu, var)

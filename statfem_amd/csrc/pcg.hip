@@ -1940,7 +1940,9 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
         for (int q = 0; q <= nseg; ++q) srow[q] = seg_rows ? seg_rows[q] : (q == 0 ? 0 : s->ndof);
         const std::vector<int> g = grids_of(s, nseg, srow);
         if (!g.empty()) max_lpr = 16;
-        if (g.size() == 3 && o.precond != 1 && k < 16 && (long long)g[1] * g[2] > kMGCoarse) min_kp = 16;
+        bool big = false;   // a grid with a multigrid hierarchy (one grid or a batch)
+        for (size_t q = 0; q < g.size(); q += 3) big = big || (long long)g[q + 1] * g[q + 2] > kMGCoarse;
+        if (big && o.precond != 1 && k < 16) min_kp = 16;
     }
     if (o.precision == 0 && (o.layout == 0 || o.layout == 4) && nseg == 1 && g3_applies(s)) {
         max_lpr = 16;

@@ -118,6 +118,13 @@ _SIGS = {
     "sfh_mlp_predict": (i32, [v, i32, v, v]),
     "sfh_mcmc_da": (i32, [v, v, i32, i32, i32, v, v, C.c_uint64, i32, v, v, v, v, v]),
     "sfh_gp": (i32, [v, i32, i32, v, v, dbl, dbl, dbl, i32, v, v, v, v]),
+    "sfh_lhs_create": (i32, [v, i32, v, i32, i32, v, dbl, dbl, dbl, i32, i32, v, i32, i32, v, v, C.POINTER(v)]),
+    "sfh_lhs_destroy": (i32, [v]),
+    "sfh_lhs_set_data": (i32, [v, i32, v, i32, v, dbl, dbl, dbl, i32, v, v]),
+    "sfh_lhs_update": (i32, [v, v, C.POINTER(SolveOpts), v, C.POINTER(SolveStats)]),
+    "sfh_lhs_fix_mean": (i32, [v, i32]),
+    "sfh_lhs_log_target": (i32, [v, i32, v, i32, C.POINTER(SolveOpts), v, v, C.POINTER(SolveStats)]),
+    "sfh_lhs_mcmc": (i32, [v, i32, v, v, C.c_uint64, i32, C.POINTER(SolveOpts), v, v, C.POINTER(SolveStats)]),
 }
 
 

@@ -26,6 +26,16 @@ FILES = {
                         "del2/apps/reference/laplaceRHSinfer/sensorCoordinates.dat",
                         "del2/apps/reference/laplaceRHSinfer/phiMat.dat",
                         "del2/apps/reference/laplaceRHSinfer/watchDog.0.mk.inferredFEcondData.dat"],
+    # conductivity (LHS) inverse problem (del2/apps/reference/laplaceInverse/watchDog.0.mk:21-29)
+    "laplaceInverse": ["del2/apps/reference/laplaceInverse/bar.smf",
+                       "del2/apps/reference/laplaceInverse/input.dat",
+                       "del2/apps/reference/laplaceInverse/inferenceInput.dat",
+                       "del2/apps/reference/laplaceInverse/allHyperParam.dat",
+                       "del2/apps/reference/laplaceInverse/gpCoordinates.dat",
+                       "del2/apps/reference/laplaceInverse/phiMat.dat",
+                       "del2/apps/reference/laplaceInverse/sensorCoordinates.dat",
+                       "del2/apps/reference/laplaceInverse/sensorValuesRead5.dat",
+                       "del2/apps/reference/laplaceInverse/hParMean.dat"],
     # RW-MH statistics (statX/apps/reference/mcmcGaussian)
     "mcmcGaussian": ["statX/apps/reference/mcmcGaussian/input.dat"],
 }
@@ -49,6 +59,11 @@ def main():
         # log-density column check: max |l - log N(s;1,0.25)|
         ln = np.log(1.0 / (np.sqrt(2 * np.pi) * 0.5) * np.exp(-(s - 1.0) ** 2 / (2 * 0.25)))
         fh.write(f"max_abs_logpdf_dev {np.max(np.abs(l - ln)):.3g}\n")
+    # every 10th row of the recorded laplaceInverse chain (states after burn-in
+    # and their log target; the 236 KB file itself is not copied)
+    rows = open(os.path.join(REF, "del2/apps/reference/laplaceInverse/logMLsamples.dat")).read().splitlines()
+    with open(os.path.join(HERE, "laplaceInverse", "logMLsamples_every10.dat"), "w") as fh:
+        fh.write("\n".join(rows[::10]) + "\n")
     with open(os.path.join(HERE, "gp", "expected.txt"), "w") as fh:
         fh.write("# statX/apps/reference/gaussianProcess/watchDog.0.mk:19\nlogL -4.837171259\n")
     with open(os.path.join(HERE, "laplaceRHSinfer", "expected.txt"), "w") as fh:

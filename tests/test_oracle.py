@@ -379,3 +379,62 @@ def test_ess_of_ar1_chains_matches_closed_form():
     assert D.split_rhat(iid) < 1.01
     shifted = iid + np.arange(m)[:, None]
     assert D.split_rhat(shifted) > 2
+
+
+# ----------------------------------------------- conductivity (LHS) uncertainty
+def test_laplace_inverse_targets_golden(orc):
+    """The oracle's laplaceInverse log target against the recorded chain
+    (every 10th row of logMLsamples.dat: natural hyperparameters and their log
+    target, 6 significant digits). The reference evaluates the likelihood at
+    the FE solution of the map's own conductivities (inference.hpp:139; the
+    chain never re-solves): that reproduces every row to print precision,
+    while re-solving at each row's conductivities does not."""
+    from . import lhs_deck
+    d = lhs_deck.load()
+    m = d["mesh"]
+    P = lhs_deck.projection(d, m.nnodes)
+    u, _ = orc.lhs_solution(m, d["bc"], d["anchors"], d["w0"], *d["gp"], nq1d=3)
+    mean = P @ u
+    dev = [lhs_deck.oracle_target(orc, d, np.log(r[:6]), mean) - r[6] for r in d["golden"]]
+    assert len(dev) == 350
+    assert np.abs(dev).max() < 1e-3
+    r = d["golden"][100]
+    ur, _ = orc.lhs_solution(m, d["bc"], d["anchors"], r[:5], *d["gp"], nq1d=3)
+    assert abs(lhs_deck.oracle_target(orc, d, np.log(r[:6]), P @ ur) - r[6]) > 10.0
+
+
+def test_laplace_inverse_conductivity_interpolates_anchors(orc):
+    """The GP-mean conductivity passes (to the nugget) through the anchor
+    values and is linear in them."""
+    m = M.line1d(400)
+    A = np.array([[0.0], [0.25], [0.5], [0.75], [1.0]])
+    w = np.array([0.7, 1.0, 0.7, 0.4, 0.7])
+    k = orc.lhs_conductivity(m, A, w, 1.0, 0.32, 5e-5)
+    cen = m.coords[m.conn].mean(axis=1)[:, 0]
+    for a, v in zip(A[:, 0], w):
+        e = np.argmin(np.abs(cen - a))
+        assert abs(k[e] - v) < 0.02
+    k2 = orc.lhs_conductivity(m, A, 2.0 * w + 1.0, 1.0, 0.32, 5e-5)
+    k1 = orc.lhs_conductivity(m, A, np.ones(5), 1.0, 0.32, 5e-5)
+    assert np.allclose(k2, 2.0 * k + k1, rtol=0, atol=1e-12)
+
+
+def test_lhs_draws_counter_layout(orc):
+    """Each pair of proposal increments is the Box-Muller pair of Philox block
+    {step, chain, 32 + i, 0}; the uniform is block {step, chain, 31, 0}."""
+    seed, chain, step = 0x1234567890ABCDEF, 3, 17
+    z, u = orc.lhs_draws(seed, chain, step, 6)
+    key = np.array([seed & 0xFFFFFFFF, seed >> 32], np.uint32)
+
+    def unit(hi, lo):
+        return ((int(hi) << 32 | int(lo)) >> 11) * 2.0 ** -53 + 2.0 ** -54
+
+    for i in range(3):
+        a = orc.philox(np.array([step, chain, 32 + i, 0], np.uint32), key)
+        r = np.sqrt(-2 * np.log(unit(a[0], a[1])))
+        t = 2 * np.pi * unit(a[2], a[3])
+        assert abs(z[2 * i] - r * np.cos(t)) < 1e-14 and abs(z[2 * i + 1] - r * np.sin(t)) < 1e-14
+    a = orc.philox(np.array([step, chain, 31, 0], np.uint32), key)
+    assert u == unit(a[0], a[1])
+    zs = np.array([orc.lhs_draws(7, c, t, 6)[0] for c in range(20) for t in range(100)])
+    assert abs(zs.mean()) < 0.05 and abs(zs.std() - 1.0) < 0.05
