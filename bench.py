@@ -328,6 +328,62 @@ def c4_run(args):
     }
 
 
+def lhs_run(args):
+    """Conductivity (LHS) uncertainty with every proposal re-solved
+    (ForwardGaussLHS + the laplaceInverse MCMC, re-solving): `lhs_chains` chains
+    on one n x n Q1 Poisson problem whose conductivity is the GP mean of 5
+    anchor values; each MCMC step interpolates, assembles and solves every
+    chain's field in one batched pass (one grid copy and multigrid hierarchy
+    per chain) and evaluates the 64-sensor likelihood. CPU sample: the oracle's
+    assembly and banded direct solve of a few proposals (1 thread)."""
+    from statfem_amd import configs, hip
+    from statfem_amd.lhs import ForwardGaussLHSHip
+    cfg = configs.lhs_inputs(seed=0, n=args.lhs_n)
+    m, C, steps = cfg["mesh"], args.lhs_chains, args.lhs_steps
+    ctx = hip.Context(0)
+    opts = hip.default_opts(tol=1e-10)
+    t0 = time.perf_counter()
+    fg = ForwardGaussLHSHip(ctx, m, cfg["bc"], cfg["anchors"], cfg["gp"], cfg["cols"], cfg["vals"], copies=C,
+                            nq1d=2, opts=opts)
+    t_setup = time.perf_counter() - t0
+    # synthetic readings: the forward model at the true anchor values plus noise
+    mean, _ = fg.updateSolution(np.r_[np.log(cfg["w_true"]), np.log(cfg["sigma_e"])])
+    rng = np.random.default_rng(cfg["noise_seed"])
+    Y = mean[0][:, None] + cfg["sigma_e"] * rng.standard_normal((len(cfg["cols"]), cfg["no"]))
+    fg.set_data(Y, cfg["xs"], rho=1.0, mis_sigma=0.0, mis_ell=0.3, prior_mu=cfg["prior_mu"],
+                prior_var=cfg["prior_var"])
+    th0 = np.repeat(cfg["init"][None, :], C, axis=0)
+    fg.mcmc(th0, cfg["propsd"], 2, seed=3, resolve=True, keep_samples=False)   # warm-up
+    ctx.sync()
+    t0 = time.perf_counter()
+    samples, acc, st = fg.mcmc(th0, cfg["propsd"], steps, seed=1, resolve=True)
+    t = time.perf_counter() - t0
+    N = m.nnodes
+    props = C * steps / t
+    log(f"LHS {C} chains x {steps} re-solving steps on {N}-DOF grids: {t:.2f}s ({props:.0f} proposals/s), "
+        f"{st['inner_iters']} inner iterations, acceptance {acc.mean():.2f}, precond {st['precond']}")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        k, t0 = 0, time.perf_counter()
+        while k < 2 or (time.perf_counter() - t0 < 10.0 and k < 50):
+            orc.lhs_solution(m, cfg["bc"], cfg["anchors"], np.exp(samples[k % C, k % steps, :5]), *cfg["gp"])
+            k += 1
+        tc = time.perf_counter() - t0
+        cpu = {"value": k / tc, "unit": "proposals/s", "cores": 1, "kind": "port",
+               "sample": f"{k} proposals: oracle assembly + banded LU solve of the {N}-DOF system each"}
+    fg.close()
+    return {
+        "workload": f"conductivity MCMC, {C} chains, Poisson {args.lhs_n}x{args.lhs_n} Q1 ({N} DOFs per chain), "
+                    f"GP conductivity from 5 anchors, {len(cfg['cols'])} sensors x {cfg['no']} readings, "
+                    f"every proposal re-assembled and re-solved",
+        "chains": C, "steps": steps, "s": t, "setup_s": t_setup, "proposals_per_s": props,
+        "dof_solves_per_s": props * N, "inner_iters": st["inner_iters"], "precond": st["precond"],
+        "acceptance": float(acc.mean()), "cpu_baseline": cpu,
+        "gpu_vs_cpu": props / cpu["value"] if cpu else None,
+    }
+
+
 def cpu_baseline(args, budget_s):
     """The oracle (reference algorithm restated in C++, FP64, 1 thread):
     Triplet sort-merge assembly, banded Cholesky of the eliminated operator,
@@ -381,6 +437,10 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the surrogate-MCMC leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the 3D elasticity leg")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched uneven-problems leg")
+    ap.add_argument("--no-lhs", action="store_true", help="skip the conductivity-MCMC leg")
+    ap.add_argument("--lhs-n", type=int, default=128)
+    ap.add_argument("--lhs-chains", type=int, default=64)
+    ap.add_argument("--lhs-steps", type=int, default=100)
     ap.add_argument("--c4-nprob", type=int, default=64)
     ap.add_argument("--c3-n", type=int, default=64)
     ap.add_argument("--c5-n", type=int, default=256)
@@ -408,6 +468,7 @@ def main():
     c5 = None if args.no_c5 else c5_run(args)
     c3 = None if args.no_c3 else c3_run(args)
     c4 = None if args.no_c4 else c4_run(args)
+    lhs = None if args.no_lhs else lhs_run(args)
     cpu = None if args.no_cpu else cpu_baseline(args, 30.0)
     line = {
         "metric": "assembled+solved DOF-RHS/s; MCMC ESS/s surrogate vs plain GPU at 1e-8 rel err",
@@ -434,6 +495,7 @@ def main():
         "mcmc": c5,
         "elasticity": c3,
         "batched": c4,
+        "lhs_mcmc": lhs,
     }
     print(json.dumps(line), flush=True)
 

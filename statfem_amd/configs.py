@@ -131,3 +131,27 @@ def c4_inputs(seed=0, nprob=64, nmin=32, nmax=256, smin=16, smax=128):
         probs.append(dict(n=n, mesh=m, bc=M.dirichlet_all_boundary(m), xs=xs, cols=cols, vals=vals,
                           sigma_f=0.1, ell_f=ell, reg=1e-3, sigma_e=1e-3))
     return probs
+
+
+def lhs_inputs(seed=0, n=128, ns=64, no=5):
+    """Conductivity (LHS) inverse problem in the laplaceInverse form on a 2D grid:
+    Poisson on n x n Q1 with unit body force and homogeneous Dirichlet on the
+    whole boundary; the conductivity of each element is the GP mean
+    (sigma 1, length 0.4, nugget 5e-5) at its centroid of 5 anchor values at
+    the corners of the inner square [0.2, 0.8]^2 and its centre; true anchor
+    values (0.7, 1.0, 0.7, 0.4, 0.7), `ns` interior sensors, `no` readings
+    with noise sigma_e = 0.01 drawn from the data model (the readings are made
+    by the caller from a forward solve at the true values). Log-normal priors
+    on the anchors (medians 0.8, 1.1, 0.8, 0.4, 0.8, log-variance 0.05), normal
+    prior on sigma_e (0.0075, 0.002), proposal SDs 0.02 and 0.01, as the
+    reference deck."""
+    m = M.grid2d(n, kind=M.QUAD4)
+    bc = M.dirichlet_all_boundary(m)
+    rng = np.random.default_rng(seed)
+    xs = rng.uniform(0.0, 1.0, size=(ns, 2)) * (1 - 2.0 / n) + 1.0 / n
+    cols, vals = M.sensor_rows(m, xs)
+    anchors = np.array([[0.2, 0.2], [0.8, 0.2], [0.5, 0.5], [0.2, 0.8], [0.8, 0.8]])
+    return dict(mesh=m, bc=bc, xs=xs, cols=cols, vals=vals, no=no, anchors=anchors, gp=(1.0, 0.4, 5e-5),
+                w_true=np.array([0.7, 1.0, 0.7, 0.4, 0.7]), sigma_e=0.01, noise_seed=seed + 77,
+                prior_mu=np.r_[np.log([0.8, 1.1, 0.8, 0.4, 0.8]), 0.0075], prior_var=np.r_[0.05 * np.ones(5), 0.002],
+                init=np.log([0.8, 1.1, 0.8, 0.4, 0.8, 0.0075]), propsd=np.r_[0.02 * np.ones(5), 0.01])

@@ -178,6 +178,7 @@ struct sfh_system {
     sfh::DevBuf dia32, dia64;
     int dia_state = 0;           // 0 stale, 1 valid, -1 mesh does not fit the stencil
     std::vector<int> sgrid;      // batch of 2D grids: (first row, W, H) per problem
+    sfh::DevBuf dia_tab;         // the same table on the device (stencil-copy build)
     sfh::MGHier mg;              // built with the stencil copy of declared 2D grids
     sfh::MG3Hier mg3;            // declared 3D grid
     sfh::SolveWork work;

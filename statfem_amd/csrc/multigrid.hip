@@ -113,21 +113,25 @@ __global__ void k_mg_level(int n, const double* __restrict__ A, float* __restric
 
 // Dense inverse of each grid's coarsest level, one work-group per grid (grid
 // tab[6 r ..] = {W, H, first node in the level's arrays, offset of its n x n
-// inverse in inv, offset of its 2 n^2 FP64 work area, grid index q}): the symmetrised FP64
-// matrix of the 9-point stencil, its lower Cholesky factor in place (column
-// by column, the trailing update spread over the work-group), then one thread
-// per column c solves L L^T x = e_c and writes x in FP32. bad[q] = 1 when the
-// operator is not positive definite, 0 otherwise.
+// inverse in inv, offset of its n^2 FP64 work area, grid index q}): the
+// symmetrised FP64 matrix of the 9-point stencil is inverted in place by
+// Gauss-Jordan elimination without pivoting (the pivots of an SPD matrix are
+// its positive Schur-complement diagonals), every pivot step spread over the
+// work-group; the matrix lives in LDS (LDS = true, n^2 doubles of dynamic
+// shared memory) or in the global work area. The inverse is written in FP32.
+// bad[q] = 1 when a pivot is not positive (the operator is not SPD), else 0.
+template <bool LDS>
 __global__ __launch_bounds__(256) void k_mg_coarse_inverse(const int* __restrict__ tab,
                                                            const double* __restrict__ a9,
                                                            double* __restrict__ work, float* __restrict__ inv,
                                                            int* __restrict__ bad) {
+    extern __shared__ double smem[];
+    __shared__ double col[kMGCoarse];
     const int* t = tab + 6 * blockIdx.x;
-    const int W = t[0], H = t[1], n = W * H, q = t[5];
+    const int W = t[0], n = W * t[1], q = t[5];
     const double* s9 = a9 + (size_t)t[2] * 9;
     float* out = inv + t[3];
-    double* A = work + t[4];
-    double* X = A + (size_t)n * n;
+    double* A = LDS ? smem : work + t[4];
     const int tid = threadIdx.x, bs = blockDim.x;
     for (int e = tid; e < n * n; e += bs) {
         const int i = e / n, j = e % n;
@@ -140,38 +144,29 @@ __global__ __launch_bounds__(256) void k_mg_coarse_inverse(const int* __restrict
         A[e] = v;
     }
     __syncthreads();
-    for (int j = 0; j < n; ++j) {
-        const double d = A[(size_t)j * n + j];
-        if (!(d > 0.0)) {   // the same value in every thread
+    for (int k = 0; k < n; ++k) {
+        const double p = A[(size_t)k * n + k];
+        if (!(p > 0.0)) {   // the same value in every thread
             if (tid == 0) bad[q] = 1;
             return;
         }
-        const double l = sqrt(d);
+        __syncthreads();   // every thread has read the pivot
+        for (int i = tid; i < n; i += bs) col[i] = A[(size_t)i * n + k];
         __syncthreads();
-        if (tid == 0) A[(size_t)j * n + j] = l;
-        for (int i = j + 1 + tid; i < n; i += bs) A[(size_t)i * n + j] /= l;
+        for (int j = tid; j < n; j += bs) A[(size_t)k * n + j] = j == k ? 1.0 / p : A[(size_t)k * n + j] / p;
         __syncthreads();
-        const int m = n - j - 1;   // trailing lower triangle, rows and columns j+1 .. n-1
-        for (int e = tid; e < m * m; e += bs) {
-            const int i = j + 1 + e / m, k = j + 1 + e % m;
-            if (k <= i) A[(size_t)i * n + k] -= A[(size_t)i * n + j] * A[(size_t)k * n + j];
+        const double pk = 1.0 / p;
+        // rows i from tid / 32 in steps of bs / 32, columns j from tid % 32 in steps of 32
+        for (int i = tid >> 5; i < n; i += bs >> 5) {
+            if (i == k) continue;
+            const double ci = col[i];
+            double* Ai = A + (size_t)i * n;
+            for (int j = tid & 31; j < n; j += 32) Ai[j] = j == k ? -ci * pk : Ai[j] - ci * A[(size_t)k * n + j];
         }
         __syncthreads();
     }
     if (tid == 0) bad[q] = 0;
-    for (int c = tid; c < n; c += bs) {
-        for (int i = 0; i < n; ++i) {   // L y = e_c
-            double s = i == c ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) s -= A[(size_t)i * n + k] * X[(size_t)k * n + c];
-            X[(size_t)i * n + c] = s / A[(size_t)i * n + i];
-        }
-        for (int i = n - 1; i >= 0; --i) {   // L^T x = y
-            double s = X[(size_t)i * n + c];
-            for (int k = i + 1; k < n; ++k) s -= A[(size_t)k * n + i] * X[(size_t)k * n + c];
-            X[(size_t)i * n + c] = s / A[(size_t)i * n + i];
-        }
-        for (int i = 0; i < n; ++i) out[(size_t)i * n + c] = (float)X[(size_t)i * n + c];
-    }
+    for (int e = tid; e < n * n; e += bs) out[e] = (float)A[e];
 }
 
 }  // namespace
@@ -231,7 +226,7 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
         m.cinv_off[q] = (int)ctot;
         woff[q] = wtot;
         ctot += (size_t)n * n;
-        wtot += 2 * (size_t)n * n;
+        wtot += (size_t)n * n;
         m.cmax = std::max(m.cmax, n);
     }
     if (ctot > (size_t)INT32_MAX || wtot > (size_t)INT32_MAX) { set_error("coarse levels too large"); return SFH_EARG; }
@@ -283,8 +278,17 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
     SFH_TRY(m.cbad.ensure(sizeof(int) * (size_t)ng));
     for (int l = 1; l <= m.L; ++l) {
         if (crow[l] == 0) continue;
-        k_mg_coarse_inverse<<<crow[l], 256, 0, st>>>(m.gtab.as<int>() + cofs[l], m.coef64[l].as<double>(),
-                                                     m.cwork.as<double>(), m.cinv.as<float>(), m.cbad.as<int>());
+        int nmax = 0;
+        for (int r = 0; r < crow[l]; ++r) nmax = std::max(nmax, tab[cofs[l] + 6 * r] * tab[cofs[l] + 6 * r + 1]);
+        const size_t lds = sizeof(double) * (size_t)nmax * nmax;
+        if (lds + sizeof(double) * kMGCoarse <= s->ctx->lds_per_block)
+            k_mg_coarse_inverse<true><<<crow[l], 256, lds, st>>>(m.gtab.as<int>() + cofs[l], m.coef64[l].as<double>(),
+                                                                 m.cwork.as<double>(), m.cinv.as<float>(),
+                                                                 m.cbad.as<int>());
+        else
+            k_mg_coarse_inverse<false><<<crow[l], 256, 0, st>>>(m.gtab.as<int>() + cofs[l], m.coef64[l].as<double>(),
+                                                                m.cwork.as<double>(), m.cinv.as<float>(),
+                                                                m.cbad.as<int>());
         SFH_LAUNCH_CHECK();
     }
     std::vector<int> bad(ng);

@@ -1004,8 +1004,10 @@ __global__ __launch_bounds__(kB) void k_cg_update_b3(int kp, int nnodes, int npb
 
 // CSR -> DIA-9. Every entry must couple a grid neighbour of its row's node;
 // anything else (err) keeps the solver on the CSR kernels.
-__global__ void k_build_dia9(int base, int W, int H, const int* __restrict__ rp, const int* __restrict__ ci,
+__global__ void k_build_dia9(const int* __restrict__ tab, const int* __restrict__ rp, const int* __restrict__ ci,
                              const double* __restrict__ va, float* d32, double* d64, int* err) {
+    const int* t = tab + 3 * blockIdx.y;   // grid: first row, W, H
+    const int base = t[0], W = t[1], H = t[2];
     const int loc = blockIdx.x * blockDim.x + threadIdx.x;
     if (loc >= W * H) return;
     const int nd = base + loc;
@@ -1515,11 +1517,17 @@ int prepare_operator_on(sfh_system* s, hipStream_t st, const std::vector<int>& g
         DevBuf err;
         SFH_TRY(err.ensure(sizeof(int)));
         SFH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
-        for (size_t q = 0; q < grids.size() / 3; ++q) {
-            const int base = grids[3 * q], W = grids[3 * q + 1], H = grids[3 * q + 2];
-            k_build_dia9<<<(W * H + 255) / 256, 256, 0, st>>>(base, W, H, s->rowptr.as<int>(), s->colind.as<int>(),
-                                                               s->vals.as<double>(), s->dia32.as<float>(),
-                                                               s->dia64.as<double>(), err.as<int>());
+        // every grid in one launch: work-group row blockIdx.y is grid (first row, W, H)
+        const int ng = (int)grids.size() / 3;
+        int nmax = 0;
+        for (int q = 0; q < ng; ++q) nmax = std::max(nmax, grids[3 * q + 1] * grids[3 * q + 2]);
+        SFH_TRY(s->dia_tab.ensure(sizeof(int) * grids.size()));
+        SFH_CHECK_HIP(hipMemcpyAsync(s->dia_tab.p, grids.data(), sizeof(int) * grids.size(), hipMemcpyHostToDevice, st));
+        for (int q0 = 0; q0 < ng; q0 += 65535) {
+            const int qn = std::min(65535, ng - q0);
+            k_build_dia9<<<dim3((nmax + 255) / 256, qn), 256, 0, st>>>(
+                s->dia_tab.as<int>() + 3 * (size_t)q0, s->rowptr.as<int>(), s->colind.as<int>(), s->vals.as<double>(),
+                s->dia32.as<float>(), s->dia64.as<double>(), err.as<int>());
             SFH_LAUNCH_CHECK();
         }
         int he = 0;
