@@ -154,11 +154,12 @@ int sfh_system_device_ptrs(sfh_system* sys, const int** rowptr_dev, const int** 
 typedef struct {
     double tol;          /* FP64 relative residual target per RHS (default 1e-10) */
     double inner_tol;    /* FP64 residual replacement once the FP32 residual fell by this factor
-                            (default 0: 0.01 with the Jacobi preconditioner, 1e-4 with multigrid) */
+                            (default 0: 0.01 with the Jacobi preconditioner; 1e-14 with multigrid, whose
+                            FP64 recurrences need no replacement before the target) */
     int max_outer;       /* cap on residual replacements (default 500) */
     int max_inner;       /* cap on PCG iterations (default 50000) */
     int check_every;     /* inner iterations between host convergence checks
-                            (default 0: 32 with the Jacobi preconditioner, 4 with multigrid) */
+                            (default 0: 32 with the Jacobi preconditioner, 2 with multigrid) */
     int precision;       /* 0: FP32 recurrences + FP64 replacement; 1: FP64 throughout */
     int rhs_tile;        /* >0: solve the right-hand sides in column tiles of this width (0: all at once) */
     int layout;          /* 0 (default): auto, 1: CSR, 2: 9-coefficient stencil with LDS-tiled kernels on
