@@ -50,6 +50,7 @@ typedef struct sfh_system sfh_system;
 typedef struct sfh_lik sfh_lik;
 typedef struct sfh_mlp sfh_mlp;
 typedef struct sfh_lhs sfh_lhs;
+typedef struct sfh_gpq sfh_gpq;
 
 const char* sfh_status_string(int status);
 const char* sfh_last_error(void);
@@ -421,6 +422,32 @@ int sfh_lhs_log_target(sfh_lhs* lhs, int M, const double* theta, int resolve, co
 int sfh_lhs_mcmc(sfh_lhs* lhs, int steps, const double* theta0, const double* propsd, uint64_t seed,
                  int resolve, const sfh_solve_opts* opts, double* samples, double* acc,
                  sfh_solve_stats* stats);
+
+/* ------------------------------------------ sparse-precision GP (SPDE) */
+/* GaussianProcessQ (statX/inferQ/GaussianProcessQ.hpp:65-159) on the scalar
+ * mesh of sys with the integer-beta precision of DriverMaxLikelihood
+ * (statX/inferQ/DriverMaxLikelihood.hpp:130-290):
+ *   Q = tau^2 kappa^(4 beta) (A M^-1)^(2 beta - 1) A, beta = (nu + d/2) / 2,
+ * A the Matern matrix assembled by the library (SFH_MATERN, factor 1/kappa^2,
+ * no constraints), M its lumped mass (nq1d-point rule), P the projection
+ * (ns rows of pn (column, value) pairs, host), readings Y (no x ns, host,
+ * reading-major). P_r = P_l = I (integer beta). Declare a tensor grid
+ * (sfh_system_set_grid) to solve with multigrid. */
+int sfh_gpq_create(sfh_system* sys, int nq1d, int ns, int pn, const int* pcols, const double* pvals, int no,
+                   const double* Y, sfh_gpq** out);
+int sfh_gpq_destroy(sfh_gpq* gpq);
+/* giveLogLikelihood (GaussianProcessQ.ipp:37-99) for M hyperparameter sets
+ * hps (M x 4, host: gaussianProcess.sigma, kappa, nu, eps), computed through
+ * G = P Q^-1 P^T (determinant lemma, push-through identity) from multi-RHS
+ * solves with A instead of two sparse Cholesky factorisations; sets sharing
+ * (sigma, kappa, nu) share one set of solves. out, status: M (host). */
+int sfh_gpq_loglik(sfh_gpq* gpq, int M, const double* hps, const sfh_solve_opts* opts, double* out, int* status,
+                   sfh_solve_stats* stats);
+/* predictMean / predictCovariance (GaussianProcessQ.ipp:101-186) for one set
+ * hp (4, host): mean_dev (N) and/or cov_dev (N x N, dense; N + 1 solves with
+ * N columns, for moderate N); logl (host, nullable) the log-likelihood. */
+int sfh_gpq_predict(sfh_gpq* gpq, const double* hp, const sfh_solve_opts* opts, double* mean_dev, double* cov_dev,
+                    double* logl, sfh_solve_stats* stats);
 
 #ifdef __cplusplus
 }

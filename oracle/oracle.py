@@ -425,3 +425,42 @@ def lhs_chain(theta0, propsd, steps, seed, chain, target):
         out[t, :-1] = x
         out[t, -1] = y
     return out, acc / max(steps, 1)
+
+
+# ------------------------------------- sparse-precision GP (GaussianProcessQ)
+def gpq_precision_dense(rp, ci, va, mass, sigma, kappa, nu, dim):
+    """The SPDE precision Q of PrecisionMatrixRational::givePrecisionMatrix for
+    integer beta (statX/inferQ/PrecisionMatrixRational.ipp:24-39,177-209,278-298)
+    as a dense matrix: precision_apply on the identity."""
+    n = len(rp) - 1
+    return precision_apply(rp, ci, va, mass, sigma, kappa, nu, dim, np.eye(n))
+
+
+def gpq_loglik(Q, P, Y, eps):
+    """GaussianProcessQ::giveLogLikelihood (statX/inferQ/GaussianProcessQ.ipp:37-99)
+    with P_r = I (integer beta), dense Cholesky factors in place of the sparse
+    LLT. Q: N x N, P: ns x N, Y: ns x no (the reference's obsValues layout)."""
+    ns, no = Y.shape
+    N = Q.shape[0]
+    Am = eps * eps * Q + no * P.T @ P                                  # step 1a
+    L = np.linalg.cholesky(Am)
+    s = Y.sum(axis=1)
+    fixed = P @ np.linalg.solve(L.T, np.linalg.solve(L, P.T @ s))
+    tmp1 = -0.5 * sum(Y[:, i] @ ((Y[:, i] - fixed) / (eps * eps)) for i in range(no))   # step 1b
+    tmp2a = 2.0 * np.log(np.diag(L)).sum()                            # step 2a
+    tmp2b = 2.0 * np.log(np.diag(np.linalg.cholesky(Q))).sum()        # step 2b
+    diff = ns * no - N                                                 # step 2c
+    tmp2 = -0.5 * (diff * np.log(eps * eps) + tmp2a - tmp2b)
+    tmp3 = -0.5 * no * ns * np.log(2.0 * np.pi)                       # step 3
+    return tmp1 + tmp2 + tmp3
+
+
+def gpq_predict(Q, P, Y, eps):
+    """predictMean and predictCovariance (GaussianProcessQ.ipp:101-186) with
+    P_r = I: mean (sigma_e^2 Q + n_o P^T P)^-1 P^T sum_i y_i, covariance
+    sigma_e^2 (sigma_e^2 Q + n_o P^T P)^-1."""
+    no = Y.shape[1]
+    Am = eps * eps * Q + no * P.T @ P
+    mean = np.linalg.solve(Am, P.T @ Y.sum(axis=1))
+    cov = eps * eps * np.linalg.inv(Am)
+    return mean, cov

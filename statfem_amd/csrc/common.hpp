@@ -253,6 +253,12 @@ int loglik_batch(sfh_lik* L, int M, const double* theta_dev, double* out_dev, in
 int loglik_sets(sfh_ctx* c, int ns, int no, int dim, const double* xs, const double* Cu, const double* mean,
                 int mstride, const double* Y, double eps, const double* eps_v, int M, const double* theta,
                 double* out, int* status, double* logdet, DevBuf& work);
+// the sparse-precision GP's sensor-space log-likelihood for M noise levels
+// sharing G = g_scale * Graw (ns x ns, device; Y n_o x ns; eps_dev, out,
+// status: M); wout (M x ns, nullable) receives B^-1 s
+int gpq_sensor_loglik(sfh_ctx* c, int ns, int no, const double* Graw, double g_scale, int M, const double* eps_dev,
+                      const double* Y, double* out, int* status, double* wout, DevBuf& work);
+int gpq_hmat(sfh_ctx* c, int ns, int no, const double* Graw, double g_scale, double eps, double* h);
 int dense_spd_solve(sfh_ctx* c, int n, double* A_dev, int m, double* B_dev, int* status_dev);
 int mcmc_run(sfh_lik* L, int C, int steps, const double* theta0, const double* propsd, uint64_t seed,
              int mode, const double* pm, const double* pv, double* samples, double* acc);

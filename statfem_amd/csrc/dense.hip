@@ -706,6 +706,82 @@ __global__ void k_gp_predict(int n, int dim, int np, const double* __restrict__ 
     var[p] = se_kernel(dim, xp, xp, sigma, ell) - q;
 }
 
+// ------------------------------------------------- sparse-precision GP (sensor space)
+// GaussianProcessQ::giveLogLikelihood (statX/inferQ/GaussianProcessQ.ipp:37-99)
+// for integer beta, written through G = P Q^-1 P^T (ns x ns). With
+// B = sigma_e^2 I + n_o G the matrix determinant lemma gives
+//   log|sigma_e^2 Q + n_o P^T P| - log|Q| = N log sigma_e^2 + log|B| - ns log sigma_e^2
+// and the push-through identity P (sigma_e^2 Q + n_o P^T P)^-1 P^T = G B^-1, so
+//   log p(Y) = -(|Y|^2 - s^T G B^-1 s) / (2 sigma_e^2)
+//              - (ns (n_o - 1) log sigma_e^2 + log|B|) / 2 - n_o ns log(2 pi) / 2,
+// s the sum of the readings. One work-group per set b: G = g_scale * Graw
+// (shared by the sets of one precision), sigma_e = eps[b]; Y is n_o x ns.
+__global__ __launch_bounds__(256) void k_gpq_loglik(int ns, int no, const double* __restrict__ Graw, double g_scale,
+                                                    const double* __restrict__ eps, const double* __restrict__ Y,
+                                                    double* __restrict__ work, double* __restrict__ out,
+                                                    int* __restrict__ status, double* __restrict__ wout) {
+    extern __shared__ double sh[];
+    double* col = sh;              // ns
+    double* w = sh + ns;           // ns: s, then B^-1 s
+    double* red = w + ns;          // blockDim
+    __shared__ int sflag;
+    const int b = blockIdx.x, tid = threadIdx.x, bs = blockDim.x;
+    const double e2 = eps[b] * eps[b];
+    double* a = work + (size_t)b * ns * ns;
+    for (int idx = tid; idx < ns * ns; idx += bs) {
+        const int i = idx / ns, j = idx % ns;
+        a[idx] = j <= i ? no * g_scale * Graw[idx] + (i == j ? e2 : 0.0) : 0.0;
+    }
+    double yy = 0.0;
+    for (int i = tid; i < ns; i += bs) {
+        double si = 0.0;
+        for (int k = 0; k < no; ++k) {
+            const double y = Y[(size_t)k * ns + i];
+            si += y;
+            yy += y * y;
+        }
+        w[i] = si;
+    }
+    yy = block_sum(yy, red);
+    if (!chol_block(ns, a, ns, col, &sflag)) {
+        if (tid == 0) { out[b] = -INFINITY; status[b] = SFH_ENOTSPD; }
+        return;
+    }
+    double ls = 0.0;
+    for (int i = tid; i < ns; i += bs) ls += log(a[(size_t)i * ns + i]);
+    const double logdet = 2.0 * block_sum(ls, red);
+    // s^T G B^-1 s with the unchanged s kept in col
+    for (int i = tid; i < ns; i += bs) col[i] = w[i];
+    __syncthreads();
+    forward_block(ns, a, ns, w, 1);
+    backward_block(ns, a, ns, w, 1);
+    if (wout)
+        for (int i = tid; i < ns; i += bs) wout[(size_t)b * ns + i] = w[i];
+    double q = 0.0;
+    for (int i = tid; i < ns; i += bs) {
+        double gi = 0.0;
+        for (int j = 0; j < ns; ++j) gi += Graw[(size_t)i * ns + j] * w[j];
+        q += col[i] * g_scale * gi;
+    }
+    q = block_sum(q, red);
+    if (tid == 0) {
+        const double tmp1 = -0.5 * (yy - q) / e2;
+        const double tmp2 = -0.5 * (ns * (no - 1) * log(e2) + logdet);
+        const double tmp3 = -0.5 * no * ns * log(2.0 * M_PI);
+        out[b] = tmp1 + tmp2 + tmp3;
+        status[b] = SFH_OK;
+    }
+}
+
+// h = (sigma_e^2 / n_o) I + g_scale * Graw (lower and upper), for the
+// posterior covariance's Woodbury solve
+__global__ void k_gpq_hmat(int ns, int no, const double* __restrict__ Graw, double g_scale, double eps,
+                           double* __restrict__ h) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ns * ns) return;
+    h[t] = g_scale * Graw[t] + (t / ns == t % ns ? eps * eps / no : 0.0);
+}
+
 inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -839,6 +915,22 @@ int loglik_sets(sfh_ctx* c, int ns, int no, int dim, const double* xs, const dou
 int loglik_batch(sfh_lik* L, int M, const double* theta, double* out, int* status, double* logdet) {
     return loglik_sets(L->ctx, L->ns, L->no, L->dim, L->xs.as<double>(), L->Cu.as<double>(), L->mean.as<double>(), 0,
                        L->Y.as<double>(), L->eps, nullptr, M, theta, out, status, logdet, L->work);
+}
+
+int gpq_sensor_loglik(sfh_ctx* c, int ns, int no, const double* Graw, double g_scale, int M, const double* eps_dev,
+                      const double* Y, double* out, int* status, double* wout, DevBuf& work) {
+    SFH_TRY(work.ensure((size_t)M * ns * ns * 8));
+    const size_t lds = sizeof(double) * (2 * (size_t)ns + 256);
+    if (lds > c->lds_per_block) { set_error("%d sensors too many for LDS", ns); return SFH_EARG; }
+    k_gpq_loglik<<<M, 256, lds, c->stream>>>(ns, no, Graw, g_scale, eps_dev, Y, work.as<double>(), out, status, wout);
+    SFH_LAUNCH_CHECK();
+    return SFH_OK;
+}
+
+int gpq_hmat(sfh_ctx* c, int ns, int no, const double* Graw, double g_scale, double eps, double* h) {
+    k_gpq_hmat<<<nblocks((long long)ns * ns, 256), 256, 0, c->stream>>>(ns, no, Graw, g_scale, eps, h);
+    SFH_LAUNCH_CHECK();
+    return SFH_OK;
 }
 
 int dense_spd_solve(sfh_ctx* c, int n, double* A, int m, double* B, int* status) {

@@ -1,0 +1,132 @@
+"""Sparse-precision GP (GaussianProcessQ with the integer-beta SPDE precision)
+on the GPU against the oracle's dense restatement of the reference algorithm."""
+import numpy as np
+import pytest
+
+from statfem_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(ctx, orc, dim, declare):
+    from statfem_amd import hip
+    from statfem_amd.gpq import GaussianProcessQHip
+    m = M.grid2d(20, 15) if dim == 2 else M.line1d(70)
+    rng = np.random.default_rng(dim + 10 * declare)
+    s = hip.SystemSolveHip(ctx, m)
+    if declare:
+        s.set_grid()
+    ns, no = 13, 3
+    if dim == 2:
+        xs = rng.uniform(0.05, 0.95, size=(ns, 2))
+        cols, vals = M.sensor_rows(m, xs)
+    else:
+        cols = rng.choice(m.nnodes, ns, replace=False).astype(np.int32).reshape(ns, 1)
+        vals = np.ones((ns, 1))
+    Y = 0.4 * rng.standard_normal((ns, no))
+    gp = GaussianProcessQHip(s, cols, vals, Y, nq1d=2)
+    P = M.dense_projection(m.nnodes, cols, vals)
+    return m, gp, P, Y
+
+
+def _oracle_q(orc, m, sigma, kappa, nu):
+    rp, ci, va, _ = orc.assemble(m, orc.MATERN, kappa, 1.0, nq1d=2, factor=1.0 / kappa ** 2)
+    mass = orc.body_force(m, nq1d=2, fconst=1.0)
+    return orc.gpq_precision_dense(rp, ci, va, mass, sigma, kappa, nu, m.dim)
+
+
+def _joint_loglik(orc, m, P, Y, sigma, kappa, nu, eps):
+    """log p(Y) as the density of the stacked readings (covariance
+    J (x) G + eps^2 I) with G = P Q^-1 P^T from Q^-1 = (1/c) A^-1 (M A^-1)^(2 beta - 1)
+    by dense solves with A (condition ~ cond(A), where Q's is ~ cond(A)^(2 beta))."""
+    rp, ci, va, _ = orc.assemble(m, orc.MATERN, kappa, 1.0, nq1d=2, factor=1.0 / kappa ** 2)
+    n = len(rp) - 1
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, ci[rp[i]:rp[i + 1]]] += va[rp[i]:rp[i + 1]]
+    mass = orc.body_force(m, nq1d=2, fconst=1.0)
+    beta = int(round((nu + m.dim / 2) / 2))
+    c = orc.matern_tau(sigma, kappa, nu, m.dim) ** 2 * kappa ** (4 * beta)
+    Ai = np.linalg.inv(A)
+    X = Ai.copy()
+    for _ in range(2 * beta - 1):
+        X = Ai @ (mass[:, None] * X)
+    G = P @ X @ P.T / c
+    ns, no = Y.shape
+    K = np.kron(np.ones((no, no)), G) + eps * eps * np.eye(ns * no)
+    yv = Y.T.reshape(-1)
+    _, ld = np.linalg.slogdet(K)
+    return -0.5 * (yv @ np.linalg.solve(K, yv) + ld + ns * no * np.log(2 * np.pi)), beta
+
+
+@pytest.mark.parametrize("dim,declare", [(1, 0), (2, 0), (2, 1)])
+def test_loglik_matches_oracle(ctx, orc, dim, declare):
+    """Three (sigma, kappa, nu) groups, two of them beta = 2, each with several
+    noise levels, in one call; 2D both on CSR and on the declared grid
+    (multigrid)."""
+    from statfem_amd import hip
+    m, gp, P, Y = _case(ctx, orc, dim, declare)
+    nus = (1.5, 3.5) if dim == 1 else (1.0, 3.0)
+    sets = []
+    for sigma, ell, nu in [(0.8, 0.3, nus[0]), (1.3, 0.15, nus[0]), (0.6, 0.4, nus[1])]:
+        kap = np.sqrt(2 * nu) / ell
+        for eps in (0.03, 0.1, 0.4):
+            sets.append([sigma, kap, nu, eps])
+    ll, st, stats = gp.giveLogLikelihood(sets, hip.default_opts(tol=1e-12))
+    assert np.all(st == 0)
+    if declare:
+        assert stats["precond"] == 2
+    for row, l in zip(sets, ll):
+        # the reference algorithm: two Cholesky log-determinants of Q and
+        # eps^2 Q + n_o P^T P, whose difference loses ~cond(Q) eps64 (Q's
+        # condition grows as cond(A)^(2 beta): ~1e10 at beta = 2 here)
+        Q = _oracle_q(orc, m, row[0], row[1], row[2])
+        ref = orc.gpq_loglik(Q, P, Y, row[3])
+        exact, beta = _joint_loglik(orc, m, P, Y, *row)
+        assert abs(l - exact) <= 1e-9 * abs(exact), (row, l, exact)
+        assert abs(l - ref) <= (1e-8 if beta == 1 else 1e-6) * abs(ref), (row, l, ref)
+    gp.close()
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_posterior_matches_oracle(ctx, orc, dim):
+    from statfem_amd import hip
+    m, gp, P, Y = _case(ctx, orc, dim, dim == 2)
+    nu = 1.0 if dim == 2 else 1.5
+    hp = [0.9, np.sqrt(2 * nu) / 0.25, nu, 0.08]
+    o = hip.default_opts(tol=1e-12)
+    mean = gp.predictMean(hp, o)
+    cov = gp.predictCovariance(hp, o)
+    Q = _oracle_q(orc, m, *hp[:3])
+    mref, cref = orc.gpq_predict(Q, P, Y, hp[3])
+    assert np.abs(mean - mref).max() <= 1e-8 * np.abs(mref).max()
+    assert np.abs(cov - cref).max() <= 1e-8 * np.abs(cref).max()
+    var = gp.predictVariance(hp, o)
+    assert np.abs(var - np.diag(cref)).max() <= 1e-8 * np.abs(cref).max()
+    gp.close()
+
+
+def test_driver_keys_and_fractional_beta(ctx, orc):
+    """DriverMaxLikelihood's map: kappa given directly or through the length
+    (kappa = sqrt(2 nu) / length); a fractional exponent is refused."""
+    from statfem_amd import hip
+    from statfem_amd.gpq import DriverMaxLikelihoodHip
+    m, gp, P, Y = _case(ctx, orc, 2, 1)
+    drv = DriverMaxLikelihoodHip(gp)
+    drv.setAllHps({"gaussianProcess.sigma": 0.9, "gaussianProcess.nu": 1.0, "gaussianProcess.length": 0.25,
+                   "gaussianProcess.eps": 0.05})
+    l1 = drv.computeLogLikelihood(hip.default_opts(tol=1e-12))
+    h = drv.giveAllHps()
+    h.pop("gaussianProcess.length")
+    h["gaussianProcess.kappa"] = np.sqrt(2.0) / 0.25
+    drv.setAllHps(h)
+    l2 = drv.computeLogLikelihood(hip.default_opts(tol=1e-12))
+    assert abs(l1 - l2) <= 1e-10 * abs(l1)
+    drv.addOptimHps("gaussianProcess.eps")
+    ll, st = drv.computeLogLikelihoods([[0.02], [0.05], [0.2]], hip.default_opts(tol=1e-12))
+    assert np.all(st == 0) and abs(ll[1] - l2) <= 1e-10 * abs(l2)
+    Q = _oracle_q(orc, m, 0.9, np.sqrt(2.0) / 0.25, 1.0)
+    assert abs(ll[0] - orc.gpq_loglik(Q, P, Y, 0.02)) <= 1e-8 * abs(ll[0])
+    with pytest.raises(hip.SfhError):
+        gp.giveLogLikelihood([[0.9, 5.0, 1.5, 0.05]])   # beta = 1.25 in 2D
+    gp.close()

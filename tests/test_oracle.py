@@ -438,3 +438,72 @@ def test_lhs_draws_counter_layout(orc):
     assert u == unit(a[0], a[1])
     zs = np.array([orc.lhs_draws(7, c, t, 6)[0] for c in range(20) for t in range(100)])
     assert abs(zs.mean()) < 0.05 and abs(zs.std() - 1.0) < 0.05
+
+
+# ------------------------------------------ sparse-precision GP (GaussianProcessQ)
+def _gpq_case(orc, dim=2):
+    m = M.grid2d(9, 7) if dim == 2 else M.line1d(30)
+    nu = 1.0 if dim == 2 else 1.5
+    kap = np.sqrt(2 * nu) / 0.3
+    rp, ci, va, _ = orc.assemble(m, orc.MATERN, kap, 1.0, nq1d=2, factor=1.0 / kap ** 2)
+    mass = orc.body_force(m, nq1d=2, fconst=1.0)
+    Q = orc.gpq_precision_dense(rp, ci, va, mass, 0.7, kap, nu, dim)
+    rng = np.random.default_rng(dim)
+    ns, no = 11, 4
+    P = np.zeros((ns, m.nnodes))
+    for i, j in enumerate(rng.choice(m.nnodes, ns, replace=False)):
+        P[i, j] = 1.0
+    P[3, :] = 0.0                     # one sensor interpolating between two nodes
+    P[3, [1, 2]] = [0.4, 0.6]
+    Y = 0.3 * rng.standard_normal((ns, no))
+    return Q, P, Y
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_gpq_loglik_is_the_joint_gaussian_density(orc, dim):
+    """The oracle's GaussianProcessQ log-likelihood (two Cholesky log-dets as
+    the reference) equals the density of the stacked readings under the model
+    y_i = P u + e_i with one shared u ~ N(0, Q^-1): covariance J (x) P Q^-1 P^T +
+    sigma_e^2 I, evaluated by brute force."""
+    Q, P, Y = _gpq_case(orc, dim)
+    ns, no = Y.shape
+    G = P @ np.linalg.inv(Q) @ P.T
+    for eps in (0.05, 0.2):
+        K = np.kron(np.ones((no, no)), G) + eps * eps * np.eye(ns * no)
+        yv = Y.T.reshape(-1)
+        sign, ld = np.linalg.slogdet(K)
+        ref = -0.5 * (yv @ np.linalg.solve(K, yv) + ld + ns * no * np.log(2 * np.pi))
+        assert sign > 0
+        assert abs(orc.gpq_loglik(Q, P, Y, eps) - ref) <= 1e-10 * abs(ref)
+
+
+def test_gpq_posterior_is_the_gaussian_conditional(orc):
+    """predictMean / predictCovariance against the conditional of the joint
+    Gaussian (u, y_1 .. y_no) by brute force."""
+    Q, P, Y = _gpq_case(orc, 2)
+    ns, no = Y.shape
+    eps = 0.1
+    Ci = np.linalg.inv(Q)
+    Pb = np.vstack([P] * no)
+    K = Pb @ Ci @ Pb.T + eps * eps * np.eye(ns * no)
+    mean_ref = Ci @ Pb.T @ np.linalg.solve(K, Y.T.reshape(-1))
+    cov_ref = Ci - Ci @ Pb.T @ np.linalg.solve(K, Pb @ Ci)
+    mean, cov = orc.gpq_predict(Q, P, Y, eps)
+    assert np.abs(mean - mean_ref).max() <= 1e-10 * np.abs(mean_ref).max()
+    assert np.abs(cov - cov_ref).max() <= 1e-10 * np.abs(cov_ref).max()
+
+
+def test_precision_beta2_is_matern_7_2(orc):
+    """beta = 2 (nu = 7/2 in 1D, Q = c (A M^-1)^3 A): the covariance Q^-1 is the
+    Matern-7/2 kernel sigma^2 e^-a (1 + a + 2a^2/5 + a^3/15), a = sqrt(7) r / ell,
+    at the interior (variance and three lags)."""
+    sigma, nu, ell = 0.7, 3.5, 0.05
+    m, kap, rp, ci, va, mass = _matern_1d(orc, n=400, ell=ell, sigma=sigma, nu=nu)
+    Q = orc.precision_apply(rp, ci, va, mass, sigma, kap, nu, 1, np.eye(m.nnodes))
+    Cm = np.linalg.inv(Q)
+    c = m.nnodes // 2
+    x = m.coords[:, 0]
+    for k in (0, 10, 20, 40):
+        a = np.sqrt(7) * abs(x[c + k] - x[c]) / ell
+        ref = sigma ** 2 * np.exp(-a) * (1 + a + 0.4 * a * a + a ** 3 / 15)
+        assert abs(Cm[c, c + k] - ref) < 0.003 * sigma ** 2, (k, Cm[c, c + k], ref)
