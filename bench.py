@@ -384,6 +384,67 @@ def lhs_run(args):
     }
 
 
+def gpq_run(args):
+    """Sparse-precision SPDE GP likelihood surface (GaussianProcessQ through
+    DriverMaxLikelihood, nu = 1, beta = 1): a 256 x 256 Q1 grid (66,049
+    nodes), 200 interior sensors, 8 readings drawn from the model; 8 length
+    scales x 16 noise levels = 128 log-likelihood evaluations in one call
+    (the noise levels of one length scale share its 200-column solve). CPU
+    sample: the same sensor-space computation for one length scale with the
+    oracle's banded LU of A (1 thread)."""
+    from statfem_amd import hip, mesh as M
+    from statfem_amd.gpq import GaussianProcessQHip
+    n, ns, no = args.gpq_n, 200, 8
+    m = M.grid2d(n)
+    rng = np.random.default_rng(21)
+    xs = rng.uniform(0.02, 0.98, size=(ns, 2))
+    cols, vals = M.sensor_rows(m, xs)
+    Y = np.sin(3 * xs[:, :1]) * np.cos(2 * xs[:, 1:]) + 0.05 * rng.standard_normal((ns, no))
+    ctx = hip.Context(0)
+    s = hip.SystemSolveHip(ctx, m)
+    s.set_grid()
+    gp = GaussianProcessQHip(s, cols, vals, Y, nq1d=2)
+    nu, sigma = 1.0, 1.0
+    ells = np.geomspace(0.05, 0.6, 8)
+    epss = np.geomspace(0.01, 0.5, 16)
+    sets = [[sigma, np.sqrt(2 * nu) / ell, nu, e] for ell in ells for e in epss]
+    opts = hip.default_opts(tol=1e-10)
+    gp.giveLogLikelihood(sets[:16], opts)     # warm-up
+    ctx.sync()
+    t0 = time.perf_counter()
+    ll, st, stats = gp.giveLogLikelihood(sets, opts)
+    t = time.perf_counter() - t0
+    best = int(np.argmax(ll))
+    log(f"GPQ {m.nnodes} nodes, {ns} sensors x {no}: {len(sets)} log-likelihoods in {t:.3f}s "
+        f"({stats['inner_iters']} inner iterations, precond {stats['precond']}); best ell "
+        f"{ells[best // 16]:.3f}, eps {epss[best % 16]:.3f}")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        t0 = time.perf_counter()
+        kap = np.sqrt(2 * nu) / ells[0]
+        rp, ci, va, _ = orc.assemble(m, orc.MATERN, kap, 1.0, nq1d=2, factor=1.0 / kap ** 2)
+        mass = orc.body_force(m, nq1d=2, fconst=1.0)
+        P = M.dense_projection(m.nnodes, cols, vals)
+        X = orc.LU(rp, ci, va).solve(P.T.copy())
+        G = X.T @ (mass[:, None] * X) / (orc.matern_tau(sigma, kap, nu, 2) ** 2 * kap ** 4)
+        for e in epss:
+            B = e * e * np.eye(ns) + no * G
+            np.linalg.slogdet(B)
+            np.linalg.solve(B, Y.sum(axis=1))
+        tc = time.perf_counter() - t0
+        cpu = {"value": len(epss) / tc, "unit": "log-likelihoods/s", "cores": 1, "kind": "port",
+               "sample": f"one length scale (16 noise levels): oracle assembly + banded LU + {ns} solves on "
+                         f"{m.nnodes} nodes + sensor-space algebra, {tc:.1f}s"}
+    gp.close()
+    return {
+        "workload": f"SPDE GP (nu = 1) on {n}x{n} Q1 ({m.nnodes} nodes), {ns} sensors x {no} readings, "
+                    f"{len(sets)} log-likelihoods (8 length scales x 16 noise levels)",
+        "s": t, "loglik_per_s": len(sets) / t, "inner_iters": stats["inner_iters"], "precond": stats["precond"],
+        "cpu_baseline": cpu, "gpu_vs_cpu": (len(sets) / t) / cpu["value"] if cpu else None,
+    }
+
+
 def cpu_baseline(args, budget_s):
     """The oracle (reference algorithm restated in C++, FP64, 1 thread):
     Triplet sort-merge assembly, banded Cholesky of the eliminated operator,
@@ -438,6 +499,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 3D elasticity leg")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched uneven-problems leg")
     ap.add_argument("--no-lhs", action="store_true", help="skip the conductivity-MCMC leg")
+    ap.add_argument("--no-gpq", action="store_true", help="skip the sparse-precision GP leg")
+    ap.add_argument("--gpq-n", type=int, default=256)
     ap.add_argument("--lhs-n", type=int, default=128)
     ap.add_argument("--lhs-chains", type=int, default=64)
     ap.add_argument("--lhs-steps", type=int, default=100)
@@ -469,6 +532,7 @@ def main():
     c3 = None if args.no_c3 else c3_run(args)
     c4 = None if args.no_c4 else c4_run(args)
     lhs = None if args.no_lhs else lhs_run(args)
+    gpq = None if args.no_gpq else gpq_run(args)
     cpu = None if args.no_cpu else cpu_baseline(args, 30.0)
     line = {
         "metric": "assembled+solved DOF-RHS/s; MCMC ESS/s surrogate vs plain GPU at 1e-8 rel err",
@@ -496,6 +560,7 @@ def main():
         "elasticity": c3,
         "batched": c4,
         "lhs_mcmc": lhs,
+        "spde_gp": gpq,
     }
     print(json.dumps(line), flush=True)
 
