@@ -325,7 +325,7 @@ int sfh_matern_tau(double sigma, double kappa, double nu, int dim, double* tau);
  * (statX/inferQ/PrecisionMatrixRational.ipp:24-39,177-209,278-298):
  * Y = tau^2 kappa^{4 beta} (A M^-1)^{2 alpha - 1} A X, alpha = max(1, floor(beta)),
  * A the system's assembled Matern matrix (M + kappa^-2 G: SFH_MATERN with
- * factor 1/kappa^2, DriverMaxLikelihood.hpp:199-229), mass_dev the lumped mass.
+ * factor 1/kappa^2, DriverMaxLikelihood.hpp:202-230), mass_dev the lumped mass.
  * Fractional beta returns SFH_EARG (the reference needs the baryrat package). */
 int sfh_precision_apply(sfh_system* sys, const double* mass_dev, double sigma, double kappa,
                         double nu, int k, const double* X_dev, double* Y_dev);
@@ -426,7 +426,7 @@ int sfh_lhs_mcmc(sfh_lhs* lhs, int steps, const double* theta0, const double* pr
 /* ------------------------------------------ sparse-precision GP (SPDE) */
 /* GaussianProcessQ (statX/inferQ/GaussianProcessQ.hpp:65-159) on the scalar
  * mesh of sys with the integer-beta precision of DriverMaxLikelihood
- * (statX/inferQ/DriverMaxLikelihood.hpp:130-290):
+ * (statX/inferQ/DriverMaxLikelihood.hpp:202-312):
  *   Q = tau^2 kappa^(4 beta) (A M^-1)^(2 beta - 1) A, beta = (nu + d/2) / 2,
  * A the Matern matrix assembled by the library (SFH_MATERN, factor 1/kappa^2,
  * no constraints), M its lumped mass (nq1d-point rule), P the projection

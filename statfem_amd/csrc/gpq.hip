@@ -1,6 +1,6 @@
 // GaussianProcessQ (statX/inferQ/GaussianProcessQ.hpp:65-159,
 // GaussianProcessQ.ipp:37-186) with the integer-beta SPDE precision that
-// DriverMaxLikelihood assembles (statX/inferQ/DriverMaxLikelihood.hpp:130-290,
+// DriverMaxLikelihood assembles (statX/inferQ/DriverMaxLikelihood.hpp:202-312,
 // PrecisionMatrixRational.ipp:24-39,177-209,278-298):
 //   Q = c (A M^-1)^(2 beta - 1) A,  c = tau^2 kappa^(4 beta),
 // A the Matern matrix M_c + kappa^-2 S (SFH_MATERN with factor 1/kappa^2), M
@@ -159,7 +159,7 @@ int sfh_gpq_create(sfh_system* s, int nq1d, int ns, int pn, const int* pcols, co
             (rc = g->T.ensure(sizeof(double) * (size_t)N * ns)) || (rc = g->G.ensure(sizeof(double) * (size_t)ns * ns)) ||
             (rc = pc.ensure(sizeof(int) * (size_t)ns * pn)) || (rc = pv.ensure(sizeof(double) * (size_t)ns * pn)))
             break;
-        // the lumped mass (DriverMaxLikelihood::computeLumpedMassVector_, :231-258)
+        // the lumped mass (DriverMaxLikelihood::computeLumpedMassVector_, DriverMaxLikelihood.hpp:234-262)
         if ((rc = assemble_rhs(s, nq1d, one, nullptr, 1.0, 0, g->mass.as<double>()))) break;
         if (hipMemsetAsync(g->PT.p, 0, sizeof(double) * (size_t)N * ns, st) != hipSuccess ||
             hipMemcpyAsync(pc.p, pcols, sizeof(int) * ns * pn, hipMemcpyHostToDevice, st) != hipSuccess ||

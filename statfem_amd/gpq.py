@@ -1,7 +1,7 @@
 """Sparse-precision (SPDE) Gaussian-process regression on the GPU:
 GaussianProcessQ (statX/inferQ/GaussianProcessQ.hpp:65-159,
 GaussianProcessQ.ipp:37-186) with the integer-beta precision that
-DriverMaxLikelihood assembles (statX/inferQ/DriverMaxLikelihood.hpp:84-290).
+DriverMaxLikelihood assembles (statX/inferQ/DriverMaxLikelihood.hpp:52-313).
 
 The likelihood and posterior go through the sensor-space matrix
 G = P Q^-1 P^T, formed from multi-right-hand-side solves with the Matern
@@ -89,7 +89,7 @@ class GaussianProcessQHip:
 
 
 class DriverMaxLikelihoodHip:
-    """DriverMaxLikelihood (statX/inferQ/DriverMaxLikelihood.hpp:84-290) over a
+    """DriverMaxLikelihood (statX/inferQ/DriverMaxLikelihood.hpp:52-313) over a
     hyperparameter map with the reference's keys: gaussianProcess.sigma, .nu,
     .kappa (or .length, kappa = sqrt(2 nu) / length, :270-290) and .eps."""
 
