@@ -91,7 +91,7 @@ class GaussianProcessQHip:
 class DriverMaxLikelihoodHip:
     """DriverMaxLikelihood (statX/inferQ/DriverMaxLikelihood.hpp:52-313) over a
     hyperparameter map with the reference's keys: gaussianProcess.sigma, .nu,
-    .kappa (or .length, kappa = sqrt(2 nu) / length, :270-290) and .eps."""
+    .kappa (or .length, kappa = sqrt(2 nu) / length, :288-312) and .eps."""
 
     def __init__(self, gp):
         self.gp = gp
