@@ -45,3 +45,41 @@ def test_cpp_dropin_header_compiles_and_client_is_built():
     assert r.returncode == 0, r.stderr
     from statfem_amd import build
     assert os.path.exists(build.CPP_CLIENT)
+
+
+def test_vtu_writer_round_trip(tmp_path):
+    """The VTU output parses as XML with the reference's layout: padded
+    coordinates, connectivity in element order, offsets, VTK types, 2D point
+    vectors padded to 3 components, cell data."""
+    import xml.etree.ElementTree as ET
+    import numpy as np
+    from statfem_amd import io, mesh as M
+    m = M.grid2d(3, 2)
+    u = np.arange(m.nnodes, dtype=float)
+    g = np.stack([u, -u], axis=1)
+    k = np.linspace(1, 2, m.nel)
+    path = tmp_path / "out.vtu"
+    io.write_vtu(str(path), m, point_data={"u": u, "grad": g}, cell_data={"kappa": k})
+    root = ET.parse(str(path)).getroot()
+    piece = root.find("UnstructuredGrid/Piece")
+    assert int(piece.get("NumberOfPoints")) == m.nnodes and int(piece.get("NumberOfCells")) == m.nel
+
+    def arr(node, name):
+        for d in node.iter("DataArray"):
+            if d.get("Name") == name:
+                return d, np.array(d.text.split(), float)
+        raise KeyError(name)
+
+    d, xyz = arr(piece, "Coordinates")
+    assert d.get("NumberOfComponents") == "3"
+    assert np.allclose(xyz.reshape(-1, 3)[:, :2], m.coords) and np.all(xyz.reshape(-1, 3)[:, 2] == 0)
+    _, c = arr(piece, "connectivity")
+    assert np.array_equal(c.reshape(m.nel, 4), m.conn)
+    _, off = arr(piece, "offsets")
+    assert np.array_equal(off, 4 * np.arange(1, m.nel + 1))
+    _, ty = arr(piece, "types")
+    assert np.all(ty == 9)
+    d, gg = arr(piece.find("PointData"), "grad")
+    assert d.get("NumberOfComponents") == "3" and np.allclose(gg.reshape(-1, 3)[:, :2], g)
+    _, kk = arr(piece.find("CellData"), "kappa")
+    assert np.allclose(kk, k)
