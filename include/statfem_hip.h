@@ -51,6 +51,7 @@ typedef struct sfh_lik sfh_lik;
 typedef struct sfh_mlp sfh_mlp;
 typedef struct sfh_lhs sfh_lhs;
 typedef struct sfh_gpq sfh_gpq;
+typedef struct sfh_fpert sfh_fpert;
 
 const char* sfh_status_string(int status);
 const char* sfh_last_error(void);
@@ -448,6 +449,28 @@ int sfh_gpq_loglik(sfh_gpq* gpq, int M, const double* hps, const sfh_solve_opts*
  * N columns, for moderate N); logl (host, nullable) the log-likelihood. */
 int sfh_gpq_predict(sfh_gpq* gpq, const double* hp, const sfh_solve_opts* opts, double* mean_dev, double* cov_dev,
                     double* logl, sfh_solve_stats* stats);
+
+/* ------------------------------------ first-order perturbation (LHS + RHS) */
+/* ForwardPerturbation (statX/infer/ForwardPerturbation.hpp:43-215) on the
+ * assembled and constrained system sys (its right-hand side is f): the
+ * projection P (ns rows of pn (column, value) pairs, host), nk material
+ * parameters with their stiffness derivatives D_e = dA/dk_e as dense
+ * nb x nb blocks on nb DOFs each (dofs: nk x nb, blk: nk x nb x nb, host;
+ * setStiffnessDerivatives, :84-89) and their coordinates for the
+ * CovarianceMaterial kernel (pcoords: nk x pdim, CovarianceMaterial.hpp:85-101),
+ * reg_factor of the constructor (:66-70). nk nb <= 65536; dense N x N and
+ * (nk nb)^2 work matrices, so for moderate meshes. */
+int sfh_fpert_create(sfh_system* sys, int nq1d, int ns, int pn, const int* pcols, const double* pvals, int nk, int nb,
+                     const int* dofs, const double* blk, int pdim, const double* pcoords, double reg_factor,
+                     sfh_fpert** out);
+int sfh_fpert_destroy(sfh_fpert* fp);
+/* updateSolution (:108-190) with forcing.sigma/length and material.sigma/
+ * length: mean = P u (ns, host), cov = R C_f R^T + sum_ed C_k[e,d] R D_e
+ * (A^-1 C_f A^-T + u u^T) D_d^T R^T + (reg sigma_k)^2 P P^T (ns x ns, host),
+ * with the full A^-1 (the reference fills n_s rows of it; equal when
+ * n_s = N). u_dev (N, nullable) receives the FE solution. */
+int sfh_fpert_update(sfh_fpert* fp, double sigma_f, double ell_f, double sigma_k, double ell_k,
+                     const sfh_solve_opts* opts, double* mean, double* cov, double* u_dev, sfh_solve_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -464,3 +464,39 @@ def gpq_predict(Q, P, Y, eps):
     mean = np.linalg.solve(Am, P.T @ Y.sum(axis=1))
     cov = eps * eps * np.linalg.inv(Am)
     return mean, cov
+
+
+# ------------------------------------------ first-order perturbation (ForwardPerturbation)
+def fpert_update(rp, ci, va, F, coords, covvec, sigma_f, ell_f, P, ddofs, dblock, pcoords, sigma_k, ell_k,
+                 reg_factor=1e-4):
+    """ForwardPerturbation::updateSolution (statX/infer/ForwardPerturbation.hpp:108-190)
+    with dense matrices: u = A^-1 F, R = P A^-1, the forcing part
+    A^-1 C_f A^-T + u u^T with the full inverse (the reference fills the first
+    n_s rows of its aInv, lines 127-141: equal when n_s = N), the material
+    covariance C_k = k(x_e, x_d) of CovarianceMaterial (CovarianceMaterial.hpp:85-101)
+    and the stiffness derivatives D_e = sum_ab dblock[e][a][b] e_{ddofs[e][a]} e_{ddofs[e][b]}^T:
+      cov = R C_f R^T + sum_e sum_d C_k[e, d] R D_e (A^-1 C_f A^-T + u u^T) D_d^T R^T
+            + (reg_factor sigma_k)^2 P P^T.
+    Returns (P u, cov, u)."""
+    lu = LU(rp, ci, va)
+    u = lu.solve(F)
+    N = len(u)
+    R = lu.solve(np.ascontiguousarray(P.T), trans=True).T              # rows A^-T p_i
+    Ainv = lu.solve(np.eye(N), trans=True).T                           # rows A^-T e_i = rows of A^-1
+    force = Ainv @ covforcing_apply(coords, covvec, sigma_f, ell_f, np.ascontiguousarray(Ainv.T)) \
+        + np.outer(u, u)
+    pc = np.asarray(pcoords, np.float64).reshape(len(ddofs), -1)
+    Ck = mismatch_cov(pc, sigma_k, ell_k, 0.0)
+    ns = P.shape[0]
+    RD, GE = [], []
+    for e in range(len(ddofs)):
+        De = np.zeros((N, N))
+        De[np.ix_(ddofs[e], ddofs[e])] += dblock[e]
+        RD.append(R @ De)
+        GE.append(RD[-1] @ force)
+    cov_m = np.zeros((ns, ns))
+    for e in range(len(ddofs)):
+        for d in range(len(ddofs)):
+            cov_m += Ck[e, d] * GE[e] @ RD[d].T
+    cov = covforcing_project(coords, covvec, sigma_f, ell_f, R) + cov_m + (reg_factor * sigma_k) ** 2 * P @ P.T
+    return P @ u, cov, u

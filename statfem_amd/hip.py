@@ -129,6 +129,9 @@ _SIGS = {
     "sfh_gpq_destroy": (i32, [v]),
     "sfh_gpq_loglik": (i32, [v, i32, v, C.POINTER(SolveOpts), v, v, C.POINTER(SolveStats)]),
     "sfh_gpq_predict": (i32, [v, v, C.POINTER(SolveOpts), v, v, C.POINTER(dbl), C.POINTER(SolveStats)]),
+    "sfh_fpert_create": (i32, [v, i32, i32, i32, v, v, i32, i32, v, v, i32, v, dbl, C.POINTER(v)]),
+    "sfh_fpert_destroy": (i32, [v]),
+    "sfh_fpert_update": (i32, [v, dbl, dbl, dbl, dbl, C.POINTER(SolveOpts), v, v, v, C.POINTER(SolveStats)]),
 }
 
 

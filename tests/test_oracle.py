@@ -507,3 +507,77 @@ def test_precision_beta2_is_matern_7_2(orc):
         a = np.sqrt(7) * abs(x[c + k] - x[c]) / ell
         ref = sigma ** 2 * np.exp(-a) * (1 + a + 0.4 * a * a + a ** 3 / 15)
         assert abs(Cm[c, c + k] - ref) < 0.003 * sigma ** 2, (k, Cm[c, c + k], ref)
+
+
+# --------------------------------------------- first-order perturbation (ForwardPerturbation)
+def _fpert_case(orc, nx=5, ny=4):
+    m = M.grid2d(nx, ny)
+    bc = M.dirichlet_all_boundary(m)
+    kap = 1.0 + 0.3 * np.sin(np.arange(m.nel))
+    F = orc.body_force(m, nq1d=2, fconst=1.0)
+    rp, ci, va, F = orc.assemble(m, orc.POISSON, kap, nq1d=2, bc_dofs=bc, F=F)
+    # stiffness derivative of each element's conductivity: the unit-conductivity
+    # element matrix on its nodes, zero on constrained rows (row-zero form)
+    Ke = orc.element_matrices(m, orc.POISSON, np.ones(m.nel), nq1d=2)
+    mask = np.zeros(m.nnodes, bool)
+    mask[bc] = True
+    ddofs = m.conn.astype(np.int32)
+    dblock = np.array([np.where(mask[m.conn[e]][:, None], 0.0, Ke[e]) for e in range(m.nel)])
+    cen = m.coords[m.conn].mean(axis=1)
+    rng = np.random.default_rng(2)
+    ns = 6
+    P = np.zeros((ns, m.nnodes))
+    P[np.arange(ns), rng.choice(np.setdiff1d(np.arange(m.nnodes), bc), ns, replace=False)] = 1.0
+    cv = orc.body_force(m, nq1d=2, fconst=1.0)
+    cv[bc] = 0.0
+    return m, bc, kap, rp, ci, va, F, P, ddofs, dblock, cen, cv
+
+
+def test_fpert_material_term_is_the_linearised_covariance(orc):
+    """Without forcing covariance, the material part is the first-order
+    covariance of P u(kappa) under kappa ~ N(kappa_bar, C_k): J C_k J^T with
+    J = dPu/dkappa from central finite differences of the forward solve."""
+    m, bc, kap, rp, ci, va, F, P, ddofs, dblock, cen, cv = _fpert_case(orc)
+    mean, cov, u = orc.fpert_update(rp, ci, va, F, m.coords, cv, 0.0, 0.3, P, ddofs, dblock, cen, 0.2, 0.4,
+                                    reg_factor=0.0)
+    J = np.zeros((P.shape[0], m.nel))
+    f0 = orc.body_force(m, nq1d=2, fconst=1.0)
+    for e in range(m.nel):
+        us = []
+        for sgn in (1, -1):
+            k2 = kap.copy()
+            k2[e] += sgn * 1e-4
+            r2, c2, v2, F2 = orc.assemble(m, orc.POISSON, k2, nq1d=2, bc_dofs=bc, F=f0.copy())
+            us.append(P @ orc.LU(r2, c2, v2).solve(F2))
+        J[:, e] = (us[0] - us[1]) / 2e-4
+    Ck = orc.mismatch_cov(cen, 0.2, 0.4, 0.0)
+    ref = J @ Ck @ J.T
+    assert np.abs(cov - ref).max() <= 1e-6 * np.abs(ref).max()
+    assert np.abs(mean - P @ u).max() == 0.0
+
+
+def test_fpert_forcing_terms_by_factorised_forcing(orc):
+    """The forcing parts through a factor of C_f (C_f = L L^T, A^-1 L by direct
+    solves): R C_f R^T + sum C_k R D_e A^-1 L (R D_d A^-1 L)^T. Distinguishes A^-1
+    from A^-T (the row-zero operator is not symmetric)."""
+    m, bc, kap, rp, ci, va, F, P, ddofs, dblock, cen, cv = _fpert_case(orc)
+    sf, lf, sk, lk = 0.5, 0.3, 0.2, 0.4
+    _, cov, u = orc.fpert_update(rp, ci, va, F, m.coords, cv, sf, lf, P, ddofs, dblock, cen, sk, lk, reg_factor=0.0)
+    N = m.nnodes
+    Cf = orc.covforcing_apply(m.coords, cv, sf, lf, np.eye(N))
+    w, V = np.linalg.eigh(0.5 * (Cf + Cf.T))
+    L = V * np.sqrt(np.clip(w, 0, None))
+    lu = orc.LU(rp, ci, va)
+    AL = lu.solve(L)
+    R = lu.solve(np.ascontiguousarray(P.T), trans=True).T
+    Ck = orc.mismatch_cov(cen, sk, lk, 0.0)
+    B = []
+    for e in range(m.nel):
+        De = np.zeros((N, N))
+        De[np.ix_(ddofs[e], ddofs[e])] += dblock[e]
+        B.append(np.hstack([R @ De @ AL, (R @ De @ u)[:, None]]))
+    ref = (R @ L) @ (R @ L).T
+    for e in range(m.nel):
+        for d in range(m.nel):
+            ref += Ck[e, d] * B[e] @ B[d].T
+    assert np.abs(cov - ref).max() <= 1e-10 * np.abs(ref).max()
