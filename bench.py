@@ -445,6 +445,74 @@ def gpq_run(args):
     }
 
 
+def fpert_run(args):
+    """First-order perturbation (ForwardPerturbation) with an element-wise
+    conductivity: Poisson on an n x n Q1 grid with homogeneous Dirichlet,
+    one material parameter per element (n^2 parameters, their stiffness
+    derivatives and centroids), 32 sensors, SE forcing covariance; one
+    updateSolution (u, A^-1 by one N-column solve, F = A^-1 C_f A^-T + u u^T,
+    the structured product over parameter pairs, R C_f R^T). CPU sample: the
+    oracle on a 16 x 16 grid (256 parameters), with the GPU timed there too."""
+    from statfem_amd import hip, mesh as M
+    from statfem_amd.fpert import ForwardPerturbationHip, conductivity_derivatives
+    prm = {"forcing.sigma": 0.3, "forcing.length": 0.2, "material.sigma": 0.2, "material.length": 0.25}
+    ctx = hip.Context(0)
+    opts = hip.default_opts(tol=1e-10)
+
+    def build(n):
+        m = M.grid2d(n)
+        bc = M.dirichlet_all_boundary(m)
+        kap = np.exp(0.3 * np.sin(7 * m.coords[m.conn].mean(axis=1)[:, 0]))
+        s = hip.SystemSolveHip(ctx, m)
+        s.set_grid()
+        s.assemble(hip.POISSON, kap, nq1d=2)
+        s.assemble_rhs(nq1d=2, fconst=1.0)
+        s.applyConstraints(bc)
+        rng = np.random.default_rng(3)
+        xs = rng.uniform(0.05, 0.95, size=(32, 2))
+        cols, vals = M.sensor_rows(m, xs)
+        dofs, blocks = conductivity_derivatives(m, bc)
+        fp = ForwardPerturbationHip(s, cols, vals, dofs, blocks, m.coords[m.conn].mean(axis=1))
+        return m, bc, kap, s, cols, vals, dofs, blocks, fp
+
+    def timed(fp, reps):
+        fp.updateSolution(prm, opts)              # warm-up
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fp.updateSolution(prm, opts)
+        return (time.perf_counter() - t0) / reps
+
+    m, bc, kap, s, cols, vals, dofs, blocks, fp = build(args.fpert_n)
+    t = timed(fp, 3)
+    fp.close()
+    ms, bcs, kaps, ss, cs, vs, ds, bs, fps = build(16)
+    ts = timed(fps, 5)
+    fps.close()
+    log(f"FPERT {m.nnodes} nodes, {m.nel} parameters: {1e3 * t:.1f} ms per update; 16^2: {1e3 * ts:.2f} ms")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        F = orc.body_force(ms, nq1d=2, fconst=1.0)
+        rp, ci, va, F = orc.assemble(ms, orc.POISSON, kaps, nq1d=2, bc_dofs=bcs, F=F)
+        cv = orc.body_force(ms, nq1d=2, fconst=1.0)
+        cv[bcs] = 0.0
+        P = M.dense_projection(ms.nnodes, cs, vs)
+        t0 = time.perf_counter()
+        orc.fpert_update(rp, ci, va, F, ms.coords, cv, prm["forcing.sigma"], prm["forcing.length"], P, ds, bs,
+                         ms.coords[ms.conn].mean(axis=1), prm["material.sigma"], prm["material.length"])
+        tc = time.perf_counter() - t0
+        cpu = {"value": 1.0 / tc, "unit": "updates/s", "cores": 1, "kind": "port",
+               "sample": f"oracle updateSolution on 16^2 Q1 (289 nodes, 256 parameters): {tc:.2f}s; the GPU at the "
+                         f"same size: {1e3 * ts:.2f} ms"}
+    return {
+        "workload": f"ForwardPerturbation on {args.fpert_n}x{args.fpert_n} Q1 ({m.nnodes} nodes), {m.nel} element "
+                    f"conductivities, 32 sensors",
+        "ms_per_update": 1e3 * t, "updates_per_s": 1.0 / t, "ms_per_update_16": 1e3 * ts, "cpu_baseline": cpu,
+        "gpu_vs_cpu_16": (1.0 / ts) / cpu["value"] if cpu else None,
+    }
+
+
 def cpu_baseline(args, budget_s):
     """The oracle (reference algorithm restated in C++, FP64, 1 thread):
     Triplet sort-merge assembly, banded Cholesky of the eliminated operator,
@@ -501,6 +569,8 @@ def main():
     ap.add_argument("--no-lhs", action="store_true", help="skip the conductivity-MCMC leg")
     ap.add_argument("--no-gpq", action="store_true", help="skip the sparse-precision GP leg")
     ap.add_argument("--gpq-n", type=int, default=256)
+    ap.add_argument("--no-fpert", action="store_true", help="skip the first-order perturbation leg")
+    ap.add_argument("--fpert-n", type=int, default=64)
     ap.add_argument("--lhs-n", type=int, default=128)
     ap.add_argument("--lhs-chains", type=int, default=64)
     ap.add_argument("--lhs-steps", type=int, default=100)
@@ -533,6 +603,7 @@ def main():
     c4 = None if args.no_c4 else c4_run(args)
     lhs = None if args.no_lhs else lhs_run(args)
     gpq = None if args.no_gpq else gpq_run(args)
+    fpert = None if args.no_fpert else fpert_run(args)
     cpu = None if args.no_cpu else cpu_baseline(args, 30.0)
     line = {
         "metric": "assembled+solved DOF-RHS/s; MCMC ESS/s surrogate vs plain GPU at 1e-8 rel err",
@@ -561,6 +632,7 @@ def main():
         "batched": c4,
         "lhs_mcmc": lhs,
         "spde_gp": gpq,
+        "perturbation": fpert,
     }
     print(json.dumps(line), flush=True)
 
