@@ -27,7 +27,9 @@
 struct sfh_gpq {
     sfh_system* sys = nullptr;
     int nq1d = 2, ns = 0, no = 0, N = 0;
+    double yy = 0.0, ss = 0.0;   // sum of squared readings, |s|^2 (s: the readings' sum)
     sfh::DevBuf PT, mass, Y, X, T, G, eps, out, st, work, w, v;
+    sfh::DevBuf srow, zeros, th3, ld;   // inputs of the register-resident factorisation
 };
 
 namespace sfh {
@@ -64,6 +66,35 @@ __global__ void k_gpq_transpose(int rows, int cols, const double* __restrict__ i
 }
 
 inline unsigned nb(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// rho = sqrt(n_o g_scale), no mismatch kernel: the register factorisation then
+// works on B = sigma_e^2 I + n_o G with the data row s
+__global__ void k_gpq_theta(int m, double rho, double* __restrict__ t3) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m) return;
+    t3[3 * b] = rho;
+    t3[3 * b + 1] = 0.0;
+    t3[3 * b + 2] = 1.0;
+}
+
+// log p(Y) from log|B| and the factorisation's value ll = -ns log(2 pi)/2 -
+// log|B|/2 - |L^-1 s|^2/2: s^T B^-1 s = |L^-1 s|^2 and s^T G B^-1 s =
+// (|s|^2 - sigma_e^2 s^T B^-1 s) / n_o. A non-negative quadratic form flag is
+// not a failure here (s may vanish).
+__global__ void k_gpq_combine(int m, int ns, int no, double yy, double ss, const double* __restrict__ eps,
+                              const double* ll, const double* __restrict__ logdet, double* out,
+                              int* __restrict__ st) {   // ll and out may be one buffer
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m) return;
+    if (st[b] == SFH_ENOTSPD) { out[b] = -INFINITY; return; }
+    const double e2 = eps[b] * eps[b];
+    const double ld = logdet[b];
+    const double qs = -2.0 * (ll[b] + 0.5 * ns * log(2.0 * M_PI) + 0.5 * ld);
+    const double tmp1 = -0.5 * (yy - (ss - e2 * qs) / no) / e2;
+    const double tmp2 = -0.5 * (ns * (no - 1) * log(e2) + ld);
+    out[b] = tmp1 + tmp2 - 0.5 * no * ns * log(2.0 * M_PI);
+    st[b] = SFH_OK;
+}
 
 struct Prec {
     double scale = 0.0;   // 1 / c
@@ -169,6 +200,22 @@ int sfh_gpq_create(sfh_system* s, int nq1d, int ns, int pn, const int* pcols, co
             rc = SFH_EDEVICE;
             break;
         }
+        // the readings' sum s and the squared norms the likelihood needs (input data)
+        std::vector<double> sv(ns, 0.0), zv(ns, 0.0);
+        for (int k = 0; k < no; ++k)
+            for (int i = 0; i < ns; ++i) {
+                const double y = Y[(size_t)k * ns + i];
+                sv[i] += y;
+                g->yy += y * y;
+            }
+        for (int i = 0; i < ns; ++i) g->ss += sv[i] * sv[i];
+        if ((rc = g->srow.ensure(sizeof(double) * ns)) || (rc = g->zeros.ensure(sizeof(double) * ns))) break;
+        if (hipMemcpyAsync(g->srow.p, sv.data(), sizeof(double) * ns, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(g->zeros.p, zv.data(), sizeof(double) * ns, hipMemcpyHostToDevice, st) != hipSuccess) {
+            set_error("upload failed");
+            rc = SFH_EDEVICE;
+            break;
+        }
         k_gpq_scatter_pt<<<nb(ns, 64), 64, 0, st>>>(ns, pn, N, pc.as<int>(), pv.as<double>(), g->PT.as<double>());
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
             set_error("projection set-up failed");
@@ -200,6 +247,8 @@ int sfh_gpq_loglik(sfh_gpq* g, int M, const double* hps, const sfh_solve_opts* o
     SFH_TRY(g->eps.ensure(sizeof(double) * M));
     SFH_TRY(g->out.ensure(sizeof(double) * M));
     SFH_TRY(g->st.ensure(sizeof(int) * M));
+    SFH_TRY(g->th3.ensure(sizeof(double) * 3 * M));
+    SFH_TRY(g->ld.ensure(sizeof(double) * M));
     std::vector<double> ev(M), ov(M);
     std::vector<int> sv(M);
     sfh_solve_stats acc{};
@@ -211,8 +260,16 @@ int sfh_gpq_loglik(sfh_gpq* g, int M, const double* hps, const sfh_solve_opts* o
         const int m = (int)idx.size();
         for (int i = 0; i < m; ++i) ev[i] = hps[4 * idx[i] + 3];
         SFH_CHECK_HIP(hipMemcpyAsync(g->eps.p, ev.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
-        SFH_TRY(gpq_sensor_loglik(g->sys->ctx, g->ns, g->no, g->G.as<double>(), pr.scale, m, g->eps.as<double>(),
-                                  g->Y.as<double>(), g->out.as<double>(), g->st.as<int>(), nullptr, g->work));
+        k_gpq_theta<<<nb(m, 64), 64, 0, st>>>(m, std::sqrt(g->no * pr.scale), g->th3.as<double>());
+        SFH_LAUNCH_CHECK();
+        // one data row (s), zero mean, mismatch sigma 0: B's log-determinant and |L^-1 s|^2
+        SFH_TRY(loglik_sets(g->sys->ctx, g->ns, 1, 1, g->zeros.as<double>(), g->G.as<double>(), g->zeros.as<double>(),
+                            0, g->srow.as<double>(), 0.0, g->eps.as<double>(), m, g->th3.as<double>(),
+                            g->out.as<double>(), g->st.as<int>(), g->ld.as<double>(), g->work));
+        k_gpq_combine<<<nb(m, 64), 64, 0, st>>>(m, g->ns, g->no, g->yy, g->ss, g->eps.as<double>(),
+                                                g->out.as<double>(), g->ld.as<double>(), g->out.as<double>(),
+                                                g->st.as<int>());
+        SFH_LAUNCH_CHECK();
         SFH_CHECK_HIP(hipMemcpyAsync(ov.data(), g->out.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
         SFH_CHECK_HIP(hipMemcpyAsync(sv.data(), g->st.p, sizeof(int) * m, hipMemcpyDeviceToHost, st));
         SFH_CHECK_HIP(hipStreamSynchronize(st));
