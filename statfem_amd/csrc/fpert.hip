@@ -27,6 +27,8 @@ struct sfh_fpert {
     int ns = 0, nk = 0, nb = 0, pdim = 0, N = 0, nq1d = 2;
     double reg = 1e-4;
     sfh::DevBuf Pd, PT, dofs, blk, pc;
+    // work matrices, kept between updates
+    sfh::DevBuf cv, u, I, Ainv, T1, T2, R, HT, Ck, Psi, TT, Cm, RCR, PPt, md, work;
 };
 
 namespace sfh {
@@ -184,7 +186,9 @@ int sfh_fpert_update(sfh_fpert* f, double sigma_f, double ell_f, double sigma_k,
     hipStream_t st = c->stream;
     const int N = f->N, ns = f->ns, nk = f->nk, nb = f->nb;
     const long long K = (long long)nk * nb;
-    DevBuf cv, u, I, Ainv, T1, T2, R, HT, Ck, Psi, TT, Cm, RCR, PPt, md, work;
+    DevBuf &cv = f->cv, &u = f->u, &I = f->I, &Ainv = f->Ainv, &T1 = f->T1, &T2 = f->T2, &R = f->R, &HT = f->HT,
+           &Ck = f->Ck, &Psi = f->Psi, &TT = f->TT, &Cm = f->Cm, &RCR = f->RCR, &PPt = f->PPt, &md = f->md,
+           &work = f->work;
     SFH_TRY(cv.ensure(sizeof(double) * N));
     SFH_TRY(u.ensure(sizeof(double) * N));
     SFH_TRY(I.ensure(sizeof(double) * (size_t)N * N));

@@ -117,7 +117,6 @@ void add_stats(sfh_solve_stats& t, const sfh_solve_stats& a) {
 int gpq_precision(sfh_gpq* g, double sigma, double kappa, double nu, const sfh_solve_opts* o, sfh_solve_stats& acc,
                   Prec& pr) {
     sfh_system* s = g->sys;
-    hipStream_t st = s->ctx->stream;
     const int N = g->N, ns = g->ns;
     if (!(kappa > 0.0) || !(sigma > 0.0) || !(nu > 0.0)) { set_error("sigma, kappa and nu must be positive"); return SFH_EARG; }
     const double beta = (nu + s->dim / 2.) / 2.;
@@ -141,7 +140,6 @@ int gpq_precision(sfh_gpq* g, double sigma, double kappa, double nu, const sfh_s
     }
     SFH_TRY(sfh_row_scale(s->ctx, N, ns, g->mass.as<double>(), g->X.as<double>(), g->T.as<double>()));
     SFH_TRY(gram(s->ctx, N, ns, ns, g->X.as<double>(), ns, g->T.as<double>(), ns, g->G.as<double>(), s->scratch3));
-    (void)st;
     return SFH_OK;
 }
 

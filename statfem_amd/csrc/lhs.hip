@@ -281,14 +281,11 @@ int lhs_sized(sfh_lhs* L, int M) {
     SFH_TRY(L->ycur.ensure(sizeof(double) * m));
     SFH_TRY(L->tgt.ensure(sizeof(double) * m));
     SFH_TRY(L->nacc.ensure(sizeof(int) * m));
-    if (L->theta3.bytes < sizeof(double) * 3 * m) {
-        SFH_TRY(L->theta3.ensure(sizeof(double) * 3 * m));
-        L->have_data = false;   // refilled below
-    }
+    SFH_TRY(L->theta3.ensure(sizeof(double) * 3 * m));
+    // (rho, mismatch sigma, mismatch length) of every set, for the likelihood
     k_fill_theta3<<<nb((long long)m, 256), 256, 0, L->sys->ctx->stream>>>((int)m, L->rho, L->sd, L->ld,
                                                                           L->theta3.as<double>());
     SFH_LAUNCH_CHECK();
-    L->have_data = true;
     return SFH_OK;
 }
 
@@ -427,9 +424,9 @@ int sfh_lhs_set_data(sfh_lhs* L, int no, const double* Y, int sdim, const double
     SFH_CHECK_HIP(hipMemcpyAsync(L->xs.p, xs, sizeof(double) * (size_t)L->ns * sdim, hipMemcpyHostToDevice, st));
     SFH_CHECK_HIP(hipMemcpyAsync(L->pm_d.p, L->pm.data(), sizeof(double) * L->np, hipMemcpyHostToDevice, st));
     SFH_CHECK_HIP(hipMemcpyAsync(L->pv_d.p, L->pv.data(), sizeof(double) * L->np, hipMemcpyHostToDevice, st));
-    L->theta3.release();   // refilled with the new rho and mismatch parameters
     SFH_TRY(lhs_sized(L, L->nch));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
+    L->have_data = true;
     return SFH_OK;
 }
 

@@ -200,3 +200,23 @@ def test_laplace_inverse_deck_runs_and_writes_outputs(ctx, tmp_path):
 
 def d_rec():
     return lhs_deck.load()["hparmean"]
+
+
+def test_log_target_needs_the_data(ctx, orc):
+    """Evaluations before sfh_lhs_set_data (or, for resolve = 0, before
+    sfh_lhs_fix_mean) are refused, also after an updateSolution."""
+    from statfem_amd import hip
+    from statfem_amd.lhs import ForwardGaussLHSHip
+    d = lhs_deck.load()
+    fg = ForwardGaussLHSHip(ctx, d["mesh"], d["bc"], d["anchors"], d["gp"], d["cols"], d["vals"], copies=2, nq1d=3)
+    fg.updateSolution(np.repeat(d["init"][None, :], 2, axis=0))
+    with pytest.raises(hip.SfhError):
+        fg.log_target(np.repeat(d["init"][None, :], 2, axis=0), resolve=True)
+    hp = d["hp"]
+    fg.set_data(d["Y"], d["xs"], rho=hp["rho"], mis_sigma=hp["mismatch.sigma"], mis_ell=hp["mismatch.length"],
+                prior_mu=d["pm"], prior_var=d["pv"])
+    with pytest.raises(hip.SfhError):
+        fg.log_target(d["init"][None, :], resolve=False)
+    y, st, _ = fg.log_target(np.repeat(d["init"][None, :], 2, axis=0), resolve=True)
+    assert np.all(st == 0) and np.all(np.isfinite(y))
+    fg.close()
