@@ -1471,7 +1471,7 @@ struct Kern {
     static void vcycle(const Plan& P, const MGHier& m, hipStream_t st, int it, int mode, const T* coef0,
                        const unsigned char* cmask, const double* r0, T* z, T* mgv, double* acc, double* p0,
                        double* p1, const double* S) {
-        if constexpr (std::is_same<T, float>::value && LPR >= 4 && LPR <= 16) {
+        if constexpr (std::is_same<T, float>::value && LPR >= 1 && LPR <= 16) {
             // restriction: 32 columns per work-group (half the LDS of a
             // 64-column tile, so four work-groups share a CU)
             constexpr int ML = LPR > 8 ? 8 : LPR;
@@ -1620,8 +1620,12 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     // a declared 2D grid: the system's grid for one problem, the declared
     // segment grids for a batch
     const std::vector<int> grids = grids_of(s, ns, P.srow);
+    // one lane per node only for a batch that solves with multigrid (a
+    // problem per column, as the conductivity chains)
+    bool mg_grid = false;
+    for (size_t q = 0; q < grids.size(); q += 3) mg_grid = mg_grid || (long long)grids[q + 1] * grids[q + 2] > kMGCoarse;
     const bool use_grid = !grids.empty() && (o.layout == 2 || o.layout == 0) && std::is_same<T, float>::value &&
-                          P.lpr >= 4 && P.lpr <= 16;
+                          (P.lpr >= 2 || (P.lpr == 1 && ns > 1 && o.precond != 1 && mg_grid)) && P.lpr <= 16;
     SFH_TRY(prepare_operator_on(s, st, use_grid ? grids : std::vector<int>()));
     if (use_grid && s->dia_state == 1) plan_grid(P, grids);
     // declared 3D grid: 27-point block stencil with its multigrid hierarchy
@@ -1641,7 +1645,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     // multigrid preconditioner: stencil layout of one grid with a hierarchy
     const MGHier& mg = s->mg;
     bool use_mg = P.grid && o.precond != 1 && mg.built && mg.L > 0 && mg.nseg == (int)grids.size() / 3 &&
-                  P.lpr >= 4;
+                  P.lpr >= 1;
     for (int q = 0; use_mg && q < mg.nseg; ++q)
         use_mg = mg.gbase[q] == grids[3 * q] && mg.gW[q] == grids[3 * q + 1] && mg.gH[q] == grids[3 * q + 2];
     if (use_mg) plan_mg(P, mg);
@@ -1995,9 +1999,9 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
     // Declared 2D grids (stencil kernels): at most 16 lanes (64 columns) per
     // column tile, wider solves run their column tiles side by side in the
     // same launches; narrow solves on a grid with a multigrid hierarchy run as
-    // 16-column tiles (the stencil kernels' narrowest): a dozen multigrid
-    // iterations on padded columns beat thousands of Jacobi ones on the CSR
-    // kernels
+    // 16-column tiles (8-column tiles for a batch of problems): a dozen
+    // multigrid iterations on padded columns beat thousands of Jacobi ones on
+    // the CSR kernels
     int min_kp = 4, max_lpr = 64;
     if (o.precision == 0 && (o.layout == 0 || o.layout == 2)) {
         std::vector<int> srow(nseg + 1);
@@ -2006,7 +2010,9 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
         if (!g.empty()) max_lpr = 16;
         bool big = false;   // a grid with a multigrid hierarchy (one grid or a batch)
         for (size_t q = 0; q < g.size(); q += 3) big = big || (long long)g[q + 1] * g[q + 2] > kMGCoarse;
-        if (big && o.precond != 1 && k < 16) min_kp = 16;
+        // a batch of narrow problems (one column per problem, as the
+        // conductivity chains) runs 8-column tiles: half the lanes per node
+        if (big && o.precond != 1 && k < 16) min_kp = nseg > 1 ? (k <= 4 ? 4 : 8) : 16;
     }
     if (o.precision == 0 && (o.layout == 0 || o.layout == 4) && nseg == 1 && g3_applies(s)) {
         max_lpr = 16;

@@ -137,3 +137,42 @@ def test_batched_forcing_covariance_equals_per_problem(ctx):
         hip._check(hip.lib().sfh_covforcing_grid(ctx.h, len(ax), len(ay), 1, hip._p(ax), hip._p(ay), None, cq.ptr,
                                                  sig[q], ell[q], k, Wq.ptr, Vq.ptr))
         assert np.array_equal(Vb[a:b], Vq.download())
+
+
+@pytest.mark.parametrize("k,precond", [(1, 0), (6, 0), (6, 1), (3, 1)])
+def test_narrow_batch_matches_oracle(ctx, orc, k, precond):
+    """A batch of three grids (two with a multigrid hierarchy) solved with
+    1 to 6 columns per problem: 8-column tiles (2 lanes per node) for the
+    multigrid and the Jacobi stencil kernels (Jacobi with 3 columns: CSR);
+    every problem against the oracle's direct solve."""
+    from statfem_amd import hip, mesh as M
+    ms = [M.grid2d(24, 17), M.grid2d(9, 7), M.grid2d(31, 12)]
+    um, off = M.union(ms)
+    seg = np.ascontiguousarray(off, np.int32)
+    s = hip.SystemSolveHip(ctx, um)
+    nx = np.array([len(m.grid[0]) for m in ms], np.int32)
+    ny = np.array([len(m.grid[1]) for m in ms], np.int32)
+    hip._check(hip.lib().sfh_system_set_segment_grids(s.h, 3, hip._p(seg), hip._p(nx), hip._p(ny)))
+    rng = np.random.default_rng(k + 7 * precond)
+    kap = np.exp(0.4 * rng.standard_normal(um.nel))
+    bcs = [M.dirichlet_all_boundary(m) for m in ms]
+    ubc = np.concatenate([b + off[q] for q, b in enumerate(bcs)]).astype(np.int32)
+    s.assemble(hip.POISSON, kap, nq1d=2)
+    s.applyConstraints(ubc)
+    B = rng.standard_normal((um.nnodes, k))
+    X = ctx.empty((um.nnodes, k))
+    st = hip.SolveStats()
+    o = hip.default_opts(tol=1e-12, precond=precond)
+    Bd = ctx.to_device(B)
+    hip._check(hip.lib().sfh_solve_batched(s.h, 3, hip._p(seg), 0, k, Bd.ptr, X.ptr, C.byref(o), C.byref(st)))
+    # Jacobi with at most 4 columns stays on the CSR kernels (4-column tiles)
+    assert st.layout == (1 if precond == 1 and k <= 4 else 2) and st.precond == (1 if precond == 1 else 2)
+    Xh = X.download().reshape(um.nnodes, k)
+    e0 = 0
+    for q, m in enumerate(ms):
+        kq = kap[e0:e0 + m.nel]
+        e0 += m.nel
+        rp, ci, va, _ = orc.assemble(m, orc.POISSON, kq, nq1d=2, bc_dofs=bcs[q])
+        ref = orc.LU(rp, ci, va).solve(B[off[q]:off[q + 1]])
+        got = Xh[off[q]:off[q + 1]]
+        assert np.abs(got - ref).max() <= 1e-8 * np.abs(ref).max(), q
