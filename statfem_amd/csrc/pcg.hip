@@ -1062,22 +1062,33 @@ __global__ __launch_bounds__(kB) void k_cg_update_b3(int kp, int nnodes, int npb
 // anything else (err) keeps the solver on the CSR kernels.
 __global__ void k_build_dia9(const int* __restrict__ tab, const int* __restrict__ rp, const int* __restrict__ ci,
                              const double* __restrict__ va, float* d32, double* d64, int* err) {
-    const int* t = tab + 3 * blockIdx.y;   // grid: first row, W, H
+    const int* t = tab + 3 * blockIdx.y;   // grid: first row, W, H (W >= 3)
     const int base = t[0], W = t[1], H = t[2];
     const int loc = blockIdx.x * blockDim.x + threadIdx.x;
     if (loc >= W * H) return;
     const int nd = base + loc;
     const int x = loc % W, y = loc / W;
-    for (int k = 0; k < 9; ++k) { d32[(size_t)nd * 9 + k] = 0.f; d64[(size_t)nd * 9 + k] = 0.0; }
+    double a[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = 0.0;
     for (int e = rp[nd]; e < rp[nd + 1]; ++e) {
-        const int c = ci[e];
-        const int cl = c - base;
-        const int cx = cl % W, cy = cl / W;
-        const int dx = cx - x, dy = cy - y;
-        if (cl < 0 || cl >= W * H || dx < -1 || dx > 1 || dy < -1 || dy > 1) { atomicOr(err, 1); continue; }
+        // column offset -> (dy, dx) without a division: |dx| <= 1 < W - 1
+        const int off = ci[e] - nd;
+        const int dy = off < -1 ? -1 : (off > 1 ? 1 : 0);
+        const int dx = off - dy * W;
+        if (dx < -1 || dx > 1 || x + dx < 0 || x + dx >= W || y + dy < 0 || y + dy >= H) {
+            atomicOr(err, 1);
+            continue;
+        }
         const int k = (dy + 1) * 3 + (dx + 1);
-        d32[(size_t)nd * 9 + k] = (float)va[e];
-        d64[(size_t)nd * 9 + k] = va[e];
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            if (q == k) a[q] = va[e];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        d64[(size_t)nd * 9 + k] = a[k];
+        d32[(size_t)nd * 9 + k] = (float)a[k];
     }
 }
 
