@@ -426,17 +426,30 @@ int sfh_lhs_mcmc(sfh_lhs* lhs, int steps, const double* theta0, const double* pr
 
 /* ------------------------------------------ sparse-precision GP (SPDE) */
 /* GaussianProcessQ (statX/inferQ/GaussianProcessQ.hpp:65-159) on the scalar
- * mesh of sys with the integer-beta precision of DriverMaxLikelihood
+ * mesh of sys with the precision of DriverMaxLikelihood
  * (statX/inferQ/DriverMaxLikelihood.hpp:202-312):
- *   Q = tau^2 kappa^(4 beta) (A M^-1)^(2 beta - 1) A, beta = (nu + d/2) / 2,
+ *   Q = tau^2 kappa^(4 beta) (A M^-1)^(2 alpha - 1) A, beta = (nu + d/2) / 2,
+ *   alpha = max(1, floor beta),
  * A the Matern matrix assembled by the library (SFH_MATERN, factor 1/kappa^2,
  * no constraints), M its lumped mass (nq1d-point rule), P the projection
  * (ns rows of pn (column, value) pairs, host), readings Y (no x ns, host,
- * reading-major). P_r = P_l = I (integer beta). Declare a tensor grid
+ * reading-major). Integer beta: P_r = P_l = I; a fractional beta needs its
+ * rational factors (sfh_gpq_set_rational). Declare a tensor grid
  * (sfh_system_set_grid) to solve with multigrid. */
 int sfh_gpq_create(sfh_system* sys, int nq1d, int ns, int pn, const int* pcols, const double* pvals, int no,
                    const double* Y, sfh_gpq** out);
 int sfh_gpq_destroy(sfh_gpq* gpq);
+/* The rational approximation x^exponent ~ a prod (x - c_i) / (b prod (x - d_j))
+ * for exponent = beta - alpha, as PrecisionMatrixRational::readRoots_ reads it
+ * from top.out and bot.out (statX/inferQ/PrecisionMatrixRational.ipp:300-372):
+ * top = (a, c_0 .. c_{degree-1}), bot = (b, d_0 .. d_{degree-1}), host. It
+ * defines P_l = b prod_j (I - d_j M^-1 A) (giveLeftFracMatrix, :41-98),
+ * P_r = a prod_i (I - c_i M^-1 A) (giveRightFracMatrix, :100-161),
+ * Q_v = P_l^T Q P_l (givePrecisionMatrix, :24-39) and u = P_r v
+ * (GaussianProcessQ.ipp:54,140,182). The poles d_j must be negative (each
+ * costs one solve with the SPD matrix M - d_j A). One set per exponent; a
+ * second call with the same exponent replaces it. */
+int sfh_gpq_set_rational(sfh_gpq* gpq, double exponent, int degree, const double* top, const double* bot);
 /* giveLogLikelihood (GaussianProcessQ.ipp:37-99) for M hyperparameter sets
  * hps (M x 4, host: gaussianProcess.sigma, kappa, nu, eps), computed through
  * G = P Q^-1 P^T (determinant lemma, push-through identity) from multi-RHS

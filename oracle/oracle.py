@@ -436,12 +436,36 @@ def gpq_precision_dense(rp, ci, va, mass, sigma, kappa, nu, dim):
     return precision_apply(rp, ci, va, mass, sigma, kappa, nu, dim, np.eye(n))
 
 
-def gpq_loglik(Q, P, Y, eps):
-    """GaussianProcessQ::giveLogLikelihood (statX/inferQ/GaussianProcessQ.ipp:37-99)
-    with P_r = I (integer beta), dense Cholesky factors in place of the sparse
-    LLT. Q: N x N, P: ns x N, Y: ns x no (the reference's obsValues layout)."""
+def gpq_frac_factor(rp, ci, va, mass, roots):
+    """roots[0] prod_j (I - roots[j] M^-1 A), dense: P_l of giveLeftFracMatrix
+    (statX/inferQ/PrecisionMatrixRational.ipp:41-98) for roots = bot.out's
+    (b, d_0 ..), P_r of giveRightFracMatrix (:100-161) for top.out's (a, c_0 ..)."""
+    n = len(rp) - 1
+    T = spmv(rp, ci, va, np.eye(n)) / np.asarray(mass)[:, None]
+    F = np.eye(n)
+    for r in roots[1:]:
+        F = F @ (np.eye(n) - r * T)
+    return roots[0] * F
+
+
+def gpq_precision_frac_dense(rp, ci, va, mass, sigma, kappa, nu, dim, top, bot):
+    """givePrecisionMatrix (PrecisionMatrixRational.ipp:24-39) for a fractional
+    beta, Q_v = P_l^T Q P_l with Q from the integer part alpha = max(1, floor beta)
+    (:177-209), and giveRightFracMatrix's P_r. Returns (Q_v, P_r)."""
+    Q = gpq_precision_dense(rp, ci, va, mass, sigma, kappa, nu, dim)
+    Pl = gpq_frac_factor(rp, ci, va, mass, bot)
+    return Pl.T @ Q @ Pl, gpq_frac_factor(rp, ci, va, mass, top)
+
+
+def gpq_loglik(Q, P, Y, eps, Pr=None):
+    """GaussianProcessQ::giveLogLikelihood (statX/inferQ/GaussianProcessQ.ipp:37-99),
+    dense Cholesky factors in place of the sparse LLT, with the effective
+    projection P_v = P P_r (:54; P_r = I for integer beta). Q: N x N, P: ns x N,
+    Y: ns x no (the reference's obsValues layout)."""
     ns, no = Y.shape
     N = Q.shape[0]
+    if Pr is not None:
+        P = P @ Pr
     Am = eps * eps * Q + no * P.T @ P                                  # step 1a
     L = np.linalg.cholesky(Am)
     s = Y.sum(axis=1)
@@ -455,14 +479,18 @@ def gpq_loglik(Q, P, Y, eps):
     return tmp1 + tmp2 + tmp3
 
 
-def gpq_predict(Q, P, Y, eps):
-    """predictMean and predictCovariance (GaussianProcessQ.ipp:101-186) with
-    P_r = I: mean (sigma_e^2 Q + n_o P^T P)^-1 P^T sum_i y_i, covariance
-    sigma_e^2 (sigma_e^2 Q + n_o P^T P)^-1."""
+def gpq_predict(Q, P, Y, eps, Pr=None):
+    """predictMean and predictCovariance (GaussianProcessQ.ipp:101-186): with
+    P_v = P P_r, mean P_r (sigma_e^2 Q + n_o P_v^T P_v)^-1 P_v^T sum_i y_i,
+    covariance P_r sigma_e^2 (sigma_e^2 Q + n_o P_v^T P_v)^-1 P_r^T (P_r = I
+    for integer beta)."""
     no = Y.shape[1]
-    Am = eps * eps * Q + no * P.T @ P
-    mean = np.linalg.solve(Am, P.T @ Y.sum(axis=1))
-    cov = eps * eps * np.linalg.inv(Am)
+    N = Q.shape[0]
+    Pr = np.eye(N) if Pr is None else Pr
+    Pv = P @ Pr
+    Am = eps * eps * Q + no * Pv.T @ Pv
+    mean = Pr @ np.linalg.solve(Am, Pv.T @ Y.sum(axis=1))
+    cov = Pr @ (eps * eps * np.linalg.inv(Am)) @ Pr.T
     return mean, cov
 
 

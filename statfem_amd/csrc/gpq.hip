@@ -1,20 +1,31 @@
 // GaussianProcessQ (statX/inferQ/GaussianProcessQ.hpp:65-159,
-// GaussianProcessQ.ipp:37-186) with the integer-beta SPDE precision that
+// GaussianProcessQ.ipp:37-186) with the SPDE precision that
 // DriverMaxLikelihood assembles (statX/inferQ/DriverMaxLikelihood.hpp:202-312,
 // PrecisionMatrixRational.ipp:24-39,177-209,278-298):
-//   Q = c (A M^-1)^(2 beta - 1) A,  c = tau^2 kappa^(4 beta),
-// A the Matern matrix M_c + kappa^-2 S (SFH_MATERN with factor 1/kappa^2), M
+//   Q = c (A M^-1)^(2 alpha - 1) A,  c = tau^2 kappa^(4 beta),  alpha = max(1, floor beta),
+// A the Matern matrix M + kappa^-2 S (SFH_MATERN with factor 1/kappa^2), M
 // its lumped mass, tau from (sigma, kappa, nu).
+//
+// A fractional beta adds the rational factors of PrecisionMatrixRational.ipp:41-161
+// (x^(beta - alpha) ~ a prod (x - c_i) / (b prod (x - d_j))):
+//   Q_v = P_l^T Q P_l, P_l = b prod_j (I - d_j M^-1 A), P_r = a prod_i (I - c_i M^-1 A),
+// and GaussianProcessQ works with u = P_r v, P_v = P P_r. P_l and P_r are
+// polynomials in T = M^-1 A, so they commute and
+//   P_v Q_v^-1 P_v^T = (a/b)^2 P r~(T)^2 Q^-1 P^T,  r~(T) = prod_i (I - c_i T) prod_j (I - d_j T)^-1,
+// with (I - d_j T)^-1 = (M - d_j A)^-1 M: one more multi-right-hand-side
+// solve per pole (M - d_j A is SPD for the negative poles of the best
+// approximation) and one sparse product per zero.
 //
 // The reference factorises sigma_e^2 Q + n_o P^T P and Q with a sparse
 // Cholesky per likelihood evaluation. Here everything goes through the
 // sensor-space matrix G = P Q^-1 P^T:
-//   Q^-1 = (1/c) (A^-1 M)^(2 beta - 1) A^-1, so with X = (A^-1 M)^(beta - 1) A^-1 P^T
-//   (beta multi-right-hand-side solves with A, one column per sensor)
-//   G = (1/c) X^T M X,
+//   Q^-1 = (1/c) (A^-1 M)^(2 alpha - 1) A^-1, so with X = r~(T) (A^-1 M)^(alpha - 1) A^-1 P^T
+//   (alpha multi-right-hand-side solves with A, one column per sensor; r~ = I
+//   for integer beta)
+//   G = ((a/b)^2 / c) X^T M X,
 // the log-determinant difference and the quadratic form follow from the
 // determinant lemma and the push-through identity (k_gpq_loglik), and the
-// posterior uses Q^-1 P^T = (1/c) (A^-1 M)^beta X. The solves are the
+// posterior uses P_r Q_v^-1 P_v^T = ((a/b)^2 / c) r~(T) (A^-1 M)^alpha X. The solves are the
 // library's PCG (multigrid on declared grids); no sparse factorisation is
 // formed. All noise levels that share (sigma, kappa, nu) share G.
 #include <cmath>
@@ -25,7 +36,15 @@
 #include "common.hpp"
 
 struct sfh_gpq {
+    struct Rational {
+        double exponent = 0.0;            // beta - alpha
+        std::vector<double> top, bot;     // a, c_0 .. c_{m-1};  b, d_0 .. d_{m-1}
+    };
     sfh_system* sys = nullptr;
+    std::vector<Rational> rats;
+    // the operator the system holds: A (d = 0) or M - d A, for kappa
+    bool held = false;
+    double held_kappa = 0.0, held_d = 0.0;
     int nq1d = 2, ns = 0, no = 0, N = 0;
     double yy = 0.0, ss = 0.0;   // sum of squared readings, |s|^2 (s: the readings' sum)
     sfh::DevBuf PT, mass, Y, X, T, G, eps, out, st, work, w, v;
@@ -97,9 +116,19 @@ __global__ void k_gpq_combine(int m, int ns, int no, double yy, double ss, const
 }
 
 struct Prec {
-    double scale = 0.0;   // 1 / c
-    int beta = 0;
+    double scale = 0.0;   // (a/b)^2 / c  (1 / c for integer beta)
+    int beta = 0;         // alpha = max(1, floor beta)
+    double kappa = 0.0;
+    const sfh_gpq::Rational* rat = nullptr;
 };
+
+// V -= c T / mass (N x k): one zero factor (I - c M^-1 A) after T = A V
+__global__ void k_gpq_zero_factor(long long total, int k, double c, const double* __restrict__ mass,
+                                  const double* __restrict__ T, double* __restrict__ V) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    V[t] -= c * T[t] / mass[t / k];
+}
 
 void add_stats(sfh_solve_stats& t, const sfh_solve_stats& a) {
     t.inner_iters += a.inner_iters;
@@ -112,8 +141,51 @@ void add_stats(sfh_solve_stats& t, const sfh_solve_stats& a) {
     t.levels = a.levels;
 }
 
-// A(kappa), then X = (A^-1 M)^(beta-1) A^-1 P^T (N x ns) in g->X and the raw
-// sensor matrix X^T M X in g->G
+// Puts A = M + kappa^-2 S (d = 0) or M - d A = (1 - d) M + (-d) kappa^-2 S
+// (d < 0) into the system, unless it already holds it.
+int gpq_operator(sfh_gpq* g, double kappa, double d) {
+    if (g->held && g->held_kappa == kappa && g->held_d == d) return SFH_OK;
+    g->held = false;
+    if (d == 0.0)
+        SFH_TRY(assemble_matrix(g->sys, SFH_MATERN, g->nq1d, nullptr, nullptr, kappa, 1.0, 1.0 / (kappa * kappa)));
+    else
+        SFH_TRY(assemble_matrix(g->sys, SFH_MATERN, g->nq1d, nullptr, nullptr, std::sqrt(1.0 - d),
+                                -d / (kappa * kappa), 1.0));
+    g->held = true;
+    g->held_kappa = kappa;
+    g->held_d = d;
+    return SFH_OK;
+}
+
+// V <- r~(T)^times V for k columns (scratch T: N x k); leading coefficients
+// are in Prec::scale
+int gpq_rational(sfh_gpq* g, const Prec& pr, int times, int k, double* V, double* T, const sfh_solve_opts* o,
+                 sfh_solve_stats& acc) {
+    const sfh_gpq::Rational& r = *pr.rat;
+    const int m = (int)r.top.size() - 1;
+    const long long total = (long long)g->N * k;
+    hipStream_t st = g->sys->ctx->stream;
+    SFH_TRY(gpq_operator(g, pr.kappa, 0.0));
+    for (int t = 0; t < times; ++t)
+        for (int i = 0; i < m; ++i) {
+            SFH_TRY(spmv(g->sys, k, V, T));
+            k_gpq_zero_factor<<<nb(total, 256), 256, 0, st>>>(total, k, r.top[i + 1], g->mass.as<double>(), T, V);
+            SFH_LAUNCH_CHECK();
+        }
+    sfh_solve_stats one{};
+    for (int j = 0; j < (int)r.bot.size() - 1; ++j) {
+        SFH_TRY(gpq_operator(g, pr.kappa, r.bot[j + 1]));
+        for (int t = 0; t < times; ++t) {
+            SFH_TRY(sfh_row_scale(g->sys->ctx, g->N, k, g->mass.as<double>(), V, T));
+            SFH_TRY(solve(g->sys, 0, k, T, k, V, k, o, &one));
+            add_stats(acc, one);
+        }
+    }
+    return SFH_OK;
+}
+
+// A(kappa), then X = r~(T) (A^-1 M)^(alpha-1) A^-1 P^T (N x ns) in g->X and
+// the raw sensor matrix X^T M X in g->G
 int gpq_precision(sfh_gpq* g, double sigma, double kappa, double nu, const sfh_solve_opts* o, sfh_solve_stats& acc,
                   Prec& pr) {
     sfh_system* s = g->sys;
@@ -121,15 +193,26 @@ int gpq_precision(sfh_gpq* g, double sigma, double kappa, double nu, const sfh_s
     if (!(kappa > 0.0) || !(sigma > 0.0) || !(nu > 0.0)) { set_error("sigma, kappa and nu must be positive"); return SFH_EARG; }
     const double beta = (nu + s->dim / 2.) / 2.;
     const double alpha = std::max(1., std::floor(beta));
+    pr.rat = nullptr;
     if (std::fabs(beta - alpha) >= 1e-12) {
-        set_error("fractional SPDE exponent (beta = %g) needs the rational approximation", beta);
-        return SFH_EARG;
+        for (const auto& r : g->rats)
+            if (std::fabs(r.exponent - (beta - alpha)) < 1e-12) pr.rat = &r;
+        if (!pr.rat) {
+            set_error("fractional SPDE exponent %.17g (beta = %g) needs its rational factors", beta - alpha, beta);
+            return SFH_EARG;
+        }
     }
     double tau = 0.0;
     SFH_TRY(sfh_matern_tau(sigma, kappa, nu, s->dim, &tau));
     pr.beta = (int)alpha;
+    pr.kappa = kappa;
     pr.scale = 1.0 / (tau * tau * std::pow(kappa, 4.0 * beta));
-    SFH_TRY(assemble_matrix(s, SFH_MATERN, g->nq1d, nullptr, nullptr, kappa, 1.0, 1.0 / (kappa * kappa)));
+    if (pr.rat) {
+        const double ab = pr.rat->top[0] / pr.rat->bot[0];
+        pr.scale *= ab * ab;
+    }
+    g->held = false;   // the caller may have re-assembled the system since the last call
+    SFH_TRY(gpq_operator(g, kappa, 0.0));
     sfh_solve_stats one{};
     SFH_TRY(solve(s, 0, ns, g->PT.as<double>(), ns, g->X.as<double>(), ns, o, &one));
     add_stats(acc, one);
@@ -138,15 +221,17 @@ int gpq_precision(sfh_gpq* g, double sigma, double kappa, double nu, const sfh_s
         SFH_TRY(solve(s, 0, ns, g->T.as<double>(), ns, g->X.as<double>(), ns, o, &one));
         add_stats(acc, one);
     }
+    if (pr.rat) SFH_TRY(gpq_rational(g, pr, 1, ns, g->X.as<double>(), g->T.as<double>(), o, acc));
     SFH_TRY(sfh_row_scale(s->ctx, N, ns, g->mass.as<double>(), g->X.as<double>(), g->T.as<double>()));
     SFH_TRY(gram(s->ctx, N, ns, ns, g->X.as<double>(), ns, g->T.as<double>(), ns, g->G.as<double>(), s->scratch3));
     return SFH_OK;
 }
 
 // V <- (A^-1 M)^times V for k columns (V, scratch T: N x k)
-int gpq_apply_inv_m(sfh_gpq* g, int times, int k, double* V, double* T, const sfh_solve_opts* o,
+int gpq_apply_inv_m(sfh_gpq* g, const Prec& pr, int times, int k, double* V, double* T, const sfh_solve_opts* o,
                     sfh_solve_stats& acc) {
     sfh_solve_stats one{};
+    SFH_TRY(gpq_operator(g, pr.kappa, 0.0));
     for (int r = 0; r < times; ++r) {
         SFH_TRY(sfh_row_scale(g->sys->ctx, g->N, k, g->mass.as<double>(), V, T));
         SFH_TRY(solve(g->sys, 0, k, T, k, V, k, o, &one));
@@ -233,6 +318,36 @@ int sfh_gpq_destroy(sfh_gpq* g) {
     return SFH_OK;
 }
 
+int sfh_gpq_set_rational(sfh_gpq* g, double exponent, int degree, const double* top, const double* bot) {
+    if (!g || !top || !bot || degree < 1 || degree > 32) { set_error("bad argument"); return SFH_EARG; }
+    if (!(std::fabs(exponent) > 0.0 && std::fabs(exponent) < 1.0)) {
+        set_error("the exponent must lie in (-1, 0) or (0, 1)");
+        return SFH_EARG;
+    }
+    if (!(top[0] != 0.0) || !(bot[0] != 0.0) || !std::isfinite(top[0] / bot[0])) {
+        set_error("leading coefficients must be finite and nonzero");
+        return SFH_EARG;
+    }
+    for (int j = 1; j <= degree; ++j) {
+        if (!std::isfinite(top[j])) { set_error("zero %d is not finite", j - 1); return SFH_EARG; }
+        if (!(bot[j] < 0.0) || !std::isfinite(bot[j])) {
+            set_error("pole %d (%g) must be negative: M - d A is then positive definite", j - 1, bot[j]);
+            return SFH_EARG;
+        }
+    }
+    sfh_gpq::Rational r;
+    r.exponent = exponent;
+    r.top.assign(top, top + degree + 1);
+    r.bot.assign(bot, bot + degree + 1);
+    for (auto& q : g->rats)
+        if (std::fabs(q.exponent - exponent) < 1e-12) {
+            q = r;
+            return SFH_OK;
+        }
+    g->rats.push_back(r);
+    return SFH_OK;
+}
+
 int sfh_gpq_loglik(sfh_gpq* g, int M, const double* hps, const sfh_solve_opts* o, double* out, int* status,
                    sfh_solve_stats* stats) {
     if (!g || M < 1 || !hps || !out) { set_error("bad argument"); return SFH_EARG; }
@@ -306,17 +421,19 @@ int sfh_gpq_predict(sfh_gpq* g, const double* hp, const sfh_solve_opts* o, doubl
     if (s0 != SFH_OK) { set_error("sigma_e^2 I + n_o G is not positive definite"); return s0; }
     if (logl) *logl = l0;
     if (mean_dev) {
-        // u = (1/c) (A^-1 M)^beta X B^-1 s
+        // u = ((a/b)^2 / c) r~(T) (A^-1 M)^alpha X B^-1 s
         SFH_TRY(g->v.ensure(sizeof(double) * 2 * (size_t)N));
         double* v = g->v.as<double>();
         SFH_TRY(dgemm_shared_a(c, N, 1, ns, g->X.as<double>(), ns, g->w.as<double>(), 1, 0, v, 1, 0, 1));
-        SFH_TRY(gpq_apply_inv_m(g, pr.beta, 1, v, v + N, o, acc));
+        SFH_TRY(gpq_apply_inv_m(g, pr, pr.beta, 1, v, v + N, o, acc));
+        if (pr.rat) SFH_TRY(gpq_rational(g, pr, 1, 1, v, v + N, o, acc));
         k_gpq_scale<<<nb(N, 256), 256, 0, st>>>(N, pr.scale, v, mean_dev, 0);
         SFH_LAUNCH_CHECK();
     }
     if (cov_dev) {
-        // sigma_e^2 (sigma_e^2 Q + n_o P^T P)^-1 = Q^-1 - Z H^-1 Z^T,
-        // Z = Q^-1 P^T = (1/c) (A^-1 M)^beta X, H = (sigma_e^2 / n_o) I + G
+        // P_r sigma_e^2 (sigma_e^2 Q_v + n_o P_v^T P_v)^-1 P_r^T = C - Z H^-1 Z^T with
+        // C = P_r Q_v^-1 P_r^T = ((a/b)^2 / c) r~(T)^2 (A^-1 M)^(2 alpha - 1) A^-1,
+        // Z = P_r Q_v^-1 P_v^T = ((a/b)^2 / c) r~(T) (A^-1 M)^alpha X, H = (sigma_e^2 / n_o) I + G
         DevBuf Z, Zt, H, big, bigT, hs;
         SFH_TRY(Z.ensure(sizeof(double) * (size_t)N * ns));
         SFH_TRY(Zt.ensure(sizeof(double) * (size_t)N * ns));
@@ -325,17 +442,20 @@ int sfh_gpq_predict(sfh_gpq* g, const double* hp, const sfh_solve_opts* o, doubl
         SFH_TRY(bigT.ensure(sizeof(double) * (size_t)N * N));
         SFH_TRY(hs.ensure(sizeof(int)));
         SFH_CHECK_HIP(hipMemcpyAsync(Z.p, g->X.p, sizeof(double) * (size_t)N * ns, hipMemcpyDeviceToDevice, st));
-        SFH_TRY(gpq_apply_inv_m(g, pr.beta, ns, Z.as<double>(), g->T.as<double>(), o, acc));
+        SFH_TRY(gpq_apply_inv_m(g, pr, pr.beta, ns, Z.as<double>(), g->T.as<double>(), o, acc));
+        if (pr.rat) SFH_TRY(gpq_rational(g, pr, 1, ns, Z.as<double>(), g->T.as<double>(), o, acc));
         k_gpq_scale<<<nb((long long)N * ns, 256), 256, 0, st>>>((long long)N * ns, pr.scale, Z.as<double>(),
                                                                Z.as<double>(), 0);
         SFH_LAUNCH_CHECK();
-        // Q^-1 = (1/c) (A^-1 M)^(2 beta - 1) A^-1 on the identity, into cov_dev
+        // C on the identity, into cov_dev
         k_gpq_identity<<<nb((long long)N * N, 256), 256, 0, st>>>(N, big.as<double>());
         SFH_LAUNCH_CHECK();
         sfh_solve_stats one{};
+        SFH_TRY(gpq_operator(g, pr.kappa, 0.0));
         SFH_TRY(solve(g->sys, 0, N, big.as<double>(), N, cov_dev, N, o, &one));
         add_stats(acc, one);
-        SFH_TRY(gpq_apply_inv_m(g, 2 * pr.beta - 1, N, cov_dev, bigT.as<double>(), o, acc));
+        SFH_TRY(gpq_apply_inv_m(g, pr, 2 * pr.beta - 1, N, cov_dev, bigT.as<double>(), o, acc));
+        if (pr.rat) SFH_TRY(gpq_rational(g, pr, 2, N, cov_dev, bigT.as<double>(), o, acc));
         k_gpq_scale<<<nb((long long)N * N, 256), 256, 0, st>>>((long long)N * N, pr.scale, cov_dev, cov_dev, 0);
         SFH_LAUNCH_CHECK();
         SFH_TRY(gpq_hmat(c, ns, g->no, g->G.as<double>(), pr.scale, hp[3], H.as<double>()));

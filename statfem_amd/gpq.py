@@ -1,7 +1,11 @@
 """Sparse-precision (SPDE) Gaussian-process regression on the GPU:
 GaussianProcessQ (statX/inferQ/GaussianProcessQ.hpp:65-159,
-GaussianProcessQ.ipp:37-186) with the integer-beta precision that
-DriverMaxLikelihood assembles (statX/inferQ/DriverMaxLikelihood.hpp:52-313).
+GaussianProcessQ.ipp:37-186) with the precision that DriverMaxLikelihood
+assembles (statX/inferQ/DriverMaxLikelihood.hpp:52-313). A fractional beta
+uses the rational factors of PrecisionMatrixRational (statX/inferQ/
+PrecisionMatrixRational.ipp:24-161); they come from `rational.brasil_factors`
+for the exponent beta - floor(beta) at the reference's degree 4 (its
+brasil_ call, :300-344), or from `set_rational`.
 
 The likelihood and posterior go through the sensor-space matrix
 G = P Q^-1 P^T, formed from multi-right-hand-side solves with the Matern
@@ -12,6 +16,7 @@ import ctypes as C
 import numpy as np
 
 from . import hip
+from . import rational
 
 
 def _rows(cols, vals):
@@ -31,9 +36,12 @@ class GaussianProcessQHip:
     """obsValues Y (ns x no, as the reference stores them) and the projection
     P (rows of (DOF columns, values)) on `system`'s scalar mesh."""
 
-    def __init__(self, system, cols, vals, Y, nq1d=2):
+    def __init__(self, system, cols, vals, Y, nq1d=2, degree=4):
         self.system = system
         self.ctx = system.ctx
+        self.dim = system.mesh.dim
+        self.degree = degree
+        self.rationals = {}
         self.ns = len(cols)
         Y = np.asarray(Y, np.float64).reshape(self.ns, -1)
         self.no = Y.shape[1]
@@ -53,9 +61,33 @@ class GaussianProcessQHip:
     def __del__(self):
         self.close()
 
+    def exponent(self, nu):
+        """beta - alpha with beta = (nu + d/2) / 2, alpha = max(1, floor beta)
+        (PrecisionMatrixRational.ipp:64-67); 0 for integer beta."""
+        beta = (nu + self.dim / 2.0) / 2.0
+        e = beta - max(1.0, np.floor(beta))
+        return 0.0 if abs(e) < 1e-12 else e
+
+    def set_rational(self, exponent, top, bot):
+        """The rational factors for one exponent: top = (a, c_0 ..), bot = (b, d_0 ..)."""
+        top = hip._h(np.asarray(top, np.float64))
+        bot = hip._h(np.asarray(bot, np.float64))
+        if len(top) != len(bot) or len(top) < 2:
+            raise ValueError("top and bot need the same degree + 1 entries")
+        hip._check(hip.lib().sfh_gpq_set_rational(self.h, float(exponent), len(top) - 1, hip._p(top), hip._p(bot)))
+        self.rationals[float(exponent)] = (top, bot)
+
+    def _rationals_for(self, nus):
+        for nu in nus:
+            e = self.exponent(float(nu))
+            if e != 0.0 and not any(abs(e - k) < 1e-12 for k in self.rationals):
+                top, bot, _ = rational.brasil_factors(e, self.degree)
+                self.set_rational(e, top, bot)
+
     def giveLogLikelihood(self, hps, opts=None):
         """hps: M x 4 sets (sigma, kappa, nu, eps). Returns (logL, status, stats)."""
         t = np.ascontiguousarray(np.asarray(hps, np.float64).reshape(-1, 4))
+        self._rationals_for(np.unique(t[:, 2]))
         out = np.zeros(t.shape[0])
         st = np.zeros(t.shape[0], np.int32)
         ss = hip.SolveStats()
@@ -66,6 +98,7 @@ class GaussianProcessQHip:
 
     def _predict(self, hp, mean, cov, opts):
         t = hip._h(np.asarray(hp, np.float64).reshape(4))
+        self._rationals_for([t[2]])
         n = self.system.ndof
         md = self.ctx.empty((n,)) if mean else None
         cd = self.ctx.empty((n, n)) if cov else None

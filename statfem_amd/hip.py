@@ -127,6 +127,7 @@ _SIGS = {
     "sfh_lhs_mcmc": (i32, [v, i32, v, v, C.c_uint64, i32, C.POINTER(SolveOpts), v, v, C.POINTER(SolveStats)]),
     "sfh_gpq_create": (i32, [v, i32, i32, i32, v, v, i32, v, C.POINTER(v)]),
     "sfh_gpq_destroy": (i32, [v]),
+    "sfh_gpq_set_rational": (i32, [v, dbl, i32, v, v]),
     "sfh_gpq_loglik": (i32, [v, i32, v, C.POINTER(SolveOpts), v, v, C.POINTER(SolveStats)]),
     "sfh_gpq_predict": (i32, [v, v, C.POINTER(SolveOpts), v, v, C.POINTER(dbl), C.POINTER(SolveStats)]),
     "sfh_fpert_create": (i32, [v, i32, i32, i32, v, v, i32, i32, v, v, i32, v, dbl, C.POINTER(v)]),

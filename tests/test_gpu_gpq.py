@@ -1,5 +1,6 @@
-"""Sparse-precision GP (GaussianProcessQ with the integer-beta SPDE precision)
-on the GPU against the oracle's dense restatement of the reference algorithm."""
+"""Sparse-precision GP (GaussianProcessQ with the SPDE precision, integer and
+fractional beta) on the GPU against the oracle's dense restatement of the
+reference algorithm and the exact Gaussian densities."""
 import numpy as np
 import pytest
 
@@ -108,7 +109,11 @@ def test_posterior_matches_oracle(ctx, orc, dim):
 
 def test_driver_keys_and_fractional_beta(ctx, orc):
     """DriverMaxLikelihood's map: kappa given directly or through the length
-    (kappa = sqrt(2 nu) / length); a fractional exponent is refused."""
+    (kappa = sqrt(2 nu) / length); the C ABI refuses a fractional exponent
+    without its rational factors and a non-negative pole, and the Python
+    mirror supplies the factors by itself."""
+    import ctypes as C
+    from statfem_amd import rational
     from statfem_amd import hip
     from statfem_amd.gpq import DriverMaxLikelihoodHip
     m, gp, P, Y = _case(ctx, orc, 2, 1)
@@ -127,6 +132,75 @@ def test_driver_keys_and_fractional_beta(ctx, orc):
     assert np.all(st == 0) and abs(ll[1] - l2) <= 1e-10 * abs(l2)
     Q = _oracle_q(orc, m, 0.9, np.sqrt(2.0) / 0.25, 1.0)
     assert abs(ll[0] - orc.gpq_loglik(Q, P, Y, 0.02)) <= 1e-8 * abs(ll[0])
+    t = np.array([[0.9, 5.0, 1.5, 0.05]])            # beta = 1.25 in 2D
+    out, st = np.zeros(1), np.zeros(1, np.int32)
+    o = hip.default_opts(tol=1e-12)
+    rc = hip.lib().sfh_gpq_loglik(gp.h, 1, hip._p(t), C.byref(o), hip._p(out), hip._p(st), None)
+    assert hip.STATUS[rc] == "SFH_EARG"
+    top, bot, _ = rational.brasil_factors(0.25, 4)
+    bad = bot.copy()
+    bad[2] = 0.1
     with pytest.raises(hip.SfhError):
-        gp.giveLogLikelihood([[0.9, 5.0, 1.5, 0.05]])   # beta = 1.25 in 2D
+        gp.set_rational(0.25, top, bad)
+    l_auto = gp.giveLogLikelihood(t, o)[0][0]          # factors computed on first use
+    gp.set_rational(0.25, top, bot)
+    l_set = gp.giveLogLikelihood(t, o)[0][0]
+    assert np.isfinite(l_auto) and abs(l_auto - l_set) <= 1e-12 * abs(l_set)
+    gp.close()
+
+
+def _exact_fractional(orc, m, sigma, kappa, nu, top, bot):
+    """The field covariance P_r Q_v^-1 P_r^T in the generalized eigenbasis of
+    (A, M): (1/c) V diag(r(1/lam)^2 lam^(-2 alpha)) V^T, r the factor form."""
+    from scipy.linalg import eigh
+    from statfem_amd import rational
+    rp, ci, va, _ = orc.assemble(m, orc.MATERN, kappa, 1.0, nq1d=2, factor=1.0 / kappa ** 2)
+    mass = orc.body_force(m, nq1d=2, fconst=1.0)
+    lam, V = eigh(orc.spmv(rp, ci, va, np.eye(m.nnodes)), np.diag(mass))
+    beta = (nu + m.dim / 2) / 2
+    alpha = max(1.0, np.floor(beta))
+    c = orc.matern_tau(sigma, kappa, nu, m.dim) ** 2 * kappa ** (4 * beta)
+    rr = rational.factor_form(top, bot, 1.0 / lam)
+    return V @ np.diag(rr ** 2 * lam ** (-2 * alpha)) @ V.T / c, (rp, ci, va, mass)
+
+
+@pytest.mark.parametrize("dim,declare,nu", [(1, 0, 1.0), (2, 0, 0.5), (2, 1, 0.5), (2, 1, 1.5), (2, 1, 2.5)])
+def test_fractional_beta_matches_oracle(ctx, orc, dim, declare, nu):
+    """Fractional beta (2D: 0.75, 1.25, 1.75; 1D: 0.75) with the rational
+    factors of the reference's degree 4: likelihood at three noise levels,
+    posterior mean and covariance, on CSR and on the declared grid."""
+    from statfem_amd import hip, rational
+    m, gp, P, Y = _case(ctx, orc, dim, declare)
+    e = gp.exponent(nu)
+    assert e != 0.0
+    top, bot, _ = rational.brasil_factors(e, 4)
+    gp.set_rational(e, top, bot)
+    sigma, kap = 0.8, np.sqrt(2 * nu) / 0.3
+    o = hip.default_opts(tol=1e-12)
+    sets = [[sigma, kap, nu, eps] for eps in (0.03, 0.1, 0.4)]
+    ll, st, stats = gp.giveLogLikelihood(sets, o)
+    assert np.all(st == 0)
+    if declare:
+        assert stats["precond"] == 2
+    Cu, (rp, ci, va, mass) = _exact_fractional(orc, m, sigma, kap, nu, top, bot)
+    Qv, Pr = orc.gpq_precision_frac_dense(rp, ci, va, mass, sigma, kap, nu, m.dim, top, bot)
+    ns, no = Y.shape
+    Pb = np.vstack([P] * no)
+    yv = Y.T.reshape(-1)
+    for row, l in zip(sets, ll):
+        K = Pb @ Cu @ Pb.T + row[3] ** 2 * np.eye(ns * no)
+        _, ld = np.linalg.slogdet(K)
+        exact = -0.5 * (yv @ np.linalg.solve(K, yv) + ld + ns * no * np.log(2 * np.pi))
+        assert abs(l - exact) <= 1e-8 * abs(exact), (row, l, exact)
+        # the reference algorithm on Q_v (condition ~ cond(Q) cond(P_l)^2)
+        ref = orc.gpq_loglik(Qv, P, Y, row[3], Pr)
+        assert abs(l - ref) <= 1e-6 * abs(ref), (row, l, ref)
+    hp = [sigma, kap, nu, 0.1]
+    mean = gp.predictMean(hp, o)
+    cov = gp.predictCovariance(hp, o)
+    K = Pb @ Cu @ Pb.T + hp[3] ** 2 * np.eye(ns * no)
+    mref = Cu @ Pb.T @ np.linalg.solve(K, yv)
+    cref = Cu - Cu @ Pb.T @ np.linalg.solve(K, Pb @ Cu)
+    assert np.abs(mean - mref).max() <= 1e-8 * np.abs(mref).max()
+    assert np.abs(cov - cref).max() <= 1e-8 * np.abs(cref).max()
     gp.close()

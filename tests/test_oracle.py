@@ -493,6 +493,87 @@ def test_gpq_posterior_is_the_gaussian_conditional(orc):
     assert np.abs(cov - cov_ref).max() <= 1e-10 * np.abs(cov_ref).max()
 
 
+def _gpq_frac_case(orc, nu, dim=2, degree=4):
+    from statfem_amd import rational
+    from scipy.linalg import eigh
+    m = M.grid2d(8, 7) if dim == 2 else M.line1d(40)
+    kap = np.sqrt(2 * nu) / 0.3
+    rp, ci, va, _ = orc.assemble(m, orc.MATERN, kap, 1.0, nq1d=2, factor=1.0 / kap ** 2)
+    mass = orc.body_force(m, nq1d=2, fconst=1.0)
+    beta = (nu + dim / 2) / 2
+    alpha = max(1.0, np.floor(beta))
+    top, bot, _ = rational.brasil_factors(beta - alpha, degree)
+    n = m.nnodes
+    A = orc.spmv(rp, ci, va, np.eye(n))
+    lam, V = eigh(A, np.diag(mass))            # A V = M V diag(lam), V^T M V = I
+    return m, kap, rp, ci, va, mass, beta, alpha, top, bot, lam, V
+
+
+@pytest.mark.parametrize("nu,dim", [(0.5, 2), (1.5, 2), (1.0, 1)])
+def test_gpq_fractional_factors_are_spectral_polynomials(orc, nu, dim):
+    """P_l and P_r of the oracle against the generalized eigenvectors of
+    (A, M): lead prod_j (I - r_j M^-1 A) = V diag(lead prod_j (1 - r_j lam)) V^T M."""
+    m, kap, rp, ci, va, mass, beta, alpha, top, bot, lam, V = _gpq_frac_case(orc, nu, dim)
+    for roots in (top, bot):
+        ref = V @ np.diag(roots[0] * np.prod(1.0 - np.outer(lam, roots[1:]), axis=1)) @ V.T @ np.diag(mass)
+        got = orc.gpq_frac_factor(rp, ci, va, mass, roots)
+        assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("nu,dim", [(0.5, 2), (1.5, 2), (1.0, 1)])
+def test_gpq_fractional_covariance_is_the_spectral_power(orc, nu, dim):
+    """The field covariance P_r Q_v^-1 P_r^T of the oracle's fractional precision
+    is (1/c) V diag(r(1/lam)^2 lam^(-2 alpha)) V^T exactly (r the factor form
+    a prod (x - c_i) / (b prod (x - d_j))) and, within the rational
+    approximation's error, the fractional Matern covariance (1/c) V lam^(-2 beta) V^T."""
+    from statfem_amd import rational
+    m, kap, rp, ci, va, mass, beta, alpha, top, bot, lam, V = _gpq_frac_case(orc, nu, dim)
+    sigma = 0.7
+    Qv, Pr = orc.gpq_precision_frac_dense(rp, ci, va, mass, sigma, kap, nu, dim, top, bot)
+    C = Pr @ np.linalg.solve(Qv, Pr.T)
+    c = orc.matern_tau(sigma, kap, nu, dim) ** 2 * kap ** (4 * beta)
+    rr = rational.factor_form(top, bot, 1.0 / lam)
+    exact = V @ np.diag(rr ** 2 * lam ** (-2 * alpha)) @ V.T / c
+    # (Q_v's condition is cond(Q) times cond(P_l)^2, ~1e10 in 1D here)
+    assert np.abs(C - exact).max() <= 1e-6 * np.abs(exact).max()
+    frac = V @ np.diag(lam ** (-2 * beta)) @ V.T / c
+    x = 1.0 / lam
+    relerr = np.abs(rr / x ** (beta - alpha) - 1.0).max()
+    assert relerr < 0.05
+    assert np.abs(C - frac).max() <= 2.5 * relerr * np.abs(frac).max()
+    # the integer part alone is off by the missing power
+    assert np.abs(V @ np.diag(lam ** (-2 * alpha)) @ V.T / c - frac).max() > 10 * np.abs(C - frac).max()
+
+
+def test_gpq_fractional_loglik_and_posterior_are_the_joint_gaussian(orc):
+    """With u = P_r v, v ~ N(0, Q_v^-1): the reference's two-log-determinant
+    likelihood and its P_r-projected posterior against the joint density and
+    the Gaussian conditional of (u, y_1 .. y_no) by brute force."""
+    m, kap, rp, ci, va, mass, beta, alpha, top, bot, lam, V = _gpq_frac_case(orc, 0.5)
+    Qv, Pr = orc.gpq_precision_frac_dense(rp, ci, va, mass, 0.7, kap, 0.5, 2, top, bot)
+    rng = np.random.default_rng(5)
+    ns, no = 9, 3
+    P = np.zeros((ns, m.nnodes))
+    for i, j in enumerate(rng.choice(m.nnodes, ns, replace=False)):
+        P[i, j] = 1.0
+    Y = 0.3 * rng.standard_normal((ns, no))
+    Cu = Pr @ np.linalg.solve(Qv, Pr.T)
+    Pb = np.vstack([P] * no)
+    yv = Y.T.reshape(-1)
+    for eps in (0.05, 0.2):
+        K = Pb @ Cu @ Pb.T + eps * eps * np.eye(ns * no)
+        _, ld = np.linalg.slogdet(K)
+        ref = -0.5 * (yv @ np.linalg.solve(K, yv) + ld + ns * no * np.log(2 * np.pi))
+        assert abs(orc.gpq_loglik(Qv, P, Y, eps, Pr) - ref) <= 1e-9 * abs(ref)
+    eps = 0.1
+    K = Pb @ Cu @ Pb.T + eps * eps * np.eye(ns * no)
+    mean_ref = Cu @ Pb.T @ np.linalg.solve(K, yv)
+    cov_ref = Cu - Cu @ Pb.T @ np.linalg.solve(K, Pb @ Cu)
+    mean, cov = orc.gpq_predict(Qv, P, Y, eps, Pr)
+    assert np.abs(mean - mean_ref).max() <= 1e-9 * np.abs(mean_ref).max()
+    assert np.abs(cov - cov_ref).max() <= 1e-9 * np.abs(cov_ref).max()
+
+
 def test_precision_beta2_is_matern_7_2(orc):
     """beta = 2 (nu = 7/2 in 1D, Q = c (A M^-1)^3 A): the covariance Q^-1 is the
     Matern-7/2 kernel sigma^2 e^-a (1 + a + 2a^2/5 + a^3/15), a = sqrt(7) r / ell,
