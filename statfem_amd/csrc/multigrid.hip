@@ -36,66 +36,66 @@ __host__ __device__ inline double pweight(int x, int X, int n) {
     return ((x - 1) / 2 == X || (x + 1) / 2 == X) ? 0.5 : 0.0;
 }
 
-// coarse parents of fine index x: count, indices and weights
-__device__ inline int parents(int x, int n, int* P, double* w) {
-    if ((x & 1) == 0) { P[0] = x / 2; w[0] = 1.0; return 1; }
-    if (x == n - 1) { P[0] = (x - 1) / 2; w[0] = 1.0; return 1; }
-    P[0] = (x - 1) / 2; w[0] = 0.5;
-    P[1] = (x + 1) / 2; w[1] = 0.5;
-    return 2;
-}
-
-// A_c = P^T A P on coarse node I: sum over the fine nodes i of I's support and
-// their stencil neighbours j of P[i][I] a_ij P[j][J]. cmask (level 0 only)
-// drops constrained fine nodes from P. A coarse node without free support
-// gets a decoupled unit row.
-__device__ inline void galerkin_node(int W, int H, int Wc, int I, const double* __restrict__ A,
-                                     const unsigned char* __restrict__ cmask, double* __restrict__ Ac) {
-    const int Ix = I % Wc, Iy = I / Wc;
-    double a9[9];
-    for (int k = 0; k < 9; ++k) a9[k] = 0.0;
-    for (int dy = -1; dy <= 1; ++dy) {
-        const int iy = 2 * Iy + dy;
-        if (iy < 0 || iy >= H) continue;
-        const double wy = pweight(iy, Iy, H);
-        if (wy == 0.0) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int ix = 2 * Ix + dx;
-            if (ix < 0 || ix >= W) continue;
-            const double wi = wy * pweight(ix, Ix, W);
-            const int i = iy * W + ix;
-            if (wi == 0.0 || (cmask && cmask[i])) continue;
-            for (int k = 0; k < 9; ++k) {
-                const int jx = ix + k % 3 - 1, jy = iy + k / 3 - 1;
-                if (jx < 0 || jx >= W || jy < 0 || jy >= H) continue;
-                const double a = A[(size_t)i * 9 + k];
-                if (a == 0.0 || (cmask && cmask[jy * W + jx])) continue;
-                int PX[2], PY[2];
-                double WX[2], WY[2];
-                const int nx = parents(jx, W, PX, WX), ny = parents(jy, H, PY, WY);
-                for (int u = 0; u < ny; ++u)
-                    for (int v = 0; v < nx; ++v)
-                        a9[(PY[u] - Iy + 1) * 3 + (PX[v] - Ix + 1)] += wi * a * WY[u] * WX[v];
+// One coarsening step of every grid that has level l + 1, in one launch:
+// work-group row blockIdx.y is grid tab[6 q ..] = {W, H, Wc, Hc, fine base,
+// coarse base} (bases: first node of the grid in its level's arrays).
+// A_c = P^T A P, one coefficient per lane: lane K of a group of nine forms
+// coefficient K of coarse node I (seven nodes per 64-lane wave, lane 63 idle)
+// as the sum over the fine nodes i of I's support and their stencil
+// neighbours j of P[i][I] a_ij P[j][J], J = I + offset(K). cmask (level 0
+// only) drops constrained fine nodes from P; a coarse node without free
+// support gets a decoupled unit row. Each lane sums in a fixed (i, stencil
+// entry) order into one register, with no nine-entry array indexed at run time.
+constexpr int kGalNodesPerWave = 7;
+__global__ __launch_bounds__(256) void k_mg_galerkin9(const int* __restrict__ tab, const double* __restrict__ A,
+                                                      const unsigned char* __restrict__ cmask,
+                                                      double* __restrict__ Ac) {
+    const int* t = tab + 6 * blockIdx.y;
+    const int W = t[0], H = t[1], Wc = t[2], Hc = t[3], fb = t[4], cb = t[5];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int node = lane / 9, K = lane - 9 * node;
+    const int I = (blockIdx.x * (blockDim.x >> 6) + wave) * kGalNodesPerWave + node;
+    const bool live = node < kGalNodesPerWave && I < Wc * Hc;
+    double v = 0.0;
+    if (live) {
+        const double* Af = A + (size_t)fb * 9;
+        const unsigned char* cm = cmask ? cmask + fb : nullptr;
+        const int Ix = I % Wc, Iy = I / Wc;
+        const int Jx = Ix + K % 3 - 1, Jy = Iy + K / 3 - 1;
+        if (Jx >= 0 && Jx < Wc && Jy >= 0 && Jy < Hc) {
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int iy = 2 * Iy + dy;
+                if (iy < 0 || iy >= H) continue;
+                const double wy = pweight(iy, Iy, H);
+                if (wy == 0.0) continue;
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int ix = 2 * Ix + dx;
+                    if (ix < 0 || ix >= W) continue;
+                    const double wi = wy * pweight(ix, Ix, W);
+                    const int i = iy * W + ix;
+                    if (wi == 0.0 || (cm && cm[i])) continue;
+                    // neighbours j of i inside J's support (|j - 2J| <= 1 per axis),
+                    // in ascending stencil-entry order
+                    const int jy0 = max(max(iy - 1, 2 * Jy - 1), 0), jy1 = min(min(iy + 1, 2 * Jy + 1), H - 1);
+                    const int jx0 = max(max(ix - 1, 2 * Jx - 1), 0), jx1 = min(min(ix + 1, 2 * Jx + 1), W - 1);
+                    for (int jy = jy0; jy <= jy1; ++jy) {
+                        const double wyj = pweight(jy, Jy, H);
+                        if (wyj == 0.0) continue;
+                        for (int jx = jx0; jx <= jx1; ++jx) {
+                            const double wxj = pweight(jx, Jx, W);
+                            if (wxj == 0.0) continue;
+                            const double a = Af[(size_t)i * 9 + (jy - iy + 1) * 3 + (jx - ix + 1)];
+                            if (a == 0.0 || (cm && cm[jy * W + jx])) continue;
+                            v += wi * a * wyj * wxj;
+                        }
+                    }
+                }
             }
         }
     }
-    if (!(a9[4] > 0.0)) {
-        for (int k = 0; k < 9; ++k) a9[k] = 0.0;
-        a9[4] = 1.0;
-    }
-    for (int k = 0; k < 9; ++k) Ac[(size_t)I * 9 + k] = a9[k];
-}
-
-// one coarsening step of every grid that has level l + 1, in one launch:
-// work-group row blockIdx.y is grid tab[6 q ..] = {W, H, Wc, Hc, fine base,
-// coarse base} (bases: first node of the grid in its level's arrays)
-__global__ void k_mg_galerkin(const int* __restrict__ tab, const double* __restrict__ A,
-                              const unsigned char* __restrict__ cmask, double* __restrict__ Ac) {
-    const int* t = tab + 6 * blockIdx.y;
-    const int W = t[0], H = t[1], Wc = t[2], Hc = t[3], fb = t[4], cb = t[5];
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= Wc * Hc) return;
-    galerkin_node(W, H, Wc, I, A + (size_t)fb * 9, cmask ? cmask + fb : nullptr, Ac + (size_t)cb * 9);
+    // a coarse node without free support gets a decoupled unit row
+    const double centre = __shfl(v, node * 9 + 4, 64);
+    if (live) Ac[((size_t)cb + I) * 9 + K] = centre > 0.0 ? v : (K == 4 ? 1.0 : 0.0);
 }
 
 // smoother weights omega / sum_j |a_ij| and (levels >= 1) the FP32 stencil
@@ -167,6 +167,100 @@ __global__ __launch_bounds__(256) void k_mg_coarse_inverse(const int* __restrict
     }
     if (tid == 0) bad[q] = 0;
     for (int e = tid; e < n * n; e += bs) out[e] = (float)A[e];
+}
+
+// The same elimination with the matrix in registers, for n <= kMGCoarse:
+// 1024 threads, thread (r, c) = (tid / 32, tid % 32) holds the entries of rows
+// r + 32 a and columns c + 32 b. Each step publishes the pivot column and the
+// pivot row through LDS (two buffers, so one barrier per step) and every
+// entry is updated with the operations of k_mg_coarse_inverse, so the inverse
+// is the same. The LDS kernel's step is a chain of dependent LDS loads and
+// stores; here the loads of a step are independent.
+constexpr int kCinvR = (kMGCoarse + 31) / 32;
+__global__ __launch_bounds__(1024) void k_mg_coarse_inverse_reg(const int* __restrict__ tab,
+                                                                const double* __restrict__ a9,
+                                                                float* __restrict__ inv, int* __restrict__ bad) {
+    __shared__ double colb[2][kCinvR * 32], rowb[2][kCinvR * 32];
+    const int* t = tab + 6 * blockIdx.x;
+    const int W = t[0], n = W * t[1], q = t[5];
+    const double* s9 = a9 + (size_t)t[2] * 9;
+    float* out = inv + t[3];
+    const int c = threadIdx.x & 31, r = threadIdx.x >> 5;
+    double v[kCinvR][kCinvR];
+#pragma unroll
+    for (int a = 0; a < kCinvR; ++a)
+#pragma unroll
+        for (int b = 0; b < kCinvR; ++b) {
+            const int i = r + 32 * a, j = c + 32 * b;
+            double x = 0.0;
+            if (i < n && j < n) {
+                const int dx = j % W - i % W, dy = j / W - i / W;
+                if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) {
+                    const int k = (dy + 1) * 3 + (dx + 1);
+                    x = i == j ? s9[(size_t)i * 9 + 4] : 0.5 * (s9[(size_t)i * 9 + k] + s9[(size_t)j * 9 + (8 - k)]);
+                }
+            }
+            v[a][b] = x;
+        }
+    for (int k = 0; k < n; ++k) {
+        double* cb = colb[k & 1];
+        double* rb = rowb[k & 1];
+        const int kh = k >> 5, kl = k & 31;
+        if (c == kl)   // owners of column k
+#pragma unroll
+            for (int a = 0; a < kCinvR; ++a) {
+                double x = v[a][0];
+#pragma unroll
+                for (int b = 1; b < kCinvR; ++b) x = b == kh ? v[a][b] : x;
+                if (r + 32 * a < n) cb[r + 32 * a] = x;
+            }
+        if (r == kl)   // owners of row k
+#pragma unroll
+            for (int b = 0; b < kCinvR; ++b) {
+                double x = v[0][b];
+#pragma unroll
+                for (int a = 1; a < kCinvR; ++a) x = a == kh ? v[a][b] : x;
+                if (c + 32 * b < n) rb[c + 32 * b] = x;
+            }
+        __syncthreads();
+        const double p = rb[k];
+        if (!(p > 0.0)) {   // the same value in every thread
+            if (threadIdx.x == 0) bad[q] = 1;
+            return;
+        }
+        const double pk = 1.0 / p;
+        double sj[kCinvR], ci[kCinvR];
+#pragma unroll
+        for (int b = 0; b < kCinvR; ++b) {
+            const int j = c + 32 * b;
+            sj[b] = j < n ? rb[j] / p : 0.0;   // the scaled pivot row
+        }
+#pragma unroll
+        for (int a = 0; a < kCinvR; ++a) {
+            const int i = r + 32 * a;
+            ci[a] = i < n ? cb[i] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < kCinvR; ++a)
+#pragma unroll
+            for (int b = 0; b < kCinvR; ++b) {
+                const int i = r + 32 * a, j = c + 32 * b;
+                if (i < n && j < n) {
+                    if (i == k)
+                        v[a][b] = j == k ? pk : sj[b];
+                    else
+                        v[a][b] = j == k ? -ci[a] * pk : v[a][b] - ci[a] * sj[b];
+                }
+            }
+    }
+    if (threadIdx.x == 0) bad[q] = 0;
+#pragma unroll
+    for (int a = 0; a < kCinvR; ++a)
+#pragma unroll
+        for (int b = 0; b < kCinvR; ++b) {
+            const int i = r + 32 * a, j = c + 32 * b;
+            if (i < n && j < n) out[(size_t)i * n + j] = (float)v[a][b];
+        }
 }
 
 }  // namespace
@@ -261,7 +355,8 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
         const double* Af = l == 0 ? s->dia64.as<double>() : m.coef64[l].as<double>();
         for (int r0 = 0; r0 < grow[l]; r0 += 65535) {
             const int rn = std::min(65535, grow[l] - r0);
-            k_mg_galerkin<<<dim3((gmax[l] + 255) / 256, rn), 256, 0, st>>>(
+            constexpr int per_block = kGalNodesPerWave * 4;   // 256 threads
+            k_mg_galerkin9<<<dim3((gmax[l] + per_block - 1) / per_block, rn), 256, 0, st>>>(
                 m.gtab.as<int>() + gofs[l] + 6 * (size_t)r0, Af, l == 0 ? s->cmask.as<unsigned char>() : nullptr,
                 m.coef64[l + 1].as<double>());
             SFH_LAUNCH_CHECK();
@@ -281,7 +376,10 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
         int nmax = 0;
         for (int r = 0; r < crow[l]; ++r) nmax = std::max(nmax, tab[cofs[l] + 6 * r] * tab[cofs[l] + 6 * r + 1]);
         const size_t lds = sizeof(double) * (size_t)nmax * nmax;
-        if (lds + sizeof(double) * kMGCoarse <= s->ctx->lds_per_block)
+        if (nmax <= kCinvR * 32)
+            k_mg_coarse_inverse_reg<<<crow[l], 1024, 0, st>>>(m.gtab.as<int>() + cofs[l], m.coef64[l].as<double>(),
+                                                             m.cinv.as<float>(), m.cbad.as<int>());
+        else if (lds + sizeof(double) * kMGCoarse <= s->ctx->lds_per_block)
             k_mg_coarse_inverse<true><<<crow[l], 256, lds, st>>>(m.gtab.as<int>() + cofs[l], m.coef64[l].as<double>(),
                                                                  m.cwork.as<double>(), m.cinv.as<float>(),
                                                                  m.cbad.as<int>());
