@@ -206,9 +206,15 @@ __global__ __launch_bounds__(kB) void k_rescale(int kp, Seg G, const double* __r
 constexpr int kTX = 32, kTY = 6, kG = 512;
 constexpr int kHN = (kTX + 2) * (kTY + 2);   // staged nodes per tile
 
+// threads per work-group of the tile kernels: kG, or 128 lane groups when
+// LPR < 4 (a tile then keeps every lane busy; at kG threads and one lane per
+// node most lane groups would have no node of the tile)
+template <int LPR> struct GB { static constexpr int v = LPR >= 4 ? kG : 128 * LPR; };
+inline int grid_block(int lpr) { return lpr >= 4 ? kG : 128 * lpr; }
+
 inline size_t grid_lds_bytes(int lpr) {
     const size_t tile = (size_t)kHN * lpr * sizeof(float4);
-    const size_t red = (size_t)kG * 4 * sizeof(double);   // block_accumulate after the tile is consumed
+    const size_t red = (size_t)grid_block(lpr) * 4 * sizeof(double);   // block_accumulate after the tile is consumed
     return tile > red ? tile : red;
 }
 
@@ -252,7 +258,7 @@ __device__ inline V4<float> stencil9(const float4* sm, const float* __restrict__
 }
 
 template <int LPR>
-__global__ __launch_bounds__(kG) void k_dir_spmv_t2(const int* __restrict__ gtab, int kp, int ks, int it,
+__global__ __launch_bounds__(GB<LPR>::v) void k_dir_spmv_t2(const int* __restrict__ gtab, int kp, int ks, int it,
                                                     const float* __restrict__ coef, double* __restrict__ S,
                                                     const float* __restrict__ z, const float* __restrict__ pold,
                                                     float* __restrict__ pnew, double* __restrict__ acc) {
@@ -300,7 +306,7 @@ __global__ __launch_bounds__(kG) void k_dir_spmv_t2(const int* __restrict__ gtab
 }
 
 template <int LPR>
-__global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gtab, int kp, int ks, int it,
+__global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_t2(const int* __restrict__ gtab, int kp, int ks, int it,
                                                      const float* __restrict__ coef, const float* __restrict__ dinv,
                                                      const float* __restrict__ diag, const double* __restrict__ S,
                                                      const float* __restrict__ p, float* __restrict__ x,
@@ -370,7 +376,7 @@ __global__ __launch_bounds__(kG) void k_cg_update_t2(const int* __restrict__ gta
 template <int LPR>
 __device__ inline void stage_grid_f32(float4* sm, const float* __restrict__ v, int kp, int lc, bool ok, int base,
                                       int W, int H, int tx0, int ty0, int grp, int lane) {
-    constexpr int ngrp = kG / LPR, kSt = (kHN + ngrp - 1) / ngrp;
+    constexpr int ngrp = GB<LPR>::v / LPR, kSt = (kHN + ngrp - 1) / ngrp;
     float4 a[kSt];
     bool in[kSt];
 #pragma unroll
@@ -405,7 +411,7 @@ __device__ inline V4<double> stencil9d(const float4* sm, const double* __restric
 
 // p_new = m z + b p_old (FP32), p.Ap in FP64 from the FP64 stencil
 template <int LPR>
-__global__ __launch_bounds__(kG) void k_dir_spmv_mg(const int* __restrict__ gtab, int kp, int ks, int it,
+__global__ __launch_bounds__(GB<LPR>::v) void k_dir_spmv_mg(const int* __restrict__ gtab, int kp, int ks, int it,
                                                     const double* __restrict__ coef, double* __restrict__ S,
                                                     const float* __restrict__ z, const float* __restrict__ pold,
                                                     float* __restrict__ pnew, double* __restrict__ acc) {
@@ -424,7 +430,7 @@ __global__ __launch_bounds__(kG) void k_dir_spmv_mg(const int* __restrict__ gtab
     const V4<float> b = {cb[lc], cb[lc + 1], cb[lc + 2], cb[lc + 3]};
     const V4<float> m = {cm[lc], cm[lc + 1], cm[lc + 2], cm[lc + 3]};
     {
-        constexpr int ngrp = kG / LPR, kSt = (kHN + ngrp - 1) / ngrp;
+        constexpr int ngrp = GB<LPR>::v / LPR, kSt = (kHN + ngrp - 1) / ngrp;
         const int l0 = ok ? t.col0 : 0;
         V4<float> za[kSt], pa[kSt];
         bool in[kSt];
@@ -494,7 +500,7 @@ __global__ __launch_bounds__(kG) void k_dir_spmv_mg(const int* __restrict__ gtab
 // the residual scaled by the column's s (unit norm at the last replacement), x
 // the iterate itself, so no correction has to be flushed into it
 template <int LPR>
-__global__ __launch_bounds__(kG) void k_cg_update_mg(const int* __restrict__ gtab, int kp, int ks, int it,
+__global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restrict__ gtab, int kp, int ks, int it,
                                                      const double* __restrict__ coef, const double* __restrict__ S,
                                                      const float* __restrict__ p, double* __restrict__ x,
                                                      double* __restrict__ r, double* __restrict__ acc) {
@@ -596,7 +602,7 @@ constexpr int kMT = 9;
 inline size_t mg_restrict_lds(int lpr) { return (size_t)kRW * kRH * (lpr * sizeof(float4) + sizeof(float)); }
 inline size_t mg_prolong_lds(int lpr) {
     const size_t tile = (size_t)(kPW * kPH + kCW * kCH) * lpr * sizeof(float4);
-    const size_t red = (size_t)kG * 4 * sizeof(double);
+    const size_t red = (size_t)grid_block(lpr) * 4 * sizeof(double);
     return tile > red ? tile : red;
 }
 
@@ -634,14 +640,14 @@ __device__ inline bool mg_cols_done(const double* __restrict__ S, int ks, int so
 }
 
 template <int LPR, class TR>
-__global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab, int kp, const float* __restrict__ coef,
+__global__ __launch_bounds__(GB<LPR>::v) void k_mg_restrict(const int* __restrict__ tab, int kp, const float* __restrict__ coef,
                                                     const float* __restrict__ dsm,
                                                     const unsigned char* __restrict__ cmask,
                                                     const TR* __restrict__ r, float* __restrict__ rc,
                                                     const double* __restrict__ S, int ks) {
     extern __shared__ float4 sm[];
     float* sd = reinterpret_cast<float*>(sm + kRW * kRH * LPR);
-    constexpr int ngrp = kG / LPR;
+    constexpr int ngrp = GB<LPR>::v / LPR;
     constexpr int kIW = 2 * kRX + 1, kIn = kIW * (2 * kRY + 1);   // fine nodes whose residual is restricted
     constexpr int kPer = (kIn + ngrp - 1) / ngrp;
     constexpr int kSt = (kRW * kRH + ngrp - 1) / ngrp;
@@ -744,7 +750,7 @@ __global__ __launch_bounds__(kG) void k_mg_restrict(const int* __restrict__ tab,
 // per-tile partial rows of r.e and r.r (after a residual replacement; padding
 // tiles write zeros)
 template <int LPR, bool TOP, class TR>
-__global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, int kp, int ks, int it, int mode,
+__global__ __launch_bounds__(GB<LPR>::v) void k_mg_prolong(const int* __restrict__ tab, int kp, int ks, int it, int mode,
                                                    const float* __restrict__ coef, const float* __restrict__ dsm,
                                                    const unsigned char* __restrict__ cmask,
                                                    const TR* __restrict__ r, const float* __restrict__ ec,
@@ -753,7 +759,7 @@ __global__ __launch_bounds__(kG) void k_mg_prolong(const int* __restrict__ tab, 
                                                    const double* __restrict__ S) {
     extern __shared__ float4 sm[];
     float4* sc = sm + kPW * kPH * LPR;
-    constexpr int ngrp = kG / LPR;
+    constexpr int ngrp = GB<LPR>::v / LPR;
     constexpr int kCs = (kCW * kCH + ngrp - 1) / ngrp;   // staged coarse nodes per lane group
     constexpr int kSt = (kPW * kPH + ngrp - 1) / ngrp;   // staged fine nodes per lane group
     Tile t;
@@ -1455,26 +1461,26 @@ struct Kern {
     static void dir_spmv_g(const Plan& P, hipStream_t st, int it, const T* coef, double* S, const T* z, const T* pold,
                            T* pnew, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_dir_spmv_t2<LPR><<<dim3(P.gblk, P.ntile), kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, S, z, pold,
+            k_dir_spmv_t2<LPR><<<dim3(P.gblk, P.ntile), GB<LPR>::v, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, S, z, pold,
                                                                        pnew, acc);
     }
     static void update_g(const Plan& P, hipStream_t st, int it, const T* coef, const T* dinv, const T* diag,
                          const double* S, const T* p, T* x, T* z, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_cg_update_t2<LPR><<<dim3(P.gblk, P.ntile), kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, dinv, diag,
+            k_cg_update_t2<LPR><<<dim3(P.gblk, P.ntile), GB<LPR>::v, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef, dinv, diag,
                                                                         S, p, x, z, acc);
     }
     // multigrid-preconditioned iteration: FP64 x, r and stencil products
     static void dir_spmv_mg(const Plan& P, hipStream_t st, int it, const double* coef64, double* S, const T* z,
                             const T* pold, T* pnew, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_dir_spmv_mg<LPR><<<dim3(P.gblk, P.ntile), kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef64, S, z, pold,
+            k_dir_spmv_mg<LPR><<<dim3(P.gblk, P.ntile), GB<LPR>::v, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef64, S, z, pold,
                                                                        pnew, acc);
     }
     static void update_mg(const Plan& P, hipStream_t st, int it, const double* coef64, const double* S, const T* p,
                           double* x, double* r, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_cg_update_mg<LPR><<<dim3(P.gblk, P.ntile), kG, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef64, S, p, x, r,
+            k_cg_update_mg<LPR><<<dim3(P.gblk, P.ntile), GB<LPR>::v, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef64, S, p, x, r,
                                                                         acc);
     }
     // z = M r (one V-cycle) with r.z into the iteration's slot (mode 0) or r.z,
@@ -1490,20 +1496,20 @@ struct Kern {
             auto rvec = [&](int l) -> float* { return mgv + P.mgv_off[l]; };
             auto evec = [&](int l) -> float* { return l == 0 ? z : mgv + P.mgv_off[l] + (size_t)m.N[l] * kp; };
             const double* Sk = mode == 0 ? S : nullptr;   // work-group skips only inside an iteration
-            k_mg_restrict<ML, double><<<dim3(P.mg_rn[0], ntm), kG, mg_restrict_lds(ML), st>>>(
+            k_mg_restrict<ML, double><<<dim3(P.mg_rn[0], ntm), GB<ML>::v, mg_restrict_lds(ML), st>>>(
                 P.d_mgtab + (size_t)kMT * P.mg_roff[0], kp, coef0, m.dsm[0].as<float>(), cmask, r0, rvec(1), Sk,
                 P.ks);
             for (int l = 1; l < L; ++l)
-                k_mg_restrict<ML, float><<<dim3(P.mg_rn[l], ntm), kG, mg_restrict_lds(ML), st>>>(
+                k_mg_restrict<ML, float><<<dim3(P.mg_rn[l], ntm), GB<ML>::v, mg_restrict_lds(ML), st>>>(
                     P.d_mgtab + (size_t)kMT * P.mg_roff[l], kp, m.coef32[l].as<float>(), m.dsm[l].as<float>(),
                     nullptr, rvec(l), rvec(l + 1), Sk, P.ks);
             k_mg_coarse<<<dim3((kp + kCoarseCols - 1) / kCoarseCols, m.nseg), 256, mg_coarse_lds(m.cmax), st>>>(
                 P.d_mgctab, kp, m.cinv.as<float>(), mgv);
             for (int l = L - 1; l >= 1; --l)
-                k_mg_prolong<LPR, false, float><<<dim3(P.mg_pn[l], P.ntile), kG, mg_prolong_lds(LPR), st>>>(
+                k_mg_prolong<LPR, false, float><<<dim3(P.mg_pn[l], P.ntile), GB<LPR>::v, mg_prolong_lds(LPR), st>>>(
                     P.d_mgtab + (size_t)kMT * P.mg_poff[l], kp, P.ks, it, mode, m.coef32[l].as<float>(),
                     m.dsm[l].as<float>(), nullptr, rvec(l), evec(l + 1), evec(l), nullptr, nullptr, nullptr, Sk);
-            k_mg_prolong<LPR, true, double><<<dim3(P.mg_pn[0], P.ntile), kG, mg_prolong_lds(LPR), st>>>(
+            k_mg_prolong<LPR, true, double><<<dim3(P.mg_pn[0], P.ntile), GB<LPR>::v, mg_prolong_lds(LPR), st>>>(
                 P.d_mgtab + (size_t)kMT * P.mg_poff[0], kp, P.ks, it, mode, coef0, m.dsm[0].as<float>(), cmask, r0,
                 evec(1), z, acc, p0, p1, Sk);
         }
