@@ -74,6 +74,7 @@ struct SolveWork {
     DevBuf acc;                      // per-column FP64 accumulators of the iteration kernels
     DevBuf tab;                      // row-partition tables of the current solve
     DevBuf mgv;                      // multigrid: restricted residuals and corrections of the coarse levels
+    DevBuf cnb;                      // rows with a constrained neighbour (forward solves)
 };
 
 // Geometric multigrid hierarchies of a batch of 2D tensor grids (one grid for

@@ -1124,6 +1124,44 @@ __global__ __launch_bounds__(kB) void k_resid64(int kp, Seg G, const int* __rest
     block_partial<LPR>(t, kp, acc, red, part);
 }
 
+// FP64 residual r = b - A x on the declared grids (tile table of the iteration
+// kernels) from the FP64 stencil, each row's products in the order of its CSR
+// entries (ascending columns), so r is the CSR residual's; |r|^2 is added into
+// rn2 per segment and column (cleared before the launch).
+template <int LPR>
+__global__ __launch_bounds__(GB<LPR>::v) void k_resid_grid64(const int* __restrict__ gtab, int kp,
+                                                            const double* __restrict__ coef,
+                                                            const double* __restrict__ x,
+                                                            const double* __restrict__ b, double* __restrict__ r,
+                                                            double* __restrict__ rn2) {
+    extern __shared__ float4 sm[];
+    int d[kGT];
+    const Tile t = grid_tile_t<LPR>(kp, gtab, d);
+    const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
+    if (W == 0) return;
+    V4<double> acc = {0.0, 0.0, 0.0, 0.0};
+    if (t.col0 < kp) {
+        for (int o = t.grp; o < kTX * kTY; o += t.ngrp) {
+            const int gx = tx0 + o % kTX, gy = ty0 + o / kTX;
+            if (gx >= W || gy >= H) continue;
+            const size_t nd = (size_t)base + (size_t)gy * W + gx;
+            V4<double> sv = ld4(b + nd * kp + t.col0);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int jx = gx + k % 3 - 1, jy = gy + k / 3 - 1;
+                if (jx < 0 || jx >= W || jy < 0 || jy >= H) continue;
+                const double a = coef[nd * 9 + k];
+                const V4<double> v = ld4(x + ((size_t)base + (size_t)jy * W + jx) * kp + t.col0);
+                sv.x -= a * v.x; sv.y -= a * v.y; sv.z -= a * v.z; sv.w -= a * v.w;
+            }
+            st4(r + nd * kp + t.col0, sv);
+            acc.x += sv.x * sv.x; acc.y += sv.y * sv.y; acc.z += sv.z * sv.z; acc.w += sv.w * sv.w;
+        }
+    }
+    __syncthreads();
+    block_accumulate<LPR>(t, kp, acc, reinterpret_cast<double*>(sm), rn2 + t.so);
+}
+
 // x64 += x_inner / scale on active columns (scale > 0); x_inner restarts at 0.
 // srow: nseg + 1 segment row bounds (the row's segment selects its scales).
 template <class T>
@@ -1234,17 +1272,21 @@ __global__ void k_pack_entries(long long nnz, const int* __restrict__ ci, const 
 
 // b64 (N x kp): free rows carry B (non-transposed: minus the constrained-column
 // couplings A[f][c] x_c with x_c = B_c / A[c][c]); constrained rows are zero.
+// (cnb: rows with a constrained neighbour, nullptr when there is none; I: the
+// index type of n kp)
+template <class I>
 __global__ void k_prepare(int n, int k, int kp, int trans, const double* __restrict__ B, int ldb,
                           const int* __restrict__ rp, const int* __restrict__ ci,
                           const double* __restrict__ va, const int* __restrict__ diag,
-                          const unsigned char* __restrict__ cmask, double* __restrict__ b64) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)n * kp) return;
-    const int i = (int)(t / kp), c = (int)(t % kp);
+                          const unsigned char* __restrict__ cmask, const unsigned char* __restrict__ cnb,
+                          double* __restrict__ b64) {
+    const I t = (I)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (I)n * kp) return;
+    const int i = (int)(t / (I)kp), c = (int)(t - (I)i * kp);
     double v = 0.0;
     if (c < k && !cmask[i]) {
         v = B[(size_t)i * ldb + c];
-        if (!trans) {
+        if (!trans && cnb && cnb[i]) {
             for (int e = rp[i]; e < rp[i + 1]; ++e) {
                 const int j = ci[e];
                 if (cmask[j]) v -= va[e] * (B[(size_t)j * ldb + c] / va[diag[j]]);
@@ -1254,15 +1296,27 @@ __global__ void k_prepare(int n, int k, int kp, int trans, const double* __restr
     b64[t] = v;
 }
 
+// rows (not constrained themselves) with a constrained neighbour
+__global__ void k_cnbr(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                       const unsigned char* __restrict__ cmask, unsigned char* __restrict__ cnb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned char f = 0;
+    if (!cmask[i])
+        for (int e = rp[i]; e < rp[i + 1]; ++e) f |= cmask[ci[e]];
+    cnb[i] = f;
+}
+
 // X from the eliminated solution; see the file comment for the constrained rows.
+template <class I>
 __global__ void k_finalize(int n, int k, int kp, int trans, const double* __restrict__ B, int ldb,
                            const double* __restrict__ x64, const int* __restrict__ rp,
                            const int* __restrict__ ci, const double* __restrict__ va,
                            const int* __restrict__ diag, const unsigned char* __restrict__ cmask,
                            double* __restrict__ X, int ldx) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)n * k) return;
-    const int i = (int)(t / k), c = (int)(t % k);
+    const I t = (I)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (I)n * k) return;
+    const int i = (int)(t / (I)k), c = (int)(t - (I)i * k);
     double v;
     if (!cmask[i]) {
         v = x64[(size_t)i * kp + c];
@@ -1745,9 +1799,22 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_CHECK_HIP(hipEventCreate(&ev0));
     SFH_CHECK_HIP(hipEventCreate(&ev1));
     SFH_CHECK_HIP(hipEventRecord(ev0, st));
-    k_prepare<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>(n, k, kp, trans, B, ldb, rp, ci, va,
-                                                          s->diagpos.as<int>(), s->cmask.as<unsigned char>(),
-                                                          w.b64.as<double>());
+    const unsigned char* cnb = nullptr;
+    if (!trans && s->nc > 0) {
+        SFH_TRY(w.cnb.ensure((size_t)n));
+        k_cnbr<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, rp, ci, s->cmask.as<unsigned char>(),
+                                                            w.cnb.as<unsigned char>());
+        SFH_LAUNCH_CHECK();
+        cnb = w.cnb.as<unsigned char>();
+    }
+    if (nk < (size_t)INT32_MAX)
+        k_prepare<unsigned><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>(
+            n, k, kp, trans, B, ldb, rp, ci, va, s->diagpos.as<int>(), s->cmask.as<unsigned char>(), cnb,
+            w.b64.as<double>());
+    else
+        k_prepare<long long><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>(
+            n, k, kp, trans, B, ldb, rp, ci, va, s->diagpos.as<int>(), s->cmask.as<unsigned char>(), cnb,
+            w.b64.as<double>());
     SFH_LAUNCH_CHECK();
     SFH_CHECK_HIP(hipMemsetAsync(w.x64.p, 0, nk * 8, st));
     if (!fp64_rec) SFH_CHECK_HIP(hipMemsetAsync(w.x32.p, 0, nk * sizeof(T), st));
@@ -1909,15 +1976,27 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         if (!fp64_rec)   // (the multigrid iterations update x64 themselves)
             k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
                                                                      S + S_SCALE * ks, x, w.x64.as<double>());
-        Kern<T, LPR>::resid(P, G, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), p0);
-        k_red_norm<<<gred, kRedB, 0, st>>>(kp, d_sb, p0, S + S_RN2 * ks);
+        if (P.grid) {
+            SFH_CHECK_HIP(hipMemsetAsync(S + S_RN2 * ks, 0, (size_t)ks * 8, st));
+            k_resid_grid64<LPR><<<dim3(P.gblk, P.ntile), GB<LPR>::v, grid_lds_bytes(LPR), st>>>(
+                P.d_gtab, kp, coef64, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(), S + S_RN2 * ks);
+        } else {
+            Kern<T, LPR>::resid(P, G, st, rp, ci, va, w.x64.as<double>(), w.b64.as<double>(), w.r64.as<double>(),
+                                p0);
+            k_red_norm<<<gred, kRedB, 0, st>>>(kp, d_sb, p0, S + S_RN2 * ks);
+        }
         SFH_LAUNCH_CHECK();
         SFH_CHECK_HIP(hipMemcpyAsync(rn2.data(), S + S_RN2 * ks, ks * 8, hipMemcpyDeviceToHost, st));
         SFH_CHECK_HIP(hipStreamSynchronize(st));
     }
-    k_finalize<<<(unsigned)(((size_t)n * k + 255) / 256), 256, 0, st>>>(
-        n, k, kp, trans, B, ldb, w.x64.as<double>(), rp, ci, va, s->diagpos.as<int>(),
-        s->cmask.as<unsigned char>(), X, ldx);
+    if ((size_t)n * k < (size_t)INT32_MAX)
+        k_finalize<unsigned><<<(unsigned)(((size_t)n * k + 255) / 256), 256, 0, st>>>(
+            n, k, kp, trans, B, ldb, w.x64.as<double>(), rp, ci, va, s->diagpos.as<int>(),
+            s->cmask.as<unsigned char>(), X, ldx);
+    else
+        k_finalize<long long><<<(unsigned)(((size_t)n * k + 255) / 256), 256, 0, st>>>(
+            n, k, kp, trans, B, ldb, w.x64.as<double>(), rp, ci, va, s->diagpos.as<int>(),
+            s->cmask.as<unsigned char>(), X, ldx);
     SFH_LAUNCH_CHECK();
     SFH_CHECK_HIP(hipEventRecord(ev1, st));
     SFH_CHECK_HIP(hipEventSynchronize(ev1));
