@@ -169,28 +169,30 @@ __global__ __launch_bounds__(256) void k_mg_coarse_inverse(const int* __restrict
     for (int e = tid; e < n * n; e += bs) out[e] = (float)A[e];
 }
 
-// The same elimination with the matrix in registers, for n <= kMGCoarse:
+// The same elimination with the matrix in registers, for n <= 32 NR:
 // 1024 threads, thread (r, c) = (tid / 32, tid % 32) holds the entries of rows
-// r + 32 a and columns c + 32 b. Each step publishes the pivot column and the
+// r + 32 a and columns c + 32 b (a, b < NR; NR = 3 up to 96 nodes, so that a
+// 9 x 9 coarsest grid does not carry the 5 x 5 blocks of 160 nodes). Each step publishes the pivot column and the
 // pivot row through LDS (two buffers, so one barrier per step) and every
 // entry is updated with the operations of k_mg_coarse_inverse, so the inverse
 // is the same. The LDS kernel's step is a chain of dependent LDS loads and
 // stores; here the loads of a step are independent.
 constexpr int kCinvR = (kMGCoarse + 31) / 32;
+template <int NR>
 __global__ __launch_bounds__(1024) void k_mg_coarse_inverse_reg(const int* __restrict__ tab,
                                                                 const double* __restrict__ a9,
                                                                 float* __restrict__ inv, int* __restrict__ bad) {
-    __shared__ double colb[2][kCinvR * 32], rowb[2][kCinvR * 32];
+    __shared__ double colb[2][NR * 32], rowb[2][NR * 32];
     const int* t = tab + 6 * blockIdx.x;
     const int W = t[0], n = W * t[1], q = t[5];
     const double* s9 = a9 + (size_t)t[2] * 9;
     float* out = inv + t[3];
     const int c = threadIdx.x & 31, r = threadIdx.x >> 5;
-    double v[kCinvR][kCinvR];
+    double v[NR][NR];
 #pragma unroll
-    for (int a = 0; a < kCinvR; ++a)
+    for (int a = 0; a < NR; ++a)
 #pragma unroll
-        for (int b = 0; b < kCinvR; ++b) {
+        for (int b = 0; b < NR; ++b) {
             const int i = r + 32 * a, j = c + 32 * b;
             double x = 0.0;
             if (i < n && j < n) {
@@ -208,18 +210,18 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_inverse_reg(const int* __res
         const int kh = k >> 5, kl = k & 31;
         if (c == kl)   // owners of column k
 #pragma unroll
-            for (int a = 0; a < kCinvR; ++a) {
+            for (int a = 0; a < NR; ++a) {
                 double x = v[a][0];
 #pragma unroll
-                for (int b = 1; b < kCinvR; ++b) x = b == kh ? v[a][b] : x;
+                for (int b = 1; b < NR; ++b) x = b == kh ? v[a][b] : x;
                 if (r + 32 * a < n) cb[r + 32 * a] = x;
             }
         if (r == kl)   // owners of row k
 #pragma unroll
-            for (int b = 0; b < kCinvR; ++b) {
+            for (int b = 0; b < NR; ++b) {
                 double x = v[0][b];
 #pragma unroll
-                for (int a = 1; a < kCinvR; ++a) x = a == kh ? v[a][b] : x;
+                for (int a = 1; a < NR; ++a) x = a == kh ? v[a][b] : x;
                 if (c + 32 * b < n) rb[c + 32 * b] = x;
             }
         __syncthreads();
@@ -229,21 +231,21 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_inverse_reg(const int* __res
             return;
         }
         const double pk = 1.0 / p;
-        double sj[kCinvR], ci[kCinvR];
+        double sj[NR], ci[NR];
 #pragma unroll
-        for (int b = 0; b < kCinvR; ++b) {
+        for (int b = 0; b < NR; ++b) {
             const int j = c + 32 * b;
             sj[b] = j < n ? rb[j] / p : 0.0;   // the scaled pivot row
         }
 #pragma unroll
-        for (int a = 0; a < kCinvR; ++a) {
+        for (int a = 0; a < NR; ++a) {
             const int i = r + 32 * a;
             ci[a] = i < n ? cb[i] : 0.0;
         }
 #pragma unroll
-        for (int a = 0; a < kCinvR; ++a)
+        for (int a = 0; a < NR; ++a)
 #pragma unroll
-            for (int b = 0; b < kCinvR; ++b) {
+            for (int b = 0; b < NR; ++b) {
                 const int i = r + 32 * a, j = c + 32 * b;
                 if (i < n && j < n) {
                     if (i == k)
@@ -255,9 +257,9 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_inverse_reg(const int* __res
     }
     if (threadIdx.x == 0) bad[q] = 0;
 #pragma unroll
-    for (int a = 0; a < kCinvR; ++a)
+    for (int a = 0; a < NR; ++a)
 #pragma unroll
-        for (int b = 0; b < kCinvR; ++b) {
+        for (int b = 0; b < NR; ++b) {
             const int i = r + 32 * a, j = c + 32 * b;
             if (i < n && j < n) out[(size_t)i * n + j] = (float)v[a][b];
         }
@@ -376,9 +378,14 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
         int nmax = 0;
         for (int r = 0; r < crow[l]; ++r) nmax = std::max(nmax, tab[cofs[l] + 6 * r] * tab[cofs[l] + 6 * r + 1]);
         const size_t lds = sizeof(double) * (size_t)nmax * nmax;
-        if (nmax <= kCinvR * 32)
-            k_mg_coarse_inverse_reg<<<crow[l], 1024, 0, st>>>(m.gtab.as<int>() + cofs[l], m.coef64[l].as<double>(),
-                                                             m.cinv.as<float>(), m.cbad.as<int>());
+        if (nmax <= 3 * 32)
+            k_mg_coarse_inverse_reg<3><<<crow[l], 1024, 0, st>>>(m.gtab.as<int>() + cofs[l],
+                                                                m.coef64[l].as<double>(), m.cinv.as<float>(),
+                                                                m.cbad.as<int>());
+        else if (nmax <= kCinvR * 32)
+            k_mg_coarse_inverse_reg<kCinvR><<<crow[l], 1024, 0, st>>>(m.gtab.as<int>() + cofs[l],
+                                                                     m.coef64[l].as<double>(), m.cinv.as<float>(),
+                                                                     m.cbad.as<int>());
         else if (lds + sizeof(double) * kMGCoarse <= s->ctx->lds_per_block)
             k_mg_coarse_inverse<true><<<crow[l], 256, lds, st>>>(m.gtab.as<int>() + cofs[l], m.coef64[l].as<double>(),
                                                                  m.cwork.as<double>(), m.cinv.as<float>(),
