@@ -46,21 +46,44 @@ __host__ __device__ inline double pweight(int x, int X, int n) {
 // only) drops constrained fine nodes from P; a coarse node without free
 // support gets a decoupled unit row. Each lane sums in a fixed (i, stencil
 // entry) order into one register, with no nine-entry array indexed at run time.
+// The nine lanes of a node first stage its support's fine stencils (lane K
+// loads the 9 coefficients of support node K, one contiguous row) and the
+// constraint flags of the 5 x 5 fine nodes around it in LDS, so the sums read
+// LDS instead of issuing dependent global loads behind their branches.
 constexpr int kGalNodesPerWave = 7;
 __global__ __launch_bounds__(256) void k_mg_galerkin9(const int* __restrict__ tab, const double* __restrict__ A,
                                                       const unsigned char* __restrict__ cmask,
                                                       double* __restrict__ Ac) {
+    constexpr int kNodes = 4 * kGalNodesPerWave;   // 256 threads
+    __shared__ double sa[kNodes][9][9];
+    __shared__ unsigned char sc[kNodes][25];
     const int* t = tab + 6 * blockIdx.y;
     const int W = t[0], H = t[1], Wc = t[2], Hc = t[3], fb = t[4], cb = t[5];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int node = lane / 9, K = lane - 9 * node;
     const int I = (blockIdx.x * (blockDim.x >> 6) + wave) * kGalNodesPerWave + node;
     const bool live = node < kGalNodesPerWave && I < Wc * Hc;
+    const int bn = wave * kGalNodesPerWave + node;   // the node's slot in this work-group
+    const double* Af = A + (size_t)fb * 9;
+    const unsigned char* cm = cmask ? cmask + fb : nullptr;
+    const int Ix = live ? I % Wc : 0, Iy = live ? I / Wc : 0;
+    if (live) {
+        const int ix = 2 * Ix + K % 3 - 1, iy = 2 * Iy + K / 3 - 1;   // support node K
+        const bool in = ix >= 0 && ix < W && iy >= 0 && iy < H;
+        const size_t i = in ? (size_t)iy * W + ix : 0;
+        double row[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) row[k] = in ? Af[i * 9 + k] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sa[bn][K][k] = row[k];
+        for (int e = K; e < 25; e += 9) {
+            const int fx = 2 * Ix - 2 + e % 5, fy = 2 * Iy - 2 + e / 5;
+            sc[bn][e] = (cm && fx >= 0 && fx < W && fy >= 0 && fy < H) ? cm[fy * W + fx] : 0;
+        }
+    }
+    __syncthreads();
     double v = 0.0;
     if (live) {
-        const double* Af = A + (size_t)fb * 9;
-        const unsigned char* cm = cmask ? cmask + fb : nullptr;
-        const int Ix = I % Wc, Iy = I / Wc;
         const int Jx = Ix + K % 3 - 1, Jy = Iy + K / 3 - 1;
         if (Jx >= 0 && Jx < Wc && Jy >= 0 && Jy < Hc) {
             for (int dy = -1; dy <= 1; ++dy) {
@@ -72,8 +95,8 @@ __global__ __launch_bounds__(256) void k_mg_galerkin9(const int* __restrict__ ta
                     const int ix = 2 * Ix + dx;
                     if (ix < 0 || ix >= W) continue;
                     const double wi = wy * pweight(ix, Ix, W);
-                    const int i = iy * W + ix;
-                    if (wi == 0.0 || (cm && cm[i])) continue;
+                    if (wi == 0.0 || sc[bn][(dy + 2) * 5 + (dx + 2)]) continue;
+                    const double* ai = sa[bn][(dy + 1) * 3 + (dx + 1)];
                     // neighbours j of i inside J's support (|j - 2J| <= 1 per axis),
                     // in ascending stencil-entry order
                     const int jy0 = max(max(iy - 1, 2 * Jy - 1), 0), jy1 = min(min(iy + 1, 2 * Jy + 1), H - 1);
@@ -84,8 +107,8 @@ __global__ __launch_bounds__(256) void k_mg_galerkin9(const int* __restrict__ ta
                         for (int jx = jx0; jx <= jx1; ++jx) {
                             const double wxj = pweight(jx, Jx, W);
                             if (wxj == 0.0) continue;
-                            const double a = Af[(size_t)i * 9 + (jy - iy + 1) * 3 + (jx - ix + 1)];
-                            if (a == 0.0 || (cm && cm[jy * W + jx])) continue;
+                            const double a = ai[(jy - iy + 1) * 3 + (jx - ix + 1)];
+                            if (a == 0.0 || sc[bn][(jy - 2 * Iy + 2) * 5 + (jx - 2 * Ix + 2)]) continue;
                             v += wi * a * wyj * wxj;
                         }
                     }
