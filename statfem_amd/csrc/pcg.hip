@@ -1127,7 +1127,9 @@ __global__ __launch_bounds__(kB) void k_resid64(int kp, Seg G, const int* __rest
 // FP64 residual r = b - A x on the declared grids (tile table of the iteration
 // kernels) from the FP64 stencil, each row's products in the order of its CSR
 // entries (ascending columns), so r is the CSR residual's; |r|^2 is added into
-// rn2 per segment and column (cleared before the launch).
+// rn2 per segment and column (cleared before the launch). A lane group walks
+// a strip of SY nodes down one tile column and keeps the 3 x 3 window of x in
+// registers: one new row of three values per node instead of nine.
 template <int LPR>
 __global__ __launch_bounds__(GB<LPR>::v) void k_resid_grid64(const int* __restrict__ gtab, int kp,
                                                             const double* __restrict__ coef,
@@ -1135,27 +1137,54 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_resid_grid64(const int* __restri
                                                             const double* __restrict__ b, double* __restrict__ r,
                                                             double* __restrict__ rn2) {
     extern __shared__ float4 sm[];
+    constexpr int NG = GB<LPR>::v / LPR;
+    constexpr int SY = NG <= 32 ? kTY : (NG <= 64 ? kTY / 2 : 1);
+    constexpr int NS = kTX * (kTY / SY);
     int d[kGT];
     const Tile t = grid_tile_t<LPR>(kp, gtab, d);
     const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
     if (W == 0) return;
     V4<double> acc = {0.0, 0.0, 0.0, 0.0};
     if (t.col0 < kp) {
-        for (int o = t.grp; o < kTX * kTY; o += t.ngrp) {
-            const int gx = tx0 + o % kTX, gy = ty0 + o / kTX;
-            if (gx >= W || gy >= H) continue;
-            const size_t nd = (size_t)base + (size_t)gy * W + gx;
-            V4<double> sv = ld4(b + nd * kp + t.col0);
+        for (int sidx = t.grp; sidx < NS; sidx += t.ngrp) {
+            const int gx = tx0 + sidx % kTX, gy0 = ty0 + (sidx / kTX) * SY;
+            if (gx >= W || gy0 >= H) continue;
+            V4<double> wv[3][3];   // [row y - 1, y, y + 1][column x - 1, x, x + 1]
+            auto load_row = [&](int gy, V4<double>* dst) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const int jx = gx + k % 3 - 1, jy = gy + k / 3 - 1;
-                if (jx < 0 || jx >= W || jy < 0 || jy >= H) continue;
-                const double a = coef[nd * 9 + k];
-                const V4<double> v = ld4(x + ((size_t)base + (size_t)jy * W + jx) * kp + t.col0);
-                sv.x -= a * v.x; sv.y -= a * v.y; sv.z -= a * v.z; sv.w -= a * v.w;
+                for (int dx = 0; dx < 3; ++dx) {
+                    const int jx = gx + dx - 1;
+                    dst[dx] = (jx >= 0 && jx < W && gy >= 0 && gy < H)
+                                  ? ld4(x + ((size_t)base + (size_t)gy * W + jx) * kp + t.col0)
+                                  : V4<double>{0.0, 0.0, 0.0, 0.0};
+                }
+            };
+            load_row(gy0 - 1, wv[0]);
+            load_row(gy0, wv[1]);
+            for (int u = 0; u < SY; ++u) {
+                const int gy = gy0 + u;
+                if (gy >= H) break;
+                load_row(gy + 1, wv[2]);
+                const size_t nd = (size_t)base + (size_t)gy * W + gx;
+                V4<double> sv = ld4(b + nd * kp + t.col0);
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 3; ++dx) {
+                        const int jx = gx + dx - 1, jy = gy + dy - 1;
+                        if (jx < 0 || jx >= W || jy < 0 || jy >= H) continue;
+                        const double a = coef[nd * 9 + dy * 3 + dx];
+                        const V4<double> v = wv[dy][dx];
+                        sv.x -= a * v.x; sv.y -= a * v.y; sv.z -= a * v.z; sv.w -= a * v.w;
+                    }
+                st4(r + nd * kp + t.col0, sv);
+                acc.x += sv.x * sv.x; acc.y += sv.y * sv.y; acc.z += sv.z * sv.z; acc.w += sv.w * sv.w;
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) {
+                    wv[0][dx] = wv[1][dx];
+                    wv[1][dx] = wv[2][dx];
+                }
             }
-            st4(r + nd * kp + t.col0, sv);
-            acc.x += sv.x * sv.x; acc.y += sv.y * sv.y; acc.z += sv.z * sv.z; acc.w += sv.w * sv.w;
         }
     }
     __syncthreads();
