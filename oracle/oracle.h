@@ -34,6 +34,7 @@ void* or_chol_factor(int n, const int* rowptr, const int* colind, const double* 
                      const unsigned char* constrained, int* status);
 void or_chol_free(void* h);
 int or_chol_solve(void* h, int k, const double* B, double* X);
+int or_chol_solve_mt(void* h, int k, const double* B, double* X, int nthreads);
 int or_spmv(int n, const int* rowptr, const int* colind, const double* vals, int k,
             const double* X, double* Y);
 

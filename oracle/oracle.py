@@ -49,6 +49,7 @@ def lib():
         "or_chol_factor": (v, [i, _ip, _ip, _dp, v, C.POINTER(C.c_int)]),
         "or_chol_free": (None, [v]),
         "or_chol_solve": (i, [v, i, _dp, _dp]),
+        "or_chol_solve_mt": (i, [v, i, _dp, _dp, i]),
         "or_spmv": (i, [i, _ip, _ip, _dp, i, _dp, _dp]),
         "or_philox4x32_10": (None, [_up, _up, _up]),
         "or_mcmc_normals": (i, [C.c_uint64, i, i, _dp, _dp]),
@@ -207,12 +208,16 @@ class Chol:
             lib().or_chol_free(self.h)
             self.h = None
 
-    def solve(self, B):
+    def solve(self, B, threads=1):
+        """L L^T X = B; threads > 1 splits the columns over std::threads."""
         B = _c(B)
         squeeze = B.ndim == 1
         B2 = B.reshape(self.n, -1)
         X = np.zeros_like(B2)
-        lib().or_chol_solve(self.h, B2.shape[1], _c(B2), X)
+        if threads > 1:
+            lib().or_chol_solve_mt(self.h, B2.shape[1], _c(B2), X, int(threads))
+        else:
+            lib().or_chol_solve(self.h, B2.shape[1], _c(B2), X)
         return X.ravel() if squeeze else X
 
 

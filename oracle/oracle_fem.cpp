@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <cstring>
 #include <utility>
+#include <thread>
 #include <vector>
 
 #include "oracle.h"
@@ -603,13 +604,12 @@ void* or_chol_factor(int n, const int* rowptr, const int* colind, const double* 
 
 void or_chol_free(void* h) { delete static_cast<BandChol*>(h); }
 
-// Solves L L^T X = B for k right-hand sides, row-major n x k.
-int or_chol_solve(void* h, int k, const double* B, double* X) {
-    BandChol* f = static_cast<BandChol*>(h);
+// Solves L L^T X = B for columns c0 .. c1-1 of k right-hand sides, row-major n x k.
+static void chol_solve_cols(const BandChol* f, int k, int c0, int c1, const double* B, double* X) {
     const int n = f->n, b = f->b, ld = f->ld;
     const double* L = f->l.data();
     std::vector<double> x((size_t)n);
-    for (int c = 0; c < k; ++c) {
+    for (int c = c0; c < c1; ++c) {
         for (int i = 0; i < n; ++i) x[i] = B[(size_t)i * k + c];
         for (int i = 0; i < n; ++i) {
             const double* ri = L + (size_t)i * ld + b - i;
@@ -625,6 +625,24 @@ int or_chol_solve(void* h, int k, const double* B, double* X) {
         }
         for (int i = 0; i < n; ++i) X[(size_t)i * k + c] = x[i];
     }
+}
+
+int or_chol_solve(void* h, int k, const double* B, double* X) {
+    chol_solve_cols(static_cast<BandChol*>(h), k, 0, k, B, X);
+    return OR_OK;
+}
+
+// The same solves with the columns split over nthreads std::threads (the
+// right-hand sides are independent; each thread keeps its own work vector).
+int or_chol_solve_mt(void* h, int k, const double* B, double* X, int nthreads) {
+    const BandChol* f = static_cast<BandChol*>(h);
+    nthreads = std::max(1, std::min(nthreads, k));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t) {
+        const int c0 = (int)((long long)k * t / nthreads), c1 = (int)((long long)k * (t + 1) / nthreads);
+        pool.emplace_back(chol_solve_cols, f, k, c0, c1, B, X);
+    }
+    for (auto& th : pool) th.join();
     return OR_OK;
 }
 

@@ -662,3 +662,18 @@ def test_fpert_forcing_terms_by_factorised_forcing(orc):
         for d in range(m.nel):
             ref += Ck[e, d] * B[e] @ B[d].T
     assert np.abs(cov - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_threaded_banded_solve_matches_serial(orc):
+    """The CPU baseline's std::thread variant of the banded Cholesky solve
+    (columns split over threads) returns the serial result exactly."""
+    m = M.grid2d(30)
+    bc = M.dirichlet_all_boundary(m)
+    F = orc.body_force(m, nq1d=2, fconst=1.0)
+    rp, ci, va, F = orc.assemble(m, orc.POISSON, 1.0 + 0.1 * np.arange(m.nel) % 3, nq1d=2, bc_dofs=bc, F=F)
+    mask = np.zeros(m.nnodes, np.uint8)
+    mask[bc] = 1
+    ch = orc.Chol(rp, ci, va, mask)
+    B = np.random.default_rng(3).standard_normal((m.nnodes, 11))
+    B[bc] = 0.0
+    assert np.array_equal(ch.solve(B), ch.solve(B, threads=4))
