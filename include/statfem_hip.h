@@ -327,9 +327,22 @@ int sfh_matern_tau(double sigma, double kappa, double nu, int dim, double* tau);
  * Y = tau^2 kappa^{4 beta} (A M^-1)^{2 alpha - 1} A X, alpha = max(1, floor(beta)),
  * A the system's assembled Matern matrix (M + kappa^-2 G: SFH_MATERN with
  * factor 1/kappa^2, DriverMaxLikelihood.hpp:202-230), mass_dev the lumped mass.
- * Fractional beta returns SFH_EARG (the reference needs the baryrat package). */
+ * Fractional beta returns SFH_EARG here; sfh_precision_apply_rational applies
+ * the fractional precision from its rational factors. */
 int sfh_precision_apply(sfh_system* sys, const double* mass_dev, double sigma, double kappa,
                         double nu, int k, const double* X_dev, double* Y_dev);
+/* giveLeftFracMatrix / giveRightFracMatrix (statX/inferQ/PrecisionMatrixRational.ipp:41-161)
+ * applied to X: Y = lead prod_j (I - r_j M^-1 A) X, or with transpose != 0 its
+ * transpose lead prod_j (I - r_j A M^-1) X; roots = (lead, r_0 .. r_{degree-1})
+ * (host; top.out for P_r, bot.out for P_l), A the assembled Matern matrix. */
+int sfh_frac_apply(sfh_system* sys, const double* mass_dev, int degree, const double* roots, int transpose, int k,
+                   const double* X_dev, double* Y_dev);
+/* givePrecisionMatrix for a fractional beta (PrecisionMatrixRational.ipp:24-39)
+ * applied to X: Q_v X = P_l^T Q P_l X, Q the integer-part precision of
+ * sfh_precision_apply with alpha = max(1, floor(beta)) and the scale
+ * tau^2 kappa^{4 beta}, P_l from bot = (b, d_0 .. d_{degree-1}) (host). */
+int sfh_precision_apply_rational(sfh_system* sys, const double* mass_dev, double sigma, double kappa, double nu,
+                                 int degree, const double* bot, int k, const double* X_dev, double* Y_dev);
 
 /* ---------------------------------------------------- learned surrogate
  * No reference counterpart. An MLP 3 -> 64 -> 64 -> 1 (tanh) of the

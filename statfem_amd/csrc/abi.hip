@@ -875,6 +875,35 @@ int sfh_precision_apply(sfh_system* s, const double* mass, double sigma, double 
     return precision_apply(s, mass, tau * tau * std::pow(kappa, 4. * beta), (int)alpha, k, X, Y);
 }
 
+int sfh_frac_apply(sfh_system* s, const double* mass, int degree, const double* roots, int transpose, int k,
+                   const double* X, double* Y) {
+    if (!s || !mass || degree < 0 || degree > 32 || !roots || k <= 0 || !X || !Y || s->dofpn != 1) {
+        set_error("bad argument");
+        return SFH_EARG;
+    }
+    return frac_apply(s, mass, degree, roots, transpose ? 1 : 0, k, X, Y);
+}
+
+int sfh_precision_apply_rational(sfh_system* s, const double* mass, double sigma, double kappa, double nu,
+                                 int degree, const double* bot, int k, const double* X, double* Y) {
+    if (!s || !mass || degree < 0 || degree > 32 || !bot || k <= 0 || !X || !Y || s->dofpn != 1) {
+        set_error("bad argument");
+        return SFH_EARG;
+    }
+    double tau = 0.0;
+    SFH_TRY(sfh_matern_tau(sigma, kappa, nu, s->dim, &tau));
+    const double beta = (nu + s->dim / 2.) / 2.;
+    const double alpha = std::max(1., std::floor(beta));
+    const size_t total = (size_t)s->ndof * k;
+    DevBuf a, b;
+    SFH_TRY(a.ensure(sizeof(double) * total));
+    SFH_TRY(b.ensure(sizeof(double) * total));
+    SFH_TRY(frac_apply(s, mass, degree, bot, 0, k, X, a.as<double>()));              // P_l X
+    SFH_TRY(precision_apply(s, mass, tau * tau * std::pow(kappa, 4. * beta), (int)alpha, k, a.as<double>(),
+                            b.as<double>()));                                         // Q P_l X
+    return frac_apply(s, mass, degree, bot, 1, k, b.as<double>(), Y);                 // P_l^T Q P_l X
+}
+
 int sfh_gp(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell, double eps, int np,
            const double* Xp, double* logl, double* mean, double* var) {
     if (!c || n <= 0 || dim <= 0 || !X || !y || !logl || (np > 0 && (!Xp || !mean || !var))) {

@@ -266,6 +266,8 @@ int mcmc_run(sfh_lik* L, int C, int steps, const double* theta0, const double* p
 int typical_readings(sfh_lik* L, double rho, double sd, double ld, int nr, uint64_t seed, double* Y);
 int spmv(sfh_system* s, int k, const double* X, double* Y);
 int precision_apply(sfh_system* s, const double* mass, double tau2k4b, int alpha, int k, const double* X, double* Y);
+int frac_apply(sfh_system* s, const double* mass, int degree, const double* roots, int transpose, int k,
+               const double* X, double* Y);
 int gp_run(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell,
            double eps, int np, const double* Xp, double* logl, double* mean, double* var);
 }  // namespace sfh

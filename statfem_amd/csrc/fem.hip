@@ -737,6 +737,48 @@ int spmv(sfh_system* s, int k, const double* X, double* Y) {
     return SFH_OK;
 }
 
+// V -= c M^-1 T (m: the lumped mass) or V -= c T (m == nullptr), N x k
+__global__ void k_frac_step(long long total, int k, double c, const double* __restrict__ m,
+                            const double* __restrict__ T, double* __restrict__ V) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    V[t] -= m ? c * T[t] / m[t / k] : c * T[t];
+}
+
+// Y = lead prod_j (I - r_j M^-1 A) X (transpose = 0: P_l or P_r of
+// statX/inferQ/PrecisionMatrixRational.ipp:41-161, the factors applied right to
+// left as that matrix product applies them to X) or the transpose
+// lead prod_j (I - r_j A M^-1) X (transpose = 1, the factors in the reverse
+// order); roots = (lead, r_0 .. r_{degree-1}) on the host.
+int frac_apply(sfh_system* s, const double* mass, int degree, const double* roots, int transpose, int k,
+               const double* X, double* Y) {
+    hipStream_t st = s->ctx->stream;
+    const long long total = (long long)s->ndof * k;
+    DevBuf v, t, u;
+    SFH_TRY(v.ensure(sizeof(double) * total));
+    SFH_TRY(t.ensure(sizeof(double) * total));
+    if (transpose) SFH_TRY(u.ensure(sizeof(double) * total));
+    double* V = v.as<double>();
+    SFH_CHECK_HIP(hipMemcpyAsync(V, X, sizeof(double) * total, hipMemcpyDeviceToDevice, st));
+    for (int q = 0; q < degree; ++q) {
+        const double r = roots[1 + (transpose ? q : degree - 1 - q)];
+        if (!transpose) {
+            SFH_TRY(spmv(s, k, V, t.as<double>()));
+            k_frac_step<<<grid_for(total, 256), 256, 0, st>>>(total, k, r, mass, t.as<double>(), V);
+        } else {
+            k_rowdiv<<<grid_for(total, 256), 256, 0, st>>>(total, k, 1.0, mass, V, t.as<double>());
+            SFH_LAUNCH_CHECK();
+            SFH_TRY(spmv(s, k, t.as<double>(), u.as<double>()));
+            k_frac_step<<<grid_for(total, 256), 256, 0, st>>>(total, k, r, nullptr, u.as<double>(), V);
+        }
+        SFH_LAUNCH_CHECK();
+    }
+    k_rowdiv<<<grid_for(total, 256), 256, 0, st>>>(total, 1, roots[0], nullptr, V, Y);
+    SFH_LAUNCH_CHECK();
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    return SFH_OK;
+}
+
 // Q X = tau^2 kappa^{4 beta} (A M^-1)^{2 alpha - 1} A X  (integer beta,
 // statX/inferQ/PrecisionMatrixRational.ipp:177-209,278-298)
 int precision_apply(sfh_system* s, const double* mass, double tau2k4b, int alpha, int k, const double* X, double* Y) {

@@ -108,6 +108,8 @@ _SIGS = {
     "sfh_spmv": (i32, [v, i32, v, v]),
     "sfh_matern_tau": (i32, [dbl, dbl, dbl, i32, C.POINTER(dbl)]),
     "sfh_precision_apply": (i32, [v, v, dbl, dbl, dbl, i32, v, v]),
+    "sfh_frac_apply": (i32, [v, v, i32, v, i32, i32, v, v]),
+    "sfh_precision_apply_rational": (i32, [v, v, dbl, dbl, dbl, i32, v, i32, v, v]),
     "sfh_mlp_create": (i32, [v, C.c_uint64, C.POINTER(v)]),
     "sfh_mlp_destroy": (i32, [v]),
     "sfh_mlp_set_normalization": (i32, [v, v]),
@@ -373,13 +375,31 @@ class SystemSolveHip:
         _check(lib().sfh_spmv(self.h, k, Xd.ptr, Y.ptr))
         return Y
 
-    def precision_apply(self, mass, sigma, kappa, nu, X):
-        """PrecisionMatrixRational::givePrecisionMatrix (integer beta) applied to X."""
+    def precision_apply(self, mass, sigma, kappa, nu, X, bot=None):
+        """PrecisionMatrixRational::givePrecisionMatrix applied to X: integer
+        beta, or a fractional one with its rational denominator factors bot
+        (b, d_0 ..): Q_v X = P_l^T Q P_l X."""
         Xd = X if isinstance(X, DeviceArray) else self.ctx.to_device(np.asarray(X).reshape(self.ndof, -1))
         k = int(np.prod(Xd.shape) // self.ndof)
         md = mass if isinstance(mass, DeviceArray) else self.ctx.to_device(mass)
         Y = self.ctx.empty((self.ndof, k))
-        _check(lib().sfh_precision_apply(self.h, md.ptr, sigma, kappa, nu, k, Xd.ptr, Y.ptr))
+        if bot is None:
+            _check(lib().sfh_precision_apply(self.h, md.ptr, sigma, kappa, nu, k, Xd.ptr, Y.ptr))
+        else:
+            b = _h(np.asarray(bot, np.float64))
+            _check(lib().sfh_precision_apply_rational(self.h, md.ptr, sigma, kappa, nu, len(b) - 1, _p(b), k,
+                                                      Xd.ptr, Y.ptr))
+        return Y
+
+    def frac_apply(self, mass, roots, X, transpose=False):
+        """P_l or P_r (giveLeftFracMatrix / giveRightFracMatrix) applied to X:
+        roots[0] prod_j (I - roots[j] M^-1 A) X, or the transpose."""
+        Xd = X if isinstance(X, DeviceArray) else self.ctx.to_device(np.asarray(X).reshape(self.ndof, -1))
+        k = int(np.prod(Xd.shape) // self.ndof)
+        md = mass if isinstance(mass, DeviceArray) else self.ctx.to_device(mass)
+        r = _h(np.asarray(roots, np.float64))
+        Y = self.ctx.empty((self.ndof, k))
+        _check(lib().sfh_frac_apply(self.h, md.ptr, len(r) - 1, _p(r), int(bool(transpose)), k, Xd.ptr, Y.ptr))
         return Y
 
     def covforcing_apply(self, covvec, sigma, ell, W, mode=0):

@@ -215,7 +215,33 @@ def test_precision_operator_matches_oracle(ctx, orc, dim):
     Ys = s.spmv(X).download()
     assert np.abs(Ys - orc.spmv(rp, ci, va, X)).max() <= 1e-13 * np.abs(Ys).max()
     with pytest.raises(hip.SfhError):
-        s.precision_apply(mass, 0.8, kap, nu + 0.3, X)   # fractional beta
+        s.precision_apply(mass, 0.8, kap, nu + 0.3, X)   # fractional beta without its factors
+
+
+@pytest.mark.parametrize("dim,nu", [(1, 1.0), (2, 0.5), (2, 1.5)])
+def test_fractional_precision_operator_matches_oracle(ctx, orc, dim, nu):
+    """P_l, P_r, P_l^T and Q_v = P_l^T Q P_l applied to a block on the GPU
+    against the oracle's dense matrices, with the degree-4 rational factors."""
+    from statfem_amd import hip, rational
+    m = {1: M.line1d(60), 2: M.grid2d(14, 11)}[dim]
+    kap = np.sqrt(2 * nu) / 0.2
+    beta = (nu + dim / 2) / 2
+    top, bot, _ = rational.brasil_factors(beta - max(1.0, np.floor(beta)), 4)
+    s = hip.SystemSolveHip(ctx, m)
+    s.assemble(hip.MATERN, kap, 1.0, nq1d=2, factor=1.0 / kap ** 2)
+    mass = s.lumped_mass(2)
+    X = np.random.default_rng(12).normal(size=(m.nnodes, 5))
+    rp, ci, va, _ = orc.assemble(m, orc.MATERN, kap, 1.0, nq1d=2, factor=1.0 / kap ** 2)
+    mh = orc.body_force(m, nq1d=2, fconst=1.0)
+    for roots in (top, bot):
+        F = orc.gpq_frac_factor(rp, ci, va, mh, roots)
+        for tr, ref in ((False, F @ X), (True, F.T @ X)):
+            got = s.frac_apply(mass, roots, X, transpose=tr).download()
+            assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (tr, roots[0])
+    Qv, _ = orc.gpq_precision_frac_dense(rp, ci, va, mh, 0.8, kap, nu, dim, top, bot)
+    ref = Qv @ X
+    got = s.precision_apply(mass, 0.8, kap, nu, X, bot=bot).download()
+    assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
 
 
 @pytest.mark.parametrize("ns,no", [(20, 3), (100, 4), (150, 9), (215, 9), (240, 6)])
