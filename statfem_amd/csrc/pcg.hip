@@ -1325,7 +1325,8 @@ __global__ void k_prepare(int n, int k, int kp, int trans, const double* __restr
     b64[t] = v;
 }
 
-// rows (not constrained themselves) with a constrained neighbour
+// rows (not constrained themselves) with a constrained neighbour, gathered
+// row by row (any pattern)
 __global__ void k_cnbr(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                        const unsigned char* __restrict__ cmask, unsigned char* __restrict__ cnb) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1334,6 +1335,19 @@ __global__ void k_cnbr(int n, const int* __restrict__ rp, const int* __restrict_
     if (!cmask[i])
         for (int e = rp[i]; e < rp[i + 1]; ++e) f |= cmask[ci[e]];
     cnb[i] = f;
+}
+
+// the same flags scattered from the constrained rows (cnb cleared first): for a
+// structurally symmetric pattern, i couples to a constrained j exactly when j's
+// row holds i, and only the few constrained rows walk their entries
+__global__ void k_cnbr_scatter(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                               const unsigned char* __restrict__ cmask, unsigned char* __restrict__ cnb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !cmask[i]) return;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+        const int j = ci[e];
+        if (!cmask[j]) cnb[j] = 1;
+    }
 }
 
 // X from the eliminated solution; see the file comment for the constrained rows.
@@ -1831,8 +1845,14 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     const unsigned char* cnb = nullptr;
     if (!trans && s->nc > 0) {
         SFH_TRY(w.cnb.ensure((size_t)n));
-        k_cnbr<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, rp, ci, s->cmask.as<unsigned char>(),
-                                                            w.cnb.as<unsigned char>());
+        if (s->ncoo == 0) {   // whole-mesh pattern: element couplings, structurally symmetric
+            SFH_CHECK_HIP(hipMemsetAsync(w.cnb.p, 0, (size_t)n, st));
+            k_cnbr_scatter<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, rp, ci, s->cmask.as<unsigned char>(),
+                                                                        w.cnb.as<unsigned char>());
+        } else {
+            k_cnbr<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, rp, ci, s->cmask.as<unsigned char>(),
+                                                                w.cnb.as<unsigned char>());
+        }
         SFH_LAUNCH_CHECK();
         cnb = w.cnb.as<unsigned char>();
     }
