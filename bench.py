@@ -4,16 +4,17 @@ Workload (one step): Poisson on the unit square, 512 x 512 bilinear Q1 cells
 (263,169 DOFs), element diffusivity exp(GP(ell=0.1, sigma=0.5)), homogeneous
 Dirichlet boundary, 256 sensors. A step is
   GPU assembly (matrix FP64 + FP32 shadow, body force) + row-zero constraints
-  + R^T = A^-T P^T           (256-RHS Jacobi-PCG in column tiles of 64, FP32
-                               recurrences with FP64 residual replacement)
+  + R^T = A^-T P^T           (256-RHS PCG on the declared grid: multigrid
+                               V-cycle preconditioner, FP64 recurrences)
   + mean = R F, V = C_f R^T  (separable SE forcing covariance, FP64 MFMA)
   + C_u = R C_f R^T          (Gram, FP64 MFMA)
   + C_u P^T = A^-1 V (+reg)  (second 256-RHS solve) and ubar = A^-1 F.
 All inputs are resident in HBM before the timed region. value =
 263,169 * 256 / (seconds per step).
 
-Prints one JSON line (rank 0). --gpus N > 1 is not supported: exits with a
-message.
+Prints one JSON line (rank 0), with the other legs (3D elasticity, batched
+problems, surrogate MCMC, conductivity MCMC, SPDE GP, perturbation) as keys of
+the same line. --gpus N > 1 is not supported: exits with a message.
 """
 import argparse
 import json
@@ -612,8 +613,8 @@ def main():
                     help="also run a 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see the sharding notes in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only (there is no multi-GPU path); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     res, dom = gpu_run(args)
     if args.copy_cal:
