@@ -184,3 +184,44 @@ def test_solve_residual_at_scale(ctx, grid):
     R = P - A.T @ Xh
     rel = np.linalg.norm(R, axis=0) / np.linalg.norm(P, axis=0)
     assert st["converged"] == 1 and rel.max() < 1e-9, (rel.max(), st)
+
+
+def test_c2_full_size_residuals_and_linearity(ctx, orc):
+    """The headline workload at its full size (512 x 512 Q1, 263,169 DOFs, 256
+    sensors) on the declared grid: the FP64 residuals of the transposed and the
+    forward 256-column solves against the oracle's own assembly of the operator
+    (every column to 1e-9), and linearity of the solve (B1 + 2 B2 against the
+    two solutions combined)."""
+    import scipy.sparse as sp
+    from statfem_amd import hip
+    from statfem_amd import configs
+    cfg = configs.c2_inputs(seed=0)
+    m = cfg["mesh"]
+    ns = len(cfg["cols"])
+    s = hip.SystemSolveHip(ctx, m)
+    s.set_grid()
+    s.assemble(hip.POISSON, cfg["kappa"], nq1d=2)
+    s.applyConstraints(cfg["bc"])
+    P = np.zeros((m.nnodes, ns))
+    for j in range(ns):
+        P[cfg["cols"][j], j] += cfg["vals"][j]
+    o = hip.default_opts(tol=1e-10)
+    Rt, st1 = s.solve(P, trans=True, opts=o)
+    Rt = Rt.download()
+    assert st1["converged"] == 1 and st1["precond"] == 2
+    rp, ci, va, _ = orc.assemble(m, orc.POISSON, cfg["kappa"], nq1d=2, bc_dofs=cfg["bc"],
+                                 F=np.zeros(m.nnodes))
+    A = sp.csr_matrix((va, ci, rp), shape=(m.nnodes, m.nnodes))
+    rel = np.linalg.norm(P - A.T @ Rt, axis=0) / np.linalg.norm(P, axis=0)
+    assert rel.max() < 1e-9, rel.max()
+    B = Rt.copy()
+    B[cfg["bc"]] = 0.0
+    X, st2 = s.solve(B, opts=o)
+    X = X.download()
+    rel = np.linalg.norm(B - A @ X, axis=0) / np.linalg.norm(B, axis=0)
+    assert st2["converged"] == 1 and rel.max() < 1e-9, rel.max()
+    Y, _ = s.solve(P[:, :8] + 2.0 * P[:, 8:16], trans=True, opts=o)
+    Z = Rt[:, :8] + 2.0 * Rt[:, 8:16]
+    # (the two sides differ by the solves' errors, each about cond(A) times the
+    # residual tolerance at worst)
+    assert np.abs(Y.download() - Z).max() < 1e-7 * np.abs(Z).max()
