@@ -97,3 +97,24 @@ def test_elasticity_multigrid_on_anisotropic_cells(ctx, orc):
     Xr = orc.LU(rp, ci, va).solve(B)
     assert st["layout"] == 4 and st["converged"] == 1, st
     assert np.abs(X.download() - Xr).max() <= 1e-8 * np.abs(Xr).max(), st
+
+
+def test_elasticity_residual_at_scale(ctx, orc):
+    """A larger clamped block (32^3 cells, 107,811 DOFs) with 16 right-hand
+    sides, both orientations: the FP64 residual of every column against the
+    oracle's own assembly of the operator meets the tolerance."""
+    import scipy.sparse as sp
+    from statfem_amd import hip
+    m, E, bc, rp, ci, va = _elastic(orc, 32, 32, 32, seed=5)
+    n = 3 * m.nnodes
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    s = _elastic_system(ctx, m, E, bc)
+    B = np.random.default_rng(6).normal(size=(n, 16))
+    B[bc] = 0.0
+    for trans in (False, True):
+        X, st = s.solve(B, trans=trans, opts=hip.default_opts(tol=1e-10))
+        assert st["layout"] == 4 and st["precond"] == 2 and st["converged"] == 1, st
+        X = X.download()
+        R = B - (A.T @ X if trans else A @ X)
+        rel = np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
+        assert rel.max() < 1e-9, (trans, rel.max())
