@@ -247,6 +247,77 @@ def c5_run(args):
     }
 
 
+def c1_run(args):
+    """Config C1: 64 x 64 cells split into P1 triangles (4,225 nodes), f = 1 +
+    GP(0.1, 0.2), 32 interior sensors, 10 readings. One statFEM update on the
+    GPU: re-assembly and constraints, the 32-column transposed solve, the
+    separable forcing covariance and the Gram product (prior mean and C_u at
+    the sensors, tolerance 1e-12), then the log-likelihood of 64 (rho,
+    sigma_d, ell_d) candidates; 20 timed updates after one untimed one. CPU:
+    the oracle's update (sparse direct LU, the dense forcing-covariance
+    projection, the same 64 likelihoods) on one core."""
+    from statfem_amd import configs, hip, mesh as M
+    c = configs.c1_inputs(0)
+    m = c["mesh"]
+    ctx = hip.Context(0)
+    s = hip.SystemSolveHip(ctx, m)
+    rng = np.random.default_rng(3)
+    thetas = np.column_stack([rng.uniform(0.7, 1.3, 64), rng.uniform(0.01, 0.1, 64), rng.uniform(0.1, 0.5, 64)])
+    opts = hip.default_opts(tol=1e-12)
+    state = {}
+
+    def update():
+        s.assemble(hip.POISSON, 1.0, nq1d=1)
+        s.assemble_rhs(nq1d=1, fconst=1.0)
+        s.applyConstraints(c["bc"])
+        if "fg" not in state:
+            s.set_grid()
+            state["fg"] = hip.ForwardGaussRHSHip(s, c["cols"], c["vals"], reg_eps=c["reg"])
+        mean, Cu, st = state["fg"].updateSolution(c["sigma_f"], c["ell_f"], opts=opts)
+        Y = c["rho"] * mean[None, :] + c["d"] + c["e"]
+        lik = hip.LikelihoodKOHHip(ctx, c["xs"], Cu, mean, Y, c["sigma_e"])
+        ll, stt = lik.giveLogLikelihood(thetas)
+        if not np.all(stt == 0):
+            raise RuntimeError("C1 likelihood failed")
+        return st, Y
+
+    update()
+    ctx.sync()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st, Y = update()
+    ctx.sync()
+    t = (time.perf_counter() - t0) / reps
+    log(f"C1 {m.nnodes} nodes, {len(c['cols'])} sensors: {1e3 * t:.2f} ms per update with 64 likelihoods "
+        f"({st['inner_iters']} inner iterations, precond {st['precond']})")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        P = M.dense_projection(m.nnodes, c["cols"], c["vals"])
+        t0 = time.perf_counter()
+        F1 = orc.body_force(m, nq1d=1, fconst=1.0)
+        rp, ci, va, F1 = orc.assemble(m, orc.POISSON, 1.0, nq1d=1, bc_dofs=c["bc"], F=F1)
+        R = orc.LU(rp, ci, va).solve(P.T.copy(), trans=True).T.copy()
+        cv = orc.body_force(m, nq1d=1, fconst=1.0)
+        cv[c["bc"]] = 0.0
+        Cu = orc.covforcing_project(m.coords, cv, c["sigma_f"], c["ell_f"], R) \
+            + (c["sigma_f"] * c["reg"]) ** 2 * P @ P.T
+        mean = R @ F1
+        for th in thetas:
+            orc.loglik(Cu, mean, Y, orc.mismatch_cov(c["xs"], th[1], th[2], c["sigma_e"]), th[0])
+        tc = time.perf_counter() - t0
+        cpu = {"value": 1.0 / tc, "unit": "updates/s", "cores": 1, "kind": "port",
+               "sample": f"one update: oracle assembly + sparse LU + 32 transposed solves + dense C_f projection "
+                         f"+ 64 likelihoods, {tc:.2f}s"}
+    return {
+        "workload": f"statFEM update on 64x64 P1 ({m.nnodes} nodes), 32 sensors x 10 readings, "
+                    f"64 likelihood candidates",
+        "ms_per_update": 1e3 * t, "updates_per_s": 1.0 / t, "inner_iters": st["inner_iters"],
+        "precond": st["precond"], "cpu_baseline": cpu, "gpu_vs_cpu": (1.0 / t) / cpu["value"] if cpu else None,
+    }
+
+
 def c3_run(args):
     """3D linear elasticity with an SPDE Matern body force: n^3 hex8 cells
     (64^3: 823,875 DOFs), 128 top-face sensors. Prior at the sensors (one
@@ -596,6 +667,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-rhs", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c1", action="store_true", help="skip the small statFEM update leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the surrogate-MCMC leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the 3D elasticity leg")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched uneven-problems leg")
@@ -631,6 +703,7 @@ def main():
         hip._check(hip.lib().sfh_memcpy_d2d(c.h, b.ptr, a.ptr, ctypes.c_size_t(1 << 30)))
         c.sync()
         log("copy calibration: 1 GiB device-to-device")
+    c1 = None if args.no_c1 else c1_run(args)
     c5 = None if args.no_c5 else c5_run(args)
     c3 = None if args.no_c3 else c3_run(args)
     c4 = None if args.no_c4 else c4_run(args)
@@ -660,6 +733,7 @@ def main():
         "roofline": {"kernel": res["kernel"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": dom.get("traffic")},
         "cpu_baseline": cpu,
+        "statfem_update": c1,
         "mcmc": c5,
         "elasticity": c3,
         "batched": c4,
