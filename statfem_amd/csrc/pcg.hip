@@ -1739,7 +1739,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     bool mg_grid = false;
     for (size_t q = 0; q < grids.size(); q += 3) mg_grid = mg_grid || (long long)grids[q + 1] * grids[q + 2] > kMGCoarse;
     const bool use_grid = !grids.empty() && (o.layout == 2 || o.layout == 0) && std::is_same<T, float>::value &&
-                          (P.lpr >= 2 || (P.lpr == 1 && ns > 1 && o.precond != 1 && mg_grid)) && P.lpr <= 16;
+                          (P.lpr >= 2 || (P.lpr == 1 && o.precond != 1 && mg_grid)) && P.lpr <= 16;
     SFH_TRY(prepare_operator_on(s, st, use_grid ? grids : std::vector<int>()));
     if (use_grid && s->dia_state == 1) plan_grid(P, grids);
     // declared 3D grid: 27-point block stencil with its multigrid hierarchy
@@ -2144,9 +2144,9 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
     // Declared 2D grids (stencil kernels): at most 16 lanes (64 columns) per
     // column tile, wider solves run their column tiles side by side in the
     // same launches; narrow solves on a grid with a multigrid hierarchy run as
-    // 16-column tiles (8-column tiles for a batch of problems): a dozen
-    // multigrid iterations on padded columns beat thousands of Jacobi ones on
-    // the CSR kernels
+    // 4-, 8- or 16-column tiles (1, 2 or 4 lanes per node): a dozen multigrid
+    // iterations on padded columns beat thousands of Jacobi ones on the CSR
+    // kernels
     int min_kp = 4, max_lpr = 64;
     if (o.precision == 0 && (o.layout == 0 || o.layout == 2)) {
         std::vector<int> srow(nseg + 1);
@@ -2155,9 +2155,7 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
         if (!g.empty()) max_lpr = 16;
         bool big = false;   // a grid with a multigrid hierarchy (one grid or a batch)
         for (size_t q = 0; q < g.size(); q += 3) big = big || (long long)g[q + 1] * g[q + 2] > kMGCoarse;
-        // a batch of narrow problems (one column per problem, as the
-        // conductivity chains) runs 8-column tiles: half the lanes per node
-        if (big && o.precond != 1 && k < 16) min_kp = nseg > 1 ? (k <= 4 ? 4 : 8) : 16;
+        if (big && o.precond != 1 && k < 16) min_kp = k <= 4 ? 4 : (k <= 8 ? 8 : 16);
     }
     if (o.precision == 0 && (o.layout == 0 || o.layout == 4) && nseg == 1 && g3_applies(s)) {
         max_lpr = 16;
