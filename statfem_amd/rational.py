@@ -71,11 +71,26 @@ class _Bary:
         ev = ev[np.argsort(np.abs(ev))][: n - 1]
         return np.real_if_close(ev, tol=1e6)
 
+    def _polish(self, roots, vals, steps=8):
+        """Newton steps on g(x) = sum_i w_i v_i / (x - s_i), whose roots are the
+        pencil's (simple and real here)."""
+        x = np.real(np.asarray(roots, np.complex128)).copy()
+        for _ in range(steps):
+            d = x[:, None] - self.s
+            if np.any(d == 0.0):
+                break
+            g = (self.w * vals / d).sum(-1)
+            dg = -(self.w * vals / d ** 2).sum(-1)
+            ok = dg != 0.0
+            x = np.where(ok, x - np.where(ok, g / np.where(ok, dg, 1.0), 0.0), x)
+        return x
+
     def zeros(self):
-        return self._pencil_roots(self.fs)
+        return self._polish(self._pencil_roots(self.fs), self.fs)
 
     def poles(self):
-        return self._pencil_roots(np.ones_like(self.fs))
+        one = np.ones_like(self.fs)
+        return self._polish(self._pencil_roots(one), one)
 
     def numerator(self, x):
         """sum_i w_i f_i prod_{k != i} (x - s_k)"""
@@ -133,13 +148,15 @@ def brasil(f, interval, degree, tol=1e-8, maxiter=2000, step=0.5, max_factor=2.0
     Returns (r, info) with info = dict(iters, spread, errors, extrema)."""
     a, b = float(interval[0]), float(interval[1])
     n = 2 * degree + 1
-    # start from Chebyshev nodes graded towards the left end
+    # start from Chebyshev nodes graded towards the left end: geometrically
+    # between a and b when a > 0 (the singular end of x^gamma), else cubically
     k = np.arange(n)
     u = (1.0 - np.cos(np.pi * (k + 0.5) / n)) / 2.0
-    nodes = a + (b - a) * u ** 3
+    nodes = a * (b / a) ** u if a > 0.0 else a + (b - a) * u ** 3
     L = np.diff(np.r_[a, nodes, b])
     best = None
     spread = np.inf
+    stall = 0
     for it in range(maxiter):
         nodes = a + np.cumsum(L)[:-1]
         r = _interpolant(f, nodes)
@@ -148,6 +165,12 @@ def brasil(f, interval, degree, tol=1e-8, maxiter=2000, step=0.5, max_factor=2.0
         spread = (E.max() - E.min()) / E.max()
         if best is None or spread < best[1]:
             best = (r, spread, E.copy(), X.copy(), it)
+            stall = 0
+        else:
+            stall += 1
+            if stall >= 50:   # no progress: smaller steps
+                step *= 0.5
+                stall = 0
         if spread < tol:
             break
         mean = np.exp(np.log(E).mean())
@@ -158,16 +181,19 @@ def brasil(f, interval, degree, tol=1e-8, maxiter=2000, step=0.5, max_factor=2.0
     return r, {"iters": it, "spread": spread, "errors": E, "extrema": X}
 
 
-def brasil_factors(exponent, degree=4, tol=1e-8):
+def brasil_factors(exponent, degree=4, tol=1e-8, maxiter=2000):
     """(top, bot) of interfaceBrasil.rational (interfaceBrasil.py:39-58) for
     f(x) = x^exponent on [delta, 1]: leading coefficient then the degree roots
     of the numerator, and of the denominator."""
     if not (0.0 < abs(exponent) < 1.0):
         raise ValueError("the exponent must lie in (-1, 0) or (0, 1)")
     f = lambda x: np.asarray(x, np.float64) ** exponent
-    r, info = brasil(f, [_delta(exponent, degree), 1.0], degree, tol=tol)
-    c = np.sort(np.real(r.zeros()))
-    d = np.sort(np.real(r.poles()))
+    r, info = brasil(f, [_delta(exponent, degree), 1.0], degree, tol=tol, maxiter=maxiter)
+    if not info["spread"] < 1e-4:   # baryrat's own default tolerance
+        raise RuntimeError(f"best rational approximation of x^{exponent} (degree {degree}) did not converge: "
+                           f"relative spread of the error maxima {info['spread']:.2e}")
+    c = np.sort(r.zeros())
+    d = np.sort(r.poles())
     x0 = 0.5
     a = r.numerator(x0) / np.prod(x0 - c)
     bm = r.denominator(x0) / np.prod(x0 - d)

@@ -192,9 +192,11 @@ def test_fractional_beta_matches_oracle(ctx, orc, dim, declare, nu):
         _, ld = np.linalg.slogdet(K)
         exact = -0.5 * (yv @ np.linalg.solve(K, yv) + ld + ns * no * np.log(2 * np.pi))
         assert abs(l - exact) <= 1e-8 * abs(exact), (row, l, exact)
-        # the reference algorithm on Q_v (condition ~ cond(Q) cond(P_l)^2)
+        # the reference algorithm on Q_v: its two log-determinants carry the
+        # condition of Q_v, cond(Q) cond(P_l)^2 (largest in 1D), so it agrees
+        # to about 1e-6 only; the exact density above is the pin
         ref = orc.gpq_loglik(Qv, P, Y, row[3], Pr)
-        assert abs(l - ref) <= 1e-6 * abs(ref), (row, l, ref)
+        assert abs(l - ref) <= 1e-5 * abs(ref), (row, l, ref)
     hp = [sigma, kap, nu, 0.1]
     mean = gp.predictMean(hp, o)
     cov = gp.predictCovariance(hp, o)
