@@ -93,6 +93,17 @@ def gpu_run(args):
 
     kt = ctx.kernel_times()
     log(f"sampled inner-PCG launches: {kt}")
+    # parity sample for the error against the FP64 oracle (after the timed
+    # region): the last step's projected mean and R^T on the first columns,
+    # from the same solve path as the step
+    mean_last = fg.mean.copy()
+    P = np.zeros((m.nnodes, ns))
+    for j in range(ns):
+        P[cfg["cols"][j], j] += cfg["vals"][j]
+    Rt_dev, _ = s.solve(P, trans=True, opts=opts)
+    kcols = min(ns, args.err_cols)
+    rt_sample = Rt_dev.download()[:, :kcols].copy()
+    del Rt_dev
     # assembly alone (kernel + its status check), averaged
     ctx.timer_start()
     for _ in range(10):
@@ -147,6 +158,7 @@ def gpu_run(args):
         "assembly": {"ms": asm_ms, "GBs": asm_bytes / (asm_ms * 1e-3) / 1e9, "bytes": asm_bytes},
         "dense": dense,
         "N": N, "nnz": nnz,
+        "mean": mean_last, "rt_sample": rt_sample,
     }
     return res, per[dom]
 
@@ -605,12 +617,40 @@ def fpert_run(args):
     }
 
 
-def cpu_baseline(args, budget_s):
+def c2_errors(orc, cfg, rp, ci, va, ch, F, res):
+    """Relative error of the GPU's C2 outputs against the oracle's FP64 direct
+    solves: R^T = A^-T P^T on the sampled columns (2-norm per column, the
+    constrained components restored as r_c = p_c - sum_f A[f][c] r_f) and the
+    projected mean P A^-1 F on all sensors (max norm)."""
+    import scipy.sparse as sp
+    from statfem_amd import mesh as M
+    m, bc = cfg["mesh"], cfg["bc"]
+    N = m.nnodes
+    A = sp.csr_matrix((va, ci, rp), shape=(N, N))
+    P = M.dense_projection(N, cfg["cols"], cfg["vals"]).T.copy()
+    kc = res["rt_sample"].shape[1]
+    Pk = P[:, :kc].copy()
+    Pf = Pk.copy()
+    Pf[bc] = 0.0
+    nt = max(1, min(16, len(os.sched_getaffinity(0))))
+    R = ch.solve(Pf, threads=nt)
+    R[bc] = 0.0
+    R[bc] = Pk[bc] - (A.T @ R)[bc]
+    e_rt = np.linalg.norm(res["rt_sample"] - R, axis=0) / np.linalg.norm(R, axis=0)
+    mref = P.T @ ch.solve(F)
+    e_mean = float(np.abs(res["mean"] - mref).max() / np.abs(mref).max())
+    return {"Rt": float(e_rt.max()), "mean": e_mean, "Rt_columns": kc, "sensors": int(P.shape[1]),
+            "oracle": "banded Cholesky of the eliminated operator, FP64"}
+
+
+def cpu_baseline(args, budget_s, res=None):
     """The oracle (reference algorithm restated in C++, FP64, 1 thread):
     Triplet sort-merge assembly, banded Cholesky of the eliminated operator,
     then transposed + forward solve pairs for a sample of right-hand sides,
     extrapolated to all 256. The dense C_f product is not included (it would
-    be 6.9e10 kernel evaluations per RHS set in the reference form)."""
+    be 6.9e10 kernel evaluations per RHS set in the reference form). With the
+    GPU results `res` of the same workload it also returns their error against
+    these FP64 solves."""
     from oracle import oracle as orc
     from statfem_amd import configs, mesh as M
     cfg = configs.c2_inputs(seed=0, n=args.cpu_n, ns=args.ns)
@@ -643,7 +683,11 @@ def cpu_baseline(args, budget_s):
     ch.solve(X, threads=nt)
     t_solve_kt = time.perf_counter() - t0
     t_total_mt = t_asm + t_fac + t_solve_kt * args.ns / kt
-    return {
+    err = None
+    if res is not None and args.cpu_n == args.n:
+        err = c2_errors(orc, cfg, rp, ci, va, ch, F, res)
+        log(f"C2 error vs FP64 oracle: {err}")
+    return err, {
         "value": N * args.ns / t_total_mt, "unit": "DOF-RHS/s", "cores": nt, "kind": "port",
         "sample": (f"{args.cpu_n}x{args.cpu_n} Q1 ({N} DOFs): assembly {t_asm:.2f}s + banded Cholesky "
                    f"{t_fac:.2f}s (1 thread) + {kt} of {args.ns} solve pairs on {nt} threads {t_solve_kt:.2f}s, "
@@ -666,6 +710,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="right-hand sides per solve (0: all in one solve)")
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-rhs", type=int, default=8)
+    ap.add_argument("--err-cols", type=int, default=32,
+                    help="R^T columns compared with the oracle's FP64 solves (with the CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c1", action="store_true", help="skip the small statFEM update leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the surrogate-MCMC leg")
@@ -710,7 +756,7 @@ def main():
     lhs = None if args.no_lhs else lhs_run(args)
     gpq = None if args.no_gpq else gpq_run(args)
     fpert = None if args.no_fpert else fpert_run(args)
-    cpu = None if args.no_cpu else cpu_baseline(args, 30.0)
+    err, cpu = (None, None) if args.no_cpu else cpu_baseline(args, 30.0, res)
     line = {
         "metric": "assembled+solved DOF-RHS/s; MCMC ESS/s surrogate vs plain GPU at 1e-8 rel err",
         "value": res["value"],
@@ -727,7 +773,8 @@ def main():
         "config": {"workload": f"Poisson {args.n}x{args.n} Q1, {res['N']} DOFs, exp(GP) diffusivity, "
                                f"{args.ns} sensors/RHS: assembly + A^-T P^T + C_f + Gram + A^-1 C_f R^T",
                    "dofs": res["N"], "rhs": args.ns, "nnz": res["nnz"], "tol": args.tol, "rhs_tile": args.tile,
-                   "inner_iters": res["inner_iters"], "max_relres": res["max_relres"], "operator_layout": res["layout"],
+                   "inner_iters": res["inner_iters"], "max_relres": res["max_relres"],
+                   "max_rel_err_vs_fp64": err, "operator_layout": res["layout"],
                    "precond": res["precond"], "mg_levels": res["mg_levels"],
                    "kernels": res["kernels"], "assembly": res["assembly"]},
         "roofline": {"kernel": res["kernel"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,

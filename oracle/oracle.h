@@ -48,6 +48,8 @@ int or_covforcing_project(int n, int dim, const double* coords, const double* co
                           double* Cu);
 int or_covforcing_apply(int n, int dim, const double* coords, const double* covvec,
                         double sigma, double ell, int k, const double* W, double* V);
+int or_covforcing_rows(int n, int dim, const double* coords, const double* covvec, double sigma,
+                       double ell, int k, const double* W, int nr, const int* rows, double* V);
 int or_mismatch_cov(int ns, int dim, const double* xs, double sigma, double ell, double eps,
                     double* Ks);
 int or_loglik(int ns, int no, const double* Cu, const double* mean, const double* Y,

@@ -57,6 +57,7 @@ def lib():
         "or_se_kernel": (d, [i, _dp, _dp, d, d]),
         "or_covforcing_project": (i, [i, i, _dp, _dp, d, d, d, i, _dp, _dp]),
         "or_covforcing_apply": (i, [i, i, _dp, _dp, d, d, i, _dp, _dp]),
+        "or_covforcing_rows": (i, [i, i, _dp, _dp, d, d, i, _dp, i, _ip, _dp]),
         "or_mismatch_cov": (i, [i, i, _dp, d, d, d, _dp]),
         "or_loglik": (i, [i, i, _dp, _dp, _dp, _dp, d, C.POINTER(C.c_double)]),
         "or_post_proj": (i, [i, i, _dp, _dp, _dp, _dp, d, _dp, _dp]),
@@ -268,6 +269,19 @@ def covforcing_apply(coords, covvec, sigma, ell, W):
     W = _c(W).reshape(n, -1)
     V = np.zeros_like(W)
     lib().or_covforcing_apply(n, dim, coords, _c(covvec), sigma, ell, W.shape[1], W, V)
+    return V
+
+
+def covforcing_rows(coords, covvec, sigma, ell, W, rows):
+    """rows `rows` of V = C_f W by direct n-term sums (spot checks at large n)"""
+    coords = _c(coords)
+    n, dim = coords.shape
+    W = _c(W).reshape(n, -1)
+    rows = _c(rows, np.int32)
+    V = np.zeros((len(rows), W.shape[1]))
+    rc = lib().or_covforcing_rows(n, dim, coords, _c(covvec), sigma, ell, W.shape[1], W, len(rows), rows, V)
+    if rc != 0:
+        raise ValueError(f"or_covforcing_rows: {rc}")
     return V
 
 

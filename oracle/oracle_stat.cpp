@@ -282,6 +282,28 @@ int or_covforcing_apply(int n, int dim, const double* coords, const double* covv
     return OR_OK;
 }
 
+// Rows `rows` (nr indices) of V = C_f W, each an n-term sum of the same C_f
+// entries as or_covforcing_apply (CovarianceForcing.ipp:37-109): spot checks
+// of the product at sizes where the whole of C_f is out of reach.
+int or_covforcing_rows(int n, int dim, const double* coords, const double* covvec, double sigma,
+                       double ell, int k, const double* W, int nr, const int* rows, double* V) {
+    std::vector<double> acc((size_t)k);
+    for (int q = 0; q < nr; ++q) {
+        const int i = rows[q];
+        if (i < 0 || i >= n) return OR_EARG;
+        std::fill(acc.begin(), acc.end(), 0.0);
+        for (int j = 0; j < n; ++j) {
+            const double cij =
+                covvec[i] * se(dim, coords + (size_t)i * dim, coords + (size_t)j * dim, sigma, ell) * covvec[j];
+            if (cij == 0.0) continue;
+            const double* w = W + (size_t)j * k;
+            for (int c = 0; c < k; ++c) acc[c] += cij * w[c];
+        }
+        for (int c = 0; c < k; ++c) V[(size_t)q * k + c] = acc[c];
+    }
+    return OR_OK;
+}
+
 // MismatchCovariance::giveCovariance (statX/infer/MismatchCovariance.hpp:76-99)
 int or_mismatch_cov(int ns, int dim, const double* xs, double sigma, double ell, double eps,
                     double* Ks) {

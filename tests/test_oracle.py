@@ -105,6 +105,79 @@ def test_hex8_elasticity_rigid_body_modes_and_symmetry(orc):
     assert rp[3 * centre + 1] - rp[3 * centre] == 81
 
 
+def _dense(rp, ci, va, n):
+    A = np.zeros((n, n))
+    for r in range(n):
+        A[r, ci[rp[r]:rp[r + 1]]] = va[rp[r]:rp[r + 1]]
+    return A
+
+
+def _hex8_meshes():
+    """a unit cube, a 2^3 block, and a 3 x 2 x 2 block on the box [0,2]x[0,1]x[0,0.5]
+    with its interior nodes moved (general trilinear geometry)"""
+    rng = np.random.default_rng(11)
+    box = M.grid3d(3, 2, 2)
+    X = box.coords * np.array([2.0, 1.0, 0.5])
+    inner = np.all((box.coords > 1e-9) & (box.coords < 1 - 1e-9), axis=1)
+    X[inner] += rng.uniform(-0.08, 0.08, size=X[inner].shape)
+    return [(M.grid3d(1), 1.0), (M.grid3d(2), 1.0), (M.Mesh(M.HEX8, X, box.conn), 1.0)]
+
+
+def test_hex8_elasticity_strain_energy_closed_forms(orc):
+    """Linear displacement fields u = G x are reproduced exactly by the
+    isoparametric hex8 element, so 1/2 u^T K u equals the continuum strain
+    energy 1/2 |Omega| (lambda tr(eps)^2 + 2 mu eps:eps), eps = sym(G), with the
+    Lame constants of solid/material/svenant/SVenant.ipp:32-37. Uniaxial
+    strain gives 1/2 (lambda + 2 mu) e^2 |Omega|, simple shear 1/2 mu g^2 |Omega|,
+    dilatation 3/2 (3 lambda + 2 mu) e^2 |Omega|: a lambda/mu swap or a wrong
+    Lame formula changes each of them."""
+    E, nu = 2.0, 0.3
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    e, g = 1e-2, 3e-2
+    cases = [
+        (np.diag([e, 0.0, 0.0]), 0.5 * (lam + 2 * mu) * e * e),
+        (np.array([[0.0, g, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]]), 0.5 * mu * g * g),
+        (np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, g, 0.0]]), 0.5 * mu * g * g),
+        (e * np.eye(3), 1.5 * (3 * lam + 2 * mu) * e * e),
+    ]
+    for m, _ in _hex8_meshes():
+        vol = orc.body_force(m, nq1d=2, fconst=1.0).sum()
+        n = 3 * m.nnodes
+        rp, ci, va, _ = orc.assemble(m, orc.ELASTICITY, E, nu)
+        A = _dense(rp, ci, va, n)
+        for G, dens in cases:
+            u = (m.coords @ G.T).ravel()
+            energy = 0.5 * u @ A @ u
+            assert abs(energy - dens * vol) <= 1e-12 * dens * vol, (energy, dens * vol)
+
+
+def test_hex8_elasticity_patch_test(orc):
+    """Constant strain is an equilibrium state: for u = G x (any G) the nodal
+    forces K u vanish at every interior node, and the boundary forces equal the
+    tractions sigma n of the constant stress sigma = lambda tr(eps) I + 2 mu eps
+    integrated over the boundary (their sum is zero and their moment balances)."""
+    E, nu = 1.0, 0.3
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    rng = np.random.default_rng(5)
+    G = rng.normal(size=(3, 3))
+    eps = 0.5 * (G + G.T)
+    sig = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+    m = M.grid3d(3)
+    rp, ci, va, _ = orc.assemble(m, orc.ELASTICITY, E, nu)
+    A = _dense(rp, ci, va, 3 * m.nnodes)
+    f = (A @ (m.coords @ G.T).ravel()).reshape(-1, 3)
+    inner = np.all((m.coords > 1e-9) & (m.coords < 1 - 1e-9), axis=1)
+    assert np.abs(f[inner]).max() < 1e-13 * np.abs(f).max()
+    # on the face x = 1 the traction is sigma e_x; its resultant over the face is sigma e_x (area 1)
+    face = np.abs(m.coords[:, 0] - 1.0) < 1e-12
+    edge_free = face & np.all((m.coords[:, 1:] > 1e-9) & (m.coords[:, 1:] < 1 - 1e-9), axis=1)
+    # an interior face node carries the traction times its tributary area h^2 = 1/9
+    assert np.abs(f[edge_free] - sig[:, 0] / 9.0).max() < 1e-13
+    assert np.abs(f.sum(axis=0)).max() < 1e-13
+
+
 def test_matern_row_sums_are_kappa2_lumped_mass(orc):
     m = M.grid2d(5)
     kap = 3.0
@@ -260,6 +333,8 @@ def test_covforcing_reference_form_equals_dense_triple_product(orc):
     assert np.abs(Cu - R @ Cf @ R.T).max() < 1e-14
     W = rng.normal(size=(m.nnodes, 2))
     assert np.abs(orc.covforcing_apply(m.coords, cv, 0.1, 0.2, W) - Cf @ W).max() < 1e-15
+    rows = np.array([0, 7, 12, 35], np.int32)
+    assert np.abs(orc.covforcing_rows(m.coords, cv, 0.1, 0.2, W, rows) - (Cf @ W)[rows]).max() < 1e-15
 
 
 # --------------------------------------------------------------- MCMC
