@@ -24,6 +24,11 @@
 // Solves are the library's mixed-precision multi-RHS PCG to a relative FP64
 // residual (default 1e-10, see options()); reuseFactoredMatrix is accepted
 // for interface compatibility (there is no factorisation to reuse).
+//
+// A solver handed to AssemblerHip (AssemblerHip.hpp) holds a whole-mesh
+// system instead: the operator is assembled on the device for all elements at
+// once, constraints act on its rows there, and per-element insertToMatrix is
+// refused (insertToRhs still accumulates into the host right-hand side).
 #pragma once
 
 #include <algorithm>
@@ -46,6 +51,8 @@
 #endif
 
 namespace statfem_hip {
+
+class AssemblerHip;
 
 namespace detail {
 template <class V>
@@ -108,6 +115,7 @@ public:
     // ---------------------------------------------------------- insertion
     template <class MAT, class IDX>
     void insertToMatrix(const MAT& eStiff, const IDX& rowIndices, const IDX& colIndices) {
+        STATFEM_HIP_VERIFY(!meshMode_, "insertToMatrix on a system assembled by AssemblerHip");
         for (std::size_t i = 0; i < rowIndices.size(); ++i)
             for (std::size_t j = 0; j < colIndices.size(); ++j)
                 push(static_cast<int>(rowIndices[i]), static_cast<int>(colIndices[j]), eStiff(i, j));
@@ -119,12 +127,29 @@ public:
 
     // --------------------------------------------------------- constraints
     void applyConstraints(const Constraints& constraints, const double factor, const double scalar) {
+        if (meshMode_) {   // rows of the device operator, in one call
+            std::vector<int> dofs;
+            std::vector<double> vals;
+            for (const auto& c : constraints) {
+                dofs.push_back(static_cast<int>(c.first));
+                vals.push_back(c.second);
+                F_[c.first] = factor * c.second * scalar;
+            }
+            if (!dofs.empty())
+                detail::check(sfh_apply_constraints(sys_, static_cast<int>(dofs.size()), dofs.data(), vals.data(),
+                                                    factor, scalar), "sfh_apply_constraints");
+            return;
+        }
         for (const auto& c : constraints) {
             zeroRow(c.first, scalar);
             F_[c.first] = factor * c.second * scalar;
         }
     }
     void fixDOF(const unsigned index, const double scalar = 1.) {
+        if (meshMode_) {
+            applyConstraints(Constraints{{index, 0.0}}, 1.0, scalar);
+            return;
+        }
         zeroRow(index, scalar);
         F_[index] = 0.;
     }
@@ -175,6 +200,10 @@ public:
     void giveForce(const IDX& dofIndices, VEC& result) const { gather(F_, dofIndices, result); }
 
     void clearMatrix() {
+        if (meshMode_) {
+            detail::check(sfh_clear_matrix(sys_), "sfh_clear_matrix");
+            return;
+        }
         for (auto& v : vals_) v = 0.0;
         cursor_ = 0;        // a re-assembly that repeats the insertion sequence keeps the pattern
         replay_ = !rows_.empty();
@@ -202,8 +231,26 @@ public:
     const sfh_solve_stats& lastStats() const { return stats_; }
     sfh_system* handle() { prepare(); return sys_; }
     sfh_ctx* context() const { return ctx_; }
+    unsigned numDofs() const { return n_; }
+    bool wholeMesh() const { return meshMode_; }
 
 private:
+    friend class AssemblerHip;
+    // AssemblerHip hands over its whole-mesh system (the solver owns it from here)
+    void bindMesh(sfh_system* sys) {
+        if (sys_) sfh_system_destroy(sys_);
+        sys_ = sys;
+        meshMode_ = true;
+        int nd = 0;
+        detail::check(sfh_system_info(sys_, &nd, &nnz_), "sfh_system_info");
+        STATFEM_HIP_VERIFY(static_cast<unsigned>(nd) == n_, "mesh DOFs differ from the solver's numDofs");
+        rows_.clear(); cols_.clear(); vals_.clear(); epoch_.clear();
+        patternChanged_ = matrixChanged_ = false;
+    }
+    void addForce(const std::vector<double>& f) {
+        for (unsigned i = 0; i < n_; ++i) F_[i] += f[i];
+    }
+
     void push(int r, int c, double v) {
         const int epoch = static_cast<int>(rowsSeen());
         if (replay_) {   // same insertion sequence after clearMatrix: only values change
@@ -242,6 +289,10 @@ private:
     }
 
     void prepare() {
+        if (meshMode_) {
+            detail::check(sfh_system_set_rhs(sys_, F_.data()), "sfh_system_set_rhs");
+            return;
+        }
         if (replay_ && cursor_ != rows_.size()) {   // fewer insertions than before: new pattern
             rows_.resize(cursor_); cols_.resize(cursor_); vals_.resize(cursor_); epoch_.resize(cursor_);
             patternChanged_ = true;
@@ -328,7 +379,7 @@ private:
     std::vector<int> rowEpoch_;
     std::vector<unsigned> constrained_;
     std::size_t cursor_ = 0, events_ = 0;
-    bool replay_ = false, patternChanged_ = true, matrixChanged_ = true;
+    bool replay_ = false, patternChanged_ = true, matrixChanged_ = true, meshMode_ = false;
     sfh_solve_opts opts_;
     sfh_solve_stats stats_{};
 };

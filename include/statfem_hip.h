@@ -27,10 +27,13 @@ enum {
     SFH_OK = 0,
     SFH_EARG = -1,        /* bad argument / size */
     SFH_ENEGJAC = -2,     /* "Negative Jacobian." corlib/ElementBasic.ipp:199 */
-    SFH_ESINGULAR = -3,   /* singular system, corlib/SystemSolveEigenSparse.hpp:303-305 */
+    SFH_ESINGULAR = -3,   /* singular system, corlib/SystemSolveEigenSparse.hpp:303-305: a non-finite
+                             residual, or an iterative solve whose FP64 residual stopped falling (an
+                             operator without the constraints that make it invertible, or a right-hand
+                             side outside its range) */
     SFH_ENOTSPD = -4,     /* LLT failure, statX/infer/LikelihoodKOH.ipp:93 */
     SFH_EQUADFORM = -5,   /* non-negative quadratic form, statX/infer/LikelihoodKOH.ipp:113 */
-    SFH_ENOCONV = -6,     /* iterative solve hit its iteration cap */
+    SFH_ENOCONV = -6,     /* iterative solve hit its iteration cap while its residual still fell */
     SFH_EDEVICE = -7,     /* HIP runtime error */
     SFH_ENOMEM = -8,      /* device allocation failed */
     SFH_ESTATE = -9       /* call out of order (e.g. solve before assembly) */
@@ -268,6 +271,11 @@ int sfh_covforcing_batched(sfh_ctx* ctx, int nseg, const int* seg_rows, const in
  * field-by-sensor prior covariance C_u P^T = A^-1 C_f R^T + (sigma_f reg_eps)^2 P^T
  * (N x ns), and ubar_dev (N) = A^-1 F, the inputs of the Woodbury posterior.
  * Outputs mean (ns) and Cu (ns x ns) are host arrays. */
+/* The weights covVec of CovarianceForcing (statX/infer/CovarianceForcing.hpp:70-114:
+ * insertToRhs of the unit body force, zeroed by applyConstraints) that the
+ * following sfh_forward_gauss calls use in C_f, host (ndof); NULL returns to
+ * the default, the system's lumped mass zeroed on its constrained DOFs. */
+int sfh_system_set_covvec(sfh_system* sys, const double* covvec);
 int sfh_forward_gauss(sfh_system* sys, int ns, int pn, const int* pcols, const double* pvals,
                       double sigma_f, double ell_f, double reg_eps, const sfh_solve_opts* opts,
                       double* mean, double* Cu, double* field_dev, double* ubar_dev,
@@ -297,6 +305,15 @@ int sfh_lik_post_proj(sfh_lik* lik, double rho, double sigma_d, double ell_d, do
  * posterior variance at the sensors comes from sfh_lik_post_proj. */
 int sfh_lik_post_field(sfh_lik* lik, double rho, double sigma_d, double ell_d, int N,
                        const double* field_dev, const double* ubar_dev, double* mean_dev);
+/* giveFEconditionedDataWoodbury covariance (statX/infer/LikelihoodKOH.ipp:459-488):
+ * C_{u|y} = D C_u = C_u - (C_u P^T) S^-1 (P C_u), S = C_u^proj + K_s/(rho^2 n_o),
+ * from field_dev = C_u P^T (N x ns, sfh_forward_gauss) and the prior field
+ * covariance. full = 0: prior_dev is the prior variance diag(C_u) (N) and
+ * out_dev receives the posterior variance (N); full = 1: prior_dev is the
+ * prior covariance C_u (N x N, the reference's updateSolution with P = I) and
+ * out_dev the posterior covariance (N x N; may alias prior_dev). Device arrays. */
+int sfh_lik_post_field_cov(sfh_lik* lik, double rho, double sigma_d, double ell_d, int N,
+                           const double* field_dev, const double* prior_dev, int full, double* out_dev);
 
 /* MCMC<3>::sample (statX/infer/MCMC.hpp:123-158) with the target of
  * hyperParamLearning (del2/apps/reference/laplaceRHSinfer/hyperParamLearning.hpp:172-190),
@@ -376,6 +393,11 @@ int sfh_mcmc_da(sfh_lik* lik, sfh_mlp* mlp, int chains, int steps, int sub, cons
 int sfh_gp(sfh_ctx* ctx, int n, int dim, const double* X, const double* y, double sigma,
            double ell, double eps, int np, const double* Xp, double* logl, double* mean,
            double* var);
+/* GaussianProcess::predictCovariance (statX/infer/GaussianProcess.hpp:160-190):
+ * cov (np x np, host) = k(X*, X*) - k(X, X*)^T (K + eps^2 I)^-1 k(X, X*) of the
+ * process fitted as in sfh_gp. */
+int sfh_gp_predict_cov(sfh_ctx* ctx, int n, int dim, const double* X, const double* y, double sigma,
+                       double ell, double eps, int np, const double* Xp, double* cov);
 
 /* ------------------------------------------- LHS (conductivity) uncertainty */
 /* ForwardGaussLHS (statX/infer/ForwardGaussLHS.hpp:41-105) with the solve

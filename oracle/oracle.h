@@ -61,6 +61,8 @@ int or_post_woodbury(int n, int ns, int no, const double* CuFull, const double* 
                      double* mean_out, double* cov_out);
 int or_gp(int n, int dim, const double* X, const double* y, double sigma, double ell, double eps,
           int np, const double* Xp, double* logl, double* mean, double* var);
+int or_gp_cov(int n, int dim, const double* X, double sigma, double ell, double eps, int np,
+              const double* Xp, double* cov);
 int or_mcmc_koh(int ns, int no, int dim, const double* Cu, const double* mean, const double* Y,
                 const double* xs, double eps, const double* theta0, const double* propsd,
                 int nsteps, uint64_t seed, int chain, int prior_mode, const double* prior_mu,

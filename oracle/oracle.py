@@ -63,6 +63,7 @@ def lib():
         "or_post_proj": (i, [i, i, _dp, _dp, _dp, _dp, d, _dp, _dp]),
         "or_post_woodbury": (i, [i, i, i, _dp, _dp, _dp, _dp, _dp, d, _dp, v]),
         "or_gp": (i, [i, i, _dp, _dp, d, d, d, i, _dp, C.POINTER(C.c_double), _dp, _dp]),
+        "or_gp_cov": (i, [i, i, _dp, d, d, d, i, _dp, _dp]),
         "or_mcmc_koh": (i, [i, i, i, _dp, _dp, _dp, _dp, d, _dp, _dp, i, C.c_uint64, i, i,
                             _dp, _dp, _dp, C.POINTER(C.c_double)]),
         "or_mcmc_gauss": (i, [d, d, d, d, i, C.c_uint64, i, _dp, C.POINTER(C.c_double)]),
@@ -339,6 +340,18 @@ def gp(X, y, sigma, ell, eps, Xp):
     if rc != 0:
         raise RuntimeError(f"or_gp: {rc}")
     return ll.value, mean, var
+
+
+def gp_cov(X, sigma, ell, eps, Xp, n):
+    """predictCovariance at the test locations Xp (np x dim) of the GP on X (n x dim)"""
+    X = _c(X).reshape(n, -1)
+    dim = X.shape[1]
+    Xp = _c(Xp).reshape(-1, dim)
+    cov = np.zeros((Xp.shape[0], Xp.shape[0]))
+    rc = lib().or_gp_cov(n, dim, X, sigma, ell, eps, Xp.shape[0], Xp, cov)
+    if rc != 0:
+        raise RuntimeError(f"or_gp_cov: {rc}")
+    return cov
 
 
 def mcmc_koh(Cu, mean, Y, xs, eps, theta0, propsd, nsteps, seed=0, chain=0, prior_mode=0,

@@ -456,6 +456,31 @@ int or_gp(int n, int dim, const double* X, const double* y, double sigma, double
     return OR_OK;
 }
 
+// GaussianProcess::predictCovariance (statX/infer/GaussianProcess.hpp:160-190):
+// cov = k(X*, X*) - K*^T (K + eps^2 I)^-1 K*, K* = k(X, X*), np x np row-major.
+int or_gp_cov(int n, int dim, const double* X, double sigma, double ell, double eps, int np,
+              const double* Xp, double* cov) {
+    std::vector<double> K((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            K[(size_t)i * n + j] = se(dim, X + (size_t)i * dim, X + (size_t)j * dim, sigma, ell) + (i == j ? eps * eps : 0.0);
+    if (!llt(n, K)) return OR_ENOTSPD;
+    // W = K^-1 K* column by column
+    std::vector<double> ks((size_t)n * np), w(n);
+    for (int p = 0; p < np; ++p)
+        for (int i = 0; i < n; ++i) ks[(size_t)p * n + i] = se(dim, X + (size_t)i * dim, Xp + (size_t)p * dim, sigma, ell);
+    for (int q = 0; q < np; ++q) {
+        for (int i = 0; i < n; ++i) w[i] = ks[(size_t)q * n + i];
+        llt_solve(n, K, w.data());
+        for (int p = 0; p < np; ++p) {
+            double acc = 0.0;
+            for (int i = 0; i < n; ++i) acc += ks[(size_t)p * n + i] * w[i];
+            cov[(size_t)p * np + q] = se(dim, Xp + (size_t)p * dim, Xp + (size_t)q * dim, sigma, ell) - acc;
+        }
+    }
+    return OR_OK;
+}
+
 // Random-walk Metropolis on theta = log(rho, sigma_d, ell_d) (statX/infer/MCMC.hpp:123-158)
 // with the target of hyperParamLearning.hpp:172-190:
 //   prior_mode 0: logL(e^theta) + sum(theta)                (flat prior on the hyperparameters)

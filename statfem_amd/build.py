@@ -55,15 +55,19 @@ def build(verbose=False):
 
 
 CPP_CLIENT = os.path.join(OUT, "system_solve_hip")
+CPP_PIPELINE = os.path.join(OUT, "statfem_pipeline")
 
 
 def build_cpp_client():
-    """The C++ drop-in test client (tests/cpp/system_solve_hip.cpp) against the
-    header-only SystemSolveHip and the library, with the host compiler."""
-    src = os.path.join(HERE, "..", "tests", "cpp", "system_solve_hip.cpp")
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(HERE, "..", "include"), src,
-           "-L", HERE, "-lstatfem_hip", "-Wl,-rpath,$ORIGIN/..", "-o", CPP_CLIENT]
-    subprocess.run(cmd, check=True)
+    """The C++ drop-in test clients (tests/cpp/system_solve_hip.cpp: the
+    SystemSolve concept; tests/cpp/statfem_pipeline.cpp: AssemblerHip and the
+    statX classes) against the header-only classes and the library, with the
+    host compiler."""
+    for src, exe in (("system_solve_hip.cpp", CPP_CLIENT), ("statfem_pipeline.cpp", CPP_PIPELINE)):
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(HERE, "..", "include"),
+               os.path.join(HERE, "..", "tests", "cpp", src), "-L", HERE, "-lstatfem_hip",
+               "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+        subprocess.run(cmd, check=True)
     return CPP_CLIENT
 
 

@@ -104,6 +104,46 @@ __global__ void k_post_build(int ns, int no, int dim, const double* xs, const do
     }
 }
 
+// B = I (n x n)
+__global__ void k_eye(int n, double* B) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (long long)n * n) B[t] = (t / n == t % n) ? 1.0 : 0.0;
+}
+
+// out_i = prior_i - sum_s H[i][s] F[i][s] (N rows of ns): the diagonal of
+// F S^-1 F^T subtracted from the prior variance, one wave per row
+__global__ __launch_bounds__(256) void k_post_var(int N, int ns, const double* __restrict__ F,
+                                                  const double* __restrict__ H, const double* __restrict__ prior,
+                                                  double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= N) return;
+    const double* f = F + (size_t)row * ns;
+    const double* h = H + (size_t)row * ns;
+    double s = 0.0;
+    for (int c = lane; c < ns; c += 64) s += f[c] * h[c];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) out[row] = prior[row] - s;
+}
+
+// At (C x R) = A^T for A R x C, through 32 x 32 LDS tiles
+__global__ __launch_bounds__(256) void k_transpose(int R, int C, const double* __restrict__ A, double* __restrict__ At) {
+    __shared__ double t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8)
+        if (r0 + r < R && c0 + tx < C) t[r][tx] = A[(size_t)(r0 + r) * C + c0 + tx];
+    __syncthreads();
+    for (int c = ty; c < 32; c += 8)
+        if (c0 + c < C && r0 + tx < R) At[(size_t)(c0 + c) * R + r0 + tx] = t[tx][c];
+}
+
+// out = prior - T, elementwise (out may alias prior)
+__global__ void k_sub(long long n, const double* prior, const double* T, double* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = prior[i] - T[i];
+}
+
 // cov = Cu - T[:, :ns], mean = mean + T[:, ns]
 __global__ void k_post_finish(int ns, const double* Cu, const double* mean, const double* T, double* cov,
                               double* mpost) {
@@ -578,6 +618,19 @@ int sfh_covforcing_batched(sfh_ctx* c, int nseg, const int* seg_rows, const int*
     return SFH_OK;
 }
 
+int sfh_system_set_covvec(sfh_system* s, const double* covvec) {
+    if (!s) { set_error("bad argument"); return SFH_EARG; }
+    if (!covvec) {
+        s->fg_user = false;
+        return SFH_OK;
+    }
+    SFH_TRY(s->fg_user_cv.ensure((size_t)s->ndof * 8));
+    SFH_CHECK_HIP(hipMemcpyAsync(s->fg_user_cv.p, covvec, (size_t)s->ndof * 8, hipMemcpyHostToDevice, s->ctx->stream));
+    SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->fg_user = true;
+    return SFH_OK;
+}
+
 int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const double* pvals, double sigma_f,
                       double ell_f, double reg_eps, const sfh_solve_opts* opts, double* mean, double* Cu,
                       double* field, double* ubar, sfh_solve_stats* stats) {
@@ -607,10 +660,15 @@ int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const dou
     double* cv = s->fg_cv.as<double>();
     double* dCu = s->fg_cu.as<double>();
     double* dmean = dCu + (size_t)ns * ns;
-    // covVec = int phi, zero on constrained DOFs (CovarianceForcing.hpp:101-114)
-    const double one = 1.0;
-    SFH_TRY(assemble_rhs(s, s->nq1d, &one, nullptr, 1.0, 0, cv));
-    k_mask_zero<<<nb(N, 256), 256, 0, st>>>(N, s->cmask.as<unsigned char>(), cv);
+    // covVec = int phi, zero on constrained DOFs (CovarianceForcing.hpp:101-114),
+    // or the caller's weights
+    if (s->fg_user) {
+        SFH_CHECK_HIP(hipMemcpyAsync(cv, s->fg_user_cv.p, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
+    } else {
+        const double one = 1.0;
+        SFH_TRY(assemble_rhs(s, s->nq1d, &one, nullptr, 1.0, 0, cv));
+        k_mask_zero<<<nb(N, 256), 256, 0, st>>>(N, s->cmask.as<unsigned char>(), cv);
+    }
     // P^T (N x ns)
     SFH_CHECK_HIP(hipMemsetAsync(Pt, 0, nk * 8, st));
     SFH_CHECK_HIP(hipMemcpyAsync(s->fg_cols.p, pcols, sizeof(int) * ns * pn, hipMemcpyHostToDevice, st));
@@ -814,6 +872,49 @@ int sfh_lik_post_field(sfh_lik* L, double rho, double sd, double ld, int N, cons
     return SFH_OK;
 }
 
+int sfh_lik_post_field_cov(sfh_lik* L, double rho, double sd, double ld, int N, const double* field,
+                           const double* prior, int full, double* out) {
+    if (!L || N <= 0 || !field || !prior || !out) { set_error("bad argument"); return SFH_EARG; }
+    if (full && (long long)N * N > (1LL << 31)) { set_error("full field covariance too large (N = %d)", N); return SFH_EARG; }
+    const int ns = L->ns;
+    hipStream_t st = L->ctx->stream;
+    if (!(rho != 0.0)) { set_error("rho must be non-zero"); return SFH_EARG; }
+    // S = C_u + K_s / (rho^2 n_o) and its inverse from the Cholesky factor (LikelihoodKOH.ipp:466-474)
+    DevBuf S, B, Si, H, dst;
+    SFH_TRY(S.ensure(8 * (size_t)ns * ns));
+    SFH_TRY(B.ensure(8 * (size_t)ns * (ns + 1)));
+    SFH_TRY(Si.ensure(8 * (size_t)ns * ns));
+    SFH_TRY(H.ensure(8 * (size_t)N * ns));
+    SFH_TRY(dst.ensure(sizeof(int)));
+    k_post_build<<<nb((long long)ns * ns, 256), 256, 0, st>>>(ns, L->no, L->dim, L->xs.as<double>(), L->Cu.as<double>(),
+                                                             L->mean.as<double>(), L->Y.as<double>(), L->eps, rho, sd,
+                                                             ld, S.as<double>(), B.as<double>());
+    k_eye<<<nb((long long)ns * ns, 256), 256, 0, st>>>(ns, Si.as<double>());
+    SFH_LAUNCH_CHECK();
+    SFH_TRY(dense_spd_solve(L->ctx, ns, S.as<double>(), ns, Si.as<double>(), dst.as<int>()));
+    int h = 0;
+    SFH_CHECK_HIP(hipMemcpyAsync(&h, dst.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (h != SFH_OK) { set_error("C_u + K_s/(rho^2 n_o) not SPD"); return h; }
+    // H = (C_u P^T) S^-1 (N x ns); D C_u = C_u - H (P C_u) with P C_u = (C_u P^T)^T (:484-488)
+    SFH_TRY(dgemm_shared_a(L->ctx, N, ns, ns, field, ns, Si.as<double>(), ns, 0, H.as<double>(), ns, 0, 1));
+    if (!full) {
+        k_post_var<<<nb(N, 4), 256, 0, st>>>(N, ns, field, H.as<double>(), prior, out);
+        SFH_LAUNCH_CHECK();
+    } else {
+        DevBuf Ft, T;
+        SFH_TRY(Ft.ensure(8 * (size_t)N * ns));
+        SFH_TRY(T.ensure(8 * (size_t)N * N));
+        k_transpose<<<dim3((ns + 31) / 32, (N + 31) / 32), 256, 0, st>>>(N, ns, field, Ft.as<double>());
+        SFH_LAUNCH_CHECK();
+        SFH_TRY(dgemm_shared_a(L->ctx, N, N, ns, H.as<double>(), ns, Ft.as<double>(), N, 0, T.as<double>(), N, 0, 1));
+        k_sub<<<nb((long long)N * N, 256), 256, 0, st>>>((long long)N * N, prior, T.as<double>(), out);
+        SFH_LAUNCH_CHECK();
+    }
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    return SFH_OK;
+}
+
 int sfh_lik_logdet_batch(sfh_lik* L, int M, const double* theta, double* logdet, int* status) {
     if (!L || M <= 0 || !theta || !logdet) { set_error("bad argument"); return SFH_EARG; }
     hipStream_t st = L->ctx->stream;
@@ -910,7 +1011,18 @@ int sfh_gp(sfh_ctx* c, int n, int dim, const double* X, const double* y, double 
         set_error("bad argument");
         return SFH_EARG;
     }
-    return gp_run(c, n, dim, X, y, sigma, ell, eps, np, Xp, logl, mean, var);
+    return gp_run(c, n, dim, X, y, sigma, ell, eps, np, Xp, logl, mean, var, nullptr);
+}
+
+int sfh_gp_predict_cov(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell,
+                       double eps, int np, const double* Xp, double* cov) {
+    if (!c || n <= 0 || dim <= 0 || !X || !y || np <= 0 || !Xp || !cov) {
+        set_error("bad argument");
+        return SFH_EARG;
+    }
+    std::vector<double> m(np), v(np);
+    double logl = 0.0;
+    return gp_run(c, n, dim, X, y, sigma, ell, eps, np, Xp, &logl, m.data(), v.data(), cov);
 }
 
 }  // extern "C"

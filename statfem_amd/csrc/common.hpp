@@ -87,6 +87,10 @@ struct SolveWork {
 // coarsest level.
 constexpr int kMGMaxLev = 16;
 constexpr int kMGCoarse = 160;       // coarsest level: at most this many nodes
+// smallest pivot of the coarsest level's factorisation, relative to its row's
+// diagonal, that counts as positive (a singular operator leaves pivots at the
+// rounding level of the diagonal)
+constexpr double kPivotRel = 1e-12;
 struct MGHier {
     bool built = false;              // current for the stencil copy (reset when that is rebuilt)
     int L = 0;                       // coarse levels of the deepest grid (0: no hierarchy)
@@ -186,6 +190,9 @@ struct sfh_system {
     sfh::SolveWork work2;        // workspace of a concurrent second solve
     sfh::DevBuf scratch0, scratch1, scratch2, scratch3;
     sfh::DevBuf fg_pt, fg_rt, fg_v, fg_cv, fg_cu, fg_gram, fg_cols, fg_vals;
+    // forcing-covariance weights given by the caller (sfh_system_set_covvec)
+    sfh::DevBuf fg_user_cv;
+    bool fg_user = false;
 };
 
 struct sfh_lik {
@@ -269,5 +276,5 @@ int precision_apply(sfh_system* s, const double* mass, double tau2k4b, int alpha
 int frac_apply(sfh_system* s, const double* mass, int degree, const double* roots, int transpose, int k,
                const double* X, double* Y);
 int gp_run(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell,
-           double eps, int np, const double* Xp, double* logl, double* mean, double* var);
+           double eps, int np, const double* Xp, double* logl, double* mean, double* var, double* cov);
 }  // namespace sfh

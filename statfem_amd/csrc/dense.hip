@@ -679,31 +679,42 @@ __global__ __launch_bounds__(256) void k_gp_fit(int n, int dim, const double* __
     }
 }
 
-// mean k*^T alpha, variance k** - |L^-1 k*|^2, one thread per prediction point
+// mean k*^T alpha, variance k** - |L^-1 k*|^2, one thread per prediction point;
+// v = L^-1 k* is kept in work as column p of an n x np row-major block (the
+// threads of a wave touch consecutive entries of each row)
 __global__ void k_gp_predict(int n, int dim, int np, const double* __restrict__ X, const double* __restrict__ Xp,
                              double sigma, double ell, const double* __restrict__ L,
-                             const double* __restrict__ alpha, double* __restrict__ work,
-                             double* __restrict__ mean, double* __restrict__ var) {
+                             const double* __restrict__ alpha, const int* __restrict__ status,
+                             double* __restrict__ work, double* __restrict__ mean, double* __restrict__ var) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
+    if (p >= np || *status != SFH_OK) return;   // (no factor when the fit failed)
     const double* xp = Xp + (size_t)p * dim;
-    double* v = work + (size_t)p * n;
+    double* v = work + p;
     double m = 0.0;
     for (int i = 0; i < n; ++i) {
         const double ks = se_kernel(dim, X + (size_t)i * dim, xp, sigma, ell);
-        v[i] = ks;
+        v[(size_t)i * np] = ks;
         m += ks * alpha[i];
     }
     double q = 0.0;
     for (int i = 0; i < n; ++i) {
-        double s = v[i];
-        for (int k = 0; k < i; ++k) s -= L[(size_t)i * n + k] * v[k];
+        double s = v[(size_t)i * np];
+        for (int k = 0; k < i; ++k) s -= L[(size_t)i * n + k] * v[(size_t)k * np];
         s /= L[(size_t)i * n + i];
-        v[i] = s;
+        v[(size_t)i * np] = s;
         q += s * s;
     }
     mean[p] = m;
     var[p] = se_kernel(dim, xp, xp, sigma, ell) - q;
+}
+
+// predictCovariance: cov = k(x*, x*) - V^T V with V = L^-1 K* (C = V^T V given)
+__global__ void k_gp_cov(int np, int dim, const double* __restrict__ Xp, double sigma, double ell,
+                         const double* __restrict__ C, double* __restrict__ cov) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)np * np) return;
+    const int i = (int)(t / np), j = (int)(t % np);
+    cov[t] = se_kernel(dim, Xp + (size_t)i * dim, Xp + (size_t)j * dim, sigma, ell) - C[t];
 }
 
 // ------------------------------------------------- sparse-precision GP (sensor space)
@@ -1007,7 +1018,7 @@ int typical_readings(sfh_lik* L, double rho, double sd, double ld, int nr, uint6
 }
 
 int gp_run(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell, double eps,
-           int np, const double* Xp, double* logl, double* mean, double* var) {
+           int np, const double* Xp, double* logl, double* mean, double* var, double* cov) {
     hipStream_t st = c->stream;
     DevBuf dX, dy, dL, da, dXp, dw, dm, dv, dl, ds;
     SFH_TRY(dX.ensure(sizeof(double) * n * dim));
@@ -1035,14 +1046,26 @@ int gp_run(sfh_ctx* c, int n, int dim, const double* X, const double* y, double 
         SFH_TRY(dv.ensure(sizeof(double) * np));
         SFH_CHECK_HIP(hipMemcpyAsync(dXp.p, Xp, sizeof(double) * np * dim, hipMemcpyHostToDevice, st));
         k_gp_predict<<<nblocks(np, 64), 64, 0, st>>>(n, dim, np, dX.as<double>(), dXp.as<double>(), sigma, ell,
-                                                     dL.as<double>(), da.as<double>(), dw.as<double>(),
+                                                     dL.as<double>(), da.as<double>(), ds.as<int>(), dw.as<double>(),
                                                      dm.as<double>(), dv.as<double>());
         SFH_LAUNCH_CHECK();
+        if (cov) {
+            // GaussianProcess::predictCovariance (statX/infer/GaussianProcess.hpp:160-190)
+            DevBuf dC, dcov, gw;
+            SFH_TRY(dC.ensure(sizeof(double) * (size_t)np * np));
+            SFH_TRY(dcov.ensure(sizeof(double) * (size_t)np * np));
+            SFH_TRY(gram(c, n, np, np, dw.as<double>(), np, dw.as<double>(), np, dC.as<double>(), gw));
+            k_gp_cov<<<nblocks((long long)np * np, 256), 256, 0, st>>>(np, dim, dXp.as<double>(), sigma, ell,
+                                                                       dC.as<double>(), dcov.as<double>());
+            SFH_LAUNCH_CHECK();
+            SFH_CHECK_HIP(hipMemcpyAsync(cov, dcov.p, sizeof(double) * (size_t)np * np, hipMemcpyDeviceToHost, st));
+        }
         SFH_CHECK_HIP(hipMemcpyAsync(mean, dm.p, sizeof(double) * np, hipMemcpyDeviceToHost, st));
         SFH_CHECK_HIP(hipMemcpyAsync(var, dv.p, sizeof(double) * np, hipMemcpyDeviceToHost, st));
         SFH_CHECK_HIP(hipStreamSynchronize(st));
     }
     return SFH_OK;
 }
+
 
 }  // namespace sfh

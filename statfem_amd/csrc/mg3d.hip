@@ -211,8 +211,9 @@ bool coarse_inverse3(const G3& g, int dpn, const std::vector<double>& a, std::ve
             A[(size_t)i * n + j] = A[(size_t)j * n + i] = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
     for (int j = 0; j < n; ++j) {
         double d = A[(size_t)j * n + j];
+        const double d0 = d;
         for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
-        if (!(d > 0.0)) return false;
+        if (!(d > kPivotRel * d0)) return false;   // not SPD, or singular to rounding
         d = std::sqrt(d);
         A[(size_t)j * n + j] = d;
         for (int i = j + 1; i < n; ++i) {

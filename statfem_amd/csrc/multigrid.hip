@@ -142,7 +142,9 @@ __global__ void k_mg_level(int n, const double* __restrict__ A, float* __restric
 // its positive Schur-complement diagonals), every pivot step spread over the
 // work-group; the matrix lives in LDS (LDS = true, n^2 doubles of dynamic
 // shared memory) or in the global work area. The inverse is written in FP32.
-// bad[q] = 1 when a pivot is not positive (the operator is not SPD), else 0.
+// bad[q] = 1 when a pivot is not above kPivotRel times its row's diagonal
+// (the operator is not SPD, or singular to rounding: an operator without
+// Dirichlet rows), else 0.
 template <bool LDS>
 __global__ __launch_bounds__(256) void k_mg_coarse_inverse(const int* __restrict__ tab,
                                                            const double* __restrict__ a9,
@@ -169,7 +171,7 @@ __global__ __launch_bounds__(256) void k_mg_coarse_inverse(const int* __restrict
     __syncthreads();
     for (int k = 0; k < n; ++k) {
         const double p = A[(size_t)k * n + k];
-        if (!(p > 0.0)) {   // the same value in every thread
+        if (!(p > kPivotRel * s9[(size_t)k * 9 + 4])) {   // the same value in every thread
             if (tid == 0) bad[q] = 1;
             return;
         }
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_inverse_reg(const int* __res
             }
         __syncthreads();
         const double p = rb[k];
-        if (!(p > 0.0)) {   // the same value in every thread
+        if (!(p > kPivotRel * s9[(size_t)k * 9 + 4])) {   // the same value in every thread
             if (threadIdx.x == 0) bad[q] = 1;
             return;
         }

@@ -1902,10 +1902,16 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     // cycles aim at what the floor allows instead of iterating below it
     std::vector<double> dlt(ks, inner), rn2start(ks, 0.0);
     std::vector<char> reached(ks, 0);
+    // largest relative FP64 residual at each replacement, and a non-finite one
+    std::vector<double> hist;
+    int bad_col = -1;
     for (;;) {
         // residual replacement: decide per column, rescale, keep directions
         int nact = 0;
         maxrel = 0.0;
+        for (int c = 0; c < ks && bad_col < 0; ++c)
+            if (!std::isfinite(rn2[c])) bad_col = c;
+        if (bad_col >= 0) break;
         for (int c = 0; c < ks; ++c) {
             const double rel = bn2[c] > 0.0 ? std::sqrt(rn2[c] / bn2[c]) : 0.0;
             maxrel = std::max(maxrel, rel);
@@ -1925,6 +1931,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             nact += act;
         }
         std::fill(reached.begin(), reached.end(), 0);
+        hist.push_back(maxrel);
         if (nact == 0) { done = true; break; }
         if (updates > o.max_outer || iters >= o.max_inner) break;
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_SCALE * ks, scale.data(), ks * 8, hipMemcpyHostToDevice, st));
@@ -2071,7 +2078,28 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         stats->precond = fp64_rec ? 2 : 1;
         stats->levels = use_mg ? P.mgL + 1 : (use_g3 ? s->mg3.L + 1 : 1);
     }
+    if (bad_col >= 0) {
+        // (the reference's SparseLU reports the singular factorisation, corlib/SystemSolveEigenSparse.hpp:303-305)
+        set_error("non-finite FP64 residual (column %d) after %d iterations: the operator is singular or not "
+                  "positive definite", bad_col % kp, iters);
+        return SFH_ESINGULAR;
+    }
     if (!done) {
+        // a residual whose best value over the last replacements is not below
+        // half the best before them (stagnating, or growing as the iterate
+        // drifts along a null space) marks an operator that cannot be inverted
+        // on the right-hand side (singular, or a right-hand side outside its
+        // range); a falling one, the iteration cap
+        constexpr int kWin = 10;
+        const size_t nh = hist.size();
+        if (nh > (size_t)kWin &&
+            !(*std::min_element(hist.end() - kWin, hist.end()) <
+              0.5 * *std::min_element(hist.begin(), hist.end() - kWin))) {
+            set_error("the FP64 residual stagnated at %.3g of the right-hand side over the last %d replacements "
+                      "(%d iterations): the operator is singular or the right-hand side is not in its range",
+                      maxrel, kWin, iters);
+            return SFH_ESINGULAR;
+        }
         set_error("solve did not reach tol %.3g (max rel. residual %.3g after %d iterations, %d replacements)",
                   o.tol, maxrel, iters, updates);
         return SFH_ENOCONV;

@@ -103,6 +103,8 @@ _SIGS = {
     "sfh_lik_mismatch": (i32, [v, dbl, dbl, v]),
     "sfh_lik_post_proj": (i32, [v, dbl, dbl, dbl, v, v]),
     "sfh_lik_post_field": (i32, [v, dbl, dbl, dbl, i32, v, v, v]),
+    "sfh_lik_post_field_cov": (i32, [v, dbl, dbl, dbl, i32, v, v, i32, v]),
+    "sfh_system_set_covvec": (i32, [v, v]),
     "sfh_mcmc": (i32, [v, i32, i32, v, v, C.c_uint64, i32, v, v, v, v]),
     "sfh_lik_typical_readings": (i32, [v, dbl, dbl, dbl, i32, C.c_uint64, v]),
     "sfh_spmv": (i32, [v, i32, v, v]),
@@ -120,6 +122,7 @@ _SIGS = {
     "sfh_mlp_predict": (i32, [v, i32, v, v]),
     "sfh_mcmc_da": (i32, [v, v, i32, i32, i32, v, v, C.c_uint64, i32, v, v, v, v, v]),
     "sfh_gp": (i32, [v, i32, i32, v, v, dbl, dbl, dbl, i32, v, v, v, v]),
+    "sfh_gp_predict_cov": (i32, [v, i32, i32, v, v, dbl, dbl, dbl, i32, v, v]),
     "sfh_lhs_create": (i32, [v, i32, v, i32, i32, v, dbl, dbl, dbl, i32, i32, v, i32, i32, v, v, C.POINTER(v)]),
     "sfh_lhs_destroy": (i32, [v]),
     "sfh_lhs_set_data": (i32, [v, i32, v, i32, v, dbl, dbl, dbl, i32, v, v]),
@@ -552,6 +555,17 @@ class LikelihoodKOHHip:
         _check(lib().sfh_lik_post_field(self.h, rho, sd, ld, N, field.ptr, ubar.ptr, out.ptr))
         return out
 
+    def giveFEconditionedCovarianceWoodbury(self, rho, sd, ld, field, prior, full=True):
+        """Posterior field covariance C_u - (C_u P^T) S^-1 (P C_u) of
+        giveFEconditionedDataWoodbury (statX/infer/LikelihoodKOH.ipp:459-488)
+        from field = C_u P^T (device N x ns) and the prior: the N x N prior
+        covariance (full) or its diagonal (full=False, returns the variance)."""
+        N = field.shape[0]
+        pr = prior if isinstance(prior, DeviceArray) else self.ctx.to_device(prior)
+        out = self.ctx.empty((N, N) if full else (N,))
+        _check(lib().sfh_lik_post_field_cov(self.h, rho, sd, ld, N, field.ptr, pr.ptr, int(bool(full)), out.ptr))
+        return out
+
     def giveTypicalReadings(self, rho, sd, ld, nr, seed=0):
         Y = np.zeros((nr, self.ns))
         _check(lib().sfh_lik_typical_readings(self.h, rho, sd, ld, nr, seed, _p(Y)))
@@ -579,6 +593,19 @@ def gaussian_process(ctx, X, y, sigma, ell, eps, Xp):
     _check(lib().sfh_gp(ctx.h, len(y), dim, _p(X), _p(y), sigma, ell, eps, Xp.shape[0], _p(Xp), C.byref(logl),
                         _p(mean), _p(var)))
     return logl.value, mean, var
+
+
+def gp_predict_covariance(ctx, X, y, sigma, ell, eps, Xp):
+    """GaussianProcess::predictCovariance (statX/infer/GaussianProcess.hpp:160-190)
+    at the test locations Xp (np x dim); returns the np x np covariance."""
+    y = _h(y)
+    X = _h(X).reshape(len(y), -1)
+    dim = X.shape[1]
+    Xp = _h(Xp).reshape(-1, dim)
+    cov = np.zeros((Xp.shape[0], Xp.shape[0]))
+    _check(lib().sfh_gp_predict_cov(ctx.h, len(y), dim, _p(X), _p(y), sigma, ell, eps, Xp.shape[0], _p(Xp),
+                                    _p(cov)))
+    return cov
 
 
 def matern_tau(sigma, kappa, nu, dim):

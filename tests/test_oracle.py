@@ -304,6 +304,44 @@ def test_laplace_rhs_infer_woodbury_golden(orc):
     assert np.abs(cp - p["P"] @ cov @ p["P"].T).max() < 1e-8 * np.abs(cp).max()
 
 
+def test_woodbury_field_covariance_equals_information_form(orc):
+    """The Woodbury posterior covariance D C_u (LikelihoodKOH.ipp:484-488) on the
+    whole field equals the information form (C_u^-1 + rho^2 n_o P^T K_s^-1 P)^-1,
+    formed here with dense inverses (C_u carries the (sigma_f reg)^2 ridge)."""
+    p = laplace_rhs_infer_problem(orc)
+    m = p["mesh"]
+    lu = orc.LU(p["rp"], p["ci"], p["va"])
+    Rfull = lu.solve(np.eye(m.nnodes), trans=True).T
+    CuFull = orc.covforcing_project(m.coords, p["covvec"], p["sf"], p["lf"], Rfull) \
+        + (p["sf"] * p["reg"]) ** 2 * np.eye(m.nnodes)
+    rho, sd, ld = p["hp"]
+    Ks = orc.mismatch_cov(p["xs"], sd, ld, p["eps"])
+    _, cov = orc.post_woodbury(CuFull, Rfull @ p["F"], p["P"], Ks, p["Y"], rho)
+    no = p["Y"].shape[0]
+    info = np.linalg.inv(np.linalg.inv(CuFull) + rho * rho * no * p["P"].T @ np.linalg.solve(Ks, p["P"]))
+    assert np.abs(cov - info).max() < 1e-8 * np.abs(info).max()
+    assert np.abs(cov - cov.T).max() < 1e-12 * np.abs(cov).max()
+
+
+def test_gp_predict_covariance(orc):
+    """predictCovariance (GaussianProcess.hpp:160-190): its diagonal is the
+    golden predictive variance, and on well-separated test points it equals
+    the inverse of the test block of the joint precision matrix of
+    (y(X), f(X*)), the conditional covariance computed another way."""
+    hp, X, y, Xp, mean, var = _gp_fixture()
+    sg, lg, eg = (float(hp[f"gaussianProcess.{k}"]) for k in ("sigma", "length", "eps"))
+    cov = orc.gp_cov(X, sg, lg, eg, Xp, len(y))
+    assert np.abs(np.diag(cov) - var).max() < 1e-8
+    assert np.abs(cov - cov.T).max() < 1e-15
+    Xo = np.array([[0.0], [0.3], [0.55], [0.8], [1.0]])
+    Xs = np.array([[0.12], [0.41], [0.68], [0.93]])
+    s, l, e = 1.3, 0.25, 0.1
+    k = lambda A, B: s * s * np.exp(-(A[:, None, 0] - B[None, :, 0]) ** 2 / (2 * l * l))
+    J = np.block([[k(Xo, Xo) + e * e * np.eye(5), k(Xo, Xs)], [k(Xs, Xo), k(Xs, Xs)]])
+    ref = np.linalg.inv(np.linalg.inv(J)[5:, 5:])
+    assert np.abs(orc.gp_cov(Xo, s, l, e, Xs, 5) - ref).max() < 1e-11
+
+
 def test_loglik_equals_dense_gaussian_logpdf(orc):
     rng = np.random.default_rng(3)
     ns, no = 7, 4
