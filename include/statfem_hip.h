@@ -393,6 +393,18 @@ int sfh_mcmc_da(sfh_lik* lik, sfh_mlp* mlp, int chains, int steps, int sub, cons
 int sfh_gp(sfh_ctx* ctx, int n, int dim, const double* X, const double* y, double sigma,
            double ell, double eps, int np, const double* Xp, double* logl, double* mean,
            double* var);
+/* Allocation-free form of sfh_gp, with no counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). Inputs and outputs are
+ * device arrays (X n x dim, y n, Xp np x dim; logl 1, mean and var np,
+ * status 1 = SFH_OK or SFH_ENOTSPD), the work is enqueued on the context
+ * stream and the call returns without waiting; the scratch (Cholesky factor,
+ * weights, prediction block) is a fixed array of this GPU (about 4.7 MB, held
+ * until the process ends), so n <= 256 and np <= 2048, else SFH_EARG. Calls
+ * that overlap on one GPU race on that array. */
+int sfh_gp_noalloc(sfh_ctx* ctx, int n, int dim, const double* X_dev, const double* y_dev, double sigma, double ell,
+                   double eps, int np, const double* Xp_dev, double* logl_dev, double* mean_dev, double* var_dev,
+                   int* status_dev);
 /* GaussianProcess::predictCovariance (statX/infer/GaussianProcess.hpp:160-190):
  * cov (np x np, host) = k(X*, X*) - k(X, X*)^T (K + eps^2 I)^-1 k(X, X*) of the
  * process fitted as in sfh_gp. */

@@ -189,10 +189,13 @@ int sfh_ctx_create(int device, sfh_ctx** out) {
     SFH_CHECK_HIP(hipSetDevice(device));
     hipDeviceProp_t prop;
     SFH_CHECK_HIP(hipGetDeviceProperties(&prop, device));
+    double* scr = nullptr;
+    SFH_TRY(gp_scratch_init(device, &scr));   // this GPU's fixed GP scratch (sfh_gp_noalloc)
     sfh_ctx* c = new sfh_ctx;
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
     c->lds_per_block = prop.sharedMemPerBlock;
+    c->gp_scratch = scr;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
@@ -1012,6 +1015,21 @@ int sfh_gp(sfh_ctx* c, int n, int dim, const double* X, const double* y, double 
         return SFH_EARG;
     }
     return gp_run(c, n, dim, X, y, sigma, ell, eps, np, Xp, logl, mean, var, nullptr);
+}
+
+int sfh_gp_noalloc(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell, double eps,
+                   int np, const double* Xp, double* logl, double* mean, double* var, int* status) {
+    if (!c || n <= 0 || dim <= 0 || !X || !y || !logl || !status || np < 0 || (np > 0 && (!Xp || !mean || !var))) {
+        set_error("bad argument");
+        return SFH_EARG;
+    }
+    if (n > kGpFixedN || np > kGpFixedNP) {
+        set_error("sfh_gp_noalloc takes at most %d observations and %d prediction points (got %d, %d)", kGpFixedN,
+                  kGpFixedNP, n, np);
+        return SFH_EARG;
+    }
+    if (!c->gp_scratch) { set_error("no GP scratch on device %d", c->device); return SFH_ESTATE; }
+    return gp_run_fixed(c, n, dim, X, y, sigma, ell, eps, np, Xp, logl, mean, var, status);
 }
 
 int sfh_gp_predict_cov(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell,

@@ -91,6 +91,11 @@ constexpr int kMGCoarse = 160;       // coarsest level: at most this many nodes
 // diagonal, that counts as positive (a singular operator leaves pivots at the
 // rounding level of the diagonal)
 constexpr double kPivotRel = 1e-12;
+// allocation-free dense GP (sfh_gp_noalloc): at most kGpFixedN observations
+// and kGpFixedNP prediction points, scratch of L, alpha and L^-1 k*
+constexpr int kGpFixedN = 256, kGpFixedNP = 2048;
+constexpr size_t kGpFixedScratch = (size_t)kGpFixedN * kGpFixedN + kGpFixedN + (size_t)kGpFixedN * kGpFixedNP;
+constexpr int kMaxDevices = 64;
 struct MGHier {
     bool built = false;              // current for the stencil copy (reset when that is rebuilt)
     int L = 0;                       // coarse levels of the deepest grid (0: no hierarchy)
@@ -148,6 +153,8 @@ struct sfh_ctx {
     std::vector<hipEvent_t> kev;     // 4 events per sample
     double ksum[3] = {0.0, 0.0, 0.0};
     long long kcount = 0;
+    // device address of this GPU's fixed GP scratch (sfh_gp_noalloc)
+    double* gp_scratch = nullptr;
 };
 
 struct sfh_system {
@@ -275,6 +282,9 @@ int spmv(sfh_system* s, int k, const double* X, double* Y);
 int precision_apply(sfh_system* s, const double* mass, double tau2k4b, int alpha, int k, const double* X, double* Y);
 int frac_apply(sfh_system* s, const double* mass, int degree, const double* roots, int transpose, int k,
                const double* X, double* Y);
+int gp_scratch_init(int device, double** ptr);
+int gp_run_fixed(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell, double eps,
+                 int np, const double* Xp, double* logl, double* mean, double* var, int* status);
 int gp_run(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell,
            double eps, int np, const double* Xp, double* logl, double* mean, double* var, double* cov);
 }  // namespace sfh

@@ -10,6 +10,7 @@
 //  * random-walk Metropolis in lockstep across chains (statX/infer/MCMC.hpp:123-158)
 //    with Philox4x32-10 streams.
 #include <cmath>
+#include <mutex>
 
 #include "common.hpp"
 #include "mcmc.hpp"
@@ -1067,5 +1068,53 @@ int gp_run(sfh_ctx* c, int n, int dim, const double* X, const double* y, double 
     return SFH_OK;
 }
 
+
+// ------------------------------------------- allocation-free GP (fixed scratch)
+// The scratch of the GP kernels -- the Cholesky factor L (n x n), the weights
+// alpha (n) and the prediction block L^-1 k* (n x np) -- in one file-scope
+// device array, so that sfh_gp_noalloc allocates nothing. Each GPU holds its
+// own copy; its device address is looked up once per device when a context is
+// created there (gp_scratch_init) and only that address is passed to kernels.
+// Every call on a GPU reuses the array: the kernels write every entry they
+// read within the call, and calls that overlap on one GPU race.
+static __device__ __align__(256) double scratch[kGpFixedScratch];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+int gp_scratch_init(int device, double** ptr) {
+    if (device < 0 || device >= kMaxDevices) { set_error("device %d beyond %d", device, kMaxDevices); return SFH_EARG; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            return SFH_EDEVICE;
+        }
+        g_scratch_dev[device] = static_cast<double*>(p);
+    }
+    *ptr = g_scratch_dev[device];
+    return SFH_OK;
+}
+
+int gp_run_fixed(sfh_ctx* c, int n, int dim, const double* X, const double* y, double sigma, double ell, double eps,
+                 int np, const double* Xp, double* logl, double* mean, double* var, int* status) {
+    hipStream_t st = c->stream;
+    double* L = c->gp_scratch;
+    double* alpha = L + (size_t)kGpFixedN * kGpFixedN;
+    double* work = alpha + kGpFixedN;
+    const size_t lds = sizeof(double) * (n + 256);
+    if (lds > c->lds_per_block) { set_error("GP too large"); return SFH_EARG; }
+    k_gp_fit<<<1, 256, lds, st>>>(n, dim, X, y, sigma, ell, eps, L, alpha, logl, status);
+    SFH_LAUNCH_CHECK();
+    if (np > 0) {
+        k_gp_predict<<<nblocks(np, 64), 64, 0, st>>>(n, dim, np, X, Xp, sigma, ell, L, alpha, status, work, mean, var);
+        SFH_LAUNCH_CHECK();
+    }
+    return SFH_OK;
+}
 
 }  // namespace sfh

@@ -123,6 +123,7 @@ _SIGS = {
     "sfh_mcmc_da": (i32, [v, v, i32, i32, i32, v, v, C.c_uint64, i32, v, v, v, v, v]),
     "sfh_gp": (i32, [v, i32, i32, v, v, dbl, dbl, dbl, i32, v, v, v, v]),
     "sfh_gp_predict_cov": (i32, [v, i32, i32, v, v, dbl, dbl, dbl, i32, v, v]),
+    "sfh_gp_noalloc": (i32, [v, i32, i32, v, v, dbl, dbl, dbl, i32, v, v, v, v, v]),
     "sfh_lhs_create": (i32, [v, i32, v, i32, i32, v, dbl, dbl, dbl, i32, i32, v, i32, i32, v, v, C.POINTER(v)]),
     "sfh_lhs_destroy": (i32, [v]),
     "sfh_lhs_set_data": (i32, [v, i32, v, i32, v, dbl, dbl, dbl, i32, v, v]),
