@@ -28,10 +28,54 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(args):
+    """HBM bytes per launch of the C2 step's kernels, measured by this run: two
+    rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) over a one-step
+    child run of this benchmark, each scaled by what the same counter reads
+    for a 1 GiB device-to-device copy in that run. Runs before this process
+    touches the GPU. Returns {kernel name: bytes per launch} (the largest-grid
+    dispatch of each name), or None when the profiler is not there or fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        log("rocprofv3 not found: traffic from the committed counter summary")
+        return None
+    from statfem_amd import rocprof_summary as rs
+    tmp = tempfile.mkdtemp(prefix="sfh_pmc_", dir="/tmp")
+    child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu", "--no-c1",
+             "--no-c5", "--no-c3", "--no-c4", "--no-lhs", "--no-gpq", "--no-fpert", "--copy-cal", "--no-traffic",
+             "--n", str(args.n), "--ns", str(args.ns), "--tol", str(args.tol)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        log(f"counter pass {counter}")
+        cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", counter, "--output-format", "csv",
+               "-d", os.path.join(tmp, counter), "-o", "c2", "--"] + child
+        r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, env=env, cwd="/tmp")
+        if r.returncode != 0:
+            log(f"counter pass {counter} failed ({r.returncode}): traffic from the committed counter summary")
+            return None
+    try:
+        t = rs.traffic(os.path.join(tmp, "FETCH_SIZE", "c2_counter_collection.csv"),
+                       os.path.join(tmp, "WRITE_SIZE", "c2_counter_collection.csv"))
+    except (OSError, KeyError, ValueError) as e:
+        log(f"counter summary failed ({e}): traffic from the committed counter summary")
+        return None
+    out = {}
+    for key, v in t["kernels"].items():
+        name, grid = key.rsplit(" grid=", 1)
+        if name not in out or int(grid) > out[name][0]:
+            out[name] = (int(grid), v["bytes"])
+    log(f"measured traffic (scales fetch {t['fetch_scale']:.3f}, write {t['write_scale']:.3f})")
+    return {k: b for k, (g, b) in out.items()}
 
 
 def profiled_traffic(kernel):
@@ -50,7 +94,7 @@ def profiled_traffic(kernel):
     return None
 
 
-def gpu_run(args):
+def gpu_run(args, traffic=None):
     from statfem_amd import configs, hip
 
     n, ns = args.n, args.ns
@@ -119,6 +163,10 @@ def gpu_run(args):
         # node; direction: z, p_old read, p written (FP32); update: p read
         # (FP32), FP64 x and r read and written
         algo = {"dir_spmv": 72.0 * N + 3 * vec, "cg_update": 72.0 * N + 9 * vec}
+        # the survey's FP32 model of the same two phases (per node and column:
+        # direction 3 words, update 5 words; FP32 CSR matrix once per 64-column tile)
+        mat32 = (8.0 * nnz + 4.0 * (N + 1)) * max(1, kp // 64)
+        algo_8d = {"dir_spmv": mat32 + 3 * vec, "cg_update": mat32 + 5 * vec}
         kname = {"dir_spmv": "k_dir_spmv_mg<{lpr}>", "cg_update": "k_cg_update_mg<{lpr}>"}
         # V-cycle, all levels: per level FP32 stencil + smoother weight (40 B
         # per node, read twice); level 0 reads the FP64 residual twice and
@@ -138,7 +186,11 @@ def gpu_run(args):
         mat = 8.0 * nnz + 4.0 * (N + 1)                   # packed (column, value) entries + row pointers
         algo = {"dir_spmv": mat + 3 * vec, "cg_update": mat + 8.0 * N + 5 * vec}
         kname = {"dir_spmv": "k_dir_spmv<float, {lpr}>", "cg_update": "k_cg_update<float, {lpr}>"}
-    per = {k: {"ms": kt[k], "GBs": algo[k] / (kt[k] * 1e-3) / 1e9, "bytes": algo[k], "samples": kt["samples"]}
+    if stats[-1]["layout"] != 2 or stats[-1]["precond"] != 2:
+        algo_8d = dict(algo)
+    per = {k: {"ms": kt[k], "GBs": algo_8d[k] / (kt[k] * 1e-3) / 1e9, "bytes": algo_8d[k],
+               "GBs_fp64_recurrences": algo[k] / (kt[k] * 1e-3) / 1e9, "bytes_fp64_recurrences": algo[k],
+               "samples": kt["samples"]}
            for k in algo}
     if stats[-1]["precond"] == 2 and kt.get("vcycle", 0) > 0:
         per["vcycle"] = {"ms": kt["vcycle"], "GBs": algo_vc / (kt["vcycle"] * 1e-3) / 1e9, "bytes": algo_vc,
@@ -146,7 +198,11 @@ def gpu_run(args):
     dom = max(("dir_spmv", "cg_update"), key=lambda k: kt[k])
     lpr = min(16 if stats[-1]["layout"] == 2 else 64, 1 << max(0, (kp // 4 - 1).bit_length()))
     for k in algo:
-        per[k]["traffic"] = profiled_traffic(kname[k].format(lpr=lpr))
+        kn = kname[k].format(lpr=lpr)
+        if traffic is not None and kn in traffic:
+            per[k]["traffic"], per[k]["traffic_source"] = traffic[kn], "counter passes of this run"
+        else:
+            per[k]["traffic"], per[k]["traffic_source"] = profiled_traffic(kn), "profiles/pcg_c2_counters.txt"
     asm_bytes = m.nnodes * 2 * 8 + m.nel * 4 * 4 + m.nel * 8 + nnz * 8 + nnz * 4
     dense = dense_stages(ctx, s, ns, cfg)
     res = {
@@ -358,6 +414,29 @@ def c3_run(args):
         f"(R^T {st['Rt']['inner_iters']} it, Matern {st['matern']['inner_iters']} it), "
         f"posterior {pipe.times['posterior']:.2f}s")
     fm = post["field_mean"]
+    # elasticity assembly alone (hex8, 24 x 24 element matrices into the CSR
+    # values, FP64 + FP32 copy): device time per assembly
+    se = pipe.se
+    se.assemble(hip.ELASTICITY, cfg["E"], cfg["nu"], nq1d=cfg["nq1d"])
+    ctx.timer_start()
+    for _ in range(5):
+        se.assemble(hip.ELASTICITY, cfg["E"], cfg["nu"], nq1d=cfg["nq1d"])
+    asm_ms = ctx.timer_stop() / 5
+    se.applyConstraints(cfg["bc"])
+    m = cfg["mesh"]
+    asm_bytes = 12.0 * se.nnz + 24.0 * m.nnodes + 32.0 * m.nel   # CSR values FP64 + FP32, coordinates, connectivity
+    asm_flop = 30e3 * m.nel                                       # element stiffness at 8 Gauss points
+    bound_ms = max(asm_bytes / (HBM_PEAK_GBS * 1e9), asm_flop / (FP64_PEAK_TFLOPS * 1e12)) * 1e3
+    log(f"C3 elasticity assembly {asm_ms:.3f} ms (bound {bound_ms:.3f} ms)")
+    # the same pipeline at the size the FP64 oracle covers (16^3), for the error
+    small = None
+    if not args.no_cpu:
+        cfg16 = configs.c3_inputs(seed=0, n=16)
+        p16 = SPDEElasticity(ctx, cfg16, opts=opts, opts_matern=hip.default_opts(tol=1e-10))
+        m16, C16 = p16.prior()
+        y16, _, _ = p16.synthesize()
+        post16 = p16.posterior(m16, C16, y16, field=True)
+        small = {"cfg": cfg16, "mean": m16, "Cu": C16, "y": y16, "post": post16, "field": p16._field.download()}
     return {
         "workload": f"elasticity {args.c3_n}^3 hex8, {pipe.ndof} DOFs, {pipe.ns} sensors, SPDE Matern load",
         "dofs": pipe.ndof, "rhs": pipe.ns,
@@ -367,7 +446,11 @@ def c3_run(args):
         "max_relres": max(v["max_relres"] for v in st.values()),
         "logdet": post["logdet"], "posterior_sd_mean": float(np.sqrt(post["var"]).mean()),
         "field_mean_vs_truth_rel": float(np.abs(fm - u).max() / np.abs(u).max()),
-    }
+        "assembly_roofline": {"kernel": "k_assemble_elastic", "ms": asm_ms, "bytes": asm_bytes, "gflop": asm_flop / 1e9,
+                              "bound": "fp64" if asm_flop / (FP64_PEAK_TFLOPS * 1e12) > asm_bytes / (HBM_PEAK_GBS * 1e9)
+                              else "hbm",
+                              "bound_ms": bound_ms, "frac": bound_ms / asm_ms},
+    }, small
 
 
 def c4_run(args):
@@ -403,18 +486,20 @@ def c4_run(args):
     t_seq = time.perf_counter() - t0
     rm, rC, _ = seq.updateSolution(hip.default_opts(tol=1e-12, precision=1))
     seq.close()
+    # (against the library's own FP64-recurrence solve of every problem; the
+    # error against the FP64 oracle is computed in the CPU leg on a sample)
     err = [max(np.abs(C - R).max() / np.abs(R).max(), np.abs(m - r).max() / np.abs(r).max())
            for C, R, m, r in zip(Cus, rC, means, rm)]
     log(f"C4 batched {t_batch:.3f}s ({st['inner_iters']} it), sequential {t_seq:.3f}s ({sit} it), "
-        f"max rel err vs FP64 {max(err):.2e}")
+        f"max rel diff vs the GPU FP64 path {max(err):.2e}")
     return {
         "workload": f"{len(probs)} Poisson Q1 problems, n in [32, 256], {int(dofs_rhs)} DOF-RHS in total",
         "dofs": int(bf.mesh.nnodes), "rhs_tile": tile, "tile_cost_ms": {str(k): v for k, v in cost.items()},
         "batched_s": t_batch, "sequential_s": t_seq, "speedup": t_seq / t_batch, "setup_s": t_setup,
         "batched_dof_rhs_per_s": dofs_rhs / t_batch, "sequential_dof_rhs_per_s": dofs_rhs / t_seq,
         "batched_iters": st["inner_iters"], "sequential_iters": sit,
-        "max_rel_err_vs_fp64": float(max(err)), "median_rel_err_vs_fp64": float(np.median(err)),
-    }
+        "max_rel_diff_vs_gpu_fp64": float(max(err)),
+    }, {"probs": probs, "means": means, "Cus": Cus}
 
 
 def lhs_run(args):
@@ -617,6 +702,53 @@ def fpert_run(args):
     }
 
 
+def cpu_baseline_c3(small):
+    """The elasticity statFEM at 16^3 (14,739 DOFs, 128 sensors) by the FP64
+    oracle (banded Cholesky of both operators, one host thread for the
+    factorisations): the error of the GPU's prior and posterior at that size,
+    and the oracle's own time as the CPU baseline of this leg."""
+    from oracle.forward import elasticity_spde_reference
+    cfg = small["cfg"]
+    t0 = time.perf_counter()
+    ref = elasticity_spde_reference(cfg, small["y"])
+    t_cpu = time.perf_counter() - t0
+    rel = lambda a, b: float(np.abs(a - b).max() / np.abs(b).max())
+    post = small["post"]
+    err = {"prior_mean": rel(small["mean"], ref["mean"]), "Cu": rel(small["Cu"], ref["Cu"]),
+           "field": rel(small["field"], ref["field"]), "post_mean": rel(post["mean"], ref["post_mean"]),
+           "post_var": rel(post["var"], ref["post_var"]),
+           "logdet": float(abs(post["logdet"] - ref["logdet"]) / abs(ref["logdet"])),
+           "field_mean": rel(post["field_mean"], ref["field_mean"]), "size": "16^3 hex8, 14739 DOFs, 128 sensors"}
+    N = 3 * cfg["mesh"].nnodes
+    cpu = {"value": N * len(cfg["cols"]) / t_cpu, "unit": "DOF-RHS/s", "cores": 16, "kind": "port",
+           "sample": f"oracle elasticity statFEM at 16^3 ({N} DOFs, 128 sensors): assembly, banded Cholesky of the "
+                     f"elasticity and Matern operators (1 thread), multi-RHS solves (16 threads), prior, field "
+                     f"block and posterior: {t_cpu:.2f}s"}
+    log(f"C3 16^3 error vs FP64 oracle: {err}")
+    return err, cpu
+
+
+def cpu_baseline_c4(gpu, nsample=8):
+    """Error of the batched forward model against the FP64 oracle on a sample
+    of the 64 problems spread over the grid sizes (mean and C_u per problem)."""
+    from oracle.forward import ForwardGauss2D
+    probs = gpu["probs"]
+    order = np.argsort([p["n"] for p in probs])
+    pick = sorted(set(int(order[i]) for i in np.linspace(0, len(order) - 1, nsample).round().astype(int)))
+    errs = []
+    t0 = time.perf_counter()
+    for q in pick:
+        p = probs[q]
+        ref = ForwardGauss2D(p["mesh"], p["bc"], 1.0).forward_gauss(p["cols"], p["vals"], p["sigma_f"], p["ell_f"],
+                                                                     p["reg"])
+        errs.append(max(float(np.abs(gpu["means"][q] - ref["mean"]).max() / np.abs(ref["mean"]).max()),
+                        float(np.abs(gpu["Cus"][q] - ref["Cu"]).max() / np.abs(ref["Cu"]).max())))
+    log(f"C4 error vs FP64 oracle on problems {pick}: {errs} ({time.perf_counter() - t0:.1f}s)")
+    return {"max": max(errs), "median": float(np.median(errs)), "problems": pick,
+            "grid_sizes": [int(probs[q]["n"]) for q in pick],
+            "oracle": "banded Cholesky + separable C_f checked by direct sums, FP64"}
+
+
 def c2_errors(orc, cfg, rp, ci, va, ch, F, res):
     """Relative error of the GPU's C2 outputs against the oracle's FP64 direct
     solves: R^T = A^-T P^T on the sampled columns (2-norm per column, the
@@ -732,6 +864,8 @@ def main():
     ap.add_argument("--c5-steps", type=int, default=20000)
     ap.add_argument("--c5-da-steps", type=int, default=2000)
     ap.add_argument("--c5-sub", type=int, default=32)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure the kernels' HBM traffic")
     ap.add_argument("--copy-cal", action="store_true",
                     help="also run a 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -739,7 +873,8 @@ def main():
         print("bench.py measures one GPU only (there is no multi-GPU path); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
-    res, dom = gpu_run(args)
+    traffic = None if args.no_traffic else measured_traffic(args)
+    res, dom = gpu_run(args, traffic)
     if args.copy_cal:
         from statfem_amd import hip
         c = hip.Context(0)
@@ -751,12 +886,16 @@ def main():
         log("copy calibration: 1 GiB device-to-device")
     c1 = None if args.no_c1 else c1_run(args)
     c5 = None if args.no_c5 else c5_run(args)
-    c3 = None if args.no_c3 else c3_run(args)
-    c4 = None if args.no_c4 else c4_run(args)
+    c3, c3_small = (None, None) if args.no_c3 else c3_run(args)
+    c4, c4_gpu = (None, None) if args.no_c4 else c4_run(args)
     lhs = None if args.no_lhs else lhs_run(args)
     gpq = None if args.no_gpq else gpq_run(args)
     fpert = None if args.no_fpert else fpert_run(args)
     err, cpu = (None, None) if args.no_cpu else cpu_baseline(args, 30.0, res)
+    if c3 is not None and c3_small is not None:
+        c3["max_rel_err_vs_fp64"], c3["cpu_baseline"] = cpu_baseline_c3(c3_small)
+    if c4 is not None and not args.no_cpu:
+        c4["max_rel_err_vs_fp64"] = cpu_baseline_c4(c4_gpu)
     line = {
         "metric": "assembled+solved DOF-RHS/s; MCMC ESS/s surrogate vs plain GPU at 1e-8 rel err",
         "value": res["value"],
@@ -778,7 +917,12 @@ def main():
                    "precond": res["precond"], "mg_levels": res["mg_levels"],
                    "kernels": res["kernels"], "assembly": res["assembly"]},
         "roofline": {"kernel": res["kernel"], "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": dom.get("traffic")},
+                     "unit": "GB/s", "frac": dom["GBs"] / HBM_PEAK_GBS, "traffic": dom.get("traffic"),
+                     "traffic_source": dom.get("traffic_source"),
+                     "model": "FP32 words per node and column (x, r, p read; x, r written) + FP32 CSR per tile",
+                     "achieved_fp64_recurrences": dom["GBs_fp64_recurrences"],
+                     "frac_fp64_recurrences": dom["GBs_fp64_recurrences"] / HBM_PEAK_GBS,
+                     "bytes_per_launch": dom["bytes"], "bytes_per_launch_fp64_recurrences": dom["bytes_fp64_recurrences"]},
         "cpu_baseline": cpu,
         "statfem_update": c1,
         "mcmc": c5,

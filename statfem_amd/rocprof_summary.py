@@ -2,6 +2,7 @@
 
     python -m statfem_amd.rocprof_summary stats  <kernel_stats.csv> [rows]
     python -m statfem_amd.rocprof_summary traffic <fetch counter csv> <write counter csv> [kernel substring]
+    python -m statfem_amd.rocprof_summary mfma <counter csv> <kernel trace csv> [kernel substring]
 
 `traffic` reports mean FETCH_SIZE / WRITE_SIZE (KiB per dispatch) per kernel
 and grid size, and converts them to bytes with the factors that the same
@@ -66,6 +67,39 @@ def traffic(fetch_csv, write_csv, copy_bytes=1 << 30, match=""):
     return out
 
 
+def mfma(counter_csv, trace_csv, match="", simds=1024, peak_tflops=78.6):
+    """Per kernel (name, grid): FP64 MFMA flops (SQ_INSTS_VALU_MFMA_MOPS_F64 x
+    512) per dispatch, their rate over the dispatch's duration from the kernel
+    trace of the same run, and the MFMA busy fraction
+    SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x SIMDs)."""
+    dur = {}
+    for r in csv.DictReader(open(trace_csv)):
+        dur[r["Dispatch_Id"]] = float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    for r in csv.DictReader(open(counter_csv)):
+        per[(short(r["Kernel_Name"]), int(r["Grid_Size"]), r["Dispatch_Id"])][r["Counter_Name"]] += \
+            float(r["Counter_Value"])
+    agg = collections.defaultdict(list)
+    for (k, g, d), c in per.items():
+        if match and match not in k:
+            continue
+        flops = c.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0) * 512
+        ns = dur.get(d)
+        busy, act = c.get("SQ_VALU_MFMA_BUSY_CYCLES"), c.get("GRBM_GUI_ACTIVE")
+        agg[(k, g)].append((flops, ns, busy / (act * simds) if busy is not None and act else None))
+    out = {}
+    for (k, g), v in agg.items():
+        fl = sum(x[0] for x in v) / len(v)
+        ts = [x[1] for x in v if x[1]]
+        us = [x[2] for x in v if x[2] is not None]
+        t = sum(ts) / len(ts) if ts else None
+        out[f"{k} grid={g}"] = {"dispatches": len(v), "gflop": fl / 1e9, "us": t / 1e3 if t else None,
+                               "tflops": fl / t / 1e3 if t else None,
+                               "frac": fl / t / 1e3 / peak_tflops if t else None,
+                               "mfma_busy": sum(us) / len(us) if us else None}
+    return out
+
+
 def main(argv):
     if argv[0] == "stats":
         print(stats(argv[1], top=int(argv[2]) if len(argv) > 2 else 20))
@@ -79,6 +113,15 @@ def main(argv):
                 continue
             print(f"{k[:60]:60s} {v['dispatches']:6d} {v['fetch_KiB']:12.1f} {v['write_KiB']:12.1f} "
                   f"{v['bytes']:14.4e}")
+    elif argv[0] == "mfma":
+        t = mfma(argv[1], argv[2], match=argv[3] if len(argv) > 3 else "")
+        print(f"{'kernel':48s} {'n':>5s} {'GFLOP':>9s} {'us':>9s} {'TFLOP/s':>9s} {'of peak':>8s} {'busy':>7s}")
+        for k, v in sorted(t.items(), key=lambda kv: -(kv[1]["gflop"] or 0)):
+            if v["gflop"] <= 0:
+                continue
+            f = lambda x, fmt: format(x, fmt) if x is not None else "-"
+            print(f"{k[:48]:48s} {v['dispatches']:5d} {v['gflop']:9.3f} {f(v['us'], '9.1f')} "
+                  f"{f(v['tflops'], '9.2f')} {f(v['frac'], '8.3f')} {f(v['mfma_busy'], '7.3f')}")
     else:
         raise SystemExit(__doc__)
 

@@ -176,3 +176,25 @@ def test_narrow_batch_matches_oracle(ctx, orc, k, precond):
         ref = orc.LU(rp, ci, va).solve(B[off[q]:off[q + 1]])
         got = Xh[off[q]:off[q + 1]]
         assert np.abs(got - ref).max() <= 1e-8 * np.abs(ref).max(), q
+
+
+def test_batched_forward_gauss_full_size_vs_fp64_oracle(ctx, orc):
+    """The batched-uneven workload at its real sizes (64 problems, n in
+    [32, 256], 16-128 sensors): the batched forward model of four problems
+    (the smallest and the largest grid, two between) against the FP64 oracle,
+    mean and C_u within 1e-8."""
+    from oracle.forward import ForwardGauss2D
+    from statfem_amd import configs, hip
+    from statfem_amd.batch import BatchedForwardGauss
+    probs = configs.c4_inputs(seed=0)
+    bf = BatchedForwardGauss(ctx, probs)
+    means, Cus, st = bf.updateSolution(hip.default_opts(tol=1e-10))
+    order = np.argsort([p["n"] for p in probs])
+    for q in (order[0], order[len(order) // 3], order[2 * len(order) // 3], order[-1]):
+        p = probs[q]
+        ref = ForwardGauss2D(p["mesh"], p["bc"], 1.0).forward_gauss(p["cols"], p["vals"], p["sigma_f"],
+                                                                     p["ell_f"], p["reg"])
+        em = np.abs(means[q] - ref["mean"]).max() / np.abs(ref["mean"]).max()
+        ec = np.abs(Cus[q] - ref["Cu"]).max() / np.abs(ref["Cu"]).max()
+        print(f"problem {q}: n {p['n']}, {len(p['cols'])} sensors: mean {em:.1e}, C_u {ec:.1e}", flush=True)
+        assert em <= 1e-8 and ec <= 1e-8, (q, em, ec)

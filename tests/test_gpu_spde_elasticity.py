@@ -77,3 +77,34 @@ def test_elasticity_multi_rhs_solves_match_oracle(ctx, orc):
         X, st = s.solve(B, trans=trans, opts=hip.default_opts(tol=1e-12))
         ref = lu.solve(B, trans=trans)
         assert np.abs(X.download() - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def test_spde_elasticity_16_cubed_vs_fp64_oracle(ctx, orc):
+    """The 3D elasticity statFEM at 16^3 cells (14,739 DOFs, 128 top-face
+    sensors) on its production path (declared 3D grids, multigrid, tolerance
+    1e-10): prior mean and C_u, the field block, the synthetic reading, and the
+    posterior mean, variance and log det within 1e-8 of the FP64 oracle."""
+    from statfem_amd import configs, hip
+    from statfem_amd.spde_elasticity import SPDEElasticity
+    cfg = configs.c3_inputs(seed=0, n=16, ns=128)
+    pipe = SPDEElasticity(ctx, cfg, opts=hip.default_opts(tol=1e-10), opts_matern=hip.default_opts(tol=1e-10))
+    mean, Cu = pipe.prior()
+    assert pipe.stats["Rt"]["layout"] == 4 and pipe.stats["Rt"]["precond"] == 2
+    y, g, u = pipe.synthesize()
+    post = pipe.posterior(mean, Cu, y, field=True)
+    from oracle.forward import elasticity_spde_reference
+    ref = elasticity_spde_reference(cfg, y)
+    rel = lambda a, b: np.abs(a - b).max() / np.abs(b).max()
+    errs = {"mean": rel(mean, ref["mean"]), "Cu": rel(Cu, ref["Cu"]),
+            "field": rel(pipe._field.download(), ref["field"]),
+            "post_mean": rel(post["mean"], ref["post_mean"]), "post_var": rel(post["var"], ref["post_var"]),
+            "logdet": abs(post["logdet"] - ref["logdet"]) / abs(ref["logdet"]),
+            "field_mean": rel(post["field_mean"], ref["field_mean"])}
+    print("C3 16^3 errors vs FP64:", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    assert ref["sign"] > 0
+    for k, v in errs.items():
+        assert v <= 1e-8, (k, v)
+    # the reading is the forward model at the sampled load
+    Fg = ref["F"].copy()
+    Fg[2::3] += ref["w"] * g
+    assert np.abs(ref["ch"].solve(Fg) - u).max() <= 1e-8 * np.abs(u).max()

@@ -790,3 +790,64 @@ def test_threaded_banded_solve_matches_serial(orc):
     B = np.random.default_rng(3).standard_normal((m.nnodes, 11))
     B[bc] = 0.0
     assert np.array_equal(ch.solve(B), ch.solve(B, threads=4))
+
+
+def test_forward_gauss_2d_composition_matches_reference_form(orc):
+    """oracle.forward (banded Cholesky, restored constrained components,
+    separable C_f) against the reference form at a small size: transposed LU
+    solves and the dense C_f projection (CovarianceForcing.ipp:37-109)."""
+    from oracle.forward import ForwardGauss2D
+    from statfem_amd import configs
+    cfg = configs.c2_inputs(seed=4, n=14, ns=9)
+    m = cfg["mesh"]
+    fg = ForwardGauss2D(m, cfg["bc"], cfg["kappa"], threads=2)
+    ref = fg.forward_gauss(cfg["cols"], cfg["vals"], 0.1, 0.2, 1e-3, sample=[0, 3, 8])
+    P = M.dense_projection(m.nnodes, cfg["cols"], cfg["vals"])
+    lu = orc.LU(fg.rp, fg.ci, fg.va)
+    R = lu.solve(P.T.copy(), trans=True)
+    assert np.abs(ref["Rt"] - R[:, [0, 3, 8]]).max() <= 1e-12 * np.abs(R).max()
+    Cu = orc.covforcing_project(m.coords, fg.cv, 0.1, 0.2, R.T.copy()) + (1e-4) ** 2 * P @ P.T
+    assert np.abs(ref["Cu"] - Cu[np.ix_([0, 3, 8], [0, 3, 8])]).max() <= 1e-12 * np.abs(Cu).max()
+    mean = R.T @ fg.F
+    assert np.abs(ref["mean"] - mean).max() <= 1e-12 * np.abs(mean).max()
+
+
+def test_elasticity_spde_reference_matches_dense_composition(orc):
+    """oracle.forward.elasticity_spde_reference (banded Cholesky factors, sparse
+    products) against the dense composition of the same oracle operators at 4^3:
+    C_u = R B Q^-1 B^T R^T, the field block C_u P^T and the Woodbury field mean
+    of or_post_woodbury."""
+    from oracle.forward import elasticity_spde_reference
+    from statfem_amd import configs
+    cfg = configs.c3_inputs(seed=3, n=4, ns=6)
+    m = cfg["mesh"]
+    y = np.random.default_rng(0).normal(size=6) * 1e-3
+    ref = elasticity_spde_reference(cfg, y)
+    F = orc.body_force(m, nq1d=2, dofpn=3, fconst=[0.0, 0.0, cfg["fbar"]])
+    rp, ci, va, F = orc.assemble(m, orc.ELASTICITY, cfg["E"], cfg["nu"], nq1d=2, bc_dofs=cfg["bc"], F=F)
+    Ae = orc.LU(rp, ci, va)
+    kap = cfg["kappa"]
+    rpm, cim, vam, _ = orc.assemble(m, orc.MATERN, kap, 1.0, nq1d=2, factor=1.0 / kap ** 2)
+    Am = _dense(rpm, cim, vam, m.nnodes)
+    mass = orc.body_force(m, nq1d=2, fconst=1.0)
+    tau = orc.matern_tau(cfg["sigma_g"], kap, cfg["nu_m"], 3)
+    Ainv = np.linalg.inv(Am)
+    Cg = Ainv @ np.diag(mass) @ Ainv / (tau * kap ** 2) ** 2
+    w = mass.copy()
+    w[cfg["bottom"]] = 0.0
+    N = 3 * m.nnodes
+    B = np.zeros((N, m.nnodes))
+    B[3 * np.arange(m.nnodes) + 2, np.arange(m.nnodes)] = w
+    P = np.zeros((len(cfg["cols"]), N))
+    for s in range(P.shape[0]):
+        np.add.at(P[s], cfg["cols"][s], cfg["vals"][s])
+    Rt = Ae.solve(P.T.copy(), trans=True)
+    Cu = Rt.T @ B @ Cg @ B.T @ Rt
+    ubar = Ae.solve(F)
+    CuFull = Ae.solve(B @ Cg @ B.T @ Ae.solve(np.eye(N), trans=True), trans=False)
+    Ks = orc.mismatch_cov(cfg["xs"], 0.0, 1.0, cfg["sigma_e"])
+    fm, _ = orc.post_woodbury(CuFull, ubar, P, Ks, np.atleast_2d(y), 1.0, want_cov=False)
+    assert np.abs(ref["Cu"] - Cu).max() <= 1e-12 * np.abs(Cu).max()
+    assert np.abs(ref["mean"] - P @ ubar).max() <= 1e-12 * np.abs(P @ ubar).max()
+    assert np.abs(ref["field"] - CuFull @ P.T).max() <= 1e-12 * np.abs(CuFull @ P.T).max()
+    assert np.abs(ref["field_mean"] - fm).max() <= 1e-12 * np.abs(fm).max()
