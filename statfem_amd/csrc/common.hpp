@@ -167,6 +167,10 @@ struct sfh_system {
     sfh::DevBuf n2e_ptr, n2e;    // node -> (element<<3 | local node) adjacency, element order
     sfh::DevBuf emap;            // per adjacency slot x nen: position of the element node in the node row
     sfh::DevBuf rowptr_node, colind_node;
+    // two-phase hex8 elasticity: element matrices (36 blocks of 3 x 3 each) and
+    // the per-(node, neighbour) contribution lists, built at the first assembly
+    sfh::DevBuf el_ke, el_cptr, el_cent;
+    bool el_ready = false;
     sfh::DevBuf rowptr, colind, diagpos;
     sfh::DevBuf vals, vals32, F, dinv32, dinv64;
     sfh::DevBuf diag32, diag64;  // A_ii (the PCG keeps z = D^-1 r and needs both)

@@ -257,24 +257,85 @@ __global__ void k_diagpos(int n, const int* rp, const int* ci, int* diag) {
     diag[r] = pos;
 }
 
-// exclusive scan of n ints (single block; setup only)
-__global__ void k_scan(int n, const int* in, int* out) {
-    __shared__ long long part[1024];
-    const int t = threadIdx.x, T = blockDim.x;
-    const long long chunk = ((long long)n + T - 1) / T;
-    const long long b = t * chunk, e = b + chunk < n ? b + chunk : n;
+// exclusive scan of n ints in three launches: per-block sums of kScanTile
+// entries, a scan of the block sums (one work-group), and the block scans
+// with their offsets added; out has n + 1 entries (the total last)
+constexpr int kScanTile = 4096;   // entries per work-group (256 threads x 16)
+__global__ __launch_bounds__(256) void k_scan_sums(int n, const int* __restrict__ in, long long* __restrict__ sums) {
+    __shared__ long long red[256];
+    const long long b0 = (long long)blockIdx.x * kScanTile;
     long long s = 0;
-    for (long long i = b; i < e; ++i) s += in[i];
+    for (int k = threadIdx.x; k < kScanTile; k += 256)
+        if (b0 + k < n) s += in[b0 + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_offsets(int nb, long long* sums) {
+    __shared__ long long part[1024];
+    const int t = threadIdx.x;
+    const int chunk = (nb + 1023) / 1024, b = t * chunk, e = min(nb, b + chunk);
+    long long s = 0;
+    for (int i = b; i < e; ++i) s += sums[i];
     part[t] = s;
     __syncthreads();
     if (t == 0) {
         long long acc = 0;
-        for (int i = 0; i < T; ++i) { const long long v = part[i]; part[i] = acc; acc += v; }
+        for (int i = 0; i < 1024; ++i) { const long long v = part[i]; part[i] = acc; acc += v; }
     }
     __syncthreads();
     long long acc = part[t];
-    for (long long i = b; i < e; ++i) { const int v = in[i]; out[i] = (int)acc; acc += v; }
-    if (t == T - 1) out[n] = (int)acc;
+    for (int i = b; i < e; ++i) { const long long v = sums[i]; sums[i] = acc; acc += v; }
+}
+
+// block scan: each thread scans 16 consecutive entries, the 256 thread totals
+// are scanned in LDS, then written with the block's offset
+__global__ __launch_bounds__(256) void k_scan_apply(int n, const int* __restrict__ in, const long long* __restrict__ offs,
+                                                   int* __restrict__ out) {
+    __shared__ long long part[256];
+    constexpr int kPer = kScanTile / 256;
+    const long long b0 = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kPer;
+    int v[kPer];
+    long long s = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        v[k] = b0 + k < n ? in[b0 + k] : 0;
+        s += v[k];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long acc = offs[blockIdx.x];
+        for (int i = 0; i < 256; ++i) { const long long x = part[i]; part[i] = acc; acc += x; }
+    }
+    __syncthreads();
+    long long acc = part[threadIdx.x];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (b0 + k < n) out[b0 + k] = (int)acc;
+        acc += v[k];
+        if (b0 + k == n - 1) out[n] = (int)acc;
+    }
+}
+
+int scan_exclusive(int n, const int* in, int* out, hipStream_t st) {
+    const int nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 0) return SFH_OK;
+    DevBuf sums;
+    SFH_TRY(sums.ensure(sizeof(long long) * nb));
+    k_scan_sums<<<nb, 256, 0, st>>>(n, in, sums.as<long long>());
+    SFH_LAUNCH_CHECK();
+    k_scan_offsets<<<1, 1024, 0, st>>>(nb, sums.as<long long>());
+    SFH_LAUNCH_CHECK();
+    k_scan_apply<<<nb, 256, 0, st>>>(n, in, sums.as<long long>(), out);
+    SFH_LAUNCH_CHECK();
+    SFH_CHECK_HIP(hipStreamSynchronize(st));   // (sums is freed on return)
+    return SFH_OK;
 }
 
 // -------------------------------------------------------------- assembly
@@ -296,46 +357,71 @@ __global__ __launch_bounds__(kBlock) void k_assemble_scalar(
     const int r0 = rowptr[i], len = rowptr[i + 1] - r0;
     for (int p = 0; p < len; ++p) buf[p * kBlock + tid] = 0.0;
     const int nq = (KIND == SFH_TRI3) ? 1 : (D == 1 ? nq1d : (D == 2 ? nq1d * nq1d : nq1d * nq1d * nq1d));
-    for (int s = n2e_ptr[i]; s < n2e_ptr[i + 1]; ++s) {
-        const int pk = n2e[s];
-        const int e = pk >> 3, a = pk & 7;
-        double X[NEN][D];
-        load_X<KIND>(coords, conn + (size_t)e * NEN, X);
-        const double k1 = p1 ? p1[e] : s1;
-        const double k2 = p2 ? p2[e] : s2;
-        double row[NEN];
+    // kUA adjacent elements at a time: their indices, connectivity, coefficients
+    // and coordinates are all requested before any is used (three dependent
+    // levels of loads per element otherwise serialise element by element)
+    constexpr int kUA = D == 3 ? 2 : 4;
+    const int s0 = n2e_ptr[i], s1e = n2e_ptr[i + 1];
+    for (int sb = s0; sb < s1e; sb += kUA) {
+        int pks[kUA], cns[kUA][NEN];
 #pragma unroll
-        for (int b = 0; b < NEN; ++b) row[b] = 0.0;
-        for (int q = 0; q < nq; ++q) {
-            double xi[3], w, phi[NEN], g[D][NEN];
-            quad_point<KIND>(q, nq1d, xi, w);
-            const double detJ = gderiv<KIND>(X, xi, phi, g);
-            if (!(detJ > 0.0)) atomicOr(err, 4);
-            if (PHYS == SFH_POISSON) {
-                // del2/fem/ElementPotential.ipp:71-72
-                const double sc = k1 * detJ * w;
+        for (int u = 0; u < kUA; ++u) pks[u] = n2e[sb + u < s1e ? sb + u : s0];
 #pragma unroll
-                for (int b = 0; b < NEN; ++b) {
-                    double m = 0.0;
+        for (int u = 0; u < kUA; ++u)
 #pragma unroll
-                    for (int dd = 0; dd < D; ++dd) m += g[dd][a] * g[dd][b];
-                    row[b] += sc * m;
-                }
-            } else {
-                // del2/fem/ElementMatern.ipp:68-79 (scalar diffusivity k2)
+            for (int K = 0; K < NEN; ++K) cns[u][K] = conn[(size_t)(pks[u] >> 3) * NEN + K];
+        double Xs[kUA][NEN][D], k1s[kUA], k2s[kUA];
 #pragma unroll
-                for (int b = 0; b < NEN; ++b) {
-                    double m = 0.0;
+        for (int u = 0; u < kUA; ++u) {
+            const int e = pks[u] >> 3;
+            k1s[u] = p1 ? p1[e] : s1;
+            k2s[u] = p2 ? p2[e] : s2;
 #pragma unroll
-                    for (int dd = 0; dd < D; ++dd) m += (k2 * g[dd][a]) * g[dd][b];
-                    row[b] += detJ * w * m;
-                }
-                row[a] += detJ * w * (k1 * k1 * phi[a]);
-            }
+            for (int K = 0; K < NEN; ++K)
+#pragma unroll
+                for (int dd = 0; dd < D; ++dd) Xs[u][K][dd] = coords[(size_t)cns[u][K] * D + dd];
         }
-        const unsigned char* em = emap + (size_t)s * NEN;
 #pragma unroll
-        for (int b = 0; b < NEN; ++b) buf[em[b] * kBlock + tid] += row[b] * factor;
+        for (int u = 0; u < kUA; ++u) {
+            const int s = sb + u;
+            if (s >= s1e) break;
+            const int a = pks[u] & 7;
+            const double (*X)[D] = Xs[u];
+            const double k1 = k1s[u], k2 = k2s[u];
+            double row[NEN];
+#pragma unroll
+            for (int b = 0; b < NEN; ++b) row[b] = 0.0;
+            for (int q = 0; q < nq; ++q) {
+                double xi[3], w, phi[NEN], g[D][NEN];
+                quad_point<KIND>(q, nq1d, xi, w);
+                const double detJ = gderiv<KIND>(X, xi, phi, g);
+                if (!(detJ > 0.0)) atomicOr(err, 4);
+                if (PHYS == SFH_POISSON) {
+                    // del2/fem/ElementPotential.ipp:71-72
+                    const double sc = k1 * detJ * w;
+#pragma unroll
+                    for (int b = 0; b < NEN; ++b) {
+                        double m = 0.0;
+#pragma unroll
+                        for (int dd = 0; dd < D; ++dd) m += g[dd][a] * g[dd][b];
+                        row[b] += sc * m;
+                    }
+                } else {
+                    // del2/fem/ElementMatern.ipp:68-79 (scalar diffusivity k2)
+#pragma unroll
+                    for (int b = 0; b < NEN; ++b) {
+                        double m = 0.0;
+#pragma unroll
+                        for (int dd = 0; dd < D; ++dd) m += (k2 * g[dd][a]) * g[dd][b];
+                        row[b] += detJ * w * m;
+                    }
+                    row[a] += detJ * w * (k1 * k1 * phi[a]);
+                }
+            }
+            const unsigned char* em = emap + (size_t)s * NEN;
+#pragma unroll
+            for (int b = 0; b < NEN; ++b) buf[em[b] * kBlock + tid] += row[b] * factor;
+        }
     }
     for (int p = 0; p < len; ++p) {
         const double v = buf[p * kBlock + tid];
@@ -404,6 +490,301 @@ __global__ __launch_bounds__(kBlockEl) void k_assemble_elastic(
         const double v = buf[p * kBlockEl + tid];
         vals[r0 + p] = v;
         vals32[r0 + p] = (float)v;
+    }
+}
+
+// ------------------------------------- hex8 elasticity in two atomic-free phases
+// Phase 1 (k_elastic_ke): every element's 24 x 24 stiffness once, the same
+// integrand as k_assemble_elastic. A work-group holds 32 elements, 8 threads
+// each: per batch of 8 Gauss points every thread forms one point's gradients
+// g = J^-T dphi and detJ w in LDS, then every thread accumulates its share of
+// the 36 node pairs (a <= n; the matrix is symmetric) over the batch. The
+// element matrix is stored as 36 blocks of 3 x 3, block (a, n) holding
+// K[(a, c), (n, k)] at c * 3 + k.
+// Phase 2 (k_elastic_gather): one thread per (node i, neighbour slot q) of the
+// node-level pattern sums the blocks of the elements that couple i and its
+// neighbour, in a fixed order (the contribution list of the pair, built once
+// from the element adjacency: elements in mesh order, then local nodes), and
+// writes the 3 x 3 block into the three DOF rows of node i, FP64 and FP32.
+constexpr int kElPerBlk = 32;                  // elements per work-group of phase 1
+constexpr int kKeStride = 36 * 9;              // doubles per element matrix (upper block triangle)
+constexpr int kGStride = 25;                   // g[3][8] and detJ w per Gauss point in LDS
+
+__host__ __device__ inline int hex_pair(int a, int n) { return a * 8 - a * (a - 1) / 2 + (n - a); }   // a <= n
+
+__global__ __launch_bounds__(256) void k_elastic_ke(int nel, int nq1d, const double* __restrict__ coords,
+                                                    const int* __restrict__ conn, const double* __restrict__ p1,
+                                                    const double* __restrict__ p2, double s1, double s2,
+                                                    double factor, double* __restrict__ ke, int* err) {
+    __shared__ double sX[kElPerBlk][8][3];
+    __shared__ double sg[kElPerBlk][8 * kGStride];
+    const int el = threadIdx.x >> 3, t = threadIdx.x & 7;
+    const int e = blockIdx.x * kElPerBlk + el;
+    const bool live = e < nel;
+    if (live) {
+        const int nd = conn[(size_t)e * 8 + t];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) sX[el][t][d] = coords[(size_t)nd * 3 + d];
+    }
+    double E = 0.0, nu = 0.0;
+    if (live) {
+        E = p1 ? p1[e] : s1;
+        nu = p2 ? p2[e] : s2;
+    }
+    const double lambda = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu);   // SVenant.ipp:32-37
+    const double mu = E / 2.0 / (1.0 + nu);
+    // this thread's node pairs p = t + 8 j (j < 5, p < 36)
+    int pa[5], pn[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        int r = t + 8 * j, a = 0;
+        if (r >= 36) r = 35;
+        while (r >= 8 - a) { r -= 8 - a; ++a; }
+        pa[j] = a;
+        pn[j] = a + r;
+    }
+    double B[5][9];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int x = 0; x < 9; ++x) B[j][x] = 0.0;
+    __syncthreads();
+    const int nq = nq1d * nq1d * nq1d;
+    for (int qb = 0; qb < nq; qb += 8) {
+        const int q = qb + t;
+        double* gq = &sg[el][t * kGStride];
+        if (live && q < nq) {
+            double X[8][3];
+#pragma unroll
+            for (int K = 0; K < 8; ++K)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) X[K][d] = sX[el][K][d];
+            double xi[3], w, phi[8], g[3][8];
+            quad_point<SFH_HEX8>(q, nq1d, xi, w);
+            const double detJ = gderiv<SFH_HEX8>(X, xi, phi, g);
+            if (!(detJ > 0.0)) atomicOr(err, 4);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int K = 0; K < 8; ++K) gq[d * 8 + K] = g[d][K];
+            gq[24] = detJ * w;
+        } else {
+            gq[24] = 0.0;
+        }
+        __syncthreads();
+        const int nb = min(8, nq - qb);
+        for (int qq = 0; qq < nb; ++qq) {
+            const double* G = &sg[el][qq * kGStride];
+            const double dw = G[24];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                const double ga[3] = {G[pa[j]], G[8 + pa[j]], G[16 + pa[j]]};
+                const double gn[3] = {G[pn[j]], G[8 + pn[j]], G[16 + pn[j]]};
+                const double dot = ga[0] * gn[0] + ga[1] * gn[1] + ga[2] * gn[2];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        double v = lambda * ga[c] * gn[k] + mu * ga[k] * gn[c];
+                        if (k == c) v += mu * dot;
+                        B[j][c * 3 + k] += v * dw;
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+    double* out = ke + (size_t)e * kKeStride;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const int p = t + 8 * j;
+        if (p < 36)
+#pragma unroll
+            for (int x = 0; x < 9; ++x) out[p * 9 + x] = B[j][x] * factor;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_elastic_gather(int nnodes, const int* __restrict__ rpn,
+                                                        const int* __restrict__ cptr, const int* __restrict__ cent,
+                                                        const double* __restrict__ ke, const int* __restrict__ rowptr,
+                                                        double* __restrict__ vals, float* __restrict__ vals32) {
+    // a half-wave per node, a lane per neighbour slot; consecutive nodes
+    // (which share elements) on one XCD: hardware block b runs on XCD b % 8,
+    // so each XCD takes a contiguous range of nodes
+    const int nb = gridDim.x;
+    const int lb = (nb % 8) ? (int)blockIdx.x : ((int)blockIdx.x % 8) * (nb / 8) + (int)blockIdx.x / 8;
+    const int i = lb * 8 + (threadIdx.x >> 5), lane = threadIdx.x & 31;
+    if (i >= nnodes) return;
+    const int b = rpn[i], len = rpn[i + 1] - b;
+    const int r0 = rowptr[3 * i], rstride = 3 * len;   // rows 3i + c start at r0 + c * 3 len
+    for (int q = lane; q < len; q += 32) {
+        const int t = b + q;
+        double acc[9];
+#pragma unroll
+        for (int x = 0; x < 9; ++x) acc[x] = 0.0;
+        // kUC contributions at a time: their codes, then all their blocks, are
+        // requested before any is added (a block (n, a) with n < a is read and
+        // used transposed: K[(a,c),(n,k)] = K[(n,k),(a,c)])
+        constexpr int kUC = 4;
+        const int c0 = cptr[t], c1 = cptr[t + 1];
+        for (int cb = c0; cb < c1; cb += kUC) {
+            int code[kUC];
+#pragma unroll
+            for (int u = 0; u < kUC; ++u) code[u] = cent[cb + u < c1 ? cb + u : c0];
+            double v[kUC][9];
+            bool tr[kUC];
+#pragma unroll
+            for (int u = 0; u < kUC; ++u) {
+                const int e = code[u] >> 6, a = (code[u] >> 3) & 7, n = code[u] & 7;
+                tr[u] = a > n;
+                const double* Bk = ke + (size_t)e * kKeStride + (tr[u] ? hex_pair(n, a) : hex_pair(a, n)) * 9;
+#pragma unroll
+                for (int x = 0; x < 9; ++x) v[u][x] = Bk[x];
+            }
+#pragma unroll
+            for (int u = 0; u < kUC; ++u) {
+                if (cb + u >= c1) break;
+#pragma unroll
+                for (int x = 0; x < 9; ++x) acc[x] += tr[u] ? v[u][(x % 3) * 3 + x / 3] : v[u][x];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int pos = r0 + c * rstride + 3 * q;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                vals[pos + k] = acc[c * 3 + k];
+                vals32[pos + k] = (float)acc[c * 3 + k];
+            }
+        }
+    }
+}
+
+// The same two phases for the scalar integrands on hex8 meshes (Poisson,
+// Matern; the integrands of k_assemble_scalar): the element matrix is the
+// 36 entries K[a][n], a <= n, and the gather writes one CSR entry per thread.
+constexpr int kGStrideS = 33;                  // g[3][8], phi[8] and detJ w per Gauss point in LDS
+template <int PHYS>
+__global__ __launch_bounds__(256) void k_hex_scalar_ke(int nel, int nq1d, const double* __restrict__ coords,
+                                                       const int* __restrict__ conn, const double* __restrict__ p1,
+                                                       const double* __restrict__ p2, double s1, double s2,
+                                                       double factor, double* __restrict__ ke, int* err) {
+    __shared__ double sX[kElPerBlk][8][3];
+    __shared__ double sg[kElPerBlk][8 * kGStrideS];
+    const int el = threadIdx.x >> 3, t = threadIdx.x & 7;
+    const int e = blockIdx.x * kElPerBlk + el;
+    const bool live = e < nel;
+    if (live) {
+        const int nd = conn[(size_t)e * 8 + t];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) sX[el][t][d] = coords[(size_t)nd * 3 + d];
+    }
+    const double k1 = live ? (p1 ? p1[e] : s1) : 0.0;
+    const double k2 = live ? (p2 ? p2[e] : s2) : 0.0;
+    int pa[5], pn[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        int r = t + 8 * j, a = 0;
+        if (r >= 36) r = 35;
+        while (r >= 8 - a) { r -= 8 - a; ++a; }
+        pa[j] = a;
+        pn[j] = a + r;
+    }
+    double B[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    __syncthreads();
+    const int nq = nq1d * nq1d * nq1d;
+    for (int qb = 0; qb < nq; qb += 8) {
+        const int q = qb + t;
+        double* gq = &sg[el][t * kGStrideS];
+        if (live && q < nq) {
+            double X[8][3];
+#pragma unroll
+            for (int K = 0; K < 8; ++K)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) X[K][d] = sX[el][K][d];
+            double xi[3], w, phi[8], g[3][8];
+            quad_point<SFH_HEX8>(q, nq1d, xi, w);
+            const double detJ = gderiv<SFH_HEX8>(X, xi, phi, g);
+            if (!(detJ > 0.0)) atomicOr(err, 4);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int K = 0; K < 8; ++K) gq[d * 8 + K] = g[d][K];
+#pragma unroll
+            for (int K = 0; K < 8; ++K) gq[24 + K] = phi[K];
+            gq[32] = detJ * w;
+        } else {
+            gq[32] = 0.0;
+        }
+        __syncthreads();
+        const int nb = min(8, nq - qb);
+        for (int qq = 0; qq < nb; ++qq) {
+            const double* G = &sg[el][qq * kGStrideS];
+            const double dw = G[32];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                const double dot = G[pa[j]] * G[pn[j]] + G[8 + pa[j]] * G[8 + pn[j]] + G[16 + pa[j]] * G[16 + pn[j]];
+                if (PHYS == SFH_POISSON) {
+                    B[j] += k1 * dw * dot;                                   // ElementPotential.ipp:71-72
+                } else {
+                    B[j] += dw * (k2 * dot);                                 // ElementMatern.ipp:68-79
+                    if (pa[j] == pn[j]) B[j] += dw * (k1 * k1 * G[24 + pa[j]]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const int p = t + 8 * j;
+        if (p < 36) ke[(size_t)e * 36 + p] = B[j] * factor;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_hex_scalar_gather(int nnz, const int* __restrict__ cptr,
+                                                           const int* __restrict__ cent, const double* __restrict__ ke,
+                                                           double* __restrict__ vals, float* __restrict__ vals32) {
+    const int nb = gridDim.x;
+    const int lb = (nb % 8) ? (int)blockIdx.x : ((int)blockIdx.x % 8) * (nb / 8) + (int)blockIdx.x / 8;
+    const int t = lb * blockDim.x + threadIdx.x;
+    if (t >= nnz) return;
+    double acc = 0.0;
+    constexpr int kUC = 4;   // contributions requested together before any is added
+    const int c0 = cptr[t], c1 = cptr[t + 1];
+    for (int cb = c0; cb < c1; cb += kUC) {
+        double v[kUC];
+#pragma unroll
+        for (int u = 0; u < kUC; ++u) {
+            const int code = cent[cb + u < c1 ? cb + u : c0];
+            const int e = code >> 6, a = (code >> 3) & 7, n = code & 7;
+            v[u] = ke[(size_t)e * 36 + (a <= n ? hex_pair(a, n) : hex_pair(n, a))];
+        }
+#pragma unroll
+        for (int u = 0; u < kUC; ++u)
+            if (cb + u < c1) acc += v[u];
+    }
+    vals[t] = acc;
+    vals32[t] = (float)acc;
+}
+
+// contribution lists of the node-level pattern: for node i's slot q, the
+// (element, local index of i, local index of the neighbour) triples, counted
+// (fill = 0) or written at cptr (fill = 1, cur = zeroed cursors)
+__global__ void k_elastic_contrib(int nnodes, int fill, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
+                                  const unsigned char* __restrict__ emap, const int* __restrict__ rpn,
+                                  const int* __restrict__ cptr, int* cnt, int* cent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    const int b = rpn[i];
+    for (int s = n2e_ptr[i]; s < n2e_ptr[i + 1]; ++s) {
+        const int pk = n2e[s];
+        const int e = pk >> 3, a = pk & 7;
+        for (int n = 0; n < 8; ++n) {
+            const int slot = b + emap[(size_t)s * 8 + n];
+            if (fill) cent[cptr[slot] + cnt[slot]] = (e << 6) | (a << 3) | n;
+            ++cnt[slot];
+        }
     }
 }
 
@@ -504,7 +885,7 @@ int build_pattern(sfh_system* s) {
     const int* conn = s->conn.as<int>();
     k_adj_count<<<grid_for(nadj, kBlock), kBlock, 0, st>>>(s->nel, nen, conn, cnt.as<int>());
     SFH_LAUNCH_CHECK();
-    k_scan<<<1, 1024, 0, st>>>(N, cnt.as<int>(), s->n2e_ptr.as<int>());
+    SFH_TRY(scan_exclusive(N, cnt.as<int>(), s->n2e_ptr.as<int>(), st));
     SFH_LAUNCH_CHECK();
     k_adj_fill<<<grid_for(nadj, kBlock), kBlock, 0, st>>>(s->nel, nen, conn, s->n2e_ptr.as<int>(),
                                                           fill.as<int>(), s->n2e.as<int>());
@@ -516,7 +897,7 @@ int build_pattern(sfh_system* s) {
                                                   s->n2e.as<int>(), cnt.as<int>(), err.as<int>());
     SFH_LAUNCH_CHECK();
     SFH_TRY(s->rowptr_node.ensure(sizeof(int) * (N + 1)));
-    k_scan<<<1, 1024, 0, st>>>(N, cnt.as<int>(), s->rowptr_node.as<int>());
+    SFH_TRY(scan_exclusive(N, cnt.as<int>(), s->rowptr_node.as<int>(), st));
     SFH_LAUNCH_CHECK();
     int nnz_node = 0, herr = 0;
     SFH_CHECK_HIP(hipMemcpyAsync(&nnz_node, s->rowptr_node.as<int>() + N, sizeof(int),
@@ -596,6 +977,31 @@ int build_pattern(sfh_system* s) {
     return SFH_OK;
 }
 
+// contribution lists of the two-phase hex8 assemblies, built once per system
+int elastic_contrib_setup(sfh_system* s, hipStream_t st) {
+    if (s->el_ready) return SFH_OK;
+    const long long nzn = s->nnz_node;
+    const int nn = s->nnodes;
+    DevBuf cnt;
+    SFH_TRY(cnt.ensure(sizeof(int) * (size_t)nzn));
+    SFH_TRY(s->el_cptr.ensure(sizeof(int) * (size_t)(nzn + 1)));
+    SFH_TRY(s->el_cent.ensure(sizeof(int) * (size_t)s->nel * 64));   // 8 nodes x 8 neighbours per element
+    SFH_CHECK_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * (size_t)nzn, st));
+    const unsigned g = (unsigned)((nn + 255) / 256);
+    k_elastic_contrib<<<g, 256, 0, st>>>(nn, 0, s->n2e_ptr.as<int>(), s->n2e.as<int>(), s->emap.as<unsigned char>(),
+                                         s->rowptr_node.as<int>(), nullptr, cnt.as<int>(), nullptr);
+    SFH_LAUNCH_CHECK();
+    SFH_TRY(scan_exclusive((int)nzn, cnt.as<int>(), s->el_cptr.as<int>(), st));
+    SFH_CHECK_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * (size_t)nzn, st));
+    k_elastic_contrib<<<g, 256, 0, st>>>(nn, 1, s->n2e_ptr.as<int>(), s->n2e.as<int>(), s->emap.as<unsigned char>(),
+                                         s->rowptr_node.as<int>(), s->el_cptr.as<int>(), cnt.as<int>(),
+                                         s->el_cent.as<int>());
+    SFH_LAUNCH_CHECK();
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    s->el_ready = true;
+    return SFH_OK;
+}
+
 int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, const double* p2,
                     double s1, double s2, double factor) {
     hipStream_t st = s->ctx->stream;
@@ -611,12 +1017,40 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
             set_error("elasticity needs HEX8 elements with 3 DOFs per node");
             return SFH_EARG;
         }
-        const size_t lds = sizeof(double) * kBlockEl * s->maxrow;
-        if (lds > s->ctx->lds_per_block) { set_error("row too long for LDS (%d)", s->maxrow); return SFH_EARG; }
-        k_assemble_elastic<<<grid_for(s->ndof, kBlockEl), kBlockEl, lds, st>>>(
-            s->ndof, nq1d, X, c, s->n2e_ptr.as<int>(), s->n2e.as<int>(), s->emap.as<unsigned char>(),
-            s->rowptr.as<int>(), p1, p2, s1, s2, factor, s->vals.as<double>(), s->vals32.as<float>(),
-            err.as<int>());
+        if (s->nel < (1 << 25) && 64LL * s->nel < INT32_MAX && s->nnz_node < INT32_MAX) {
+            SFH_TRY(elastic_contrib_setup(s, st));
+            SFH_TRY(s->el_ke.ensure(sizeof(double) * kKeStride * (size_t)s->nel));
+            k_elastic_ke<<<(unsigned)((s->nel + kElPerBlk - 1) / kElPerBlk), 256, 0, st>>>(
+                s->nel, nq1d, X, c, p1, p2, s1, s2, factor, s->el_ke.as<double>(), err.as<int>());
+            SFH_LAUNCH_CHECK();
+            k_elastic_gather<<<(unsigned)(((s->nnodes + 7) / 8 + 7) / 8 * 8), 256, 0, st>>>(
+                s->nnodes, s->rowptr_node.as<int>(), s->el_cptr.as<int>(), s->el_cent.as<int>(),
+                s->el_ke.as<double>(), s->rowptr.as<int>(), s->vals.as<double>(), s->vals32.as<float>());
+            SFH_LAUNCH_CHECK();
+        } else {   // meshes beyond the contribution lists' 32-bit codes: one thread per DOF row
+            const size_t lds = sizeof(double) * kBlockEl * s->maxrow;
+            if (lds > s->ctx->lds_per_block) { set_error("row too long for LDS (%d)", s->maxrow); return SFH_EARG; }
+            k_assemble_elastic<<<grid_for(s->ndof, kBlockEl), kBlockEl, lds, st>>>(
+                s->ndof, nq1d, X, c, s->n2e_ptr.as<int>(), s->n2e.as<int>(), s->emap.as<unsigned char>(),
+                s->rowptr.as<int>(), p1, p2, s1, s2, factor, s->vals.as<double>(), s->vals32.as<float>(),
+                err.as<int>());
+            SFH_LAUNCH_CHECK();
+        }
+    } else if ((physics == SFH_POISSON || physics == SFH_MATERN) && s->kind == SFH_HEX8 && s->dofpn == 1 &&
+               s->nel < (1 << 25) && 64LL * s->nel < INT32_MAX) {
+        SFH_TRY(elastic_contrib_setup(s, st));
+        SFH_TRY(s->el_ke.ensure(sizeof(double) * 36 * (size_t)s->nel));
+        const unsigned ge = (unsigned)((s->nel + kElPerBlk - 1) / kElPerBlk);
+        if (physics == SFH_POISSON)
+            k_hex_scalar_ke<SFH_POISSON><<<ge, 256, 0, st>>>(s->nel, nq1d, X, c, p1, p2, s1, s2, factor,
+                                                              s->el_ke.as<double>(), err.as<int>());
+        else
+            k_hex_scalar_ke<SFH_MATERN><<<ge, 256, 0, st>>>(s->nel, nq1d, X, c, p1, p2, s1, s2, factor,
+                                                             s->el_ke.as<double>(), err.as<int>());
+        SFH_LAUNCH_CHECK();
+        k_hex_scalar_gather<<<(unsigned)(((s->nnz + 255) / 256 + 7) / 8 * 8), 256, 0, st>>>(
+            (int)s->nnz, s->el_cptr.as<int>(), s->el_cent.as<int>(), s->el_ke.as<double>(), s->vals.as<double>(),
+            s->vals32.as<float>());
         SFH_LAUNCH_CHECK();
     } else if (physics == SFH_POISSON || physics == SFH_MATERN) {
         if (s->dofpn != 1) { set_error("scalar integrand needs 1 DOF per node"); return SFH_EARG; }

@@ -225,3 +225,26 @@ def test_c2_full_size_residuals_and_linearity(ctx, orc):
     # (the two sides differ by the solves' errors, each about cond(A) times the
     # residual tolerance at worst)
     assert np.abs(Y.download() - Z).max() < 1e-7 * np.abs(Z).max()
+
+
+@pytest.mark.parametrize("nq", [2, 3])
+def test_hex8_elasticity_two_phase_assembly(ctx, orc, nq):
+    """Element matrices once per element, then the per-(node, neighbour) block
+    gather: a perturbed 6 x 5 x 4 hex mesh with per-element E and nu, against
+    the oracle's Triplet assembly; a re-assembly with other values; an
+    inverted element reports SFH_ENEGJAC."""
+    from statfem_amd import hip
+    m = _perturbed(M.grid3d(6, 5, 4), amp=0.2, seed=4)
+    rng = np.random.default_rng(nq)
+    s = hip.SystemSolveHip(ctx, m, 3)
+    for rep in range(2):
+        E = rng.uniform(0.5, 3.0, m.nel)
+        nu = rng.uniform(0.1, 0.45, m.nel)
+        s.assemble(hip.ELASTICITY, E, nu, nq1d=nq, factor=0.75)
+        _, _, ov, _ = orc.assemble(m, orc.ELASTICITY, E, nu, nq1d=nq, factor=0.75)
+        assert np.abs(s.values() - ov).max() <= 1e-13 * np.abs(ov).max()
+    bad = M.Mesh(m.kind, m.coords, m.conn[:, [1, 0, 3, 2, 5, 4, 7, 6]].copy(), grid=None)
+    sb = hip.SystemSolveHip(ctx, bad, 3)
+    with pytest.raises(hip.SfhError) as e:
+        sb.assemble(hip.ELASTICITY, 1.0, 0.3, nq1d=nq)
+    assert e.value.status == -2
