@@ -177,6 +177,9 @@ typedef struct {
                             trilinear prolongation, Galerkin coarse stencils, l1-Jacobi smoothing, dense
                             coarsest solve) with FP64 recurrences, on declared 2D grids (one or a batch)
                             and declared 3D grids; auto uses 2 wherever it applies, else 1 */
+    int graph;           /* 1 (default): on declared 2D grids with multigrid, replay each pair of iterations
+                            between host convergence checks as a captured HIP graph (captured once per
+                            workspace and launch shape, replayed across solves); 0: launch kernel by kernel */
 } sfh_solve_opts;
 
 typedef struct {
@@ -189,6 +192,7 @@ typedef struct {
                              4 3D block stencil */
     int precond;          /* preconditioner used: 1 Jacobi, 2 multigrid */
     int levels;           /* multigrid levels including the fine grid (1 for Jacobi) */
+    int graph_iters;      /* iterations run as HIP graph replays */
 } sfh_solve_stats;
 
 void sfh_solve_opts_default(sfh_solve_opts* o);

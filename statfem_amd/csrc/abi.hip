@@ -745,6 +745,7 @@ int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const dou
         stats->layout = s1.layout;
         stats->precond = s1.precond;
         stats->levels = s1.levels;
+        stats->graph_iters = s1.graph_iters + s2.graph_iters + s3.graph_iters;
     }
     return SFH_OK;
 }

@@ -64,6 +64,25 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Captured HIP graphs of the multigrid iteration pairs of one workspace (the
+// host checks convergence every two iterations): the first pair of a cycle
+// and the pairs starting at j = 2, 4, 6 mod 6 (the accumulator slots rotate
+// with j mod 2 and j mod 3). `key` lists every launch argument and shape the
+// graphs were captured with; a solve with another key captures them again.
+struct GraphCache {
+    std::vector<unsigned long long> key;
+    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
+    GraphCache() = default;
+    GraphCache(const GraphCache&) = delete;
+    GraphCache& operator=(const GraphCache&) = delete;
+    ~GraphCache() { clear(); }
+    void clear() {
+        for (auto& e : exec)
+            if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
+        key.clear();
+    }
+};
+
 // Multi-RHS mixed-precision PCG workspace (per system, sized for N x kp).
 struct SolveWork {
     int kp = 0;
@@ -75,6 +94,7 @@ struct SolveWork {
     DevBuf tab;                      // row-partition tables of the current solve
     DevBuf mgv;                      // multigrid: restricted residuals and corrections of the coarse levels
     DevBuf cnb;                      // rows with a constrained neighbour (forward solves)
+    GraphCache graphs;               // multigrid iteration pairs as HIP graphs
 };
 
 // Geometric multigrid hierarchies of a batch of 2D tensor grids (one grid for

@@ -1905,6 +1905,72 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     // largest relative FP64 residual at each replacement, and a non-finite one
     std::vector<double> hist;
     int bad_col = -1;
+    // HIP graphs of the multigrid iteration pairs (2D grids): valid while
+    // every launch argument and shape equals the key they were captured with
+    const bool graphs = use_mg && o.graph != 0 && check == 2;
+    int graph_iters = 0;
+    if (graphs) {
+        std::vector<unsigned long long> key = {
+            (unsigned long long)LPR, sizeof(T), (unsigned long long)kp, (unsigned long long)ks, (unsigned long long)ns,
+            (unsigned long long)P.gblk, (unsigned long long)P.ntile, (unsigned long long)P.mgL,
+            (unsigned long long)mg.nseg, (unsigned long long)mg.cmax, (unsigned long long)(uintptr_t)st,
+            (unsigned long long)(uintptr_t)P.d_gtab, (unsigned long long)(uintptr_t)P.d_mgtab,
+            (unsigned long long)(uintptr_t)P.d_mgctab, (unsigned long long)(uintptr_t)coef64,
+            (unsigned long long)(uintptr_t)coef, (unsigned long long)(uintptr_t)cmask,
+            (unsigned long long)(uintptr_t)rmg, (unsigned long long)(uintptr_t)z, (unsigned long long)(uintptr_t)mgv,
+            (unsigned long long)(uintptr_t)acc, (unsigned long long)(uintptr_t)p0, (unsigned long long)(uintptr_t)p1,
+            (unsigned long long)(uintptr_t)S, (unsigned long long)(uintptr_t)pa, (unsigned long long)(uintptr_t)pb,
+            (unsigned long long)(uintptr_t)w.x64.p, (unsigned long long)(uintptr_t)mg.cinv.p};
+        for (int l = 0; l <= P.mgL; ++l) {
+            key.push_back((unsigned long long)P.mg_rn[l]);
+            key.push_back((unsigned long long)P.mg_pn[l]);
+            key.push_back((unsigned long long)P.mg_roff[l]);
+            key.push_back((unsigned long long)P.mg_poff[l]);
+            key.push_back((unsigned long long)P.mgv_off[l]);
+            key.push_back((unsigned long long)mg.N[l]);
+            key.push_back((unsigned long long)(uintptr_t)mg.coef32[l].p);
+            key.push_back((unsigned long long)(uintptr_t)mg.dsm[l].p);
+        }
+        if (key != w.graphs.key) {
+            w.graphs.clear();
+            w.graphs.key = key;
+        }
+    }
+    // the graph of the iteration pair starting at j (captured on first use):
+    // j = 0, then j = 2, 4, 0 mod 6 (j >= 6 repeats those slot patterns)
+    auto pair_graph = [&](int j, hipGraphExec_t* out) -> int {
+        const int g = j == 0 ? 0 : (j % 6 == 2 ? 1 : (j % 6 == 4 ? 2 : 3));
+        const int j0 = g == 0 ? 0 : (g == 1 ? 2 : (g == 2 ? 4 : 6));
+        if (!w.graphs.exec[g]) {
+            SFH_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            T* a = pa;
+            T* b = pb;
+            for (int jj = j0; jj < j0 + 2; ++jj) {
+                Kern<T, LPR>::dir_spmv_mg(P, st, jj, coef64, S, z, a, b, acc);
+                Kern<T, LPR>::update_mg(P, st, jj, coef64, S, b, w.x64.as<double>(), rmg, acc);
+                Kern<T, LPR>::vcycle(P, mg, st, jj, 0, coef, cmask, rmg, z, mgv, acc, p0, p1, S);
+                std::swap(a, b);
+            }
+            hipGraph_t graph = nullptr;
+            const hipError_t e1 = hipGetLastError();
+            const hipError_t e2 = hipStreamEndCapture(st, &graph);
+            if (e1 != hipSuccess || e2 != hipSuccess || !graph) {
+                if (graph) (void)hipGraphDestroy(graph);
+                set_error("graph capture of the multigrid iterations failed: %s",
+                          hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+                return SFH_EDEVICE;
+            }
+            const hipError_t e3 = hipGraphInstantiate(&w.graphs.exec[g], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e3 != hipSuccess) {
+                w.graphs.exec[g] = nullptr;
+                set_error("graph instantiation failed: %s", hipGetErrorString(e3));
+                return SFH_EDEVICE;
+            }
+        }
+        *out = w.graphs.exec[g];
+        return SFH_OK;
+    };
     for (;;) {
         // residual replacement: decide per column, rescale, keep directions
         int nact = 0;
@@ -1961,6 +2027,19 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         // iteration, the step lengths and directions from the rotating slots
         int j = 0;
         for (;; ++j) {
+            // an iteration pair as one graph replay, unless a timing sample is due in it
+            if (graphs && (j & 1) == 0 && iters + 2 <= o.max_inner &&
+                !(ismp < nsmp && (iters % cx->ksample == 0 || (iters + 1) % cx->ksample == 0))) {
+                hipGraphExec_t ex = nullptr;
+                SFH_TRY(pair_graph(j, &ex));
+                SFH_CHECK_HIP(hipGraphLaunch(ex, st));
+                iters += 2;
+                graph_iters += 2;
+                ++j;
+                if (iters >= o.max_inner) break;
+                goto checkpoint;
+            }
+            {
             const bool smp = ismp < nsmp && iters % cx->ksample == 0;
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp], st));
             if (use_g3)
@@ -1994,6 +2073,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             std::swap(pa, pb);
             ++iters;
             if (iters >= o.max_inner) break;
+            }
+        checkpoint:
             if ((j + 1) % check == 0) {
                 // live columns left, and columns waiting for replacement (frozen
                 // now or by the r.r of this iteration, or inactive and unconverged)
@@ -2077,6 +2158,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         stats->layout = use_g3 ? 4 : (P.grid ? 2 : (P.bsr ? 3 : 1));
         stats->precond = fp64_rec ? 2 : 1;
         stats->levels = use_mg ? P.mgL + 1 : (use_g3 ? s->mg3.L + 1 : 1);
+        stats->graph_iters = graph_iters;
     }
     if (bad_col >= 0) {
         // (the reference's SparseLU reports the singular factorisation, corlib/SystemSolveEigenSparse.hpp:303-305)
@@ -2146,7 +2228,7 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
     // column tiles: each tile is an independent multi-RHS solve whose working
     // set can stay in the last-level cache
     if (o.rhs_tile > 0 && o.rhs_tile < k) {
-        sfh_solve_stats acc = {0, 0, 0.0, 0.0, 1, 0, 1, 1};
+        sfh_solve_stats acc = {0, 0, 0.0, 0.0, 1, 0, 1, 1, 0};
         for (int c0 = 0; c0 < k; c0 += o.rhs_tile) {
             const int kc = std::min(o.rhs_tile, k - c0);
             sfh_solve_stats st1;
@@ -2161,6 +2243,7 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
             acc.layout = st1.layout;
             acc.precond = st1.precond;
             acc.levels = st1.levels;
+            acc.graph_iters += st1.graph_iters;
             if (rc != SFH_OK) {
                 if (stats) *stats = acc;
                 return rc;
@@ -2225,4 +2308,5 @@ extern "C" void sfh_solve_opts_default(sfh_solve_opts* o) {
     o->rhs_tile = 0;
     o->layout = 0;
     o->precond = 0;
+    o->graph = 1;
 }

@@ -33,13 +33,13 @@ class SfhError(RuntimeError):
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("inner_tol", C.c_double), ("max_outer", C.c_int),
                 ("max_inner", C.c_int), ("check_every", C.c_int), ("precision", C.c_int),
-                ("rhs_tile", C.c_int), ("layout", C.c_int), ("precond", C.c_int)]
+                ("rhs_tile", C.c_int), ("layout", C.c_int), ("precond", C.c_int), ("graph", C.c_int)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("outer", C.c_int), ("inner_iters", C.c_int), ("max_relres", C.c_double),
                 ("ms", C.c_double), ("converged", C.c_int), ("layout", C.c_int), ("precond", C.c_int),
-                ("levels", C.c_int)]
+                ("levels", C.c_int), ("graph_iters", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -165,3 +165,34 @@ def test_multigrid_on_a_graded_grid(ctx, orc):
     Xr = orc.LU(rp, ci, va).solve(B, trans=True)
     assert st["precond"] == 2 and st["converged"] == 1, st
     assert np.abs(Xd.download() - Xr).max() <= 1e-8 * np.abs(Xr).max(), st
+
+
+def test_graph_replays_match_eager_launches(ctx, orc):
+    """The multigrid iteration pairs replayed as captured HIP graphs: several
+    solves in a row on one workspace (the graphs are captured once and
+    replayed across solves, with eager work and synchronisations between),
+    a re-assembly with new coefficients (same buffers, new values) and a
+    narrower solve (new launch shapes, new graphs), each against the eager
+    launches and the oracle's direct solve."""
+    from statfem_amd import configs, hip
+    cfg = configs.c2_inputs(seed=2, n=48, ns=24)
+    m = cfg["mesh"]
+    s = hip.SystemSolveHip(ctx, m)
+    s.set_grid()
+    rng = np.random.default_rng(5)
+    for rep in range(3):
+        kap = cfg["kappa"] if rep < 2 else np.exp(0.4 * rng.standard_normal(m.nel))
+        s.assemble(hip.POISSON, kap, nq1d=2)
+        s.applyConstraints(cfg["bc"])
+        rp, ci, va, _ = orc.assemble(m, orc.POISSON, kap, nq1d=2, bc_dofs=cfg["bc"])
+        lu = orc.LU(rp, ci, va)
+        for k, trans in ((24, True), (24, False), (7, True)):
+            B = rng.standard_normal((m.nnodes, k))
+            Xg, sg = s.solve(B, trans=trans, opts=hip.default_opts(tol=1e-12, graph=1))
+            Xe, se = s.solve(B, trans=trans, opts=hip.default_opts(tol=1e-12, graph=0))
+            ref = lu.solve(B, trans=trans)
+            assert sg["graph_iters"] > 0 and se["graph_iters"] == 0, (sg, se)
+            assert abs(sg["inner_iters"] - se["inner_iters"]) <= 2
+            xg, xe = Xg.download(), Xe.download()
+            assert np.abs(xg - xe).max() <= 1e-12 * np.abs(xe).max()
+            assert np.abs(xg - ref).max() <= 1e-9 * np.abs(ref).max()
