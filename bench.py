@@ -462,7 +462,7 @@ def c4_run(args):
     update: solves, C_f, Gram); error per problem against an FP64 solve
     (precision 1, tol 1e-12)."""
     from statfem_amd import configs, hip
-    from statfem_amd.autotune import tune_rhs_tile
+    from statfem_amd.autotune import tune_solve
     from statfem_amd.batch import BatchedForwardGauss, SequentialForwardGauss
     probs = configs.c4_inputs(seed=0, nprob=args.c4_nprob)
     ctx = hip.Context(0)
@@ -470,9 +470,9 @@ def c4_run(args):
     t0 = time.perf_counter()
     bf = BatchedForwardGauss(ctx, probs)
     t_setup = time.perf_counter() - t0
-    tile, cost = tune_rhs_tile(bf.sys, bf.Pt, bf.kmax, trans=True, seg=bf.seg, candidates=(16, 32, 64, 128))
-    log(f"C4 autotuner: rhs_tile {tile} (ms per column-iteration {cost})")
-    opts = hip.default_opts(tol=1e-10, rhs_tile=tile)
+    choice, cost = tune_solve(bf.sys, bf.Pt, bf.kmax, trans=True, seg=bf.seg, tiles=(0, 64))
+    log(f"C4 autotuner: {choice} (device ms of the capped probe solve per lanes/rhs_tile: {cost})")
+    opts = hip.default_opts(tol=1e-10, rhs_tile=choice["rhs_tile"], lanes=choice["lanes"])
     bf.updateSolution(opts)                      # warm-up
     ctx.sync()
     t0 = time.perf_counter()
@@ -494,7 +494,7 @@ def c4_run(args):
         f"max rel diff vs the GPU FP64 path {max(err):.2e}")
     return {
         "workload": f"{len(probs)} Poisson Q1 problems, n in [32, 256], {int(dofs_rhs)} DOF-RHS in total",
-        "dofs": int(bf.mesh.nnodes), "rhs_tile": tile, "tile_cost_ms": {str(k): v for k, v in cost.items()},
+        "dofs": int(bf.mesh.nnodes), "launch_shape": choice, "probe_ms": cost,
         "batched_s": t_batch, "sequential_s": t_seq, "speedup": t_seq / t_batch, "setup_s": t_setup,
         "batched_dof_rhs_per_s": dofs_rhs / t_batch, "sequential_dof_rhs_per_s": dofs_rhs / t_seq,
         "batched_iters": st["inner_iters"], "sequential_iters": sit,

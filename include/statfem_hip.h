@@ -65,6 +65,9 @@ const char* sfh_version(void);
 int sfh_ctx_create(int device, sfh_ctx** out);
 int sfh_ctx_destroy(sfh_ctx* ctx);
 int sfh_ctx_sync(sfh_ctx* ctx);
+/* The context's device: ordinal, compute units and architecture name (e.g. "gfx950:sramecc+:xnack-",
+ * truncated to len - 1 characters); keys of persisted launch-shape choices. */
+int sfh_ctx_info(sfh_ctx* ctx, int* device, int* num_cu, char* arch, int len);
 /* Device memory helpers for callers without a GPU array library. */
 int sfh_malloc(sfh_ctx* ctx, size_t bytes, void** dptr);
 int sfh_free(sfh_ctx* ctx, void* dptr);
@@ -180,6 +183,10 @@ typedef struct {
     int graph;           /* 1 (default): on declared 2D grids with multigrid, replay each pair of iterations
                             between host convergence checks as a captured HIP graph (captured once per
                             workspace and launch shape, replayed across solves); 0: launch kernel by kernel */
+    int lanes;           /* lanes per node of the iteration kernels, 4 right-hand sides per lane: 0
+                            (default) automatic (up to 16 on the stencil layouts, 64 on CSR), else 1, 2, 4,
+                            8 or 16 (a power of two at most this). Narrow column tiles let the work-groups
+                            of a batch skip the padding and converged columns of each problem */
 } sfh_solve_opts;
 
 typedef struct {

@@ -1,5 +1,6 @@
 // extern "C" entry points of libstatfem_hip (declared in include/statfem_hip.h).
 // No exception or abort crosses this boundary: every call returns a status.
+#include <cstdio>
 #include <cstring>
 
 #include <string>
@@ -195,6 +196,7 @@ int sfh_ctx_create(int device, sfh_ctx** out) {
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
     c->lds_per_block = prop.sharedMemPerBlock;
+    std::snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
     c->gp_scratch = scr;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
@@ -205,6 +207,14 @@ int sfh_ctx_create(int device, sfh_ctx** out) {
         return SFH_EDEVICE;
     }
     *out = c;
+    return SFH_OK;
+}
+
+int sfh_ctx_info(sfh_ctx* c, int* device, int* num_cu, char* arch, int len) {
+    if (!c) { set_error("null context"); return SFH_EARG; }
+    if (device) *device = c->device;
+    if (num_cu) *num_cu = c->num_cu;
+    if (arch && len > 0) std::snprintf(arch, (size_t)len, "%s", c->arch);
     return SFH_OK;
 }
 

@@ -107,6 +107,30 @@ struct SolveWork {
 // coarsest level.
 constexpr int kMGMaxLev = 16;
 constexpr int kMGCoarse = 160;       // coarsest level: at most this many nodes
+// Coarsening of one grid axis of n nodes: the coarse axis keeps every even
+// fine node and, when n is even, the last fine node as well (n / 2 + 1 coarse
+// nodes, the last coarse interval half as long), so that both ends of the axis
+// are coarse nodes and the interpolation is linear up to the edge. (Dropping
+// the last node and extending the last coarse value as a constant makes the
+// coarse correction weak near that edge; on grids whose levels have even node
+// counts it roughly doubles the PCG iterations.)
+__host__ __device__ inline int mg_coarse_n(int n) { return (n & 1) ? (n + 1) / 2 : n / 2 + 1; }
+// weight of coarse index X in the interpolated value at fine index x (n fine
+// nodes): 1 for the coinciding coarse node (even x, and the last node of an
+// even axis), 1/2 for the two neighbours of an odd interior x
+__host__ __device__ inline float mg_axis_w(int x, int X, int n) {
+    if ((x & 1) == 0) return x == 2 * X ? 1.f : 0.f;
+    if (x == n - 1) return (x + 1) / 2 == X ? 1.f : 0.f;
+    return ((x - 1) / 2 == X || (x + 1) / 2 == X) ? 0.5f : 0.f;
+}
+// coarse parents of fine index x: the first parent p0, their count (1 or 2:
+// p0 and p0 + 1) and the weight of each
+__host__ __device__ inline int mg_axis_parents(int x, int n, int& p0, float& w) {
+    if ((x & 1) == 0) { p0 = x / 2; w = 1.f; return 1; }
+    if (x == n - 1) { p0 = (x + 1) / 2; w = 1.f; return 1; }
+    p0 = (x - 1) / 2; w = 0.5f;
+    return 2;
+}
 // smallest pivot of the coarsest level's factorisation, relative to its row's
 // diagonal, that counts as positive (a singular operator leaves pivots at the
 // rounding level of the diagonal)
@@ -164,6 +188,7 @@ struct sfh_ctx {
     hipEvent_t tev0 = nullptr, tev1 = nullptr;  // public timer
     int num_cu = 0;
     size_t lds_per_block = 0;
+    char arch[64] = {0};
     std::vector<void*> allocs;
     // sampled in-solve timing of the inner-PCG launches (direction + SpMV,
     // SpMV + update, multigrid V-cycle): every ksample-th iteration is bracketed by events

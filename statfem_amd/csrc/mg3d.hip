@@ -71,19 +71,10 @@ __device__ inline int nb27(int k, int x, int y, int z, const G3& g) {
 }
 
 // 1D trilinear weight of coarse index X in fine index x (n fine nodes)
-__host__ __device__ inline float w1(int x, int X, int n) {
-    if ((x & 1) == 0) return x == 2 * X ? 1.f : 0.f;
-    if (x == n - 1) return (x - 1) / 2 == X ? 1.f : 0.f;
-    return ((x - 1) / 2 == X || (x + 1) / 2 == X) ? 0.5f : 0.f;
-}
+__host__ __device__ inline float w1(int x, int X, int n) { return mg_axis_w(x, X, n); }
 
 // coarse parents of fine index x: first parent, count, weight of each
-__host__ __device__ inline int parents1(int x, int n, int& p0, float& w) {
-    if ((x & 1) == 0) { p0 = x / 2; w = 1.f; return 1; }
-    if (x == n - 1) { p0 = (x - 1) / 2; w = 1.f; return 1; }
-    p0 = (x - 1) / 2; w = 0.5f;
-    return 2;
-}
+__host__ __device__ inline int parents1(int x, int n, int& p0, float& w) { return mg_axis_parents(x, n, p0, w); }
 
 // node tile of work-group (logical) b: nodes [b npb, min(n, (b + 1) npb))
 template <int LPR>
@@ -676,7 +667,7 @@ int g3_prepare(sfh_system* s, hipStream_t st) {
     while ((long long)m.W[l] * m.H[l] * m.D[l] * dpn > kMGCoarse && l < kMGMaxLev &&
            (m.W[l] > 2 || m.H[l] > 2 || m.D[l] > 2)) {
         const G3 gf = {m.W[l], m.H[l], m.D[l]};
-        const G3 gc = {(gf.W + 1) / 2, (gf.H + 1) / 2, (gf.D + 1) / 2};
+        const G3 gc = {mg_coarse_n(gf.W), mg_coarse_n(gf.H), mg_coarse_n(gf.D)};
         m.W[l + 1] = gc.W;
         m.H[l + 1] = gc.H;
         m.D[l + 1] = gc.D;

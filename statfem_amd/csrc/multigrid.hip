@@ -5,11 +5,12 @@
 // makes the iteration count independent of the grid size.
 //
 // Levels: level 0 is the W x H node grid of the operator (its 9-coefficient
-// stencil); a batch of grids gets one hierarchy per grid, levels side by side. Level l+1 keeps every other node of level l in each direction,
-// ((W+1)/2) x ((H+1)/2) nodes. Prolongation P is bilinear: a fine node at an
-// even index copies its coarse parent, one at an odd index averages the two
-// neighbouring parents, and an odd last node (even W) copies its single parent
-// (P reproduces constants up to the grid edge). On level 0, P is restricted to
+// stencil); a batch of grids gets one hierarchy per grid, levels side by side.
+// Level l+1 keeps every other node of level l in each direction and the last
+// node of an axis with an even node count (mg_coarse_n: (W+1)/2 nodes for odd
+// W, W/2+1 for even W). Prolongation P is bilinear: a fine node that coincides
+// with a coarse node copies it, one at an odd interior index averages its two
+// neighbouring parents. On level 0, P is restricted to
 // the free (unconstrained) nodes, so the coarse spaces see the operator's free
 // block. Coarse operators are Galerkin products A_{l+1} = P^T A_l P, formed in
 // FP64; for a 9-point stencil and bilinear P they are 9-point again.
@@ -30,11 +31,7 @@ namespace {
 constexpr double kOmega = 4.0 / 3.0;
 
 // weight of coarse index X in fine index x (grid of n fine nodes)
-__host__ __device__ inline double pweight(int x, int X, int n) {
-    if ((x & 1) == 0) return x == 2 * X ? 1.0 : 0.0;
-    if (x == n - 1) return (x - 1) / 2 == X ? 1.0 : 0.0;
-    return ((x - 1) / 2 == X || (x + 1) / 2 == X) ? 0.5 : 0.0;
-}
+__host__ __device__ inline double pweight(int x, int X, int n) { return (double)mg_axis_w(x, X, n); }
 
 // One coarsening step of every grid that has level l + 1, in one launch:
 // work-group row blockIdx.y is grid tab[6 q ..] = {W, H, Wc, Hc, fine base,
@@ -307,8 +304,8 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
     for (int q = 0; q < ng; ++q) {
         int w = grids[3 * q + 1], h = grids[3 * q + 2], l = 0;
         while (l < kMGMaxLev && (l == 0 || (long long)w * h > kMGCoarse) && (w > 2 || h > 2)) {
-            w = (w + 1) / 2;
-            h = (h + 1) / 2;
+            w = mg_coarse_n(w);
+            h = mg_coarse_n(h);
             ++l;
         }
         m.segL[q] = l;
@@ -322,7 +319,7 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
         for (int q = 0; q < ng; ++q) {
             if (l > m.segL[q]) continue;
             int w = grids[3 * q + 1], h = grids[3 * q + 2];
-            for (int u = 0; u < l; ++u) { w = (w + 1) / 2; h = (h + 1) / 2; }
+            for (int u = 0; u < l; ++u) { w = mg_coarse_n(w); h = mg_coarse_n(h); }
             const size_t o = (size_t)l * ng + q;
             m.gW[o] = w;
             m.gH[o] = h;

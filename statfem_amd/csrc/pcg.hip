@@ -422,7 +422,7 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_dir_spmv_mg(const int* __restric
     const Tile t = grid_tile_t<LPR>(kp, gtab, d);
     const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
     if (W == 0) return;
-    if (block_done(t, kp, ks, S)) return;
+    if (block_idle(t, kp, ks, S, it)) return;
     __shared__ float cb[4 * LPR], cm[4 * LPR];
     shared_dir_coef<float, LPR>(it, kp, ks, t.so, S, acc, cb, cm);
     const bool ok = t.col0 < kp;
@@ -510,7 +510,7 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restri
     const Tile t = grid_tile_t<LPR>(kp, gtab, d);
     const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
     if (W == 0) return;
-    if (block_done(t, kp, ks, S)) return;
+    if (block_idle(t, kp, ks, S, it + 1)) return;
     __shared__ double ca[4 * LPR], cx[4 * LPR];
     shared_alpha<double, LPR>(it, kp, ks, t.so, S, acc, ca);
     for (int c = threadIdx.x; c < 4 * LPR; c += blockDim.x) {
@@ -608,11 +608,7 @@ inline size_t mg_prolong_lds(int lpr) {
 
 // weight of coarse index X in fine index x of an n-node line (bilinear P,
 // constant continuation at an odd last node)
-__device__ inline float mg_w(int x, int X, int n) {
-    if ((x & 1) == 0) return x == 2 * X ? 1.f : 0.f;
-    if (x == n - 1) return (x - 1) / 2 == X ? 1.f : 0.f;
-    return ((x - 1) / 2 == X || (x + 1) / 2 == X) ? 0.5f : 0.f;
-}
+__device__ inline float mg_w(int x, int X, int n) { return mg_axis_w(x, X, n); }
 
 
 
@@ -819,11 +815,10 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_prolong(const int* __restrict
             float4 x = zero4();
             if (in[u] && ok && !(TOP && cmask[fbase + gy * W + gx])) {
                 x = make_float4(w[u] * v[u].x, w[u] * v[u].y, w[u] * v[u].z, w[u] * v[u].w);
-                const int ax = (gx & 1) ? (gx - 1) / 2 : gx / 2;   // first coarse parent
-                const int ay = (gy & 1) ? (gy - 1) / 2 : gy / 2;
-                const int nx = ((gx & 1) && gx != W - 1) ? 2 : 1;
-                const int ny = ((gy & 1) && gy != H - 1) ? 2 : 1;
-                const float fx = nx == 2 ? 0.5f : 1.f, fy = ny == 2 ? 0.5f : 1.f;
+                int ax, ay;   // first coarse parents
+                float fx, fy;
+                const int nx = mg_axis_parents(gx, W, ax, fx);
+                const int ny = mg_axis_parents(gy, H, ay, fy);
                 for (int a = 0; a < ny; ++a)
                     for (int b = 0; b < nx; ++b)
                         fma4(fx * fy, sc[((ay + a - cy0) * kCW + (ax + b - cx0)) * LPR + t.lane], x);
@@ -2272,6 +2267,11 @@ int solve(sfh_system* s, int trans, int k, const double* B, int ldb, double* X, 
         max_lpr = 16;
         if (o.precond != 1 && k < 16) min_kp = 16;
     }
+    if (o.lanes > 0) {   // caller's cap on the column tile
+        int l = 1;
+        while (2 * l <= std::min(o.lanes, 64)) l *= 2;
+        max_lpr = std::min(max_lpr, l);
+    }
     const Plan P = make_plan(s->ndof, k, s->ctx->num_cu, nseg, seg_rows, min_kp, max_lpr);
     if (o.precision == 1) return dispatch_solve<double>(s, P, trans, k, B, ldb, X, ldx, o, stats);
     return dispatch_solve<float>(s, P, trans, k, B, ldb, X, ldx, o, stats);
@@ -2309,4 +2309,5 @@ extern "C" void sfh_solve_opts_default(sfh_solve_opts* o) {
     o->layout = 0;
     o->precond = 0;
     o->graph = 1;
+    o->lanes = 0;
 }

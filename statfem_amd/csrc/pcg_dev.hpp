@@ -117,7 +117,7 @@ __device__ inline void dir_coef(int j, int ks, int c, double* S, const double* _
         if (!frozen && (!(acc[(A_RR + (j - 1) % 3) * ks + c] > S[S_TRIG * ks + c]) ||
                         !(rz_at(j, ks, c, S, acc) > 0.0))) {
             frozen = true;
-            S[S_FROZEN * ks + c] = 1.0;
+            S[S_FROZEN * ks + c] = (double)(j + 1);   // the freezing iteration + 1
             S[S_RZPREV * ks + c] = rz_at(j - 1, ks, c, S, acc);
         }
         const double rzo = rz_at(j - 1, ks, c, S, acc);
@@ -196,6 +196,23 @@ __device__ inline bool block_done(const Tile& t, int kp, int ks, const double* _
     if (t.col0 < kp) {
         const double* sc = S + S_SCALE * ks + t.so + t.col0;
         live = sc[0] > 0.0 || sc[1] > 0.0 || sc[2] > 0.0 || sc[3] > 0.0;
+    }
+    return !__syncthreads_or(live);
+}
+
+// The multigrid iteration kernels also skip columns frozen at an iteration
+// before `before` (S_FROZEN holds the freezing iteration + 1; 0: not frozen):
+// the freezing iteration wrote p_new = p_old, so both direction buffers hold
+// the column's kept direction, and its step length is 0, so the iteration
+// changes nothing of it. Columns that freeze in the current launch (their flag
+// is 0 or is being set to it + 1 by sibling work-groups) stay live.
+__device__ inline bool block_idle(const Tile& t, int kp, int ks, const double* __restrict__ S, int before) {
+    bool live = false;
+    if (t.col0 < kp) {
+        const double* sc = S + S_SCALE * ks + t.so + t.col0;
+        const double* fz = S + S_FROZEN * ks + t.so + t.col0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) live = live || (sc[q] > 0.0 && !(fz[q] > 0.0 && fz[q] <= (double)before));
     }
     return !__syncthreads_or(live);
 }

@@ -33,7 +33,8 @@ class SfhError(RuntimeError):
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("inner_tol", C.c_double), ("max_outer", C.c_int),
                 ("max_inner", C.c_int), ("check_every", C.c_int), ("precision", C.c_int),
-                ("rhs_tile", C.c_int), ("layout", C.c_int), ("precond", C.c_int), ("graph", C.c_int)]
+                ("rhs_tile", C.c_int), ("layout", C.c_int), ("precond", C.c_int), ("graph", C.c_int),
+                ("lanes", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -54,6 +55,7 @@ _SIGS = {
     "sfh_ctx_create": (i32, [i32, C.POINTER(v)]),
     "sfh_ctx_destroy": (i32, [v]),
     "sfh_ctx_sync": (i32, [v]),
+    "sfh_ctx_info": (i32, [v, C.POINTER(i32), C.POINTER(i32), C.c_char_p, i32]),
     "sfh_malloc": (i32, [v, sz, C.POINTER(v)]),
     "sfh_free": (i32, [v, v]),
     "sfh_memcpy_h2d": (i32, [v, v, v, sz]),
@@ -200,6 +202,13 @@ class Context:
 
     def sync(self):
         _check(lib().sfh_ctx_sync(self.h))
+
+    def info(self):
+        """{"device", "num_cu", "arch"} of the context's GPU"""
+        d, n = C.c_int(), C.c_int()
+        buf = C.create_string_buffer(64)
+        _check(lib().sfh_ctx_info(self.h, C.byref(d), C.byref(n), buf, 64))
+        return {"device": d.value, "num_cu": n.value, "arch": buf.value.decode()}
 
     def empty(self, shape, dtype=np.float64):
         return DeviceArray(self, shape, dtype)

@@ -83,3 +83,35 @@ def test_vtu_writer_round_trip(tmp_path):
     assert d.get("NumberOfComponents") == "3" and np.allclose(gg.reshape(-1, 3)[:, :2], g)
     _, kk = arr(piece.find("CellData"), "kappa")
     assert np.allclose(kk, k)
+
+
+def test_launch_shape_cache_round_trip(tmp_path, monkeypatch):
+    """The autotuner's persisted choices: keys separate devices, shapes,
+    transposition and batch layouts; a stored choice is read back whole, and
+    a second store keeps the first."""
+    import numpy as np
+    from statfem_amd import autotune
+    path = str(tmp_path / "sub" / "shapes.json")
+    monkeypatch.setenv("STATFEM_AMD_TUNE_CACHE", path)
+    assert autotune.cache_path() == path
+    assert autotune.load_cache() == {}
+    info = {"arch": "gfx950:sramecc+:xnack-", "num_cu": 256}
+    seg = np.array([0, 100, 250], np.int32)
+    k1 = autotune.problem_key(info, 250, 127, True, seg, 1e-10)
+    keys = {k1, autotune.problem_key(info, 250, 127, False, seg, 1e-10),
+            autotune.problem_key(info, 250, 126, True, seg, 1e-10),
+            autotune.problem_key(info, 250, 127, True, np.array([0, 101, 250]), 1e-10),
+            autotune.problem_key(dict(info, num_cu=304), 250, 127, True, seg, 1e-10),
+            autotune.problem_key(info, 250, 127, True, seg, 1e-12)}
+    assert len(keys) == 6
+    assert k1 == autotune.problem_key(info, 250, 127, True, seg.astype(np.int64), 1e-10)
+    autotune.store_choice(k1, {"lanes": 8, "rhs_tile": 0, "ms": 1.5})
+    k2 = autotune.problem_key(info, 90, 4, False, None, 1e-10)
+    autotune.store_choice(k2, {"lanes": 1, "rhs_tile": 0, "ms": 0.2})
+    d = autotune.load_cache()
+    assert d[k1] == {"lanes": 8, "rhs_tile": 0, "ms": 1.5} and d[k2]["lanes"] == 1
+    with open(path, "w") as f:   # a damaged file reads as empty, and is replaced by the next store
+        f.write("{not json")
+    assert autotune.load_cache() == {}
+    autotune.store_choice(k1, {"lanes": 16, "rhs_tile": 64})
+    assert autotune.load_cache() == {k1: {"lanes": 16, "rhs_tile": 64}}

@@ -198,3 +198,68 @@ def test_batched_forward_gauss_full_size_vs_fp64_oracle(ctx, orc):
         ec = np.abs(Cus[q] - ref["Cu"]).max() / np.abs(ref["Cu"]).max()
         print(f"problem {q}: n {p['n']}, {len(p['cols'])} sensors: mean {em:.1e}, C_u {ec:.1e}", flush=True)
         assert em <= 1e-8 and ec <= 1e-8, (q, em, ec)
+
+
+@pytest.mark.parametrize("lanes", [2, 4, 8])
+def test_narrow_column_tiles_match_oracle(ctx, orc, lanes):
+    """A batch whose problems carry different column counts (the rest of each
+    problem's columns zero) solved with 8-, 16- or 32-column tiles (lanes per
+    node capped by sfh_solve_opts.lanes) on the multigrid path, and with
+    padding and converged tiles skipped: every problem against its own
+    direct solve, padding columns exactly zero."""
+    from statfem_amd import hip, mesh as M
+    ms = [M.grid2d(40, 33), M.grid2d(17, 17), M.grid2d(52, 29)]
+    kq = [5, 23, 40]
+    um, off = M.union(ms)
+    seg = np.ascontiguousarray(off, np.int32)
+    s = hip.SystemSolveHip(ctx, um)
+    nx = np.array([len(m.grid[0]) for m in ms], np.int32)
+    ny = np.array([len(m.grid[1]) for m in ms], np.int32)
+    hip._check(hip.lib().sfh_system_set_segment_grids(s.h, 3, hip._p(seg), hip._p(nx), hip._p(ny)))
+    rng = np.random.default_rng(31 + lanes)
+    kap = np.exp(0.5 * rng.standard_normal(um.nel))
+    bcs = [M.dirichlet_all_boundary(m) for m in ms]
+    s.assemble(hip.POISSON, kap, nq1d=2)
+    s.applyConstraints(np.concatenate([b + off[q] for q, b in enumerate(bcs)]).astype(np.int32))
+    k = max(kq)
+    B = rng.standard_normal((um.nnodes, k))
+    for q in range(3):
+        B[off[q]:off[q + 1], kq[q]:] = 0.0
+    X = ctx.empty((um.nnodes, k))
+    st = hip.SolveStats()
+    o = hip.default_opts(tol=1e-12, lanes=lanes)
+    Bd = ctx.to_device(B)
+    hip._check(hip.lib().sfh_solve_batched(s.h, 3, hip._p(seg), 1, k, Bd.ptr, X.ptr, C.byref(o), C.byref(st)))
+    assert st.precond == 2 and st.layout == 2 and st.converged == 1, st.as_dict()
+    Xh = X.download().reshape(um.nnodes, k)
+    e0 = 0
+    for q, m in enumerate(ms):
+        kk = kap[e0:e0 + m.nel]
+        e0 += m.nel
+        rp, ci, va, _ = orc.assemble(m, orc.POISSON, kk, nq1d=2, bc_dofs=bcs[q])
+        ref = orc.LU(rp, ci, va).solve(B[off[q]:off[q + 1]], trans=True)
+        got = Xh[off[q]:off[q + 1]]
+        assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max(), q
+        assert np.all(got[:, kq[q]:] == 0.0)
+
+
+def test_autotuned_launch_shape_is_persisted(ctx, tmp_path, monkeypatch):
+    """tune_solve probes (lanes, RHS tile) candidates once, stores the winner
+    under the device and problem key and returns it from the cache on the
+    next call; the solve with the stored shape converges."""
+    from statfem_amd import autotune, configs, hip
+    from statfem_amd.batch import BatchedForwardGauss
+    monkeypatch.setenv("STATFEM_AMD_TUNE_CACHE", str(tmp_path / "shapes.json"))
+    probs = configs.c4_inputs(seed=3, nprob=6, nmin=20, nmax=60, smin=4, smax=40)
+    bf = BatchedForwardGauss(ctx, probs)
+    c1, cost = autotune.tune_solve(bf.sys, bf.Pt, bf.kmax, trans=True, seg=bf.seg, lanes=(4, 16), tiles=(0, 16),
+                                   probe_iters=8, repeats=1)
+    assert c1["source"] == "probe" and set(cost) == {"4/0", "16/0", "4/16", "16/16"}
+    assert cost[f"{c1['lanes']}/{c1['rhs_tile']}"] == min(cost.values())
+    c2, cost2 = autotune.tune_solve(bf.sys, bf.Pt, bf.kmax, trans=True, seg=bf.seg, lanes=(4, 16), tiles=(0, 16))
+    assert c2["source"] == "cache" and cost2 == {}
+    assert (c2["lanes"], c2["rhs_tile"]) == (c1["lanes"], c1["rhs_tile"])
+    key = autotune.problem_key(ctx.info(), bf.sys.ndof, bf.kmax, True, bf.seg, 1e-10)
+    assert key in autotune.load_cache()
+    means, Cus, st = bf.updateSolution(hip.default_opts(tol=1e-10, lanes=c2["lanes"], rhs_tile=c2["rhs_tile"]))
+    assert st["converged"] == 1 and all(np.isfinite(Cq).all() for Cq in Cus)
