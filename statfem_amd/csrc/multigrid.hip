@@ -16,11 +16,24 @@
 // FP64; for a 9-point stencil and bilinear P they are 9-point again.
 //
 // Smoother: l1-Jacobi (Baker, Falgout, Kolev, Yang, SIAM J. Sci. Comput. 33,
-// 2011) with D_l1 = sum_j |a_ij| and weight omega = 4/3. A <= D_l1 for any
-// symmetric A, so the sweep contracts on every level whatever the
-// coefficients; on a constant-coefficient interior row it is 2/3-damped
-// Jacobi. The coarsest level (at most kMGCoarse nodes) is solved with a dense
-// inverse, computed in FP64 on the device.
+// 2011), S = diag(omega_i / D_l1,i) with D_l1,i = sum_j |a_ij|. A <= D_l1 for
+// any symmetric A, so with omega_i <= 16/9 < 2 the sweep contracts in the
+// A-norm on every level whatever the coefficients, and the symmetric V-cycle
+// is SPD. omega_i comes from a local Fourier analysis of node i's stencil
+// taken as constant: its symbol S(t) = sum_k a_k cos(t . d_k) on a 5 x 5 grid
+// of frequencies t in [0, pi]^2 gives lambda_i = max S / D_l1,i (<= 1), the top
+// of the smoothed spectrum. omega_i = 4 / (3 lambda_i) centres the
+// high-frequency half [lambda_i / 2, lambda_i] on 1; it is used for stencils of
+// the 5-point family (lambda_i > 0.9, weights 4/3 .. 1.48), every other row
+// takes the Q1 interior's 16/9. (Rows near a boundary and on the coarse levels
+// read lambda between 3/4 and 0.9 from the constant-stencil analysis, which
+// does not hold there; on them 16/9 measured fewer iterations than
+// 4 / (3 lambda_i), and no row's weight exceeds 16/9.) On the Q1
+// stencil (lambda = 3/4) that is 16/9, Jacobi damped by 8/9, on the 5-point
+// stencil of P1 triangles (lambda = 1) 4/3: the smoothing factor is 1/3 on
+// both, where one fixed weight gives 1/2 on one of them and costs a fifth more
+// PCG iterations. The coarsest level (at most kMGCoarse nodes) is solved with
+// a dense inverse, computed in FP64 on the device.
 #include <cmath>
 
 #include "common.hpp"
@@ -28,7 +41,8 @@
 namespace sfh {
 namespace {
 
-constexpr double kOmega = 4.0 / 3.0;
+constexpr double kOmegaMax = 16.0 / 9.0;
+constexpr double kLambda5pt = 0.9;   // smallest lambda_i of the 5-point family
 
 // weight of coarse index X in fine index x (grid of n fine nodes)
 __host__ __device__ inline double pweight(int x, int X, int n) { return (double)mg_axis_w(x, X, n); }
@@ -118,17 +132,34 @@ __global__ __launch_bounds__(256) void k_mg_galerkin9(const int* __restrict__ ta
     if (live) Ac[((size_t)cb + I) * 9 + K] = centre > 0.0 ? v : (K == 4 ? 1.0 : 0.0);
 }
 
-// smoother weights omega / sum_j |a_ij| and (levels >= 1) the FP32 stencil
+// smoother weights omega_i / sum_j |a_ij| (omega_i from the stencil's symbol,
+// frequencies m pi / 4, m = 0..4 per axis) and (levels >= 1) the FP32 stencil
 __global__ void k_mg_level(int n, const double* __restrict__ A, float* __restrict__ dsm, float* __restrict__ A32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double s = 0.0;
+    double a[9], s = 0.0;
     for (int k = 0; k < 9; ++k) {
-        const double a = A[(size_t)i * 9 + k];
-        s += fabs(a);
-        if (A32) A32[(size_t)i * 9 + k] = (float)a;
+        a[k] = A[(size_t)i * 9 + k];
+        s += fabs(a[k]);
+        if (A32) A32[(size_t)i * 9 + k] = (float)a[k];
     }
-    dsm[i] = s > 0.0 ? (float)(kOmega / s) : 0.f;
+    if (!(s > 0.0)) {
+        dsm[i] = 0.f;
+        return;
+    }
+    // cos(m pi / 4) for m = -8..8
+    const double h = 0.70710678118654752440;
+    const double cs[17] = {1, h, 0, -h, -1, -h, 0, h, 1, h, 0, -h, -1, -h, 0, h, 1};
+    double smax = 0.0;
+    for (int ty = 0; ty <= 4; ++ty)
+        for (int tx = 0; tx <= 4; ++tx) {
+            double v = 0.0;
+            for (int k = 0; k < 9; ++k) v += a[k] * cs[8 + (k % 3 - 1) * tx + (k / 3 - 1) * ty];
+            smax = fmax(smax, v);
+        }
+    const double lam = smax / s;   // <= 1
+    const double om = lam > kLambda5pt ? 4.0 / (3.0 * lam) : kOmegaMax;
+    dsm[i] = (float)(om / s);
 }
 
 // Dense inverse of each grid's coarsest level, one work-group per grid (grid
