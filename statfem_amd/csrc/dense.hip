@@ -529,6 +529,271 @@ __global__ __launch_bounds__(1024) void k_loglik_reg(int ns, int no, int dim, co
     }
 }
 
+// ------------------------------------- MFMA blocked Cholesky log-likelihood
+// The log-likelihood of k_loglik_reg by a blocked right-looking Cholesky of
+// the same augmented matrix on 16 x 16 tiles, with v_mfma_f64_16x16x4f64 for
+// the panel and trailing updates. Rows and columns, in whole tiles:
+//   0 .. ns-1        K_s + rho^2 C_u + eps^2 I (the observation covariance)
+//   ns .. nsp-1      identity (pads the sensor block to whole tiles: pivots 1,
+//                    no coupling, so neither the factor nor log det changes)
+//   nsp .. nsp+no-1  data rows (y_m - rho mean)^T, zero in the data block
+// One 1024-thread work-group (16 waves) per hyperparameter set. Wave 0
+// factors the diagonal tiles; waves 1..15 own the lower triangle's tiles
+// (tile t = ib (ib + 1) / 2 + jb to wave 1 + t % 15, slot t / 15), held as
+// MFMA accumulators, register r of lane l holding tile entry
+// ((l >> 4) + 4 r, l & 15). Pivot step kb of nsp / 16:
+//   (1) the owners of tile column kb store its tiles in LDS in the MFMA operand
+//       layout Q[kk][l] = X[l & 15][4 kk + (l >> 4)], which serves both the
+//       A operand of X and the B operand of X^T;
+//   (2) wave 0 factors the diagonal tile in registers (lane r holds row r; each
+//       step's pivot and column are read with readlane) and forms L^-1 by rows
+//       the same way, stored in the operand layout;
+//   (3) the owners turn the tiles below the diagonal into
+//       L_ib,kb = A_ib,kb L_kk^-T (four MFMAs each) and store them back;
+//   (4) the owners update their tiles right of column kb,
+//       A_ib,jb -= L_ib,kb L_jb,kb^T (four MFMAs each, operands from LDS).
+// Look-ahead: in step (4) the owners first update and store tile column
+// kb + 1, then wave 0 factors its diagonal tile while they finish the rest of
+// (4). Three barriers per step, passed by both kinds of wave; consecutive tile
+// columns use alternate LDS buffers. The factoring wave holds no tiles, so its
+// unrolled registers do not add to the owners'. The data block ends as the Schur
+// complement -D L^-T L^-1 D^T: its diagonal holds -|L^-1 d_m|^2, as in the
+// register kernel.
+constexpr int kLLT = 14;                                // tiles per side at most (224 rows)
+constexpr int kLLMfmaMin = 48;                          // sensors from which the blocked kernel is used
+constexpr int kLLOwners = 15;                           // tile-owning waves
+constexpr int kLLS = (kLLT * (kLLT + 1) / 2 + kLLOwners - 1) / kLLOwners;   // accumulator tiles per owner
+
+__device__ inline double readlane_f64(double x, int src) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffull), src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// sqrt(d) and 1 / sqrt(d) for d > 0: the hardware reciprocal square root and
+// two Newton steps y += y (1 - d y^2) / 2 (each squares the relative error),
+// then one correction of the square root, l += y (d - l^2) / 2
+__device__ inline void sqrt_rsqrt(double d, double& l, double& y) {
+    y = __builtin_amdgcn_rsq(d);
+    y = fma(0.5 * y, fma(-d * y, y, 1.0), y);
+    y = fma(0.5 * y, fma(-d * y, y, 1.0), y);
+    l = d * y;
+    l = fma(0.5 * y, fma(-l, l, d), l);
+}
+
+// lower-triangle tile t -> (ib, jb), t = ib (ib + 1) / 2 + jb
+__device__ inline void tri_tile(int t, int& ib, int& jb) {
+    int i = 0;
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    ib = i;
+    jb = t - i * (i + 1) / 2;
+}
+
+// accumulator tile (MFMA result layout) -> operand layout in LDS
+__device__ inline void store_operand(double* q, const v4d& v, int lane) {
+    const int col = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q[(col >> 2) * 64 + (col & 3) * 16 + (lane >> 4) + 4 * r] = v[r];
+}
+
+// A -= L_i L_j^T for operand tiles Li, Lj in LDS
+__device__ inline void tile_update(v4d& acc, const double* __restrict__ Li, const double* __restrict__ Lj, int lane) {
+    double x[4], y[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        x[kk] = Li[kk * 64 + lane];
+        y[kk] = Lj[kk * 64 + lane];
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-x[kk], y[kk], acc, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(1024) void k_loglik_mfma(int ns, int no, int dim, const double* __restrict__ xs,
+                                                     const double* __restrict__ Cu, const double* __restrict__ mean0,
+                                                     int mstride, const double* __restrict__ Y, double eps0,
+                                                     const double* __restrict__ eps_v,
+                                                     const double* __restrict__ theta, double* __restrict__ out,
+                                                     int* __restrict__ status, double* __restrict__ logdet_out) {
+    __shared__ double Pb[2][kLLT][256];
+    __shared__ double Li[256];
+    __shared__ double qd[kLLT * 16];
+    __shared__ double pivprod[kLLT];
+    __shared__ int sh_bad;
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nsp = (ns + 15) & ~15, ts = nsp >> 4;
+    const int T = ts + ((no + 15) >> 4);
+    if (threadIdx.x == 0) sh_bad = 0;
+    if (wave == 0) {
+        // ---- the factoring wave: diagonal tile kb while the owners finish the
+        // trailing update of step kb - 1
+        const int r = lane & 15;
+        __syncthreads();   // (A) column 0 is in LDS
+        for (int kb = 0; kb < ts; ++kb) {
+            const double* D = &Pb[kb & 1][kb][0];
+            double a[16], x[16], iv[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                a[c] = D[(c >> 2) * 64 + (c & 3) * 16 + r];
+                x[c] = c == r ? 1.0 : 0.0;
+            }
+            bool bad = false;
+            double pr = 1.0;   // product of the pivots (their logs are summed at the end)
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                // factor step c: pivot, column c of L (zero above the diagonal), trailing columns
+                const double d = readlane_f64(a[c], c);
+                bad = bad || !(d > 0.0);
+                double inv, l;
+                sqrt_rsqrt(d, l, inv);
+                pr *= l;
+                iv[c] = inv;
+                a[c] = r > c ? a[c] * inv : (r == c ? l : 0.0);
+#pragma unroll
+                for (int c2 = c + 1; c2 < 16; ++c2) a[c2] -= a[c] * readlane_f64(a[c], c2);
+            }
+            // rows of X = L^-1: row k is final once divided by L[k][k], then it is
+            // eliminated from the rows below (a[k] = L[r][k])
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                if (r == k) {
+#pragma unroll
+                    for (int j = 0; j <= k; ++j) x[j] *= iv[k];
+                }
+#pragma unroll
+                for (int j = 0; j <= k; ++j) {
+                    const double xk = readlane_f64(x[j], k);
+                    if (r > k) x[j] -= a[k] * xk;
+                }
+            }
+            if (lane < 16) {   // L^-1 (r, j) in the operand layout
+#pragma unroll
+                for (int j = 0; j < 16; ++j) Li[(j >> 2) * 64 + (j & 3) * 16 + r] = x[j];
+            }
+            if (lane == 0) {
+                pivprod[kb] = pr;
+                if (bad) sh_bad = 1;
+            }
+            __syncthreads();   // (B) L^-1 is in LDS
+            if (sh_bad) {
+                if (threadIdx.x == 0) {
+                    out[b] = -INFINITY;
+                    status[b] = SFH_ENOTSPD;
+                    if (logdet_out) logdet_out[b] = NAN;
+                }
+                return;
+            }
+            __syncthreads();   // (C) the panel is in LDS
+            __syncthreads();   // (A) column kb + 1 is in LDS
+        }
+        __syncthreads();   // (D) the data diagonal is in LDS
+        if (threadIdx.x == 0) {
+            double slog = 0.0;
+            for (int k = 0; k < ts; ++k) slog += log(pivprod[k]);
+            double tmp3 = 0.0;
+            int badq = 0;
+            for (int m = 0; m < no; ++m) {
+                const double c = 0.5 * qd[m];
+                if (!(c < 0.0)) badq = 1;
+                tmp3 += c;
+            }
+            const double tmp1 = -0.5 * ns * log(2. * M_PI);
+            out[b] = no * (tmp1 - slog) + tmp3;
+            status[b] = badq ? SFH_EQUADFORM : SFH_OK;
+            if (logdet_out) logdet_out[b] = 2.0 * slog;
+        }
+        return;
+    }
+    // ---- the tile-owning waves
+    const int ow = wave - 1;
+    const int ntile = T * (T + 1) / 2;
+    const double rho = theta[3 * b], sd = theta[3 * b + 1], ld = theta[3 * b + 2];
+    const double* mean = mean0 + (size_t)b * mstride;
+    const double eps = eps_v ? eps_v[b] : eps0;
+    int tib[kLLS], tjb[kLLS];
+    v4d acc[kLLS];
+#pragma unroll
+    for (int s = 0; s < kLLS; ++s) {
+        const int t = ow + kLLOwners * s;
+        int ib = -1, jb = -1;
+        if (t < ntile) tri_tile(t, ib, jb);
+        tib[s] = ib;
+        tjb[s] = jb;
+        v4d v = {0.0, 0.0, 0.0, 0.0};
+        if (ib >= 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * ib + (lane >> 4) + 4 * r, j = 16 * jb + (lane & 15);
+                double e = 0.0;
+                if (i < ns && j < ns) {
+                    e = se_kernel(dim, xs + (size_t)i * dim, xs + (size_t)j * dim, sd, ld) + rho * rho * Cu[(size_t)i * ns + j];
+                    if (i == j) e += eps * eps;
+                } else if (i < nsp) {
+                    e = i == j ? 1.0 : 0.0;
+                } else if (j < ns && i - nsp < no) {
+                    e = Y[(size_t)(i - nsp) * ns + j] - rho * mean[j];
+                }
+                v[r] = e;
+            }
+        }
+        acc[s] = v;
+    }
+    // (1) tile column 0 into LDS
+#pragma unroll
+    for (int s = 0; s < kLLS; ++s)
+        if (tjb[s] == 0) store_operand(&Pb[0][0][0] + 256 * tib[s], acc[s], lane);
+    __syncthreads();   // (A)
+    for (int kb = 0; kb < ts; ++kb) {
+        double* P = &Pb[kb & 1][0][0];
+        __syncthreads();   // (B) wave 0 has stored L_kk^-1
+        if (sh_bad) return;
+        // (3) panel: L_ib,kb = A_ib,kb L_kk^-T
+#pragma unroll
+        for (int s = 0; s < kLLS; ++s)
+            if (tjb[s] == kb && tib[s] > kb) {
+                double* X = P + 256 * tib[s];
+                double xa[4], lb[4];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    xa[kk] = X[kk * 64 + lane];
+                    lb[kk] = Li[kk * 64 + lane];
+                }
+                v4d z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) z = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[kk], lb[kk], z, 0, 0, 0);
+                store_operand(X, z, lane);
+            }
+        __syncthreads();   // (C)
+        // (4a) tile column kb + 1 first, stored for the next diagonal step (after
+        // the last pivot step: the first data column, stored but unused)
+        double* Pn = &Pb[(kb + 1) & 1][0][0];
+#pragma unroll
+        for (int s = 0; s < kLLS; ++s)
+            if (tjb[s] == kb + 1) {
+                tile_update(acc[s], P + 256 * tib[s], P + 256 * tjb[s], lane);
+                store_operand(Pn + 256 * tib[s], acc[s], lane);
+            }
+        __syncthreads();   // (A)
+        // (4b) the rest of the trailing update, beside the next factorisation
+#pragma unroll
+        for (int s = 0; s < kLLS; ++s)
+            if (tjb[s] > kb + 1) tile_update(acc[s], P + 256 * tib[s], P + 256 * tjb[s], lane);
+    }
+    // data rows: diagonal of the Schur complement
+#pragma unroll
+    for (int s = 0; s < kLLS; ++s)
+        if (tib[s] >= ts && tib[s] == tjb[s]) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = (lane >> 4) + 4 * r;
+                const int m = 16 * tib[s] + row - nsp;
+                if (row == (lane & 15) && m < no) qd[m] = acc[s][r];
+            }
+        }
+    __syncthreads();   // (D)
+}
+
 // in-place SPD solve A X = B (A n x n lower used, B n x m), one work-group
 __global__ __launch_bounds__(256) void k_spd_solve(int n, double* A, int m, double* B, int* status) {
     extern __shared__ double col[];
@@ -903,6 +1168,14 @@ int loglik_sets(sfh_ctx* c, int ns, int no, int dim, const double* xs, const dou
     hipStream_t st = c->stream;
     const int na = ns + no;
     if (M <= 0) return SFH_OK;
+    const int tiles = (ns + 15) / 16 + (no + 15) / 16;
+    if (ns >= kLLMfmaMin && tiles <= kLLT) {
+        // blocked MFMA factorisation (sensor block padded to whole 16 x 16 tiles)
+        k_loglik_mfma<<<M, 1024, 0, st>>>(ns, no, dim, xs, Cu, mean, mstride, Y, eps, eps_v, theta, out, status,
+                                        logdet);
+        SFH_LAUNCH_CHECK();
+        return SFH_OK;
+    }
     if (na <= 224) {
         // register-resident factorisation (sizes up to 224 augmented rows)
 #define SFH_LLREG(NB)                                                                                           \

@@ -244,11 +244,15 @@ def test_fractional_precision_operator_matches_oracle(ctx, orc, dim, nu):
     assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("ns,no", [(20, 3), (100, 4), (150, 9), (215, 9), (240, 6)])
+@pytest.mark.parametrize("ns,no", [(20, 3), (48, 16), (64, 30), (100, 4), (150, 9), (177, 31), (200, 8), (215, 9),
+                                   (240, 6)])
 def test_loglik_sizes_match_oracle(ctx, orc, ns, no):
-    """Every factorisation path of the batched log-likelihood (register-resident
-    up to 224 augmented rows, work-matrix form beyond) against the oracle,
-    with its log-determinant, over several hyperparameter sets."""
+    """Every factorisation path of the batched log-likelihood (the blocked MFMA
+    Cholesky from 48 sensors while the padded sensor tiles and the data tiles
+    fit 14 tiles of 16, the register-resident one up to 224 augmented rows
+    otherwise, the work-matrix form beyond) against the oracle, with its
+    log-determinant, over several hyperparameter sets; sensor counts off and
+    on the tile size, one and two data tiles."""
     from statfem_amd import hip
     rng = np.random.default_rng(ns)
     xs = rng.uniform(size=(ns, 2))
