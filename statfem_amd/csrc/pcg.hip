@@ -1729,8 +1729,10 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     // a declared 2D grid: the system's grid for one problem, the declared
     // segment grids for a batch
     const std::vector<int> grids = grids_of(s, ns, P.srow);
-    // one lane per node only for a batch that solves with multigrid (a
-    // problem per column, as the conductivity chains)
+    // one lane per node (4-column tiles) only when a grid has a multigrid
+    // hierarchy and the preconditioner is not Jacobi: narrow solves on one
+    // grid (the one-column mean solve, the single-column SPDE solves) and
+    // batches with a problem per column (the conductivity chains)
     bool mg_grid = false;
     for (size_t q = 0; q < grids.size(); q += 3) mg_grid = mg_grid || (long long)grids[q + 1] * grids[q + 2] > kMGCoarse;
     const bool use_grid = !grids.empty() && (o.layout == 2 || o.layout == 0) && std::is_same<T, float>::value &&

@@ -67,11 +67,13 @@ def traffic(fetch_csv, write_csv, copy_bytes=1 << 30, match=""):
     return out
 
 
-def mfma(counter_csv, trace_csv, match="", simds=1024, peak_tflops=78.6):
+def mfma(counter_csv, trace_csv, match="", simds=1024, xcds=8, peak_tflops=78.6):
     """Per kernel (name, grid): FP64 MFMA flops (SQ_INSTS_VALU_MFMA_MOPS_F64 x
     512) per dispatch, their rate over the dispatch's duration from the kernel
     trace of the same run, and the MFMA busy fraction
-    SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x SIMDs)."""
+    SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / XCDs x SIMDs): the busy
+    cycles are summed over every SIMD, GRBM_GUI_ACTIVE over the 8 XCDs, each
+    of which counts the same elapsed cycles."""
     dur = {}
     for r in csv.DictReader(open(trace_csv)):
         dur[r["Dispatch_Id"]] = float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
@@ -86,7 +88,7 @@ def mfma(counter_csv, trace_csv, match="", simds=1024, peak_tflops=78.6):
         flops = c.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0) * 512
         ns = dur.get(d)
         busy, act = c.get("SQ_VALU_MFMA_BUSY_CYCLES"), c.get("GRBM_GUI_ACTIVE")
-        agg[(k, g)].append((flops, ns, busy / (act * simds) if busy is not None and act else None))
+        agg[(k, g)].append((flops, ns, busy / (act / xcds * simds) if busy is not None and act else None))
     out = {}
     for (k, g), v in agg.items():
         fl = sum(x[0] for x in v) / len(v)
