@@ -1,5 +1,6 @@
 // extern "C" entry points of libstatfem_hip (declared in include/statfem_hip.h).
 // No exception or abort crosses this boundary: every call returns a status.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 
@@ -36,6 +37,27 @@ int DevBuf::ensure(size_t b) {
 
 void DevBuf::release() {
     if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int PinnedBuf::ensure(size_t b) {
+    if (b <= bytes && p) return SFH_OK;
+    release();
+    if (b == 0) b = 8;
+    if (hipHostMalloc(&p, b, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        bytes = 0;
+        set_error("page-locked host allocation of %zu bytes failed", b);
+        return SFH_ENOMEM;
+    }
+    bytes = b;
+    return SFH_OK;
+}
+
+void PinnedBuf::release() {
+    if (p) (void)hipHostFree(p);
     p = nullptr;
     bytes = 0;
 }
@@ -727,25 +749,36 @@ int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const dou
     }
     SFH_TRY(bg.join());
     SFH_CHECK_HIP(hipEventRecord(e1, st));
-    std::vector<double> hCu((size_t)ns * ns), hm(ns);
-    SFH_CHECK_HIP(hipMemcpyAsync(hCu.data(), dCu, hCu.size() * 8, hipMemcpyDeviceToHost, st));
-    SFH_CHECK_HIP(hipMemcpyAsync(hm.data(), dmean, ns * 8, hipMemcpyDeviceToHost, st));
+    // C_u and the mean are adjacent on the device: one copy into page-locked memory
+    SFH_TRY(s->fg_host.ensure(((size_t)ns * ns + ns) * 8));
+    SFH_CHECK_HIP(hipMemcpyAsync(s->fg_host.p, dCu, ((size_t)ns * ns + ns) * 8, hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
+    double* hCu = s->fg_host.as<double>();
+    const double* hm = hCu + (size_t)ns * ns;
     float ms = 0.f;
     SFH_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // + (sigma_f reg_eps)^2 P P^T (ForwardGaussRHS.hpp:131-136), from the sparse rows
-    for (int a = 0; a < ns; ++a)
-        for (int b = 0; b < ns; ++b) {
-            double acc = 0.0;
-            for (int qa = 0; qa < pn; ++qa)
-                for (int qb = 0; qb < pn; ++qb)
-                    if (pcols[a * pn + qa] == pcols[b * pn + qb]) acc += pvals[a * pn + qa] * pvals[b * pn + qb];
-            hCu[(size_t)a * ns + b] += reg * reg * acc;
+    // + (sigma_f reg_eps)^2 P P^T (ForwardGaussRHS.hpp:131-136), from the sparse
+    // rows: the entries of every DOF that several sensor rows share, grouped by
+    // sorting the (DOF, sensor) pairs
+    {
+        std::vector<std::pair<int, int>> e((size_t)ns * pn);
+        for (int a = 0; a < ns * pn; ++a) e[a] = {pcols[a], a};
+        std::sort(e.begin(), e.end());
+        const double r2 = reg * reg;
+        for (size_t i = 0; i < e.size();) {
+            size_t j = i;
+            while (j < e.size() && e[j].first == e[i].first) ++j;
+            for (size_t u = i; u < j; ++u)
+                for (size_t v = i; v < j; ++v)
+                    hCu[(size_t)(e[u].second / pn) * ns + e[v].second / pn] +=
+                        r2 * pvals[e[u].second] * pvals[e[v].second];
+            i = j;
         }
-    if (Cu) std::memcpy(Cu, hCu.data(), hCu.size() * 8);
-    if (mean) std::memcpy(mean, hm.data(), ns * 8);
+    }
+    if (Cu) std::memcpy(Cu, hCu, (size_t)ns * ns * 8);
+    if (mean) std::memcpy(mean, hm, (size_t)ns * 8);
     if (stats) {
         stats->outer = std::max(s1.outer, std::max(s2.outer, s3.outer));
         stats->inner_iters = s1.inner_iters + s2.inner_iters + s3.inner_iters;

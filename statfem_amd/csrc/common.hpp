@@ -64,6 +64,22 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Page-locked host buffer (hipHostMalloc), grown on demand: device-to-host
+// copies into it are plain DMA transfers, where copies into pageable memory are
+// staged by the runtime and cost tens of microseconds more each.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { release(); }
+    int ensure(size_t b);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
 // Captured HIP graphs of the multigrid iteration pairs of one workspace (the
 // host checks convergence every two iterations): the first pair of a cycle
 // and the pairs starting at j = 2, 4, 6 mod 6 (the accumulator slots rotate
@@ -95,6 +111,8 @@ struct SolveWork {
     DevBuf mgv;                      // multigrid: restricted residuals and corrections of the coarse levels
     DevBuf cnb;                      // rows with a constrained neighbour (forward solves)
     GraphCache graphs;               // multigrid iteration pairs as HIP graphs
+    PinnedBuf host;                  // the per-column state read back at each host check
+    std::vector<int> tab_host;       // the partition tables last uploaded to `tab`
 };
 
 // Geometric multigrid hierarchies of a batch of 2D tensor grids (one grid for
@@ -103,8 +121,8 @@ struct SolveWork {
 // level l. The levels of all grids are stored side by side: grid q's level-l
 // nodes start at node gbase[l * nseg + q] of level l's arrays (level 0: the
 // grid's first row of the operator). Per level the FP32 stencil (levels >= 1)
-// and the l1-Jacobi smoother weights; per grid the dense FP32 inverse of its
-// coarsest level.
+// and the smoother weights; per grid the dense FP32 inverse of its coarsest
+// level.
 constexpr int kMGMaxLev = 16;
 constexpr int kMGCoarse = 160;       // coarsest level: at most this many nodes
 // Coarsening of one grid axis of n nodes: the coarse axis keeps every even
@@ -246,6 +264,7 @@ struct sfh_system {
     sfh::SolveWork work2;        // workspace of a concurrent second solve
     sfh::DevBuf scratch0, scratch1, scratch2, scratch3;
     sfh::DevBuf fg_pt, fg_rt, fg_v, fg_cv, fg_cu, fg_gram, fg_cols, fg_vals;
+    sfh::PinnedBuf fg_host;          // C_u and the mean on their way to the caller
     // forcing-covariance weights given by the caller (sfh_system_set_covvec)
     sfh::DevBuf fg_user_cv;
     bool fg_user = false;

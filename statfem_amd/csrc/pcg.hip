@@ -1796,8 +1796,13 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         h.insert(h.end(), P.mgsb.begin(), P.mgsb.end());
         h.insert(h.end(), P3.ctab, P3.ctab + 4);
         h.insert(h.end(), sb3, sb3 + 2);
-        SFH_CHECK_HIP(hipMemcpyAsync(w.tab.p, h.data(), sizeof(int) * ntab, hipMemcpyHostToDevice, st));
-        SFH_CHECK_HIP(hipStreamSynchronize(st));
+        // a solve of the same shape as the last one on this workspace finds
+        // its tables in place (no upload, no wait for the queued work)
+        if (h != w.tab_host || w.tab.bytes < sizeof(int) * ntab) {
+            SFH_CHECK_HIP(hipMemcpyAsync(w.tab.p, h.data(), sizeof(int) * ntab, hipMemcpyHostToDevice, st));
+            SFH_CHECK_HIP(hipStreamSynchronize(st));
+            w.tab_host.swap(h);
+        }
     }
     const int* d_brow = w.tab.as<int>();
     const int* d_bseg = d_brow + P.brow.size();
@@ -1876,8 +1881,14 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_LAUNCH_CHECK();
     // per (segment, column) state, index seg * kp + col
     std::vector<double> bn2(ks), rn2(ks), scale(ks, 0.0), ratio(ks, 0.0), tgt(ks, 0.0);
-    SFH_CHECK_HIP(hipMemcpyAsync(bn2.data(), S + S_BN2 * ks, ks * 8, hipMemcpyDeviceToHost, st));
+    // page-locked read-back area: the five check rows, then one row for the norms
+    SFH_TRY(w.host.ensure(9 * (size_t)ks * 8));
+    double* hs = w.host.as<double>();   // ACTIVE, FROZEN, SCALE, TRIG and the latest r.r per column
+    double* hn = hs + 5 * (size_t)ks;   // the norms
+    double* hu = hs + 6 * (size_t)ks;   // scale, ratio and target of the next cycle, on their way to the device
+    SFH_CHECK_HIP(hipMemcpyAsync(hn, S + S_BN2 * ks, ks * 8, hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
+    std::copy(hn, hn + ks, bn2.begin());
     rn2 = bn2;
     const double tol2 = o.tol * o.tol;
     // FP32 reduction per refinement cycle and host check interval: a multigrid
@@ -1888,7 +1899,6 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     int updates = 0, iters = 0;
     double maxrel = 0.0;
     bool done = false;
-    std::vector<double> hs(5 * (size_t)ks);   // ACTIVE, FROZEN, SCALE, TRIG and the latest r.r per column
     const int check = o.check_every > 0 ? o.check_every : (fp64_rec ? 2 : 32);
     int jlast = -1;                           // last iteration of the previous cycle
     std::vector<double> rrchk(ks);            // r.r of every column at the previous check of the cycle
@@ -1997,9 +2007,12 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         hist.push_back(maxrel);
         if (nact == 0) { done = true; break; }
         if (updates > o.max_outer || iters >= o.max_inner) break;
-        SFH_CHECK_HIP(hipMemcpyAsync(S + S_SCALE * ks, scale.data(), ks * 8, hipMemcpyHostToDevice, st));
-        SFH_CHECK_HIP(hipMemcpyAsync(S + S_RATIO * ks, ratio.data(), ks * 8, hipMemcpyHostToDevice, st));
-        SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * ks, tgt.data(), ks * 8, hipMemcpyHostToDevice, st));
+        std::copy(scale.begin(), scale.end(), hu);
+        std::copy(ratio.begin(), ratio.end(), hu + ks);
+        std::copy(tgt.begin(), tgt.end(), hu + 2 * (size_t)ks);
+        SFH_CHECK_HIP(hipMemcpyAsync(S + S_SCALE * ks, hu, ks * 8, hipMemcpyHostToDevice, st));
+        SFH_CHECK_HIP(hipMemcpyAsync(S + S_RATIO * ks, hu + ks, ks * 8, hipMemcpyHostToDevice, st));
+        SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * ks, hu + 2 * (size_t)ks, ks * 8, hipMemcpyHostToDevice, st));
         if (use_g3) {
             k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, 1, d_srow,
                                                                       S + S_SCALE * ks, S + S_RATIO * ks, rmg, rfm,
@@ -2076,12 +2089,12 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                 // live columns left, and columns waiting for replacement (frozen
                 // now or by the r.r of this iteration, or inactive and unconverged)
                 SFH_LAUNCH_CHECK();
-                const int slot[4] = {S_ACTIVE, S_FROZEN, S_SCALE, S_TRIG};
-                for (int q = 0; q < 4; ++q)
-                    SFH_CHECK_HIP(hipMemcpyAsync(hs.data() + (size_t)q * ks, S + (size_t)slot[q] * ks, ks * 8,
-                                                 hipMemcpyDeviceToHost, st));
-                SFH_CHECK_HIP(hipMemcpyAsync(hs.data() + 4 * (size_t)ks, acc + (size_t)(A_RR + j % 3) * ks, ks * 8,
-                                             hipMemcpyDeviceToHost, st));
+                // ACTIVE, FROZEN, SCALE, TRIG and r.r (copied next to them) in one
+                // transfer into page-locked memory
+                SFH_CHECK_HIP(hipMemcpyAsync(S + (size_t)S_CHKRR * ks, acc + (size_t)(A_RR + j % 3) * ks, ks * 8,
+                                             hipMemcpyDeviceToDevice, st));
+                SFH_CHECK_HIP(hipMemcpyAsync(hs, S + (size_t)S_ACTIVE * ks, 5 * (size_t)ks * 8, hipMemcpyDeviceToHost,
+                                             st));
                 SFH_CHECK_HIP(hipStreamSynchronize(st));
                 // a live column whose r.r did not fall over a whole check
                 // interval has stalled in FP32: replace its residual now
@@ -2120,8 +2133,9 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             k_red_norm<<<gred, kRedB, 0, st>>>(kp, d_sb, p0, S + S_RN2 * ks);
         }
         SFH_LAUNCH_CHECK();
-        SFH_CHECK_HIP(hipMemcpyAsync(rn2.data(), S + S_RN2 * ks, ks * 8, hipMemcpyDeviceToHost, st));
+        SFH_CHECK_HIP(hipMemcpyAsync(hn, S + S_RN2 * ks, ks * 8, hipMemcpyDeviceToHost, st));
         SFH_CHECK_HIP(hipStreamSynchronize(st));
+        std::copy(hn, hn + ks, rn2.begin());
     }
     if ((size_t)n * k < (size_t)INT32_MAX)
         k_finalize<unsigned><<<(unsigned)(((size_t)n * k + 255) / 256), 256, 0, st>>>(

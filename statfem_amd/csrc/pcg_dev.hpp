@@ -83,8 +83,10 @@ __device__ inline void block_partial(const Tile& t, int kp, const V4<double>& a,
     }
 }
 
-// per-RHS scalar slots (each kp doubles)
-enum { S_RZ = 0, S_RZPREV, S_FROZEN, S_BETA, S_TRIG, S_TGT, S_ACTIVE, S_SCALE, S_RATIO, S_RN2, S_BN2,
+// per-RHS scalar slots (each ks doubles). The first five are what the host
+// reads back at every convergence check, in one copy: the state flags, the
+// target trigger and S_CHKRR, the latest recursive r.r copied there for the check
+enum { S_ACTIVE = 0, S_FROZEN, S_SCALE, S_TRIG, S_CHKRR, S_RZ, S_RZPREV, S_BETA, S_TGT, S_RATIO, S_RN2, S_BN2,
        S_NSLOT };
 
 // Per-column FP64 accumulators of the iteration kernels (ks doubles each),

@@ -492,7 +492,7 @@ class ForwardGaussRHSHip:
         ns = self.ns
         mean = np.zeros(ns)
         Cu = np.zeros((ns, ns))
-        if want_field:
+        if want_field and getattr(self, "field", None) is None:   # kept across updates
             self.field = self.sys.ctx.empty((self.sys.ndof, ns))
             self.ubar = self.sys.ctx.empty((self.sys.ndof,))
         st = SolveStats()
