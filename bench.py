@@ -161,8 +161,10 @@ def gpu_run(args, traffic=None):
     if stats[-1]["layout"] == 2 and stats[-1]["precond"] == 2:
         # multigrid-preconditioned iteration: 9 FP64 stencil coefficients per
         # node; direction: z, p_old read, p written (FP32); update: p read
-        # (FP32), FP64 x and r read and written
-        algo = {"dir_spmv": 72.0 * N + 3 * vec, "cg_update": 72.0 * N + 9 * vec}
+        # (FP32), FP64 r read and written, and on every other iteration FP64 x
+        # read and written with the previous p read (the steps of x go in
+        # pairs): 7.5 FP32 words per node and column on average
+        algo = {"dir_spmv": 72.0 * N + 3 * vec, "cg_update": 72.0 * N + 7.5 * vec}
         # the survey's FP32 model of the same two phases (per node and column:
         # direction 3 words, update 5 words; FP32 CSR matrix once per 64-column tile)
         mat32 = (8.0 * nnz + 4.0 * (N + 1)) * max(1, kp // 64)
@@ -837,8 +839,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--ns", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--sample-every", type=int, default=16,
-                    help="time every n-th inner PCG iteration with HIP events inside the timed solves")
+    ap.add_argument("--sample-every", type=int, default=15,
+                    help="time every n-th inner PCG iteration with HIP events inside the timed solves (odd, so "
+                         "that the samples alternate between the update's two kinds of iteration)")
     ap.add_argument("--tile", type=int, default=0, help="right-hand sides per solve (0: all in one solve)")
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-rhs", type=int, default=8)

@@ -498,25 +498,33 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_dir_spmv_mg(const int* __restric
 
 // x += (alpha / s) p, r -= alpha A p, r.r: FP64 x, r, stencil and sums; r is
 // the residual scaled by the column's s (unit norm at the last replacement), x
-// the iterate itself, so no correction has to be flushed into it
+// the iterate itself. x moves every other iteration: on odd iterations j by
+// (alpha_{j-1} p_{j-1} + alpha_j p_j) / s, p_{j-1} read from the other
+// direction buffer (pprev), alpha_{j-1} formed again from its accumulator slots
+// (the p.Ap slot of j - 1 is cleared only by the V-cycle after this launch);
+// a cycle that ends after an even iteration flushes its last step with
+// k_x_flush_mg. Work-groups skip as in block_idle, on odd iterations only
+// when every column was frozen before j - 1's step too.
 template <int LPR>
 __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restrict__ gtab, int kp, int ks, int it,
                                                      const double* __restrict__ coef, const double* __restrict__ S,
-                                                     const float* __restrict__ p, double* __restrict__ x,
-                                                     double* __restrict__ r, double* __restrict__ acc) {
+                                                     const float* __restrict__ p, const float* __restrict__ pprev,
+                                                     double* __restrict__ x, double* __restrict__ r,
+                                                     double* __restrict__ acc) {
     extern __shared__ float4 sm[];
     int d[kGT];
-    clear_slots(acc, A_PQ + ((it + 1) & 1), 1, ks);
+    const bool xup = (it & 1) != 0;
     const Tile t = grid_tile_t<LPR>(kp, gtab, d);
     const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
     if (W == 0) return;
-    if (block_idle(t, kp, ks, S, it + 1)) return;
-    __shared__ double ca[4 * LPR], cx[4 * LPR];
+    if (block_idle(t, kp, ks, S, xup ? it : it + 1)) return;
+    __shared__ double ca[4 * LPR], cx[4 * LPR], cx1[4 * LPR];
     shared_alpha<double, LPR>(it, kp, ks, t.so, S, acc, ca);
     for (int c = threadIdx.x; c < 4 * LPR; c += blockDim.x) {
         const int col = blockIdx.y * 4 * LPR + c;
         const double sc = col < kp ? S[S_SCALE * ks + t.so + col] : 0.0;
         cx[c] = sc > 0.0 ? ca[c] / sc : 0.0;
+        cx1[c] = (xup && sc > 0.0) ? alpha_before(it, ks, t.so + col, S, acc) / sc : 0.0;
     }
     const bool ok = t.col0 < kp;
     stage_grid_f32<LPR>(sm, p, kp, ok ? t.col0 : 0, ok, base, W, H, tx0, ty0, t.grp, t.lane);
@@ -525,6 +533,7 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restri
     if (ok) {
         const V4<double> al = {ca[t.lane * 4], ca[t.lane * 4 + 1], ca[t.lane * 4 + 2], ca[t.lane * 4 + 3]};
         const V4<double> ax = {cx[t.lane * 4], cx[t.lane * 4 + 1], cx[t.lane * 4 + 2], cx[t.lane * 4 + 3]};
+        const V4<double> ax1 = {cx1[t.lane * 4], cx1[t.lane * 4 + 1], cx1[t.lane * 4 + 2], cx1[t.lane * 4 + 3]};
         // kUN nodes per pass: all their x, r and stencil coefficients are
         // loaded before any is used (an off-grid node loads the tile's first
         // node's data and stores nothing)
@@ -541,11 +550,15 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restri
                 nd[u] = in[u] ? base + gy * W + gx : base;
             }
             V4<double> xv[kUN], rv[kUN];
+            float4 pv[kUN];
             double c[kUN][9];
 #pragma unroll
             for (int u = 0; u < kUN; ++u) {
                 const size_t off = (size_t)nd[u] * kp + t.col0;
-                xv[u] = ld4(x + off);
+                if (xup) {
+                    xv[u] = ld4(x + off);
+                    pv[u] = *reinterpret_cast<const float4*>(pprev + off);
+                }
                 rv[u] = ld4(r + off);
 #pragma unroll
                 for (int k = 0; k < 9; ++k) c[u][k] = coef[(size_t)nd[u] * 9 + k];
@@ -565,10 +578,15 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restri
                     }
                 const float4 pi = sm[((oy + 1) * (kTX + 2) + (ox + 1)) * LPR + t.lane];
                 const size_t off = (size_t)nd[u] * kp + t.col0;
-                V4<double> xn = xv[u], rn = rv[u];
-                xn.x += ax.x * pi.x; xn.y += ax.y * pi.y; xn.z += ax.z * pi.z; xn.w += ax.w * pi.w;
+                V4<double> rn = rv[u];
                 rn.x -= al.x * q.x; rn.y -= al.y * q.y; rn.z -= al.z * q.z; rn.w -= al.w * q.w;
-                st4(x + off, xn);
+                if (xup) {
+                    V4<double> xn = xv[u];
+                    const float4 pp = pv[u];
+                    xn.x += ax1.x * pp.x + ax.x * pi.x; xn.y += ax1.y * pp.y + ax.y * pi.y;
+                    xn.z += ax1.z * pp.z + ax.z * pi.z; xn.w += ax1.w * pp.w + ax.w * pi.w;
+                    st4(x + off, xn);
+                }
                 st4(r + off, rn);
                 rr.x += rn.x * rn.x; rr.y += rn.y * rn.y; rr.z += rn.z * rn.z; rr.w += rn.w * rn.w;
             }
@@ -768,6 +786,10 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_prolong(const int* __restrict
     const int* d = tab + kMT * t.bid;
     const int fbase = d[0], W = d[1], H = d[2], cbase = d[3], Wc = d[4], Hc = d[5], tx0 = d[6], ty0 = d[7];
     t.so = d[8] * kp;
+    // the p.Ap slot of the next iteration (the update kernel of this one has
+    // read the slot's previous contents, the direction kernel of the next
+    // accumulates into it)
+    if (TOP && mode == 0) clear_slots(acc, A_PQ + ((it + 1) & 1), 1, ks);
     if (W == 0 && !(TOP && mode == 1)) return;
     const bool ok = t.col0 < kp;
     if (mg_cols_done(S, ks, t.so, t.col0, ok)) return;   // (S == NULL for residual replacements)
@@ -884,6 +906,27 @@ __global__ void k_mg_rescale(long long total, int kp, int nseg, const int* __res
     r[i] = v;
     if (rf) rf[i] = (float)v;
     p[i] *= (float)ratio[c];
+}
+
+// x += alpha_j p_j / s on every node and column: the last step of a
+// multigrid cycle that ended after an even iteration j (the update kernel adds
+// the steps of x in pairs, on odd iterations); alpha_j from its accumulator
+// slots as alpha_before forms it
+__global__ void k_x_flush_mg(long long total, int kp, int nseg, const int* __restrict__ srow, int j, int ks,
+                             const double* __restrict__ S, const double* __restrict__ acc,
+                             const float* __restrict__ p, double* __restrict__ x) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int row = (int)(i / kp);
+    int lo = 0, hi = nseg - 1;   // last segment with srow[seg] <= row
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (srow[mid] <= row) lo = mid; else hi = mid - 1;
+    }
+    const int c = lo * kp + (int)(i % kp);
+    const double sc = S[S_SCALE * ks + c];
+    const double a = alpha_before(j + 1, ks, c, S, acc);
+    if (sc > 0.0 && a != 0.0) x[i] += (a / sc) * p[i];
 }
 
 // ------------------------------------------------ vector DOFs: 3 x 3 blocks
@@ -1570,10 +1613,10 @@ struct Kern {
                                                                        pnew, acc);
     }
     static void update_mg(const Plan& P, hipStream_t st, int it, const double* coef64, const double* S, const T* p,
-                          double* x, double* r, double* acc) {
+                          const T* pprev, double* x, double* r, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_cg_update_mg<LPR><<<dim3(P.gblk, P.ntile), GB<LPR>::v, grid_lds_bytes(LPR), st>>>(P.d_gtab, P.kp, P.ks, it, coef64, S, p, x, r,
-                                                                        acc);
+            k_cg_update_mg<LPR><<<dim3(P.gblk, P.ntile), GB<LPR>::v, grid_lds_bytes(LPR), st>>>(
+                P.d_gtab, P.kp, P.ks, it, coef64, S, p, pprev, x, r, acc);
     }
     // z = M r (one V-cycle) with r.z into the iteration's slot (mode 0) or r.z,
     // r.r as per-tile partial rows (mode 1)
@@ -1954,7 +1997,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             T* b = pb;
             for (int jj = j0; jj < j0 + 2; ++jj) {
                 Kern<T, LPR>::dir_spmv_mg(P, st, jj, coef64, S, z, a, b, acc);
-                Kern<T, LPR>::update_mg(P, st, jj, coef64, S, b, w.x64.as<double>(), rmg, acc);
+                Kern<T, LPR>::update_mg(P, st, jj, coef64, S, b, a, w.x64.as<double>(), rmg, acc);
                 Kern<T, LPR>::vcycle(P, mg, st, jj, 0, coef, cmask, rmg, z, mgv, acc, p0, p1, S);
                 std::swap(a, b);
             }
@@ -2067,7 +2110,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             if (use_g3)
                 g3_update(P3, s->mg3, st, j, S, (const float*)pb, w.x64.as<double>(), rmg, rfm, acc);
             else if (use_mg)
-                Kern<T, LPR>::update_mg(P, st, j, coef64, S, pb, w.x64.as<double>(), rmg, acc);
+                Kern<T, LPR>::update_mg(P, st, j, coef64, S, pb, pa, w.x64.as<double>(), rmg, acc);
             else if (P.grid)
                 Kern<T, LPR>::update_g(P, st, j, coef, dinv, diag, S, pb, x, z, acc);
             else if (P.bsr)
@@ -2119,6 +2162,11 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             }
         }
         jlast = j;
+        if (use_mg && (jlast & 1) == 0) {   // x's last step, not added yet (steps go in pairs)
+            k_x_flush_mg<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow, jlast, ks, S,
+                                                                      acc, (const float*)pa, w.x64.as<double>());
+            SFH_LAUNCH_CHECK();
+        }
         // flush the FP32 correction, true residual in FP64
         if (!fp64_rec)   // (the multigrid iterations update x64 themselves)
             k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,

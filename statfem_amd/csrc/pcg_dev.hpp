@@ -102,6 +102,18 @@ __device__ inline double rz_at(int j, int ks, int c, const double* __restrict__ 
     return j == 0 ? S[S_RZ * ks + c] : acc[(A_RZ + (j - 1) % 3) * ks + c];
 }
 
+// Step length of iteration j - 1 (j >= 1) of column c, formed again from its
+// accumulator slots exactly as the update kernel of j - 1 formed it: zero for
+// a column that was not live then (inactive, or frozen at an iteration before
+// j: S_FROZEN holds the freezing iteration + 1)
+__device__ inline double alpha_before(int j, int ks, int c, const double* __restrict__ S,
+                                      const double* __restrict__ acc) {
+    const double fz = S[S_FROZEN * ks + c];
+    const bool live = S[S_ACTIVE * ks + c] > 0.0 && !(fz > 0.0 && fz <= (double)j);
+    const double pq = acc[(A_PQ + ((j - 1) & 1)) * ks + c];
+    return (live && pq > 0.0) ? rz_at(j - 1, ks, c, S, acc) / pq : 0.0;
+}
+
 // Direction coefficients of column c at iteration j: p = m z + b p_old with
 // b = beta, m = 1 for a live column and b = 1, m = 0 for a frozen or inactive
 // one. A live column whose recursive residual r.r reached its trigger, or whose
