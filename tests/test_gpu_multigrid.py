@@ -196,3 +196,38 @@ def test_graph_replays_match_eager_launches(ctx, orc):
             xg, xe = Xg.download(), Xe.download()
             assert np.abs(xg - xe).max() <= 1e-12 * np.abs(xe).max()
             assert np.abs(xg - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def test_iterate_steps_in_pairs_any_cycle_length(ctx, orc):
+    """The update kernel adds the steps of x in pairs (on odd iterations) and a
+    cycle that ends after an even iteration adds its last step separately:
+    iterations capped at 2..7 with a check after every iteration give CG
+    iterates whose A-norm errors fall strictly (a missed last step would
+    repeat the previous iterate), and solves checked every 1 or 3 iterations
+    reach the oracle's solution."""
+    from statfem_amd import hip
+    import scipy.sparse as sp
+    m, bc, kap, nq, rp, ci, va = _problem(orc, 40, 33, M.QUAD4, "all", seed=8)
+    s = _system(ctx, m, kap, nq, bc)
+    rng = np.random.default_rng(21)
+    k = 12
+    B = rng.normal(size=(m.nnodes, k))
+    B[bc] = 0.0
+    ref = orc.LU(rp, ci, va).solve(B)
+    A = sp.csr_matrix((va, ci, rp), shape=(m.nnodes, m.nnodes))
+    free = np.setdiff1d(np.arange(m.nnodes), bc)
+    Af = A[free][:, free]
+    X = ctx.empty((m.nnodes, k))
+    Bd = ctx.to_device(B)
+    errs = []
+    for cap in range(2, 8):
+        st = hip.solve_probe(s, Bd, X, k, opts=hip.default_opts(tol=1e-14, check_every=1, max_inner=cap))
+        assert st["inner_iters"] == cap, st
+        e = (X.download() - ref)[free]
+        errs.append(np.einsum("ic,ic->c", e, Af @ e))
+    errs = np.array(errs)
+    assert np.all(errs[1:] < errs[:-1]), errs.max(axis=1)
+    for ce in (1, 3):
+        Xc, st = s.solve(B, opts=hip.default_opts(tol=1e-12, check_every=ce))
+        assert st["converged"] == 1 and st["precond"] == 2, st
+        assert np.abs(Xc.download() - ref).max() <= 1e-9 * np.abs(ref).max(), ce
