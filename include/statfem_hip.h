@@ -187,6 +187,10 @@ typedef struct {
                             (default) automatic (up to 16 on the stencil layouts, 64 on CSR), else 1, 2, 4,
                             8 or 16 (a power of two at most this). Narrow column tiles let the work-groups
                             of a batch skip the padding and converged columns of each problem */
+    int mg_split;        /* 2D multigrid V-cycle: 0 (default) runs the coarse levels from the first one
+                            below which every grid's remaining hierarchy fits one work-group's LDS as a
+                            single launch (one work-group per column and grid); 1 launches a restriction
+                            and a prolongation per level and the coarse solve separately */
 } sfh_solve_opts;
 
 typedef struct {

@@ -166,6 +166,9 @@ struct MGHier {
     std::vector<int> gW, gH, gbase;  // per (level, grid), index l * nseg + q
     long long N[kMGMaxLev + 1] = {}; // nodes of every level (all grids)
     DevBuf coef64[kMGMaxLev + 1], coef32[kMGMaxLev + 1], dsm[kMGMaxLev + 1];
+    // levels >= 1: the FP32 stencil coefficient-major (9 rows of N[l]), read
+    // by the single-launch coarse levels with one coalesced load per coefficient
+    DevBuf coefT32[kMGMaxLev + 1];
     DevBuf cinv;                     // coarsest-level inverses, grid after grid
     std::vector<int> cinv_off;
     int cmax = 0;                    // nodes of the largest coarsest level

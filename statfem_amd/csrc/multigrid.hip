@@ -133,8 +133,10 @@ __global__ __launch_bounds__(256) void k_mg_galerkin9(const int* __restrict__ ta
 }
 
 // smoother weights omega_i / sum_j |a_ij| (omega_i from the stencil's symbol,
-// frequencies m pi / 4, m = 0..4 per axis) and (levels >= 1) the FP32 stencil
-__global__ void k_mg_level(int n, const double* __restrict__ A, float* __restrict__ dsm, float* __restrict__ A32) {
+// frequencies m pi / 4, m = 0..4 per axis) and (levels >= 1) the FP32 stencil,
+// node-major (A32) and coefficient-major (AT32, 9 rows of n)
+__global__ void k_mg_level(int n, const double* __restrict__ A, float* __restrict__ dsm, float* __restrict__ A32,
+                           float* __restrict__ AT32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double a[9], s = 0.0;
@@ -142,6 +144,7 @@ __global__ void k_mg_level(int n, const double* __restrict__ A, float* __restric
         a[k] = A[(size_t)i * 9 + k];
         s += fabs(a[k]);
         if (A32) A32[(size_t)i * 9 + k] = (float)a[k];
+        if (AT32) AT32[(size_t)k * n + i] = (float)a[k];
     }
     if (!(s > 0.0)) {
         dsm[i] = 0.f;
@@ -360,7 +363,8 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
         m.N[l] = l == 0 ? s->ndof : base;
     }
     SFH_TRY(m.dsm[0].ensure(sizeof(float) * (size_t)s->ndof));
-    k_mg_level<<<(s->ndof + 255) / 256, 256, 0, st>>>(s->ndof, s->dia64.as<double>(), m.dsm[0].as<float>(), nullptr);
+    k_mg_level<<<(s->ndof + 255) / 256, 256, 0, st>>>(s->ndof, s->dia64.as<double>(), m.dsm[0].as<float>(), nullptr,
+                                                       nullptr);
     SFH_LAUNCH_CHECK();
     // set-up tables for every level, uploaded once: the grids coarsened from
     // level l ({W, H, Wc, Hc, fine base, coarse base}) and the grids whose
@@ -406,6 +410,7 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
         const long long nc = m.N[l + 1];
         SFH_TRY(m.coef64[l + 1].ensure(sizeof(double) * 9 * (size_t)nc));
         SFH_TRY(m.coef32[l + 1].ensure(sizeof(float) * 9 * (size_t)nc));
+        SFH_TRY(m.coefT32[l + 1].ensure(sizeof(float) * 9 * (size_t)nc));
         SFH_TRY(m.dsm[l + 1].ensure(sizeof(float) * (size_t)nc));
         const double* Af = l == 0 ? s->dia64.as<double>() : m.coef64[l].as<double>();
         for (int r0 = 0; r0 < grow[l]; r0 += 65535) {
@@ -418,7 +423,8 @@ int mg_build(sfh_system* s, hipStream_t st, const std::vector<int>& grids) {
         }
         k_mg_level<<<(unsigned)((nc + 255) / 256), 256, 0, st>>>((int)nc, m.coef64[l + 1].as<double>(),
                                                                  m.dsm[l + 1].as<float>(),
-                                                                 m.coef32[l + 1].as<float>());
+                                                                 m.coef32[l + 1].as<float>(),
+                                                                 m.coefT32[l + 1].as<float>());
         SFH_LAUNCH_CHECK();
     }
     // dense inverses of the coarsest levels, on the device: one launch per

@@ -887,6 +887,235 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_prolong(const int* __restrict
     }
 }
 
+// The coarse end of the V-cycle in one launch: from level lt (the first level
+// below which the rest of every grid's hierarchy fits in one work-group's LDS)
+// down to the coarsest and back, four columns of one grid per work-group
+// (float4 per node, so the index arithmetic and the stencil loads serve four
+// columns). The level-lt residual comes from the restriction launch above;
+// the restricted residuals and corrections of the lower levels live in LDS
+// only; the level-lt correction goes back to the level buffer for the
+// prolongation launch below. The operations are those of the per-level
+// launches: down, t = S r, r - A t and its restriction P^T; on the coarsest
+// level the dense inverse; up, e0 = S r + P e_{l+1} and e = e0 + S (r - A e0).
+// Each thread issues all loads of a level phase (its nodes' smoother weights
+// and coefficient-major stencils) at once. The coarsest product reads the
+// inverse column by column (coalesced over the rows the threads own): the
+// inverse of the symmetric coarsest operator, equal to its transpose up to
+// FP32 rounding. Work-groups of one 128-byte line of columns run on one XCD
+// (xcd_block over the column groups), so their stores merge in that XCD's L2.
+constexpr int kTailB = 1024;
+constexpr int kTailVec = 4096;                           // LDS of a work-group in float4: 64 KB
+constexpr int kTailRed = kTailB;                         // partial sums of the coarsest product
+constexpr int kTailPer = 2;                              // nodes per thread: levels of at most 2,048 nodes
+constexpr int kTailLev = 8;
+// table per grid: levels nl, offset of the coarsest inverse, LDS offset (in
+// float4) of the partial sums, 0; per level u = 0 (lt) .. nl - 1 (coarsest):
+// W, H, LDS offset of r (e follows it), the level's node count (the row
+// stride of the coefficient-major stencil), then as 64-bit pairs the grid's
+// first coefficient in that stencil, its smoother weights, and the float
+// offsets of its r and e (column 0) in the level buffer
+constexpr int kTLev = 12;
+constexpr int kTT = 4 + kTLev * kTailLev;
+
+__device__ inline unsigned long long tab_u64(const int* t) {
+    return ((unsigned long long)(unsigned)t[1] << 32) | (unsigned)t[0];
+}
+
+// the smoother weights and stencils of the thread's nodes i = tid + k kTailB
+// of a level of n nodes (coefficient-major stencil, row stride ld; nodes past
+// n load node 0 and are not used)
+__device__ inline void tail_load(const float* __restrict__ cf, int ld, const float* __restrict__ ds, int n,
+                                 float (&a)[kTailPer][9], float (&w)[kTailPer]) {
+#pragma unroll
+    for (int k = 0; k < kTailPer; ++k) {
+        const int i = threadIdx.x + k * kTailB;
+        const int j = i < n ? i : 0;
+        w[k] = ds[j];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) a[k][m] = cf[(size_t)m * ld + j];
+    }
+}
+
+// q_k = R_i - sum_j a_ij X_j for the thread's nodes of a W x H level
+// (out-of-grid neighbours skipped)
+__device__ inline void tail_residual(const float4* R, const float4* X, const float (&a)[kTailPer][9], int W, int H,
+                                     float4 (&q)[kTailPer]) {
+    const int n = W * H;
+#pragma unroll
+    for (int k = 0; k < kTailPer; ++k) {
+        const int i = threadIdx.x + k * kTailB;
+        float4 s = zero4();
+        if (i < n) {
+            const int x = i % W, y = i / W;
+            s = R[i];
+#pragma unroll
+            for (int m = 0; m < 9; ++m) {
+                const int nx = x + m % 3 - 1, ny = y + m / 3 - 1;
+                if (nx >= 0 && nx < W && ny >= 0 && ny < H) fma4(-a[k][m], X[ny * W + nx], s);
+            }
+        }
+        q[k] = s;
+    }
+}
+
+__global__ __launch_bounds__(kTailB) void k_mg_tail(const int* __restrict__ tab, int kp, int ks,
+                                                     const float* __restrict__ cinv, float* __restrict__ mgv,
+                                                     const double* __restrict__ S) {
+    extern __shared__ float4 sm[];
+    const int c = 4 * xcd_block(blockIdx.x, gridDim.x), q = blockIdx.y, tid = threadIdx.x;
+    if (c >= kp) return;
+    if (mg_cols_done(S, ks, q * kp, c, true)) return;   // (S == NULL for residual replacements)
+    const int* d = tab + kTT * q;
+    const int nl = d[0];
+    {   // the top level's residual
+        const int* e = d + 4;
+        const int n = e[0] * e[1];
+        const float* rg = mgv + tab_u64(e + 8) + c;
+        float4* R = sm + e[2];
+        float4 v[kTailPer];
+#pragma unroll
+        for (int k = 0; k < kTailPer; ++k) {
+            const int i = tid + k * kTailB;
+            v[k] = ldf4(rg + (size_t)(i < n ? i : 0) * kp);
+        }
+#pragma unroll
+        for (int k = 0; k < kTailPer; ++k)
+            if (tid + k * kTailB < n) R[tid + k * kTailB] = v[k];
+    }
+    __syncthreads();
+    for (int u = 0; u + 1 < nl; ++u) {
+        const int* e = d + 4 + kTLev * u;
+        const int W = e[0], H = e[1], n = W * H;
+        float4* R = sm + e[2];
+        float4* E = R + n;
+        float a[kTailPer][9], w[kTailPer];
+        float4 t[kTailPer];
+        tail_load(reinterpret_cast<const float*>(tab_u64(e + 4)), e[3], reinterpret_cast<const float*>(tab_u64(e + 6)),
+                  n, a, w);
+#pragma unroll
+        for (int k = 0; k < kTailPer; ++k) {
+            const int i = tid + k * kTailB;
+            if (i < n) {
+                const float4 r = R[i];
+                E[i] = make_float4(w[k] * r.x, w[k] * r.y, w[k] * r.z, w[k] * r.w);
+            }
+        }
+        __syncthreads();
+        tail_residual(R, E, a, W, H, t);   // r - A S r
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTailPer; ++k) {
+            const int i = tid + k * kTailB;
+            if (i < n) E[i] = t[k];
+        }
+        __syncthreads();
+        const int* ec = e + kTLev;
+        const int Wc = ec[0], Hc = ec[1];
+        float4* Rc = sm + ec[2];
+        for (int I = tid; I < Wc * Hc; I += kTailB) {
+            const int X = I % Wc, Y = I / Wc;
+            float4 acc = zero4();
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int fy = 2 * Y + dy;
+                if (fy < 0 || fy >= H) continue;
+                const float wy = mg_axis_w(fy, Y, H);
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int fx = 2 * X + dx;
+                    if (fx < 0 || fx >= W) continue;
+                    const float wt = wy * mg_axis_w(fx, X, W);
+                    if (wt != 0.f) fma4(wt, E[fy * W + fx], acc);
+                }
+            }
+            Rc[I] = acc;
+        }
+        __syncthreads();
+    }
+    {   // coarsest level: e = C^-1 r; g = kTailB / n partial sums per row, added in a fixed order
+        const int* e = d + 4 + kTLev * (nl - 1);
+        const int n = e[0] * e[1];
+        const float4* R = sm + e[2];
+        float4* red = sm + d[2];
+        const float* ci = cinv + d[1];
+        const int ng = kTailB / n, g = tid / n, i = tid % n;
+        if (g < ng) {
+            float4 s = zero4();
+#pragma unroll 4
+            for (int j = g; j < n; j += ng) fma4(ci[(size_t)j * n + i], R[j], s);
+            red[tid] = s;
+        }
+        __syncthreads();
+        if (tid < n) {
+            float4 s = zero4();
+            for (int h = 0; h < ng; ++h) {
+                const float4 v = red[h * n + tid];
+                s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+            }
+            if (nl == 1)
+                *reinterpret_cast<float4*>(mgv + tab_u64(e + 10) + (size_t)tid * kp + c) = s;
+            else
+                sm[e[2] + n + tid] = s;
+        }
+        __syncthreads();
+    }
+    for (int u = nl - 2; u >= 0; --u) {
+        const int* e = d + 4 + kTLev * u;
+        const int W = e[0], H = e[1], n = W * H;
+        float4* R = sm + e[2];
+        float4* E = R + n;
+        const int* ec = e + kTLev;
+        const int Wc = ec[0];
+        const float4* Ec = sm + ec[2] + Wc * ec[1];
+        float a[kTailPer][9], w[kTailPer];
+        float4 z[kTailPer];
+        tail_load(reinterpret_cast<const float*>(tab_u64(e + 4)), e[3], reinterpret_cast<const float*>(tab_u64(e + 6)),
+                  n, a, w);
+#pragma unroll
+        for (int k = 0; k < kTailPer; ++k) {   // e0 = S r + P e_{l+1}
+            const int i = tid + k * kTailB;
+            if (i >= n) continue;
+            const int x = i % W, y = i / W;
+            const float4 r = R[i];
+            float4 v = make_float4(w[k] * r.x, w[k] * r.y, w[k] * r.z, w[k] * r.w);
+            int ax, ay;
+            float fx, fy;
+            const int nx = mg_axis_parents(x, W, ax, fx);
+            const int ny = mg_axis_parents(y, H, ay, fy);
+            for (int pa = 0; pa < ny; ++pa)
+                for (int pb = 0; pb < nx; ++pb) fma4(fx * fy, Ec[(ay + pa) * Wc + ax + pb], v);
+            E[i] = v;
+        }
+        __syncthreads();
+        tail_residual(R, E, a, W, H, z);
+#pragma unroll
+        for (int k = 0; k < kTailPer; ++k) {
+            const int i = tid + k * kTailB;
+            if (i < n) {
+                float4 y = E[i];
+                fma4(w[k], z[k], y);
+                z[k] = y;
+            }
+        }
+        if (u == 0) {
+            float* eg = mgv + tab_u64(e + 10) + c;
+#pragma unroll
+            for (int k = 0; k < kTailPer; ++k) {
+                const int i = tid + k * kTailB;
+                if (i < n) *reinterpret_cast<float4*>(eg + (size_t)i * kp) = z[k];
+            }
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kTailPer; ++k) {
+                const int i = tid + k * kTailB;
+                if (i < n) E[i] = z[k];
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // residual replacement with the multigrid preconditioner: r *= s' in place
 // (FP64) and, when rf is given, its FP32 copy, p *= s'/s; the row's problem
 // (srow: nseg + 1 row bounds) selects the scalars
@@ -1452,6 +1681,13 @@ struct Plan {
     // of a residual replacement)
     std::vector<int> mgctab, mgsb;
     const int* d_mgctab = nullptr;
+    // coarse end of the V-cycle in one launch (k_mg_tail): its first level
+    // (0: one restriction and one prolongation launch per level and the
+    // coarse solve), its table (kTT ints per grid) and LDS bytes
+    int mg_tail = 0;
+    size_t mgt_lds = 0;
+    std::vector<int> mgttab;
+    const int* d_mgttab = nullptr;
 };
 
 Plan make_plan(int n, int k, int num_cu, int nseg = 1, const int* seg_rows = nullptr, int min_kp = 4,
@@ -1582,6 +1818,58 @@ void plan_mg(Plan& p, const MGHier& m) {
     }
 }
 
+// k_mg_tail's first level: the finest level lt >= 1 from which every grid's
+// remaining levels (from min(lt, its coarsest) down) hold r and e of four
+// columns in kTailVec float4 with the partial-sum area, in at most kTailLev
+// levels of at most kTailPer nodes per thread; lt = L always fits (the
+// coarsest levels alone) when every coarsest level has at most kMGCoarse nodes
+void plan_mg_tail(Plan& p, const MGHier& m) {
+    const int ng = m.nseg;
+    auto nodes = [&](int l, int q) { return (long long)m.gW[(size_t)l * ng + q] * m.gH[(size_t)l * ng + q]; };
+    p.mg_tail = 0;
+    for (int l0 = 1; l0 <= m.L && !p.mg_tail; ++l0) {
+        bool fits = true;
+        for (int q = 0; q < ng && fits; ++q) {
+            const int lt = std::min(l0, m.segL[q]);
+            long long tot = 0;
+            for (int l = lt; l <= m.segL[q]; ++l) tot += nodes(l, q);
+            fits = m.segL[q] - lt + 1 <= kTailLev && 2 * tot + kTailRed <= kTailVec &&
+                   nodes(lt, q) <= (long long)kTailPer * kTailB && nodes(m.segL[q], q) <= kMGCoarse;
+        }
+        if (fits) p.mg_tail = l0;
+    }
+    p.mgttab.assign((size_t)kTT * ng, 0);
+    p.mgt_lds = 0;
+    if (!p.mg_tail) return;
+    auto put64 = [](int* t, unsigned long long v) {
+        t[0] = (int)(unsigned)(v & 0xffffffffull);
+        t[1] = (int)(unsigned)(v >> 32);
+    };
+    for (int q = 0; q < ng; ++q) {
+        int* d = p.mgttab.data() + (size_t)kTT * q;
+        const int lt = std::min(p.mg_tail, m.segL[q]), nl = m.segL[q] - lt + 1;
+        int off = 0;
+        for (int u = 0; u < nl; ++u) {
+            const int l = lt + u;
+            const size_t o = (size_t)l * ng + q;
+            int* e = d + 4 + kTLev * u;
+            e[0] = m.gW[o];
+            e[1] = m.gH[o];
+            e[2] = off;
+            e[3] = (int)m.N[l];
+            off += 2 * e[0] * e[1];
+            put64(e + 4, (unsigned long long)(uintptr_t)(m.coefT32[l].as<float>() + m.gbase[o]));
+            put64(e + 6, (unsigned long long)(uintptr_t)(m.dsm[l].as<float>() + m.gbase[o]));
+            put64(e + 8, (unsigned long long)(p.mgv_off[l] + (size_t)m.gbase[o] * p.kp));
+            put64(e + 10, (unsigned long long)(p.mgv_off[l] + ((size_t)m.N[l] + m.gbase[o]) * p.kp));
+        }
+        d[0] = nl;
+        d[1] = m.cinv_off[q];
+        d[2] = off;
+        p.mgt_lds = std::max(p.mgt_lds, sizeof(float4) * (size_t)(off + kTailRed));
+    }
+}
+
 template <class T, int LPR>
 struct Kern {
     static void dir_spmv(const Plan& P, const Seg& G, hipStream_t st, int it, const int* rp, const Entries<T>& M,
@@ -1631,16 +1919,22 @@ struct Kern {
             auto rvec = [&](int l) -> float* { return mgv + P.mgv_off[l]; };
             auto evec = [&](int l) -> float* { return l == 0 ? z : mgv + P.mgv_off[l] + (size_t)m.N[l] * kp; };
             const double* Sk = mode == 0 ? S : nullptr;   // work-group skips only inside an iteration
+            // levels from lt on: one launch (k_mg_tail), else the coarse solve
+            const int lt = P.mg_tail > 0 ? P.mg_tail : L;
             k_mg_restrict<ML, double><<<dim3(P.mg_rn[0], ntm), GB<ML>::v, mg_restrict_lds(ML), st>>>(
                 P.d_mgtab + (size_t)kMT * P.mg_roff[0], kp, coef0, m.dsm[0].as<float>(), cmask, r0, rvec(1), Sk,
                 P.ks);
-            for (int l = 1; l < L; ++l)
+            for (int l = 1; l < lt; ++l)
                 k_mg_restrict<ML, float><<<dim3(P.mg_rn[l], ntm), GB<ML>::v, mg_restrict_lds(ML), st>>>(
                     P.d_mgtab + (size_t)kMT * P.mg_roff[l], kp, m.coef32[l].as<float>(), m.dsm[l].as<float>(),
                     nullptr, rvec(l), rvec(l + 1), Sk, P.ks);
-            k_mg_coarse<<<dim3((kp + kCoarseCols - 1) / kCoarseCols, m.nseg), 256, mg_coarse_lds(m.cmax), st>>>(
-                P.d_mgctab, kp, m.cinv.as<float>(), mgv);
-            for (int l = L - 1; l >= 1; --l)
+            if (P.mg_tail > 0)
+                k_mg_tail<<<dim3((kp / 4 + 7) / 8 * 8, m.nseg), kTailB, P.mgt_lds, st>>>(P.d_mgttab, kp, P.ks,
+                                                                                       m.cinv.as<float>(), mgv, Sk);
+            else
+                k_mg_coarse<<<dim3((kp + kCoarseCols - 1) / kCoarseCols, m.nseg), 256, mg_coarse_lds(m.cmax), st>>>(
+                    P.d_mgctab, kp, m.cinv.as<float>(), mgv);
+            for (int l = lt - 1; l >= 1; --l)
                 k_mg_prolong<LPR, false, float><<<dim3(P.mg_pn[l], P.ntile), GB<LPR>::v, mg_prolong_lds(LPR), st>>>(
                     P.d_mgtab + (size_t)kMT * P.mg_poff[l], kp, P.ks, it, mode, m.coef32[l].as<float>(),
                     m.dsm[l].as<float>(), nullptr, rvec(l), evec(l + 1), evec(l), nullptr, nullptr, nullptr, Sk);
@@ -1803,6 +2097,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     for (int q = 0; use_mg && q < mg.nseg; ++q)
         use_mg = mg.gbase[q] == grids[3 * q] && mg.gW[q] == grids[3 * q + 1] && mg.gH[q] == grids[3 * q + 2];
     if (use_mg) plan_mg(P, mg);
+    if (use_mg && !o.mg_split) plan_mg_tail(P, mg);
     const bool fp64_rec = use_mg || use_g3;   // multigrid: FP64 iterate and residual
     const int nbi = std::max(std::max(P.grid ? P.gblk : P.nblk, use_mg ? P.mg_pn[0] : 0),
                              use_g3 ? P3.nsmooth : 0);   // partial rows per launch
@@ -1824,7 +2119,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     // (3D multigrid: its coarse-solve table and the partial-row bounds {0, smoothing work-groups})
     const int sb3[2] = {0, use_g3 ? P3.nsmooth : 0};
     const size_t ntab = P.brow.size() + P.bseg.size() + P.sb.size() + P.srow.size() + P.gtab.size() +
-                        P.mgtab.size() + P.mgctab.size() + P.mgsb.size() + 4 + 2;
+                        P.mgtab.size() + P.mgctab.size() + P.mgsb.size() + 4 + 2 + P.mgttab.size();
     SFH_TRY(w.tab.ensure(sizeof(int) * ntab));
     {
         std::vector<int> h;
@@ -1839,6 +2134,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         h.insert(h.end(), P.mgsb.begin(), P.mgsb.end());
         h.insert(h.end(), P3.ctab, P3.ctab + 4);
         h.insert(h.end(), sb3, sb3 + 2);
+        h.insert(h.end(), P.mgttab.begin(), P.mgttab.end());
         // a solve of the same shape as the last one on this workspace finds
         // its tables in place (no upload, no wait for the queued work)
         if (h != w.tab_host || w.tab.bytes < sizeof(int) * ntab) {
@@ -1857,6 +2153,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     const int* d_sbmg = P.d_mgctab + P.mgctab.size();
     P3.d_ctab = d_sbmg + P.mgsb.size();
     const int* d_sb3 = P3.d_ctab + 4;
+    P.d_mgttab = d_sb3 + 2;
     const Seg G = {d_brow, d_bseg, ks};
     const dim3 gred((kp + 63) / 64, ns);
     double* S = w.scal.as<double>();
@@ -1970,7 +2267,8 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             (unsigned long long)(uintptr_t)rmg, (unsigned long long)(uintptr_t)z, (unsigned long long)(uintptr_t)mgv,
             (unsigned long long)(uintptr_t)acc, (unsigned long long)(uintptr_t)p0, (unsigned long long)(uintptr_t)p1,
             (unsigned long long)(uintptr_t)S, (unsigned long long)(uintptr_t)pa, (unsigned long long)(uintptr_t)pb,
-            (unsigned long long)(uintptr_t)w.x64.p, (unsigned long long)(uintptr_t)mg.cinv.p};
+            (unsigned long long)(uintptr_t)w.x64.p, (unsigned long long)(uintptr_t)mg.cinv.p,
+            (unsigned long long)P.mg_tail, (unsigned long long)P.mgt_lds, (unsigned long long)(uintptr_t)P.d_mgttab};
         for (int l = 0; l <= P.mgL; ++l) {
             key.push_back((unsigned long long)P.mg_rn[l]);
             key.push_back((unsigned long long)P.mg_pn[l]);
@@ -1979,6 +2277,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
             key.push_back((unsigned long long)P.mgv_off[l]);
             key.push_back((unsigned long long)mg.N[l]);
             key.push_back((unsigned long long)(uintptr_t)mg.coef32[l].p);
+            key.push_back((unsigned long long)(uintptr_t)mg.coefT32[l].p);
             key.push_back((unsigned long long)(uintptr_t)mg.dsm[l].p);
         }
         if (key != w.graphs.key) {
@@ -2374,4 +2673,5 @@ extern "C" void sfh_solve_opts_default(sfh_solve_opts* o) {
     o->precond = 0;
     o->graph = 1;
     o->lanes = 0;
+    o->mg_split = 0;
 }

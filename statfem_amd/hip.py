@@ -34,7 +34,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("inner_tol", C.c_double), ("max_outer", C.c_int),
                 ("max_inner", C.c_int), ("check_every", C.c_int), ("precision", C.c_int),
                 ("rhs_tile", C.c_int), ("layout", C.c_int), ("precond", C.c_int), ("graph", C.c_int),
-                ("lanes", C.c_int)]
+                ("lanes", C.c_int), ("mg_split", C.c_int)]
 
 
 class SolveStats(C.Structure):

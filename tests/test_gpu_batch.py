@@ -263,3 +263,46 @@ def test_autotuned_launch_shape_is_persisted(ctx, tmp_path, monkeypatch):
     assert key in autotune.load_cache()
     means, Cus, st = bf.updateSolution(hip.default_opts(tol=1e-10, lanes=c2["lanes"], rhs_tile=c2["rhs_tile"]))
     assert st["converged"] == 1 and all(np.isfinite(Cq).all() for Cq in Cus)
+
+
+@pytest.mark.parametrize("mg_split", [0, 1])
+def test_batched_multigrid_coarse_levels_one_launch_or_per_level(ctx, orc, mg_split):
+    """A batch whose largest grid (201 x 201 nodes) starts the single-launch
+    coarse end of the V-cycle two levels down while its small grids reach
+    their coarsest level earlier (their coarse solve then runs in that
+    launch): every problem matches its direct solve, with the coarse levels
+    in one launch and with one launch per level."""
+    from statfem_amd import hip, mesh as M
+    sizes = [(200, 200), (9, 9), (57, 31), (120, 70)]
+    meshes = [M.grid2d(a, b) for a, b in sizes]
+    mu, off = M.union(meshes)
+    s = hip.SystemSolveHip(ctx, mu)
+    kap = np.exp(0.5 * np.random.default_rng(8).normal(size=mu.nel))
+    s.assemble(hip.POISSON, kap, nq1d=2)
+    bc = np.concatenate([M.dirichlet_all_boundary(m) + off[q] for q, m in enumerate(meshes)]).astype(np.int32)
+    s.applyConstraints(bc)
+    seg = np.ascontiguousarray(off, dtype=np.int32)
+    nx = np.array([a + 1 for a, b in sizes], np.int32)
+    ny = np.array([b + 1 for a, b in sizes], np.int32)
+    hip._check(hip.lib().sfh_system_set_segment_grids(s.h, len(sizes), hip._p(seg), hip._p(nx), hip._p(ny)))
+    rp, ci, va, _ = orc.assemble(mu, orc.POISSON, kap, nq1d=2, bc_dofs=bc)
+    lu = orc.LU(rp, ci, va)
+    rng = np.random.default_rng(12)
+    k = 9
+    B = rng.normal(size=(mu.nnodes, k))
+    B[off[1]:off[2], 4:] = 0.0
+    for trans in (0, 1):
+        X = ctx.empty((mu.nnodes, k))
+        st = hip.SolveStats()
+        o = hip.default_opts(tol=1e-11, mg_split=mg_split)
+        Bd = ctx.to_device(B)
+        hip._check(hip.lib().sfh_solve_batched(s.h, len(sizes), hip._p(seg), trans, k, Bd.ptr, X.ptr, C.byref(o),
+                                               C.byref(st)))
+        assert st.precond == 2 and st.layout == 2 and st.converged == 1, st.as_dict()
+        assert st.inner_iters < 40, st.as_dict()
+        Xh = X.download()
+        ref = lu.solve(B, trans=bool(trans))
+        for q in range(len(sizes)):
+            a, b = off[q], off[q + 1]
+            assert np.abs(Xh[a:b] - ref[a:b]).max() <= 1e-9 * np.abs(ref[a:b]).max(), (q, trans)
+        assert np.all(Xh[off[1]:off[2], 4:] == 0.0)

@@ -231,3 +231,27 @@ def test_iterate_steps_in_pairs_any_cycle_length(ctx, orc):
         Xc, st = s.solve(B, opts=hip.default_opts(tol=1e-12, check_every=ce))
         assert st["converged"] == 1 and st["precond"] == 2, st
         assert np.abs(Xc.download() - ref).max() <= 1e-9 * np.abs(ref).max(), ce
+
+
+@pytest.mark.parametrize("nx,ny", [(128, 128), (127, 90), (300, 41)])
+def test_coarse_levels_in_one_launch_match_per_level_launches(ctx, orc, nx, ny):
+    """The coarse end of the V-cycle as one launch (mg_split 0: restricted
+    residuals and corrections of the levels from 65 x 65 nodes or so down in
+    LDS, one work-group per column) against one restriction and one
+    prolongation launch per level (mg_split 1): the same iteration counts up
+    to rounding, both at the oracle's direct solve, on square, rectangular
+    and elongated grids with odd and even node counts."""
+    from statfem_amd import hip
+    m, bc, kap, nq, rp, ci, va = _problem(orc, nx, ny, M.QUAD4, "lb", seed=4)
+    s = _system(ctx, m, kap, nq, bc)
+    lu = orc.LU(rp, ci, va)
+    rng = np.random.default_rng(9)
+    for k, trans in ((40, True), (3, False), (1, True)):
+        B = rng.normal(size=(m.nnodes, k))
+        ref = lu.solve(B, trans=trans)
+        Xf, sf = s.solve(B, trans=trans, opts=hip.default_opts(tol=1e-12, mg_split=0))
+        Xs, ss = s.solve(B, trans=trans, opts=hip.default_opts(tol=1e-12, mg_split=1))
+        assert sf["precond"] == 2 and ss["precond"] == 2 and sf["converged"] == 1, (sf, ss)
+        assert abs(sf["inner_iters"] - ss["inner_iters"]) <= 2, (sf, ss)
+        for X in (Xf, Xs):
+            assert np.abs(X.download() - ref).max() <= 1e-9 * np.abs(ref).max(), (k, trans)
