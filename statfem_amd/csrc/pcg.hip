@@ -617,9 +617,21 @@ constexpr int kCW = kPX / 2 + 2, kCH = kPY / 2 + 2;   // coarse nodes a prolonga
 // prolongation: fine nodes), segment; W = 0 marks padding work-groups
 constexpr int kMT = 9;
 
-inline size_t mg_restrict_lds(int lpr) { return (size_t)kRW * kRH * (lpr * sizeof(float4) + sizeof(float)); }
+// restriction tile in LDS: the staged residual and smoother weights, the
+// stencils (9 floats each) and constraint flags of the nodes whose residual
+// it restricts
+constexpr int kRIn = (2 * kRX + 1) * (2 * kRY + 1);
+constexpr int kRCf = kRIn * 9;
+constexpr int kRCm = (kRIn + 15) / 16 * 16;
+inline size_t mg_restrict_lds(int lpr) {
+    return (size_t)kRW * kRH * (lpr * sizeof(float4) + sizeof(float)) + kRCf * sizeof(float) + kRCm;
+}
+// prolongation tile in LDS: the staged fine and coarse vectors, the inner
+// nodes' stencils (9 floats each) and the staged nodes' constraint flags
+constexpr int kPCf = kPX * kPY * 9;
+constexpr int kPCm = (kPW * kPH + 15) / 16 * 16;
 inline size_t mg_prolong_lds(int lpr) {
-    const size_t tile = (size_t)(kPW * kPH + kCW * kCH) * lpr * sizeof(float4);
+    const size_t tile = (size_t)(kPW * kPH + kCW * kCH) * lpr * sizeof(float4) + kPCf * sizeof(float) + kPCm;
     const size_t red = (size_t)grid_block(lpr) * 4 * sizeof(double);
     return tile > red ? tile : red;
 }
@@ -661,10 +673,14 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_restrict(const int* __restric
                                                     const double* __restrict__ S, int ks) {
     extern __shared__ float4 sm[];
     float* sd = reinterpret_cast<float*>(sm + kRW * kRH * LPR);
+    float* cs = sd + kRW * kRH;                                          // stencils of the kIn nodes
+    unsigned char* cmk = reinterpret_cast<unsigned char*>(cs + kRCf);   // their constraint flags
     constexpr int ngrp = GB<LPR>::v / LPR;
-    constexpr int kIW = 2 * kRX + 1, kIn = kIW * (2 * kRY + 1);   // fine nodes whose residual is restricted
+    constexpr int kIW = 2 * kRX + 1, kIn = kRIn;   // fine nodes whose residual is restricted
     constexpr int kPer = (kIn + ngrp - 1) / ngrp;
     constexpr int kSt = (kRW * kRH + ngrp - 1) / ngrp;
+    constexpr int kCf = (kRCf + GB<LPR>::v - 1) / GB<LPR>::v;   // stencil floats per thread
+    constexpr int kCm = (kIn + GB<LPR>::v - 1) / GB<LPR>::v;    // flags per thread
     const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
     const int* d = tab + kMT * xcd_block(blockIdx.x, gridDim.x);
     const int fbase = d[0], W = d[1], H = d[2], cbase = d[3], Wc = d[4], Hc = d[5], cx0 = d[6], cy0 = d[7];
@@ -675,6 +691,25 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_restrict(const int* __restric
     const int lc = ok ? col0 : 0;
     const int fx0 = 2 * cx0 - 2, fy0 = 2 * cy0 - 2;
     {
+        // the stencils (contiguous rows of kIW nodes, 9 coefficients each) and
+        // constraint flags of the residual nodes load with the residual, so
+        // the residual phase waits on LDS only
+        float cv[kCf];
+        unsigned char cmv[kCm];
+#pragma unroll
+        for (int u = 0; u < kCf; ++u) {
+            const int i = threadIdx.x + u * GB<LPR>::v, ii = i / 9;
+            const int gx = fx0 + ii % kIW + 1, gy = fy0 + ii / kIW + 1;
+            const bool in = i < kRCf && gx >= 0 && gx < W && gy >= 0 && gy < H;
+            cv[u] = coef[in ? (size_t)(fbase + gy * W + gx) * 9 + i % 9 : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kCm; ++u) {
+            const int i = threadIdx.x + u * GB<LPR>::v;
+            const int gx = fx0 + i % kIW + 1, gy = fy0 + i / kIW + 1;
+            const bool in = cmask && i < kIn && gx >= 0 && gx < W && gy >= 0 && gy < H;
+            cmv[u] = in ? cmask[fbase + gy * W + gx] : 0;
+        }
         float4 v[kSt];
         float w[kSt];
         bool in[kSt];
@@ -695,38 +730,31 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_restrict(const int* __restric
                 if (lane == 0) sd[h] = in[u] ? w[u] : 0.f;
             }
         }
+#pragma unroll
+        for (int u = 0; u < kCf; ++u)
+            if (threadIdx.x + u * GB<LPR>::v < kRCf) cs[threadIdx.x + u * GB<LPR>::v] = cv[u];
+#pragma unroll
+        for (int u = 0; u < kCm; ++u)
+            if (threadIdx.x + u * GB<LPR>::v < kIn) cmk[threadIdx.x + u * GB<LPR>::v] = cmv[u];
     }
     __syncthreads();
     float4 res[kPer];
-    {
-        float c[kPer][9];
-        bool fr[kPer];
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int i = grp + u * ngrp;
-            const int hx = i % kIW + 1, hy = i / kIW + 1;
-            const int gx = fx0 + hx, gy = fy0 + hy;
-            const bool in = i < kIn && gx >= 0 && gx < W && gy >= 0 && gy < H;
-            const int nd = in ? fbase + gy * W + gx : 0;
-            fr[u] = in && !(cmask && cmask[nd]);
+    for (int u = 0; u < kPer; ++u) {
+        const int i = grp + u * ngrp;
+        const int hx = i % kIW + 1, hy = i / kIW + 1;
+        const int gx = fx0 + hx, gy = fy0 + hy;
+        float4 q = zero4();
+        if (i < kIn && gx >= 0 && gx < W && gy >= 0 && gy < H && !cmk[i]) {
+            const float* c = cs + i * 9;
+            q = sm[(hy * kRW + hx) * LPR + lane];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) c[u][k] = coef[(size_t)nd * 9 + k];
-        }
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int i = grp + u * ngrp;
-            const int hx = i % kIW + 1, hy = i / kIW + 1;
-            float4 q = zero4();
-            if (fr[u]) {
-                q = sm[(hy * kRW + hx) * LPR + lane];
-#pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    const int h = (hy + k / 3 - 1) * kRW + hx + k % 3 - 1;
-                    fma4(-c[u][k] * sd[h], sm[h * LPR + lane], q);
-                }
+            for (int k = 0; k < 9; ++k) {
+                const int h = (hy + k / 3 - 1) * kRW + hx + k % 3 - 1;
+                fma4(-c[k] * sd[h], sm[h * LPR + lane], q);
             }
-            res[u] = q;
         }
+        res[u] = q;
     }
     __syncthreads();
 #pragma unroll
@@ -773,9 +801,13 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_prolong(const int* __restrict
                                                    const double* __restrict__ S) {
     extern __shared__ float4 sm[];
     float4* sc = sm + kPW * kPH * LPR;
+    float* cs = reinterpret_cast<float*>(sc + kCW * kCH * LPR);    // inner nodes' stencils
+    unsigned char* cmk = reinterpret_cast<unsigned char*>(cs + kPCf);   // staged nodes' flags (level 0)
     constexpr int ngrp = GB<LPR>::v / LPR;
     constexpr int kCs = (kCW * kCH + ngrp - 1) / ngrp;   // staged coarse nodes per lane group
     constexpr int kSt = (kPW * kPH + ngrp - 1) / ngrp;   // staged fine nodes per lane group
+    constexpr int kCf = (kPCf + GB<LPR>::v - 1) / GB<LPR>::v;        // stencil floats per thread
+    constexpr int kCm = (kPW * kPH + GB<LPR>::v - 1) / GB<LPR>::v;   // flags per thread
     Tile t;
     t.lane = threadIdx.x % LPR;
     t.grp = threadIdx.x / LPR;
@@ -797,6 +829,26 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_prolong(const int* __restrict
     V4<double> rz = {0.0, 0.0, 0.0, 0.0}, rr = {0.0, 0.0, 0.0, 0.0};
     if (W > 0) {
         const int cx0 = tx0 / 2 - 1, cy0 = ty0 / 2 - 1;
+        // the inner nodes' stencils (kPY rows of kPX nodes, 9 coefficients per
+        // node: contiguous rows, coalesced) and the staged nodes' constraint
+        // flags are loaded with the vectors, so the later phases wait on LDS only
+        float cv[kCf];
+        unsigned char cmv[kCm];
+#pragma unroll
+        for (int u = 0; u < kCf; ++u) {
+            const int i = threadIdx.x + u * GB<LPR>::v;
+            const int row = i / (kPX * 9), off = i % (kPX * 9);
+            const int gx = tx0 + off / 9, gy = ty0 + row;
+            const bool in = i < kPCf && gx < W && gy < H;
+            cv[u] = coef[in ? (size_t)(fbase + gy * W + gx) * 9 + off % 9 : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kCm; ++u) {
+            const int h = threadIdx.x + u * GB<LPR>::v;
+            const int gx = tx0 - 1 + h % kPW, gy = ty0 - 1 + h / kPW;
+            const bool in = TOP && h < kPW * kPH && gx >= 0 && gx < W && gy >= 0 && gy < H;
+            cmv[u] = in ? cmask[fbase + gy * W + gx] : 0;
+        }
         {
             float4 v[kCs];
             bool in[kCs];
@@ -828,14 +880,20 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_prolong(const int* __restrict
             v[u] = ldf4(r + (size_t)(in[u] ? nd : 0) * kp + lc);
             w[u] = dsm[in[u] ? nd : 0];
         }
-        __syncthreads();   // the coarse values are staged
+#pragma unroll
+        for (int u = 0; u < kCf; ++u)
+            if (threadIdx.x + u * GB<LPR>::v < kPCf) cs[threadIdx.x + u * GB<LPR>::v] = cv[u];
+#pragma unroll
+        for (int u = 0; u < kCm; ++u)
+            if (threadIdx.x + u * GB<LPR>::v < kPW * kPH) cmk[threadIdx.x + u * GB<LPR>::v] = cmv[u];
+        __syncthreads();   // the coarse values, stencils and flags are staged
 #pragma unroll
         for (int u = 0; u < kSt; ++u) {
             const int h = t.grp + u * ngrp;
             if (h >= kPW * kPH) continue;
             const int gx = tx0 - 1 + h % kPW, gy = ty0 - 1 + h / kPW;
             float4 x = zero4();
-            if (in[u] && ok && !(TOP && cmask[fbase + gy * W + gx])) {
+            if (in[u] && ok && !(TOP && cmk[h])) {
                 x = make_float4(w[u] * v[u].x, w[u] * v[u].y, w[u] * v[u].z, w[u] * v[u].w);
                 int ax, ay;   // first coarse parents
                 float fx, fy;
@@ -856,14 +914,14 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_prolong(const int* __restrict
             if (!(in[u] && ok && h < kPW * kPH && hx >= 1 && hx <= kPX && hy >= 1 && hy <= kPY)) continue;
             const int ox = hx - 1, oy = hy - 1;
             const int nd = fbase + (ty0 + oy) * W + tx0 + ox;
-            const float* c = coef + (size_t)nd * 9;
+            const float* c = cs + (oy * kPX + ox) * 9;
             float4 q = v[u];
 #pragma unroll
             for (int k = 0; k < 9; ++k)
                 fma4(-c[k], sm[((oy + k / 3) * kPW + ox + k % 3) * LPR + t.lane], q);
             float4 z = sm[h * LPR + t.lane];
             fma4(w[u], q, z);
-            if (TOP && cmask[nd]) z = zero4();
+            if (TOP && cmk[h]) z = zero4();
             *reinterpret_cast<float4*>(e + (size_t)nd * kp + t.col0) = z;
             if constexpr (TOP) {
                 const float4 a = v[u];
