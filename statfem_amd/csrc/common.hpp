@@ -237,6 +237,15 @@ struct sfh_system {
     // the per-(node, neighbour) contribution lists, built at the first assembly
     sfh::DevBuf el_ke, el_cptr, el_cent;
     bool el_ready = false;
+    // declared 3D grid whose hex8 elements are its cells (checked once per
+    // grid declaration): the element of every cell, for the closed-form
+    // assembly; state 0 unchecked, 1 usable, -1 not
+    sfh::DevBuf g3cell;
+    int g3cell_state = 0;
+    // status word of the assembly kernels and its page-locked read-back
+    // (kept across calls: no allocation per assembly)
+    sfh::DevBuf asm_err;
+    sfh::PinnedBuf asm_host;
     sfh::DevBuf rowptr, colind, diagpos;
     sfh::DevBuf vals, vals32, F, dinv32, dinv64;
     sfh::DevBuf diag32, diag64;  // A_ii (the PCG keeps z = D^-1 r and needs both)

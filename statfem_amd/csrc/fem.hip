@@ -977,6 +977,265 @@ int build_pattern(sfh_system* s) {
     return SFH_OK;
 }
 
+// -------------------- hex8 on a declared 3D grid: closed-form cell integrals
+// When the elements are the cells of a declared tensor grid, every element is
+// an axis-aligned box and its trilinear shape functions factor per axis,
+// N_a = l_a(x) l_a(y) l_a(z), so every integral of the element forms is a
+// product of three 1D integrals over the cell's widths h:
+//   int l_i l_j = h/3 (same end of the cell) or h/6,   int l_i' l_j' = +-1/h,
+//   int l_i' l_j = -1/2 (i at the low end) or +1/2,
+// the values that Gauss quadrature with two or more points per axis gives
+// (the integrands are at most quadratic per axis). One thread per (node,
+// neighbour offset) sums the 3 x 3 block (elasticity) or the entry (scalar
+// forms) over the cells the two nodes share (1, 2, 4 or 8, in the order z, y,
+// x of the cell) and writes it into the node's rows, FP64 and FP32: no element
+// matrix goes through memory. Integrands as in k_elastic_ke and
+// k_hex_scalar_ke (elasticity: lambda d_c N_i d_k N_j + mu d_k N_i d_c N_j +
+// delta_ck mu grad N_i . grad N_j; Poisson: p1 grad . grad; Matern: p2 grad .
+// grad plus p1^2 int N_i on the diagonal).
+// A work-group takes kHgN consecutive nodes of one grid line. Phase 0: the
+// material and the widths (and their reciprocals) of the cells around them.
+// Phase 1: one thread per (node, cell around it, corner of that cell) forms
+// the 3 x 3 block (or entry) of the node and the corner over the cell, in LDS;
+// every lane of a wave does the same work. Phase 2: one thread per (node,
+// neighbour offset) sums the blocks of the cells the two nodes share, in a
+// fixed order (cell z, y, x), and writes them into the node's rows (staging
+// the rows in LDS for contiguous stores was measured slower: the extra LDS
+// halves the work-groups per CU).
+// nodes per work-group: 8 for elasticity (64 blocks of 9 doubles per node in
+// LDS), 32 for the scalar forms
+template <int PHYS>
+constexpr int hex_grid_nodes() { return PHYS == SFH_ELASTICITY ? 8 : 32; }
+template <int PHYS>
+__global__ __launch_bounds__(256) void k_hex_grid(int nx, int ny, int nz, const double* __restrict__ gx,
+                                                  const double* __restrict__ gy, const double* __restrict__ gz,
+                                                  const int* __restrict__ cell, const double* __restrict__ p1,
+                                                  const double* __restrict__ p2, double s1, double s2, double factor,
+                                                  const int* __restrict__ rowptr, double* __restrict__ vals,
+                                                  float* __restrict__ vals32, int* err) {
+    constexpr int NV = PHYS == SFH_ELASTICITY ? 9 : 1;
+    constexpr int kHgN = hex_grid_nodes<PHYS>();
+    constexpr int kCx = kHgN + 1;                         // cells along x around the work-group's nodes
+    __shared__ double blk[kHgN * 64 * NV];               // (node, cell slot, corner) -> block
+    __shared__ double mat[2][2][2][kCx];                 // (lambda, mu) or (p1, p2) of cell (z, y, x slot)
+    __shared__ double hw[kCx + 4], rw[kCx + 4];          // widths, reciprocals: x cells, then y, z pairs
+    const int n[3] = {nx, ny, nz};
+    const int lx0 = blockIdx.x * kHgN, nl = min(kHgN, nx - lx0);
+    const int iy = blockIdx.y, iz = blockIdx.z;
+    const int tid = threadIdx.x;
+    // phase 0: cell (lx0 - 1 + x slot, iy - 1 + y slot, iz - 1 + z slot)
+    if (tid < 4 * kCx) {
+        const int xs = tid % kCx, ys = (tid / kCx) & 1, zs = tid / (2 * kCx);
+        const int cx = lx0 - 1 + xs, cy = iy - 1 + ys, cz = iz - 1 + zs;
+        double m0 = 0.0, m1 = 0.0;
+        if (cx >= 0 && cx <= nx - 2 && cy >= 0 && cy <= ny - 2 && cz >= 0 && cz <= nz - 2) {
+            const int e = (p1 || p2) ? cell[((size_t)cz * (ny - 1) + cy) * (nx - 1) + cx] : 0;
+            const double a = p1 ? p1[e] : s1, b = p2 ? p2[e] : s2;
+            if constexpr (PHYS == SFH_ELASTICITY) {
+                m0 = a * b / (1.0 + b) / (1.0 - 2.0 * b);   // SVenant.ipp:32-37
+                m1 = a / 2.0 / (1.0 + b);
+            } else {
+                m0 = a;
+                m1 = b;
+            }
+        }
+        mat[0][zs][ys][xs] = m0;
+        mat[1][zs][ys][xs] = m1;
+    } else if (tid < 4 * kCx + kCx + 4) {
+        const int w = tid - 4 * kCx;
+        const int ax = w < kCx ? 0 : (w < kCx + 2 ? 1 : 2);
+        const int c = ax == 0 ? lx0 - 1 + w : (ax == 1 ? iy - 1 + (w - kCx) : iz - 1 + (w - kCx - 2));
+        const double* g = ax == 0 ? gx : (ax == 1 ? gy : gz);
+        double h = 0.0, r = 0.0;
+        if (c >= 0 && c <= n[ax] - 2) {
+            h = g[c + 1] - g[c];
+            if (!(h > 0.0)) atomicOr(err, 4);
+            r = 1.0 / h;
+        }
+        hw[w] = h;
+        rw[w] = r;
+    }
+    // the row start of this thread's first phase-2 node, loaded before the
+    // phases that only wait on LDS
+    const int rp_first = tid < nl * 27
+                             ? rowptr[(NV == 9 ? 3 : 1) * (lx0 + tid / 27 + nx * (iy + ny * iz))]
+                             : 0;
+    __syncthreads();
+    // phase 1: node il, cell slot cs (bit a: the cell above the node along
+    // axis a), corner cn (bit a: the cell's upper node along a)
+    for (int b = tid; b < nl * 64; b += blockDim.x) {
+        const int il = b >> 6, cs = (b >> 3) & 7, cn = b & 7;
+        const int ix = lx0 + il;
+        const int sl[3] = {cs & 1, (cs >> 1) & 1, cs >> 2};
+        const int ii[3] = {ix, iy, iz};
+        bool ok = true;
+        double M[3], D[3], Gi[3], Gj[3], vol = 1.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int c = ii[a] - 1 + sl[a];
+            ok = ok && c >= 0 && c <= n[a] - 2;
+            const int w = a == 0 ? il + sl[0] : kCx + 2 * (a - 1) + sl[a];
+            const double h = hw[w], r = rw[w];
+            const int si = 1 - sl[a], tj = (cn >> a) & 1;   // the node's and the corner's end of the cell
+            M[a] = si == tj ? h * (1.0 / 3.0) : h * (1.0 / 6.0);
+            D[a] = si == tj ? r : -r;
+            Gi[a] = si ? 0.5 : -0.5;
+            Gj[a] = tj ? 0.5 : -0.5;
+            vol *= h;
+        }
+        if (!ok) continue;
+        // I[c][k] = int d_c N_i d_k N_j: the derivative axes' stiffness or
+        // mixed factors times the other axes' mass
+        double I[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                I[c][k] = c == k ? D[c] * M[(c + 1) % 3] * M[(c + 2) % 3] : Gi[c] * Gj[k] * M[3 - c - k];
+        const double tr = I[0][0] + I[1][1] + I[2][2];
+        const double m0 = mat[0][sl[2]][sl[1]][il + sl[0]], m1 = mat[1][sl[2]][sl[1]][il + sl[0]];
+        double* out = blk + (size_t)b * NV;
+        if constexpr (PHYS == SFH_ELASTICITY) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    double v = m0 * I[c][k] + m1 * I[k][c];
+                    if (k == c) v += m1 * tr;
+                    out[c * 3 + k] = v;
+                }
+        } else if constexpr (PHYS == SFH_POISSON) {
+            out[0] = m0 * tr;                                                   // ElementPotential.ipp:71-72
+        } else {
+            double v = m1 * tr;                                                 // ElementMatern.ipp:68-79
+            if (cn == 7 - cs) v += m0 * m0 * (vol * 0.125);                     // the node itself: lumped mass
+            out[0] = v;
+        }
+    }
+    __syncthreads();
+    // phase 2: node il, neighbour offset o (x fastest)
+    for (int b = tid; b < nl * 27; b += blockDim.x) {
+        const int il = b / 27, o = b - 27 * il;
+        const int ix = lx0 + il;
+        const int ii[3] = {ix, iy, iz};
+        const int off[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+        int cnt[3], bef[3], s0[3], s1v[3];
+        bool in = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int j = ii[a] + off[a];
+            in = in && j >= 0 && j < n[a];
+            cnt[a] = (ii[a] > 0) + 1 + (ii[a] < n[a] - 1);      // neighbour offsets in range on this axis
+            bef[a] = (off[a] >= 0 && ii[a] > 0) + (off[a] >= 1);  // of them, those below off[a]
+            // cell slots sharing the pair along a: {off} for off = -1 / +1 (slot 0 / 1), both for 0,
+            // limited to cells inside the grid
+            s0[a] = off[a] > 0 ? 1 : 0;
+            s1v[a] = off[a] < 0 ? 0 : 1;
+            if (ii[a] == 0) s0[a] = max(s0[a], 1);
+            if (ii[a] == n[a] - 1) s1v[a] = min(s1v[a], 0);
+        }
+        if (!in) continue;
+        double acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+        for (int sz = s0[2]; sz <= s1v[2]; ++sz)
+            for (int sy = s0[1]; sy <= s1v[1]; ++sy)
+                for (int sx = s0[0]; sx <= s1v[0]; ++sx) {
+                    const int cs = sx | (sy << 1) | (sz << 2);
+                    const int cn = (off[0] + 1 - sx) | ((off[1] + 1 - sy) << 1) | ((off[2] + 1 - sz) << 2);
+                    const double* src = blk + (size_t)((il * 8 + cs) * 8 + cn) * NV;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[v] += src[v];
+                }
+        const int i = ix + nx * (iy + ny * iz);
+        const int len = cnt[0] * cnt[1] * cnt[2];
+        const int q = (bef[2] * cnt[1] + bef[1]) * cnt[0] + bef[0];   // the neighbour's slot in the sorted row
+        const int rs = b == tid ? rp_first : rowptr[(NV == 9 ? 3 : 1) * i];
+        if constexpr (PHYS == SFH_ELASTICITY) {
+            const size_t r0 = (size_t)rs;   // rows 3i + c start at r0 + c * 3 len
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const size_t pos = r0 + (size_t)c * 3 * len + 3 * q + k;
+                    const double v = acc[c * 3 + k] * factor;
+                    vals[pos] = v;
+                    vals32[pos] = (float)v;
+                }
+        } else {
+            const size_t pos = (size_t)rs + q;
+            const double v = acc[0] * factor;
+            vals[pos] = v;
+            vals32[pos] = (float)v;
+        }
+    }
+}
+
+// Whether the hex8 elements are the cells of the declared 3D grid (each cell
+// exactly once, in any element order, local nodes in the reference order) and
+// every node's row holds exactly its
+// in-range 27-point neighbours in increasing order: then the cell -> element
+// table goes to the device and k_hex_grid applies. Checked once per grid
+// declaration.
+int hex_grid_setup(sfh_system* s, hipStream_t st) {
+    if (s->g3cell_state != 0) return SFH_OK;
+    s->g3cell_state = -1;
+    const int nx = s->gn[0], ny = s->gn[1], nz = s->gn[2];
+    if (s->dim != 3 || s->kind != SFH_HEX8 || nx < 2 || ny < 2 || nz < 2 ||
+        (long long)nx * ny * nz != s->nnodes || (long long)(nx - 1) * (ny - 1) * (nz - 1) != s->nel ||
+        (long long)s->nnodes * 27 >= INT32_MAX || ny > 65535 || nz > 65535)
+        return SFH_OK;
+    std::vector<int> conn((size_t)s->nel * 8), rpn((size_t)s->nnodes + 1), cin((size_t)s->nnz_node);
+    SFH_CHECK_HIP(hipMemcpyAsync(conn.data(), s->conn.p, conn.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(rpn.data(), s->rowptr_node.p, rpn.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(cin.data(), s->colind_node.p, cin.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    std::vector<int> cell((size_t)s->nel, -1);
+    for (int e = 0; e < s->nel; ++e) {
+        int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX};
+        int xyz[8][3];
+        for (int k = 0; k < 8; ++k) {
+            const int v = conn[(size_t)e * 8 + k];
+            if (v < 0 || v >= s->nnodes) return SFH_OK;
+            xyz[k][0] = v % nx;
+            xyz[k][1] = (v / nx) % ny;
+            xyz[k][2] = v / (nx * ny);
+            for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], xyz[k][a]);
+        }
+        // the cell's corners in the hex8 reference order (the counter-
+        // clockwise bottom face, then the top face): the elements the
+        // quadrature would integrate with a positive Jacobian
+        constexpr int kCorner[8] = {0, 1, 3, 2, 4, 5, 7, 6};   // dx + 2 dy + 4 dz of local node k
+        for (int k = 0; k < 8; ++k) {
+            const int dx = xyz[k][0] - lo[0], dy = xyz[k][1] - lo[1], dz = xyz[k][2] - lo[2];
+            if (dx > 1 || dy > 1 || dz > 1 || dx + 2 * dy + 4 * dz != kCorner[k]) return SFH_OK;
+        }
+        if (lo[0] > nx - 2 || lo[1] > ny - 2 || lo[2] > nz - 2) return SFH_OK;
+        const size_t c = ((size_t)lo[2] * (ny - 1) + lo[1]) * (nx - 1) + lo[0];
+        if (cell[c] >= 0) return SFH_OK;
+        cell[c] = e;
+    }
+    for (int iz = 0, i = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix, ++i) {
+                int p = rpn[i];
+                for (int oz = -1; oz <= 1; ++oz)
+                    for (int oy = -1; oy <= 1; ++oy)
+                        for (int ox = -1; ox <= 1; ++ox) {
+                            const int jx = ix + ox, jy = iy + oy, jz = iz + oz;
+                            if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz) continue;
+                            if (p >= rpn[i + 1] || cin[p] != jx + nx * (jy + ny * jz)) return SFH_OK;
+                            ++p;
+                        }
+                if (p != rpn[i + 1]) return SFH_OK;
+            }
+    SFH_TRY(s->g3cell.ensure(sizeof(int) * cell.size()));
+    SFH_CHECK_HIP(hipMemcpyAsync(s->g3cell.p, cell.data(), sizeof(int) * cell.size(), hipMemcpyHostToDevice, st));
+    SFH_CHECK_HIP(hipStreamSynchronize(st));
+    s->g3cell_state = 1;
+    return SFH_OK;
+}
+
 // contribution lists of the two-phase hex8 assemblies, built once per system
 int elastic_contrib_setup(sfh_system* s, hipStream_t st) {
     if (s->el_ready) return SFH_OK;
@@ -1007,21 +1266,37 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
     hipStream_t st = s->ctx->stream;
     if (nq1d < 1 || nq1d > 3) { set_error("nq1d must be 1..3"); return SFH_EARG; }
     if (s->kind == SFH_TRI3 && nq1d != 1) { set_error("TRI3 uses the 1-point rule"); return SFH_EARG; }
-    DevBuf err;
-    SFH_TRY(err.ensure(sizeof(int)));
-    SFH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    SFH_TRY(s->asm_err.ensure(sizeof(int)));
+    SFH_TRY(s->asm_host.ensure(sizeof(int)));
+    int* errp = s->asm_err.as<int>();
+    SFH_CHECK_HIP(hipMemsetAsync(errp, 0, sizeof(int), st));
     const int* c = s->conn.as<int>();
     const double* X = s->coords.as<double>();
+    // hex8 cells of a declared 3D grid, integrated exactly by the closed form
+    // when the rule has two or more points per axis
+    bool grid_cells = false;
+    if (nq1d >= 2 && s->kind == SFH_HEX8 && s->dim == 3) {
+        SFH_TRY(hex_grid_setup(s, st));
+        grid_cells = s->g3cell_state == 1;
+    }
     if (physics == SFH_ELASTICITY) {
         if (s->kind != SFH_HEX8 || s->dofpn != 3) {
             set_error("elasticity needs HEX8 elements with 3 DOFs per node");
             return SFH_EARG;
         }
-        if (s->nel < (1 << 25) && 64LL * s->nel < INT32_MAX && s->nnz_node < INT32_MAX) {
+        if (grid_cells) {
+            constexpr int nw = hex_grid_nodes<SFH_ELASTICITY>();
+            const dim3 gb((unsigned)((s->gn[0] + nw - 1) / nw), (unsigned)s->gn[1], (unsigned)s->gn[2]);
+            k_hex_grid<SFH_ELASTICITY><<<gb, 256, 0, st>>>(
+                s->gn[0], s->gn[1], s->gn[2], s->gaxis[0].as<double>(), s->gaxis[1].as<double>(),
+                s->gaxis[2].as<double>(), s->g3cell.as<int>(), p1, p2, s1, s2, factor, s->rowptr.as<int>(),
+                s->vals.as<double>(), s->vals32.as<float>(), errp);
+            SFH_LAUNCH_CHECK();
+        } else if (s->nel < (1 << 25) && 64LL * s->nel < INT32_MAX && s->nnz_node < INT32_MAX) {
             SFH_TRY(elastic_contrib_setup(s, st));
             SFH_TRY(s->el_ke.ensure(sizeof(double) * kKeStride * (size_t)s->nel));
             k_elastic_ke<<<(unsigned)((s->nel + kElPerBlk - 1) / kElPerBlk), 256, 0, st>>>(
-                s->nel, nq1d, X, c, p1, p2, s1, s2, factor, s->el_ke.as<double>(), err.as<int>());
+                s->nel, nq1d, X, c, p1, p2, s1, s2, factor, s->el_ke.as<double>(), errp);
             SFH_LAUNCH_CHECK();
             k_elastic_gather<<<(unsigned)(((s->nnodes + 7) / 8 + 7) / 8 * 8), 256, 0, st>>>(
                 s->nnodes, s->rowptr_node.as<int>(), s->el_cptr.as<int>(), s->el_cent.as<int>(),
@@ -1033,9 +1308,26 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
             k_assemble_elastic<<<grid_for(s->ndof, kBlockEl), kBlockEl, lds, st>>>(
                 s->ndof, nq1d, X, c, s->n2e_ptr.as<int>(), s->n2e.as<int>(), s->emap.as<unsigned char>(),
                 s->rowptr.as<int>(), p1, p2, s1, s2, factor, s->vals.as<double>(), s->vals32.as<float>(),
-                err.as<int>());
+                errp);
             SFH_LAUNCH_CHECK();
         }
+    } else if ((physics == SFH_POISSON || physics == SFH_MATERN) && s->kind == SFH_HEX8 && s->dofpn == 1 &&
+               grid_cells) {
+        constexpr int nw = hex_grid_nodes<SFH_POISSON>();   // (the same for SFH_MATERN)
+        const dim3 gb((unsigned)((s->gn[0] + nw - 1) / nw), (unsigned)s->gn[1], (unsigned)s->gn[2]);
+        if (physics == SFH_POISSON)
+            k_hex_grid<SFH_POISSON><<<gb, 256, 0, st>>>(s->gn[0], s->gn[1], s->gn[2], s->gaxis[0].as<double>(),
+                                                         s->gaxis[1].as<double>(), s->gaxis[2].as<double>(),
+                                                         s->g3cell.as<int>(), p1, p2, s1, s2, factor,
+                                                         s->rowptr.as<int>(), s->vals.as<double>(),
+                                                         s->vals32.as<float>(), errp);
+        else
+            k_hex_grid<SFH_MATERN><<<gb, 256, 0, st>>>(s->gn[0], s->gn[1], s->gn[2], s->gaxis[0].as<double>(),
+                                                        s->gaxis[1].as<double>(), s->gaxis[2].as<double>(),
+                                                        s->g3cell.as<int>(), p1, p2, s1, s2, factor,
+                                                        s->rowptr.as<int>(), s->vals.as<double>(),
+                                                        s->vals32.as<float>(), errp);
+        SFH_LAUNCH_CHECK();
     } else if ((physics == SFH_POISSON || physics == SFH_MATERN) && s->kind == SFH_HEX8 && s->dofpn == 1 &&
                s->nel < (1 << 25) && 64LL * s->nel < INT32_MAX) {
         SFH_TRY(elastic_contrib_setup(s, st));
@@ -1043,10 +1335,10 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
         const unsigned ge = (unsigned)((s->nel + kElPerBlk - 1) / kElPerBlk);
         if (physics == SFH_POISSON)
             k_hex_scalar_ke<SFH_POISSON><<<ge, 256, 0, st>>>(s->nel, nq1d, X, c, p1, p2, s1, s2, factor,
-                                                              s->el_ke.as<double>(), err.as<int>());
+                                                              s->el_ke.as<double>(), errp);
         else
             k_hex_scalar_ke<SFH_MATERN><<<ge, 256, 0, st>>>(s->nel, nq1d, X, c, p1, p2, s1, s2, factor,
-                                                             s->el_ke.as<double>(), err.as<int>());
+                                                             s->el_ke.as<double>(), errp);
         SFH_LAUNCH_CHECK();
         k_hex_scalar_gather<<<(unsigned)(((s->nnz + 255) / 256 + 7) / 8 * 8), 256, 0, st>>>(
             (int)s->nnz, s->el_cptr.as<int>(), s->el_cent.as<int>(), s->el_ke.as<double>(), s->vals.as<double>(),
@@ -1061,7 +1353,7 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
     k_assemble_scalar<K, P><<<g, kBlock, lds, st>>>(                                            \
         s->nnodes, nq1d, X, c, s->n2e_ptr.as<int>(), s->n2e.as<int>(), s->emap.as<unsigned char>(), \
         s->rowptr.as<int>(), p1, p2, s1, s2, factor, s->maxrow, s->vals.as<double>(),          \
-        s->vals32.as<float>(), err.as<int>())
+        s->vals32.as<float>(), errp)
         if (physics == SFH_POISSON) {
             switch (s->kind) {
                 case SFH_LINE2: SFH_ASM(SFH_LINE2, SFH_POISSON); break;
@@ -1083,9 +1375,9 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
         set_error("unknown physics %d", physics);
         return SFH_EARG;
     }
-    int herr = 0;
-    SFH_CHECK_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(s->asm_host.p, errp, sizeof(int), hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
+    const int herr = *s->asm_host.as<int>();
     if (herr & 4) { set_error("Negative Jacobian."); return SFH_ENEGJAC; }
     SFH_CHECK_HIP(hipMemsetAsync(s->cmask.p, 0, s->ndof, st));
     s->nc = 0;
@@ -1099,9 +1391,10 @@ int assemble_rhs(sfh_system* s, int nq1d, const double* fconst, const double* fn
                  int accumulate, double* out) {
     hipStream_t st = s->ctx->stream;
     if (nq1d < 1 || nq1d > 3) { set_error("nq1d must be 1..3"); return SFH_EARG; }
-    DevBuf err;
-    SFH_TRY(err.ensure(sizeof(int)));
-    SFH_CHECK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    SFH_TRY(s->asm_err.ensure(sizeof(int)));
+    SFH_TRY(s->asm_host.ensure(sizeof(int)));
+    int* errp = s->asm_err.as<int>();
+    SFH_CHECK_HIP(hipMemsetAsync(errp, 0, sizeof(int), st));
     const double f0 = fconst ? fconst[0] : 0.0;
     const double f1 = (fconst && s->dofpn > 1) ? fconst[1] : 0.0;
     const double f2 = (fconst && s->dofpn > 2) ? fconst[2] : 0.0;
@@ -1109,7 +1402,7 @@ int assemble_rhs(sfh_system* s, int nq1d, const double* fconst, const double* fn
 #define SFH_BF(K)                                                                                 \
     k_body_force<K><<<g, kBlock, 0, st>>>(s->nnodes, nq1d, s->dofpn, s->coords.as<double>(),      \
                                           s->conn.as<int>(), s->n2e_ptr.as<int>(), s->n2e.as<int>(), \
-                                          f0, f1, f2, fnodal, factor, accumulate, out, err.as<int>())
+                                          f0, f1, f2, fnodal, factor, accumulate, out, errp)
     switch (s->kind) {
         case SFH_LINE2: SFH_BF(SFH_LINE2); break;
         case SFH_TRI3: SFH_BF(SFH_TRI3); break;
@@ -1118,9 +1411,9 @@ int assemble_rhs(sfh_system* s, int nq1d, const double* fconst, const double* fn
     }
 #undef SFH_BF
     SFH_LAUNCH_CHECK();
-    int herr = 0;
-    SFH_CHECK_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemcpyAsync(s->asm_host.p, errp, sizeof(int), hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
+    const int herr = *s->asm_host.as<int>();
     if (herr & 4) { set_error("Negative Jacobian."); return SFH_ENEGJAC; }
     return SFH_OK;
 }
