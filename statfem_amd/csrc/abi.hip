@@ -433,7 +433,7 @@ int sfh_system_set_grid(sfh_system* s, int nx, int ny, int nz, const double* gx,
         s->gn[a] = n[a];
     }
     s->dia_state = 0;
-    s->g3cell_state = 0;
+    s->gcell_state = 0;
     SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
     return SFH_OK;
 }

@@ -237,11 +237,13 @@ struct sfh_system {
     // the per-(node, neighbour) contribution lists, built at the first assembly
     sfh::DevBuf el_ke, el_cptr, el_cent;
     bool el_ready = false;
-    // declared 3D grid whose hex8 elements are its cells (checked once per
-    // grid declaration): the element of every cell, for the closed-form
-    // assembly; state 0 unchecked, 1 usable, -1 not
-    sfh::DevBuf g3cell;
-    int g3cell_state = 0;
+    // declared 2D or 3D grid whose quad4 / hex8 elements are its cells
+    // (checked once per grid declaration): the element of every cell, for the
+    // closed-form assembly (empty when the elements are in cell order); state
+    // 0 unchecked, 1 usable, -1 not
+    sfh::DevBuf gcell;
+    int gcell_state = 0;
+    bool gcell_identity = false;
     // status word of the assembly kernels and its page-locked read-back
     // (kept across calls: no allocation per assembly)
     sfh::DevBuf asm_err;

@@ -977,6 +977,108 @@ int build_pattern(sfh_system* s) {
     return SFH_OK;
 }
 
+// -------------------- quad4 on a declared 2D grid: closed-form cell integrals
+// The 2D case of the hex8 kernel below: rectangles, N_a = l_a(x) l_a(y),
+// grad N_i . grad N_j integrated as D_x M_y + M_x D_y (D = int l_i' l_j' =
+// +-1/h, M = int l_i l_j = h/3 or h/6), exact for Gauss rules of two or more
+// points per axis. One thread per node forms its row (the 9-point
+// neighbourhood, each entry summed over the shared cells in the order y, x of
+// the cell) in LDS; the work-group's nodes (a run of one grid line) own one
+// contiguous range of the CSR values, written out coalesced, FP64 and FP32.
+// Poisson: p1 grad . grad; Matern: p2 grad . grad plus p1^2 int N_i on the
+// diagonal (the lumped mass, a quarter of the cell's area).
+constexpr int kQgB = 256;
+template <int PHYS>
+__global__ __launch_bounds__(kQgB) void k_quad_grid(int nx, int ny, const double* __restrict__ gx,
+                                                    const double* __restrict__ gy, const int* __restrict__ cell,
+                                                    const double* __restrict__ p1, const double* __restrict__ p2,
+                                                    double s1, double s2, double factor,
+                                                    const int* __restrict__ rowptr, double* __restrict__ vals,
+                                                    float* __restrict__ vals32, int* err) {
+    __shared__ double rows[kQgB * 9];
+    const int iy = blockIdx.y, lx0 = blockIdx.x * kQgB, nl = min(kQgB, nx - lx0);
+    const int i0 = lx0 + nx * iy;
+    const int base = rowptr[i0], end = rowptr[i0 + nl];
+    const int ix = lx0 + threadIdx.x;
+    if (threadIdx.x < nl) {
+        const int ii[2] = {ix, iy}, n[2] = {nx, ny};
+        const double* g[2] = {gx, gy};
+        // per axis: the cell below (0) and above (1) the node, whether it
+        // exists, its width and reciprocal width
+        bool has[2][2];
+        double h[2][2], r[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int c = ii[a] - 1 + u;
+                has[a][u] = c >= 0 && c <= n[a] - 2;
+                const int cc = has[a][u] ? c : 0;
+                h[a][u] = g[a][cc + 1] - g[a][cc];
+                if (has[a][u] && !(h[a][u] > 0.0)) atomicOr(err, 4);
+                r[a][u] = 1.0 / h[a][u];
+            }
+        double k1[2][2], k2[2][2];   // material of cell (y slot, x slot)
+#pragma unroll
+        for (int uy = 0; uy < 2; ++uy)
+#pragma unroll
+            for (int ux = 0; ux < 2; ++ux) {
+                const bool in = has[1][uy] && has[0][ux];
+                const int c = in ? (iy - 1 + uy) * (nx - 1) + ix - 1 + ux : 0;
+                const int e = cell ? cell[c] : c;
+                k1[uy][ux] = p1 ? p1[e] : s1;
+                k2[uy][ux] = p2 ? p2[e] : s2;
+            }
+        double* out = rows + (rowptr[ix + nx * iy] - base);
+        int q = 0;
+#pragma unroll
+        for (int oy = -1; oy <= 1; ++oy)
+#pragma unroll
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int off[2] = {ox, oy};
+                bool inr = true;
+#pragma unroll
+                for (int a = 0; a < 2; ++a) inr = inr && ii[a] + off[a] >= 0 && ii[a] + off[a] < n[a];
+                if (!inr) continue;
+                double acc = 0.0;
+#pragma unroll
+                for (int uy = 0; uy < 2; ++uy)
+#pragma unroll
+                    for (int ux = 0; ux < 2; ++ux) {
+                        // cell slot u shares the pair along an axis when the
+                        // neighbour lies in it: offset -1 needs the cell below,
+                        // +1 the cell above, 0 either
+                        const int u[2] = {ux, uy};
+                        bool sh = true;
+                        double D[2], Mm[2];
+#pragma unroll
+                        for (int a = 0; a < 2; ++a) {
+                            sh = sh && has[a][u[a]] && (off[a] == 0 || off[a] == 2 * u[a] - 1);
+                            const bool same = off[a] == 0;   // node and neighbour at the same end of the cell
+                            D[a] = same ? r[a][u[a]] : -r[a][u[a]];
+                            Mm[a] = same ? h[a][u[a]] * (1.0 / 3.0) : h[a][u[a]] * (1.0 / 6.0);
+                        }
+                        if (!sh) continue;
+                        const double tr = D[0] * Mm[1] + Mm[0] * D[1];
+                        if constexpr (PHYS == SFH_POISSON) {
+                            acc += k1[uy][ux] * tr;                                  // ElementPotential.ipp:71-72
+                        } else {
+                            acc += k2[uy][ux] * tr;                                  // ElementMatern.ipp:68-79
+                            if (ox == 0 && oy == 0)
+                                acc += k1[uy][ux] * k1[uy][ux] * (h[0][ux] * h[1][uy] * 0.25);
+                        }
+                    }
+                out[q++] = acc;
+            }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < end - base; p += kQgB) {
+        const double v = rows[p] * factor;
+        vals[(size_t)base + p] = v;
+        vals32[(size_t)base + p] = (float)v;
+    }
+}
+
 // -------------------- hex8 on a declared 3D grid: closed-form cell integrals
 // When the elements are the cells of a declared tensor grid, every element is
 // an axis-aligned box and its trilinear shape functions factor per axis,
@@ -1029,7 +1131,8 @@ __global__ __launch_bounds__(256) void k_hex_grid(int nx, int ny, int nz, const 
         const int cx = lx0 - 1 + xs, cy = iy - 1 + ys, cz = iz - 1 + zs;
         double m0 = 0.0, m1 = 0.0;
         if (cx >= 0 && cx <= nx - 2 && cy >= 0 && cy <= ny - 2 && cz >= 0 && cz <= nz - 2) {
-            const int e = (p1 || p2) ? cell[((size_t)cz * (ny - 1) + cy) * (nx - 1) + cx] : 0;
+            const int c = (int)(((size_t)cz * (ny - 1) + cy) * (nx - 1) + cx);
+            const int e = (p1 || p2) && cell ? cell[c] : c;
             const double a = p1 ? p1[e] : s1, b = p2 ? p2[e] : s2;
             if constexpr (PHYS == SFH_ELASTICITY) {
                 m0 = a * b / (1.0 + b) / (1.0 - 2.0 * b);   // SVenant.ipp:32-37
@@ -1171,55 +1274,61 @@ __global__ __launch_bounds__(256) void k_hex_grid(int nx, int ny, int nz, const 
     }
 }
 
-// Whether the hex8 elements are the cells of the declared 3D grid (each cell
-// exactly once, in any element order, local nodes in the reference order) and
-// every node's row holds exactly its
-// in-range 27-point neighbours in increasing order: then the cell -> element
-// table goes to the device and k_hex_grid applies. Checked once per grid
-// declaration.
-int hex_grid_setup(sfh_system* s, hipStream_t st) {
-    if (s->g3cell_state != 0) return SFH_OK;
-    s->g3cell_state = -1;
-    const int nx = s->gn[0], ny = s->gn[1], nz = s->gn[2];
-    if (s->dim != 3 || s->kind != SFH_HEX8 || nx < 2 || ny < 2 || nz < 2 ||
-        (long long)nx * ny * nz != s->nnodes || (long long)(nx - 1) * (ny - 1) * (nz - 1) != s->nel ||
+// Whether the elements are the cells of the declared grid (quad4 on a 2D
+// grid, hex8 on a 3D grid; every cell exactly once, in any element order,
+// local nodes in the reference order, i.e. the orientation the quadrature
+// integrates with a positive Jacobian) and every node's row holds exactly its
+// in-range 9- or 27-point neighbours in increasing order: then k_quad_grid /
+// k_hex_grid apply, with the cell -> element table on the device (none when
+// the elements are in cell order). Checked once per grid declaration.
+int grid_cells_setup(sfh_system* s, hipStream_t st) {
+    if (s->gcell_state != 0) return SFH_OK;
+    s->gcell_state = -1;
+    const int d = s->dim;
+    const int nx = s->gn[0], ny = s->gn[1], nz = d == 3 ? s->gn[2] : 1;
+    const int nen = d == 3 ? 8 : 4;
+    if (!((d == 3 && s->kind == SFH_HEX8) || (d == 2 && s->kind == SFH_QUAD4)) || nx < 2 || ny < 2 ||
+        (d == 3 && nz < 2) || (long long)nx * ny * nz != s->nnodes ||
+        (long long)(nx - 1) * (ny - 1) * (d == 3 ? nz - 1 : 1) != s->nel ||
         (long long)s->nnodes * 27 >= INT32_MAX || ny > 65535 || nz > 65535)
         return SFH_OK;
-    std::vector<int> conn((size_t)s->nel * 8), rpn((size_t)s->nnodes + 1), cin((size_t)s->nnz_node);
+    std::vector<int> conn((size_t)s->nel * nen), rpn((size_t)s->nnodes + 1), cin((size_t)s->nnz_node);
     SFH_CHECK_HIP(hipMemcpyAsync(conn.data(), s->conn.p, conn.size() * sizeof(int), hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipMemcpyAsync(rpn.data(), s->rowptr_node.p, rpn.size() * sizeof(int), hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipMemcpyAsync(cin.data(), s->colind_node.p, cin.size() * sizeof(int), hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
+    // the cell's corners in the reference order (counter-clockwise, then the
+    // top face of a hex8): dx + 2 dy + 4 dz of local node k
+    constexpr int kCorner[8] = {0, 1, 3, 2, 4, 5, 7, 6};
     std::vector<int> cell((size_t)s->nel, -1);
+    bool identity = true;
     for (int e = 0; e < s->nel; ++e) {
         int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX};
         int xyz[8][3];
-        for (int k = 0; k < 8; ++k) {
-            const int v = conn[(size_t)e * 8 + k];
+        for (int k = 0; k < nen; ++k) {
+            const int v = conn[(size_t)e * nen + k];
             if (v < 0 || v >= s->nnodes) return SFH_OK;
             xyz[k][0] = v % nx;
             xyz[k][1] = (v / nx) % ny;
             xyz[k][2] = v / (nx * ny);
             for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], xyz[k][a]);
         }
-        // the cell's corners in the hex8 reference order (the counter-
-        // clockwise bottom face, then the top face): the elements the
-        // quadrature would integrate with a positive Jacobian
-        constexpr int kCorner[8] = {0, 1, 3, 2, 4, 5, 7, 6};   // dx + 2 dy + 4 dz of local node k
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < nen; ++k) {
             const int dx = xyz[k][0] - lo[0], dy = xyz[k][1] - lo[1], dz = xyz[k][2] - lo[2];
             if (dx > 1 || dy > 1 || dz > 1 || dx + 2 * dy + 4 * dz != kCorner[k]) return SFH_OK;
         }
-        if (lo[0] > nx - 2 || lo[1] > ny - 2 || lo[2] > nz - 2) return SFH_OK;
+        if (lo[0] > nx - 2 || lo[1] > ny - 2 || (d == 3 && lo[2] > nz - 2)) return SFH_OK;
         const size_t c = ((size_t)lo[2] * (ny - 1) + lo[1]) * (nx - 1) + lo[0];
         if (cell[c] >= 0) return SFH_OK;
         cell[c] = e;
+        identity = identity && (size_t)e == c;
     }
+    const int oz0 = d == 3 ? -1 : 0, oz1 = d == 3 ? 1 : 0;
     for (int iz = 0, i = 0; iz < nz; ++iz)
         for (int iy = 0; iy < ny; ++iy)
             for (int ix = 0; ix < nx; ++ix, ++i) {
                 int p = rpn[i];
-                for (int oz = -1; oz <= 1; ++oz)
+                for (int oz = oz0; oz <= oz1; ++oz)
                     for (int oy = -1; oy <= 1; ++oy)
                         for (int ox = -1; ox <= 1; ++ox) {
                             const int jx = ix + ox, jy = iy + oy, jz = iz + oz;
@@ -1229,10 +1338,13 @@ int hex_grid_setup(sfh_system* s, hipStream_t st) {
                         }
                 if (p != rpn[i + 1]) return SFH_OK;
             }
-    SFH_TRY(s->g3cell.ensure(sizeof(int) * cell.size()));
-    SFH_CHECK_HIP(hipMemcpyAsync(s->g3cell.p, cell.data(), sizeof(int) * cell.size(), hipMemcpyHostToDevice, st));
-    SFH_CHECK_HIP(hipStreamSynchronize(st));
-    s->g3cell_state = 1;
+    s->gcell_identity = identity;
+    if (!identity) {
+        SFH_TRY(s->gcell.ensure(sizeof(int) * cell.size()));
+        SFH_CHECK_HIP(hipMemcpyAsync(s->gcell.p, cell.data(), sizeof(int) * cell.size(), hipMemcpyHostToDevice, st));
+        SFH_CHECK_HIP(hipStreamSynchronize(st));
+    }
+    s->gcell_state = 1;
     return SFH_OK;
 }
 
@@ -1275,10 +1387,11 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
     // hex8 cells of a declared 3D grid, integrated exactly by the closed form
     // when the rule has two or more points per axis
     bool grid_cells = false;
-    if (nq1d >= 2 && s->kind == SFH_HEX8 && s->dim == 3) {
-        SFH_TRY(hex_grid_setup(s, st));
-        grid_cells = s->g3cell_state == 1;
+    if (nq1d >= 2 && ((s->kind == SFH_HEX8 && s->dim == 3) || (s->kind == SFH_QUAD4 && s->dim == 2))) {
+        SFH_TRY(grid_cells_setup(s, st));
+        grid_cells = s->gcell_state == 1;
     }
+    const int* gcell = s->gcell_identity ? nullptr : s->gcell.as<int>();
     if (physics == SFH_ELASTICITY) {
         if (s->kind != SFH_HEX8 || s->dofpn != 3) {
             set_error("elasticity needs HEX8 elements with 3 DOFs per node");
@@ -1289,7 +1402,7 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
             const dim3 gb((unsigned)((s->gn[0] + nw - 1) / nw), (unsigned)s->gn[1], (unsigned)s->gn[2]);
             k_hex_grid<SFH_ELASTICITY><<<gb, 256, 0, st>>>(
                 s->gn[0], s->gn[1], s->gn[2], s->gaxis[0].as<double>(), s->gaxis[1].as<double>(),
-                s->gaxis[2].as<double>(), s->g3cell.as<int>(), p1, p2, s1, s2, factor, s->rowptr.as<int>(),
+                s->gaxis[2].as<double>(), gcell, p1, p2, s1, s2, factor, s->rowptr.as<int>(),
                 s->vals.as<double>(), s->vals32.as<float>(), errp);
             SFH_LAUNCH_CHECK();
         } else if (s->nel < (1 << 25) && 64LL * s->nel < INT32_MAX && s->nnz_node < INT32_MAX) {
@@ -1318,13 +1431,13 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
         if (physics == SFH_POISSON)
             k_hex_grid<SFH_POISSON><<<gb, 256, 0, st>>>(s->gn[0], s->gn[1], s->gn[2], s->gaxis[0].as<double>(),
                                                          s->gaxis[1].as<double>(), s->gaxis[2].as<double>(),
-                                                         s->g3cell.as<int>(), p1, p2, s1, s2, factor,
+                                                         gcell, p1, p2, s1, s2, factor,
                                                          s->rowptr.as<int>(), s->vals.as<double>(),
                                                          s->vals32.as<float>(), errp);
         else
             k_hex_grid<SFH_MATERN><<<gb, 256, 0, st>>>(s->gn[0], s->gn[1], s->gn[2], s->gaxis[0].as<double>(),
                                                         s->gaxis[1].as<double>(), s->gaxis[2].as<double>(),
-                                                        s->g3cell.as<int>(), p1, p2, s1, s2, factor,
+                                                        gcell, p1, p2, s1, s2, factor,
                                                         s->rowptr.as<int>(), s->vals.as<double>(),
                                                         s->vals32.as<float>(), errp);
         SFH_LAUNCH_CHECK();
@@ -1343,6 +1456,20 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
         k_hex_scalar_gather<<<(unsigned)(((s->nnz + 255) / 256 + 7) / 8 * 8), 256, 0, st>>>(
             (int)s->nnz, s->el_cptr.as<int>(), s->el_cent.as<int>(), s->el_ke.as<double>(), s->vals.as<double>(),
             s->vals32.as<float>());
+        SFH_LAUNCH_CHECK();
+    } else if ((physics == SFH_POISSON || physics == SFH_MATERN) && s->kind == SFH_QUAD4 && s->dofpn == 1 &&
+               grid_cells) {
+        const dim3 gb((unsigned)((s->gn[0] + kQgB - 1) / kQgB), (unsigned)s->gn[1]);
+        if (physics == SFH_POISSON)
+            k_quad_grid<SFH_POISSON><<<gb, kQgB, 0, st>>>(s->gn[0], s->gn[1], s->gaxis[0].as<double>(),
+                                                          s->gaxis[1].as<double>(), gcell, p1, p2, s1, s2, factor,
+                                                          s->rowptr.as<int>(), s->vals.as<double>(),
+                                                          s->vals32.as<float>(), errp);
+        else
+            k_quad_grid<SFH_MATERN><<<gb, kQgB, 0, st>>>(s->gn[0], s->gn[1], s->gaxis[0].as<double>(),
+                                                         s->gaxis[1].as<double>(), gcell, p1, p2, s1, s2, factor,
+                                                         s->rowptr.as<int>(), s->vals.as<double>(),
+                                                         s->vals32.as<float>(), errp);
         SFH_LAUNCH_CHECK();
     } else if (physics == SFH_POISSON || physics == SFH_MATERN) {
         if (s->dofpn != 1) { set_error("scalar integrand needs 1 DOF per node"); return SFH_EARG; }

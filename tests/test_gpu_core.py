@@ -249,6 +249,35 @@ def test_hex8_elasticity_two_phase_assembly(ctx, orc, nq):
         sb.assemble(hip.ELASTICITY, 1.0, 0.3, nq1d=nq)
     assert e.value.status == -2
 
+
+@pytest.mark.parametrize("nq", [2, 3])
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_quad4_grid_cells_closed_form_assembly(ctx, orc, nq, shuffle):
+    """Quad4 cells of a declared, graded 2D grid (elements in cell order, which
+    needs no cell table, or shuffled): the closed-form cell integrals against
+    the oracle's Gauss-quadrature Triplet assembly for Poisson with
+    per-element conductivities and Matern with per-element parameters."""
+    from statfem_amd import hip
+    base = M.grid2d(23, 17)
+    grade = (0.9, 0.4)
+    axes = [(g + a * g * g) / (1.0 + a) for g, a in zip(base.grid, grade)]
+    X = np.stack([(base.coords[:, d] + grade[d] * base.coords[:, d] ** 2) / (1.0 + grade[d]) for d in range(2)],
+                 axis=1)
+    conn = base.conn[np.random.default_rng(5).permutation(base.nel)] if shuffle else base.conn
+    m = M.Mesh(base.kind, X, conn.copy(), grid=tuple(axes))
+    rng = np.random.default_rng(nq)
+    s = hip.SystemSolveHip(ctx, m)
+    s.set_grid()
+    for rep in range(2):
+        kap = np.exp(rng.normal(size=m.nel) * 0.6)
+        s.assemble(hip.POISSON, kap, nq1d=nq, factor=1.5)
+        _, _, ov, _ = orc.assemble(m, orc.POISSON, kap, nq1d=nq, factor=1.5)
+        assert np.abs(s.values() - ov).max() <= 1e-13 * np.abs(ov).max(), rep
+    d2 = rng.uniform(0.5, 2.0, m.nel)
+    s.assemble(hip.MATERN, kap, d2, nq1d=nq)
+    _, _, mv, _ = orc.assemble(m, orc.MATERN, kap, d2, nq1d=nq)
+    assert np.abs(s.values() - mv).max() <= 1e-13 * np.abs(mv).max()
+
 @pytest.mark.parametrize("nq", [2, 3])
 def test_hex8_grid_cells_closed_form_assembly(ctx, orc, nq):
     """Hex8 cells of a declared, graded 3D grid (elements in shuffled order):
