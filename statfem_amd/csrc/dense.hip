@@ -185,13 +185,34 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double
             }
 }
 
-// C = sum over nz partial products (deterministic split-K reduction)
-__global__ void k_sum_partials(long long mn, int nz, const double* __restrict__ part, double* __restrict__ C) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= mn) return;
-    double s = 0.0;
-    for (int z = 0; z < nz; ++z) s += part[(size_t)z * mn + t];
-    C[t] = s;
+// C = sum over nz partial products (deterministic split-K reduction): a
+// work-group of 256 threads takes TX outputs, 256 / TX threads per output
+// each add the slices z = y, y + 256 / TX, ... in four interleaved chains (so
+// four loads are in flight), and the threads' sums are added in a fixed order
+template <int TX>
+__global__ __launch_bounds__(256) void k_sum_partials(long long mn, int nz, const double* __restrict__ part,
+                                                      double* __restrict__ C) {
+    constexpr int TZ = 256 / TX;
+    __shared__ double red[TZ][TX];
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const long long t = (long long)blockIdx.x * TX + tx;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (t < mn) {
+        int z = ty;
+        for (; z + 3 * TZ < nz; z += 4 * TZ) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) a[j] += part[(size_t)(z + j * TZ) * mn + t];
+        }
+        for (int j = 0; z < nz; z += TZ, ++j) a[j] += part[(size_t)z * mn + t];
+    }
+    red[ty][tx] = (a[0] + a[1]) + (a[2] + a[3]);
+    __syncthreads();
+    if (ty == 0 && t < mn) {
+        double s = 0.0;
+#pragma unroll
+        for (int y = 0; y < TZ; ++y) s += red[y][tx];
+        C[t] = s;
+    }
 }
 
 // ----------------------------------------------------- forcing covariance
@@ -1090,7 +1111,10 @@ int gram(sfh_ctx* c, int N, int m, int n, const double* X, int ldx, const double
     k_dgemm<true><<<dgemm_grid((long long)nM * nN * nz), 256, 0, c->stream>>>(
         m, n, N, X, ldx, V, ldv, 0, work.as<double>(), n, mn, kz, kz, nM, nN, nz);
     SFH_LAUNCH_CHECK();
-    k_sum_partials<<<nblocks(mn, 256), 256, 0, c->stream>>>(mn, nz, work.as<double>(), C);
+    if (mn >= 64LL * c->num_cu)   // enough outputs for the GPU: 64 per work-group, 4 threads each
+        k_sum_partials<64><<<nblocks(mn, 64), 256, 0, c->stream>>>(mn, nz, work.as<double>(), C);
+    else
+        k_sum_partials<16><<<nblocks(mn, 16), 256, 0, c->stream>>>(mn, nz, work.as<double>(), C);
     SFH_LAUNCH_CHECK();
     return SFH_OK;
 }
