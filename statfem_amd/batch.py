@@ -23,7 +23,11 @@ class BatchedForwardGauss:
         self.mesh, off = M.union(meshes)
         self.seg = np.ascontiguousarray(off, dtype=np.int32)
         self.nseg = len(problems)
-        self.kmax = max(len(p["cols"]) for p in problems)
+        # the widest problem's sensors, rounded up to an even count: the
+        # right-hand-side blocks then have an even row stride, which the dense
+        # products load as 16-byte pairs (the extra column is zero and the
+        # solver skips it)
+        self.kmax = (max(len(p["cols"]) for p in problems) + 1) // 2 * 2
         s = hip.SystemSolveHip(ctx, self.mesh)
         s.assemble(hip.POISSON, 1.0, nq1d=nq1d)
         s.assemble_rhs(nq1d=nq1d, fconst=1.0)

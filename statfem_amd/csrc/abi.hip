@@ -586,18 +586,9 @@ int sfh_gram(sfh_ctx* c, int N, int m, int n, const double* X, int ldx, const do
 int sfh_gram_segmented(sfh_ctx* c, int nseg, const int* seg_rows, int m, int n, const double* X, int ldx,
                        const double* V, int ldv, double* C) {
     if (!c || nseg <= 0 || !seg_rows || m <= 0 || n <= 0 || !X || !V || !C) { set_error("bad argument"); return SFH_EARG; }
-    DevBuf work;
-    for (int q = 0; q < nseg; ++q) {
-        const int r0 = seg_rows[q], nr = seg_rows[q + 1] - r0;
-        double* Cq = C + (size_t)q * m * n;
-        if (nr <= 0) {
-            SFH_CHECK_HIP(hipMemsetAsync(Cq, 0, sizeof(double) * m * n, c->stream));
-            continue;
-        }
-        SFH_TRY(gram(c, nr, m, n, X + (size_t)r0 * ldx, ldx, V + (size_t)r0 * ldv, ldv, Cq, work));
-    }
-    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
-    return SFH_OK;
+    for (int q = 0; q < nseg; ++q)
+        if (seg_rows[q + 1] < seg_rows[q] || seg_rows[q] < 0) { set_error("segment bounds must ascend"); return SFH_EARG; }
+    return gram_segmented(c, nseg, seg_rows, m, n, X, ldx, V, ldv, C);
 }
 
 int sfh_covforcing_grid(sfh_ctx* c, int nx, int ny, int nz, const double* gx, const double* gy, const double* gz,
@@ -628,30 +619,14 @@ int sfh_covforcing_batched(sfh_ctx* c, int nseg, const int* seg_rows, const int*
         set_error("bad argument");
         return SFH_EARG;
     }
-    size_t na = 0;
     for (int q = 0; q < nseg; ++q) {
-        if (nx[q] <= 0 || ny[q] <= 0 || !(ell[q] > 0) ||
+        if (nx[q] <= 0 || ny[q] <= 0 || !(ell[q] > 0) || seg_rows[q] < 0 ||
             (long long)nx[q] * ny[q] != (long long)seg_rows[q + 1] - seg_rows[q]) {
             set_error("problem %d: grid %d x %d does not match its rows", q, nx[q], ny[q]);
             return SFH_EARG;
         }
-        na += (size_t)nx[q] + ny[q];
     }
-    DevBuf dax, sc0, sc1, sc2;
-    SFH_TRY(dax.ensure(sizeof(double) * na));
-    SFH_CHECK_HIP(hipMemcpyAsync(dax.p, axes, sizeof(double) * na, hipMemcpyHostToDevice, c->stream));
-    size_t off = 0;
-    for (int q = 0; q < nseg; ++q) {
-        const int gn[3] = {nx[q], ny[q], 1};
-        const double* a = dax.as<double>() + off;
-        const double* ax[3] = {a, a + nx[q], nullptr};
-        const size_t r0 = (size_t)seg_rows[q];
-        SFH_TRY(covforcing_grid(c, ny[q] > 1 ? 2 : 1, gn, ax, cv + r0, sigma[q], ell[q], k, W + r0 * k, V + r0 * k,
-                                sc0, sc1, sc2));
-        off += (size_t)nx[q] + ny[q];
-    }
-    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
-    return SFH_OK;
+    return covforcing_batched(c, nseg, seg_rows, nx, ny, axes, cv, sigma, ell, k, W, V);
 }
 
 int sfh_system_set_covvec(sfh_system* s, const double* covvec) {

@@ -221,6 +221,9 @@ struct sfh_ctx {
     long long kcount = 0;
     // device address of this GPU's fixed GP scratch (sfh_gp_noalloc)
     double* gp_scratch = nullptr;
+    // kept between calls of the batched dense products: tables, axis factors
+    // and row factors; the intermediate block; split-K partial sums; their tables
+    sfh::DevBuf bat_tab, bat_blk, bat_part, bat_ptab;
 };
 
 struct sfh_system {
@@ -340,6 +343,13 @@ int covforcing_apply(sfh_system* s, const double* covvec, double sigma, double e
                      const double* W, double* V, int mode);
 int gram(sfh_ctx* c, int N, int m, int n, const double* X, int ldx, const double* V, int ldv,
          double* C, DevBuf& work);
+// the per-problem products of a batch in one launch each (rows seg[q]..seg[q+1]-1
+// belong to problem q): C_q = X_q^T V_q (m x n, split-K, fixed-order sums), and
+// V_q = sigma_q^2 D (Ky_q (x) Kx_q) D W_q on each problem's nx x ny grid
+int gram_segmented(sfh_ctx* c, int nseg, const int* seg, int m, int n, const double* X, int ldx, const double* V,
+                   int ldv, double* C);
+int covforcing_batched(sfh_ctx* c, int nseg, const int* seg, const int* nx, const int* ny, const double* axes,
+                       const double* cv, const double* sigma, const double* ell, int k, const double* W, double* V);
 int dgemm_shared_a(sfh_ctx* c, int M, int N, int K, const double* A, int lda, const double* B,
                    int ldb, long long strideB, double* C, int ldc, long long strideC, int batch);
 int loglik_batch(sfh_lik* L, int M, const double* theta_dev, double* out_dev, int* status_dev,

@@ -9,7 +9,9 @@
 //    conditioned moments and the dense GP (statX/infer/GaussianProcess.hpp:117-242);
 //  * random-walk Metropolis in lockstep across chains (statX/infer/MCMC.hpp:123-158)
 //    with Philox4x32-10 streams.
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 
 #include "common.hpp"
@@ -35,22 +37,15 @@ constexpr int TM = 64, TN = 128, TK = 16;
 // next 16-deep panels of A and B are loaded into registers while the current
 // ones are multiplied from LDS, then stored to the other LDS buffer (one
 // barrier per step). Interior tiles load 16-byte pairs.
+typedef double GemmAs[2][TM][TK + 1];
+typedef double GemmBs[2][TK][TN + 2];
+
+// One 64 x 128 tile (rows m0.., columns n0..) of C = op(A) B over K range
+// [kbeg, kend), by the 256 threads of a work-group, through the LDS panels.
 template <bool TRANSA>
-__global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
-                                               const double* __restrict__ B, int ldb, long long strideB,
-                                               double* __restrict__ C, int ldc, long long strideC,
-                                               int kz, int kstep, int nM, int nN, int nZ) {
-    __shared__ double As[2][TM][TK + 1];
-    __shared__ double Bs[2][TK][TN + 2];
-    const int nwg = gridDim.x;
-    const int lid = (nwg % 8) ? (int)blockIdx.x : ((int)blockIdx.x % 8) * (nwg / 8) + (int)blockIdx.x / 8;
-    if (lid >= nM * nN * nZ) return;
-    const int mt = lid % nM, nt = (lid / nM) % nN, z = lid / (nM * nN);
-    const double* Bz = B + (size_t)z * strideB;
-    double* Cz = C + (size_t)z * strideC;
-    const int kbeg = z * kstep;
-    const int kend = min(K, kbeg + kz);
-    const int m0 = mt * TM, n0 = nt * TN;
+__device__ __forceinline__ void dgemm_tile(int M, int N, const double* __restrict__ A, int lda,
+                                           const double* __restrict__ Bz, int ldb, double* __restrict__ Cz, int ldc,
+                                           int kbeg, int kend, int m0, int n0, GemmAs& As, GemmBs& Bs) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w >> 1, wn = w & 1;
     // 16-byte loads when every row of the tile's panels is in range and aligned
@@ -185,13 +180,122 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double
             }
 }
 
+// XCD-aware logical work-group index: consecutive logical tiles run on one XCD
+__device__ inline int xcd_lid() {
+    const int nwg = gridDim.x;
+    return (nwg % 8) ? (int)blockIdx.x : ((int)blockIdx.x % 8) * (nwg / 8) + (int)blockIdx.x / 8;
+}
+
+template <bool TRANSA>
+__global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
+                                               const double* __restrict__ B, int ldb, long long strideB,
+                                               double* __restrict__ C, int ldc, long long strideC,
+                                               int kz, int kstep, int nM, int nN, int nZ) {
+    __shared__ GemmAs As;
+    __shared__ GemmBs Bs;
+    const int lid = xcd_lid();
+    if (lid >= nM * nN * nZ) return;
+    const int mt = lid % nM, nt = (lid / nM) % nN, z = lid / (nM * nN);
+    const int kbeg = z * kstep;
+    dgemm_tile<TRANSA>(M, N, A, lda, B + (size_t)z * strideB, ldb, C + (size_t)z * strideC, ldc, kbeg,
+                       min(K, kbeg + kz), mt * TM, nt * TN, As, Bs);
+}
+
+// Many independent products in one launch (problems of a batch): group g
+// covers logical tiles [tile0, tile0 + nM nN nZ) with the layout of k_dgemm;
+// a work-group finds its group by binary search over the tile0 column.
+struct GemmGroup {
+    const double* A;
+    const double* B;
+    double* C;
+    long long strideB, strideC;
+    int M, N, K, lda, ldb, ldc, kz, kstep, nM, nN, nZ, tile0;
+};
+
+template <bool TRANSA>
+__global__ __launch_bounds__(256) void k_dgemm_grouped(int ngroups, int ntiles, const GemmGroup* __restrict__ gg) {
+    __shared__ GemmAs As;
+    __shared__ GemmBs Bs;
+    const int lid = xcd_lid();
+    if (lid >= ntiles) return;
+    int lo = 0, hi = ngroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (gg[mid].tile0 <= lid) lo = mid;
+        else hi = mid - 1;
+    }
+    const GemmGroup& g = gg[lo];
+    const int l = lid - g.tile0;
+    const int mt = l % g.nM, nt = (l / g.nM) % g.nN, z = l / (g.nM * g.nN);
+    const int kbeg = z * g.kstep;
+    dgemm_tile<TRANSA>(g.M, g.N, g.A, g.lda, g.B + (size_t)z * g.strideB, g.ldb, g.C + (size_t)z * g.strideC, g.ldc,
+                       kbeg, min(g.K, kbeg + g.kz), mt * TM, nt * TN, As, Bs);
+}
+
+// Matrix-vector form of the grouped split-K product for n = 1 (a batch's
+// projected means R_q F_q): one work-group per (problem, K chunk); wave w of
+// the four walks rows kbeg + w, kbeg + w + 4, ... in order, its 64 lanes over
+// the m columns (coalesced 512-byte row segments), and the four row phases
+// are added in a fixed order. Slice z of group g is written at g.C + z strideC.
+__global__ __launch_bounds__(256) void k_gemv_t_grouped(int ngroups, int ntiles, const GemmGroup* __restrict__ gg) {
+    __shared__ double red[4][64];
+    const int lid = blockIdx.x;
+    if (lid >= ntiles) return;
+    int lo = 0, hi = ngroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (gg[mid].tile0 <= lid) lo = mid;
+        else hi = mid - 1;
+    }
+    const GemmGroup& g = gg[lo];
+    const int z = lid - g.tile0;
+    const int kbeg = z * g.kstep, kend = min(g.K, kbeg + g.kz);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    double* Cz = g.C + (size_t)z * g.strideC;
+    for (int j0 = 0; j0 < g.M; j0 += 64) {
+        const int j = j0 + l;
+        double a0 = 0.0, a1 = 0.0;
+        if (j < g.M) {
+            int i = kbeg + w;
+            for (; i + 4 < kend; i += 8) {
+                a0 += g.A[(size_t)i * g.lda + j] * g.B[(size_t)i * g.ldb];
+                a1 += g.A[(size_t)(i + 4) * g.lda + j] * g.B[(size_t)(i + 4) * g.ldb];
+            }
+            if (i < kend) a0 += g.A[(size_t)i * g.lda + j] * g.B[(size_t)i * g.ldb];
+        }
+        red[w][l] = a0 + a1;
+        __syncthreads();
+        if (w == 0 && j < g.M) Cz[j] = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
+        __syncthreads();
+    }
+}
+
 // C = sum over nz partial products (deterministic split-K reduction): a
 // work-group of 256 threads takes TX outputs, 256 / TX threads per output
 // each add the slices z = y, y + 256 / TX, ... in four interleaved chains (so
 // four loads are in flight), and the threads' sums are added in a fixed order
 template <int TX>
+__device__ inline void sum_partials(long long mn, int nz, const double* __restrict__ part, double* __restrict__ C);
+
+template <int TX>
 __global__ __launch_bounds__(256) void k_sum_partials(long long mn, int nz, const double* __restrict__ part,
                                                       double* __restrict__ C) {
+    sum_partials<TX>(mn, nz, part, C);
+}
+
+// the same for the problems of a batch: problem q = blockIdx.y sums its
+// nzq[q] slices at part + offq[q] into C + q mn
+template <int TX>
+__global__ __launch_bounds__(256) void k_sum_partials_grouped(long long mn, const int* __restrict__ nzq,
+                                                              const long long* __restrict__ offq,
+                                                              const double* __restrict__ part,
+                                                              double* __restrict__ C) {
+    const int q = blockIdx.y;
+    sum_partials<TX>(mn, nzq[q], part + offq[q], C + (size_t)q * mn);
+}
+
+template <int TX>
+__device__ inline void sum_partials(long long mn, int nz, const double* __restrict__ part, double* __restrict__ C) {
     constexpr int TZ = 256 / TX;
     __shared__ double red[TZ][TX];
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
@@ -279,6 +383,36 @@ __global__ void k_row_scale(long long total, int k, double s, const double* __re
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     out[t] = s * w[t / k] * in[t];
+}
+
+// the 1D factors of a batch's separable kernels in one launch: matrix j
+// (blockIdx.y) has order an[j], its axis points at axes + aoff[j], its values
+// at K + koff[j], length scale ell[j]
+__global__ void k_axis_kernels(const int* __restrict__ an, const long long* __restrict__ aoff,
+                               const long long* __restrict__ koff, const double* __restrict__ ell,
+                               const double* __restrict__ axes, double* __restrict__ K) {
+    const int j = blockIdx.y, n = an[j];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * n) return;
+    const double* g = axes + aoff[j];
+    const double d = g[t / n] - g[t % n];
+    const double l = ell[j];
+    K[koff[j] + t] = exp(-(d * d) / (2. * l * l));
+}
+
+// rs[i] = sigma_q^2 cv[i] for the problem q whose rows seg[q]..seg[q+1]-1 hold i
+__global__ void k_seg_row_factor(int N, int nseg, const int* __restrict__ seg, const double* __restrict__ sig,
+                                 const double* __restrict__ cv, double* __restrict__ rs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (seg[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const double s = sig[lo];
+    rs[i] = (s * s) * cv[i];
 }
 
 // ------------------------------------------------------ dense SPD kernels
@@ -1183,6 +1317,200 @@ int covforcing_grid(sfh_ctx* c, int d, const int gn_[3], const double* const ax[
     }
     k_row_scale<<<nblocks((long long)total, 256), 256, 0, st>>>((long long)total, k, sigma * sigma, cv, t0, V);
     SFH_LAUNCH_CHECK();
+    return SFH_OK;
+}
+
+namespace {
+// host tables of one batched call, packed into one upload: sections aligned to 16 bytes
+struct Packer {
+    std::vector<unsigned char> h;
+    template <class T>
+    size_t add(const T* v, size_t n) {
+        const size_t off = (h.size() + 15) / 16 * 16;
+        h.resize(off + n * sizeof(T));
+        if (n) std::memcpy(h.data() + off, v, n * sizeof(T));
+        return off;
+    }
+};
+
+int grouped_launch(sfh_ctx* c, bool transa, const GemmGroup* dg, int ngroups, long long ntiles) {
+    if (ngroups <= 0 || ntiles <= 0) return SFH_OK;
+    if (ntiles > (1LL << 30)) { set_error("product too large"); return SFH_EARG; }
+    if (transa)
+        k_dgemm_grouped<true><<<dgemm_grid(ntiles), 256, 0, c->stream>>>(ngroups, (int)ntiles, dg);
+    else
+        k_dgemm_grouped<false><<<dgemm_grid(ntiles), 256, 0, c->stream>>>(ngroups, (int)ntiles, dg);
+    SFH_LAUNCH_CHECK();
+    return SFH_OK;
+}
+}  // namespace
+
+// C_q = X_q^T V_q for every problem of a batch: one split-K launch over all
+// problems' tiles and one fixed-order reduction. The K chunk is common to all
+// problems and sized so that the launch holds about four work-groups per CU;
+// a problem's products are those of gram() with its own chunking.
+int gram_segmented(sfh_ctx* c, int nseg, const int* seg, int m, int n, const double* X, int ldx, const double* V,
+                   int ldv, double* C) {
+    const int nM = (m + TM - 1) / TM, nN = (n + TN - 1) / TN;
+    const long long rows = (long long)seg[nseg] - seg[0];
+    const long long mn = (long long)m * n;
+    const int tM = n == 1 ? 1 : nM, tN = n == 1 ? 1 : nN;   // tiles per chunk
+    long long kz = (long long)tM * tN * std::max(1LL, rows) / std::max(1, 4 * c->num_cu);
+    kz = std::max<long long>(TK, (kz + TK - 1) / TK * TK);
+    std::vector<GemmGroup> g;
+    std::vector<int> nzq(nseg, 0);
+    std::vector<long long> offq(nseg, 0);
+    long long tiles = 0, slices = 0;
+    for (int q = 0; q < nseg; ++q) {
+        const int r0 = seg[q], nr = seg[q + 1] - r0;
+        offq[q] = slices * mn;
+        if (nr <= 0) continue;
+        const long long zq = (nr + kz - 1) / kz;
+        nzq[q] = (int)zq;
+        GemmGroup e;
+        e.A = X + (size_t)r0 * ldx;
+        e.B = V + (size_t)r0 * ldv;
+        e.C = nullptr;   // set once the partial buffer exists
+        e.strideB = 0;
+        e.strideC = mn;
+        e.M = m; e.N = n; e.K = nr; e.lda = ldx; e.ldb = ldv; e.ldc = n;
+        e.kz = (int)kz; e.kstep = (int)kz; e.nM = tM; e.nN = tN; e.nZ = (int)zq;
+        e.tile0 = (int)tiles;
+        g.push_back(e);
+        tiles += (long long)tM * tN * zq;
+        slices += zq;
+    }
+    SFH_TRY(c->bat_part.ensure((size_t)std::max(1LL, slices) * mn * 8));
+    double* part = c->bat_part.as<double>();
+    size_t gi = 0;
+    for (int q = 0; q < nseg; ++q)
+        if (nzq[q] > 0) g[gi++].C = part + offq[q];
+    Packer pk;
+    const size_t o_g = pk.add(g.data(), g.size());
+    const size_t o_nz = pk.add(nzq.data(), nzq.size());
+    const size_t o_off = pk.add(offq.data(), offq.size());
+    SFH_TRY(c->bat_ptab.ensure(pk.h.size()));
+    unsigned char* dt = c->bat_ptab.as<unsigned char>();
+    SFH_CHECK_HIP(hipMemcpyAsync(dt, pk.h.data(), pk.h.size(), hipMemcpyHostToDevice, c->stream));
+    if (n == 1) {   // matrix-vector products: one work-group per K chunk
+        if (!g.empty()) {
+            k_gemv_t_grouped<<<(unsigned)tiles, 256, 0, c->stream>>>((int)g.size(), (int)tiles,
+                                                                    reinterpret_cast<const GemmGroup*>(dt + o_g));
+            SFH_LAUNCH_CHECK();
+        }
+    } else {
+        SFH_TRY(grouped_launch(c, true, reinterpret_cast<const GemmGroup*>(dt + o_g), (int)g.size(), tiles));
+    }
+    const int* dnz = reinterpret_cast<const int*>(dt + o_nz);
+    const long long* doff = reinterpret_cast<const long long*>(dt + o_off);
+    if (mn * nseg >= 64LL * c->num_cu)
+        k_sum_partials_grouped<64><<<dim3(nblocks(mn, 64), nseg), 256, 0, c->stream>>>(mn, dnz, doff, part, C);
+    else
+        k_sum_partials_grouped<16><<<dim3(nblocks(mn, 16), nseg), 256, 0, c->stream>>>(mn, dnz, doff, part, C);
+    SFH_LAUNCH_CHECK();
+    // the upload reads pageable host memory: keep the tables until it is done
+    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SFH_OK;
+}
+
+// V_q = sigma_q^2 D (Ky_q (x) Kx_q) D W_q for every problem of a batch, as
+// covforcing_grid does for one grid, in five launches for the whole batch:
+// the axis factors; t = D W; V = (I (x) Kx) t (problem q: ny_q products
+// Kx_q (nx_q x nx_q) times an nx_q x k block); t = (Ky (x) I) V (one product
+// Ky_q times the ny_q x (nx_q k) block); V = sigma^2 D t. A problem with one
+// row of nodes (ny_q = 1) has Ky_q = [1], whose product copies the block.
+int covforcing_batched(sfh_ctx* c, int nseg, const int* seg, const int* nx, const int* ny, const double* axes,
+                       const double* cv, const double* sigma, const double* ell, int k, const double* W, double* V) {
+    // rows of W, V and cv are numbered from seg[0]
+    const int r00 = seg[0];
+    const int N = seg[nseg] - r00;
+    W += (size_t)r00 * k;
+    V += (size_t)r00 * k;
+    cv += r00;
+    std::vector<int> segr(nseg + 1);
+    for (int q = 0; q <= nseg; ++q) segr[q] = seg[q] - r00;
+    std::vector<int> an(2 * nseg);
+    std::vector<long long> aoff(2 * nseg), koff(2 * nseg);
+    std::vector<double> el(2 * nseg);
+    long long na = 0, nk = 0;
+    int maxn = 1;
+    for (int q = 0; q < nseg; ++q)
+        for (int a = 0; a < 2; ++a) {
+            const int j = 2 * q + a, n = a ? ny[q] : nx[q];
+            an[j] = n; aoff[j] = na; koff[j] = nk; el[j] = ell[q];
+            na += n;
+            nk += (long long)n * n;
+            maxn = std::max(maxn, n);
+        }
+    // device layout of bat_tab: axis factors, row factors, then the tables
+    const size_t kbytes = (size_t)nk * 8, rbytes = (size_t)N * 8;
+    Packer pk;
+    const size_t o_ax = pk.add(axes, (size_t)na);
+    const size_t o_an = pk.add(an.data(), an.size());
+    const size_t o_ao = pk.add(aoff.data(), aoff.size());
+    const size_t o_ko = pk.add(koff.data(), koff.size());
+    const size_t o_el = pk.add(el.data(), el.size());
+    const size_t o_sg = pk.add(segr.data(), segr.size());
+    const size_t o_si = pk.add(sigma, (size_t)nseg);
+    const size_t o_gx = pk.add((const GemmGroup*)nullptr, 0);
+    // the group tables need the device addresses of K, so they are appended
+    // after the buffer is sized: reserve their room now
+    const size_t gbytes = 2 * (size_t)nseg * sizeof(GemmGroup);
+    const size_t tab_bytes = o_gx + gbytes;
+    const size_t base = (kbytes + rbytes + 15) / 16 * 16;
+    SFH_TRY(c->bat_tab.ensure(base + tab_bytes));
+    SFH_TRY(c->bat_blk.ensure((size_t)N * k * 8));
+    unsigned char* d = c->bat_tab.as<unsigned char>();
+    double* Kd = reinterpret_cast<double*>(d);
+    double* rs = Kd + nk;
+    unsigned char* dt = d + base;
+    double* t0 = c->bat_blk.as<double>();
+    std::vector<GemmGroup> gx(nseg), gy(nseg);
+    long long tx = 0, ty = 0;
+    for (int q = 0; q < nseg; ++q) {
+        const size_t r0 = (size_t)segr[q];
+        const int n1 = nx[q], n2 = ny[q];
+        GemmGroup& a = gx[q];
+        a.A = Kd + koff[2 * q]; a.B = t0 + r0 * k; a.C = V + r0 * k;
+        a.strideB = a.strideC = (long long)n1 * k;
+        a.M = n1; a.N = k; a.K = n1; a.lda = n1; a.ldb = k; a.ldc = k;
+        a.kz = n1; a.kstep = 0; a.nM = (n1 + TM - 1) / TM; a.nN = (k + TN - 1) / TN; a.nZ = n2;
+        a.tile0 = (int)tx;
+        tx += (long long)a.nM * a.nN * a.nZ;
+        GemmGroup& b = gy[q];
+        b.A = Kd + koff[2 * q + 1]; b.B = V + r0 * k; b.C = t0 + r0 * k;
+        b.strideB = b.strideC = 0;
+        b.M = n2; b.N = n1 * k; b.K = n2; b.lda = n2; b.ldb = n1 * k; b.ldc = n1 * k;
+        b.kz = n2; b.kstep = 0; b.nM = (n2 + TM - 1) / TM; b.nN = (n1 * k + TN - 1) / TN; b.nZ = 1;
+        b.tile0 = (int)ty;
+        ty += (long long)b.nM * b.nN;
+    }
+    pk.h.resize(tab_bytes);
+    std::memcpy(pk.h.data() + o_gx, gx.data(), nseg * sizeof(GemmGroup));
+    std::memcpy(pk.h.data() + o_gx + nseg * sizeof(GemmGroup), gy.data(), nseg * sizeof(GemmGroup));
+    SFH_CHECK_HIP(hipMemcpyAsync(dt, pk.h.data(), tab_bytes, hipMemcpyHostToDevice, c->stream));
+    const double* dax = reinterpret_cast<const double*>(dt + o_ax);
+    const int* dan = reinterpret_cast<const int*>(dt + o_an);
+    const long long* dao = reinterpret_cast<const long long*>(dt + o_ao);
+    const long long* dko = reinterpret_cast<const long long*>(dt + o_ko);
+    const double* del = reinterpret_cast<const double*>(dt + o_el);
+    const int* dsg = reinterpret_cast<const int*>(dt + o_sg);
+    const double* dsi = reinterpret_cast<const double*>(dt + o_si);
+    const GemmGroup* dgx = reinterpret_cast<const GemmGroup*>(dt + o_gx);
+    const GemmGroup* dgy = dgx + nseg;
+    k_axis_kernels<<<dim3(nblocks((long long)maxn * maxn, 256), 2 * nseg), 256, 0, c->stream>>>(dan, dao, dko, del,
+                                                                                               dax, Kd);
+    SFH_LAUNCH_CHECK();
+    const long long total = (long long)N * k;
+    k_row_scale<<<nblocks(total, 256), 256, 0, c->stream>>>(total, k, 1.0, cv, W, t0);
+    SFH_LAUNCH_CHECK();
+    SFH_TRY(grouped_launch(c, false, dgx, nseg, tx));
+    SFH_TRY(grouped_launch(c, false, dgy, nseg, ty));
+    k_seg_row_factor<<<nblocks(N, 256), 256, 0, c->stream>>>(N, nseg, dsg, dsi, cv, rs);
+    SFH_LAUNCH_CHECK();
+    k_row_scale<<<nblocks(total, 256), 256, 0, c->stream>>>(total, k, 1.0, rs, t0, V);
+    SFH_LAUNCH_CHECK();
+    SFH_CHECK_HIP(hipStreamSynchronize(c->stream));
     return SFH_OK;
 }
 

@@ -1177,43 +1177,56 @@ __global__ __launch_bounds__(kTailB) void k_mg_tail(const int* __restrict__ tab,
 // residual replacement with the multigrid preconditioner: r *= s' in place
 // (FP64) and, when rf is given, its FP32 copy, p *= s'/s; the row's problem
 // (srow: nseg + 1 row bounds) selects the scalars
-__global__ void k_mg_rescale(long long total, int kp, int nseg, const int* __restrict__ srow,
-                             const double* __restrict__ scale, const double* __restrict__ ratio,
-                             double* __restrict__ r, float* __restrict__ rf, float* __restrict__ p) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int row = (int)(i / kp);
-    int lo = 0, hi = nseg - 1;   // last segment with srow[seg] <= row
+// the segment of a row: the last seg with srow[seg] <= row
+__device__ inline int row_segment(int row, int nseg, const int* __restrict__ srow) {
+    int lo = 0, hi = nseg - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (srow[mid] <= row) lo = mid; else hi = mid - 1;
     }
-    const size_t c = (size_t)lo * kp + i % kp;
-    const double v = r[i] * scale[c];
-    r[i] = v;
-    if (rf) rf[i] = (float)v;
-    p[i] *= (float)ratio[c];
+    return lo;
+}
+
+// The element-wise kernels below take four consecutive columns of one row
+// per thread (kp is a multiple of 4): one segment search per four entries and
+// 16-byte accesses. total4 = N kp / 4.
+__global__ void k_mg_rescale(long long total4, int kp, int nseg, const int* __restrict__ srow,
+                             const double* __restrict__ scale, const double* __restrict__ ratio,
+                             double* __restrict__ r, float* __restrict__ rf, float* __restrict__ p) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total4) return;
+    const long long i = 4 * t;
+    const int row = (int)(i / kp);
+    const size_t c = (size_t)row_segment(row, nseg, srow) * kp + i % kp;
+    V4<double> v = ld4(r + i);
+    const V4<double> sc = ld4(scale + c), ra = ld4(ratio + c);
+    v.x *= sc.x; v.y *= sc.y; v.z *= sc.z; v.w *= sc.w;
+    st4(r + i, v);
+    if (rf) *reinterpret_cast<float4*>(rf + i) = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+    float4 pv = *reinterpret_cast<const float4*>(p + i);
+    pv.x *= (float)ra.x; pv.y *= (float)ra.y; pv.z *= (float)ra.z; pv.w *= (float)ra.w;
+    *reinterpret_cast<float4*>(p + i) = pv;
 }
 
 // x += alpha_j p_j / s on every node and column: the last step of a
 // multigrid cycle that ended after an even iteration j (the update kernel adds
 // the steps of x in pairs, on odd iterations); alpha_j from its accumulator
 // slots as alpha_before forms it
-__global__ void k_x_flush_mg(long long total, int kp, int nseg, const int* __restrict__ srow, int j, int ks,
+__global__ void k_x_flush_mg(long long total4, int kp, int nseg, const int* __restrict__ srow, int j, int ks,
                              const double* __restrict__ S, const double* __restrict__ acc,
                              const float* __restrict__ p, double* __restrict__ x) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int row = (int)(i / kp);
-    int lo = 0, hi = nseg - 1;   // last segment with srow[seg] <= row
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (srow[mid] <= row) lo = mid; else hi = mid - 1;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total4) return;
+    const long long i0 = 4 * t;
+    const int row = (int)(i0 / kp);
+    const int c0 = row_segment(row, nseg, srow) * kp + (int)(i0 % kp);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int c = c0 + u;
+        const double sc = S[S_SCALE * ks + c];
+        const double a = alpha_before(j + 1, ks, c, S, acc);
+        if (sc > 0.0 && a != 0.0) x[i0 + u] += (a / sc) * p[i0 + u];
     }
-    const int c = lo * kp + (int)(i % kp);
-    const double sc = S[S_SCALE * ks + c];
-    const double a = alpha_before(j + 1, ks, c, S, acc);
-    if (sc > 0.0 && a != 0.0) x[i] += (a / sc) * p[i];
 }
 
 // ------------------------------------------------ vector DOFs: 3 x 3 blocks
@@ -1519,19 +1532,19 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_resid_grid64(const int* __restri
 // x64 += x_inner / scale on active columns (scale > 0); x_inner restarts at 0.
 // srow: nseg + 1 segment row bounds (the row's segment selects its scales).
 template <class T>
-__global__ void k_refine(long long total, int kp, int nseg, const int* __restrict__ srow,
+__global__ void k_refine(long long total4, int kp, int nseg, const int* __restrict__ srow,
                          const double* __restrict__ scale, T* __restrict__ xin, double* __restrict__ x64) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int row = (int)(i / kp);
-    int lo = 0, hi = nseg - 1;   // last segment with srow[seg] <= row
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (srow[mid] <= row) lo = mid; else hi = mid - 1;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total4) return;
+    const long long i0 = 4 * t;
+    const int row = (int)(i0 / kp);
+    const size_t c0 = (size_t)row_segment(row, nseg, srow) * kp + i0 % kp;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const double s = scale[c0 + u];
+        if (s > 0.0) x64[i0 + u] += (double)xin[i0 + u] / s;
+        xin[i0 + u] = (T)0;
     }
-    const double s = scale[(size_t)lo * kp + i % kp];
-    if (s > 0.0) x64[i] += (double)xin[i] / s;
-    xin[i] = (T)0;
 }
 
 // ------------------------------------------------------------ reductions
@@ -2414,13 +2427,13 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_RATIO * ks, hu + ks, ks * 8, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * ks, hu + 2 * (size_t)ks, ks * 8, hipMemcpyHostToDevice, st));
         if (use_g3) {
-            k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, 1, d_srow,
+            k_mg_rescale<<<(unsigned)((nk / 4 + 255) / 256), 256, 0, st>>>((long long)nk / 4, kp, 1, d_srow,
                                                                       S + S_SCALE * ks, S + S_RATIO * ks, rmg, rfm,
                                                                       (float*)pa);
             g3_vcycle(P3, s->mg3, st, 0, 1, cmask, rfm, (float*)z, (float*)w.mgv.p, acc, p0, p1);
             k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb3, p0, p1, S, acc, 0.0, jlast);
         } else if (use_mg) {
-            k_mg_rescale<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
+            k_mg_rescale<<<(unsigned)((nk / 4 + 255) / 256), 256, 0, st>>>((long long)nk / 4, kp, ns, d_srow,
                                                                       S + S_SCALE * ks, S + S_RATIO * ks, rmg, nullptr,
                                                                       (float*)pa);
             Kern<T, LPR>::vcycle(P, mg, st, 0, 1, coef, cmask, rmg, z, mgv, acc, p0, p1, S);
@@ -2520,13 +2533,13 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         }
         jlast = j;
         if (use_mg && (jlast & 1) == 0) {   // x's last step, not added yet (steps go in pairs)
-            k_x_flush_mg<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow, jlast, ks, S,
+            k_x_flush_mg<<<(unsigned)((nk / 4 + 255) / 256), 256, 0, st>>>((long long)nk / 4, kp, ns, d_srow, jlast, ks, S,
                                                                       acc, (const float*)pa, w.x64.as<double>());
             SFH_LAUNCH_CHECK();
         }
         // flush the FP32 correction, true residual in FP64
         if (!fp64_rec)   // (the multigrid iterations update x64 themselves)
-            k_refine<T><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>((long long)nk, kp, ns, d_srow,
+            k_refine<T><<<(unsigned)((nk / 4 + 255) / 256), 256, 0, st>>>((long long)nk / 4, kp, ns, d_srow,
                                                                      S + S_SCALE * ks, x, w.x64.as<double>());
         if (P.grid) {
             SFH_CHECK_HIP(hipMemsetAsync(S + S_RN2 * ks, 0, (size_t)ks * 8, st));

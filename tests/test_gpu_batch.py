@@ -139,6 +139,68 @@ def test_batched_forcing_covariance_equals_per_problem(ctx):
         assert np.array_equal(Vb[a:b], Vq.download())
 
 
+def test_batched_forcing_covariance_with_a_line_problem(ctx):
+    """A batch mixing a 2D grid and a single row of nodes (ny = 1, the 1D
+    product): each block equals sfh_covforcing_grid on that grid alone, bit
+    for bit."""
+    from statfem_amd import hip
+    rng = np.random.default_rng(21)
+    shapes = [(9, 1), (6, 11), (17, 1)]
+    axes_l = [(np.sort(rng.uniform(0, 1, a)), np.sort(rng.uniform(0, 1, b)) if b > 1 else np.zeros(1))
+              for a, b in shapes]
+    off = np.r_[0, np.cumsum([a * b for a, b in shapes])].astype(np.int32)
+    k = 6
+    N = int(off[-1])
+    W = rng.normal(size=(N, k))
+    cv = rng.uniform(0.5, 1.5, size=N)
+    sig = np.array([0.4, 1.3, 0.9])
+    ell = np.array([0.2, 0.15, 0.5])
+    nx = np.array([a for a, b in shapes], np.int32)
+    ny = np.array([b for a, b in shapes], np.int32)
+    axes = np.concatenate([np.r_[ax, ay] for ax, ay in axes_l])
+    Wd, cvd, V = ctx.to_device(W), ctx.to_device(cv), ctx.empty((N, k))
+    hip._check(hip.lib().sfh_covforcing_batched(ctx.h, 3, hip._p(off), hip._p(nx), hip._p(ny), hip._p(axes),
+                                                cvd.ptr, hip._p(sig), hip._p(ell), k, Wd.ptr, V.ptr))
+    Vb = V.download()
+    for q, (ax, ay) in enumerate(axes_l):
+        a, b = off[q], off[q + 1]
+        Wq, cq, Vq = ctx.to_device(W[a:b]), ctx.to_device(cv[a:b]), ctx.empty((b - a, k))
+        ax, ay = np.ascontiguousarray(ax), np.ascontiguousarray(ay)
+        hip._check(hip.lib().sfh_covforcing_grid(ctx.h, len(ax), int(ny[q]), 1, hip._p(ax), hip._p(ay), None,
+                                                 cq.ptr, sig[q], ell[q], k, Wq.ptr, Vq.ptr))
+        assert np.array_equal(Vb[a:b], Vq.download()), q
+        # and the separable kernel itself, from its definition
+        X = np.array([(x, y) for y in ay for x in ax])
+        d2 = ((X[:, None, :] - X[None, :, :]) ** 2).sum(-1)
+        ref = sig[q] ** 2 * cv[a:b, None] * (np.exp(-d2 / (2 * ell[q] ** 2)) @ (cv[a:b, None] * W[a:b]))
+        assert np.abs(Vb[a:b] - ref).max() <= 1e-12 * np.abs(ref).max(), q
+
+
+@pytest.mark.parametrize("m,n", [(7, 5), (127, 127), (127, 1)])
+def test_gram_segmented_matches_per_problem(ctx, m, n):
+    """sfh_gram_segmented (all problems in one split-K launch) against X_q^T V_q
+    per problem, with an empty problem (its block is zero) and problems from
+    a few rows to tens of thousands."""
+    from statfem_amd import hip
+    rng = np.random.default_rng(m + n)
+    sizes = [40, 0, 25000, 3, 9000, 1]
+    seg = np.r_[0, np.cumsum(sizes)].astype(np.int32)
+    N = int(seg[-1])
+    X = rng.normal(size=(N, m))
+    V = rng.normal(size=(N, n))
+    Xd, Vd, G = ctx.to_device(X), ctx.to_device(V), ctx.empty((len(sizes), m, n))
+    hip._check(hip.lib().sfh_gram_segmented(ctx.h, len(sizes), hip._p(seg), m, n, Xd.ptr, m, Vd.ptr, n, G.ptr))
+    Gh = G.download()
+    for q in range(len(sizes)):
+        a, b = seg[q], seg[q + 1]
+        ref = X[a:b].T @ V[a:b]
+        if b == a:
+            assert np.all(Gh[q] == 0.0)
+            continue
+        scale = np.sqrt((X[a:b] ** 2).sum(0))[:, None] * np.sqrt((V[a:b] ** 2).sum(0))[None, :]
+        assert np.all(np.abs(Gh[q] - ref) <= 1e-13 * scale), q
+
+
 @pytest.mark.parametrize("k,precond", [(1, 0), (6, 0), (6, 1), (3, 1)])
 def test_narrow_batch_matches_oracle(ctx, orc, k, precond):
     """A batch of three grids (two with a multigrid hierarchy) solved with
