@@ -422,10 +422,11 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_dir_spmv_mg(const int* __restric
     const Tile t = grid_tile_t<LPR>(kp, gtab, d);
     const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
     if (W == 0) return;
-    if (block_idle(t, kp, ks, S, it)) return;
+    bool lane_live;
+    if (block_idle(t, kp, ks, S, it, lane_live)) return;
     __shared__ float cb[4 * LPR], cm[4 * LPR];
     shared_dir_coef<float, LPR>(it, kp, ks, t.so, S, acc, cb, cm);
-    const bool ok = t.col0 < kp;
+    const bool ok = t.col0 < kp && lane_live;
     const int lc = t.lane * 4;
     const V4<float> b = {cb[lc], cb[lc + 1], cb[lc + 2], cb[lc + 3]};
     const V4<float> m = {cm[lc], cm[lc + 1], cm[lc + 2], cm[lc + 3]};
@@ -517,7 +518,8 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restri
     const Tile t = grid_tile_t<LPR>(kp, gtab, d);
     const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
     if (W == 0) return;
-    if (block_idle(t, kp, ks, S, xup ? it : it + 1)) return;
+    bool lane_live;
+    if (block_idle(t, kp, ks, S, xup ? it : it + 1, lane_live)) return;
     __shared__ double ca[4 * LPR], cx[4 * LPR], cx1[4 * LPR];
     shared_alpha<double, LPR>(it, kp, ks, t.so, S, acc, ca);
     for (int c = threadIdx.x; c < 4 * LPR; c += blockDim.x) {
@@ -526,7 +528,7 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restri
         cx[c] = sc > 0.0 ? ca[c] / sc : 0.0;
         cx1[c] = (xup && sc > 0.0) ? alpha_before(it, ks, t.so + col, S, acc) / sc : 0.0;
     }
-    const bool ok = t.col0 < kp;
+    const bool ok = t.col0 < kp && lane_live;
     stage_grid_f32<LPR>(sm, p, kp, ok ? t.col0 : 0, ok, base, W, H, tx0, ty0, t.grp, t.lane);
     __syncthreads();
     V4<double> rr = {0.0, 0.0, 0.0, 0.0};
@@ -653,7 +655,9 @@ __device__ inline float mg_w(int x, int X, int n) { return mg_axis_w(x, X, n); }
 // zero or padding columns of a narrower problem of a batch; frozen: at the
 // target, its z unused until the next replacement recomputes it); S == NULL
 // never skips
-__device__ inline bool mg_cols_done(const double* __restrict__ S, int ks, int so, int col0, bool ok) {
+// (`lane`: the lane's own columns are live, as in block_idle)
+__device__ inline bool mg_cols_done(const double* __restrict__ S, int ks, int so, int col0, bool ok, bool& lane) {
+    lane = ok;
     if (!S) return false;
     bool live = false;
     if (ok) {
@@ -662,6 +666,7 @@ __device__ inline bool mg_cols_done(const double* __restrict__ S, int ks, int so
         live = (sc[0] > 0.0 && !(fz[0] > 0.0)) || (sc[1] > 0.0 && !(fz[1] > 0.0)) ||
                (sc[2] > 0.0 && !(fz[2] > 0.0)) || (sc[3] > 0.0 && !(fz[3] > 0.0));
     }
+    lane = live;
     return !__syncthreads_or(live);
 }
 
@@ -686,8 +691,8 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_restrict(const int* __restric
     const int fbase = d[0], W = d[1], H = d[2], cbase = d[3], Wc = d[4], Hc = d[5], cx0 = d[6], cy0 = d[7];
     if (W == 0) return;
     const int col0 = (blockIdx.y * LPR + lane) * 4;   // column tile blockIdx.y of 4 * LPR columns
-    const bool ok = col0 < kp;
-    if (mg_cols_done(S, ks, d[8] * kp, col0, ok)) return;
+    bool ok = col0 < kp;
+    if (mg_cols_done(S, ks, d[8] * kp, col0, ok, ok)) return;
     const int lc = ok ? col0 : 0;
     const int fx0 = 2 * cx0 - 2, fy0 = 2 * cy0 - 2;
     {
@@ -823,8 +828,8 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_mg_prolong(const int* __restrict
     // accumulates into it)
     if (TOP && mode == 0) clear_slots(acc, A_PQ + ((it + 1) & 1), 1, ks);
     if (W == 0 && !(TOP && mode == 1)) return;
-    const bool ok = t.col0 < kp;
-    if (mg_cols_done(S, ks, t.so, t.col0, ok)) return;   // (S == NULL for residual replacements)
+    bool ok = t.col0 < kp;
+    if (mg_cols_done(S, ks, t.so, t.col0, ok, ok)) return;   // (S == NULL for residual replacements)
     const int lc = ok ? t.col0 : 0;
     V4<double> rz = {0.0, 0.0, 0.0, 0.0}, rr = {0.0, 0.0, 0.0, 0.0};
     if (W > 0) {
@@ -1022,7 +1027,8 @@ __global__ __launch_bounds__(kTailB) void k_mg_tail(const int* __restrict__ tab,
     extern __shared__ float4 sm[];
     const int c = 4 * xcd_block(blockIdx.x, gridDim.x), q = blockIdx.y, tid = threadIdx.x;
     if (c >= kp) return;
-    if (mg_cols_done(S, ks, q * kp, c, true)) return;   // (S == NULL for residual replacements)
+    bool lane_live;
+    if (mg_cols_done(S, ks, q * kp, c, true, lane_live)) return;   // (S == NULL for residual replacements)
     const int* d = tab + kTT * q;
     const int nl = d[0];
     {   // the top level's residual

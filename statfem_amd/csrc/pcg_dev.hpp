@@ -220,7 +220,11 @@ __device__ inline bool block_done(const Tile& t, int kp, int ks, const double* _
 // the column's kept direction, and its step length is 0, so the iteration
 // changes nothing of it. Columns that freeze in the current launch (their flag
 // is 0 or is being set to it + 1 by sibling work-groups) stay live.
-__device__ inline bool block_idle(const Tile& t, int kp, int ks, const double* __restrict__ S, int before) {
+// `lane` receives whether the lane's own four columns are live: a live
+// work-group's idle lanes neither load nor store (a batch's narrower problems
+// leave whole lanes of padding columns).
+__device__ inline bool block_idle(const Tile& t, int kp, int ks, const double* __restrict__ S, int before,
+                                  bool& lane) {
     bool live = false;
     if (t.col0 < kp) {
         const double* sc = S + S_SCALE * ks + t.so + t.col0;
@@ -228,6 +232,7 @@ __device__ inline bool block_idle(const Tile& t, int kp, int ks, const double* _
 #pragma unroll
         for (int q = 0; q < 4; ++q) live = live || (sc[q] > 0.0 && !(fz[q] > 0.0 && fz[q] <= (double)before));
     }
+    lane = live;
     return !__syncthreads_or(live);
 }
 
