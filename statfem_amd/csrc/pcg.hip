@@ -461,13 +461,14 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_dir_spmv_mg(const int* __restric
     __syncthreads();
     V4<double> pq = {0.0, 0.0, 0.0, 0.0};
     if (ok) {
-        // two nodes per pass, both nodes' stencil coefficients loaded first
-        for (int o0 = t.grp; o0 < kTX * kTY; o0 += 2 * t.ngrp) {
-            int o[2];
-            bool in[2];
-            double c[2][9];
+        // kDN nodes per pass, all their stencil coefficients loaded first
+        constexpr int kDN = 2;
+        for (int o0 = t.grp; o0 < kTX * kTY; o0 += kDN * t.ngrp) {
+            int o[kDN];
+            bool in[kDN];
+            double c[kDN][9];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < kDN; ++u) {
                 o[u] = o0 + u * t.ngrp;
                 const int gx = tx0 + o[u] % kTX, gy = ty0 + o[u] / kTX;
                 in[u] = o[u] < kTX * kTY && gx < W && gy < H;
@@ -476,7 +477,7 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_dir_spmv_mg(const int* __restric
                 for (int k = 0; k < 9; ++k) c[u][k] = coef[(size_t)nd * 9 + k];
             }
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < kDN; ++u) {
                 if (!in[u]) continue;
                 const int ox = o[u] % kTX, oy = o[u] / kTX;
                 V4<double> q = {0.0, 0.0, 0.0, 0.0};
@@ -506,7 +507,9 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_dir_spmv_mg(const int* __restric
 // a cycle that ends after an even iteration flushes its last step with
 // k_x_flush_mg. Work-groups skip as in block_idle, on odd iterations only
 // when every column was frozen before j - 1's step too.
-template <int LPR>
+// XUP = (it odd): the instantiation with the x step holds x, p_{j-1} and the
+// two step lengths; the other one needs none of their registers.
+template <int LPR, bool XUP>
 __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restrict__ gtab, int kp, int ks, int it,
                                                      const double* __restrict__ coef, const double* __restrict__ S,
                                                      const float* __restrict__ p, const float* __restrict__ pprev,
@@ -514,7 +517,7 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restri
                                                      double* __restrict__ acc) {
     extern __shared__ float4 sm[];
     int d[kGT];
-    const bool xup = (it & 1) != 0;
+    constexpr bool xup = XUP;
     const Tile t = grid_tile_t<LPR>(kp, gtab, d);
     const int base = d[0], W = d[1], H = d[2], tx0 = d[3], ty0 = d[4];
     if (W == 0) return;
@@ -522,24 +525,28 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restri
     if (block_idle(t, kp, ks, S, xup ? it : it + 1, lane_live)) return;
     __shared__ double ca[4 * LPR], cx[4 * LPR], cx1[4 * LPR];
     shared_alpha<double, LPR>(it, kp, ks, t.so, S, acc, ca);
-    for (int c = threadIdx.x; c < 4 * LPR; c += blockDim.x) {
-        const int col = blockIdx.y * 4 * LPR + c;
-        const double sc = col < kp ? S[S_SCALE * ks + t.so + col] : 0.0;
-        cx[c] = sc > 0.0 ? ca[c] / sc : 0.0;
-        cx1[c] = (xup && sc > 0.0) ? alpha_before(it, ks, t.so + col, S, acc) / sc : 0.0;
-    }
+    if (xup)
+        for (int c = threadIdx.x; c < 4 * LPR; c += blockDim.x) {
+            const int col = blockIdx.y * 4 * LPR + c;
+            const double sc = col < kp ? S[S_SCALE * ks + t.so + col] : 0.0;
+            cx[c] = sc > 0.0 ? ca[c] / sc : 0.0;
+            cx1[c] = sc > 0.0 ? alpha_before(it, ks, t.so + col, S, acc) / sc : 0.0;
+        }
     const bool ok = t.col0 < kp && lane_live;
     stage_grid_f32<LPR>(sm, p, kp, ok ? t.col0 : 0, ok, base, W, H, tx0, ty0, t.grp, t.lane);
     __syncthreads();
     V4<double> rr = {0.0, 0.0, 0.0, 0.0};
     if (ok) {
         const V4<double> al = {ca[t.lane * 4], ca[t.lane * 4 + 1], ca[t.lane * 4 + 2], ca[t.lane * 4 + 3]};
-        const V4<double> ax = {cx[t.lane * 4], cx[t.lane * 4 + 1], cx[t.lane * 4 + 2], cx[t.lane * 4 + 3]};
-        const V4<double> ax1 = {cx1[t.lane * 4], cx1[t.lane * 4 + 1], cx1[t.lane * 4 + 2], cx1[t.lane * 4 + 3]};
+        V4<double> ax = {0.0, 0.0, 0.0, 0.0}, ax1 = {0.0, 0.0, 0.0, 0.0};
+        if (xup) {
+            ax = {cx[t.lane * 4], cx[t.lane * 4 + 1], cx[t.lane * 4 + 2], cx[t.lane * 4 + 3]};
+            ax1 = {cx1[t.lane * 4], cx1[t.lane * 4 + 1], cx1[t.lane * 4 + 2], cx1[t.lane * 4 + 3]};
+        }
         // kUN nodes per pass: all their x, r and stencil coefficients are
         // loaded before any is used (an off-grid node loads the tile's first
         // node's data and stores nothing)
-        constexpr int kUN = 2;   // 3 per pass measured slower
+        constexpr int kUN = 1;
         for (int o0 = t.grp; o0 < kTX * kTY; o0 += kUN * t.ngrp) {
             int o[kUN];
             bool in[kUN];
@@ -1980,8 +1987,14 @@ struct Kern {
     static void update_mg(const Plan& P, hipStream_t st, int it, const double* coef64, const double* S, const T* p,
                           const T* pprev, double* x, double* r, double* acc) {
         if constexpr (std::is_same<T, float>::value && LPR <= 16)
-            k_cg_update_mg<LPR><<<dim3(P.gblk, P.ntile), GB<LPR>::v, grid_lds_bytes(LPR), st>>>(
-                P.d_gtab, P.kp, P.ks, it, coef64, S, p, pprev, x, r, acc);
+        {
+            if (it & 1)
+                k_cg_update_mg<LPR, true><<<dim3(P.gblk, P.ntile), GB<LPR>::v, grid_lds_bytes(LPR), st>>>(
+                    P.d_gtab, P.kp, P.ks, it, coef64, S, p, pprev, x, r, acc);
+            else
+                k_cg_update_mg<LPR, false><<<dim3(P.gblk, P.ntile), GB<LPR>::v, grid_lds_bytes(LPR), st>>>(
+                    P.d_gtab, P.kp, P.ks, it, coef64, S, p, pprev, x, r, acc);
+        }
     }
     // z = M r (one V-cycle) with r.z into the iteration's slot (mode 0) or r.z,
     // r.r as per-tile partial rows (mode 1)
