@@ -234,8 +234,10 @@ int sfh_solve_rhs(sfh_system* sys, double* X_dev, const sfh_solve_opts* opts,
                   sfh_solve_stats* stats);
 
 /* Instrumentation, no reference counterpart: sampled device time of the
- * inner-PCG launches inside real solves. With every > 0, every every-th inner
- * iteration of the following solves on this context (at most max_samples per
+ * inner-PCG launches inside real solves. With every > 0, the inner iterations
+ * numbered 0 and 1 modulo every (a pair, so that iterations with and without
+ * the multigrid update's x step are sampled alike) of the following solves on
+ * this context (at most max_samples per
  * solve; only solves of at least min_rhs right-hand sides per column tile) is
  * bracketed by HIP events on the context stream; the accumulators are reset.
  * every = 0 turns sampling off. sfh_kernel_times returns the average device

@@ -2471,7 +2471,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         for (;; ++j) {
             // an iteration pair as one graph replay, unless a timing sample is due in it
             if (graphs && (j & 1) == 0 && iters + 2 <= o.max_inner &&
-                !(ismp < nsmp && (iters % cx->ksample == 0 || (iters + 1) % cx->ksample == 0))) {
+                !(ismp < nsmp && (iters % cx->ksample <= 1 || (iters + 1) % cx->ksample == 0))) {
                 hipGraphExec_t ex = nullptr;
                 SFH_TRY(pair_graph(j, &ex));
                 SFH_CHECK_HIP(hipGraphLaunch(ex, st));
@@ -2482,7 +2482,10 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
                 goto checkpoint;
             }
             {
-            const bool smp = ismp < nsmp && iters % cx->ksample == 0;
+            // timing samples come in iteration pairs (iters = 0 and 1 modulo
+            // the sampling period), so both instantiations of the multigrid
+            // update (with and without the x step) enter the averages equally
+            const bool smp = ismp < nsmp && (iters % cx->ksample <= 1);
             if (smp) SFH_CHECK_HIP(hipEventRecord(cx->kev[4 * ismp], st));
             if (use_g3)
                 g3_dir(P3, s->mg3, st, j, S, (const float*)z, (const float*)pa, (float*)pb, acc);
