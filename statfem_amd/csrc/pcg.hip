@@ -510,7 +510,7 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_dir_spmv_mg(const int* __restric
 // XUP = (it odd): the instantiation with the x step holds x, p_{j-1} and the
 // two step lengths; the other one needs none of their registers.
 template <int LPR, bool XUP>
-__global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restrict__ gtab, int kp, int ks, int it,
+__global__ __launch_bounds__(GB<LPR>::v) __attribute__((amdgpu_waves_per_eu(4))) void k_cg_update_mg(const int* __restrict__ gtab, int kp, int ks, int it,
                                                      const double* __restrict__ coef, const double* __restrict__ S,
                                                      const float* __restrict__ p, const float* __restrict__ pprev,
                                                      double* __restrict__ x, double* __restrict__ r,
@@ -533,7 +533,51 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restri
             cx1[c] = sc > 0.0 ? alpha_before(it, ks, t.so + col, S, acc) / sc : 0.0;
         }
     const bool ok = t.col0 < kp && lane_live;
-    stage_grid_f32<LPR>(sm, p, kp, ok ? t.col0 : 0, ok, base, W, H, tx0, ty0, t.grp, t.lane);
+    // The thread's nodes o = grp, grp + ngrp, ... of the tile are software
+    // pipelined: the next node's x, r, p_prev and stencil coefficients are in
+    // flight while the current node is computed, and the first node's loads
+    // are issued right after the staging loads of p, before the barrier (an
+    // off-grid node loads the tile's first node's data and stores nothing).
+    struct Node {
+        V4<double> xv, rv;
+        float4 pv;
+        double c[9];
+        int nd;
+        bool in;
+    };
+    auto fetch = [&](int o, Node& n) {
+        const int gx = tx0 + o % kTX, gy = ty0 + o / kTX;
+        n.in = gx < W && gy < H;
+        n.nd = n.in ? base + gy * W + gx : base;
+        const size_t off = (size_t)n.nd * kp + t.col0;
+        if (xup) {
+            n.xv = ld4(x + off);
+            n.pv = *reinterpret_cast<const float4*>(pprev + off);
+        }
+        n.rv = ld4(r + off);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) n.c[k] = coef[(size_t)n.nd * 9 + k];
+    };
+    Node cur;
+    {
+        constexpr int ngrp = GB<LPR>::v / LPR, kSt = (kHN + ngrp - 1) / ngrp;
+        const int lc = ok ? t.col0 : 0;
+        float4 a[kSt];
+        bool sin[kSt];
+#pragma unroll
+        for (int u = 0; u < kSt; ++u) {
+            const int h = t.grp + u * ngrp;
+            const int gx = tx0 + h % (kTX + 2) - 1, gy = ty0 + h / (kTX + 2) - 1;
+            sin[u] = ok && h < kHN && gx >= 0 && gx < W && gy >= 0 && gy < H;
+            a[u] = *reinterpret_cast<const float4*>(p + (size_t)(sin[u] ? base + gy * W + gx : 0) * kp + lc);
+        }
+        if (ok) fetch(t.grp, cur);   // t.grp < kTX * kTY: every lane group has a first node
+#pragma unroll
+        for (int u = 0; u < kSt; ++u) {
+            const int h = t.grp + u * ngrp;
+            if (h < kHN) sm[h * LPR + t.lane] = sin[u] ? a[u] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
     __syncthreads();
     V4<double> rr = {0.0, 0.0, 0.0, 0.0};
     if (ok) {
@@ -543,55 +587,27 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restri
             ax = {cx[t.lane * 4], cx[t.lane * 4 + 1], cx[t.lane * 4 + 2], cx[t.lane * 4 + 3]};
             ax1 = {cx1[t.lane * 4], cx1[t.lane * 4 + 1], cx1[t.lane * 4 + 2], cx1[t.lane * 4 + 3]};
         }
-        // kUN nodes per pass: all their x, r and stencil coefficients are
-        // loaded before any is used (an off-grid node loads the tile's first
-        // node's data and stores nothing)
-        constexpr int kUN = 1;
-        for (int o0 = t.grp; o0 < kTX * kTY; o0 += kUN * t.ngrp) {
-            int o[kUN];
-            bool in[kUN];
-            int nd[kUN];
-#pragma unroll
-            for (int u = 0; u < kUN; ++u) {
-                o[u] = o0 + u * t.ngrp;
-                const int gx = tx0 + o[u] % kTX, gy = ty0 + o[u] / kTX;
-                in[u] = o[u] < kTX * kTY && gx < W && gy < H;
-                nd[u] = in[u] ? base + gy * W + gx : base;
-            }
-            V4<double> xv[kUN], rv[kUN];
-            float4 pv[kUN];
-            double c[kUN][9];
-#pragma unroll
-            for (int u = 0; u < kUN; ++u) {
-                const size_t off = (size_t)nd[u] * kp + t.col0;
-                if (xup) {
-                    xv[u] = ld4(x + off);
-                    pv[u] = *reinterpret_cast<const float4*>(pprev + off);
-                }
-                rv[u] = ld4(r + off);
-#pragma unroll
-                for (int k = 0; k < 9; ++k) c[u][k] = coef[(size_t)nd[u] * 9 + k];
-            }
-#pragma unroll
-            for (int u = 0; u < kUN; ++u) {
-                if (!in[u]) continue;
-                const int ox = o[u] % kTX, oy = o[u] / kTX;
+        for (int o = t.grp; o < kTX * kTY; o += t.ngrp) {
+            Node nxt;
+            if (o + t.ngrp < kTX * kTY) fetch(o + t.ngrp, nxt);
+            if (cur.in) {
+                const int ox = o % kTX, oy = o / kTX;
                 V4<double> q = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
                     for (int dx = 0; dx < 3; ++dx) {
-                        const double a = c[u][dy * 3 + dx];
+                        const double a = cur.c[dy * 3 + dx];
                         const float4 v = sm[((oy + dy) * (kTX + 2) + (ox + dx)) * LPR + t.lane];
                         q.x += a * v.x; q.y += a * v.y; q.z += a * v.z; q.w += a * v.w;
                     }
                 const float4 pi = sm[((oy + 1) * (kTX + 2) + (ox + 1)) * LPR + t.lane];
-                const size_t off = (size_t)nd[u] * kp + t.col0;
-                V4<double> rn = rv[u];
+                const size_t off = (size_t)cur.nd * kp + t.col0;
+                V4<double> rn = cur.rv;
                 rn.x -= al.x * q.x; rn.y -= al.y * q.y; rn.z -= al.z * q.z; rn.w -= al.w * q.w;
                 if (xup) {
-                    V4<double> xn = xv[u];
-                    const float4 pp = pv[u];
+                    V4<double> xn = cur.xv;
+                    const float4 pp = cur.pv;
                     xn.x += ax1.x * pp.x + ax.x * pi.x; xn.y += ax1.y * pp.y + ax.y * pi.y;
                     xn.z += ax1.z * pp.z + ax.z * pi.z; xn.w += ax1.w * pp.w + ax.w * pi.w;
                     st4(x + off, xn);
@@ -599,6 +615,7 @@ __global__ __launch_bounds__(GB<LPR>::v) void k_cg_update_mg(const int* __restri
                 st4(r + off, rn);
                 rr.x += rn.x * rn.x; rr.y += rn.y * rn.y; rr.z += rn.z * rn.z; rr.w += rn.w * rn.w;
             }
+            cur = nxt;
         }
     }
     __syncthreads();
