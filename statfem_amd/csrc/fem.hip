@@ -983,8 +983,9 @@ int build_pattern(sfh_system* s) {
 // +-1/h, M = int l_i l_j = h/3 or h/6), exact for Gauss rules of two or more
 // points per axis. One thread per node forms its row (the 9-point
 // neighbourhood, each entry summed over the shared cells in the order y, x of
-// the cell) in LDS; the work-group's nodes (a run of one grid line) own one
-// contiguous range of the CSR values, written out coalesced, FP64 and FP32.
+// the cell) in LDS; the work-group's nodes (kQgB consecutive nodes of the
+// grid, across line ends) own one contiguous range of the CSR values, written
+// out coalesced, FP64 and FP32.
 // Poisson: p1 grad . grad; Matern: p2 grad . grad plus p1^2 int N_i on the
 // diagonal (the lumped mass, a quarter of the cell's area).
 constexpr int kQgB = 256;
@@ -996,10 +997,9 @@ __global__ __launch_bounds__(kQgB) void k_quad_grid(int nx, int ny, const double
                                                     const int* __restrict__ rowptr, double* __restrict__ vals,
                                                     float* __restrict__ vals32, int* err) {
     __shared__ double rows[kQgB * 9];
-    const int iy = blockIdx.y, lx0 = blockIdx.x * kQgB, nl = min(kQgB, nx - lx0);
-    const int i0 = lx0 + nx * iy;
+    const int i0 = blockIdx.x * kQgB, nl = min(kQgB, nx * ny - i0);
     const int base = rowptr[i0], end = rowptr[i0 + nl];
-    const int ix = lx0 + threadIdx.x;
+    const int ix = (i0 + (int)threadIdx.x) % nx, iy = (i0 + (int)threadIdx.x) / nx;
     if (threadIdx.x < nl) {
         const int ii[2] = {ix, iy}, n[2] = {nx, ny};
         const double* g[2] = {gx, gy};
@@ -1459,7 +1459,7 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
         SFH_LAUNCH_CHECK();
     } else if ((physics == SFH_POISSON || physics == SFH_MATERN) && s->kind == SFH_QUAD4 && s->dofpn == 1 &&
                grid_cells) {
-        const dim3 gb((unsigned)((s->gn[0] + kQgB - 1) / kQgB), (unsigned)s->gn[1]);
+        const unsigned gb = (unsigned)(((long long)s->gn[0] * s->gn[1] + kQgB - 1) / kQgB);
         if (physics == SFH_POISSON)
             k_quad_grid<SFH_POISSON><<<gb, kQgB, 0, st>>>(s->gn[0], s->gn[1], s->gaxis[0].as<double>(),
                                                           s->gaxis[1].as<double>(), gcell, p1, p2, s1, s2, factor,
