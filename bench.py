@@ -836,7 +836,20 @@ def cpu_baseline(args, budget_s, res=None):
                    f"extrapolated to {args.ns} RHS; C_f product excluded. One thread throughout: {one:.3g} "
                    f"DOF-RHS/s ({k} solve pairs {t_solve_k:.2f}s)"),
         "single_core": one,
+        "cpu_model": cpu_model(), "host_threads": len(os.sched_getaffinity(0)),
     }
+
+
+def cpu_model():
+    """The host CPU's model name (the first processor's entry of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
