@@ -72,6 +72,21 @@ __global__ void k_scatter_pt(int ns, int pn, const int* cols, const double* vals
     for (int q = 0; q < pn; ++q) Pt[(size_t)cols[s * pn + q] * ld + s] += vals[s * pn + q];
 }
 
+// y += a P^T on the entries P^T has (the sparse sensor rows as (col, val)
+// pairs): the entries of P^T are read from its dense copy, so every touched
+// entry gets the value a dense axpy would add; a DOF repeated in one row is
+// added once
+__global__ void k_pt_axpy(int ns, int pn, const int* cols, double a, const double* Pt, int ld, double* y) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    for (int q = 0; q < pn; ++q) {
+        const int c = cols[s * pn + q];
+        bool seen = false;
+        for (int u = 0; u < q; ++u) seen = seen || cols[s * pn + u] == c;
+        if (!seen) y[(size_t)c * ld + s] += a * Pt[(size_t)c * ld + s];
+    }
+}
+
 __global__ void k_mask_zero(int n, const unsigned char* m, double* v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && m[i]) v[i] = 0.0;
@@ -720,7 +735,7 @@ int sfh_forward_gauss(sfh_system* s, int ns, int pn, const int* pcols, const dou
     if (field) {
         // C_u P^T = A^-1 C_f R^T + reg^2 P^T
         SFH_TRY(solve(s, 0, ns, V, ns, field, ns, opts, &s2));
-        k_axpy<<<nb((long long)nk, 256), 256, 0, st>>>((long long)nk, reg * reg, Pt, field);
+        k_pt_axpy<<<nb(ns, 64), 64, 0, st>>>(ns, pn, s->fg_cols.as<int>(), reg * reg, Pt, ns, field);
         SFH_LAUNCH_CHECK();
     }
     SFH_TRY(bg.join());

@@ -1220,21 +1220,26 @@ __device__ inline int row_segment(int row, int nseg, const int* __restrict__ sro
 // The element-wise kernels below take four consecutive columns of one row
 // per thread (kp is a multiple of 4): one segment search per four entries and
 // 16-byte accesses. total4 = N kp / 4.
+// r = src * scale (src: r itself, or b at the first replacement of a solve);
+// p *= ratio, and p = 0 where the ratio is 0 (a new start)
 __global__ void k_mg_rescale(long long total4, int kp, int nseg, const int* __restrict__ srow,
                              const double* __restrict__ scale, const double* __restrict__ ratio,
-                             double* __restrict__ r, float* __restrict__ rf, float* __restrict__ p) {
+                             const double* src, double* r, float* __restrict__ rf, float* __restrict__ p) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total4) return;
     const long long i = 4 * t;
     const int row = (int)(i / kp);
     const size_t c = (size_t)row_segment(row, nseg, srow) * kp + i % kp;
-    V4<double> v = ld4(r + i);
+    V4<double> v = ld4(src + i);
     const V4<double> sc = ld4(scale + c), ra = ld4(ratio + c);
     v.x *= sc.x; v.y *= sc.y; v.z *= sc.z; v.w *= sc.w;
     st4(r + i, v);
     if (rf) *reinterpret_cast<float4*>(rf + i) = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
     float4 pv = *reinterpret_cast<const float4*>(p + i);
-    pv.x *= (float)ra.x; pv.y *= (float)ra.y; pv.z *= (float)ra.z; pv.w *= (float)ra.w;
+    pv.x = ra.x != 0.0 ? pv.x * (float)ra.x : 0.f;
+    pv.y = ra.y != 0.0 ? pv.y * (float)ra.y : 0.f;
+    pv.z = ra.z != 0.0 ? pv.z * (float)ra.z : 0.f;
+    pv.w = ra.w != 0.0 ? pv.w * (float)ra.w : 0.f;
     *reinterpret_cast<float4*>(p + i) = pv;
 }
 
@@ -1492,6 +1497,48 @@ __global__ __launch_bounds__(kB) void k_resid64(int kp, Seg G, const int* __rest
     block_partial<LPR>(t, kp, acc, red, part);
 }
 
+// The start of a solve in one pass: the eliminated right-hand side b (the
+// value k_prepare forms: B on free rows, minus the couplings to constrained
+// DOFs on rows flagged in cnb for a non-transposed solve, 0 on constrained
+// rows and padding columns) and |b|^2 per column as per-block partial rows, the
+// layout k_resid64 gives the initial residual r = b. r itself is not written:
+// the first residual replacement scales b into it.
+template <int LPR>
+__global__ __launch_bounds__(kB) void k_prepare_norm(int k, int kp, Seg G, int trans, const double* __restrict__ B,
+                                                     int ldb, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                     const double* __restrict__ va, const int* __restrict__ diag,
+                                                     const unsigned char* __restrict__ cmask,
+                                                     const unsigned char* __restrict__ cnb, double* __restrict__ b64,
+                                                     double* __restrict__ part) {
+    __shared__ double red[kB * 4];
+    const Tile t = tile<LPR>(kp, G);
+    const bool ok = t.col0 < kp;
+    V4<double> acc = {0, 0, 0, 0};
+    if (ok) {
+        for (int row = t.r0 + t.grp; row < t.r1; row += t.ngrp) {
+            double v[4] = {0.0, 0.0, 0.0, 0.0};
+            if (!cmask[row]) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (t.col0 + u < k) v[u] = B[(size_t)row * ldb + t.col0 + u];
+                if (!trans && cnb && cnb[row]) {
+                    for (int e = rp[row]; e < rp[row + 1]; ++e) {
+                        const int j = ci[e];
+                        if (!cmask[j]) continue;
+                        const double a = va[e], dj = va[diag[j]];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (t.col0 + u < k) v[u] -= a * (B[(size_t)j * ldb + t.col0 + u] / dj);
+                    }
+                }
+            }
+            st4(b64 + (size_t)row * kp + t.col0, V4<double>{v[0], v[1], v[2], v[3]});
+            acc.x += v[0] * v[0]; acc.y += v[1] * v[1]; acc.z += v[2] * v[2]; acc.w += v[3] * v[3];
+        }
+    }
+    block_partial<LPR>(t, kp, acc, red, part);
+}
+
 // FP64 residual r = b - A x on the declared grids (tile table of the iteration
 // kernels) from the FP64 stencil, each row's products in the order of its CSR
 // entries (ascending columns), so r is the CSR residual's; |r|^2 is added into
@@ -1665,32 +1712,6 @@ __global__ void k_pack_entries(long long nnz, const int* __restrict__ ci, const 
                                int2* __restrict__ cv) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < nnz) cv[e] = make_int2(ci[e], __float_as_int(va32[e]));
-}
-
-// b64 (N x kp): free rows carry B (non-transposed: minus the constrained-column
-// couplings A[f][c] x_c with x_c = B_c / A[c][c]); constrained rows are zero.
-// (cnb: rows with a constrained neighbour, nullptr when there is none; I: the
-// index type of n kp)
-template <class I>
-__global__ void k_prepare(int n, int k, int kp, int trans, const double* __restrict__ B, int ldb,
-                          const int* __restrict__ rp, const int* __restrict__ ci,
-                          const double* __restrict__ va, const int* __restrict__ diag,
-                          const unsigned char* __restrict__ cmask, const unsigned char* __restrict__ cnb,
-                          double* __restrict__ b64) {
-    const I t = (I)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (I)n * kp) return;
-    const int i = (int)(t / (I)kp), c = (int)(t - (I)i * kp);
-    double v = 0.0;
-    if (c < k && !cmask[i]) {
-        v = B[(size_t)i * ldb + c];
-        if (!trans && cnb && cnb[i]) {
-            for (int e = rp[i]; e < rp[i + 1]; ++e) {
-                const int j = ci[e];
-                if (cmask[j]) v -= va[e] * (B[(size_t)j * ldb + c] / va[diag[j]]);
-            }
-        }
-    }
-    b64[t] = v;
 }
 
 // rows (not constrained themselves) with a constrained neighbour, gathered
@@ -2305,14 +2326,11 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         SFH_LAUNCH_CHECK();
         cnb = w.cnb.as<unsigned char>();
     }
-    if (nk < (size_t)INT32_MAX)
-        k_prepare<unsigned><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>(
-            n, k, kp, trans, B, ldb, rp, ci, va, s->diagpos.as<int>(), s->cmask.as<unsigned char>(), cnb,
-            w.b64.as<double>());
-    else
-        k_prepare<long long><<<(unsigned)((nk + 255) / 256), 256, 0, st>>>(
-            n, k, kp, trans, B, ldb, rp, ci, va, s->diagpos.as<int>(), s->cmask.as<unsigned char>(), cnb,
-            w.b64.as<double>());
+    // b and |b|^2 per column in one pass (the initial residual r = b is scaled
+    // into r by the first replacement)
+    k_prepare_norm<LPR><<<dim3(P.nblk, P.ntile), kB, 0, st>>>(k, kp, G, trans, B, ldb, rp, ci, va,
+                                                           s->diagpos.as<int>(), s->cmask.as<unsigned char>(), cnb,
+                                                           w.b64.as<double>(), p0);
     SFH_LAUNCH_CHECK();
     SFH_CHECK_HIP(hipMemsetAsync(w.x64.p, 0, nk * 8, st));
     if (!fp64_rec) SFH_CHECK_HIP(hipMemsetAsync(w.x32.p, 0, nk * sizeof(T), st));
@@ -2323,7 +2341,6 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     sfh_ctx* cx = s->ctx;
     const int nsmp = (!tl_env && cx->ksample > 0 && k >= cx->kmin_rhs) ? (int)cx->kev.size() / 4 : 0;
     int ismp = 0;
-    Kern<T, LPR>::resid(P, G, st, rp, ci, va, nullptr, w.b64.as<double>(), w.r64.as<double>(), p0);
     k_red_norm<<<gred, kRedB, 0, st>>>(kp, d_sb, p0, S + S_BN2 * ks);
     SFH_LAUNCH_CHECK();
     // per (segment, column) state, index seg * kp + col
@@ -2462,21 +2479,23 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_SCALE * ks, hu, ks * 8, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_RATIO * ks, hu + ks, ks * 8, hipMemcpyHostToDevice, st));
         SFH_CHECK_HIP(hipMemcpyAsync(S + S_TGT * ks, hu + 2 * (size_t)ks, ks * 8, hipMemcpyHostToDevice, st));
+        // at the first replacement the residual is b itself (never written to r)
+        const double* rsrc = updates == 0 ? w.b64.as<double>() : w.r64.as<double>();
         if (use_g3) {
             k_mg_rescale<<<(unsigned)((nk / 4 + 255) / 256), 256, 0, st>>>((long long)nk / 4, kp, 1, d_srow,
-                                                                      S + S_SCALE * ks, S + S_RATIO * ks, rmg, rfm,
-                                                                      (float*)pa);
+                                                                      S + S_SCALE * ks, S + S_RATIO * ks, rsrc, rmg,
+                                                                      rfm, (float*)pa);
             g3_vcycle(P3, s->mg3, st, 0, 1, cmask, rfm, (float*)z, (float*)w.mgv.p, acc, p0, p1);
             k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb3, p0, p1, S, acc, 0.0, jlast);
         } else if (use_mg) {
             k_mg_rescale<<<(unsigned)((nk / 4 + 255) / 256), 256, 0, st>>>((long long)nk / 4, kp, ns, d_srow,
-                                                                      S + S_SCALE * ks, S + S_RATIO * ks, rmg, nullptr,
-                                                                      (float*)pa);
+                                                                      S + S_SCALE * ks, S + S_RATIO * ks, rsrc, rmg,
+                                                                      nullptr, (float*)pa);
             Kern<T, LPR>::vcycle(P, mg, st, 0, 1, coef, cmask, rmg, z, mgv, acc, p0, p1, S);
             k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sbmg, p0, p1, S, acc, 0.0, jlast);
         } else {
             Kern<T, LPR>::rescale(P, G, st, S + S_SCALE * ks, S + S_RATIO * ks, s->dinv64.as<double>(), diag,
-                                  w.r64.as<double>(), z, pa, p0, p1);
+                                  rsrc, z, pa, p0, p1);
             k_red_reset<<<gred, kRedB, 0, st>>>(kp, ks, d_sb, p0, p1, S, acc, 0.0, jlast);
         }
         SFH_LAUNCH_CHECK();
