@@ -180,7 +180,11 @@ __global__ __launch_bounds__(kB) void k_rescale(int kp, Seg G, const double* __r
             const double di = dinv64[row];
             const V4<T> zv = {(T)(rd.x * s0 * di), (T)(rd.y * s1 * di), (T)(rd.z * s2 * di), (T)(rd.w * s3 * di)};
             V4<T> pv = ld4(p + o);
-            pv.x *= q.x; pv.y *= q.y; pv.z *= q.z; pv.w *= q.w;
+            // a zero ratio starts the direction afresh (p = 0, whatever p held)
+            pv.x = q.x != (T)0 ? pv.x * q.x : (T)0;
+            pv.y = q.y != (T)0 ? pv.y * q.y : (T)0;
+            pv.z = q.z != (T)0 ? pv.z * q.z : (T)0;
+            pv.w = q.w != (T)0 ? pv.w * q.w : (T)0;
             st4(z + o, zv);
             st4(p + o, pv);
             res_dots(zv, diag[row], rz, rr);
@@ -1740,35 +1744,51 @@ __global__ void k_cnbr_scatter(int n, const int* __restrict__ rp, const int* __r
 }
 
 // X from the eliminated solution; see the file comment for the constrained rows.
+// A thread takes four consecutive columns of one row (k4 = ceil(k / 4) column
+// groups per row): one 32-byte load of x for a free row.
+template <class I>
+__device__ inline double finalize_constrained(int i, int c, int trans, const double* __restrict__ B, int ldb,
+                                              const double* __restrict__ x64, int kp, const int* __restrict__ rp,
+                                              const int* __restrict__ ci, const double* __restrict__ va,
+                                              const int* __restrict__ diag, const unsigned char* __restrict__ cmask) {
+    if (!trans) return B[(size_t)i * ldb + c] / va[diag[i]];
+    double s = B[(size_t)i * ldb + c];
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+        const int j = ci[e];
+        if (cmask[j]) continue;
+        int lo = rp[j], hi = rp[j + 1] - 1;   // A[j][i]
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ci[mid] < i) lo = mid + 1; else hi = mid;
+        }
+        if (ci[lo] == i) s -= va[lo] * x64[(size_t)j * kp + c];
+    }
+    return s / va[diag[i]];
+}
+
 template <class I>
 __global__ void k_finalize(int n, int k, int kp, int trans, const double* __restrict__ B, int ldb,
                            const double* __restrict__ x64, const int* __restrict__ rp,
                            const int* __restrict__ ci, const double* __restrict__ va,
                            const int* __restrict__ diag, const unsigned char* __restrict__ cmask,
                            double* __restrict__ X, int ldx) {
+    const int k4 = (k + 3) / 4;
     const I t = (I)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (I)n * k) return;
-    const int i = (int)(t / (I)k), c = (int)(t - (I)i * k);
-    double v;
+    if (t >= (I)n * k4) return;
+    const int i = (int)(t / (I)k4), c0 = 4 * (int)(t - (I)i * k4);
+    double v[4];
     if (!cmask[i]) {
-        v = x64[(size_t)i * kp + c];
-    } else if (!trans) {
-        v = B[(size_t)i * ldb + c] / va[diag[i]];
+        const V4<double> a = ld4(x64 + (size_t)i * kp + c0);   // c0 + 3 < kp
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
     } else {
-        double s = B[(size_t)i * ldb + c];
-        for (int e = rp[i]; e < rp[i + 1]; ++e) {
-            const int j = ci[e];
-            if (cmask[j]) continue;
-            int lo = rp[j], hi = rp[j + 1] - 1;   // A[j][i]
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (ci[mid] < i) lo = mid + 1; else hi = mid;
-            }
-            if (ci[lo] == i) s -= va[lo] * x64[(size_t)j * kp + c];
-        }
-        v = s / va[diag[i]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            v[u] = c0 + u < k ? finalize_constrained<I>(i, c0 + u, trans, B, ldb, x64, kp, rp, ci, va, diag, cmask)
+                              : 0.0;
     }
-    X[(size_t)i * ldx + c] = v;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (c0 + u < k) X[(size_t)i * ldx + c0 + u] = v[u];
 }
 
 inline int pow2_at_least(int v) {
@@ -2334,7 +2354,7 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
     SFH_LAUNCH_CHECK();
     SFH_CHECK_HIP(hipMemsetAsync(w.x64.p, 0, nk * 8, st));
     if (!fp64_rec) SFH_CHECK_HIP(hipMemsetAsync(w.x32.p, 0, nk * sizeof(T), st));
-    SFH_CHECK_HIP(hipMemsetAsync(w.p32.p, 0, nk * sizeof(T), st));
+    // (p needs no clearing: the first replacement sets it to 0, its ratio being 0)
     SFH_CHECK_HIP(hipMemsetAsync(w.scal.p, 0, (size_t)S_NSLOT * ks * 8, st));
     SFH_CHECK_HIP(hipMemsetAsync(w.acc.p, 0, (size_t)A_N * ks * 8, st));
     double* acc = w.acc.as<double>();
@@ -2614,11 +2634,11 @@ int run_solve(sfh_system* s, const Plan& P0, int trans, int k, const double* B, 
         std::copy(hn, hn + ks, rn2.begin());
     }
     if ((size_t)n * k < (size_t)INT32_MAX)
-        k_finalize<unsigned><<<(unsigned)(((size_t)n * k + 255) / 256), 256, 0, st>>>(
+        k_finalize<unsigned><<<(unsigned)(((size_t)n * ((k + 3) / 4) + 255) / 256), 256, 0, st>>>(
             n, k, kp, trans, B, ldb, w.x64.as<double>(), rp, ci, va, s->diagpos.as<int>(),
             s->cmask.as<unsigned char>(), X, ldx);
     else
-        k_finalize<long long><<<(unsigned)(((size_t)n * k + 255) / 256), 256, 0, st>>>(
+        k_finalize<long long><<<(unsigned)(((size_t)n * ((k + 3) / 4) + 255) / 256), 256, 0, st>>>(
             n, k, kp, trans, B, ldb, w.x64.as<double>(), rp, ci, va, s->diagpos.as<int>(),
             s->cmask.as<unsigned char>(), X, ldx);
     SFH_LAUNCH_CHECK();
