@@ -237,8 +237,29 @@ __global__ __launch_bounds__(256) void k_dgemm_grouped(int ngroups, int ntiles, 
 // the four walks rows kbeg + w, kbeg + w + 4, ... in order, its 64 lanes over
 // the m columns (coalesced 512-byte row segments), and the four row phases
 // are added in a fixed order. Slice z of group g is written at g.C + z strideC.
-__global__ __launch_bounds__(256) void k_gemv_t_grouped(int ngroups, int ntiles, const GemmGroup* __restrict__ gg) {
+__device__ inline void gemv_t_chunk(int M, const double* __restrict__ A, int lda, const double* __restrict__ B,
+                                    int ldb, int kbeg, int kend, double* __restrict__ Cz) {
     __shared__ double red[4][64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    for (int j0 = 0; j0 < M; j0 += 64) {
+        const int j = j0 + l;
+        double a0 = 0.0, a1 = 0.0;
+        if (j < M) {
+            int i = kbeg + w;
+            for (; i + 4 < kend; i += 8) {
+                a0 += A[(size_t)i * lda + j] * B[(size_t)i * ldb];
+                a1 += A[(size_t)(i + 4) * lda + j] * B[(size_t)(i + 4) * ldb];
+            }
+            if (i < kend) a0 += A[(size_t)i * lda + j] * B[(size_t)i * ldb];
+        }
+        red[w][l] = a0 + a1;
+        __syncthreads();
+        if (w == 0 && j < M) Cz[j] = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gemv_t_grouped(int ngroups, int ntiles, const GemmGroup* __restrict__ gg) {
     const int lid = blockIdx.x;
     if (lid >= ntiles) return;
     int lo = 0, hi = ngroups - 1;
@@ -249,25 +270,17 @@ __global__ __launch_bounds__(256) void k_gemv_t_grouped(int ngroups, int ntiles,
     }
     const GemmGroup& g = gg[lo];
     const int z = lid - g.tile0;
-    const int kbeg = z * g.kstep, kend = min(g.K, kbeg + g.kz);
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    double* Cz = g.C + (size_t)z * g.strideC;
-    for (int j0 = 0; j0 < g.M; j0 += 64) {
-        const int j = j0 + l;
-        double a0 = 0.0, a1 = 0.0;
-        if (j < g.M) {
-            int i = kbeg + w;
-            for (; i + 4 < kend; i += 8) {
-                a0 += g.A[(size_t)i * g.lda + j] * g.B[(size_t)i * g.ldb];
-                a1 += g.A[(size_t)(i + 4) * g.lda + j] * g.B[(size_t)(i + 4) * g.ldb];
-            }
-            if (i < kend) a0 += g.A[(size_t)i * g.lda + j] * g.B[(size_t)i * g.ldb];
-        }
-        red[w][l] = a0 + a1;
-        __syncthreads();
-        if (w == 0 && j < g.M) Cz[j] = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
-        __syncthreads();
-    }
+    const int kbeg = z * g.kstep;
+    gemv_t_chunk(g.M, g.A, g.lda, g.B, g.ldb, kbeg, min(g.K, kbeg + g.kz), g.C + (size_t)z * g.strideC);
+}
+
+// one product X^T f (K x M by K x 1): chunk z of kz rows per work-group, slice
+// z of the partial sums at part + z M
+__global__ __launch_bounds__(256) void k_gemv_t(int M, int K, const double* __restrict__ A, int lda,
+                                                const double* __restrict__ B, int ldb, int kz,
+                                                double* __restrict__ part) {
+    const int z = blockIdx.x, kbeg = z * kz;
+    gemv_t_chunk(M, A, lda, B, ldb, kbeg, min(K, kbeg + kz), part + (size_t)z * M);
 }
 
 // C = sum over nz partial products (deterministic split-K reduction): a
@@ -1234,6 +1247,17 @@ int dgemm_shared_a(sfh_ctx* c, int M, int N, int K, const double* A, int lda, co
 
 int gram(sfh_ctx* c, int N, int m, int n, const double* X, int ldx, const double* V, int ldv, double* C,
          DevBuf& work) {
+    if (n == 1) {   // a matrix-vector product: one pass over X, about four work-groups per CU
+        int kz = std::max(TK, (N + 4 * c->num_cu - 1) / (4 * c->num_cu));
+        kz = (kz + TK - 1) / TK * TK;
+        const int nz = (N + kz - 1) / kz;
+        SFH_TRY(work.ensure((size_t)nz * m * 8));
+        k_gemv_t<<<nz, 256, 0, c->stream>>>(m, N, X, ldx, V, ldv, kz, work.as<double>());
+        SFH_LAUNCH_CHECK();
+        k_sum_partials<16><<<nblocks(m, 16), 256, 0, c->stream>>>(m, nz, work.as<double>(), C);
+        SFH_LAUNCH_CHECK();
+        return SFH_OK;
+    }
     const int tiles = ((m + TM - 1) / TM) * ((n + TN - 1) / TN);
     int nz = std::max(1, std::min(256, (4 * c->num_cu + tiles - 1) / tiles));
     int kz = (N + nz - 1) / nz;
