@@ -400,7 +400,7 @@ int sfh_mlp_predict(sfh_mlp* mlp, int n, const double* theta, double* logl);
  * sfh_mcmc), then the end point is accepted with the exact likelihood
  * (sfh_lik_loglik_batch) by min(1, pi(y) pi~(x) / (pi(x) pi~(y))), which keeps
  * the exact posterior invariant. Arguments as sfh_mcmc; sub_acc: surrogate
- * acceptance rate per chain. */
+ * acceptance rate per chain; sub <= 256, else SFH_EARG. */
 int sfh_mcmc_da(sfh_lik* lik, sfh_mlp* mlp, int chains, int steps, int sub, const double* theta0,
                 const double* propsd, uint64_t seed, int prior_mode, const double* prior_mu,
                 const double* prior_var, double* samples, double* acc, double* sub_acc);

@@ -320,10 +320,11 @@ __global__ __launch_bounds__(256) void k_mlp_infer(int n, const double* __restri
     out[s] = (double)y * norm[7] + norm[6];
 }
 
-// one wave per chain: lane j owns hidden unit j; returns the surrogate log
-// target in every lane
-__device__ inline double surrogate_target(const float* W, float* h1s, const double* th, const double* norm, int mode,
-                                          const double* pm, const double* pv) {
+// one wave per chain: lane j owns hidden unit j and holds row j of the second
+// layer's weights in registers (w2r); returns the surrogate log target in
+// every lane. The layer-2 sums run over k in ascending order.
+__device__ inline double surrogate_target(const float* W, const float (&w2r)[HID], float* h1s, const double* th,
+                                          const double* norm, int mode, const double* pm, const double* pv) {
     const int j = threadIdx.x;
     float x[3];
     for (int i = 0; i < 3; ++i) x[i] = (float)((th[i] - norm[i]) / norm[3 + i]);
@@ -331,8 +332,8 @@ __device__ inline double surrogate_target(const float* W, float* h1s, const doub
     h1s[j] = tanhf(a);
     __syncthreads();
     float a2 = W[OFF_B2 + j];
-    const float* w2 = W + OFF_W2 + HID * j;
-    for (int k = 0; k < HID; ++k) a2 += w2[k] * h1s[k];
+#pragma unroll
+    for (int k = 0; k < HID; ++k) a2 += w2r[k] * h1s[k];   // h1s[k]: one address for the wave (broadcast)
     float c = W[OFF_W3 + j] * tanhf(a2);
     for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
     __syncthreads();
@@ -340,7 +341,12 @@ __device__ inline double surrogate_target(const float* W, float* h1s, const doub
     return ll + log_prior(th, mode, pm, pv);
 }
 
-// L-step random-walk sub-chain on the surrogate target per chain
+// L-step random-walk sub-chain on the surrogate target per chain (L at most
+// kMaxSub). The draws of all L steps do not depend on the chain's state: the
+// wave forms them first, one step per lane (the same Philox counters and
+// transforms as one step after another), and keeps z and log u in LDS, so the
+// sequential part is the network and the accept test only.
+constexpr int kMaxSub = 256;
 __global__ __launch_bounds__(64) void k_da_subchain(int C, int L, int step, uint64_t seed, const double* __restrict__ cur,
                                                     double p0, double p1, double p2, const double* __restrict__ norm,
                                                     const uint16_t* __restrict__ bf, int mode, double m0, double m1,
@@ -349,21 +355,41 @@ __global__ __launch_bounds__(64) void k_da_subchain(int C, int L, int step, uint
                                                     int* __restrict__ nsub) {
     __shared__ float W[NP];
     __shared__ float h1s[HID];
-    const int c = blockIdx.x;
+    __shared__ double zs[kMaxSub][3], lus[kMaxSub];
+    const int c = blockIdx.x, j = threadIdx.x;
     load_weights(bf, W);
+    for (int s = j; s < L; s += blockDim.x) {
+        double z[3], u;
+        draws3(seed, (uint32_t)step, (uint32_t)c, 0x1000u + 3u * (uint32_t)s, z, &u);
+        zs[s][0] = z[0]; zs[s][1] = z[1]; zs[s][2] = z[2];
+        lus[s] = log(u);
+    }
+    // row j of the second layer straight from the BF16 weights (16-byte loads)
+    float w2r[HID];
+    {
+        const uint4* row = reinterpret_cast<const uint4*>(bf + OFF_W2 + HID * j);
+#pragma unroll
+        for (int q = 0; q < HID / 8; ++q) {
+            const uint4 v = row[q];
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                w2r[8 * q + 2 * h] = bf2f((uint16_t)(w4[h] & 0xFFFFu));
+                w2r[8 * q + 2 * h + 1] = bf2f((uint16_t)(w4[h] >> 16));
+            }
+        }
+    }
     __syncthreads();
     const double pm[3] = {m0, m1, m2}, pv[3] = {v0, v1, v2}, sd[3] = {p0, p1, p2};
     double th[3] = {cur[3 * c], cur[3 * c + 1], cur[3 * c + 2]};
-    const double y0 = surrogate_target(W, h1s, th, norm, mode, pm, pv);
+    const double y0 = surrogate_target(W, w2r, h1s, th, norm, mode, pm, pv);
     double y = y0;
     int acc = 0;
     for (int s = 0; s < L; ++s) {
-        double z[3], u;
-        draws3(seed, (uint32_t)step, (uint32_t)c, 0x1000u + 3u * (uint32_t)s, z, &u);
         double tp[3];
-        for (int i = 0; i < 3; ++i) tp[i] = th[i] + sd[i] * z[i];
-        const double yp = surrogate_target(W, h1s, tp, norm, mode, pm, pv);
-        if (log(u) <= fmin(0.0, yp - y)) {
+        for (int i = 0; i < 3; ++i) tp[i] = th[i] + sd[i] * zs[s][i];
+        const double yp = surrogate_target(W, w2r, h1s, tp, norm, mode, pm, pv);
+        if (lus[s] <= fmin(0.0, yp - y)) {
             th[0] = tp[0]; th[1] = tp[1]; th[2] = tp[2];
             y = yp;
             ++acc;
@@ -610,6 +636,7 @@ int sfh_mcmc_da(sfh_lik* L, sfh_mlp* M, int C, int steps, int sub, const double*
                 uint64_t seed, int mode, const double* pm, const double* pv, double* samples, double* acc,
                 double* sub_acc) {
     if (!L || !M || C <= 0 || steps <= 0 || sub <= 0 || !theta0 || !propsd) { set_error("bad argument"); return SFH_EARG; }
+    if (sub > kMaxSub) { set_error("at most %d surrogate steps per outer step", kMaxSub); return SFH_EARG; }
     if (mode < 0 || mode > 2 || (mode && (!pm || !pv))) { set_error("bad prior"); return SFH_EARG; }
     hipStream_t st = L->ctx->stream;
     SFH_TRY(L->mc_theta.ensure(sizeof(double) * 3 * C));
