@@ -148,11 +148,15 @@ def gpu_run(args, traffic=None):
     kcols = min(ns, args.err_cols)
     rt_sample = Rt_dev.download()[:, :kcols].copy()
     del Rt_dev
-    # assembly alone (kernel + its status check), averaged
+    # assembly alone, averaged: the device time of its kernels (HIP events
+    # around them, sfh_assembly_times) and of the whole call with its status
+    # read-back
+    ctx.kernel_sampling(every=args.sample_every, max_samples=4096, min_rhs=kk)
     ctx.timer_start()
     for _ in range(10):
         s.assemble(hip.POISSON, kappa, nq1d=2)
-    asm_ms = ctx.timer_stop() / 10
+    asm_call_ms = ctx.timer_stop() / 10
+    asm_ms, _ = ctx.assembly_times()
     s.applyConstraints(bc)
     kk = args.tile if 0 < args.tile < ns else ns
     N, nnz = s.ndof, s.nnz
@@ -220,7 +224,8 @@ def gpu_run(args, traffic=None):
         "max_relres": max(st["max_relres"] for st in stats),
         "kernel": dom, "kernels": per, "layout": {1: "csr", 2: "stencil9"}.get(stats[-1]["layout"], "?"),
         "precond": {1: "jacobi", 2: "multigrid"}.get(stats[-1]["precond"], "?"), "mg_levels": stats[-1]["levels"],
-        "assembly": {"ms": asm_ms, "GBs": asm_bytes / (asm_ms * 1e-3) / 1e9, "bytes": asm_bytes},
+        "assembly": {"kernel": "k_quad_grid (closed-form Q1 cell integrals)", "ms": asm_ms,
+                     "GBs": asm_bytes / (asm_ms * 1e-3) / 1e9, "bytes": asm_bytes, "call_ms": asm_call_ms},
         "dense": dense,
         "N": N, "nnz": nnz,
         "mean": mean_last, "rt_sample": rt_sample,
@@ -425,13 +430,17 @@ def c3_run(args):
     fm = post["field_mean"]
     # elasticity assembly alone (hex8 cells of the declared grid: closed-form
     # cell integrals summed per (node, neighbour) block into the CSR values,
-    # FP64 + FP32 copy): time per assembly call, the status read-back included
+    # FP64 + FP32 copy): device time of the kernels per call (HIP events), and
+    # of the whole call with its status read-back
     se = pipe.se
     se.assemble(hip.ELASTICITY, cfg["E"], cfg["nu"], nq1d=cfg["nq1d"])
+    ctx.kernel_sampling(every=15, max_samples=64)
     ctx.timer_start()
     for _ in range(5):
         se.assemble(hip.ELASTICITY, cfg["E"], cfg["nu"], nq1d=cfg["nq1d"])
-    asm_ms = ctx.timer_stop() / 5
+    asm_call_ms = ctx.timer_stop() / 5
+    asm_ms, _ = ctx.assembly_times()
+    ctx.kernel_sampling(every=0)
     se.applyConstraints(cfg["bc"])
     m = cfg["mesh"]
     asm_bytes = 12.0 * se.nnz + 4.0 * 3 * m.nnodes                # CSR values FP64 + FP32 written, row starts read
@@ -459,7 +468,7 @@ def c3_run(args):
         "assembly_roofline": {"kernel": "k_hex_grid<elasticity>", "ms": asm_ms, "bytes": asm_bytes, "gflop": asm_flop / 1e9,
                               "bound": "fp64" if asm_flop / (FP64_PEAK_TFLOPS * 1e12) > asm_bytes / (HBM_PEAK_GBS * 1e9)
                               else "hbm",
-                              "bound_ms": bound_ms, "frac": bound_ms / asm_ms},
+                              "bound_ms": bound_ms, "frac": bound_ms / asm_ms, "call_ms": asm_call_ms},
     }, small
 
 

@@ -247,6 +247,12 @@ int sfh_solve_rhs(sfh_system* sys, double* X_dev, const sfh_solve_opts* opts,
  * ms[1]), and the number of samples since the last sfh_kernel_sampling call. */
 int sfh_kernel_sampling(sfh_ctx* ctx, int every, int max_samples, int min_rhs);
 int sfh_kernel_times(sfh_ctx* ctx, double* ms /* 3 */, long long* samples);
+/* Instrumentation, no reference counterpart: while sampling is on (every > 0
+ * above), the device time of the matrix assembly kernels of each
+ * sfh_assemble_matrix call (HIP events around them, the status read-back
+ * excluded); returns their average in ms and their count since the last
+ * sfh_kernel_sampling call. */
+int sfh_assembly_times(sfh_ctx* ctx, double* ms, long long* calls);
 
 /* ------------------------------------------------------------- statistics */
 /* CovarianceForcing::giveProjectedCovar building block (statX/infer/CovarianceForcing.ipp:37-109):

@@ -264,6 +264,8 @@ int sfh_ctx_destroy(sfh_ctx* c) {
     (void)hipEventDestroy(c->tev0);
     (void)hipEventDestroy(c->tev1);
     for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
+    if (c->aev0) (void)hipEventDestroy(c->aev0);
+    if (c->aev1) (void)hipEventDestroy(c->aev1);
     (void)hipStreamDestroy(c->stream);
     (void)hipStreamDestroy(c->stream2);
     delete c;
@@ -571,6 +573,15 @@ int sfh_kernel_sampling(sfh_ctx* c, int every, int max_samples, int min_rhs) {
     c->kmin_rhs = min_rhs;
     c->ksum[0] = c->ksum[1] = c->ksum[2] = 0.0;
     c->kcount = 0;
+    c->asum = 0.0;
+    c->acount = 0;
+    return SFH_OK;
+}
+
+int sfh_assembly_times(sfh_ctx* c, double* ms, long long* n) {
+    if (!c || !ms) { set_error("bad argument"); return SFH_EARG; }
+    *ms = c->acount > 0 ? c->asum / (double)c->acount : 0.0;
+    if (n) *n = c->acount;
     return SFH_OK;
 }
 

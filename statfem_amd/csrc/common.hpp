@@ -219,6 +219,11 @@ struct sfh_ctx {
     std::vector<hipEvent_t> kev;     // 4 events per sample
     double ksum[3] = {0.0, 0.0, 0.0};
     long long kcount = 0;
+    // while sampling is on: device time of the matrix assembly kernels of each
+    // sfh_assemble_matrix call (events created on first use)
+    hipEvent_t aev0 = nullptr, aev1 = nullptr;
+    double asum = 0.0;
+    long long acount = 0;
     // device address of this GPU's fixed GP scratch (sfh_gp_noalloc)
     double* gp_scratch = nullptr;
     // kept between calls of the batched dense products: tables, axis factors

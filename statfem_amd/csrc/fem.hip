@@ -1392,6 +1392,13 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
         grid_cells = s->gcell_state == 1;
     }
     const int* gcell = s->gcell_identity ? nullptr : s->gcell.as<int>();
+    sfh_ctx* cx = s->ctx;
+    const bool timed = cx->ksample > 0;
+    if (timed) {
+        if (!cx->aev0) SFH_CHECK_HIP(hipEventCreate(&cx->aev0));
+        if (!cx->aev1) SFH_CHECK_HIP(hipEventCreate(&cx->aev1));
+        SFH_CHECK_HIP(hipEventRecord(cx->aev0, st));
+    }
     if (physics == SFH_ELASTICITY) {
         if (s->kind != SFH_HEX8 || s->dofpn != 3) {
             set_error("elasticity needs HEX8 elements with 3 DOFs per node");
@@ -1502,8 +1509,15 @@ int assemble_matrix(sfh_system* s, int physics, int nq1d, const double* p1, cons
         set_error("unknown physics %d", physics);
         return SFH_EARG;
     }
+    if (timed) SFH_CHECK_HIP(hipEventRecord(cx->aev1, st));
     SFH_CHECK_HIP(hipMemcpyAsync(s->asm_host.p, errp, sizeof(int), hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
+    if (timed) {
+        float ams = 0.f;
+        SFH_CHECK_HIP(hipEventElapsedTime(&ams, cx->aev0, cx->aev1));
+        cx->asum += ams;
+        ++cx->acount;
+    }
     const int herr = *s->asm_host.as<int>();
     if (herr & 4) { set_error("Negative Jacobian."); return SFH_ENEGJAC; }
     SFH_CHECK_HIP(hipMemsetAsync(s->cmask.p, 0, s->ndof, st));

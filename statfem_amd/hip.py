@@ -95,6 +95,7 @@ _SIGS = {
     "sfh_solve_rhs": (i32, [v, v, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "sfh_kernel_sampling": (i32, [v, i32, i32, i32]),
     "sfh_kernel_times": (i32, [v, v, C.POINTER(C.c_longlong)]),
+    "sfh_assembly_times": (i32, [v, v, C.POINTER(C.c_longlong)]),
     "sfh_covforcing_apply": (i32, [v, v, dbl, dbl, i32, v, v, i32]),
     "sfh_gram": (i32, [v, i32, i32, i32, v, i32, v, i32, v]),
     "sfh_forward_gauss": (i32, [v, i32, i32, v, v, dbl, dbl, dbl, C.POINTER(SolveOpts), v, v, v, v,
@@ -229,6 +230,14 @@ class Context:
         n = C.c_longlong()
         _check(lib().sfh_kernel_times(self.h, _p(ms), C.byref(n)))
         return {"dir_spmv": float(ms[0]), "cg_update": float(ms[1]), "vcycle": float(ms[2]), "samples": n.value}
+
+    def assembly_times(self):
+        """Average device time (ms) of the assembly kernels per sfh_assemble_matrix
+        call while kernel sampling is on, and the number of calls."""
+        ms = np.zeros(1)
+        n = C.c_longlong()
+        _check(lib().sfh_assembly_times(self.h, _p(ms), C.byref(n)))
+        return float(ms[0]), n.value
 
     def timer_start(self):
         _check(lib().sfh_timer_start(self.h))
