@@ -316,3 +316,25 @@ def test_hex8_grid_cells_closed_form_assembly(ctx, orc, nq):
     _, _, mv, _ = orc.assemble(m, orc.MATERN, kap, d2, nq1d=nq)
     assert np.abs(s1.values() - mv).max() <= 1e-13 * np.abs(mv).max()
 
+
+
+def test_assembly_kernel_timing_counts_calls(ctx):
+    """sfh_assembly_times: while sampling is on, every sfh_assemble_matrix call
+    adds one timed sample (positive device time); with sampling off, calls add
+    none; the assembled values are those of untimed calls."""
+    from statfem_amd import hip
+    m = M.grid2d(40)
+    s = hip.SystemSolveHip(ctx, m)
+    s.set_grid()
+    s.assemble(hip.POISSON, 1.3, nq1d=2)
+    ref = s.values()
+    ctx.kernel_sampling(every=0)
+    s.assemble(hip.POISSON, 1.3, nq1d=2)
+    assert ctx.assembly_times()[1] == 0
+    ctx.kernel_sampling(every=4, max_samples=16)
+    for _ in range(3):
+        s.assemble(hip.POISSON, 1.3, nq1d=2)
+    ms, n = ctx.assembly_times()
+    assert n == 3 and ms > 0.0
+    assert np.array_equal(s.values(), ref)
+    ctx.kernel_sampling(every=0)
