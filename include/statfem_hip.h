@@ -11,6 +11,12 @@
  *
  * Each declaration cites the reference interface it replaces (paths relative to
  * the statFEM tree).
+ *
+ * Threading: the work of a context, and of the systems, likelihoods and
+ * networks made on it, is ordered on the context's stream; call one context
+ * from one host thread at a time (a context keeps scratch between calls, e.g.
+ * for the batched products). Separate contexts may be used from separate
+ * threads.
  */
 #ifndef STATFEM_HIP_H
 #define STATFEM_HIP_H
