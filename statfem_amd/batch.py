@@ -56,32 +56,38 @@ class BatchedForwardGauss:
         self.ell = np.array([p["ell_f"] for p in problems], np.float64)
         self.PPt = [_sparse_ppt(p["cols"], p["vals"]) for p in problems]
 
+    def _buffers(self):
+        """R^T, V = C_f R^T, the Gram blocks and the means, allocated on the
+        first update and kept for the following ones."""
+        if getattr(self, "_buf", None) is None:
+            ctx, k, N = self.ctx, self.kmax, self.mesh.nnodes
+            self._buf = (ctx.empty((N, k)), ctx.empty((N, k)), ctx.empty((self.nseg, k, k)),
+                         ctx.empty((self.nseg, k, 1)))
+        return self._buf
+
     def updateSolution(self, opts=None):
         """Prior mean and covariance at each problem's sensors."""
         ctx, k = self.ctx, self.kmax
         N = self.mesh.nnodes
-        Rt = ctx.empty((N, k))
+        Rt, V, G, Mn = self._buffers()
         st = hip.SolveStats()
         o = opts or hip.default_opts()
         hip._check(hip.lib().sfh_solve_batched(self.sys.h, self.nseg, hip._p(self.seg), 1, k, self.Pt.ptr, Rt.ptr,
                                                C.byref(o), C.byref(st)))
-        V = ctx.empty((N, k))
         hip._check(hip.lib().sfh_covforcing_batched(ctx.h, self.nseg, hip._p(self.seg), hip._p(self.nx),
                                                     hip._p(self.ny), hip._p(self.axes), self.cv.ptr,
                                                     hip._p(self.sig), hip._p(self.ell), k, Rt.ptr, V.ptr))
-        G = ctx.empty((self.nseg, k, k))
         hip._check(hip.lib().sfh_gram_segmented(ctx.h, self.nseg, hip._p(self.seg), k, k, Rt.ptr, k, V.ptr, k, G.ptr))
-        Mn = ctx.empty((self.nseg, k, 1))
         hip._check(hip.lib().sfh_gram_segmented(ctx.h, self.nseg, hip._p(self.seg), k, 1, Rt.ptr, k, self.F.ptr, 1,
                                                 Mn.ptr))
-        G, Mn = G.download(), Mn.download()
+        G, Mn = G.download(), Mn.download()   # (host copies; the device blocks stay for the next update)
         means, Cus = [], []
         for q, p in enumerate(self.probs):
             ns = len(p["cols"])
             reg = (p["sigma_f"] * p["reg"]) ** 2
             Cus.append(G[q, :ns, :ns] + reg * self.PPt[q])
             means.append(Mn[q, :ns, 0].copy())
-        self.Rt = Rt
+        self.Rt = Rt   # R^T of the last update (overwritten by the next one)
         return means, Cus, st.as_dict()
 
 
