@@ -368,3 +368,25 @@ def test_batched_multigrid_coarse_levels_one_launch_or_per_level(ctx, orc, mg_sp
             a, b = off[q], off[q + 1]
             assert np.abs(Xh[a:b] - ref[a:b]).max() <= 1e-9 * np.abs(ref[a:b]).max(), (q, trans)
         assert np.all(Xh[off[1]:off[2], 4:] == 0.0)
+
+
+@pytest.mark.parametrize("nx,ny,k", [(65, 129, 5), (129, 65, 130), (64, 65, 3)])
+def test_separable_forcing_covariance_one_row_past_the_tiles(ctx, nx, ny, k):
+    """Grid axes of 64 q + 1 nodes: the GEMM tile that also computes the row
+    past the last whole tile gives the separable kernel product of the
+    definition."""
+    from statfem_amd import hip
+    rng = np.random.default_rng(nx + ny + k)
+    ax, ay = np.sort(rng.uniform(0, 1, nx)), np.sort(rng.uniform(0, 1, ny))
+    N = nx * ny
+    W = rng.normal(size=(N, k))
+    cv = rng.uniform(0.5, 1.5, size=N)
+    sig, ell = 0.7, 0.2
+    Wd, cvd, V = ctx.to_device(W), ctx.to_device(cv), ctx.empty((N, k))
+    hip._check(hip.lib().sfh_covforcing_grid(ctx.h, nx, ny, 1, hip._p(ax), hip._p(ay), None, cvd.ptr, sig, ell, k,
+                                             Wd.ptr, V.ptr))
+    Kx = np.exp(-(ax[:, None] - ax[None, :]) ** 2 / (2 * ell * ell))
+    Ky = np.exp(-(ay[:, None] - ay[None, :]) ** 2 / (2 * ell * ell))
+    T = (cv[:, None] * W).reshape(ny, nx, k)
+    ref = sig * sig * cv[:, None] * np.einsum("ab,ij,bjc->aic", Ky, Kx, T).reshape(N, k)
+    assert np.abs(V.download() - ref).max() <= 1e-12 * np.abs(ref).max()
