@@ -183,7 +183,8 @@ typedef struct {
                             problem), with the multigrid preconditioner;
                             auto uses 2, 4 or 3 where they apply, else CSR */
     int precond;         /* 0 (default): auto, 1: Jacobi, 2: geometric multigrid V(1,1)-cycle (bi- or
-                            trilinear prolongation, Galerkin coarse stencils, l1-Jacobi smoothing, dense
+                            trilinear prolongation, Galerkin coarse stencils, l1-Jacobi smoothing (nodal
+                            block Jacobi for 3D vector DOFs), dense
                             coarsest solve) with FP64 recurrences, on declared 2D grids (one or a batch)
                             and declared 3D grids; auto uses 2 wherever it applies, else 1 */
     int graph;           /* 1 (default): on declared 2D grids with multigrid, replay each pair of iterations

@@ -188,6 +188,7 @@ struct MG3Hier {
     DevBuf coef64[kMGMaxLev + 1], coef32[kMGMaxLev + 1], dsm[kMGMaxLev + 1];
     DevBuf cinv;
     int nc = 0;                      // DOFs of the coarsest level
+    double lam[kMGMaxLev + 1] = {};  // DPN 3: lambda_max(B^-1 A) estimate per smoothed level
 };
 
 // per-solve layout of the 3D V-cycle: coarse-level vectors in the

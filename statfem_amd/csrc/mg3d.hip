@@ -22,6 +22,8 @@
 // layout makes each neighbour's kp columns one contiguous run); a lane group
 // of LPR lanes owns one node and 4 columns per lane of each of its DPN rows.
 #include <cmath>
+#include <utility>
+#include <vector>
 
 #include "pcg_dev.hpp"
 
@@ -166,20 +168,119 @@ __global__ __launch_bounds__(64) void k_g3_galerkin(G3 gf, G3 gc, const double* 
         for (int b = 0; b < DPN; ++b) Ac[(((size_t)I * 27 + K) * DPN + a) * DPN + b] = acc[K * DPN + b];
 }
 
-// l1-Jacobi weights omega / sum |a| per row, and the FP32 stencil copy
-__global__ void k_g3_level(int nrow, int dpn, const double* __restrict__ A, float* __restrict__ dsm,
-                           float* __restrict__ A32) {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= nrow) return;
-    const int n = row / dpn, a = row % dpn;
-    double s = 0.0;
-    for (int k = 0; k < 27; ++k)
-        for (int b = 0; b < dpn; ++b) {
-            const size_t o = (((size_t)n * 27 + k) * dpn + a) * dpn + b;
-            s += fabs(A[o]);
-            if (A32) A32[o] = (float)A[o];
+// Smoother weights of one node, DPN^2 floats: for DPN 1 the l1-Jacobi weight
+// omega / sum |a| of the row; for DPN 3 the inverse of the node's 3x3 diagonal
+// block, with constrained DOFs (cmask, finest level) decoupled onto unit rows.
+// The block inverses are scaled afterwards by the level's omega
+// (g3_block_weights). Also the FP32 copy of the node's stencil (A32).
+template <int DPN>
+__global__ void k_g3_level(int nn, const double* __restrict__ A, const unsigned char* __restrict__ cmask,
+                           float* __restrict__ dsm, float* __restrict__ A32) {
+    constexpr int NC = 27 * DPN * DPN;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    const double* An = A + (size_t)n * NC;
+    if (A32)
+        for (int o = 0; o < NC; ++o) A32[(size_t)n * NC + o] = (float)An[o];
+    if constexpr (DPN == 1) {
+        double s = 0.0;
+        for (int k = 0; k < 27; ++k) s += fabs(An[k]);
+        dsm[n] = s > 0.0 ? (float)(kOmega3 / s) : 0.f;
+    } else {
+        double B[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) B[a][b] = An[13 * 9 + a * 3 + b];
+        for (int a = 0; a < 3; ++a)
+            if (cmask && cmask[(size_t)n * 3 + a]) {
+                for (int b = 0; b < 3; ++b) B[a][b] = B[b][a] = 0.0;
+                B[a][a] = 1.0;
+            }
+        // cofactor inverse; the block of an SPD operator is SPD, so det > 0
+        const double c00 = B[1][1] * B[2][2] - B[1][2] * B[2][1], c01 = B[1][2] * B[2][0] - B[1][0] * B[2][2],
+                     c02 = B[1][0] * B[2][1] - B[1][1] * B[2][0];
+        const double det = B[0][0] * c00 + B[0][1] * c01 + B[0][2] * c02;
+        float* w = dsm + (size_t)n * 9;
+        if (!(det > 0.0)) {   // not SPD: diagonal weights, zero where the diagonal is not positive
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) w[a * 3 + b] = a == b && B[a][a] > 0.0 ? (float)(1.0 / B[a][a]) : 0.f;
+            return;
         }
-    dsm[row] = s > 0.0 ? (float)(kOmega3 / s) : 0.f;
+        const double id = 1.0 / det;
+        const double inv[9] = {c00 * id, (B[0][2] * B[2][1] - B[0][1] * B[2][2]) * id,
+                               (B[0][1] * B[1][2] - B[0][2] * B[1][1]) * id, c01 * id,
+                               (B[0][0] * B[2][2] - B[0][2] * B[2][0]) * id,
+                               (B[0][2] * B[1][0] - B[0][0] * B[1][2]) * id, c02 * id,
+                               (B[0][1] * B[2][0] - B[0][0] * B[2][1]) * id,
+                               (B[0][0] * B[1][1] - B[0][1] * B[1][0]) * id};
+        for (int o = 0; o < 9; ++o) w[o] = (float)inv[o];
+    }
+}
+
+// Start vector of the power iteration: hashed values in [-1, 1), zero on
+// constrained DOFs
+__global__ void k_g3_pinit(int nrow, const unsigned char* __restrict__ cmask, double* __restrict__ v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrow) return;
+    unsigned h = (unsigned)i * 2654435761u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    v[i] = cmask && cmask[i] ? 0.0 : (double)(h & 0xffffu) / 32768.0 - 1.0;
+}
+
+// One power-iteration step w = B^-1 A v on a DPN-3 level (B: the nodal
+// diagonal blocks, binv their inverses from k_g3_level; w zero on
+// constrained DOFs), with the Rayleigh-quotient sums of v per work-group:
+// part[2 blk] = v.Av, part[2 blk + 1] = v.Bv, in a fixed summation order
+__global__ __launch_bounds__(256) void k_g3_power(G3 g, const double* __restrict__ A, const float* __restrict__ binv,
+                                                  const unsigned char* __restrict__ cmask,
+                                                  const double* __restrict__ v, double* __restrict__ w,
+                                                  double* __restrict__ part) {
+    __shared__ double red[2][256];
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    double vav = 0.0, vbv = 0.0;
+    if (n < g3n(g)) {
+        const int x = n % g.W, y = (n / g.W) % g.H, z = n / (g.W * g.H);
+        double av[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < 27; ++k) {
+            const int j = nb27(k, x, y, z, g);
+            if (j < 0) continue;
+            const double* c = A + ((size_t)n * 27 + k) * 9;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) av[a] += c[a * 3 + b] * v[(size_t)j * 3 + b];
+        }
+        const double* c = A + ((size_t)n * 27 + 13) * 9;
+        const float* bi = binv + (size_t)n * 9;
+        for (int a = 0; a < 3; ++a) {
+            const double va = v[(size_t)n * 3 + a];
+            vav += va * av[a];
+            for (int b = 0; b < 3; ++b) vbv += va * c[a * 3 + b] * v[(size_t)n * 3 + b];
+        }
+        for (int a = 0; a < 3; ++a) {
+            double s = 0.0;
+            for (int b = 0; b < 3; ++b) s += (double)bi[a * 3 + b] * av[b];
+            w[(size_t)n * 3 + a] = cmask && cmask[(size_t)n * 3 + a] ? 0.0 : s;
+        }
+    }
+    red[0][threadIdx.x] = vav;
+    red[1][threadIdx.x] = vbv;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + h];
+            red[1][threadIdx.x] += red[1][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = red[0][0];
+        part[2 * blockIdx.x + 1] = red[1][0];
+    }
+}
+
+__global__ void k_g3_scale(int n, float s, float* __restrict__ a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] *= s;
 }
 
 // dense FP64 Cholesky inverse of the coarsest level's stencil, FP32 result
@@ -380,12 +481,15 @@ __global__ __launch_bounds__(kB3) void k_g3_resid(G3 g, int kp, int npb, const f
             const int j = nb27(k, x, y, z, g);
             if (j < 0) continue;
             const float* c = cn + k * DPN * DPN;
-            float4 e[DPN];
+            float4 v[DPN], e[DPN];
 #pragma unroll
-            for (int b = 0; b < DPN; ++b) {
-                const float4 v = ldf4(r + ((size_t)j * DPN + b) * kp + t.col0);
-                const float w = dsm[(size_t)j * DPN + b];
-                e[b] = make_float4(w * v.x, w * v.y, w * v.z, w * v.w);
+            for (int b = 0; b < DPN; ++b) v[b] = ldf4(r + ((size_t)j * DPN + b) * kp + t.col0);
+            const float* wj = dsm + (size_t)j * DPN * DPN;
+#pragma unroll
+            for (int a = 0; a < DPN; ++a) {
+                e[a] = zero4();
+#pragma unroll
+                for (int b = 0; b < DPN; ++b) fma4(wj[a * DPN + b], v[b], e[a]);
             }
 #pragma unroll
             for (int a = 0; a < DPN; ++a)
@@ -439,12 +543,15 @@ __global__ __launch_bounds__(kB3) void k_g3_prolong(G3 gf, G3 gc, int kp, int np
         int px, py, pz;
         float fx, fy, fz;
         const int nx = parents1(x, gf.W, px, fx), ny = parents1(y, gf.H, py, fy), nz = parents1(z, gf.D, pz, fz);
-        float4 v[DPN];
+        float4 rv[DPN], v[DPN];
+#pragma unroll
+        for (int b = 0; b < DPN; ++b) rv[b] = ldf4(r + ((size_t)n * DPN + b) * kp + t.col0);
+        const float* wn = dsm + (size_t)n * DPN * DPN;
 #pragma unroll
         for (int a = 0; a < DPN; ++a) {
-            const float4 rv = ldf4(r + ((size_t)n * DPN + a) * kp + t.col0);
-            const float w = dsm[(size_t)n * DPN + a];
-            v[a] = make_float4(w * rv.x, w * rv.y, w * rv.z, w * rv.w);
+            v[a] = zero4();
+#pragma unroll
+            for (int b = 0; b < DPN; ++b) fma4(wn[a * DPN + b], rv[b], v[a]);
         }
         for (int uz = 0; uz < nz; ++uz)
             for (int uy = 0; uy < ny; ++uy)
@@ -498,11 +605,13 @@ __global__ __launch_bounds__(kB3) void k_g3_smooth(G3 g, int kp, int npb, int it
 #pragma unroll
                     for (int b = 0; b < DPN; ++b) fma4(-c[a * DPN + b], v[b], q[a]);
             }
+            const float* wn = dsm + (size_t)n * DPN * DPN;
 #pragma unroll
             for (int a = 0; a < DPN; ++a) {
                 const size_t row = (size_t)n * DPN + a;
                 float4 zz = ldf4(e0 + row * kp + t.col0);
-                fma4(dsm[row], q[a], zz);
+#pragma unroll
+                for (int b = 0; b < DPN; ++b) fma4(wn[a * DPN + b], q[b], zz);
                 if (TOP && cmask[row]) zz = zero4();
                 *reinterpret_cast<float4*>(e + row * kp + t.col0) = zz;
                 if constexpr (TOP) {
@@ -621,6 +730,56 @@ void g3_launch(int what, const G3Plan& P, const MG3Hier& m, hipStream_t st, int 
 }  // namespace
 
 // ------------------------------------------------------------------ host
+void level_weights(int dpn, int nn, const double* A, const unsigned char* cmask, float* dsm, float* A32,
+                   hipStream_t st) {
+    const unsigned b = (unsigned)((nn + 255) / 256);
+    if (dpn == 3) k_g3_level<3><<<b, 256, 0, st>>>(nn, A, cmask, dsm, A32);
+    else k_g3_level<1><<<b, 256, 0, st>>>(nn, A, cmask, dsm, A32);
+}
+
+// Block-Jacobi weight of level l (DPN 3): omega = kBlockOmegaLambda / lambda,
+// lambda the Rayleigh quotient of kPowerSteps power-iteration steps on B^-1 A,
+// a lower estimate of lambda_max(B^-1 A) (the level's 3x3 inverses are scaled
+// by omega in place). The smoother contracts in the A-norm for
+// omega lambda_max < 2; the estimate falls short of lambda_max by a few per
+// cent, so omega lambda_max stays near 1.7, where block Jacobi damps best as
+// a V-cycle smoother of hex8 elasticity.
+constexpr int kPowerSteps = 15;
+constexpr double kBlockOmegaLambda = 1.6;
+int g3_block_weights(MG3Hier& m, int l, const unsigned char* cmask, hipStream_t st) {
+    const G3 g = {m.W[l], m.H[l], m.D[l]};
+    const int nn = g3n(g), nrow = 3 * nn, nb = (nn + 255) / 256;
+    const double* A = l == 0 ? m.dia64.as<double>() : m.coef64[l].as<double>();
+    DevBuf v, w, part;
+    SFH_TRY(v.ensure(sizeof(double) * nrow));
+    SFH_TRY(w.ensure(sizeof(double) * nrow));
+    SFH_TRY(part.ensure(sizeof(double) * 2 * nb));
+    std::vector<double> hp(2 * (size_t)nb);
+    k_g3_pinit<<<(unsigned)((nrow + 255) / 256), 256, 0, st>>>(nrow, cmask, v.as<double>());
+    SFH_LAUNCH_CHECK();
+    double lam = 0.0;
+    for (int k = 0; k < kPowerSteps; ++k) {
+        k_g3_power<<<(unsigned)nb, 256, 0, st>>>(g, A, m.dsm[l].as<float>(), cmask, v.as<double>(), w.as<double>(),
+                                                 part.as<double>());
+        SFH_LAUNCH_CHECK();
+        SFH_CHECK_HIP(hipMemcpyAsync(hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost, st));
+        SFH_CHECK_HIP(hipStreamSynchronize(st));
+        double vav = 0.0, vbv = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            vav += hp[2 * b];
+            vbv += hp[2 * b + 1];
+        }
+        if (vbv > 0.0 && vav / vbv > lam) lam = vav / vbv;
+        std::swap(v.p, w.p);
+    }
+    if (!(lam > 0.0) || !std::isfinite(lam)) return 1;   // no usable estimate: no hierarchy
+    k_g3_scale<<<(unsigned)((9 * nn + 255) / 256), 256, 0, st>>>(9 * nn, (float)(kBlockOmegaLambda / lam),
+                                                                   m.dsm[l].as<float>());
+    SFH_LAUNCH_CHECK();
+    m.lam[l] = lam;
+    return SFH_OK;
+}
+
 bool g3_applies(const sfh_system* s) {
     return s->dim == 3 && (s->dofpn == 1 || s->dofpn == 3) && s->gn[0] >= 2 && s->gn[1] >= 2 && s->gn[2] >= 2 &&
            (long long)s->gn[0] * s->gn[1] * s->gn[2] == s->nnodes &&
@@ -659,9 +818,9 @@ int g3_prepare(sfh_system* s, hipStream_t st) {
     SFH_CHECK_HIP(hipMemcpyAsync(&he, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
     SFH_CHECK_HIP(hipStreamSynchronize(st));
     if (he) return SFH_OK;   // not a 27-point operator on this grid
-    SFH_TRY(m.dsm[0].ensure(sizeof(float) * (size_t)nrow0));
-    k_g3_level<<<(unsigned)((nrow0 + 255) / 256), 256, 0, st>>>((int)nrow0, dpn, m.dia64.as<double>(),
-                                                               m.dsm[0].as<float>(), nullptr);
+    SFH_TRY(m.dsm[0].ensure(sizeof(float) * (size_t)n0 * dpn * dpn));
+    level_weights(dpn, (int)n0, m.dia64.as<double>(), s->cmask.as<unsigned char>(), m.dsm[0].as<float>(), nullptr,
+                  st);
     SFH_LAUNCH_CHECK();
     int l = 0;
     while ((long long)m.W[l] * m.H[l] * m.D[l] * dpn > kMGCoarse && l < kMGMaxLev &&
@@ -674,7 +833,7 @@ int g3_prepare(sfh_system* s, hipStream_t st) {
         const long long nrc = (long long)g3n(gc) * dpn;
         SFH_TRY(m.coef64[l + 1].ensure((size_t)g3n(gc) * 27 * dpn * dpn * 8));
         SFH_TRY(m.coef32[l + 1].ensure((size_t)g3n(gc) * 27 * dpn * dpn * 4));
-        SFH_TRY(m.dsm[l + 1].ensure(sizeof(float) * (size_t)nrc));
+        SFH_TRY(m.dsm[l + 1].ensure(sizeof(float) * (size_t)g3n(gc) * dpn * dpn));
         const double* Af = l == 0 ? m.dia64.as<double>() : m.coef64[l].as<double>();
         const unsigned char* cm = l == 0 ? s->cmask.as<unsigned char>() : nullptr;
         if (dpn == 3)
@@ -682,12 +841,18 @@ int g3_prepare(sfh_system* s, hipStream_t st) {
         else
             k_g3_galerkin<1><<<(unsigned)((nrc + 63) / 64), 64, 0, st>>>(gf, gc, Af, cm, m.coef64[l + 1].as<double>());
         SFH_LAUNCH_CHECK();
-        k_g3_level<<<(unsigned)((nrc + 255) / 256), 256, 0, st>>>((int)nrc, dpn, m.coef64[l + 1].as<double>(),
-                                                                 m.dsm[l + 1].as<float>(), m.coef32[l + 1].as<float>());
+        level_weights(dpn, g3n(gc), m.coef64[l + 1].as<double>(), nullptr, m.dsm[l + 1].as<float>(),
+                      m.coef32[l + 1].as<float>(), st);
         SFH_LAUNCH_CHECK();
         ++l;
     }
     if (l == 0) return SFH_OK;
+    if (dpn == 3)
+        for (int k = 0; k < l; ++k) {
+            const int rc = g3_block_weights(m, k, k == 0 ? s->cmask.as<unsigned char>() : nullptr, st);
+            if (rc == 1) return SFH_OK;
+            SFH_TRY(rc);
+        }
     const G3 gL = {m.W[l], m.H[l], m.D[l]};
     if ((long long)g3n(gL) * dpn > kMGCoarse) return SFH_OK;   // coarsest inverse must fit the LDS-staged solve
     std::vector<double> a((size_t)g3n(gL) * 27 * dpn * dpn);
