@@ -1280,7 +1280,57 @@ __global__ __launch_bounds__(256) void k_hex_grid(int nx, int ny, int nz, const 
 // integrates with a positive Jacobian) and every node's row holds exactly its
 // in-range 9- or 27-point neighbours in increasing order: then k_quad_grid /
 // k_hex_grid apply, with the cell -> element table on the device (none when
-// the elements are in cell order). Checked once per grid declaration.
+// the elements are in cell order). Checked once per grid declaration, on the
+// device. Per element: its corners must be one cell's in the reference order
+// (counter-clockwise, then the top face of a hex8: dx + 2 dy + 4 dz of local
+// node k), and no two elements may share a cell. flags: 1 not a grid mesh,
+// 2 element order differs from the cell order. cell[] starts at -1.
+__global__ void k_grid_cells(int nel, int nen, int nnodes, int nx, int ny, int nz, const int* __restrict__ conn,
+                             int* __restrict__ cell, int* flags) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nel) return;
+    constexpr int kCorner[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX};
+    int xyz[8][3];
+    for (int k = 0; k < nen; ++k) {
+        const int v = conn[(size_t)e * nen + k];
+        if (v < 0 || v >= nnodes) { atomicOr(flags, 1); return; }
+        xyz[k][0] = v % nx;
+        xyz[k][1] = (v / nx) % ny;
+        xyz[k][2] = v / (nx * ny);
+        for (int a = 0; a < 3; ++a) lo[a] = min(lo[a], xyz[k][a]);
+    }
+    for (int k = 0; k < nen; ++k) {
+        const int dx = xyz[k][0] - lo[0], dy = xyz[k][1] - lo[1], dz = xyz[k][2] - lo[2];
+        if (dx > 1 || dy > 1 || dz > 1 || dx + 2 * dy + 4 * dz != kCorner[k]) { atomicOr(flags, 1); return; }
+    }
+    if (lo[0] > nx - 2 || lo[1] > ny - 2 || (nz > 1 && lo[2] > nz - 2)) { atomicOr(flags, 1); return; }
+    const size_t c = ((size_t)lo[2] * (ny - 1) + lo[1]) * (nx - 1) + lo[0];
+    if (atomicCAS(cell + c, -1, e) != -1) { atomicOr(flags, 1); return; }
+    if ((size_t)e != c) atomicOr(flags, 2);
+}
+
+// the node pattern must list each node's in-range 9- or 27-point neighbours
+// in increasing order (flags |= 1 otherwise)
+__global__ void k_grid_pattern(int nx, int ny, int nz, const int* __restrict__ rpn, const int* __restrict__ cin,
+                               int* flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * ny * nz) return;
+    const int ix = i % nx, iy = (i / nx) % ny, iz = i / (nx * ny);
+    const int oz0 = nz > 1 ? -1 : 0, oz1 = nz > 1 ? 1 : 0;
+    int p = rpn[i];
+    const int pe = rpn[i + 1];
+    for (int oz = oz0; oz <= oz1; ++oz)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int jx = ix + ox, jy = iy + oy, jz = iz + oz;
+                if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz) continue;
+                if (p >= pe || cin[p] != jx + nx * (jy + ny * jz)) { atomicOr(flags, 1); return; }
+                ++p;
+            }
+    if (p != pe) atomicOr(flags, 1);
+}
+
 int grid_cells_setup(sfh_system* s, hipStream_t st) {
     if (s->gcell_state != 0) return SFH_OK;
     s->gcell_state = -1;
@@ -1292,58 +1342,24 @@ int grid_cells_setup(sfh_system* s, hipStream_t st) {
         (long long)(nx - 1) * (ny - 1) * (d == 3 ? nz - 1 : 1) != s->nel ||
         (long long)s->nnodes * 27 >= INT32_MAX || ny > 65535 || nz > 65535)
         return SFH_OK;
-    std::vector<int> conn((size_t)s->nel * nen), rpn((size_t)s->nnodes + 1), cin((size_t)s->nnz_node);
-    SFH_CHECK_HIP(hipMemcpyAsync(conn.data(), s->conn.p, conn.size() * sizeof(int), hipMemcpyDeviceToHost, st));
-    SFH_CHECK_HIP(hipMemcpyAsync(rpn.data(), s->rowptr_node.p, rpn.size() * sizeof(int), hipMemcpyDeviceToHost, st));
-    SFH_CHECK_HIP(hipMemcpyAsync(cin.data(), s->colind_node.p, cin.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_TRY(s->gcell.ensure(sizeof(int) * (size_t)s->nel));
+    SFH_TRY(s->asm_err.ensure(sizeof(int)));
+    SFH_CHECK_HIP(hipMemsetAsync(s->gcell.p, 0xff, sizeof(int) * (size_t)s->nel, st));   // -1
+    SFH_CHECK_HIP(hipMemsetAsync(s->asm_err.p, 0, sizeof(int), st));
+    k_grid_cells<<<(unsigned)((s->nel + 255) / 256), 256, 0, st>>>(s->nel, nen, s->nnodes, nx, ny, nz,
+                                                                   s->conn.as<int>(), s->gcell.as<int>(),
+                                                                   s->asm_err.as<int>());
+    SFH_LAUNCH_CHECK();
+    k_grid_pattern<<<(unsigned)((s->nnodes + 255) / 256), 256, 0, st>>>(nx, ny, nz, s->rowptr_node.as<int>(),
+                                                                        s->colind_node.as<int>(),
+                                                                        s->asm_err.as<int>());
+    SFH_LAUNCH_CHECK();
+    int flags = 0;
+    SFH_CHECK_HIP(hipMemcpyAsync(&flags, s->asm_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SFH_CHECK_HIP(hipMemsetAsync(s->asm_err.p, 0, sizeof(int), st));   // the assembly's error word again
     SFH_CHECK_HIP(hipStreamSynchronize(st));
-    // the cell's corners in the reference order (counter-clockwise, then the
-    // top face of a hex8): dx + 2 dy + 4 dz of local node k
-    constexpr int kCorner[8] = {0, 1, 3, 2, 4, 5, 7, 6};
-    std::vector<int> cell((size_t)s->nel, -1);
-    bool identity = true;
-    for (int e = 0; e < s->nel; ++e) {
-        int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX};
-        int xyz[8][3];
-        for (int k = 0; k < nen; ++k) {
-            const int v = conn[(size_t)e * nen + k];
-            if (v < 0 || v >= s->nnodes) return SFH_OK;
-            xyz[k][0] = v % nx;
-            xyz[k][1] = (v / nx) % ny;
-            xyz[k][2] = v / (nx * ny);
-            for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], xyz[k][a]);
-        }
-        for (int k = 0; k < nen; ++k) {
-            const int dx = xyz[k][0] - lo[0], dy = xyz[k][1] - lo[1], dz = xyz[k][2] - lo[2];
-            if (dx > 1 || dy > 1 || dz > 1 || dx + 2 * dy + 4 * dz != kCorner[k]) return SFH_OK;
-        }
-        if (lo[0] > nx - 2 || lo[1] > ny - 2 || (d == 3 && lo[2] > nz - 2)) return SFH_OK;
-        const size_t c = ((size_t)lo[2] * (ny - 1) + lo[1]) * (nx - 1) + lo[0];
-        if (cell[c] >= 0) return SFH_OK;
-        cell[c] = e;
-        identity = identity && (size_t)e == c;
-    }
-    const int oz0 = d == 3 ? -1 : 0, oz1 = d == 3 ? 1 : 0;
-    for (int iz = 0, i = 0; iz < nz; ++iz)
-        for (int iy = 0; iy < ny; ++iy)
-            for (int ix = 0; ix < nx; ++ix, ++i) {
-                int p = rpn[i];
-                for (int oz = oz0; oz <= oz1; ++oz)
-                    for (int oy = -1; oy <= 1; ++oy)
-                        for (int ox = -1; ox <= 1; ++ox) {
-                            const int jx = ix + ox, jy = iy + oy, jz = iz + oz;
-                            if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz) continue;
-                            if (p >= rpn[i + 1] || cin[p] != jx + nx * (jy + ny * jz)) return SFH_OK;
-                            ++p;
-                        }
-                if (p != rpn[i + 1]) return SFH_OK;
-            }
-    s->gcell_identity = identity;
-    if (!identity) {
-        SFH_TRY(s->gcell.ensure(sizeof(int) * cell.size()));
-        SFH_CHECK_HIP(hipMemcpyAsync(s->gcell.p, cell.data(), sizeof(int) * cell.size(), hipMemcpyHostToDevice, st));
-        SFH_CHECK_HIP(hipStreamSynchronize(st));
-    }
+    if (flags & 1) return SFH_OK;
+    s->gcell_identity = !(flags & 2);
     s->gcell_state = 1;
     return SFH_OK;
 }

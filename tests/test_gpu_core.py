@@ -251,19 +251,24 @@ def test_hex8_elasticity_two_phase_assembly(ctx, orc, nq):
 
 
 @pytest.mark.parametrize("nq", [2, 3])
-@pytest.mark.parametrize("shuffle", [False, True])
+@pytest.mark.parametrize("shuffle", [False, True, "rotate"])
 def test_quad4_grid_cells_closed_form_assembly(ctx, orc, nq, shuffle):
     """Quad4 cells of a declared, graded 2D grid (elements in cell order, which
     needs no cell table, or shuffled): the closed-form cell integrals against
     the oracle's Gauss-quadrature Triplet assembly for Poisson with
-    per-element conductivities and Matern with per-element parameters."""
+    per-element conductivities and Matern with per-element parameters.
+    "rotate": one element lists its corners from another start (still
+    counter-clockwise), which the device check of the grid rejects, so the
+    general assembly runs; the values are the same."""
     from statfem_amd import hip
     base = M.grid2d(23, 17)
     grade = (0.9, 0.4)
     axes = [(g + a * g * g) / (1.0 + a) for g, a in zip(base.grid, grade)]
     X = np.stack([(base.coords[:, d] + grade[d] * base.coords[:, d] ** 2) / (1.0 + grade[d]) for d in range(2)],
                  axis=1)
-    conn = base.conn[np.random.default_rng(5).permutation(base.nel)] if shuffle else base.conn
+    conn = base.conn[np.random.default_rng(5).permutation(base.nel)] if shuffle is True else base.conn.copy()
+    if shuffle == "rotate":
+        conn[7] = np.roll(conn[7], 1)
     m = M.Mesh(base.kind, X, conn.copy(), grid=tuple(axes))
     rng = np.random.default_rng(nq)
     s = hip.SystemSolveHip(ctx, m)
