@@ -423,6 +423,24 @@ int sfh_system_pattern(sfh_system* s, int* rowptr, int* colind) {
     return SFH_OK;
 }
 
+namespace {
+// the lowest-numbered node that is not at its grid point (node i at
+// (i mod nx, (i / nx) mod ny, i / (nx ny)) of the axes, to 1e-12), into *bad
+__global__ void k_grid_nodes(int nnodes, int d, int nx, int ny, const double* __restrict__ X,
+                             const double* __restrict__ gx, const double* __restrict__ gy,
+                             const double* __restrict__ gz, int* bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    const int idx[3] = {i % nx, (i / nx) % ny, i / (nx * ny)};
+    const double* g[3] = {gx, gy, gz};
+    for (int a = 0; a < d; ++a)
+        if (!(fabs(X[(size_t)i * d + a] - g[a][idx[a]]) <= 1e-12)) {
+            atomicMin(bad, i);
+            return;
+        }
+}
+}  // namespace
+
 int sfh_system_set_grid(sfh_system* s, int nx, int ny, int nz, const double* gx, const double* gy, const double* gz) {
     if (s && s->kind == 0) { set_error("this system was built from triplets and has no mesh"); return SFH_ESTATE; }
     if (!s) { set_error("null system"); return SFH_EARG; }
@@ -430,20 +448,28 @@ int sfh_system_set_grid(sfh_system* s, int nx, int ny, int nz, const double* gx,
     const int n[3] = {nx, d > 1 ? ny : 1, d > 2 ? nz : 1};
     const double* g[3] = {gx, gy, gz};
     if ((long long)n[0] * n[1] * n[2] != s->nnodes) { set_error("grid size does not match node count"); return SFH_EARG; }
-    std::vector<double> X((size_t)s->nnodes * d);
-    SFH_CHECK_HIP(hipMemcpyAsync(X.data(), s->coords.p, X.size() * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+    // every node at its grid point, checked on the device against the axes
+    DevBuf tmp;
+    const size_t nax = (size_t)n[0] + (d > 1 ? n[1] : 0) + (d > 2 ? n[2] : 0);
+    SFH_TRY(tmp.ensure(sizeof(double) * nax + sizeof(int)));
+    double* ax = tmp.as<double>();
+    int* bad = reinterpret_cast<int*>(ax + nax);
+    for (int a = 0, o = 0; a < d; o += n[a], ++a)
+        SFH_CHECK_HIP(hipMemcpyAsync(ax + o, g[a], sizeof(double) * n[a], hipMemcpyHostToDevice, s->ctx->stream));
+    const int big = INT32_MAX;
+    SFH_CHECK_HIP(hipMemcpyAsync(bad, &big, sizeof(int), hipMemcpyHostToDevice, s->ctx->stream));
+    k_grid_nodes<<<(unsigned)((s->nnodes + 255) / 256), 256, 0, s->ctx->stream>>>(
+        s->nnodes, d, n[0], n[1], s->coords.as<double>(), ax, d > 1 ? ax + n[0] : nullptr,
+        d > 2 ? ax + n[0] + n[1] : nullptr, bad);
+    SFH_LAUNCH_CHECK();
+    int first = big;
+    SFH_CHECK_HIP(hipMemcpyAsync(&first, bad, sizeof(int), hipMemcpyDeviceToHost, s->ctx->stream));
     SFH_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
-    for (int k = 0; k < n[2]; ++k)
-        for (int j = 0; j < n[1]; ++j)
-            for (int i = 0; i < n[0]; ++i) {
-                const size_t id = (size_t)i + (size_t)n[0] * (j + (size_t)n[1] * k);
-                const int idx[3] = {i, j, k};
-                for (int a = 0; a < d; ++a)
-                    if (std::fabs(X[id * d + a] - g[a][idx[a]]) > 1e-12) {
-                        set_error("node %zu is not at grid point (%d,%d,%d)", id, i, j, k);
-                        return SFH_EARG;
-                    }
-            }
+    if (first != big) {
+        const int i = first % n[0], j = (first / n[0]) % n[1], k = first / (n[0] * n[1]);
+        set_error("node %d is not at grid point (%d,%d,%d)", first, i, j, k);
+        return SFH_EARG;
+    }
     for (int a = 0; a < d; ++a) {
         SFH_TRY(s->gaxis[a].ensure(sizeof(double) * n[a]));
         SFH_CHECK_HIP(hipMemcpyAsync(s->gaxis[a].p, g[a], sizeof(double) * n[a], hipMemcpyHostToDevice, s->ctx->stream));

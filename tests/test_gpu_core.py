@@ -343,3 +343,21 @@ def test_assembly_kernel_timing_counts_calls(ctx):
     assert n == 3 and ms > 0.0
     assert np.array_equal(s.values(), ref)
     ctx.kernel_sampling(every=0)
+
+
+def test_set_grid_rejects_nodes_off_the_axes(ctx):
+    """set_grid checks every node against the declared axes on the device and
+    names the lowest-numbered node off its grid point; a rejected declaration
+    leaves the system as it was, and the right axes are then accepted."""
+    from statfem_amd import hip
+    m = M.grid2d(9, 7)                      # 10 x 8 nodes, x fastest
+    s = hip.SystemSolveHip(ctx, m)
+    gx, gy = [np.array(a, float) for a in m.grid]
+    bad = gy.copy()
+    bad[3] += 1e-9                          # every node of row 3 is off; the first is 3 * 10
+    with pytest.raises(hip.SfhError) as ei:
+        s.set_grid((gx, bad))
+    assert ei.value.status == -1 and "node 30 is not at grid point (0,3,0)" in str(ei.value), str(ei.value)
+    s.set_grid((gx, gy))
+    s.assemble(hip.POISSON, 1.0, nq1d=2)
+    assert np.isfinite(s.values()).all()
